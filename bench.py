@@ -1,0 +1,264 @@
+"""Benchmark of the del-msh LBVH / ray-cast hot path on one MI355X.
+
+Workload (single large mesh, plain kernels): an octahedron-subdivided sphere
+with exactly 2^21 triangles (level 9), vertices displaced radially by seeded
+N(0, 0.01^2) noise; LBVH build with 63-bit Morton codes (centroid, unit-cube
+fit, Morton, stable radix argsort, Karras topology, refit); perspective
+ray-cast of 2048 x 2048 rays (eye (0,0,3), target origin, fovy 45, near 1,
+far 5) producing the hit triangle and hit coefficient per pixel.
+
+One step = one build + one ray-cast.  Inputs are resident in HBM before the
+timed region.  `value` is the build throughput in Mtri/s; the ray-cast
+throughput and the separate 2^22-point build / sort-alone figures are
+reported beside it.  Prints one JSON line.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import pathlib
+import sys
+import time
+
+ROOT = pathlib.Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT / "del-msh_amd"))
+
+PEAK_HBM_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--img", type=int, default=2048)
+    ap.add_argument("--level", type=int, default=9)
+    return ap.parse_args()
+
+
+def perspective_ndc2world(eye=(0.0, 0.0, 3.0), fovy=45.0, near=1.0, far=5.0):
+    import numpy as np
+
+    eye = np.asarray(eye, np.float64)
+    f = -eye / np.linalg.norm(eye)
+    up = np.array([0.0, 1.0, 0.0])
+    s = np.cross(f, up)
+    s /= np.linalg.norm(s)
+    u = np.cross(s, f)
+    view = np.eye(4)
+    view[0, :3], view[1, :3], view[2, :3] = s, u, -f
+    view[:3, 3] = -view[:3, :3] @ eye
+    g = 1.0 / math.tan(math.radians(fovy) * 0.5)
+    proj = np.array([[g, 0, 0, 0], [0, g, 0, 0],
+                     [0, 0, (far + near) / (near - far), 2 * far * near / (near - far)],
+                     [0, 0, -1, 0]])
+    return np.linalg.inv(proj @ view).astype(np.float32)
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
+              "(the batched/served paths shard meshes per GPU, which this build does not run)",
+              file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+
+    from del_msh_dlpack import _lib, primitives
+
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    L = _lib.lib()
+
+    # ---------------- inputs (resident before timing) ----------------
+    tri, vtx = primitives.octahedral_sphere(args.level, noise_sigma=0.01, seed=1)
+    num_tri, num_vtx = tri.shape[0], vtx.shape[0]
+    W = H = args.img
+    num_pix = W * H
+    n2w = perspective_ndc2world()
+    t_tri = torch.from_numpy(tri).to(dev)
+    t_vtx = torch.from_numpy(vtx).to(dev)
+    t_n2w = torch.from_numpy(np.ascontiguousarray(n2w.T)).to(dev)  # column-major
+    num_node = 2 * num_tri - 1
+    bvhnodes = torch.empty((num_node, 3), dtype=torch.uint32, device=dev)
+    aabb = torch.empty((num_node, 6), dtype=torch.float32, device=dev)
+    ws_build = _lib.workspace(L.dmsh_build_workspace_bytes(num_tri, 63), dev)
+    pix2tri = torch.empty((H, W), dtype=torch.uint32, device=dev)
+    pix2t = torch.empty((H, W), dtype=torch.float32, device=dev)
+    ws_cast = _lib.workspace(L.dmsh_pix2tri_workspace_bytes(W, H), dev)
+    stream = torch.cuda.current_stream(dev)
+    sp = stream.cuda_stream
+
+    def build():
+        _lib.check(L.dmsh_build_trimesh3(sp, num_tri, t_tri.data_ptr(), num_vtx, t_vtx.data_ptr(), 63,
+                                         bvhnodes.data_ptr(), aabb.data_ptr(), ws_build.data_ptr(),
+                                         ws_build.numel()), "build")
+
+    def cast():
+        _lib.check(L.dmsh_pix2tri(sp, pix2tri.data_ptr(), pix2t.data_ptr(), num_tri, t_tri.data_ptr(),
+                                  num_vtx, t_vtx.data_ptr(), W, H, t_n2w.data_ptr(),
+                                  bvhnodes.data_ptr(), aabb.data_ptr(), ws_cast.data_ptr(),
+                                  ws_cast.numel()), "pix2tri")
+
+    for _ in range(args.warmup):
+        build()
+        cast()
+    torch.cuda.synchronize()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True),
+           torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for e0, e1, e2 in ev:
+        e0.record(stream)
+        build()
+        e1.record(stream)
+        cast()
+        e2.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    build_ms = [a.elapsed_time(b) for a, b, _ in ev]
+    cast_ms = [b.elapsed_time(c) for _, b, c in ev]
+    errs = _lib.device_errors()
+    if errs:
+        raise RuntimeError(f"device error bits 0x{errs:x}")
+    ms_step = wall * 1e3 / args.steps
+    build_avg = float(np.mean(build_ms))
+    cast_avg = float(np.mean(cast_ms))
+    build_mtri = num_tri / (build_avg * 1e-3) / 1e6
+    cast_mrays = num_pix / (cast_avg * 1e-3) / 1e6
+
+    # ---------------- secondary: 2^22-point build and sort alone ----------------
+    rng = np.random.default_rng(2)
+    pts = torch.from_numpy(rng.random((1 << 22, 3), dtype=np.float32)).to(dev)
+    npnt = pts.shape[0]
+    p_nodes = torch.empty((2 * npnt - 1, 3), dtype=torch.uint32, device=dev)
+    p_aabb = torch.empty((2 * npnt - 1, 6), dtype=torch.float32, device=dev)
+    ws_p = _lib.workspace(L.dmsh_build_workspace_bytes(npnt, 63), dev)
+    keys0 = torch.from_numpy(rng.integers(0, 1 << 63, npnt, dtype=np.uint64)).to(dev)
+    keys = torch.empty_like(keys0)
+    idx = torch.empty((npnt,), dtype=torch.uint32, device=dev)
+    ws_s = _lib.workspace(L.dmsh_argsort_workspace_bytes(npnt, 8), dev)
+
+    def pbuild():
+        _lib.check(L.dmsh_build_points(sp, npnt, pts.data_ptr(), 63, p_nodes.data_ptr(),
+                                       p_aabb.data_ptr(), ws_p.data_ptr(), ws_p.numel()), "points")
+
+    def psort():
+        _lib.check(L.dmsh_argsort_u64(sp, npnt, keys.data_ptr(), idx.data_ptr(), 63, ws_s.data_ptr(),
+                                      ws_s.numel()), "sort")
+
+    def timed(fn, reps, prep=None):
+        out = []
+        for _ in range(reps):
+            if prep:
+                prep()
+            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            a.record(stream)
+            fn()
+            b.record(stream)
+            out.append((a, b))
+        torch.cuda.synchronize()
+        return float(np.mean([a.elapsed_time(b) for a, b in out]))
+
+    timed(pbuild, 2)
+    pbuild_ms = timed(pbuild, 10)
+    timed(psort, 2, lambda: keys.copy_(keys0))
+    psort_ms = timed(psort, 10, lambda: keys.copy_(keys0))
+
+    # ---------------- roofline of the dominant kernel ----------------
+    # build: compulsory bytes per triangle (u64 codes), see DESIGN.md
+    build_bytes = num_tri * 12 + num_vtx * 12 + num_tri * 12 + num_node * 12 + num_node * 24
+    # ray-cast: traversal bytes from the oracle's count of expanded internal
+    # nodes (row 12 B + two child boxes 48 B) and tested leaves (row 12 B +
+    # indices 12 B + vertices 36 B), plus the root box and 8 B of output per ray
+    cast_counts = None
+    cpu = None
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, str(ROOT / "oracle"))
+        import oracle
+
+        nodes_h = bvhnodes.cpu().numpy()
+        aabb_h = aabb.cpu().numpy()
+        threads = min(16, os.cpu_count() or 1)
+        tc = time.perf_counter()
+        wpix, _, (n_exp, n_leaf) = oracle.pix2tri(tri, vtx, nodes_h, aabb_h, n2w.T.reshape(-1), W, H,
+                                                  num_threads=threads, with_t=False)
+        cpu_cast_s = time.perf_counter() - tc
+        cast_counts = (n_exp, n_leaf)
+        mism = int(np.count_nonzero(wpix != pix2tri.cpu().numpy()))
+        tb = time.perf_counter()
+        oracle.build_trimesh3(tri, vtx, morton_bits=63)
+        cpu_build_s = time.perf_counter() - tb
+        cpu = {
+            "value": round(num_tri / cpu_build_s / 1e6, 3),
+            "unit": "Mtri/s",
+            "cores": 1,
+            "kind": "port",
+            "sample": (f"full 2^21-triangle LBVH build (63-bit codes), single thread as "
+                       f"del-msh-cpu; ray-cast of all {W}x{H} rays on {threads} threads: "
+                       f"{num_pix / cpu_cast_s / 1e6:.3f} Mrays/s; GPU/oracle pix2tri mismatches: {mism}"),
+            "raycast_mrays_per_s": round(num_pix / cpu_cast_s / 1e6, 3),
+            "raycast_threads": threads,
+        }
+    if cast_counts:
+        cast_bytes = num_pix * (24 + 8) + 60 * (cast_counts[0] + cast_counts[1])
+    else:
+        cast_bytes = None
+    dominant = "raycast" if cast_avg >= build_avg else "build"
+    if dominant == "raycast" and cast_bytes:
+        achieved = cast_bytes / (cast_avg * 1e-3) / 1e9
+        kern = "k_pix2tri"
+    else:
+        achieved = build_bytes / (build_avg * 1e-3) / 1e9
+        kern = "dmsh_build_trimesh3 (13 kernels)"
+    roof = {
+        "bound": "hbm",
+        "kernel": kern,
+        "achieved": round(achieved, 1),
+        "peak": PEAK_HBM_GBS,
+        "unit": "GB/s",
+        "frac": round(achieved / PEAK_HBM_GBS, 4),
+        "traffic": None,
+    }
+    line = {
+        "metric": "LBVH build Mtri/s + Mrays/s, batched/served vs plain kernel; p50/p99 latency",
+        "value": round(build_mtri, 2),
+        "unit": "Mtri/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"single large mesh, plain kernels: octahedral sphere level {args.level} "
+                        f"({num_tri} tris, radial noise N(0,0.01^2), seed 1), 63-bit Morton LBVH build "
+                        f"+ refit, perspective ray-cast {W}x{H}",
+            "num_tri": num_tri,
+            "rays": num_pix,
+        },
+        "build_ms": round(build_avg, 4),
+        "raycast_ms": round(cast_avg, 4),
+        "raycast_mrays_per_s": round(cast_mrays, 2),
+        "points_build_2p22_ms": round(pbuild_ms, 4),
+        "points_build_mpts_per_s": round(npnt / (pbuild_ms * 1e-3) / 1e6, 2),
+        "sort_alone_u64_2p22_ms": round(psort_ms, 4),
+        "sort_alone_melem_per_s": round(npnt / (psort_ms * 1e-3) / 1e6, 2),
+        "raycast_traversal": ({"internal_expansions": cast_counts[0], "leaf_tests": cast_counts[1]}
+                              if cast_counts else None),
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

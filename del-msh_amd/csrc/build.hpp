@@ -1,0 +1,234 @@
+// build.hpp -- device-resident LBVH build of a triangle mesh or a point set.
+//
+//   init        bounding-box keys, histogram / ticket / look-back / refit
+//               counters (one fill kernel: no memset nodes, graph safe)
+//   centroid    triangle centroids ((0 + p0 + p1 + p2) * (1/3), as
+//               del-msh-cpu/src/elem2center.rs:8-37) and the bounding box of
+//               the centroids, reduced through order-preserving integer keys so
+//               the box does not depend on reduction order
+//   morton      unit-cube fit of that box (Mat44/torch.py:120-140 formula),
+//               Morton code per element, and the digit histograms of every
+//               radix pass (the sort's histogram read is fused in here)
+//   sort        onesweep digit passes (sort.hpp), values = element index
+//   topology    Karras tree (topology.hpp)
+//   refit       bottom-up boxes (refit.hpp)
+#pragma once
+
+#include "morton.hpp"
+#include "refit.hpp"
+#include "sort.hpp"
+#include "topology.hpp"
+
+namespace dmsh {
+namespace bld {
+
+// keys6[0..2] = min order keys (init 0xFFFFFFFF), keys6[3..5] = max (init 0),
+// then `zero_count` words starting at `zero` are cleared.
+static __global__ __launch_bounds__(256) void k_build_init(uint32_t* __restrict__ keys6,
+                                                    uint32_t* __restrict__ zero,
+                                                    size_t zero_count) {
+  const size_t i0 = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (i0 < 6) keys6[i0] = (i0 < 3) ? 0xFFFFFFFFu : 0u;
+  for (size_t i = i0; i < zero_count; i += (size_t)gridDim.x * blockDim.x) zero[i] = 0u;
+}
+
+template <bool TRI>
+__global__ __launch_bounds__(256) void k_centroid_aabb(uint32_t n,
+                                                       const uint32_t* __restrict__ tri2vtx,
+                                                       uint32_t num_vtx,
+                                                       const float* __restrict__ xyz,
+                                                       float* __restrict__ cntr,
+                                                       uint32_t* __restrict__ keys6,
+                                                       uint32_t* __restrict__ err) {
+  __shared__ uint32_t s_red[4][6];
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t kmin[3] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu}, kmax[3] = {0u, 0u, 0u};
+  if (i < n) {
+    float c[3];
+    if (TRI) {
+      const uint32_t* tv = tri2vtx + (size_t)i * 3;
+      uint32_t v[3] = {tv[0], tv[1], tv[2]};
+      if (v[0] >= num_vtx || v[1] >= num_vtx || v[2] >= num_vtx) {
+        raise_error(err, DMSH_DEVERR_INDEX);
+        v[0] = v[1] = v[2] = 0;
+      }
+      const float ratio = 1.f / 3.f;
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        float cog = 0.f;
+        cog += xyz[(size_t)v[0] * 3 + d];
+        cog += xyz[(size_t)v[1] * 3 + d];
+        cog += xyz[(size_t)v[2] * 3 + d];
+        c[d] = cog * ratio;
+        cntr[(size_t)i * 3 + d] = c[d];
+      }
+    } else {
+#pragma unroll
+      for (int d = 0; d < 3; ++d) c[d] = xyz[(size_t)i * 3 + d];
+    }
+#pragma unroll
+    for (int d = 0; d < 3; ++d) kmin[d] = kmax[d] = f32_order_key(c[d]);
+  }
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const uint32_t a = __shfl_xor(kmin[d], off, 64), b = __shfl_xor(kmax[d], off, 64);
+      kmin[d] = a < kmin[d] ? a : kmin[d];
+      kmax[d] = b > kmax[d] ? b : kmax[d];
+    }
+  }
+  const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+  if (lane == 0) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      s_red[wave][d] = kmin[d];
+      s_red[wave][d + 3] = kmax[d];
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < 6) {
+    const uint32_t d = threadIdx.x;
+    const uint32_t nw = blockDim.x >> 6;
+    uint32_t v = s_red[0][d];
+    for (uint32_t w = 1; w < nw; ++w) {
+      const uint32_t u = s_red[w][d];
+      v = (d < 3) ? (u < v ? u : v) : (u > v ? u : v);
+    }
+    if (d < 3) atomicMin(keys6 + d, v);
+    else atomicMax(keys6 + d, v);
+  }
+}
+
+// Column-major unit-cube fit of the reduced box: s = 1 / max extent,
+// translation -center * s + 0.5.
+__device__ __forceinline__ void fit_transform(const uint32_t* __restrict__ keys6, float T[16]) {
+  float mn[3], mx[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    mn[d] = f32_from_order_key(keys6[d]);
+    mx[d] = f32_from_order_key(keys6[d + 3]);
+  }
+  float ext = mx[0] - mn[0];
+  ext = fmax_sel(ext, mx[1] - mn[1]);
+  ext = fmax_sel(ext, mx[2] - mn[2]);
+  const float s = 1.f / ext;
+#pragma unroll
+  for (int k = 0; k < 16; ++k) T[k] = 0.f;
+  T[0] = s;
+  T[5] = s;
+  T[10] = s;
+  T[15] = 1.f;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const float c = (mn[d] + mx[d]) * 0.5f;
+    T[12 + d] = (-c) * s + 0.5f;
+  }
+}
+
+template <typename KeyT>
+__global__ __launch_bounds__(256) void k_morton_hist(uint32_t n, const float* __restrict__ pts,
+                                                     const uint32_t* __restrict__ keys6,
+                                                     KeyT* __restrict__ codes,
+                                                     uint32_t num_passes, uint32_t key_bits,
+                                                     uint32_t* __restrict__ hist) {
+  __shared__ uint32_t s_h[sortk::kMaxPasses][sortk::kRadix];
+  for (uint32_t e = threadIdx.x; e < sortk::kMaxPasses * sortk::kRadix; e += blockDim.x)
+    (&s_h[0][0])[e] = 0;
+  float T[16];
+  fit_transform(keys6, T);
+  __syncthreads();
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const float* p = pts + (size_t)i * 3;
+    const KeyT code = morton3_transformed<KeyT>(T, p[0], p[1], p[2]);
+    codes[i] = code;
+    sortk::hist_accumulate(s_h, code, num_passes, key_bits);
+  }
+  __syncthreads();
+  sortk::hist_flush(s_h, num_passes, hist);
+}
+
+template <typename KeyT>
+inline uint32_t key_bits_of() {
+  return sizeof(KeyT) == 4 ? 30u : 63u;
+}
+
+struct BuildLayout {
+  uint32_t* keys6;
+  sortk::Layout sort;
+  uint32_t* flags;
+  float* cntr;
+  void* codes;
+  uint32_t* idx;
+  uint32_t* zero_begin;
+  size_t zero_words;
+};
+
+template <typename KeyT>
+inline bool carve(Arena& a, uint32_t n, bool tri, BuildLayout& L) {
+  const uint32_t kb = key_bits_of<KeyT>();
+  L.keys6 = a.take<uint32_t>(64);
+  // hist, tickets, status and the refit counters are one contiguous zero run
+  const size_t P = sortk::num_passes(kb);
+  L.sort.hist = a.take<uint32_t>(sortk::kMaxPasses * sortk::kRadix + sortk::kMaxPasses);
+  L.sort.tickets = L.sort.hist + sortk::kMaxPasses * sortk::kRadix;
+  const size_t status_words = P * (size_t)sortk::num_tiles<KeyT>(n) * sortk::kRadix;
+  L.sort.status = L.sort.tickets + sortk::kMaxPasses;
+  a.off += sizeof(uint32_t) * status_words;  // contiguous after the tickets
+  L.flags = L.sort.status + status_words;
+  a.off += sizeof(uint32_t) * (n > 1 ? n - 1 : 1);
+  L.zero_begin = L.sort.hist;
+  L.zero_words = (size_t)(L.flags - L.sort.hist) + (n > 1 ? n - 1 : 1);
+  L.sort.zero_words = 0;
+  L.cntr = tri ? a.take<float>((size_t)n * 3) : nullptr;
+  L.codes = a.take<KeyT>(n);
+  L.idx = a.take<uint32_t>(n);
+  L.sort.alt_keys = a.take<KeyT>(n);
+  L.sort.alt_vals = a.take<uint32_t>(n);
+  return a.ok();
+}
+
+template <typename KeyT>
+inline size_t workspace_bytes(uint32_t n) {
+  Arena a{nullptr, 0, 0};
+  BuildLayout L;
+  carve<KeyT>(a, n, true, L);
+  return a.off + 256;
+}
+
+template <typename KeyT, bool TRI>
+inline int build(hipStream_t s, uint32_t n, const uint32_t* tri2vtx, uint32_t num_vtx,
+                 const float* xyz, uint32_t* bvhnodes, float* aabb, void* ws, size_t ws_bytes) {
+  if (n == 0) return DMSH_OK;
+  if (!xyz || !bvhnodes || !aabb || (TRI && !tri2vtx)) return DMSH_ERR_INVALID;
+  if (n >= (1u << 30)) return DMSH_ERR_INVALID;
+  if (!ws || ws_bytes < workspace_bytes<KeyT>(n)) return DMSH_ERR_WORKSPACE;
+  Arena a{static_cast<char*>(ws), ws_bytes, 0};
+  BuildLayout L;
+  if (!carve<KeyT>(a, n, TRI, L)) return DMSH_ERR_WORKSPACE;
+  uint32_t* err = device_error_word();
+  if (!err) return DMSH_ERR_HIP;
+  const uint32_t kb = key_bits_of<KeyT>();
+  const uint32_t P = sortk::num_passes(kb);
+  int rc;
+  hipLaunchKernelGGL(k_build_init, dim3(std::min<uint32_t>(ceil_div(L.zero_words, 256), 2048)),
+                     dim3(256), 0, s, L.keys6, L.zero_begin, L.zero_words);
+  if ((rc = launch_status()) != DMSH_OK) return rc;
+  hipLaunchKernelGGL(k_centroid_aabb<TRI>, dim3(ceil_div(n, 256)), dim3(256), 0, s, n, tri2vtx,
+                     TRI ? num_vtx : n, xyz, L.cntr, L.keys6, err);
+  if ((rc = launch_status()) != DMSH_OK) return rc;
+  KeyT* codes = static_cast<KeyT*>(L.codes);
+  const uint32_t mh_blocks = std::min<uint32_t>(sortk::num_tiles<KeyT>(n), 2048);
+  hipLaunchKernelGGL(k_morton_hist<KeyT>, dim3(mh_blocks), dim3(256), 0, s, n,
+                     TRI ? L.cntr : xyz, L.keys6, codes, P, kb, L.sort.hist);
+  if ((rc = launch_status()) != DMSH_OK) return rc;
+  // P is even for 30- and 63-bit codes: the passes end back in `codes`.
+  rc = sortk::run_passes<KeyT>(s, n, kb, L.sort, codes, codes, L.idx, err);
+  if (rc != DMSH_OK) return rc;
+  rc = launch_topology<KeyT>(s, n, bvhnodes, codes, L.idx);
+  if (rc != DMSH_OK) return rc;
+  return launch_refit<TRI>(s, aabb, L.flags, n, bvhnodes, tri2vtx, num_vtx, xyz, 0.f, err);
+}
+
+}  // namespace bld
+}  // namespace dmsh

@@ -1,0 +1,149 @@
+// edgegrad.hpp -- edge-gradient backward of the rasterized silhouette (gfx950).
+//
+// For every pair of 4-neighbour pixels whose triangles differ, the pixel
+// centres are tested against the other pixel's projected triangle (2D
+// barycentric signs; background and degenerate triangles count as inside).
+// When exactly one centre lies inside the other triangle, the triangle that
+// does not cover it receives dL/dp = sum_k (g0+g1)/2 (v0-v1) through the
+// barycentric weights of its own pixel centre and the perspective Jacobian
+// of pixel y (vertical pairs) or pixel x (horizontal pairs); see
+// del-msh-cpu/src/rasterized_edge_gradient.rs:1-74 and :434-603.
+//
+// One thread owns pixel (iw, ih) and handles the pair below it and the pair
+// to its right, so a 64-wide row of threads reads pix2tri / pix2val rows
+// coalesced; the scatter to the three vertices uses hardware float atomics
+// (global_atomic_add_f32), so the summation order is not deterministic and
+// results agree with the serial CPU loop to rounding only.
+#pragma once
+
+#include "dmsh_common.hpp"
+
+namespace dmsh {
+namespace eg {
+
+__device__ __forceinline__ float tri2_area(const float p0[2], const float p1[2],
+                                           const float p2[2]) {
+  return 0.5f * ((p1[0] - p0[0]) * (p2[1] - p0[1]) - (p2[0] - p0[0]) * (p1[1] - p0[1]));
+}
+
+__device__ __forceinline__ void project_xy(const float* m, const float* p, float q[2]) {
+  const float x = p[0], y = p[1], z = p[2];
+  const float w = ((m[3] * x + m[7] * y) + m[11] * z) + m[15];
+  q[0] = (((m[0] * x + m[4] * y) + m[8] * z) + m[12]) / w;
+  q[1] = (((m[1] * x + m[5] * y) + m[9] * z) + m[13]) / w;
+}
+
+struct Ctx {
+  const uint32_t* __restrict__ tri2vtx;
+  const float* __restrict__ xyz;
+  uint32_t num_tri, num_vtx;
+  float m[16];
+  uint32_t* err;
+};
+
+// 1: barycentrics of pc in itri; 0: none (background / degenerate); -1: bad index.
+__device__ __forceinline__ int bary(const Ctx& c, const float pc[2], uint32_t itri, float b[3]) {
+  if (itri == DMSH_U32MAX) return 0;
+  if (itri >= c.num_tri) return -1;
+  const uint32_t* tv = c.tri2vtx + (size_t)itri * 3;
+  const uint32_t i0 = tv[0], i1 = tv[1], i2 = tv[2];
+  if (i0 >= c.num_vtx || i1 >= c.num_vtx || i2 >= c.num_vtx) return -1;
+  float p0[2], p1[2], p2[2];
+  project_xy(c.m, c.xyz + (size_t)i0 * 3, p0);
+  project_xy(c.m, c.xyz + (size_t)i1 * 3, p1);
+  project_xy(c.m, c.xyz + (size_t)i2 * 3, p2);
+  const float area = tri2_area(p0, p1, p2);
+  if (area == 0.f) return 0;
+  const float inv_area = 1.f / area;
+  b[0] = tri2_area(pc, p1, p2) * inv_area;
+  b[1] = tri2_area(p0, pc, p2) * inv_area;
+  b[2] = 1.f - b[0] - b[1];
+  return 1;
+}
+
+__device__ __forceinline__ bool inside(int has, const float b[3]) {
+  if (has <= 0) return true;
+  return (b[0] >= 0.f && b[1] >= 0.f && b[2] >= 0.f) || (b[0] <= 0.f && b[1] <= 0.f && b[2] <= 0.f);
+}
+
+__device__ __forceinline__ bool proj_grad(const float* m, const float* xyz, int axis, float d[3]) {
+  const float q = m[axis] * xyz[0] + m[axis + 4] * xyz[1] + m[axis + 8] * xyz[2] + m[axis + 12];
+  const float w = m[3] * xyz[0] + m[7] * xyz[1] + m[11] * xyz[2] + m[15];
+  if (fabsf(w) <= FLT_EPSILON) return false;
+  const float inv_w2 = 1.f / (w * w);
+  d[0] = (m[axis] * w - q * m[3]) * inv_w2;
+  d[1] = (m[axis + 4] * w - q * m[7]) * inv_w2;
+  d[2] = (m[axis + 8] * w - q * m[11]) * inv_w2;
+  return true;
+}
+
+__device__ __forceinline__ void edge_pair(const Ctx& c, const uint32_t* __restrict__ pix2tri,
+                                          uint32_t nv, const float* __restrict__ pix2val,
+                                          const float* __restrict__ dpix, size_t ip0, size_t ip1,
+                                          const float pc0[2], const float pc1[2], int axis,
+                                          float* __restrict__ dldw) {
+  const uint32_t t0 = pix2tri[ip0], t1 = pix2tri[ip1];
+  if (t0 == t1) return;
+  float b[3];
+  const int h0 = bary(c, pc0, t1, b);
+  const bool in0 = inside(h0, b);
+  const int h1 = bary(c, pc1, t0, b);
+  const bool in1 = inside(h1, b);
+  if (h0 < 0 || h1 < 0) {
+    raise_error(c.err, DMSH_DEVERR_INDEX);
+    return;
+  }
+  if (!in0 && !in1) return;
+  if (in0 && in1) return;  // intersecting triangles: not handled by the reference either
+  float dldpa = 0.f;
+  for (uint32_t k = 0; k < nv; ++k) {
+    const float v0 = pix2val[ip0 * nv + k], v1 = pix2val[ip1 * nv + k];
+    const float g0 = dpix[ip0 * nv + k], g1 = dpix[ip1 * nv + k];
+    dldpa += (g0 + g1) * 0.5f * (v0 - v1);
+  }
+  const uint32_t itri = in1 ? t1 : t0;
+  const float* pc = in1 ? pc1 : pc0;
+  if (bary(c, pc, itri, b) != 1) return;
+  const uint32_t* tv = c.tri2vtx + (size_t)itri * 3;
+#pragma unroll
+  for (int inode = 0; inode < 3; ++inode) {
+    const uint32_t iv = tv[inode];
+    float d[3];
+    if (!proj_grad(c.m, c.xyz + (size_t)iv * 3, axis, d)) continue;
+    unsafeAtomicAdd(dldw + (size_t)iv * 3 + 0, b[inode] * d[0] * dldpa);
+    unsafeAtomicAdd(dldw + (size_t)iv * 3 + 1, b[inode] * d[1] * dldpa);
+    unsafeAtomicAdd(dldw + (size_t)iv * 3 + 2, b[inode] * d[2] * dldpa);
+  }
+}
+
+// block (64, 4): threadIdx.x runs along a pixel row.
+static __global__ __launch_bounds__(256) void k_edge_grad_bwd(
+    const uint32_t* __restrict__ tri2vtx, uint32_t num_tri, const float* __restrict__ xyz,
+    uint32_t num_vtx, float* __restrict__ dldw, const float* __restrict__ T, uint32_t img_w,
+    uint32_t img_h, const uint32_t* __restrict__ pix2tri, uint32_t nv,
+    const float* __restrict__ pix2val, const float* __restrict__ dpix, uint32_t* err) {
+  const uint32_t iw = blockIdx.x * 64u + threadIdx.x;
+  const uint32_t ih = blockIdx.y * 4u + threadIdx.y;
+  if (iw >= img_w || ih >= img_h) return;
+  Ctx c;
+  c.tri2vtx = tri2vtx;
+  c.xyz = xyz;
+  c.num_tri = num_tri;
+  c.num_vtx = num_vtx;
+  c.err = err;
+#pragma unroll
+  for (int k = 0; k < 16; ++k) c.m[k] = T[k];
+  const size_t ip = (size_t)ih * img_w + iw;
+  const float pc[2] = {(float)iw + 0.5f, (float)ih + 0.5f};
+  if (ih + 1 < img_h) {
+    const float pcs[2] = {(float)iw + 0.5f, (float)(ih + 1) + 0.5f};
+    edge_pair(c, pix2tri, nv, pix2val, dpix, ip, ip + img_w, pc, pcs, 1, dldw);
+  }
+  if (iw + 1 < img_w) {
+    const float pce[2] = {(float)(iw + 1) + 0.5f, (float)ih + 0.5f};
+    edge_pair(c, pix2tri, nv, pix2val, dpix, ip, ip + 1, pc, pce, 0, dldw);
+  }
+}
+
+}  // namespace eg
+}  // namespace dmsh

@@ -1,0 +1,356 @@
+// sort.hpp -- stable LSD radix argsort for gfx950 (u32 / u64 keys, u32 values).
+//
+// Design (one pass per 8-bit digit, Onesweep-style):
+//   * k_hist_all reads the keys once and builds the digit histogram of every
+//     pass (LDS counters, then one global atomic per bin and block).
+//   * k_onesweep, one launch per digit: a block takes the next tile from an
+//     atomic ticket (so tiles are started in order), ranks its elements with
+//     wave64 ballots (one ballot per digit bit gives each lane the mask of
+//     lanes holding the same digit; popcount below the lane is its rank),
+//     publishes per-digit tile counts and resolves its global offset by
+//     decoupled look-back over the preceding tiles' published counts, then
+//     stages the tile in LDS in digit order and writes it out coalesced.
+// Stability: ranks follow element order inside a wave round, wave rounds go
+// in element order, waves and tiles are prefixed in order.  The look-back
+// spin is bounded; a block that gives up raises DMSH_DEVERR_SORT_PROGRESS and
+// still exits, so a launch always drains.
+#pragma once
+
+#include "dmsh_common.hpp"
+
+namespace dmsh {
+namespace sortk {
+
+constexpr int kThreads = 256;  // 4 waves
+constexpr int kRadixBits = 8;
+constexpr int kRadix = 1 << kRadixBits;
+constexpr int kMaxPasses = 8;
+constexpr uint32_t kFlagAgg = 1u << 30;
+constexpr uint32_t kFlagPrefix = 2u << 30;
+constexpr uint32_t kValueMask = (1u << 30) - 1u;
+constexpr uint32_t kSpinLimit = 1u << 22;
+
+template <typename KeyT>
+struct Tune {
+  static constexpr int kItems = 8;
+};
+
+__device__ __forceinline__ uint32_t wave_inclusive_scan(uint32_t v) {
+  const uint32_t lane = threadIdx.x & 63u;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint32_t u = __shfl_up(v, d, 64);
+    if (lane >= (uint32_t)d) v += u;
+  }
+  return v;
+}
+
+// Exclusive scan of one value per thread over a 256-thread block.
+__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* s_wsum) {
+  const uint32_t inc = wave_inclusive_scan(v);
+  const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+  if (lane == 63u) s_wsum[wave] = inc;
+  __syncthreads();
+  uint32_t base = 0;
+  for (uint32_t w = 0; w < wave; ++w) base += s_wsum[w];
+  __syncthreads();
+  return base + inc - v;
+}
+
+__device__ __forceinline__ uint32_t pass_bits(uint32_t key_bits, uint32_t p) {
+  const uint32_t lo = p * kRadixBits;
+  const uint32_t rem = key_bits > lo ? key_bits - lo : 0u;
+  return rem < (uint32_t)kRadixBits ? rem : (uint32_t)kRadixBits;
+}
+
+template <typename KeyT>
+__device__ __forceinline__ uint32_t digit_of(KeyT k, uint32_t shift, uint32_t mask) {
+  return (uint32_t)(k >> shift) & mask;
+}
+
+static __global__ __launch_bounds__(256) void k_fill_u32(uint32_t* __restrict__ p, size_t count,
+                                                  uint32_t value) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < count;
+       i += (size_t)gridDim.x * blockDim.x)
+    p[i] = value;
+}
+
+template <typename KeyT>
+__global__ __launch_bounds__(256) void k_copy_keys(const KeyT* __restrict__ a, KeyT* __restrict__ b,
+                                                   uint32_t n) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    b[i] = a[i];
+}
+
+// Accumulates the histograms of all passes into the LDS bins of this block.
+template <typename KeyT>
+__device__ __forceinline__ void hist_accumulate(uint32_t (*s_h)[kRadix], KeyT k,
+                                                uint32_t num_passes, uint32_t key_bits) {
+  for (uint32_t p = 0; p < num_passes; ++p) {
+    const uint32_t mask = (1u << pass_bits(key_bits, p)) - 1u;
+    atomicAdd(&s_h[p][digit_of(k, p * kRadixBits, mask)], 1u);
+  }
+}
+
+__device__ __forceinline__ void hist_flush(uint32_t (*s_h)[kRadix], uint32_t num_passes,
+                                           uint32_t* __restrict__ hist) {
+  for (uint32_t e = threadIdx.x; e < num_passes * kRadix; e += blockDim.x) {
+    const uint32_t v = (&s_h[0][0])[e];
+    if (v) atomicAdd(hist + e, v);
+  }
+}
+
+template <typename KeyT>
+__global__ __launch_bounds__(256) void k_hist_all(const KeyT* __restrict__ keys, uint32_t n,
+                                                  uint32_t num_passes, uint32_t key_bits,
+                                                  uint32_t* __restrict__ hist) {
+  __shared__ uint32_t s_h[kMaxPasses][kRadix];
+  for (uint32_t e = threadIdx.x; e < kMaxPasses * kRadix; e += blockDim.x) (&s_h[0][0])[e] = 0;
+  __syncthreads();
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    hist_accumulate(s_h, keys[i], num_passes, key_bits);
+  __syncthreads();
+  hist_flush(s_h, num_passes, hist);
+}
+
+// One digit pass.  IOTA: values are the element indices (first pass).
+template <typename KeyT, int ITEMS, bool IOTA>
+__global__ __launch_bounds__(256) void k_onesweep(
+    const KeyT* __restrict__ kin, const uint32_t* __restrict__ vin, KeyT* __restrict__ kout,
+    uint32_t* __restrict__ vout, uint32_t n, uint32_t shift, uint32_t dbits,
+    const uint32_t* __restrict__ hist, uint32_t* __restrict__ status,
+    uint32_t* __restrict__ ticket, uint32_t* __restrict__ err) {
+  constexpr int kTile = kThreads * ITEMS;
+  __shared__ uint32_t s_wcnt[4][kRadix];
+  __shared__ uint32_t s_excl[kRadix];
+  __shared__ uint32_t s_gbase[kRadix];
+  __shared__ uint32_t s_wsum[4];
+  __shared__ uint32_t s_tile;
+  __shared__ KeyT s_key[kTile];
+  __shared__ uint32_t s_val[kTile];
+
+  const uint32_t tid = threadIdx.x, wave = tid >> 6, lane = tid & 63u;
+  const uint32_t mask = (1u << dbits) - 1u;
+  if (tid == 0) s_tile = atomicAdd(ticket, 1u);
+  for (uint32_t e = tid; e < 4u * kRadix; e += kThreads) (&s_wcnt[0][0])[e] = 0;
+  __syncthreads();
+  const uint32_t tile = s_tile;
+  const uint32_t tile_base = tile * (uint32_t)kTile;
+  const uint32_t wave_base = tile_base + wave * 64u * ITEMS;
+
+  KeyT key[ITEMS];
+  uint32_t val[ITEMS], dig[ITEMS], pos[ITEMS];
+#pragma unroll
+  for (int i = 0; i < ITEMS; ++i) {
+    const uint32_t idx = wave_base + (uint32_t)i * 64u + lane;
+    const bool valid = idx < n;
+    key[i] = valid ? kin[idx] : (KeyT)0;
+    val[i] = valid ? (IOTA ? idx : vin[idx]) : 0u;
+    dig[i] = valid ? digit_of(key[i], shift, mask) : 0xFFFFFFFFu;
+  }
+  // Rank inside the wave: rounds in element order, lanes in element order.
+#pragma unroll
+  for (int i = 0; i < ITEMS; ++i) {
+    const bool valid = dig[i] != 0xFFFFFFFFu;
+    uint64_t m = __ballot(valid);
+    for (uint32_t b = 0; b < dbits; ++b) {
+      const bool bit = (dig[i] >> b) & 1u;
+      const uint64_t bb = __ballot(bit);
+      m &= bit ? bb : ~bb;
+    }
+    if (valid) {
+      const uint32_t rank = mask_rank(m);
+      const uint32_t old = s_wcnt[wave][dig[i]];
+      pos[i] = old + rank;
+      if (rank == 0) s_wcnt[wave][dig[i]] = old + (uint32_t)__popcll(m);
+    }
+  }
+  __syncthreads();
+  // Per digit: wave prefixes and the tile count.
+  const uint32_t d = tid;
+  const uint32_t c0 = s_wcnt[0][d], c1 = s_wcnt[1][d], c2 = s_wcnt[2][d], c3 = s_wcnt[3][d];
+  s_wcnt[0][d] = 0;
+  s_wcnt[1][d] = c0;
+  s_wcnt[2][d] = c0 + c1;
+  s_wcnt[3][d] = c0 + c1 + c2;
+  const uint32_t cnt = c0 + c1 + c2 + c3;
+  uint32_t* my_status = status + (size_t)tile * kRadix + d;
+  __hip_atomic_store(my_status, (tile == 0 ? kFlagPrefix : kFlagAgg) | cnt, __ATOMIC_RELAXED,
+                     __HIP_MEMORY_SCOPE_AGENT);
+  const uint32_t excl = block_exclusive_scan(cnt, s_wsum);
+  const uint32_t goff = block_exclusive_scan(hist[d], s_wsum);
+  // Decoupled look-back over the preceding tiles for digit d.
+  uint32_t prefix = 0;
+  if (tile > 0) {
+    uint32_t tt = tile - 1, spins = 0;
+    for (;;) {
+      const uint32_t v = __hip_atomic_load(status + (size_t)tt * kRadix + d, __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_AGENT);
+      const uint32_t flag = v & ~kValueMask;
+      if (flag == 0u) {
+        if (++spins > kSpinLimit) {
+          raise_error(err, DMSH_DEVERR_SORT_PROGRESS);
+          break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+        continue;
+      }
+      prefix += v & kValueMask;
+      if (flag == kFlagPrefix || tt == 0) break;
+      --tt;
+    }
+    __hip_atomic_store(my_status, kFlagPrefix | (prefix + cnt), __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+  }
+  s_excl[d] = excl;
+  s_gbase[d] = goff + prefix - excl;  // modular; final positions are in range
+  __syncthreads();
+  // Stage in digit order.
+#pragma unroll
+  for (int i = 0; i < ITEMS; ++i) {
+    if (dig[i] != 0xFFFFFFFFu) {
+      const uint32_t lp = s_excl[dig[i]] + s_wcnt[wave][dig[i]] + pos[i];
+      if (lp < (uint32_t)kTile) {
+        s_key[lp] = key[i];
+        s_val[lp] = val[i];
+      }
+    }
+  }
+  __syncthreads();
+  const uint32_t tile_n = (n - tile_base) < (uint32_t)kTile ? (n - tile_base) : (uint32_t)kTile;
+  for (uint32_t j = tid; j < tile_n; j += kThreads) {
+    const KeyT k = s_key[j];
+    const uint32_t g = s_gbase[digit_of(k, shift, mask)] + j;
+    if (g < n) {
+      kout[g] = k;
+      vout[g] = s_val[j];
+    } else {
+      raise_error(err, DMSH_DEVERR_SORT_PROGRESS);
+    }
+  }
+}
+
+template <typename KeyT>
+inline uint32_t num_tiles(uint32_t n) {
+  return ceil_div(n, (uint64_t)kThreads * Tune<KeyT>::kItems);
+}
+
+inline uint32_t num_passes(uint32_t key_bits) { return ceil_div(key_bits, kRadixBits); }
+
+// Workspace: hist[8][256] + tickets[8] + status[P][tiles][256] + alt keys + alt values.
+template <typename KeyT>
+inline size_t workspace_bytes(uint32_t n, uint32_t key_bits) {
+  const size_t P = num_passes(key_bits);
+  Arena a{nullptr, 0, 0};
+  a.take<uint32_t>(kMaxPasses * kRadix + kMaxPasses);
+  a.take<uint32_t>(P * (size_t)num_tiles<KeyT>(n) * kRadix);
+  a.take<KeyT>(n);
+  a.take<uint32_t>(n);
+  return a.off + 256;
+}
+
+struct Layout {
+  uint32_t* hist;     // [kMaxPasses][kRadix], then tickets[kMaxPasses]
+  uint32_t* tickets;
+  uint32_t* status;   // [P][tiles][kRadix]
+  size_t zero_words;  // hist + tickets + status are contiguous from `hist`
+  void* alt_keys;
+  uint32_t* alt_vals;
+};
+
+template <typename KeyT>
+inline bool carve(Arena& a, uint32_t n, uint32_t key_bits, Layout& L) {
+  const size_t P = num_passes(key_bits);
+  L.hist = a.take<uint32_t>(kMaxPasses * kRadix + kMaxPasses);
+  L.tickets = L.hist + kMaxPasses * kRadix;
+  const size_t status_words = P * (size_t)num_tiles<KeyT>(n) * kRadix;
+  L.status = a.take<uint32_t>(status_words);
+  L.zero_words = (size_t)(L.status - L.hist) + status_words;
+  L.alt_keys = a.take<KeyT>(n);
+  L.alt_vals = a.take<uint32_t>(n);
+  return a.ok();
+}
+
+inline int launch_fill(hipStream_t s, uint32_t* p, size_t count, uint32_t value) {
+  if (count == 0) return DMSH_OK;
+  const uint32_t blocks = (uint32_t)std::min<size_t>(ceil_div(count, 256), 4096);
+  hipLaunchKernelGGL(k_fill_u32, dim3(blocks), dim3(256), 0, s, p, count, value);
+  return launch_status();
+}
+
+// Runs the digit passes over keys already in `src_keys`.  The histogram of
+// every pass must already be in L.hist (zeroed + accumulated) and L.status /
+// L.tickets must be zero.  The sorted keys and permutation end in
+// (dst_keys, dst_vals); `src_keys` may alias dst_keys only when the number of
+// passes is even (the caller arranges this, see argsort()).
+template <typename KeyT>
+inline int run_passes(hipStream_t s, uint32_t n, uint32_t key_bits, const Layout& L,
+                      const KeyT* first_src, KeyT* dst_keys, uint32_t* dst_vals, uint32_t* err) {
+  constexpr int ITEMS = Tune<KeyT>::kItems;
+  const uint32_t P = num_passes(key_bits);
+  const uint32_t tiles = num_tiles<KeyT>(n);
+  KeyT* alt_k = static_cast<KeyT*>(L.alt_keys);
+  uint32_t* alt_v = L.alt_vals;
+  // Destination of pass p: the user buffers for the last pass, alternating back.
+  const KeyT* src_k = first_src;
+  const uint32_t* src_v = nullptr;
+  for (uint32_t p = 0; p < P; ++p) {
+    const bool to_user = ((P - 1 - p) % 2) == 0;
+    KeyT* dk = to_user ? dst_keys : alt_k;
+    uint32_t* dv = to_user ? dst_vals : alt_v;
+    const uint32_t shift = p * kRadixBits;
+    uint32_t dbits = key_bits - shift;
+    if (dbits > (uint32_t)kRadixBits) dbits = kRadixBits;
+    uint32_t* st = L.status + (size_t)p * tiles * kRadix;
+    if (p == 0) {
+      hipLaunchKernelGGL((k_onesweep<KeyT, ITEMS, true>), dim3(tiles), dim3(kThreads), 0, s, src_k,
+                         src_v, dk, dv, n, shift, dbits, L.hist + p * kRadix, st, L.tickets + p,
+                         err);
+    } else {
+      hipLaunchKernelGGL((k_onesweep<KeyT, ITEMS, false>), dim3(tiles), dim3(kThreads), 0, s,
+                         src_k, src_v, dk, dv, n, shift, dbits, L.hist + p * kRadix, st,
+                         L.tickets + p, err);
+    }
+    int rc = launch_status();
+    if (rc != DMSH_OK) return rc;
+    src_k = dk;
+    src_v = dv;
+  }
+  return DMSH_OK;
+}
+
+// Full argsort: keys_inout (sorted on return), idx_out (permutation).
+template <typename KeyT>
+inline int argsort(hipStream_t s, uint32_t n, KeyT* keys_inout, uint32_t* idx_out,
+                   uint32_t key_bits, void* ws, size_t ws_bytes) {
+  if (n == 0) return DMSH_OK;
+  if (!keys_inout || !idx_out) return DMSH_ERR_INVALID;
+  if (n >= (1u << 30)) return DMSH_ERR_INVALID;
+  if (key_bits == 0 || key_bits > 8 * sizeof(KeyT)) return DMSH_ERR_INVALID;
+  if (!ws || ws_bytes < workspace_bytes<KeyT>(n, key_bits)) return DMSH_ERR_WORKSPACE;
+  Arena a{static_cast<char*>(ws), ws_bytes, 0};
+  Layout L;
+  if (!carve<KeyT>(a, n, key_bits, L)) return DMSH_ERR_WORKSPACE;
+  uint32_t* err = device_error_word();
+  if (!err) return DMSH_ERR_HIP;
+  int rc = launch_fill(s, L.hist, L.zero_words, 0u);
+  if (rc != DMSH_OK) return rc;
+  const uint32_t P = num_passes(key_bits);
+  const uint32_t hist_blocks = std::min<uint32_t>(num_tiles<KeyT>(n), 2048);
+  hipLaunchKernelGGL(k_hist_all<KeyT>, dim3(hist_blocks), dim3(256), 0, s, keys_inout, n, P,
+                     key_bits, L.hist);
+  if ((rc = launch_status()) != DMSH_OK) return rc;
+  const KeyT* first_src = keys_inout;
+  if (P % 2 == 1) {
+    // An odd pass count would end in the scratch buffer: start from a copy.
+    KeyT* alt_k = static_cast<KeyT*>(L.alt_keys);
+    hipLaunchKernelGGL(k_copy_keys<KeyT>, dim3(std::min<uint32_t>(ceil_div(n, 256), 4096)),
+                       dim3(256), 0, s, keys_inout, alt_k, n);
+    if ((rc = launch_status()) != DMSH_OK) return rc;
+    first_src = alt_k;
+  }
+  return run_passes<KeyT>(s, n, key_bits, L, first_src, keys_inout, idx_out, err);
+}
+
+}  // namespace sortk
+}  // namespace dmsh

@@ -1,0 +1,141 @@
+// topology.hpp -- Karras (2012) binary radix tree over sorted Morton codes.
+//
+// One thread per internal node i (0 .. n-2) computes the node's key range and
+// split exactly as del-msh-cpu/src/mortons.rs:221-325 defines them, including
+// the duplicate-code rule (a node inside a run of equal codes covers the rest
+// of the run and splits at its first element), then writes the child links
+// of bvhnodes_morton.rs:18-49.  The run end of the duplicate rule is found by
+// binary search over the sorted codes instead of the reference's linear scan:
+// both return the last index of the run, the search in O(log n).
+#pragma once
+
+#include "dmsh_common.hpp"
+
+namespace dmsh {
+
+__device__ __forceinline__ int clz_key(uint32_t v) { return v ? __builtin_clz(v) : 32; }
+__device__ __forceinline__ int clz_key(uint64_t v) { return v ? __builtin_clzll(v) : 64; }
+
+template <typename KeyT>
+struct KeyTraits {
+  static constexpr int kBits = 8 * sizeof(KeyT);
+};
+
+// Range [first, last] of internal node idx (1 <= idx <= n-2, or idx == 0).
+template <typename KeyT>
+__device__ __forceinline__ void node_range(const KeyT* __restrict__ c, uint32_t n, uint32_t idx,
+                                           uint32_t& first, uint32_t& last) {
+  if (idx == 0) {
+    first = 0;
+    last = n - 1;
+    return;
+  }
+  const KeyT mc0 = c[idx - 1], mc1 = c[idx], mc2 = c[idx + 1];
+  if (mc0 == mc1 && mc1 == mc2) {
+    uint32_t lo = idx + 1, hi = n - 1;  // c[lo] == mc1 holds
+    while (lo < hi) {
+      const uint32_t mid = lo + (hi - lo + 1) / 2;
+      if (c[mid] == mc1) lo = mid;
+      else hi = mid - 1;
+    }
+    first = idx;
+    last = lo;
+    return;
+  }
+  const int64_t i = idx;
+  const int dn = clz_key(mc1 ^ mc2), dp = clz_key(mc1 ^ mc0);
+  const int64_t d = (dn - dp) > 0 ? 1 : -1;
+  const int delta_min = (d > 0) ? dp : dn;  // delta(idx, idx - d)
+  int64_t lmax = 2;
+  for (;;) {
+    const int64_t j = i + lmax * d;
+    if (j < 0 || j >= (int64_t)n) break;
+    if (clz_key(mc1 ^ c[j]) <= delta_min) break;
+    lmax *= 2;
+  }
+  int64_t l = 0;
+  for (int64_t t = lmax / 2; t >= 1; t /= 2) {
+    const int64_t j = i + (l + t) * d;
+    if (j >= 0 && j < (int64_t)n && clz_key(mc1 ^ c[j]) > delta_min) l += t;
+  }
+  const int64_t j = i + l * d;
+  if (i <= j) {
+    first = idx;
+    last = (uint32_t)j;
+  } else {
+    first = (uint32_t)j;
+    last = idx;
+  }
+}
+
+// Split of range [first, last] (first < last), mortons.rs:290-325.
+template <typename KeyT>
+__device__ __forceinline__ uint32_t node_split(const KeyT* __restrict__ c, uint32_t first,
+                                               uint32_t last) {
+  const KeyT ms = c[first];
+  const int p0 = clz_key(ms ^ c[last]);
+  if (p0 == KeyTraits<KeyT>::kBits) return first;
+  uint32_t split = first, step = last - first;
+  while (step > 1) {
+    step = (step + 1) >> 1;
+    const uint32_t nw = split + step;
+    if (nw >= last) continue;
+    if (clz_key(ms ^ c[nw]) > p0) split = nw;
+  }
+  return split;
+}
+
+template <typename KeyT>
+__global__ __launch_bounds__(256) void k_topology(uint32_t n, uint32_t* __restrict__ bvhnodes,
+                                                  const KeyT* __restrict__ codes,
+                                                  const uint32_t* __restrict__ idx2obj) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t nb = n - 1;
+  if (i >= nb) return;
+  uint32_t first, last;
+  node_range<KeyT>(codes, n, i, first, last);
+  const uint32_t split = node_split<KeyT>(codes, first, last);
+  if (first == split) {
+    const uint32_t a = nb + split;
+    bvhnodes[(size_t)i * 3 + 1] = a;
+    bvhnodes[(size_t)a * 3 + 0] = i;
+    bvhnodes[(size_t)a * 3 + 1] = idx2obj[split];
+    bvhnodes[(size_t)a * 3 + 2] = DMSH_U32MAX;
+  } else {
+    bvhnodes[(size_t)i * 3 + 1] = split;
+    bvhnodes[(size_t)split * 3 + 0] = i;
+  }
+  if (last == split + 1) {
+    const uint32_t b = nb + split + 1;
+    bvhnodes[(size_t)i * 3 + 2] = b;
+    bvhnodes[(size_t)b * 3 + 0] = i;
+    bvhnodes[(size_t)b * 3 + 1] = idx2obj[split + 1];
+    bvhnodes[(size_t)b * 3 + 2] = DMSH_U32MAX;
+  } else {
+    bvhnodes[(size_t)i * 3 + 2] = split + 1;
+    bvhnodes[(size_t)(split + 1) * 3 + 0] = i;
+  }
+  if (i == 0) bvhnodes[0] = DMSH_U32MAX;
+}
+
+static __global__ void k_topology_single(uint32_t* __restrict__ bvhnodes,
+                                  const uint32_t* __restrict__ idx2obj) {
+  bvhnodes[0] = DMSH_U32MAX;
+  bvhnodes[1] = idx2obj[0];
+  bvhnodes[2] = DMSH_U32MAX;
+}
+
+template <typename KeyT>
+inline int launch_topology(hipStream_t s, uint32_t n, uint32_t* bvhnodes, const KeyT* codes,
+                           const uint32_t* idx2obj) {
+  if (n == 0) return DMSH_OK;
+  if (n == 1) {
+    hipLaunchKernelGGL(k_topology_single, dim3(1), dim3(1), 0, s, bvhnodes, idx2obj);
+  } else {
+    hipLaunchKernelGGL(k_topology<KeyT>, dim3(ceil_div(n - 1, 256)), dim3(256), 0, s, n, bvhnodes,
+                       codes, idx2obj);
+  }
+  return launch_status();
+}
+
+}  // namespace dmsh

@@ -1,0 +1,25 @@
+"""Mid-level ray-cast (reference: del_msh_dlpack/Pix2Tri/__init__.py:1-15)."""
+
+from .. import _lib
+from ..util_torch import as_tensor
+
+
+def update_pix2tri(tri2vtx, vtx2xyz, bvhnodes, bvhnode2aabb, transform_ndc2world, pix2tri,
+                   stream_ptr=0, pix2t=None):
+    """Fills pix2tri [H, W] (and the hit coefficient pix2t if given) by BVH
+    ray-casting; transform_ndc2world is column-major (16 floats)."""
+    tri2vtx, vtx2xyz, bvhnodes, bvhnode2aabb, transform_ndc2world, pix2tri = map(
+        as_tensor, (tri2vtx, vtx2xyz, bvhnodes, bvhnode2aabb, transform_ndc2world, pix2tri))
+    dev = _lib.require_gpu(tri2vtx, vtx2xyz, bvhnodes, bvhnode2aabb, transform_ndc2world, pix2tri)
+    img_h, img_w = pix2tri.shape
+    num_tri = tri2vtx.shape[0]
+    assert bvhnodes.shape[0] == max(2 * num_tri - 1, 0)
+    ws = _lib.workspace(_lib.lib().dmsh_pix2tri_workspace_bytes(img_w, img_h), dev)
+    stream = stream_ptr or _lib.stream_of(dev)
+    _lib.check(
+        _lib.lib().dmsh_pix2tri(
+            stream, _lib.ptr(pix2tri), _lib.ptr(pix2t), num_tri, _lib.ptr(tri2vtx),
+            vtx2xyz.shape[0], _lib.ptr(vtx2xyz), img_w, img_h, _lib.ptr(transform_ndc2world),
+            _lib.ptr(bvhnodes), _lib.ptr(bvhnode2aabb), _lib.ptr(ws), ws.numel()),
+        "dmsh_pix2tri",
+    )

@@ -1,0 +1,153 @@
+"""ctypes binding of libdmsh (the gfx950 C ABI declared in include/dmsh.h).
+
+The library is loaded from ``del-msh_amd/lib/libdmsh.so`` (override with the
+``DMSH_LIBRARY`` environment variable).  There is no fallback: if the library
+is missing, or a tensor is not on a ROCm device, the call raises.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+import pathlib
+
+import torch
+
+_LIB = None
+
+c_void_p = ctypes.c_void_p
+c_u32 = ctypes.c_uint32
+c_f32 = ctypes.c_float
+c_size = ctypes.c_size_t
+c_int = ctypes.c_int
+
+# name -> (restype, argtypes)
+_PROTOTYPES = {
+    "dmsh_status_string": (ctypes.c_char_p, [c_int]),
+    "dmsh_abi_version": (c_int, []),
+    "dmsh_get_device_errors": (c_int, [ctypes.POINTER(c_u32)]),
+    "dmsh_vtx2morton_u32": (c_int, [c_void_p, c_u32, c_void_p, c_u32, c_void_p, c_void_p]),
+    "dmsh_vtx2morton_u64": (c_int, [c_void_p, c_u32, c_void_p, c_void_p, c_void_p]),
+    "dmsh_argsort_workspace_bytes": (c_size, [c_u32, c_u32]),
+    "dmsh_argsort_u32": (c_int, [c_void_p, c_u32, c_void_p, c_void_p, c_u32, c_void_p, c_size]),
+    "dmsh_argsort_u64": (c_int, [c_void_p, c_u32, c_void_p, c_void_p, c_u32, c_void_p, c_size]),
+    "dmsh_bvhnodes_from_sorted_mortons_u32": (c_int, [c_void_p, c_u32, c_void_p, c_void_p, c_void_p]),
+    "dmsh_bvhnodes_from_sorted_mortons_u64": (c_int, [c_void_p, c_u32, c_void_p, c_void_p, c_void_p]),
+    "dmsh_refit_workspace_bytes": (c_size, [c_u32]),
+    "dmsh_bvhnode2aabb_trimesh3": (
+        c_int,
+        [c_void_p, c_void_p, c_u32, c_void_p, c_u32, c_void_p, c_u32, c_void_p, c_f32, c_void_p, c_size],
+    ),
+    "dmsh_bvhnode2aabb_points": (
+        c_int,
+        [c_void_p, c_void_p, c_u32, c_void_p, c_u32, c_void_p, c_void_p, c_size],
+    ),
+    "dmsh_build_workspace_bytes": (c_size, [c_u32, c_u32]),
+    "dmsh_build_trimesh3": (
+        c_int,
+        [c_void_p, c_u32, c_void_p, c_u32, c_void_p, c_u32, c_void_p, c_void_p, c_void_p, c_size],
+    ),
+    "dmsh_build_points": (
+        c_int,
+        [c_void_p, c_u32, c_void_p, c_u32, c_void_p, c_void_p, c_void_p, c_size],
+    ),
+    "dmsh_pix2tri_workspace_bytes": (c_size, [c_u32, c_u32]),
+    "dmsh_pix2tri": (
+        c_int,
+        [c_void_p, c_void_p, c_void_p, c_u32, c_void_p, c_u32, c_void_p, c_u32, c_u32, c_void_p,
+         c_void_p, c_void_p, c_void_p, c_size],
+    ),
+    "dmsh_pix2depth": (
+        c_int,
+        [c_void_p, c_void_p, c_void_p, c_u32, c_void_p, c_u32, c_void_p, c_u32, c_u32, c_void_p],
+    ),
+    "dmsh_rasterized_edge_gradient_bwd": (
+        c_int,
+        [c_void_p, c_void_p, c_u32, c_void_p, c_u32, c_void_p, c_void_p, c_u32, c_u32, c_void_p,
+         c_u32, c_void_p, c_void_p],
+    ),
+}
+
+
+def library_path() -> pathlib.Path:
+    env = os.environ.get("DMSH_LIBRARY")
+    if env:
+        return pathlib.Path(env)
+    return pathlib.Path(__file__).resolve().parent.parent / "lib" / "libdmsh.so"
+
+
+def lib():
+    """Loads libdmsh once and attaches the C prototypes; raises if absent."""
+    global _LIB
+    if _LIB is None:
+        path = library_path()
+        if not path.exists():
+            raise RuntimeError(
+                f"libdmsh not found at {path}; build it with `python -c 'import __graft_entry__ as g; g.build()'`"
+            )
+        handle = ctypes.CDLL(str(path))
+        for name, (restype, argtypes) in _PROTOTYPES.items():
+            fn = getattr(handle, name)
+            fn.restype = restype
+            fn.argtypes = argtypes
+        _LIB = handle
+    return _LIB
+
+
+def check(status: int, what: str) -> None:
+    if status != 0:
+        msg = lib().dmsh_status_string(status).decode()
+        raise RuntimeError(f"{what} failed: {msg} (status {status})")
+
+
+def device_errors() -> int:
+    """Reads and clears the sticky device error bits (synchronizes)."""
+    bits = c_u32(0)
+    check(lib().dmsh_get_device_errors(ctypes.byref(bits)), "dmsh_get_device_errors")
+    return int(bits.value)
+
+
+DEVERR_NAMES = {
+    0x1: "index out of range",
+    0x2: "ray traversal stack overflow",
+    0x4: "radix-sort look-back gave up",
+    0x8: "broken tree in refit",
+    0x10: "singular transform",
+}
+
+
+def raise_on_device_errors() -> None:
+    bits = device_errors()
+    if bits:
+        names = [v for k, v in DEVERR_NAMES.items() if bits & k]
+        raise RuntimeError(f"libdmsh device error bits 0x{bits:x}: {', '.join(names)}")
+
+
+def require_gpu(*tensors: torch.Tensor) -> torch.device:
+    """All tensors must live on one ROCm device; returns it and makes it current."""
+    dev = tensors[0].device
+    for t in tensors:
+        if t.device.type != "cuda":
+            raise NotImplementedError(
+                "del_msh_dlpack (MI355X build) runs on ROCm device tensors only; "
+                f"got a tensor on {t.device}"
+            )
+        if t.device != dev:
+            raise ValueError(f"tensors on different devices: {dev} vs {t.device}")
+    torch.cuda.set_device(dev)
+    return dev
+
+
+def stream_of(device: torch.device) -> int:
+    return torch.cuda.current_stream(device).cuda_stream
+
+
+def ptr(t: torch.Tensor | None) -> int | None:
+    if t is None:
+        return None
+    assert t.is_contiguous()
+    return t.data_ptr()
+
+
+def workspace(nbytes: int, device: torch.device) -> torch.Tensor:
+    return torch.empty((max(int(nbytes), 1),), dtype=torch.uint8, device=device)

@@ -1,0 +1,139 @@
+"""Host-side triangle-mesh generators (inputs for the hot path).
+
+Topologies follow del-msh-cpu/src/trimesh3_primitive.rs; coordinates are
+computed in float64 and rounded once to float32, so every consumer (device
+kernels and the CPU oracle) receives identical bytes.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+
+
+def cylinder_like_topology(ndiv_side: int, ndiv_circumference: int) -> np.ndarray:
+    """trimesh3_primitive.rs:117-156: bottom fan, side quads, top fan."""
+    nc = ndiv_circumference
+    nlong = ndiv_side + 2
+    ic = np.arange(nc)
+    bottom = np.stack([np.zeros(nc, np.int64), ic + 1, (ic + 1) % nc + 1], axis=1)
+    ih = np.arange(nlong - 2)[:, None]
+    i1 = ih * nc + 1 + ic % nc
+    i2 = ih * nc + 1 + (ic + 1) % nc
+    i3 = (ih + 1) * nc + 1 + (ic + 1) % nc
+    i4 = (ih + 1) * nc + 1 + ic % nc
+    side = np.stack([i3, i2, i1, i4, i3, i1], axis=-1).reshape(-1, 3)
+    i0 = np.full(nc, nc * (nlong - 1) + 1)
+    j1 = (nlong - 2) * nc + 1 + (ic + 1) % nc
+    j2 = (nlong - 2) * nc + 1 + ic % nc
+    top = np.stack([i0, j1, j2], axis=1)
+    return np.concatenate([bottom, side, top]).astype(np.uint32)
+
+
+def sphere_yup(radius: float, n_longitude: int, n_latitude: int):
+    """trimesh3_primitive.rs:377-420 (pole on +y).
+    2 n_latitude (n_longitude - 1) triangles, 2 + (n_longitude - 1) n_latitude vertices."""
+    if n_longitude <= 1 or n_latitude <= 2:
+        return np.zeros((0, 3), np.uint32), np.zeros((0, 3), np.float32)
+    dl = np.pi / n_longitude
+    dr = 2.0 * np.pi / n_latitude
+    pts = [[0.0, radius, 0.0]]
+    ila = np.arange(1, n_longitude)[:, None]
+    ilo = np.arange(n_latitude)[None, :]
+    y0 = np.cos(dl * ila) * np.ones_like(ilo)
+    r0 = np.sin(dl * ila)
+    x0 = r0 * np.sin(dr * ilo)
+    z0 = r0 * np.cos(dr * ilo)
+    mid = radius * np.stack([x0, y0, z0], axis=-1).reshape(-1, 3)
+    # rings 0 and n_longitude keep only their first vertex (ilo = 0)
+    ys, rs = np.cos(dl * n_longitude), np.sin(dl * n_longitude)
+    south = radius * np.array([[rs * np.sin(0.0), ys, rs * np.cos(0.0)]])
+    vtx = np.concatenate([np.array(pts), mid, south]).astype(np.float32)
+    tri = cylinder_like_topology(n_longitude - 2, n_latitude)
+    return tri, vtx
+
+
+def torus_zup(major_radius: float, minor_radius: float, ndiv_major: int, ndiv_minor: int):
+    """trimesh3_primitive.rs:326-366; 2 ndiv_major ndiv_minor triangles."""
+    rlg = 2.0 * np.pi / ndiv_major
+    rlt = 2.0 * np.pi / ndiv_minor
+    ilg = np.arange(ndiv_major)[:, None]
+    ilt = np.arange(ndiv_minor)[None, :]
+    lt = ilt * rlt
+    lg = ilg * rlg
+    r0 = major_radius + minor_radius * np.cos(lt)
+    x = r0 * np.sin(lg)
+    y = r0 * np.cos(lg)
+    z = minor_radius * np.sin(lt) * np.ones_like(lg)
+    vtx = np.stack([x, y, z], axis=-1).reshape(-1, 3).astype(np.float32)
+    iug = (ilg + 1) % ndiv_major
+    iut = (ilt + 1) % ndiv_minor
+    a = ilg * ndiv_minor + ilt
+    b = iug * ndiv_minor + ilt
+    c = iug * ndiv_minor + iut
+    d = ilg * ndiv_minor + iut * np.ones_like(ilg)
+    tri = np.stack([a, c, b, a, d, c], axis=-1).reshape(-1, 3).astype(np.uint32)
+    return tri, vtx
+
+
+def cylinder_closed_end_yup(radius: float, length: float, ndiv_circumference: int,
+                            ndiv_length: int, is_center: bool = True):
+    """trimesh3_primitive.rs:159-203; 2 ndiv_circumference (ndiv_length + 1) triangles."""
+    nc = ndiv_circumference
+    dl = length / ndiv_length
+    dr = 2.0 * np.pi / nc
+    y_min = -0.5 * length if is_center else 0.0
+    il = np.arange(ndiv_length + 1)[:, None]
+    ilo = np.arange(nc)[None, :]
+    x = radius * np.cos(dr * ilo) * np.ones_like(il)
+    y = (y_min + dl * il) * np.ones_like(ilo)
+    z = radius * np.sin(dr * ilo) * np.ones_like(il)
+    ring = np.stack([x, y, z], axis=-1).reshape(-1, 3)
+    vtx = np.concatenate([[[0.0, y_min, 0.0]], ring, [[0.0, y_min + length, 0.0]]])
+    tri = cylinder_like_topology(ndiv_length, nc)
+    return tri, vtx.astype(np.float32)
+
+
+def octahedral_sphere(level: int, radius: float = 1.0, noise_sigma: float = 0.0,
+                      seed: int = 0):
+    """Closed sphere from an octahedron subdivided `level` times:
+    8 * 4^level triangles, 4 * 4^level + 2 vertices (level 9 gives exactly 2^21
+    triangles, which no icosphere level does).  Built on the octahedral
+    unfolding of the square [-M, M]^2, M = 2^level: the grid cells are split
+    along the octahedron's edges and the folded square border is welded.
+    Optional radial noise: r = radius * (1 + N(0, noise_sigma^2)) per vertex."""
+    M = 1 << level
+    g = np.arange(-M, M + 1)
+    X, Y = np.meshgrid(g, g, indexing="ij")
+    ax, ay = np.abs(X), np.abs(Y)
+    upper = ax + ay <= M
+    Z = np.where(upper, M - ax - ay, -(ax + ay - M))
+    Xf = np.where(upper, X, np.sign(X) * (M - ay))
+    Yf = np.where(upper, Y, np.sign(Y) * (M - ax))
+    key = np.stack([Xf, Yf, Z], axis=-1).reshape(-1, 3)
+    uniq, inv = np.unique(key, axis=0, return_inverse=True)
+    inv = inv.reshape(-1)
+    n = 2 * M + 1
+    gid = inv.reshape(n, n)
+    i = np.arange(2 * M)[:, None]
+    j = np.arange(2 * M)[None, :]
+    v00 = gid[i, j]
+    v10 = gid[i + 1, j]
+    v01 = gid[i, j + 1]
+    v11 = gid[i + 1, j + 1]
+    cx = i + 0.5 - M
+    cy = j + 0.5 - M
+    anti = (cx * cy) > 0  # quadrants where the face edge runs along x + y = const
+    t0 = np.where(anti[..., None], np.stack([v00, v10, v01], -1), np.stack([v00, v10, v11], -1))
+    t1 = np.where(anti[..., None], np.stack([v10, v11, v01], -1), np.stack([v00, v11, v01], -1))
+    tri = np.concatenate([t0.reshape(-1, 3), t1.reshape(-1, 3)]).astype(np.int64)
+    p = uniq.astype(np.float64)
+    p /= np.linalg.norm(p, axis=1, keepdims=True)
+    if noise_sigma > 0.0:
+        rng = np.random.default_rng(seed)
+        p *= (1.0 + rng.normal(0.0, noise_sigma, size=(p.shape[0], 1)))
+    p *= radius
+    # orient every triangle outward
+    a, b, c = p[tri[:, 0]], p[tri[:, 1]], p[tri[:, 2]]
+    flip = np.einsum("ij,ij->i", np.cross(b - a, c - a), a + b + c) < 0
+    tri[flip] = tri[flip][:, [0, 2, 1]]
+    return tri.astype(np.uint32), p.astype(np.float32)

@@ -1,0 +1,188 @@
+/*
+ * dmsh.h -- C ABI of libdmsh, the MI355X (gfx950) implementation of the
+ * del-msh LBVH / ray-cast / differentiable-rasterizer hot path.
+ *
+ * This ABI replaces the launch plumbing that del-msh reaches through
+ * del-msh-cudarc-sys + del-cudarc-sys (cuModuleLoadData / cuLaunchKernel on
+ * nvcc fatbins, del-msh-cudarc-sys/src/lib.rs:7-27) with host launchers that
+ * take plain device pointers, sizes and a HIP stream.  Conventions:
+ *   - `stream` is a hipStream_t passed as void* (NULL = default stream); every
+ *     entry point only enqueues work on it and returns without synchronizing.
+ *   - all arrays are device pointers owned by the caller; layouts are the
+ *     reference ones (row-major [n][k], uint32 indices, float32 values,
+ *     4x4 transforms column-major as the Python layer passes them: M.T).
+ *   - entry points that need scratch take (ws, ws_bytes); the matching
+ *     *_workspace_bytes() gives the size.  Nothing is allocated inside.
+ *   - return value: DMSH_OK or an error code; arguments are validated on the
+ *     host before anything is enqueued.  Data errors found on the device
+ *     (an index out of range in bvhnodes / tri2vtx, a traversal stack
+ *     overflow) set sticky bits readable with dmsh_get_device_errors().
+ */
+#ifndef DMSH_H
+#define DMSH_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DMSH_OK 0
+#define DMSH_ERR_INVALID 1   /* bad size / null pointer / unsupported option */
+#define DMSH_ERR_HIP 2       /* a HIP runtime call or launch failed          */
+#define DMSH_ERR_WORKSPACE 3 /* workspace smaller than *_workspace_bytes()    */
+
+/* device error bits (dmsh_get_device_errors) */
+#define DMSH_DEVERR_INDEX 0x1u          /* index out of range in input data   */
+#define DMSH_DEVERR_STACK 0x2u          /* ray traversal needed > 4096 frames */
+#define DMSH_DEVERR_SORT_PROGRESS 0x4u  /* radix-sort look-back gave up       */
+#define DMSH_DEVERR_REFIT 0x8u          /* refit climb found a broken tree    */
+#define DMSH_DEVERR_SINGULAR 0x10u      /* transform could not be inverted    */
+
+const char *dmsh_status_string(int status);
+int dmsh_abi_version(void);
+
+/* Reads and clears the sticky device error bits of the current device
+ * (synchronizes the device). */
+int dmsh_get_device_errors(uint32_t *bits_out);
+
+/* ---------------------------------------------------------------------
+ * Morton codes
+ * Replaces kernel `vtx2morton` (del-msh-cuda-kernels/cuda/mortons.cu:7-36)
+ * launched by mortons_vtx2morton_from_vtx2co (del-msh-dlpack/src/mortons.rs:17-77);
+ * CPU counterpart del-msh-cpu/src/mortons.rs:171-212.
+ * vtx2co [num_vtx][num_dim] (num_dim 2 or 3), transform (num_dim+1)^2
+ * column-major.  2D: 16 bits/axis; 3D: 10 bits/axis (30-bit codes).
+ * ------------------------------------------------------------------- */
+int dmsh_vtx2morton_u32(void *stream, uint32_t num_vtx, const float *vtx2co,
+                        uint32_t num_dim, const float *transform_co2unit,
+                        uint32_t *vtx2morton);
+/* 63-bit extension (21 bits/axis, 3D only); no reference counterpart. */
+int dmsh_vtx2morton_u64(void *stream, uint32_t num_vtx, const float *vtx2xyz,
+                        const float *transform_co2unit, uint64_t *vtx2morton);
+
+/* ---------------------------------------------------------------------
+ * Stable argsort of keys (LSD radix sort, `key_bits` low bits significant).
+ * Replaces the GPU branch of array1d_argsort
+ * (del-msh-dlpack/src/array1d.rs:99-154, thrust sort_u32_by_u32_key at :146);
+ * CPU counterpart: the stable `sort_by_key` of del-msh-cpu/src/mortons.rs:143.
+ * On return keys_inout holds the sorted keys and idx_out the permutation
+ * (idx_out[j] = original index of the j-th smallest key; ties keep input order).
+ * n < 2^30.
+ * ------------------------------------------------------------------- */
+size_t dmsh_argsort_workspace_bytes(uint32_t n, uint32_t key_bytes);
+int dmsh_argsort_u32(void *stream, uint32_t n, uint32_t *keys_inout,
+                     uint32_t *idx_out, uint32_t key_bits, void *ws,
+                     size_t ws_bytes);
+int dmsh_argsort_u64(void *stream, uint32_t n, uint64_t *keys_inout,
+                     uint32_t *idx_out, uint32_t key_bits, void *ws,
+                     size_t ws_bytes);
+
+/* ---------------------------------------------------------------------
+ * Karras LBVH topology from sorted codes.
+ * Replaces kernel `kernel_MortonCode_BVHTopology`
+ * (del-msh-cuda-kernels/cuda/bvhnodes_morton.cu:28-72) launched by
+ * mortons_make_bvhnodes_from_sorted_mortons (del-msh-dlpack/src/mortons.rs:79-135);
+ * CPU counterpart del-msh-cpu/src/bvhnodes_morton.rs:10-50.
+ * bvhnodes [2n-1][3] = (parent, left|obj, right|UINT32_MAX); root 0 has
+ * parent UINT32_MAX; leaves are rows n-1 .. 2n-2.  n == 1 writes the single
+ * leaf row (UINT32_MAX, idx2obj[0], UINT32_MAX).
+ * ------------------------------------------------------------------- */
+int dmsh_bvhnodes_from_sorted_mortons_u32(void *stream, uint32_t n,
+                                          uint32_t *bvhnodes,
+                                          const uint32_t *idx2morton,
+                                          const uint32_t *idx2obj);
+int dmsh_bvhnodes_from_sorted_mortons_u64(void *stream, uint32_t n,
+                                          uint32_t *bvhnodes,
+                                          const uint64_t *idx2morton,
+                                          const uint32_t *idx2obj);
+
+/* ---------------------------------------------------------------------
+ * Bottom-up AABB refit.
+ * Replaces kernel `from_trimesh3` (del-msh-cuda-kernels/cuda/bvhnode2aabb.cu:6-65)
+ * launched by bvhnode2aabb_update_aabb (del-msh-dlpack/src/bvhnode2aabb.rs:13-108);
+ * CPU counterpart del-msh-cpu/src/bvhnode2aabb3.rs:68-142 (triangles) and
+ * :6-63 (points).  bvhnode2aabb [num_bvhnode][6] = (min xyz, max xyz).
+ * The tree must be an LBVH layout (leaves in rows num_leaf-1 .. 2 num_leaf-2).
+ * ------------------------------------------------------------------- */
+size_t dmsh_refit_workspace_bytes(uint32_t num_leaf);
+int dmsh_bvhnode2aabb_trimesh3(void *stream, float *bvhnode2aabb,
+                               uint32_t num_bvhnode, const uint32_t *bvhnodes,
+                               uint32_t num_tri, const uint32_t *tri2vtx,
+                               uint32_t num_vtx, const float *vtx2xyz,
+                               float eps, void *ws, size_t ws_bytes);
+int dmsh_bvhnode2aabb_points(void *stream, float *bvhnode2aabb,
+                             uint32_t num_bvhnode, const uint32_t *bvhnodes,
+                             uint32_t num_pnt, const float *vtx2xyz, void *ws,
+                             size_t ws_bytes);
+
+/* ---------------------------------------------------------------------
+ * Whole LBVH build of a triangle mesh / a point set, device resident:
+ * centroid -> unit-cube fit -> Morton (30-bit u32 or 63-bit u64) -> stable
+ * radix argsort -> Karras topology -> refit.  Replaces the Python driver
+ * TriMesh3.torch.make_bvhnodes_bvhnode2aabb
+ * (del-msh-dlpack/del_msh_dlpack/TriMesh3/torch.py:233-285), which chains
+ * the five launches above through torch; CPU counterpart
+ * del-msh-cpu/src/bvhnodes_morton.rs:151-189 + bvhnode2aabb3.rs:168-195.
+ * morton_bits: 30 or 63.
+ * ------------------------------------------------------------------- */
+size_t dmsh_build_workspace_bytes(uint32_t num_elem, uint32_t morton_bits);
+int dmsh_build_trimesh3(void *stream, uint32_t num_tri, const uint32_t *tri2vtx,
+                        uint32_t num_vtx, const float *vtx2xyz,
+                        uint32_t morton_bits, uint32_t *bvhnodes,
+                        float *bvhnode2aabb, void *ws, size_t ws_bytes);
+int dmsh_build_points(void *stream, uint32_t num_pnt, const float *vtx2xyz,
+                      uint32_t morton_bits, uint32_t *bvhnodes,
+                      float *bvhnode2aabb, void *ws, size_t ws_bytes);
+
+/* ---------------------------------------------------------------------
+ * Per-pixel BVH ray-cast with the ordered near-first semantics of
+ * del-msh-cpu (search_bvh3.rs:103-221, pix2tri.rs:3-48).
+ * Replaces kernel `pix_to_tri` (del-msh-cuda-kernels/cuda/pix2tri.cu:12-76)
+ * launched by pix2tri_by_raycast (del-msh-dlpack/src/pix2tri.rs:15-94).
+ * pix2tri [img_h][img_w]: hit triangle or UINT32_MAX.  pix2t (nullable):
+ * ray coefficient of the hit (0 on a miss); the ray runs from NDC z = +1
+ * (t = 0) to NDC z = -1 (t = 1).
+ * ------------------------------------------------------------------- */
+size_t dmsh_pix2tri_workspace_bytes(uint32_t img_w, uint32_t img_h);
+int dmsh_pix2tri(void *stream, uint32_t *pix2tri, float *pix2t,
+                 uint32_t num_tri, const uint32_t *tri2vtx, uint32_t num_vtx,
+                 const float *vtx2xyz, uint32_t img_w, uint32_t img_h,
+                 const float *transform_ndc2world, const uint32_t *bvhnodes,
+                 const float *bvhnode2aabb, void *ws, size_t ws_bytes);
+
+/* ---------------------------------------------------------------------
+ * Depth of the stored hit, (ndc_z + 1) / 2, 0 on background.
+ * Replaces kernel `fwd` (del-msh-cuda-kernels/cuda/pix2depth.cu:11-47)
+ * launched by pix2depth_update (del-msh-dlpack/src/pix2depth.rs:14-97); the
+ * inverse transform is computed on the device (no host round trip).
+ * CPU counterpart del-msh-cpu/src/pix2depth.rs:107-146.
+ * ------------------------------------------------------------------- */
+int dmsh_pix2depth(void *stream, float *pix2depth, const uint32_t *pix2tri,
+                   uint32_t num_tri, const uint32_t *tri2vtx, uint32_t num_vtx,
+                   const float *vtx2xyz, uint32_t img_w, uint32_t img_h,
+                   const float *transform_ndc2world);
+
+/* ---------------------------------------------------------------------
+ * Edge-gradient backward of the rasterized silhouette (accumulates into
+ * dldw_vtx2xyz).  Replaces kernels `bwd_hedge` / `bwd_vedge`
+ * (del-msh-cuda-kernels/cuda/rasterized_edge_gradient.cu:387-520) launched
+ * by rasterized_edge_gradient_bwd
+ * (del-msh-dlpack/src/rasterized_edge_gradient.rs:27-150); CPU counterpart
+ * del-msh-cpu/src/rasterized_edge_gradient.rs:434-603.
+ * transform_world2pix: 16 floats column-major.
+ * ------------------------------------------------------------------- */
+int dmsh_rasterized_edge_gradient_bwd(void *stream, const uint32_t *tri2vtx,
+                                      uint32_t num_tri, const float *vtx2xyz,
+                                      uint32_t num_vtx, float *dldw_vtx2xyz,
+                                      const float *transform_world2pix,
+                                      uint32_t img_w, uint32_t img_h,
+                                      const uint32_t *pix2tri,
+                                      uint32_t num_vdim, const float *pix2val,
+                                      const float *dldw_pix2val);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DMSH_H */

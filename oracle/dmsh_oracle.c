@@ -1,0 +1,897 @@
+/*
+ * dmsh_oracle.c -- CPU oracle (TEST INFRASTRUCTURE ONLY, see dmsh_oracle.h).
+ *
+ * Plain C restatement of the del-msh-cpu functions on the LBVH / ray-cast /
+ * edge-gradient path.  Reference files are cited as del-msh-cpu/src/<file>:line.
+ * Build with -ffp-contract=off: every float expression below is evaluated in
+ * the order written, in binary32.
+ *
+ * Formula set for the del-geo-core primitives (that crate is not vendored in
+ * the reference, so these are restated from published methods):
+ *   F1 transform_homogeneous: w = ((m3 x + m7 y) + m11 z) + m15; w == 0 -> none;
+ *      out_r = (((m_r x + m_{4+r} y) + m_{8+r} z) + m_{12+r}) / w.
+ *   F2 pixel ray: x = 2 (iw + .5) / W - 1, y = 1 - 2 (ih + .5) / H;
+ *      org = F1(T, (x,y,+1)), dir = F1(T, (x,y,-1)) - org.
+ *   F3 ray-AABB slab test (Kay & Kajiya 1986; reciprocal form of Williams et
+ *      al. 2005): inv = 1/d per axis; an axis whose inv is not finite is a
+ *      parallel axis (miss iff o outside [lo,hi]); otherwise t0 = (lo-o) inv,
+ *      t1 = (hi-o) inv; tmin = max(tmin, min(t0,t1)), tmax = min(tmax,
+ *      max(t0,t1)) from (-inf, +inf); miss iff tmin > tmax or tmax < 0.
+ *   F4 ray-triangle: Moller & Trumbore 1997 (non-culling branch), |det| <
+ *      FLT_EPSILON -> miss, 0 <= u, v; u + v <= 1; ray hit needs t > FLT_EPSILON.
+ *   F5 min/max are compare-and-select: min(a,b) = b < a ? b : a,
+ *      max(a,b) = b > a ? b : a (ties keep the first operand).
+ */
+#include "dmsh_oracle.h"
+
+#include <float.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define U32MAX 0xFFFFFFFFu
+
+static inline float fmin_sel(float a, float b) { return (b < a) ? b : a; }
+static inline float fmax_sel(float a, float b) { return (b > a) ? b : a; }
+
+/* ======================================================================= */
+/* Morton codes                                                             */
+/* ======================================================================= */
+
+/* mortons.rs:3-8 */
+uint32_t orc_expand_bits2(uint32_t x) {
+  x = (x | (x << 8)) & 0x00ff00ffu;
+  x = (x | (x << 4)) & 0x0f0f0f0fu;
+  x = (x | (x << 2)) & 0x33333333u;
+  return (x | (x << 1)) & 0x55555555u;
+}
+
+/* Rust `(v).clamp(lo, hi) as u32`: NaN stays NaN through clamp and the
+ * saturating cast maps NaN to 0. */
+static inline uint32_t clamp_to_u32(float v, float hi) {
+  if (v != v) return 0u;
+  if (v < 0.f) v = 0.f;
+  if (v > hi) v = hi;
+  return (uint32_t)v;
+}
+
+/* mortons.rs:20-27 */
+uint32_t orc_morton_code2(float x, float y) {
+  uint32_t ix = clamp_to_u32(x * 65536.f, 65535.f);
+  uint32_t iy = clamp_to_u32(y * 65536.f, 65535.f);
+  ix = orc_expand_bits2(ix);
+  iy = orc_expand_bits2(iy);
+  return ix * 2u + iy;
+}
+
+/* mortons.rs:76-81 */
+uint32_t orc_expand_bits3(uint32_t x) {
+  x = (x | (x << 16)) & 0x030000FFu;
+  x = (x | (x << 8)) & 0x0300F00Fu;
+  x = (x | (x << 4)) & 0x030C30C3u;
+  return (x | (x << 2)) & 0x09249249u;
+}
+
+/* mortons.rs:94-102 */
+uint32_t orc_morton_code3(float x, float y, float z) {
+  uint32_t ix = clamp_to_u32(x * 1024.f, 1023.f);
+  uint32_t iy = clamp_to_u32(y * 1024.f, 1023.f);
+  uint32_t iz = clamp_to_u32(z * 1024.f, 1023.f);
+  ix = orc_expand_bits3(ix);
+  iy = orc_expand_bits3(iy);
+  iz = orc_expand_bits3(iz);
+  return ix * 4u + iy * 2u + iz;
+}
+
+/* 63-bit extension: the same construction with 21 bits per axis.  Written
+ * as a plain bit loop (one source bit at a time) so that it is independent
+ * of the magic-mask form the GPU kernel uses. */
+uint64_t orc_expand_bits3_u64(uint32_t x) {
+  uint64_t r = 0;
+  for (int b = 0; b < 21; ++b) {
+    if ((x >> b) & 1u) r |= (uint64_t)1 << (3 * b);
+  }
+  return r;
+}
+
+uint64_t orc_morton_code3_u64(float x, float y, float z) {
+  const float scale = 2097152.f; /* 2^21 */
+  const float hi = 2097151.f;    /* 2^21 - 1, exact in binary32 */
+  uint64_t ix = orc_expand_bits3_u64(clamp_to_u32(x * scale, hi));
+  uint64_t iy = orc_expand_bits3_u64(clamp_to_u32(y * scale, hi));
+  uint64_t iz = orc_expand_bits3_u64(clamp_to_u32(z * scale, hi));
+  return ix * 4u + iy * 2u + iz;
+}
+
+/* ======================================================================= */
+/* geometry formula set                                                     */
+/* ======================================================================= */
+
+/* F1: del-geo-core mat4_col_major::transform_homogeneous */
+int orc_transform_homogeneous(const float m[16], const float p[3], float out[3]) {
+  const float x = p[0], y = p[1], z = p[2];
+  const float w = ((m[3] * x + m[7] * y) + m[11] * z) + m[15];
+  if (w == 0.f) return 0;
+  for (int r = 0; r < 3; ++r) {
+    out[r] = (((m[r] * x + m[4 + r] * y) + m[8 + r] * z) + m[12 + r]) / w;
+  }
+  return 1;
+}
+
+/* mat3_col_major::transform_homogeneous, 2D points */
+static int transform_homogeneous2(const float m[9], const float p[2], float out[2]) {
+  const float x = p[0], y = p[1];
+  const float w = (m[2] * x + m[5] * y) + m[8];
+  if (w == 0.f) return 0;
+  for (int r = 0; r < 2; ++r) out[r] = ((m[r] * x + m[3 + r] * y) + m[6 + r]) / w;
+  return 1;
+}
+
+/* mortons.rs:171-212 */
+int orc_vtx2morton(size_t num_vtx, const float *vtx2co, int num_dim,
+                   const float *transform, uint32_t *vtx2morton) {
+  if (num_dim == 2) {
+    for (size_t i = 0; i < num_vtx; ++i) {
+      float q[2];
+      if (!transform_homogeneous2(transform, vtx2co + i * 2, q)) { q[0] = NAN; q[1] = NAN; }
+      vtx2morton[i] = orc_morton_code2(q[0], q[1]);
+    }
+    return 0;
+  }
+  if (num_dim == 3) {
+    for (size_t i = 0; i < num_vtx; ++i) {
+      float q[3];
+      if (!orc_transform_homogeneous(transform, vtx2co + i * 3, q)) { q[0] = q[1] = q[2] = NAN; }
+      vtx2morton[i] = orc_morton_code3(q[0], q[1], q[2]);
+    }
+    return 0;
+  }
+  return -1;
+}
+
+int orc_vtx2morton_u64(size_t num_vtx, const float *vtx2xyz,
+                       const float *transform, uint64_t *vtx2morton) {
+  for (size_t i = 0; i < num_vtx; ++i) {
+    float q[3];
+    if (!orc_transform_homogeneous(transform, vtx2xyz + i * 3, q)) { q[0] = q[1] = q[2] = NAN; }
+    vtx2morton[i] = orc_morton_code3_u64(q[0], q[1], q[2]);
+  }
+  return 0;
+}
+
+/* F2: del-geo-core ray_from_transform_ndc2world_and_pixel_coordinates, in the
+ * form the reference GPU kernel spells out (ray_for_pixel.h:3-19). */
+void orc_ray_for_pixel(uint32_t iw, uint32_t ih, uint32_t img_w, uint32_t img_h,
+                       const float T[16], float org[3], float dir[3]) {
+  const float x0 = 2.f * ((float)iw + 0.5f) / (float)img_w - 1.f;
+  const float y0 = 1.f - 2.f * ((float)ih + 0.5f) / (float)img_h;
+  const float a[3] = {x0, y0, +1.f};
+  const float b[3] = {x0, y0, -1.f};
+  float q0[3], q1[3];
+  if (!orc_transform_homogeneous(T, a, q0)) { q0[0] = q0[1] = q0[2] = NAN; }
+  if (!orc_transform_homogeneous(T, b, q1)) { q1[0] = q1[1] = q1[2] = NAN; }
+  for (int i = 0; i < 3; ++i) {
+    org[i] = q0[i];
+    dir[i] = q1[i] - q0[i];
+  }
+}
+
+/* F3: slab test.  The reciprocal is recomputed here from dir so that the
+ * function stands alone; 1/d is correctly rounded so this equals caching it. */
+int orc_ray_aabb(const float aabb[6], const float org[3], const float dir[3],
+                 float *tmin_out, float *tmax_out) {
+  float tmin = -INFINITY, tmax = INFINITY;
+  for (int i = 0; i < 3; ++i) {
+    const float inv = 1.f / dir[i];
+    if (!(fabsf(inv) < INFINITY)) {
+      if (org[i] < aabb[i] || org[i] > aabb[i + 3]) return 0;
+      continue;
+    }
+    const float t0 = (aabb[i] - org[i]) * inv;
+    const float t1 = (aabb[i + 3] - org[i]) * inv;
+    tmin = fmax_sel(tmin, fmin_sel(t0, t1));
+    tmax = fmin_sel(tmax, fmax_sel(t0, t1));
+  }
+  if (tmin > tmax) return 0;
+  if (tmax < 0.f) return 0;
+  *tmin_out = tmin;
+  *tmax_out = tmax;
+  return 1;
+}
+
+static inline float dot3(const float a[3], const float b[3]) {
+  return (a[0] * b[0] + a[1] * b[1]) + a[2] * b[2];
+}
+static inline void cross3(const float a[3], const float b[3], float c[3]) {
+  c[0] = a[1] * b[2] - a[2] * b[1];
+  c[1] = a[2] * b[0] - a[0] * b[2];
+  c[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+/* F4: Moller-Trumbore; `ray` selects the t > FLT_EPSILON requirement */
+static int mt_intersect(const float p0[3], const float p1[3], const float p2[3],
+                        const float org[3], const float dir[3], int ray,
+                        float *t_out, float bc[3]) {
+  float e1[3], e2[3], pvec[3], tvec[3], qvec[3];
+  for (int i = 0; i < 3; ++i) {
+    e1[i] = p1[i] - p0[i];
+    e2[i] = p2[i] - p0[i];
+  }
+  cross3(dir, e2, pvec);
+  const float det = dot3(e1, pvec);
+  if (det > -FLT_EPSILON && det < FLT_EPSILON) return 0;
+  const float inv_det = 1.f / det;
+  for (int i = 0; i < 3; ++i) tvec[i] = org[i] - p0[i];
+  const float u = inv_det * dot3(tvec, pvec);
+  if (u < 0.f || u > 1.f) return 0;
+  cross3(tvec, e1, qvec);
+  const float v = inv_det * dot3(dir, qvec);
+  if (v < 0.f || u + v > 1.f) return 0;
+  const float t = inv_det * dot3(e2, qvec);
+  if (ray && !(t > FLT_EPSILON)) return 0;
+  *t_out = t;
+  if (bc) {
+    bc[0] = 1.f - u - v;
+    bc[1] = u;
+    bc[2] = v;
+  }
+  return 1;
+}
+
+int orc_ray_tri(const float p0[3], const float p1[3], const float p2[3],
+                const float org[3], const float dir[3], float *t, float bc[3]) {
+  return mt_intersect(p0, p1, p2, org, dir, 1, t, bc);
+}
+int orc_line_tri(const float p0[3], const float p1[3], const float p2[3],
+                 const float org[3], const float dir[3], float *t, float bc[3]) {
+  return mt_intersect(p0, p1, p2, org, dir, 0, t, bc);
+}
+
+/* mat4_col_major::try_inverse_with_pivot, restated as Gauss-Jordan
+ * elimination with partial pivoting on the augmented matrix [A | I]
+ * (A[r][c] = m[c*4+r]); pivot = first row with the largest |A[r][c]|. */
+int orc_mat4_inverse(const float m[16], float inv[16]) {
+  float a[4][8];
+  for (int r = 0; r < 4; ++r) {
+    for (int c = 0; c < 4; ++c) {
+      a[r][c] = m[c * 4 + r];
+      a[r][4 + c] = (r == c) ? 1.f : 0.f;
+    }
+  }
+  for (int c = 0; c < 4; ++c) {
+    int piv = c;
+    float best = fabsf(a[c][c]);
+    for (int r = c + 1; r < 4; ++r) {
+      if (fabsf(a[r][c]) > best) { best = fabsf(a[r][c]); piv = r; }
+    }
+    if (best == 0.f) return 0;
+    if (piv != c) {
+      for (int k = 0; k < 8; ++k) { float tmp = a[c][k]; a[c][k] = a[piv][k]; a[piv][k] = tmp; }
+    }
+    const float ip = 1.f / a[c][c];
+    for (int k = 0; k < 8; ++k) a[c][k] = a[c][k] * ip;
+    for (int r = 0; r < 4; ++r) {
+      if (r == c) continue;
+      const float f = a[r][c];
+      for (int k = 0; k < 8; ++k) a[r][k] = a[r][k] - f * a[c][k];
+    }
+  }
+  for (int r = 0; r < 4; ++r)
+    for (int c = 0; c < 4; ++c) inv[c * 4 + r] = a[r][4 + c];
+  return 1;
+}
+
+/* elem2center.rs:8-37: cog starts at +0 and adds the nodes in order, then
+ * is scaled by ratio = 1/num_node. */
+void orc_tri2centroid(size_t num_tri, const uint32_t *tri2vtx,
+                      const float *vtx2xyz, float *tri2cntr) {
+  const float ratio = 1.f / 3.f;
+  for (size_t t = 0; t < num_tri; ++t) {
+    for (int d = 0; d < 3; ++d) {
+      float cog = 0.f;
+      for (int k = 0; k < 3; ++k) cog += vtx2xyz[(size_t)tri2vtx[t * 3 + k] * 3 + d];
+      tri2cntr[t * 3 + d] = cog * ratio;
+    }
+  }
+}
+
+/* Total order on binary32 that puts -0 below +0 (used for the bounding-box
+ * reduction so the result does not depend on reduction order). */
+static inline uint32_t f32_order_key(float f) {
+  uint32_t u;
+  memcpy(&u, &f, 4);
+  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+
+/* Mat44/torch.py:120-140 from_fit_vtx2xyz_into_unit_cube:
+ * s = 1 / max(extent), translation = -center * s + 0.5 */
+void orc_fit_into_unit_cube(size_t num_pnt, const float *p, float out[16]) {
+  float mn[3], mx[3];
+  for (int d = 0; d < 3; ++d) { mn[d] = p[d]; mx[d] = p[d]; }
+  for (size_t i = 1; i < num_pnt; ++i) {
+    for (int d = 0; d < 3; ++d) {
+      const float v = p[i * 3 + d];
+      if (f32_order_key(v) < f32_order_key(mn[d])) mn[d] = v;
+      if (f32_order_key(v) > f32_order_key(mx[d])) mx[d] = v;
+    }
+  }
+  float ext = mx[0] - mn[0];
+  for (int d = 1; d < 3; ++d) ext = fmax_sel(ext, mx[d] - mn[d]);
+  const float s = 1.f / ext;
+  memset(out, 0, sizeof(float) * 16);
+  out[0] = s;
+  out[5] = s;
+  out[10] = s;
+  out[15] = 1.f;
+  for (int d = 0; d < 3; ++d) {
+    const float c = (mn[d] + mx[d]) * 0.5f;
+    out[12 + d] = (-c) * s + 0.5f;
+  }
+}
+
+/* ======================================================================= */
+/* stable argsort                                                           */
+/* ======================================================================= */
+
+/* Bottom-up merge sort of indices by key: stable, like Rust's slice
+ * `sort_by_key` used at mortons.rs:143. */
+#define DEFINE_ARGSORT(NAME, KT)                                                  \
+  void NAME(size_t n, const KT *keys, uint32_t *idx2obj, KT *idx2key) {          \
+    uint32_t *a = (uint32_t *)malloc(sizeof(uint32_t) * (n ? n : 1));            \
+    uint32_t *b = (uint32_t *)malloc(sizeof(uint32_t) * (n ? n : 1));            \
+    for (size_t i = 0; i < n; ++i) a[i] = (uint32_t)i;                          \
+    for (size_t w = 1; w < n; w *= 2) {                                          \
+      for (size_t lo = 0; lo < n; lo += 2 * w) {                                 \
+        size_t mid = lo + w < n ? lo + w : n;                                    \
+        size_t hi = lo + 2 * w < n ? lo + 2 * w : n;                             \
+        size_t i = lo, j = mid, k = lo;                                          \
+        while (i < mid && j < hi) {                                              \
+          if (keys[a[j]] < keys[a[i]]) b[k++] = a[j++];                          \
+          else b[k++] = a[i++];                                                  \
+        }                                                                        \
+        while (i < mid) b[k++] = a[i++];                                         \
+        while (j < hi) b[k++] = a[j++];                                          \
+      }                                                                          \
+      uint32_t *tmp = a; a = b; b = tmp;                                         \
+    }                                                                            \
+    for (size_t i = 0; i < n; ++i) {                                             \
+      idx2obj[i] = a[i];                                                         \
+      if (idx2key) idx2key[i] = keys[a[i]];                                      \
+    }                                                                            \
+    free(a);                                                                     \
+    free(b);                                                                     \
+  }
+DEFINE_ARGSORT(orc_argsort_u32, uint32_t)
+DEFINE_ARGSORT(orc_argsort_u64, uint64_t)
+
+/* ======================================================================= */
+/* Karras topology                                                          */
+/* ======================================================================= */
+
+static inline int64_t clz32(uint32_t v) { return v ? (int64_t)__builtin_clz(v) : 32; }
+static inline int64_t clz64(uint64_t v) { return v ? (int64_t)__builtin_clzll(v) : 64; }
+
+/* mortons.rs:216-218 `delta`, mortons.rs:221-288 range,
+ * mortons.rs:290-325 split; generated for u32 (30-bit codes, nbits 32) and
+ * u64 (63-bit extension, nbits 64) keys. */
+#define DEFINE_TOPOLOGY(SUF, KT, CLZ, NBITS)                                      \
+  static inline int64_t delta_##SUF(size_t i0, size_t i1, const KT *c) {          \
+    return CLZ(c[i0] ^ c[i1]);                                                    \
+  }                                                                               \
+  void orc_range_##SUF(const KT *c, size_t n, size_t idx1, size_t *first,          \
+                       size_t *last) {                                            \
+    if (idx1 == 0) { *first = 0; *last = n - 1; return; }                        \
+    if (idx1 == n - 1) { *first = n - 1; *last = n - 1; return; }                \
+    const KT mc0 = c[idx1 - 1], mc1 = c[idx1], mc2 = c[idx1 + 1];                 \
+    if (mc0 == mc1 && mc1 == mc2) {                                               \
+      size_t jdx = idx1 + 1;                                                      \
+      while (jdx < n - 1) {                                                       \
+        jdx += 1;                                                                 \
+        if (c[jdx] != mc1) { *first = idx1; *last = jdx - 1; return; }           \
+      }                                                                           \
+      *first = idx1; *last = jdx; return;                                         \
+    }                                                                             \
+    int64_t d = delta_##SUF(idx1, idx1 + 1, c) - delta_##SUF(idx1, idx1 - 1, c);  \
+    d = d > 0 ? 1 : -1;                                                           \
+    const int64_t delta_min = delta_##SUF(idx1, (size_t)((int64_t)idx1 - d), c);  \
+    int64_t lmax = 2;                                                             \
+    for (;;) {                                                                    \
+      const int64_t jdx = (int64_t)idx1 + lmax * d;                               \
+      if (jdx < 0 || jdx >= (int64_t)n) break;                                    \
+      if (delta_##SUF(idx1, (size_t)jdx, c) <= delta_min) break;                  \
+      lmax *= 2;                                                                  \
+    }                                                                             \
+    int64_t l = 0;                                                                \
+    for (int64_t t = lmax / 2; t >= 1; t /= 2) {                                  \
+      const int64_t jdx = (int64_t)idx1 + (l + t) * d;                            \
+      if (jdx >= 0 && jdx < (int64_t)n &&                                         \
+          delta_##SUF(idx1, (size_t)jdx, c) > delta_min) {                        \
+        l += t;                                                                   \
+      }                                                                           \
+    }                                                                             \
+    const size_t jdx = (size_t)((int64_t)idx1 + l * d);                           \
+    if (idx1 <= jdx) { *first = idx1; *last = jdx; }                              \
+    else { *first = jdx; *last = idx1; }                                          \
+  }                                                                               \
+  size_t orc_split_##SUF(const KT *c, size_t i_start, size_t i_end) {             \
+    if (i_start == i_end) return (size_t)-1;                                      \
+    const KT mc_start = c[i_start];                                               \
+    const int64_t nbitcommon0 = CLZ(mc_start ^ c[i_end]);                         \
+    if (nbitcommon0 == NBITS) return i_start;                                     \
+    size_t split = i_start;                                                       \
+    size_t step = i_end - i_start;                                                \
+    while (step > 1) {                                                            \
+      step = (step + 1) / 2;                                                      \
+      const size_t inew = split + step;                                           \
+      if (inew >= i_end) continue;                                                \
+      if (CLZ(mc_start ^ c[inew]) > nbitcommon0) split = inew;                    \
+    }                                                                             \
+    return split;                                                                 \
+  }                                                                               \
+  int orc_bvhnodes_##SUF(uint32_t *bvhnodes, const uint32_t *idx2obj,             \
+                         const KT *c, size_t n) {                                 \
+    if (n == 0) return -1;                                                        \
+    bvhnodes[0] = U32MAX;                                                         \
+    const size_t num_branch = n - 1;                                              \
+    for (size_t ib = 0; ib < num_branch; ++ib) {                                  \
+      size_t r0, r1;                                                              \
+      orc_range_##SUF(c, n, ib, &r0, &r1);                                        \
+      const size_t isplit = orc_split_##SUF(c, r0, r1);                           \
+      if (isplit == (size_t)-1) return -2;                                        \
+      if (r0 == isplit) {                                                         \
+        const size_t a = num_branch + isplit;                                     \
+        bvhnodes[ib * 3 + 1] = (uint32_t)a;                                       \
+        bvhnodes[a * 3] = (uint32_t)ib;                                           \
+        bvhnodes[a * 3 + 1] = idx2obj[isplit];                                    \
+        bvhnodes[a * 3 + 2] = U32MAX;                                             \
+      } else {                                                                    \
+        bvhnodes[ib * 3 + 1] = (uint32_t)isplit;                                  \
+        bvhnodes[isplit * 3] = (uint32_t)ib;                                      \
+      }                                                                           \
+      if (r1 == isplit + 1) {                                                     \
+        const size_t b = num_branch + isplit + 1;                                 \
+        bvhnodes[ib * 3 + 2] = (uint32_t)b;                                       \
+        bvhnodes[b * 3] = (uint32_t)ib;                                           \
+        bvhnodes[b * 3 + 1] = idx2obj[isplit + 1];                                \
+        bvhnodes[b * 3 + 2] = U32MAX;                                             \
+      } else {                                                                    \
+        bvhnodes[ib * 3 + 2] = (uint32_t)(isplit + 1);                            \
+        bvhnodes[(isplit + 1) * 3] = (uint32_t)ib;                                \
+      }                                                                           \
+    }                                                                             \
+    if (n == 1) { bvhnodes[1] = idx2obj[0]; bvhnodes[2] = U32MAX; }              \
+    return 0;                                                                     \
+  }
+DEFINE_TOPOLOGY(u32, uint32_t, clz32, 32)
+DEFINE_TOPOLOGY(u64, uint64_t, clz64, 64)
+
+/* mortons.rs:330-352 */
+int orc_check_morton_code_range_split_u32(const uint32_t *c, size_t n) {
+  if (n == 0) return 1;
+  for (size_t i = 0; i + 1 < n; ++i)
+    if (c[i] > c[i + 1]) return 2;
+  for (size_t ini = 0; ini + 1 < n; ++ini) {
+    size_t r0, r1, a0, a1, b0, b1;
+    orc_range_u32(c, n, ini, &r0, &r1);
+    const size_t isplit = orc_split_u32(c, r0, r1);
+    orc_range_u32(c, n, isplit, &a0, &a1);
+    orc_range_u32(c, n, isplit + 1, &b0, &b1);
+    if (r0 != a0) return 3;
+    if (r1 != b1) return 4;
+    const size_t last1 = (isplit == r0) ? isplit : a1;
+    const size_t first1 = (isplit + 1 == r1) ? isplit + 1 : b0;
+    if (last1 + 1 != first1) return 5;
+  }
+  return 0;
+}
+
+static void increment_leaf(const uint32_t *bn, size_t inode, size_t num_idx,
+                           size_t *flag) {
+  const size_t c0 = bn[inode * 3 + 1];
+  if (c0 >= num_idx - 1) flag[c0 - (num_idx - 1)] += 1;
+  else increment_leaf(bn, c0, num_idx, flag);
+  const size_t c1 = bn[inode * 3 + 2];
+  if (c1 >= num_idx - 1) flag[c1 - (num_idx - 1)] += 1;
+  else increment_leaf(bn, c1, num_idx, flag);
+}
+
+/* mortons.rs:384-433 (checks the branch part of bvhnodes, rows 0..n-2) */
+int orc_check_binary_radix_tree_u32(const uint32_t *bn, const uint32_t *c, size_t n) {
+  if (n < 2) return 0;
+  size_t *flag = (size_t *)calloc(n, sizeof(size_t));
+  increment_leaf(bn, 0, n, flag);
+  int ret = 0;
+  for (size_t i = 0; i < n; ++i)
+    if (flag[i] != 1) { ret = 1; break; }
+  free(flag);
+  if (ret) return ret;
+  for (size_t ib = 0; ib + 1 < n; ++ib) {
+    const size_t il = bn[ib * 3 + 1];
+    const size_t isplit = il >= n - 1 ? il - (n - 1) : il;
+    size_t r0, r1;
+    orc_range_u32(c, n, ib, &r0, &r1);
+    if (orc_split_u32(c, r0, r1) != isplit) return 2;
+  }
+  return 0;
+}
+
+/* bvhnodes.rs:5-36 mark_child / check_bvh_topology (iterative walk) */
+int orc_check_bvh_topology(const uint32_t *bn, size_t num_leaf) {
+  if (num_leaf == 0) return 1;
+  const size_t num_node = 2 * num_leaf - 1;
+  size_t *cnt = (size_t *)calloc(num_leaf, sizeof(size_t));
+  size_t *stack = (size_t *)malloc(sizeof(size_t) * (num_node + 1));
+  size_t sp = 0;
+  int ret = 0;
+  stack[sp++] = 0;
+  while (sp > 0 && !ret) {
+    const size_t nd = stack[--sp];
+    if (nd >= num_node) { ret = 2; break; }
+    if (bn[nd * 3 + 2] == U32MAX) {
+      const size_t obj = bn[nd * 3 + 1];
+      if (obj >= num_leaf) { ret = 3; break; }
+      cnt[obj] += 1;
+      continue;
+    }
+    if (sp + 2 > num_node + 1) { ret = 4; break; }
+    stack[sp++] = bn[nd * 3 + 2];
+    stack[sp++] = bn[nd * 3 + 1];
+  }
+  for (size_t i = 0; i < num_leaf && !ret; ++i)
+    if (cnt[i] != 1) ret = 5;
+  for (size_t i = 0; i < num_leaf && !ret; ++i)
+    if (bn[(num_leaf - 1 + i) * 3 + 2] != U32MAX) ret = 6;
+  free(cnt);
+  free(stack);
+  return ret;
+}
+
+/* ======================================================================= */
+/* refit                                                                    */
+/* ======================================================================= */
+
+/* aabb3::set_as_cube + aabb3::add_point (vtx2xyz.rs:39-61, eps margin) */
+static inline void aabb_set_point(float *aabb, const float *p, float eps) {
+  for (int d = 0; d < 3; ++d) { aabb[d] = p[d] - eps; aabb[d + 3] = p[d] + eps; }
+}
+static inline void aabb_add_point(float *aabb, const float *p, float eps) {
+  for (int d = 0; d < 3; ++d) {
+    aabb[d] = fmin_sel(aabb[d], p[d] - eps);
+    aabb[d + 3] = fmax_sel(aabb[d + 3], p[d] + eps);
+  }
+}
+/* aabb3::from_two_aabbs */
+static inline void aabb_merge(float *out, const float *a, const float *b) {
+  for (int d = 0; d < 3; ++d) {
+    out[d] = fmin_sel(a[d], b[d]);
+    out[d + 3] = fmax_sel(a[d + 3], b[d + 3]);
+  }
+}
+
+typedef struct {
+  float *aabb;
+  const uint32_t *bn;
+  const uint32_t *tri2vtx; /* NULL: leaves store point indices */
+  const float *xyz;
+  float eps;
+  int err;
+} refit_ctx;
+
+/* bvhnode2aabb3.rs:68-142 (triangles) / :6-63 (points): recursion into
+ * child 1 then child 2, then merge. */
+static void refit_rec(refit_ctx *cx, size_t node) {
+  const uint32_t *bn = cx->bn;
+  float *box = cx->aabb + node * 6;
+  if (bn[node * 3 + 2] == U32MAX) {
+    const size_t e = bn[node * 3 + 1];
+    if (cx->tri2vtx) {
+      const uint32_t *v = cx->tri2vtx + e * 3;
+      aabb_set_point(box, cx->xyz + (size_t)v[0] * 3, cx->eps);
+      aabb_add_point(box, cx->xyz + (size_t)v[1] * 3, cx->eps);
+      aabb_add_point(box, cx->xyz + (size_t)v[2] * 3, cx->eps);
+    } else {
+      aabb_set_point(box, cx->xyz + e * 3, 0.f);
+    }
+    return;
+  }
+  const size_t c0 = bn[node * 3 + 1], c1 = bn[node * 3 + 2];
+  if (bn[c0 * 3] != node || bn[c1 * 3] != node) { cx->err = 1; return; }
+  refit_rec(cx, c0);
+  refit_rec(cx, c1);
+  aabb_merge(box, cx->aabb + c0 * 6, cx->aabb + c1 * 6);
+}
+
+int orc_bvhnode2aabb_trimesh3(float *aabb, const uint32_t *bn, size_t num_node,
+                              const uint32_t *tri2vtx, const float *vtx2xyz,
+                              float eps) {
+  if (num_node == 0) return -1;
+  refit_ctx cx = {aabb, bn, tri2vtx, vtx2xyz, eps, 0};
+  refit_rec(&cx, 0);
+  return cx.err ? -2 : 0;
+}
+
+int orc_bvhnode2aabb_points(float *aabb, const uint32_t *bn, size_t num_node,
+                            const float *vtx2xyz) {
+  if (num_node == 0) return -1;
+  refit_ctx cx = {aabb, bn, NULL, vtx2xyz, 0.f, 0};
+  refit_rec(&cx, 0);
+  return cx.err ? -2 : 0;
+}
+
+/* ======================================================================= */
+/* ray-cast                                                                 */
+/* ======================================================================= */
+
+typedef struct {
+  const float *org, *dir;
+  const uint32_t *tri2vtx;
+  const float *xyz;
+  const uint32_t *bn;
+  const float *aabb;
+  orc_trace_stats st;
+} cast_ctx;
+
+typedef struct {
+  float t;
+  uint32_t tri;
+  float bc[3];
+} hit_rec;
+
+/* Rust tuple `<` on (tmin, tmax): lexicographic with partial_cmp. */
+static inline int tuple_lt(float a0, float a1, float b0, float b1) {
+  if (a0 == b0) return a1 < b1;
+  return a0 < b0;
+}
+
+/* search_bvh3.rs:103-221 first_intersection_ray with dis = +inf */
+static int fir(cast_ctx *cx, size_t node, hit_rec *out) {
+  float tn0, tn1;
+  if (!orc_ray_aabb(cx->aabb + node * 6, cx->org, cx->dir, &tn0, &tn1)) return 0;
+  const uint32_t *bn = cx->bn;
+  if (bn[node * 3 + 2] == U32MAX) {
+    const uint32_t tri = bn[node * 3 + 1];
+    const uint32_t *v = cx->tri2vtx + (size_t)tri * 3;
+    float t, bc[3];
+    cx->st.tris_tested++;
+    if (!orc_ray_tri(cx->xyz + (size_t)v[0] * 3, cx->xyz + (size_t)v[1] * 3,
+                     cx->xyz + (size_t)v[2] * 3, cx->org, cx->dir, &t, bc))
+      return 0;
+    if (!(t < INFINITY)) return 0;
+    out->t = t;
+    out->tri = tri;
+    memcpy(out->bc, bc, sizeof(bc));
+    return 1;
+  }
+  cx->st.nodes_visited++; /* an internal node whose two child boxes are tested */
+  const size_t left = bn[node * 3 + 1], right = bn[node * 3 + 2];
+  float tl0, tl1, tr0, tr1;
+  const int hl = orc_ray_aabb(cx->aabb + left * 6, cx->org, cx->dir, &tl0, &tl1);
+  const int hr = orc_ray_aabb(cx->aabb + right * 6, cx->org, cx->dir, &tr0, &tr1);
+  int near_is_left;
+  if (hl && hr) near_is_left = tuple_lt(tl0, tl1, tr0, tr1);
+  else if (!hl && !hr) return 0;
+  else near_is_left = hl;
+  const size_t nnear = near_is_left ? left : right;
+  const size_t nfar = near_is_left ? right : left;
+  hit_rec rn, rf;
+  const int has_n = fir(cx, nnear, &rn);
+  if (has_n) {
+    float tf0, tf1;
+    if (orc_ray_aabb(cx->aabb + nfar * 6, cx->org, cx->dir, &tf0, &tf1)) {
+      if (tf0 > rn.t) { *out = rn; return 1; }
+    } else {
+      *out = rn;
+      return 1;
+    }
+  }
+  const int has_f = fir(cx, nfar, &rf);
+  if (has_n && has_f) { *out = (rn.t < rf.t) ? rn : rf; return 1; }
+  if (has_n) { *out = rn; return 1; }
+  if (has_f) { *out = rf; return 1; }
+  return 0;
+}
+
+int orc_first_intersection_ray(const float org[3], const float dir[3],
+                               const uint32_t *tri2vtx, const float *vtx2xyz,
+                               const uint32_t *bvhnodes, size_t num_node,
+                               const float *aabb, float *t, uint32_t *i_tri,
+                               float bc[3], orc_trace_stats *stats) {
+  if (num_node == 0) return 0;
+  cast_ctx cx = {org, dir, tri2vtx, vtx2xyz, bvhnodes, aabb, {0, 0}};
+  hit_rec h;
+  const int hit = fir(&cx, 0, &h);
+  if (stats) {
+    stats->nodes_visited += cx.st.nodes_visited;
+    stats->tris_tested += cx.st.tris_tested;
+  }
+  if (!hit) return 0;
+  *t = h.t;
+  *i_tri = h.tri;
+  if (bc) memcpy(bc, h.bc, sizeof(h.bc));
+  return 1;
+}
+
+/* pix2tri.rs:3-48 (rayon par_iter over pixels -> OpenMP parallel for) */
+int orc_pix2tri(uint32_t *pix2tri, float *pix2t, size_t num_tri,
+                const uint32_t *tri2vtx, const float *vtx2xyz,
+                const uint32_t *bvhnodes, const float *aabb, uint32_t img_w,
+                uint32_t img_h, const float T[16], int num_threads,
+                orc_trace_stats *stats) {
+  const size_t num_node = num_tri ? 2 * num_tri - 1 : 0;
+  const int64_t num_pix = (int64_t)img_w * (int64_t)img_h;
+  uint64_t nv = 0, nt = 0;
+#ifdef _OPENMP
+  const int nth = num_threads > 0 ? num_threads : 1;
+#pragma omp parallel for schedule(dynamic, 256) num_threads(nth) reduction(+ : nv, nt)
+#endif
+  for (int64_t ip = 0; ip < num_pix; ++ip) {
+    const uint32_t ih = (uint32_t)(ip / img_w);
+    const uint32_t iw = (uint32_t)(ip - (int64_t)ih * img_w);
+    float org[3], dir[3], t = 0.f;
+    uint32_t tri = U32MAX;
+    orc_ray_for_pixel(iw, ih, img_w, img_h, T, org, dir);
+    orc_trace_stats st = {0, 0};
+    if (!orc_first_intersection_ray(org, dir, tri2vtx, vtx2xyz, bvhnodes, num_node,
+                                    aabb, &t, &tri, NULL, &st)) {
+      tri = U32MAX;
+      t = 0.f;
+    }
+    nv += st.nodes_visited;
+    nt += st.tris_tested;
+    pix2tri[ip] = tri;
+    if (pix2t) pix2t[ip] = t;
+  }
+  (void)num_threads;
+  if (stats) {
+    stats->nodes_visited += nv;
+    stats->tris_tested += nt;
+  }
+  return 0;
+}
+
+/* pix2depth.rs:107-146: world2ndc = inverse(ndc2world); recompute the hit of
+ * the stored triangle with the line test; depth = (ndc_z + 1) / 2, 0 on miss. */
+int orc_pix2depth(float *pix2depth, const uint32_t *pix2tri,
+                  const uint32_t *tri2vtx, const float *vtx2xyz, uint32_t img_w,
+                  uint32_t img_h, const float T[16]) {
+  float w2n[16];
+  if (!orc_mat4_inverse(T, w2n)) return -1;
+  const size_t num_pix = (size_t)img_w * img_h;
+  for (size_t ip = 0; ip < num_pix; ++ip) {
+    const uint32_t tri = pix2tri[ip];
+    if (tri == U32MAX) { pix2depth[ip] = 0.f; continue; }
+    const uint32_t iw = (uint32_t)(ip % img_w), ih = (uint32_t)(ip / img_w);
+    float org[3], dir[3], t, pos[3], ndc[3];
+    orc_ray_for_pixel(iw, ih, img_w, img_h, T, org, dir);
+    const uint32_t *v = tri2vtx + (size_t)tri * 3;
+    if (!orc_line_tri(vtx2xyz + (size_t)v[0] * 3, vtx2xyz + (size_t)v[1] * 3,
+                      vtx2xyz + (size_t)v[2] * 3, org, dir, &t, NULL)) {
+      pix2depth[ip] = 0.f; /* the reference unwraps here */
+      continue;
+    }
+    for (int i = 0; i < 3; ++i) pos[i] = t * dir[i] + org[i]; /* vec3::axpy */
+    if (!orc_transform_homogeneous(w2n, pos, ndc)) { pix2depth[ip] = 0.f; continue; }
+    pix2depth[ip] = (ndc[2] + 1.f) * 0.5f;
+  }
+  return 0;
+}
+
+/* ======================================================================= */
+/* edge-gradient backward (rasterized_edge_gradient.rs)                     */
+/* ======================================================================= */
+
+/* tri2::area: signed area, 0.5 * ((p1-p0) x (p2-p0)) */
+static inline float tri2_area(const float p0[2], const float p1[2], const float p2[2]) {
+  return 0.5f * ((p1[0] - p0[0]) * (p2[1] - p0[1]) - (p2[0] - p0[0]) * (p1[1] - p0[1]));
+}
+
+/* transform_homogeneous(...).xy(); the reference unwraps (w != 0 assumed) */
+static inline void project_xy(const float m[16], const float *p, float q[2]) {
+  const float x = p[0], y = p[1], z = p[2];
+  const float w = ((m[3] * x + m[7] * y) + m[11] * z) + m[15];
+  q[0] = (((m[0] * x + m[4] * y) + m[8] * z) + m[12]) / w;
+  q[1] = (((m[1] * x + m[5] * y) + m[9] * z) + m[13]) / w;
+}
+
+/* rasterized_edge_gradient.rs:1-35 fn_barycentric with tri2::barycentric_coords
+ * (signed-area ratios; degenerate -> none). Returns 1 with b, 0 for none. */
+static int fn_barycentric(const uint32_t *tri2vtx, const float *xyz,
+                          const float pc[2], uint32_t itri, const float m[16],
+                          float b[3]) {
+  if (itri == U32MAX) return 0;
+  const uint32_t *v = tri2vtx + (size_t)itri * 3;
+  float p0[2], p1[2], p2[2];
+  project_xy(m, xyz + (size_t)v[0] * 3, p0);
+  project_xy(m, xyz + (size_t)v[1] * 3, p1);
+  project_xy(m, xyz + (size_t)v[2] * 3, p2);
+  const float area = tri2_area(p0, p1, p2);
+  if (area == 0.f) return 0;
+  const float inv_area = 1.f / area;
+  b[0] = tri2_area(pc, p1, p2) * inv_area;
+  b[1] = tri2_area(p0, pc, p2) * inv_area;
+  b[2] = 1.f - b[0] - b[1];
+  return 1;
+}
+
+/* rasterized_edge_gradient.rs:37-48 fn_inside */
+static int fn_inside(int has_b, const float b[3]) {
+  if (!has_b) return 1;
+  return (b[0] >= 0.f && b[1] >= 0.f && b[2] >= 0.f) ||
+         (b[0] <= 0.f && b[1] <= 0.f && b[2] <= 0.f);
+}
+
+/* rasterized_edge_gradient.rs:52-74 fn_projection_gradient */
+static int fn_projection_gradient(const float m[16], const float *xyz, int axis,
+                                  float d[3]) {
+  const float q = m[axis] * xyz[0] + m[axis + 4] * xyz[1] + m[axis + 8] * xyz[2] + m[axis + 12];
+  const float w = m[3] * xyz[0] + m[7] * xyz[1] + m[11] * xyz[2] + m[15];
+  if (fabsf(w) <= FLT_EPSILON) return 0;
+  const float inv_w2 = 1.f / (w * w);
+  d[0] = (m[axis] * w - q * m[3]) * inv_w2;
+  d[1] = (m[axis + 4] * w - q * m[7]) * inv_w2;
+  d[2] = (m[axis + 8] * w - q * m[11]) * inv_w2;
+  return 1;
+}
+
+static void edge_pair(const uint32_t *tri2vtx, const float *xyz, float *dldw,
+                      const float m[16], const uint32_t *pix2tri, uint32_t nv,
+                      const float *pix2val, const float *dpix, size_t ipix0,
+                      size_t ipix1, const float pc0[2], const float pc1[2],
+                      int axis) {
+  const uint32_t itri0 = pix2tri[ipix0], itri1 = pix2tri[ipix1];
+  if (itri0 == itri1) return;
+  float b[3];
+  const int in0 = fn_inside(fn_barycentric(tri2vtx, xyz, pc0, itri1, m, b), b);
+  const int in1 = fn_inside(fn_barycentric(tri2vtx, xyz, pc1, itri0, m, b), b);
+  if (!in0 && !in1) return;
+  float dldpa = 0.f;
+  for (uint32_t k = 0; k < nv; ++k) {
+    const float val0 = pix2val[ipix0 * nv + k], val1 = pix2val[ipix1 * nv + k];
+    const float dv0 = dpix[ipix0 * nv + k], dv1 = dpix[ipix1 * nv + k];
+    dldpa += (dv0 + dv1) * 0.5f * (val0 - val1);
+  }
+  if (in0 && in1) return; /* intersection: `todo` in the reference */
+  uint32_t itri;
+  const float *pc;
+  if (in1) { itri = itri1; pc = pc1; } /* only tri1 receives the gradient */
+  else { itri = itri0; pc = pc0; }     /* only tri0 receives the gradient */
+  if (!fn_barycentric(tri2vtx, xyz, pc, itri, m, b)) return; /* unwrap in ref */
+  for (int inode = 0; inode < 3; ++inode) {
+    const size_t iv = tri2vtx[(size_t)itri * 3 + inode];
+    float d[3];
+    if (!fn_projection_gradient(m, xyz + iv * 3, axis, d)) continue;
+    dldw[iv * 3 + 0] += b[inode] * d[0] * dldpa;
+    dldw[iv * 3 + 1] += b[inode] * d[1] * dldpa;
+    dldw[iv * 3 + 2] += b[inode] * d[2] * dldpa;
+  }
+}
+
+/* rasterized_edge_gradient.rs:434-603 bwd: vertically adjacent pairs
+ * (pixel-y Jacobian) with iw outer / ih0 inner, then horizontally adjacent
+ * pairs (pixel-x Jacobian) with iw0 outer / ih inner. */
+void orc_edge_gradient_bwd(const uint32_t *tri2vtx, const float *xyz,
+                           float *dldw, const float m[16], uint32_t img_w,
+                           uint32_t img_h, const uint32_t *pix2tri, uint32_t nv,
+                           const float *pix2val, const float *dpix) {
+  for (uint32_t iw = 0; iw < img_w; ++iw) {
+    for (uint32_t ih0 = 0; ih0 + 1 < img_h; ++ih0) {
+      const uint32_t ih1 = ih0 + 1;
+      const float pc0[2] = {(float)iw + 0.5f, (float)ih0 + 0.5f};
+      const float pc1[2] = {(float)iw + 0.5f, (float)ih1 + 0.5f};
+      edge_pair(tri2vtx, xyz, dldw, m, pix2tri, nv, pix2val, dpix,
+                (size_t)ih0 * img_w + iw, (size_t)ih1 * img_w + iw, pc0, pc1, 1);
+    }
+  }
+  for (uint32_t iw0 = 0; iw0 + 1 < img_w; ++iw0) {
+    for (uint32_t ih = 0; ih < img_h; ++ih) {
+      const uint32_t iw1 = iw0 + 1;
+      const float pc0[2] = {(float)iw0 + 0.5f, (float)ih + 0.5f};
+      const float pc1[2] = {(float)iw1 + 0.5f, (float)ih + 0.5f};
+      edge_pair(tri2vtx, xyz, dldw, m, pix2tri, nv, pix2val, dpix,
+                (size_t)ih * img_w + iw0, (size_t)ih * img_w + iw1, pc0, pc1, 0);
+    }
+  }
+}

@@ -1,0 +1,139 @@
+/*
+ * dmsh_oracle.h -- CPU oracle for the del-msh LBVH / ray-cast / edge-gradient
+ * hot path.  TEST INFRASTRUCTURE ONLY: the parity tests, the smoke check and
+ * the CPU-baseline leg of the benchmark load this library as the checker.
+ * Nothing in the product (libdmsh, the Python drop-in) may call it.
+ *
+ * Every function restates one function of del-msh-cpu (Rust) in plain C and
+ * cites the reference file:line it follows.  The float primitives that
+ * del-msh-cpu takes from the un-vendored crate del-geo-core are restated from
+ * the published methods named beside each one; all of them are evaluated
+ * left to right in IEEE binary32 with no FMA contraction (-ffp-contract=off).
+ *
+ * Index type: uint32_t, as in the Python entry points (torch.uint32).
+ * Missing triangle / no hit: UINT32_MAX.
+ */
+#ifndef DMSH_ORACLE_H
+#define DMSH_ORACLE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- Morton codes (del-msh-cpu/src/mortons.rs) ------------------------- */
+uint32_t orc_expand_bits2(uint32_t x);                 /* mortons.rs:3-8     */
+uint32_t orc_morton_code2(float x, float y);           /* mortons.rs:20-27   */
+uint32_t orc_expand_bits3(uint32_t x);                 /* mortons.rs:76-81   */
+uint32_t orc_morton_code3(float x, float y, float z);  /* mortons.rs:94-102  */
+/* 63-bit extension (no reference counterpart): 21 bits per axis. */
+uint64_t orc_expand_bits3_u64(uint32_t x);
+uint64_t orc_morton_code3_u64(float x, float y, float z);
+
+/* mortons.rs:171-212; transform is column-major (num_dim+1)^2.
+ * Returns 0, or -1 on a bad num_dim. */
+int orc_vtx2morton(size_t num_vtx, const float *vtx2co, int num_dim,
+                   const float *transform_co2unit, uint32_t *vtx2morton);
+int orc_vtx2morton_u64(size_t num_vtx, const float *vtx2xyz,
+                       const float *transform_co2unit, uint64_t *vtx2morton);
+
+/* ---- stable argsort (mortons.rs:139-146: `sort_by_key`, stable) -------- */
+void orc_argsort_u32(size_t n, const uint32_t *keys, uint32_t *idx2obj,
+                     uint32_t *idx2key);
+void orc_argsort_u64(size_t n, const uint64_t *keys, uint32_t *idx2obj,
+                     uint64_t *idx2key);
+
+/* ---- Karras topology (mortons.rs:216-325, bvhnodes_morton.rs:10-50) ---- */
+void orc_range_u32(const uint32_t *idx2morton, size_t n, size_t idx,
+                   size_t *first, size_t *last);        /* mortons.rs:221-288 */
+size_t orc_split_u32(const uint32_t *idx2morton, size_t first,
+                     size_t last);                      /* mortons.rs:290-325 */
+void orc_range_u64(const uint64_t *idx2morton, size_t n, size_t idx,
+                   size_t *first, size_t *last);
+size_t orc_split_u64(const uint64_t *idx2morton, size_t first, size_t last);
+/* bvhnodes [2n-1][3] = (parent, left|obj, right|UINT32_MAX) */
+int orc_bvhnodes_u32(uint32_t *bvhnodes, const uint32_t *idx2obj,
+                     const uint32_t *idx2morton, size_t n);
+int orc_bvhnodes_u64(uint32_t *bvhnodes, const uint32_t *idx2obj,
+                     const uint64_t *idx2morton, size_t n);
+
+/* Validators; return 0 when the invariant holds, a nonzero code otherwise.
+ * mortons.rs:330-352, mortons.rs:384-433, bvhnodes.rs:22-36 */
+int orc_check_morton_code_range_split_u32(const uint32_t *idx2morton, size_t n);
+int orc_check_binary_radix_tree_u32(const uint32_t *bvhnodes,
+                                    const uint32_t *idx2morton, size_t n);
+int orc_check_bvh_topology(const uint32_t *bvhnodes, size_t num_leaf);
+
+/* ---- refit (bvhnode2aabb3.rs) ------------------------------------------ */
+/* bvhnode2aabb3.rs:68-142 with elem2vtx = tri2vtx, num_noel = 3, no CCD.
+ * aabb [num_node][6] = (xmin,ymin,zmin,xmax,ymax,zmax). */
+int orc_bvhnode2aabb_trimesh3(float *bvhnode2aabb, const uint32_t *bvhnodes,
+                              size_t num_node, const uint32_t *tri2vtx,
+                              const float *vtx2xyz, float eps);
+/* bvhnode2aabb3.rs:6-63 (points, no CCD) */
+int orc_bvhnode2aabb_points(float *bvhnode2aabb, const uint32_t *bvhnodes,
+                            size_t num_node, const float *vtx2xyz);
+
+/* ---- geometry formula set (del-geo-core restated) ---------------------- */
+/* mat4_col_major::transform_homogeneous; returns 0 when w == 0 */
+int orc_transform_homogeneous(const float m[16], const float p[3], float out[3]);
+/* mat4_col_major::ray_from_transform_ndc2world_and_pixel_coordinates */
+void orc_ray_for_pixel(uint32_t iw, uint32_t ih, uint32_t img_w, uint32_t img_h,
+                       const float ndc2world[16], float org[3], float dir[3]);
+/* aabb3 slab test; returns 1 and (tmin,tmax) on hit */
+int orc_ray_aabb(const float aabb[6], const float org[3], const float dir[3],
+                 float *tmin, float *tmax);
+/* tri3::intersection_against_ray (Moller-Trumbore); returns 1 on hit */
+int orc_ray_tri(const float p0[3], const float p1[3], const float p2[3],
+                const float org[3], const float dir[3], float *t, float bc[3]);
+/* tri3::intersection_against_line (no t >= 0 requirement) */
+int orc_line_tri(const float p0[3], const float p1[3], const float p2[3],
+                 const float org[3], const float dir[3], float *t, float bc[3]);
+/* mat4_col_major::try_inverse_with_pivot; returns 0 when singular */
+int orc_mat4_inverse(const float m[16], float inv[16]);
+/* centroid of triangles, elem2center.rs:8-37 (sum * (1/3)) */
+void orc_tri2centroid(size_t num_tri, const uint32_t *tri2vtx,
+                      const float *vtx2xyz, float *tri2cntr);
+/* Mat44/torch.py:120-140 fit into the unit cube; out is column-major */
+void orc_fit_into_unit_cube(size_t num_pnt, const float *vtx2xyz, float out[16]);
+
+/* ---- ray-cast (search_bvh3.rs, pix2tri.rs, pix2depth.rs) --------------- */
+typedef struct {
+  uint64_t nodes_visited; /* internal nodes expanded (both child boxes tested) */
+  uint64_t tris_tested;   /* leaves reached (ray-triangle tests performed)     */
+} orc_trace_stats;
+
+/* search_bvh3.rs:103-221 from the root with dis = +inf.
+ * Returns 1 on hit with (t, tri, bc). */
+int orc_first_intersection_ray(const float org[3], const float dir[3],
+                               const uint32_t *tri2vtx, const float *vtx2xyz,
+                               const uint32_t *bvhnodes, size_t num_node,
+                               const float *bvhnode2aabb, float *t,
+                               uint32_t *i_tri, float bc[3],
+                               orc_trace_stats *stats);
+/* pix2tri.rs:3-48; pix2t (may be NULL) receives the hit ray coefficient.
+ * num_threads <= 1 runs serially (the reference runs it on a rayon pool). */
+int orc_pix2tri(uint32_t *pix2tri, float *pix2t, size_t num_tri,
+                const uint32_t *tri2vtx, const float *vtx2xyz,
+                const uint32_t *bvhnodes, const float *bvhnode2aabb,
+                uint32_t img_w, uint32_t img_h, const float ndc2world[16],
+                int num_threads, orc_trace_stats *stats);
+/* pix2depth.rs:107-146 */
+int orc_pix2depth(float *pix2depth, const uint32_t *pix2tri,
+                  const uint32_t *tri2vtx, const float *vtx2xyz,
+                  uint32_t img_w, uint32_t img_h, const float ndc2world[16]);
+
+/* ---- differentiable rasterizer ----------------------------------------- */
+/* rasterized_edge_gradient.rs:434-603 (accumulates into dldw_vtx2xyz) */
+void orc_edge_gradient_bwd(const uint32_t *tri2vtx, const float *vtx2xyz,
+                           float *dldw_vtx2xyz, const float world2pix[16],
+                           uint32_t img_w, uint32_t img_h,
+                           const uint32_t *pix2tri, uint32_t num_vdim,
+                           const float *pix2val, const float *dldw_pix2val);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,279 @@
+"""numpy front end of the CPU oracle (liboracle.so, built from dmsh_oracle.c).
+
+TEST INFRASTRUCTURE ONLY.  The oracle restates del-msh-cpu's hot-path
+functions in C; only tests/, __graft_entry__.smoke() and the CPU-baseline leg
+of bench.py may use it, and only as the checker.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import pathlib
+
+import numpy as np
+
+_HERE = pathlib.Path(__file__).resolve().parent
+_L = None
+
+_u32p = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
+_u64p = np.ctypeslib.ndpointer(dtype=np.uint64, flags="C_CONTIGUOUS")
+_f32p = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+_sz = ctypes.c_size_t
+
+
+class TraceStats(ctypes.Structure):
+    _fields_ = [("nodes_visited", ctypes.c_uint64), ("tris_tested", ctypes.c_uint64)]
+
+
+def lib():
+    global _L
+    if _L is None:
+        path = _HERE / "liboracle.so"
+        if not path.exists():
+            raise RuntimeError(f"{path} missing: run `make -C oracle`")
+        L = ctypes.CDLL(str(path))
+        sig = {
+            "orc_expand_bits2": (ctypes.c_uint32, [ctypes.c_uint32]),
+            "orc_expand_bits3": (ctypes.c_uint32, [ctypes.c_uint32]),
+            "orc_expand_bits3_u64": (ctypes.c_uint64, [ctypes.c_uint32]),
+            "orc_morton_code2": (ctypes.c_uint32, [ctypes.c_float] * 2),
+            "orc_morton_code3": (ctypes.c_uint32, [ctypes.c_float] * 3),
+            "orc_morton_code3_u64": (ctypes.c_uint64, [ctypes.c_float] * 3),
+            "orc_vtx2morton": (ctypes.c_int, [_sz, _f32p, ctypes.c_int, _f32p, _u32p]),
+            "orc_vtx2morton_u64": (ctypes.c_int, [_sz, _f32p, _f32p, _u64p]),
+            "orc_argsort_u32": (None, [_sz, _u32p, _u32p, _u32p]),
+            "orc_argsort_u64": (None, [_sz, _u64p, _u32p, _u64p]),
+            "orc_split_u32": (_sz, [_u32p, _sz, _sz]),
+            "orc_bvhnodes_u32": (ctypes.c_int, [_u32p, _u32p, _u32p, _sz]),
+            "orc_bvhnodes_u64": (ctypes.c_int, [_u32p, _u32p, _u64p, _sz]),
+            "orc_check_morton_code_range_split_u32": (ctypes.c_int, [_u32p, _sz]),
+            "orc_check_binary_radix_tree_u32": (ctypes.c_int, [_u32p, _u32p, _sz]),
+            "orc_check_bvh_topology": (ctypes.c_int, [_u32p, _sz]),
+            "orc_bvhnode2aabb_trimesh3": (ctypes.c_int, [_f32p, _u32p, _sz, _u32p, _f32p, ctypes.c_float]),
+            "orc_bvhnode2aabb_points": (ctypes.c_int, [_f32p, _u32p, _sz, _f32p]),
+            "orc_transform_homogeneous": (ctypes.c_int, [_f32p, _f32p, _f32p]),
+            "orc_ray_for_pixel": (None, [ctypes.c_uint32] * 4 + [_f32p, _f32p, _f32p]),
+            "orc_ray_aabb": (ctypes.c_int, [_f32p, _f32p, _f32p, ctypes.POINTER(ctypes.c_float),
+                                            ctypes.POINTER(ctypes.c_float)]),
+            "orc_ray_tri": (ctypes.c_int, [_f32p, _f32p, _f32p, _f32p, _f32p,
+                                           ctypes.POINTER(ctypes.c_float), _f32p]),
+            "orc_line_tri": (ctypes.c_int, [_f32p, _f32p, _f32p, _f32p, _f32p,
+                                            ctypes.POINTER(ctypes.c_float), _f32p]),
+            "orc_mat4_inverse": (ctypes.c_int, [_f32p, _f32p]),
+            "orc_tri2centroid": (None, [_sz, _u32p, _f32p, _f32p]),
+            "orc_fit_into_unit_cube": (None, [_sz, _f32p, _f32p]),
+            "orc_first_intersection_ray": (
+                ctypes.c_int,
+                [_f32p, _f32p, _u32p, _f32p, _u32p, _sz, _f32p, ctypes.POINTER(ctypes.c_float),
+                 ctypes.POINTER(ctypes.c_uint32), _f32p, ctypes.POINTER(TraceStats)]),
+            "orc_pix2tri": (
+                ctypes.c_int,
+                [_u32p, ctypes.c_void_p, _sz, _u32p, _f32p, _u32p, _f32p, ctypes.c_uint32,
+                 ctypes.c_uint32, _f32p, ctypes.c_int, ctypes.POINTER(TraceStats)]),
+            "orc_pix2depth": (ctypes.c_int, [_f32p, _u32p, _u32p, _f32p, ctypes.c_uint32,
+                                             ctypes.c_uint32, _f32p]),
+            "orc_edge_gradient_bwd": (
+                None,
+                [_u32p, _f32p, _f32p, _f32p, ctypes.c_uint32, ctypes.c_uint32, _u32p,
+                 ctypes.c_uint32, _f32p, _f32p]),
+        }
+        for name, (res, args) in sig.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _L = L
+    return _L
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def _u32(a):
+    return np.ascontiguousarray(a, dtype=np.uint32)
+
+
+def colmajor(m_rowmajor) -> np.ndarray:
+    """Row-major 4x4 (as the Python API takes it) -> 16 floats column-major."""
+    return _f32(np.asarray(m_rowmajor, dtype=np.float32).T.reshape(-1))
+
+
+# --- Morton / sort / topology ------------------------------------------------
+def vtx2morton(vtx2co, transform_colmajor) -> np.ndarray:
+    co = _f32(vtx2co)
+    out = np.empty(co.shape[0], np.uint32)
+    rc = lib().orc_vtx2morton(co.shape[0], co.reshape(-1), co.shape[1], _f32(transform_colmajor), out)
+    assert rc == 0
+    return out
+
+
+def vtx2morton_u64(vtx2xyz, transform_colmajor) -> np.ndarray:
+    co = _f32(vtx2xyz)
+    out = np.empty(co.shape[0], np.uint64)
+    lib().orc_vtx2morton_u64(co.shape[0], co.reshape(-1), _f32(transform_colmajor), out)
+    return out
+
+
+def argsort(keys):
+    keys = np.ascontiguousarray(keys)
+    n = keys.shape[0]
+    idx = np.empty(n, np.uint32)
+    sk = np.empty(n, keys.dtype)
+    if keys.dtype == np.uint64:
+        lib().orc_argsort_u64(n, keys, idx, sk)
+    else:
+        lib().orc_argsort_u32(n, _u32(keys), idx, sk)
+    return idx, sk
+
+
+def bvhnodes_from_sorted(idx2obj, idx2morton) -> np.ndarray:
+    n = idx2obj.shape[0]
+    bn = np.zeros((2 * n - 1) * 3, np.uint32)
+    if idx2morton.dtype == np.uint64:
+        rc = lib().orc_bvhnodes_u64(bn, _u32(idx2obj), np.ascontiguousarray(idx2morton), n)
+    else:
+        rc = lib().orc_bvhnodes_u32(bn, _u32(idx2obj), _u32(idx2morton), n)
+    assert rc == 0, rc
+    return bn.reshape(-1, 3)
+
+
+def check_bvh_topology(bvhnodes, num_leaf) -> int:
+    return lib().orc_check_bvh_topology(_u32(bvhnodes).reshape(-1), num_leaf)
+
+
+def check_morton_code_range_split(sorted_codes) -> int:
+    c = _u32(sorted_codes)
+    return lib().orc_check_morton_code_range_split_u32(c, c.shape[0])
+
+
+def check_binary_radix_tree(bvhnodes, sorted_codes) -> int:
+    c = _u32(sorted_codes)
+    return lib().orc_check_binary_radix_tree_u32(_u32(bvhnodes).reshape(-1), c, c.shape[0])
+
+
+# --- build ---------------------------------------------------------------------
+def tri2centroid(tri2vtx, vtx2xyz) -> np.ndarray:
+    t = _u32(tri2vtx)
+    out = np.empty((t.shape[0], 3), np.float32)
+    lib().orc_tri2centroid(t.shape[0], t.reshape(-1), _f32(vtx2xyz).reshape(-1), out.reshape(-1))
+    return out
+
+
+def fit_into_unit_cube(pts) -> np.ndarray:
+    p = _f32(pts)
+    out = np.empty(16, np.float32)
+    lib().orc_fit_into_unit_cube(p.shape[0], p.reshape(-1), out)
+    return out
+
+
+def refit_trimesh3(bvhnodes, tri2vtx, vtx2xyz, eps=0.0) -> np.ndarray:
+    bn = _u32(bvhnodes).reshape(-1)
+    num_node = bn.shape[0] // 3
+    out = np.zeros(num_node * 6, np.float32)
+    rc = lib().orc_bvhnode2aabb_trimesh3(out, bn, num_node, _u32(tri2vtx).reshape(-1),
+                                         _f32(vtx2xyz).reshape(-1), eps)
+    assert rc == 0, rc
+    return out.reshape(-1, 6)
+
+
+def refit_points(bvhnodes, vtx2xyz) -> np.ndarray:
+    bn = _u32(bvhnodes).reshape(-1)
+    num_node = bn.shape[0] // 3
+    out = np.zeros(num_node * 6, np.float32)
+    rc = lib().orc_bvhnode2aabb_points(out, bn, num_node, _f32(vtx2xyz).reshape(-1))
+    assert rc == 0, rc
+    return out.reshape(-1, 6)
+
+
+def build_trimesh3(tri2vtx, vtx2xyz, morton_bits=30):
+    """del-msh-cpu pipeline: centroid, fit, Morton, stable sort, topology, refit."""
+    cntr = tri2centroid(tri2vtx, vtx2xyz)
+    T = fit_into_unit_cube(cntr)
+    codes = vtx2morton_u64(cntr, T) if morton_bits == 63 else vtx2morton(cntr, T)
+    idx, sk = argsort(codes)
+    bn = bvhnodes_from_sorted(idx, sk)
+    aabb = refit_trimesh3(bn, tri2vtx, vtx2xyz)
+    return bn, aabb
+
+
+def build_points(pts, morton_bits=30):
+    T = fit_into_unit_cube(pts)
+    codes = vtx2morton_u64(pts, T) if morton_bits == 63 else vtx2morton(pts, T)
+    idx, sk = argsort(codes)
+    bn = bvhnodes_from_sorted(idx, sk)
+    return bn, refit_points(bn, pts)
+
+
+# --- ray-cast ------------------------------------------------------------------
+def pix2tri(tri2vtx, vtx2xyz, bvhnodes, bvhnode2aabb, ndc2world_colmajor, img_w, img_h,
+            num_threads=1, with_t=True):
+    t = _u32(tri2vtx)
+    out = np.empty(img_w * img_h, np.uint32)
+    tt = np.empty(img_w * img_h, np.float32) if with_t else None
+    st = TraceStats()
+    lib().orc_pix2tri(out, tt.ctypes.data if with_t else None, t.shape[0], t.reshape(-1),
+                      _f32(vtx2xyz).reshape(-1), _u32(bvhnodes).reshape(-1),
+                      _f32(bvhnode2aabb).reshape(-1), img_w, img_h, _f32(ndc2world_colmajor),
+                      num_threads, ctypes.byref(st))
+    res = out.reshape(img_h, img_w)
+    stats = (int(st.nodes_visited), int(st.tris_tested))
+    if with_t:
+        return res, tt.reshape(img_h, img_w), stats
+    return res, None, stats
+
+
+def first_intersection_ray(org, dirv, tri2vtx, vtx2xyz, bvhnodes, bvhnode2aabb):
+    t = ctypes.c_float(0)
+    tri = ctypes.c_uint32(0)
+    bc = np.zeros(3, np.float32)
+    st = TraceStats()
+    bn = _u32(bvhnodes).reshape(-1)
+    hit = lib().orc_first_intersection_ray(_f32(org), _f32(dirv), _u32(tri2vtx).reshape(-1),
+                                           _f32(vtx2xyz).reshape(-1), bn, bn.shape[0] // 3,
+                                           _f32(bvhnode2aabb).reshape(-1), ctypes.byref(t),
+                                           ctypes.byref(tri), bc, ctypes.byref(st))
+    return (float(t.value), int(tri.value), bc) if hit else None
+
+
+def ray_for_pixel(iw, ih, img_w, img_h, ndc2world_colmajor):
+    o = np.zeros(3, np.float32)
+    d = np.zeros(3, np.float32)
+    lib().orc_ray_for_pixel(iw, ih, img_w, img_h, _f32(ndc2world_colmajor), o, d)
+    return o, d
+
+
+def ray_tri(p0, p1, p2, org, dirv):
+    t = ctypes.c_float(0)
+    bc = np.zeros(3, np.float32)
+    hit = lib().orc_ray_tri(_f32(p0), _f32(p1), _f32(p2), _f32(org), _f32(dirv), ctypes.byref(t), bc)
+    return (float(t.value), bc) if hit else None
+
+
+def ray_aabb(aabb, org, dirv):
+    a = ctypes.c_float(0)
+    b = ctypes.c_float(0)
+    hit = lib().orc_ray_aabb(_f32(aabb), _f32(org), _f32(dirv), ctypes.byref(a), ctypes.byref(b))
+    return (float(a.value), float(b.value)) if hit else None
+
+
+def mat4_inverse(m_colmajor):
+    out = np.zeros(16, np.float32)
+    ok = lib().orc_mat4_inverse(_f32(m_colmajor), out)
+    return out if ok else None
+
+
+def pix2depth(pix2tri_, tri2vtx, vtx2xyz, ndc2world_colmajor, img_w, img_h):
+    out = np.zeros(img_w * img_h, np.float32)
+    rc = lib().orc_pix2depth(out, _u32(pix2tri_).reshape(-1), _u32(tri2vtx).reshape(-1),
+                             _f32(vtx2xyz).reshape(-1), img_w, img_h, _f32(ndc2world_colmajor))
+    assert rc == 0
+    return out.reshape(img_h, img_w)
+
+
+def edge_gradient_bwd(tri2vtx, vtx2xyz, world2pix_colmajor, pix2tri_, pix2val, dldw_pix2val):
+    h, w, nv = pix2val.shape
+    out = np.zeros(_f32(vtx2xyz).size, np.float32)
+    lib().orc_edge_gradient_bwd(_u32(tri2vtx).reshape(-1), _f32(vtx2xyz).reshape(-1), out,
+                                _f32(world2pix_colmajor), w, h, _u32(pix2tri_).reshape(-1), nv,
+                                _f32(pix2val).reshape(-1), _f32(dldw_pix2val).reshape(-1))
+    return out.reshape(-1, 3)

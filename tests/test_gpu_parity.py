@@ -1,0 +1,331 @@
+"""GPU parity: libdmsh's HIP kernels (called through the C ABI via the
+del_msh_dlpack drop-in) against the CPU oracle on the same inputs.
+
+Integer and min/max outputs must be bit-exact; the edge-gradient backward
+(float atomics, order-dependent) is compared with a stated tolerance.
+"""
+
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from del_msh_dlpack import _lib, primitives
+import del_msh_dlpack.Mortons.torch as Mortons
+import del_msh_dlpack.Array1D.torch as Array1D
+import del_msh_dlpack.TriMesh3.torch as TriMesh3
+import del_msh_dlpack.Pix2Tri.torch as Pix2Tri
+import del_msh_dlpack.Pix2Depth.torch as Pix2Depth
+import del_msh_dlpack.RasterizedEdgeGradient.torch as RasterizedEdgeGradient
+
+pytestmark = pytest.mark.gpu
+
+U32MAX = 0xFFFFFFFF
+
+
+def G(a, dev):
+    return torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+
+
+def H(t):
+    return t.detach().cpu().numpy()
+
+
+def no_device_errors():
+    bits = _lib.device_errors()
+    assert bits == 0, f"device error bits 0x{bits:x}"
+
+
+def bits_equal(a, b):
+    a = np.ascontiguousarray(a)
+    b = np.ascontiguousarray(b)
+    return a.shape == b.shape and np.array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
+# ---------------------------------------------------------------------------
+# Morton codes
+# ---------------------------------------------------------------------------
+def _morton_inputs(n, dim, seed):
+    rng = np.random.default_rng(seed)
+    pts = rng.uniform(-0.3, 1.3, (n, dim)).astype(np.float32)
+    pts[:8] = np.array([[np.nan] * dim, [0.0] * dim, [1.0] * dim, [-0.0] * dim,
+                        [np.inf] * dim, [-np.inf] * dim, [0.99999994] * dim, [1e-30] * dim],
+                       np.float32)[:, :dim]
+    return pts
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_vtx2morton_u32(gpu, oracle, dim):
+    pts = _morton_inputs(100_003, dim, 11)
+    m = np.eye(dim + 1, dtype=np.float32)
+    m[:dim, :dim] *= np.float32(0.7)
+    m[:dim, dim] = np.float32(0.1)
+    got = H(Mortons.make_vtx2morton_from_vtx2co(G(pts, gpu), G(m, gpu)))
+    want = oracle.vtx2morton(pts, m.T.reshape(-1))
+    assert np.array_equal(got, want)
+    no_device_errors()
+
+
+def test_vtx2morton_u64(gpu, oracle):
+    pts = _morton_inputs(100_003, 3, 12)
+    m = np.array([[0.8, 0.1, 0, 0.05], [0, 0.9, 0, 0.02], [0.05, 0, 0.7, 0.1], [0, 0, 0, 1]], np.float32)
+    got = H(Mortons.make_vtx2morton63_from_vtx2xyz(G(pts, gpu), G(m, gpu)))
+    want = oracle.vtx2morton_u64(pts, m.T.reshape(-1))
+    assert np.array_equal(got, want)
+
+
+# ---------------------------------------------------------------------------
+# argsort
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("n", [1, 2, 255, 2048, 2049, 100_000, 1 << 20])
+@pytest.mark.parametrize("spread", ["dups", "wide"])
+def test_argsort_u32(gpu, oracle, n, spread):
+    rng = np.random.default_rng(n)
+    hi = 1000 if spread == "dups" else 1 << 32
+    keys = rng.integers(0, hi, n, dtype=np.uint64).astype(np.uint32)
+    idx, sk = Array1D.argsort(G(keys, gpu))
+    widx, wsk = oracle.argsort(keys)
+    assert np.array_equal(H(sk), wsk)
+    assert np.array_equal(H(idx), widx)
+    no_device_errors()
+
+
+@pytest.mark.parametrize("n", [3, 4096, 300_001])
+@pytest.mark.parametrize("key_bits", [63, 64, 17])
+def test_argsort_u64(gpu, oracle, n, key_bits):
+    rng = np.random.default_rng(n + key_bits)
+    keys = rng.integers(0, 1 << 62, n, dtype=np.uint64) * np.uint64(2)
+    keys[: n // 3] = keys[n // 2]  # a long duplicate run
+    keys &= np.uint64((1 << key_bits) - 1) if key_bits < 64 else np.uint64(0xFFFFFFFFFFFFFFFF)
+    idx, sk = Array1D.argsort(G(keys, gpu), key_bits=key_bits)
+    widx, wsk = oracle.argsort(keys)
+    assert np.array_equal(H(sk), wsk)
+    assert np.array_equal(H(idx), widx)
+
+
+# ---------------------------------------------------------------------------
+# topology, refit, whole build
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("kind", ["u32", "u32dups", "u64", "u64dups"])
+def test_topology(gpu, oracle, kind):
+    rng = np.random.default_rng(21)
+    n = 50_000
+    if kind.startswith("u32"):
+        hi = 5000 if kind.endswith("dups") else 1 << 30
+        codes = np.sort(rng.integers(0, hi, n)).astype(np.uint32)
+    else:
+        hi = 5000 if kind.endswith("dups") else 1 << 63
+        codes = np.sort(rng.integers(0, hi, n, dtype=np.uint64))
+    idx2obj = rng.permutation(n).astype(np.uint32)
+    got = H(Mortons.make_bvhnodes_from_sorted_mortons(G(idx2obj, gpu), G(codes, gpu)))
+    want = oracle.bvhnodes_from_sorted(idx2obj, codes)
+    assert np.array_equal(got, want)
+
+
+def test_topology_single_and_pair(gpu, oracle):
+    for n in (1, 2):
+        codes = np.arange(n, dtype=np.uint32)
+        idx = np.arange(n, dtype=np.uint32)[::-1].copy()
+        got = H(Mortons.make_bvhnodes_from_sorted_mortons(G(idx, gpu), G(codes, gpu)))
+        assert np.array_equal(got, oracle.bvhnodes_from_sorted(idx, codes))
+
+
+MESHES = {
+    "sphere_c1": lambda: primitives.sphere_yup(1.0, 64, 128),
+    "torus": lambda: primitives.torus_zup(1.3, 0.4, 64, 32),
+    "octa6_noise": lambda: primitives.octahedral_sphere(6, noise_sigma=0.01, seed=1),
+    "cylinder": lambda: primitives.cylinder_closed_end_yup(0.5, 2.0, 40, 30),
+}
+
+
+@pytest.mark.parametrize("mesh", list(MESHES))
+@pytest.mark.parametrize("bits", [30, 63])
+def test_build_trimesh3(gpu, oracle, mesh, bits):
+    tri, vtx = MESHES[mesh]()
+    bn, aabb = TriMesh3.make_bvhnodes_bvhnode2aabb(G(tri, gpu), G(vtx, gpu), morton_bits=bits)
+    wbn, waabb = oracle.build_trimesh3(tri, vtx, morton_bits=bits)
+    assert np.array_equal(H(bn), wbn)
+    assert bits_equal(H(aabb), waabb)
+    no_device_errors()
+
+
+def test_refit_standalone_matches_oracle(gpu, oracle):
+    tri, vtx = MESHES["torus"]()
+    wbn, waabb = oracle.build_trimesh3(tri, vtx)
+    aabb = TriMesh3.make_bvhnode2aabb_from_bvhnodes(G(tri, gpu), G(vtx, gpu), G(wbn, gpu))
+    assert bits_equal(H(aabb), waabb)
+
+
+@pytest.mark.parametrize("bits", [30, 63])
+def test_build_points(gpu, oracle, bits):
+    rng = np.random.default_rng(31)
+    pts = rng.random((200_000, 3), dtype=np.float32)
+    bn, aabb = TriMesh3.make_bvhnodes_bvhnode2aabb_for_points(G(pts, gpu), morton_bits=bits)
+    wbn, waabb = oracle.build_points(pts, morton_bits=bits)
+    assert np.array_equal(H(bn), wbn)
+    assert bits_equal(H(aabb), waabb)
+
+
+# ---------------------------------------------------------------------------
+# ray-cast and depth
+# ---------------------------------------------------------------------------
+def _ortho_c1():
+    return np.diag([1.2, 1.2, 2.0, 1.0]).astype(np.float32)
+
+
+def _perspective(eye=(0.0, 0.0, 3.0), fovy=45.0, near=1.0, far=5.0):
+    import del_msh_dlpack.Mat44.torch as Mat44
+
+    w2c = Mat44.look_at(eye, (0, 0, 0), (0, 1, 0), dtype=torch.float64).numpy()
+    proj = Mat44.perspective(fovy, 1.0, near, far, dtype=torch.float64).numpy()
+    w2n = proj @ w2c
+    return np.linalg.inv(w2n).astype(np.float32), w2n.astype(np.float32)
+
+
+@pytest.mark.parametrize("mesh", ["sphere_c1", "torus", "octa6_noise"])
+@pytest.mark.parametrize("camera", ["ortho", "persp"])
+def test_pix2tri(gpu, oracle, mesh, camera):
+    tri, vtx = MESHES[mesh]()
+    bn, aabb = oracle.build_trimesh3(tri, vtx)
+    n2w = _ortho_c1() if camera == "ortho" else _perspective()[0]
+    W, H_ = 256, 192
+    pix, t = Pix2Tri.by_raycasting_with_coeff(G(tri, gpu), G(vtx, gpu), G(bn, gpu), G(aabb, gpu),
+                                              G(n2w, gpu), (W, H_))
+    wpix, wt, _ = oracle.pix2tri(tri, vtx, bn, aabb, n2w.T.reshape(-1), W, H_, num_threads=8)
+    assert np.array_equal(H(pix), wpix)
+    assert bits_equal(H(t), wt)
+    pix2 = Pix2Tri.by_raycasting(G(tri, gpu), G(vtx, gpu), G(bn, gpu), G(aabb, gpu), G(n2w, gpu), (W, H_))
+    assert np.array_equal(H(pix2), wpix)
+    no_device_errors()
+
+
+def test_pix2tri_deep_stack_path(gpu, oracle):
+    """A run of equal Morton codes whose triangles are stacked along the view
+    ray, listed far to near, makes the near child the rest of the run at every
+    level: the traversal needs ~K frames, more than the LDS stack holds, so the
+    rays go through the global-stack kernel."""
+    K = 80
+    quad = []
+    vtx = []
+    for k in range(K):  # far (k=0) to near; centroids differ by < 1 Morton cell
+        z = -0.001 + k * 2.0e-5
+        base = len(vtx)
+        vtx += [[-0.5, -0.5, z], [0.6, -0.45, z], [-0.45, 0.6, z]]
+        quad.append([base, base + 1, base + 2])
+    # two anchors far away set the unit-cube fit
+    for s in (-1.0, 1.0):
+        base = len(vtx)
+        vtx += [[s * 4, s * 4, 0], [s * 4 + 0.01, s * 4, 0], [s * 4, s * 4 + 0.01, 0]]
+        quad.append([base, base + 1, base + 2])
+    tri = np.array(quad, np.uint32)
+    vtx = np.array(vtx, np.float32)
+    bn, aabb = oracle.build_trimesh3(tri, vtx)
+    n2w = np.diag([1.0, 1.0, 1.0, 1.0]).astype(np.float32)
+    pix, t = Pix2Tri.by_raycasting_with_coeff(G(tri, gpu), G(vtx, gpu), G(bn, gpu), G(aabb, gpu),
+                                              G(n2w, gpu), (32, 32))
+    wpix, wt, _ = oracle.pix2tri(tri, vtx, bn, aabb, n2w.T.reshape(-1), 32, 32)
+    assert np.array_equal(H(pix), wpix)
+    assert bits_equal(H(t), wt)
+    assert (wpix == K - 1).sum() > 0  # the nearest sheet is what the rays see
+    no_device_errors()
+
+
+def test_pix2tri_empty_and_miss(gpu, oracle):
+    tri, vtx = MESHES["torus"]()
+    bn, aabb = oracle.build_trimesh3(tri, vtx)
+    n2w = np.diag([1.0, 1.0, 1.0, 1.0]).astype(np.float32)
+    n2w[0, 3] = 50.0  # camera looks far away from the mesh
+    pix = Pix2Tri.by_raycasting(G(tri, gpu), G(vtx, gpu), G(bn, gpu), G(aabb, gpu), G(n2w, gpu), (64, 64))
+    assert np.all(H(pix) == U32MAX)
+
+
+@pytest.mark.parametrize("camera", ["ortho", "persp"])
+def test_pix2depth(gpu, oracle, camera):
+    tri, vtx = MESHES["torus"]()
+    bn, aabb = oracle.build_trimesh3(tri, vtx)
+    n2w = _ortho_c1() if camera == "ortho" else _perspective()[0]
+    W, H_ = 128, 128
+    wpix, _, _ = oracle.pix2tri(tri, vtx, bn, aabb, n2w.T.reshape(-1), W, H_, num_threads=8)
+    depth = torch.zeros((H_, W), dtype=torch.float32, device=gpu)
+    Pix2Depth.update(depth, G(wpix, gpu), G(tri, gpu), G(vtx, gpu), G(n2w, gpu))
+    want = oracle.pix2depth(wpix, tri, vtx, n2w.T.reshape(-1), W, H_)
+    assert bits_equal(H(depth), want)
+    no_device_errors()
+
+
+# ---------------------------------------------------------------------------
+# edge-gradient backward
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("camera", ["ortho", "persp"])
+@pytest.mark.parametrize("nv", [1, 3])
+def test_edge_gradient_bwd(gpu, oracle, camera, nv):
+    tri, vtx = primitives.sphere_yup(0.8, 48, 96)
+    bn, aabb = oracle.build_trimesh3(tri, vtx)
+    if camera == "ortho":
+        w2n = np.diag([1.0, 1.0, 0.5, 1.0]).astype(np.float32)
+        n2w = np.linalg.inv(w2n).astype(np.float32)
+    else:
+        n2w, w2n = _perspective(eye=(0.3, 0.2, 3.0))
+    W, H_ = 160, 128
+    n2p = np.array([[0.5 * W, 0, 0, 0.5 * W], [0, -0.5 * H_, 0, 0.5 * H_], [0, 0, 0.5, 0.5], [0, 0, 0, 1]],
+                   np.float32)
+    w2p = (n2p @ w2n).astype(np.float32)
+    wpix, _, _ = oracle.pix2tri(tri, vtx, bn, aabb, n2w.T.reshape(-1), W, H_, num_threads=8)
+    rng = np.random.default_rng(40 + nv)
+    occ = (wpix != U32MAX).astype(np.float32)[..., None]
+    pix2val = np.concatenate([occ, rng.random((H_, W, nv - 1), dtype=np.float32)], axis=2)
+    dldw = rng.normal(size=(H_, W, nv)).astype(np.float32)
+    got = H(RasterizedEdgeGradient.bwd(G(tri, gpu), G(vtx, gpu), G(w2p, gpu), G(wpix, gpu),
+                                       G(pix2val, gpu), G(dldw, gpu)))
+    want = oracle.edge_gradient_bwd(tri, vtx, w2p.T.reshape(-1), wpix, pix2val, dldw)
+    # float atomics reorder the per-vertex sums: compare to summation rounding
+    scale = np.abs(want).max()
+    assert scale > 0
+    assert np.abs(got - want).max() <= 1e-5 * scale + 5e-6
+    no_device_errors()
+
+
+# ---------------------------------------------------------------------------
+# full-size properties (2^21 triangles, 2048^2 rays, 2^22 points)
+# ---------------------------------------------------------------------------
+def test_full_size_build_and_cast(gpu, oracle):
+    tri, vtx = primitives.octahedral_sphere(9, noise_sigma=0.01, seed=1)
+    assert tri.shape[0] == 1 << 21
+    t_tri, t_vtx = G(tri, gpu), G(vtx, gpu)
+    bn, aabb = TriMesh3.make_bvhnodes_bvhnode2aabb(t_tri, t_vtx, morton_bits=63)
+    wbn, waabb = oracle.build_trimesh3(tri, vtx, morton_bits=63)
+    assert np.array_equal(H(bn), wbn)
+    assert bits_equal(H(aabb), waabb)
+    assert oracle.check_bvh_topology(wbn, tri.shape[0]) == 0
+    n2w, _ = _perspective()
+    W = H_ = 2048
+    pix, t = Pix2Tri.by_raycasting_with_coeff(t_tri, t_vtx, bn, aabb, G(n2w, gpu), (W, H_))
+    pix, t = H(pix), H(t)
+    no_device_errors()
+    # sampled bit-exact check against the oracle's ordered search
+    rng = np.random.default_rng(5)
+    sel = rng.integers(0, W * H_, 20_000)
+    T = n2w.T.reshape(-1)
+    for ip in sel[:4000]:
+        ih, iw = divmod(int(ip), W)
+        o, d = oracle.ray_for_pixel(iw, ih, W, H_, T)
+        r = oracle.first_intersection_ray(o, d, tri, vtx, wbn, waabb)
+        if r is None:
+            assert pix[ih, iw] == U32MAX
+        else:
+            assert pix[ih, iw] == r[1] and np.float32(t[ih, iw]) == np.float32(r[0])
+    # coverage of a sphere of radius ~1 seen from distance 3 with fovy 45
+    frac = np.count_nonzero(pix != U32MAX) / (W * H_)
+    half = math.asin(1.0 / 3.0)
+    want = (math.tan(half) / math.tan(math.radians(22.5))) ** 2 * math.pi / 4
+    assert abs(frac - want) < 0.02
+
+
+def test_full_size_points_build(gpu, oracle):
+    rng = np.random.default_rng(2)
+    pts = rng.random((1 << 22, 3), dtype=np.float32)
+    bn, aabb = TriMesh3.make_bvhnodes_bvhnode2aabb_for_points(G(pts, gpu), morton_bits=63)
+    wbn, waabb = oracle.build_points(pts, morton_bits=63)
+    assert np.array_equal(H(bn), wbn)
+    assert bits_equal(H(aabb), waabb)
