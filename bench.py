@@ -90,7 +90,7 @@ def main():
     ws_build = _lib.workspace(L.dmsh_build_workspace_bytes(num_tri, 63), dev)
     pix2tri = torch.empty((H, W), dtype=torch.uint32, device=dev)
     pix2t = torch.empty((H, W), dtype=torch.float32, device=dev)
-    ws_cast = _lib.workspace(L.dmsh_pix2tri_workspace_bytes(W, H), dev)
+    ws_cast = _lib.workspace(L.dmsh_pix2tri_workspace_bytes(num_tri, W, H), dev)
     stream = torch.cuda.current_stream(dev)
     sp = stream.cuda_stream
 
@@ -186,10 +186,10 @@ def main():
         aabb_h = aabb.cpu().numpy()
         threads = min(16, os.cpu_count() or 1)
         tc = time.perf_counter()
-        wpix, _, (n_exp, n_leaf) = oracle.pix2tri(tri, vtx, nodes_h, aabb_h, n2w.T.reshape(-1), W, H,
-                                                  num_threads=threads, with_t=False)
+        wpix, _, (n_exp, n_leaf, fhist) = oracle.pix2tri(tri, vtx, nodes_h, aabb_h, n2w.T.reshape(-1),
+                                                         W, H, num_threads=threads, with_t=False)
         cpu_cast_s = time.perf_counter() - tc
-        cast_counts = (n_exp, n_leaf)
+        cast_counts = (n_exp, n_leaf, int(np.nonzero(fhist)[0].max()))
         mism = int(np.count_nonzero(wpix != pix2tri.cpu().numpy()))
         tb = time.perf_counter()
         oracle.build_trimesh3(tri, vtx, morton_bits=63)
@@ -252,7 +252,10 @@ def main():
         "points_build_mpts_per_s": round(npnt / (pbuild_ms * 1e-3) / 1e6, 2),
         "sort_alone_u64_2p22_ms": round(psort_ms, 4),
         "sort_alone_melem_per_s": round(npnt / (psort_ms * 1e-3) / 1e6, 2),
-        "raycast_traversal": ({"internal_expansions": cast_counts[0], "leaf_tests": cast_counts[1]}
+        "raycast_traversal": ({"internal_expansions": cast_counts[0], "leaf_tests": cast_counts[1],
+                               "max_stack_frames": cast_counts[2],
+                               "lds_stack_frames": L.dmsh_get_tuning(b"pix2tri.stack"),
+                               "waves_per_block": L.dmsh_get_tuning(b"pix2tri.waves")}
                               if cast_counts else None),
         "roofline": roof,
         "cpu_baseline": cpu,

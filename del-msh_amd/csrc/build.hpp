@@ -41,9 +41,9 @@ __global__ __launch_bounds__(256) void k_centroid_aabb(uint32_t n,
                                                        uint32_t* __restrict__ keys6,
                                                        uint32_t* __restrict__ err) {
   __shared__ uint32_t s_red[4][6];
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   uint32_t kmin[3] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu}, kmax[3] = {0u, 0u, 0u};
-  if (i < n) {
+  // grid-stride: a bounded number of blocks keeps the 6 global atomics cheap
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     float c[3];
     if (TRI) {
       const uint32_t* tv = tri2vtx + (size_t)i * 3;
@@ -67,7 +67,11 @@ __global__ __launch_bounds__(256) void k_centroid_aabb(uint32_t n,
       for (int d = 0; d < 3; ++d) c[d] = xyz[(size_t)i * 3 + d];
     }
 #pragma unroll
-    for (int d = 0; d < 3; ++d) kmin[d] = kmax[d] = f32_order_key(c[d]);
+    for (int d = 0; d < 3; ++d) {
+      const uint32_t key = f32_order_key(c[d]);
+      kmin[d] = key < kmin[d] ? key : kmin[d];
+      kmax[d] = key > kmax[d] ? key : kmax[d];
+    }
   }
 #pragma unroll
   for (int off = 32; off >= 1; off >>= 1) {
@@ -157,6 +161,9 @@ struct BuildLayout {
   uint32_t* keys6;
   sortk::Layout sort;
   uint32_t* flags;
+  uint32_t* start_count;
+  uint32_t* starts;
+  uint32_t* node2end;
   float* cntr;
   void* codes;
   uint32_t* idx;
@@ -177,9 +184,13 @@ inline bool carve(Arena& a, uint32_t n, bool tri, BuildLayout& L) {
   a.off += sizeof(uint32_t) * status_words;  // contiguous after the tickets
   L.flags = L.sort.status + status_words;
   a.off += sizeof(uint32_t) * (n > 1 ? n - 1 : 1);
+  L.start_count = L.flags + (n > 1 ? n - 1 : 1);
+  a.off += sizeof(uint32_t);
   L.zero_begin = L.sort.hist;
-  L.zero_words = (size_t)(L.flags - L.sort.hist) + (n > 1 ? n - 1 : 1);
+  L.zero_words = (size_t)(L.start_count - L.sort.hist) + 1;
   L.sort.zero_words = 0;
+  L.starts = a.take<uint32_t>(2 * (size_t)n);
+  L.node2end = a.take<uint32_t>(n > 1 ? n - 1 : 1);
   L.cntr = tri ? a.take<float>((size_t)n * 3) : nullptr;
   L.codes = a.take<KeyT>(n);
   L.idx = a.take<uint32_t>(n);
@@ -214,8 +225,8 @@ inline int build(hipStream_t s, uint32_t n, const uint32_t* tri2vtx, uint32_t nu
   hipLaunchKernelGGL(k_build_init, dim3(std::min<uint32_t>(ceil_div(L.zero_words, 256), 2048)),
                      dim3(256), 0, s, L.keys6, L.zero_begin, L.zero_words);
   if ((rc = launch_status()) != DMSH_OK) return rc;
-  hipLaunchKernelGGL(k_centroid_aabb<TRI>, dim3(ceil_div(n, 256)), dim3(256), 0, s, n, tri2vtx,
-                     TRI ? num_vtx : n, xyz, L.cntr, L.keys6, err);
+  hipLaunchKernelGGL(k_centroid_aabb<TRI>, dim3(std::min<uint32_t>(ceil_div(n, 256), 1024)),
+                     dim3(256), 0, s, n, tri2vtx, TRI ? num_vtx : n, xyz, L.cntr, L.keys6, err);
   if ((rc = launch_status()) != DMSH_OK) return rc;
   KeyT* codes = static_cast<KeyT*>(L.codes);
   const uint32_t mh_blocks = std::min<uint32_t>(sortk::num_tiles<KeyT>(n), 2048);
@@ -225,9 +236,10 @@ inline int build(hipStream_t s, uint32_t n, const uint32_t* tri2vtx, uint32_t nu
   // P is even for 30- and 63-bit codes: the passes end back in `codes`.
   rc = sortk::run_passes<KeyT>(s, n, kb, L.sort, codes, codes, L.idx, err);
   if (rc != DMSH_OK) return rc;
-  rc = launch_topology<KeyT>(s, n, bvhnodes, codes, L.idx);
+  rc = launch_topology<KeyT>(s, n, bvhnodes, codes, L.idx, L.node2end);
   if (rc != DMSH_OK) return rc;
-  return launch_refit<TRI>(s, aabb, L.flags, n, bvhnodes, tri2vtx, num_vtx, xyz, 0.f, err);
+  return launch_refit_two_phase<TRI>(s, aabb, L.flags, L.start_count, L.starts, n, bvhnodes,
+                                     L.node2end, tri2vtx, num_vtx, xyz, 0.f, err);
 }
 
 }  // namespace bld

@@ -1,5 +1,7 @@
 // dmsh_render.hip -- C ABI: ray-cast (pix2tri + hit t), depth, edge-gradient backward.
 #include <algorithm>
+#include <atomic>
+#include <cstring>
 
 #include "edgegrad.hpp"
 #include "raycast.hpp"
@@ -9,37 +11,87 @@ using namespace dmsh;
 
 namespace {
 
-constexpr int kCastStack = 16;  // LDS frames per lane
-constexpr int kCastWaves = 2;   // waves per workgroup
+// Defaults from a sweep on the 2^21-triangle / 2048^2-ray workload (MI355X):
+// 16 frames x 1 wave 7.38 ms, 24 x 2 7.89 ms, 32 x 4 11.2 ms.
+std::atomic<int> g_cast_stack{16};  // LDS frames per lane (tier 1)
+std::atomic<int> g_cast_waves{1};   // waves per workgroup (tier 1)
 
 struct CastLayout {
-  uint32_t* ovf_count;
-  uint32_t* ovf_list;
+  uint32_t* counts;
+  uint32_t* list2;
+  uint32_t* list3;
   uint2* deep;
+  rc::InnerNode* inner;
+  rc::LeafTri* leaf;
 };
 
-inline bool carve_cast(Arena& a, uint32_t num_pix, CastLayout& L) {
-  L.ovf_count = a.take<uint32_t>(64);
-  L.ovf_list = a.take<uint32_t>(num_pix ? num_pix : 1);
+inline bool carve_cast(Arena& a, uint32_t num_tri, uint32_t num_pix, CastLayout& L) {
+  L.counts = a.take<uint32_t>(64);
+  L.list2 = a.take<uint32_t>(num_pix ? num_pix : 1);
+  L.list3 = a.take<uint32_t>(num_pix ? num_pix : 1);
   L.deep = a.take<uint2>((size_t)rc::kDeepSlots * rc::kDeepFrames);
+  L.inner = a.take<rc::InnerNode>(num_tri > 1 ? num_tri - 1 : 1);
+  L.leaf = a.take<rc::LeafTri>(num_tri ? num_tri : 1);
   return a.ok();
 }
 
-inline bool scene_ok(uint32_t num_tri, const uint32_t* tri2vtx, const float* vtx2xyz,
-                     const uint32_t* bvhnodes, const float* aabb) {
-  if (num_tri >= rc::kNoneTri) return false;
-  if (num_tri > 0 && (!tri2vtx || !vtx2xyz || !bvhnodes || !aabb)) return false;
-  return true;
+template <int S, int WAVES>
+void launch_tier1(hipStream_t s, const rc::Packed& pk, const float* T, const rc::CastArgs& a) {
+  hipLaunchKernelGGL((rc::k_pix2tri<S, WAVES>), dim3(ceil_div(a.num_tiles, WAVES)),
+                     dim3(64 * WAVES), 0, s, pk, T, a);
+}
+
+template <int S>
+bool launch_tier1_waves(int waves, hipStream_t s, const rc::Packed& pk, const float* T,
+                        const rc::CastArgs& a) {
+  switch (waves) {
+    case 1: launch_tier1<S, 1>(s, pk, T, a); return true;
+    case 2: launch_tier1<S, 2>(s, pk, T, a); return true;
+    case 4: launch_tier1<S, 4>(s, pk, T, a); return true;
+    default: return false;
+  }
+}
+
+bool launch_tier1_any(int stack, int waves, hipStream_t s, const rc::Packed& pk, const float* T,
+                      const rc::CastArgs& a) {
+  switch (stack) {
+    case 16: return launch_tier1_waves<16>(waves, s, pk, T, a);
+    case 24: return launch_tier1_waves<24>(waves, s, pk, T, a);
+    case 32: return launch_tier1_waves<32>(waves, s, pk, T, a);
+    default: return false;
+  }
 }
 
 }  // namespace
 
 extern "C" {
 
-size_t dmsh_pix2tri_workspace_bytes(uint32_t img_w, uint32_t img_h) {
+int dmsh_set_tuning(const char* key, int value) {
+  if (!key) return DMSH_ERR_INVALID;
+  if (std::strcmp(key, "pix2tri.stack") == 0) {
+    if (value != 16 && value != 24 && value != 32) return DMSH_ERR_INVALID;
+    g_cast_stack.store(value);
+    return DMSH_OK;
+  }
+  if (std::strcmp(key, "pix2tri.waves") == 0) {
+    if (value != 1 && value != 2 && value != 4) return DMSH_ERR_INVALID;
+    g_cast_waves.store(value);
+    return DMSH_OK;
+  }
+  return DMSH_ERR_INVALID;
+}
+
+int dmsh_get_tuning(const char* key) {
+  if (!key) return -1;
+  if (std::strcmp(key, "pix2tri.stack") == 0) return g_cast_stack.load();
+  if (std::strcmp(key, "pix2tri.waves") == 0) return g_cast_waves.load();
+  return -1;
+}
+
+size_t dmsh_pix2tri_workspace_bytes(uint32_t num_tri, uint32_t img_w, uint32_t img_h) {
   Arena a{nullptr, 0, 0};
   CastLayout L;
-  carve_cast(a, img_w * img_h, L);
+  carve_cast(a, num_tri, img_w * img_h, L);
   return a.off + 256;
 }
 
@@ -50,18 +102,29 @@ int dmsh_pix2tri(void* stream, uint32_t* pix2tri, float* pix2t, uint32_t num_tri
   const uint64_t num_pix = (uint64_t)img_w * img_h;
   if (num_pix == 0) return DMSH_OK;
   if (num_pix >= (1ull << 32) || !pix2tri || !transform_ndc2world) return DMSH_ERR_INVALID;
-  if (!scene_ok(num_tri, tri2vtx, vtx2xyz, bvhnodes, bvhnode2aabb)) return DMSH_ERR_INVALID;
-  if (!ws || ws_bytes < dmsh_pix2tri_workspace_bytes(img_w, img_h)) return DMSH_ERR_WORKSPACE;
+  if (num_tri >= rc::kNoneTri) return DMSH_ERR_INVALID;
+  if (num_tri > 0 && (!tri2vtx || !vtx2xyz || !bvhnodes || !bvhnode2aabb)) return DMSH_ERR_INVALID;
+  if (!ws || ws_bytes < dmsh_pix2tri_workspace_bytes(num_tri, img_w, img_h))
+    return DMSH_ERR_WORKSPACE;
   Arena ar{static_cast<char*>(ws), ws_bytes, 0};
   CastLayout L;
-  if (!carve_cast(ar, (uint32_t)num_pix, L)) return DMSH_ERR_WORKSPACE;
+  if (!carve_cast(ar, num_tri, (uint32_t)num_pix, L)) return DMSH_ERR_WORKSPACE;
   uint32_t* err = device_error_word();
   if (!err) return DMSH_ERR_HIP;
   hipStream_t s = as_stream(stream);
-  int st = sortk::launch_fill(s, L.ovf_count, 1, 0u);
+  int st = sortk::launch_fill(s, L.counts, 2, 0u);
   if (st != DMSH_OK) return st;
-  rc::Scene sc{tri2vtx, vtx2xyz, bvhnodes, bvhnode2aabb, num_tri, num_vtx,
-               num_tri ? 2 * num_tri - 1 : 0};
+  if (num_tri > 1) {
+    hipLaunchKernelGGL(rc::k_pack_inner, dim3(ceil_div(num_tri - 1, 256)), dim3(256), 0, s,
+                       L.inner, bvhnodes, bvhnode2aabb, num_tri, err);
+    if ((st = launch_status()) != DMSH_OK) return st;
+  }
+  if (num_tri > 0) {
+    hipLaunchKernelGGL(rc::k_pack_leaf, dim3(ceil_div(num_tri, 256)), dim3(256), 0, s, L.leaf,
+                       bvhnodes, tri2vtx, vtx2xyz, num_tri, num_tri, num_vtx, err);
+    if ((st = launch_status()) != DMSH_OK) return st;
+  }
+  rc::Packed pk{L.inner, L.leaf, bvhnode2aabb, num_tri};
   rc::CastArgs a;
   a.pix2tri = pix2tri;
   a.pix2t = pix2t;
@@ -69,32 +132,44 @@ int dmsh_pix2tri(void* stream, uint32_t* pix2tri, float* pix2t, uint32_t num_tri
   a.img_h = img_h;
   a.tiles_x = ceil_div(img_w, 8);
   a.num_tiles = a.tiles_x * ceil_div(img_h, 8);
-  a.ovf_count = L.ovf_count;
-  a.ovf_list = L.ovf_list;
+  a.counts = L.counts;
+  a.list2 = L.list2;
+  a.list3 = L.list3;
   a.deep_stack = L.deep;
   a.err = err;
-  hipLaunchKernelGGL((rc::k_pix2tri<kCastStack, kCastWaves>), dim3(ceil_div(a.num_tiles, kCastWaves)),
-                     dim3(64 * kCastWaves), 0, s, sc, transform_ndc2world, a);
+  if (!launch_tier1_any(g_cast_stack.load(), g_cast_waves.load(), s, pk, transform_ndc2world, a))
+    return DMSH_ERR_INVALID;
   if ((st = launch_status()) != DMSH_OK) return st;
-  hipLaunchKernelGGL(rc::k_pix2tri_deep, dim3(rc::kDeepSlots / 64), dim3(64), 0, s, sc,
+  hipLaunchKernelGGL(rc::k_pix2tri_tier2, dim3(rc::kTier2Blocks), dim3(64), 0, s, pk,
+                     transform_ndc2world, a);
+  if ((st = launch_status()) != DMSH_OK) return st;
+  hipLaunchKernelGGL(rc::k_pix2tri_tier3, dim3(rc::kDeepSlots / 64), dim3(64), 0, s, pk,
                      transform_ndc2world, a);
   return launch_status();
 }
 
+size_t dmsh_pix2depth_workspace_bytes(void) { return 256; }
+
 int dmsh_pix2depth(void* stream, float* pix2depth, const uint32_t* pix2tri, uint32_t num_tri,
                    const uint32_t* tri2vtx, uint32_t num_vtx, const float* vtx2xyz,
-                   uint32_t img_w, uint32_t img_h, const float* transform_ndc2world) {
+                   uint32_t img_w, uint32_t img_h, const float* transform_ndc2world, void* ws,
+                   size_t ws_bytes) {
   const uint64_t num_pix = (uint64_t)img_w * img_h;
   if (num_pix == 0) return DMSH_OK;
   if (num_pix >= (1ull << 32) || !pix2depth || !pix2tri || !transform_ndc2world)
     return DMSH_ERR_INVALID;
   if (num_tri > 0 && (!tri2vtx || !vtx2xyz)) return DMSH_ERR_INVALID;
+  if (!ws || ws_bytes < dmsh_pix2depth_workspace_bytes()) return DMSH_ERR_WORKSPACE;
   uint32_t* err = device_error_word();
   if (!err) return DMSH_ERR_HIP;
-  rc::Scene sc{tri2vtx, vtx2xyz, nullptr, nullptr, num_tri, num_vtx, 0};
-  hipLaunchKernelGGL(rc::k_pix2depth, dim3(ceil_div(num_pix, 256)), dim3(256), 0,
-                     as_stream(stream), pix2depth, pix2tri, sc, img_w, img_h, transform_ndc2world,
-                     err);
+  hipStream_t s = as_stream(stream);
+  float* inv = static_cast<float*>(ws);
+  hipLaunchKernelGGL(rc::k_mat4_inverse, dim3(1), dim3(1), 0, s, transform_ndc2world, inv, err);
+  int st = launch_status();
+  if (st != DMSH_OK) return st;
+  rc::TriMesh sc{tri2vtx, vtx2xyz, num_tri, num_vtx};
+  hipLaunchKernelGGL(rc::k_pix2depth, dim3(ceil_div(num_pix, 256)), dim3(256), 0, s, pix2depth,
+                     pix2tri, sc, img_w, img_h, transform_ndc2world, inv, err);
   return launch_status();
 }
 

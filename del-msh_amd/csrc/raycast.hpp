@@ -12,12 +12,22 @@
 // Executed as a loop over an explicit frame stack.  A frame is pushed only
 // where both children are hit (a single hit child is a tail call) and holds
 // 8 bytes: before the near subtree returns it stores (far child, far entry t);
-// after the decision to descend far it stores the near result (tri, t).  The
-// stack lives in LDS, lane-interleaved ([depth][thread]) so a wave's pushes
-// and pops hit 64 distinct banks; a ray that needs more than S frames is
-// queued and re-run by a second kernel with a 4096-frame stack in global
-// memory.  Pixels are assigned to waves in 8x8 tiles and tiles to XCDs in
-// contiguous runs so that neighbouring rays share BVH nodes in one L2.
+// after the decision to descend far it stores the near result (tri, t); a far
+// descent with no near result keeps no frame.
+//
+// Layout: before traversal the BVH is packed into
+//   inner[i] (64 B, internal node i): both child boxes + both child refs
+//   leaf[k]  (48 B, leaf k):          the triangle's three vertices + its id
+// so one node expansion is one 64-byte load and one leaf test one 48-byte load
+// (the reference layout needs a row load, then dependent box / index / vertex
+// gathers).  The values are copies, so the arithmetic is unchanged.
+//
+// Stack tiers: the main kernel keeps S frames per lane in LDS
+// ([depth][thread], conflict-free for 64-bit accesses); a ray that needs more
+// is queued and restarted by a kernel with 64 LDS frames, and a ray that needs
+// more than that by one with a 4096-frame stack in global memory.  Pixels go
+// to waves in 8x8 tiles and tiles to XCDs in contiguous runs, so neighbouring
+// rays share BVH nodes in one L2.
 #pragma once
 
 #include "dmsh_common.hpp"
@@ -25,21 +35,126 @@
 namespace dmsh {
 namespace rc {
 
-constexpr uint32_t kState1 = 0x80000000u;
-constexpr uint32_t kNoneTri = 0x7FFFFFFFu;
+constexpr uint32_t kLeafBit = 0x40000000u;   // child ref: leaf slot
+constexpr uint32_t kIndexMask = 0x3FFFFFFFu;
+constexpr uint32_t kState1 = 0x80000000u;    // frame holds a near result
+constexpr uint32_t kNoneTri = 0x3FFFFFFFu;   // frame: near result is none
+constexpr int kTier2Frames = 64;
 constexpr uint32_t kDeepFrames = 4096;
 constexpr uint32_t kDeepSlots = 256;
+constexpr uint32_t kTier2Blocks = 512;
 
 enum : int { kTraceOk = 0, kTraceOverflow = 1, kTraceBadIndex = 2 };
 
-struct Scene {
-  const uint32_t* __restrict__ tri2vtx;
-  const float* __restrict__ xyz;
-  const uint32_t* __restrict__ bvhnodes;
-  const float* __restrict__ aabb;
-  uint32_t num_tri, num_vtx, num_node;
+struct alignas(16) InnerNode {
+  float4 a;  // c0.lo.xyz, c0.hi.x
+  float4 b;  // c0.hi.yz, c1.lo.xy
+  float4 c;  // c1.lo.z, c1.hi.xyz
+  uint4 d;   // ref c0, ref c1, -, -
+};
+struct alignas(16) LeafTri {
+  float4 a;  // p0.xyz, p1.x
+  float4 b;  // p1.yz, p2.xy
+  float4 c;  // p2.z, tri id (bits), -, -
 };
 
+struct Packed {
+  const InnerNode* __restrict__ inner;
+  const LeafTri* __restrict__ leaf;
+  const float* __restrict__ root_box;  // bvhnode2aabb row 0
+  uint32_t num_leaf;
+};
+
+// ---------------------------------------------------------------------------
+// packing
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t child_ref(const uint32_t* __restrict__ bvhnodes, uint32_t c,
+                                              uint32_t num_leaf, bool& bad) {
+  const uint32_t nb = num_leaf - 1, num_node = 2 * num_leaf - 1;
+  if (c >= num_node) {
+    bad = true;
+    return 0;
+  }
+  const bool leaf = bvhnodes[(size_t)c * 3 + 2] == DMSH_U32MAX;
+  if (leaf != (c >= nb)) {  // not an LBVH layout
+    bad = true;
+    return 0;
+  }
+  return leaf ? (kLeafBit | (c - nb)) : c;
+}
+
+static __global__ __launch_bounds__(256) void k_pack_inner(InnerNode* __restrict__ inner,
+                                                           const uint32_t* __restrict__ bvhnodes,
+                                                           const float* __restrict__ aabb,
+                                                           uint32_t num_leaf,
+                                                           uint32_t* __restrict__ err) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i + 1 >= num_leaf) return;
+  const uint32_t* row = bvhnodes + (size_t)i * 3;
+  const uint32_t c0 = row[1], c1 = row[2];
+  bool bad = (c1 == DMSH_U32MAX);
+  const uint32_t r0 = child_ref(bvhnodes, c0, num_leaf, bad);
+  const uint32_t r1 = child_ref(bvhnodes, c1, num_leaf, bad);
+  InnerNode nd;
+  if (bad) {
+    raise_error(err, DMSH_DEVERR_INDEX);
+    // an empty box on both sides: every ray misses this node
+    nd.a = make_float4(1.f, 1.f, 1.f, 0.f);
+    nd.b = make_float4(0.f, 0.f, 1.f, 1.f);
+    nd.c = make_float4(1.f, 0.f, 0.f, 0.f);
+    nd.d = make_uint4(kLeafBit, kLeafBit, 0u, 0u);
+  } else {
+    const float* b0 = aabb + (size_t)c0 * 6;
+    const float* b1 = aabb + (size_t)c1 * 6;
+    nd.a = make_float4(b0[0], b0[1], b0[2], b0[3]);
+    nd.b = make_float4(b0[4], b0[5], b1[0], b1[1]);
+    nd.c = make_float4(b1[2], b1[3], b1[4], b1[5]);
+    nd.d = make_uint4(r0, r1, 0u, 0u);
+  }
+  inner[i] = nd;
+}
+
+static __global__ __launch_bounds__(256) void k_pack_leaf(LeafTri* __restrict__ leaf,
+                                                          const uint32_t* __restrict__ bvhnodes,
+                                                          const uint32_t* __restrict__ tri2vtx,
+                                                          const float* __restrict__ xyz,
+                                                          uint32_t num_leaf, uint32_t num_tri,
+                                                          uint32_t num_vtx,
+                                                          uint32_t* __restrict__ err) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= num_leaf) return;
+  const uint32_t* row = bvhnodes + (size_t)(num_leaf - 1 + k) * 3;
+  const uint32_t tri = row[1];
+  LeafTri lt;
+  bool ok = row[2] == DMSH_U32MAX && tri < num_tri;
+  uint32_t v[3] = {0, 0, 0};
+  if (ok) {
+    const uint32_t* tv = tri2vtx + (size_t)tri * 3;
+    v[0] = tv[0];
+    v[1] = tv[1];
+    v[2] = tv[2];
+    ok = v[0] < num_vtx && v[1] < num_vtx && v[2] < num_vtx;
+  }
+  if (!ok) {
+    raise_error(err, DMSH_DEVERR_INDEX);
+    // a degenerate triangle: never hit
+    lt.a = make_float4(0.f, 0.f, 0.f, 0.f);
+    lt.b = make_float4(0.f, 0.f, 0.f, 0.f);
+    lt.c = make_float4(0.f, __uint_as_float(DMSH_U32MAX), 0.f, 0.f);
+  } else {
+    const float* p0 = xyz + (size_t)v[0] * 3;
+    const float* p1 = xyz + (size_t)v[1] * 3;
+    const float* p2 = xyz + (size_t)v[2] * 3;
+    lt.a = make_float4(p0[0], p0[1], p0[2], p1[0]);
+    lt.b = make_float4(p1[1], p1[2], p2[0], p2[1]);
+    lt.c = make_float4(p2[2], __uint_as_float(tri), 0.f, 0.f);
+  }
+  leaf[k] = lt;
+}
+
+// ---------------------------------------------------------------------------
+// traversal
+// ---------------------------------------------------------------------------
 struct Hit {
   bool has;
   float t;
@@ -62,83 +177,80 @@ struct GlobalStack {
   __device__ __forceinline__ uint2 load(int i) const { return base[i]; }
 };
 
-__device__ __forceinline__ void load_box(const float* __restrict__ aabb, uint32_t node, float lo[3],
-                                         float hi[3]) {
-  const float2* p = reinterpret_cast<const float2*>(aabb + (size_t)node * 6);
-  const float2 a = p[0], b = p[1], c = p[2];
-  lo[0] = a.x;
-  lo[1] = a.y;
-  lo[2] = b.x;
-  hi[0] = b.y;
-  hi[1] = c.x;
-  hi[2] = c.y;
-}
-
 // Rust's tuple `<` on (tmin, tmax).
 __device__ __forceinline__ bool tuple_lt(float a0, float a1, float b0, float b1) {
   return (a0 == b0) ? (a1 < b1) : (a0 < b0);
 }
 
+__device__ __forceinline__ void leaf_test(const Packed& pk, uint32_t slot, const float org[3],
+                                          const float dir[3], bool& has, float& t_out,
+                                          uint32_t& tri_out) {
+  const LeafTri lt = pk.leaf[slot];
+  const float p0[3] = {lt.a.x, lt.a.y, lt.a.z};
+  const float p1[3] = {lt.a.w, lt.b.x, lt.b.y};
+  const float p2[3] = {lt.b.z, lt.b.w, lt.c.x};
+  float t;
+  has = tri_intersect(p0, p1, p2, org, dir, true, t, nullptr) && (t < __builtin_inff());
+  t_out = has ? t : 0.f;
+  tri_out = has ? __float_as_uint(lt.c.y) : DMSH_U32MAX;
+}
+
 template <class Stack>
-__device__ __forceinline__ int trace(const Scene& sc, const float org[3], const float dir[3],
+__device__ __forceinline__ int trace(const Packed& pk, const float org[3], const float dir[3],
                                      Stack& stk, Hit& out) {
   out.has = false;
   out.t = 0.f;
   out.tri = DMSH_U32MAX;
-  if (sc.num_node == 0) return kTraceOk;
+  if (pk.num_leaf == 0) return kTraceOk;
   RaySlab r;
   make_slab(org, dir, r);
-  float lo[3], hi[3], ta, tb;
-  load_box(sc.aabb, 0, lo, hi);
-  if (!slab_hit(r, lo, hi, ta, tb)) return kTraceOk;
-
-  uint32_t node = 0;
-  int sp = 0;
-  bool descend = true;
+  {
+    const float lo[3] = {pk.root_box[0], pk.root_box[1], pk.root_box[2]};
+    const float hi[3] = {pk.root_box[3], pk.root_box[4], pk.root_box[5]};
+    float ta, tb;
+    if (!slab_hit(r, lo, hi, ta, tb)) return kTraceOk;
+  }
   bool res_has = false;
   float res_t = 0.f;
   uint32_t res_tri = DMSH_U32MAX;
+  if (pk.num_leaf == 1) {
+    leaf_test(pk, 0, org, dir, res_has, res_t, res_tri);
+    out.has = res_has;
+    out.t = res_t;
+    out.tri = res_tri;
+    return kTraceOk;
+  }
+  uint32_t node = 0;
+  int sp = 0;
+  bool expand = true;
   for (;;) {
-    if (descend) {
-      if (node >= sc.num_node) return kTraceBadIndex;
-      const uint32_t* row = sc.bvhnodes + (size_t)node * 3;
-      const uint32_t c0 = row[1], c1 = row[2];
-      if (c1 == DMSH_U32MAX) {  // leaf: c0 is the triangle
-        if (c0 >= sc.num_tri) return kTraceBadIndex;
-        const uint32_t* tv = sc.tri2vtx + (size_t)c0 * 3;
-        const uint32_t i0 = tv[0], i1 = tv[1], i2 = tv[2];
-        if (i0 >= sc.num_vtx || i1 >= sc.num_vtx || i2 >= sc.num_vtx) return kTraceBadIndex;
-        float p0[3], p1[3], p2[3], t;
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-          p0[k] = sc.xyz[(size_t)i0 * 3 + k];
-          p1[k] = sc.xyz[(size_t)i1 * 3 + k];
-          p2[k] = sc.xyz[(size_t)i2 * 3 + k];
-        }
-        res_has = tri_intersect(p0, p1, p2, org, dir, true, t, nullptr) && (t < __builtin_inff());
-        res_t = res_has ? t : 0.f;
-        res_tri = res_has ? c0 : DMSH_U32MAX;
-        descend = false;
-        continue;
-      }
-      if (c0 >= sc.num_node || c1 >= sc.num_node) return kTraceBadIndex;
-      float l0[3], h0[3], l1[3], h1[3], tl0, tl1, tr0, tr1;
-      load_box(sc.aabb, c0, l0, h0);
-      load_box(sc.aabb, c1, l1, h1);
+    if (expand) {
+      const InnerNode nd = pk.inner[node];
+      const float l0[3] = {nd.a.x, nd.a.y, nd.a.z}, h0[3] = {nd.a.w, nd.b.x, nd.b.y};
+      const float l1[3] = {nd.b.z, nd.b.w, nd.c.x}, h1[3] = {nd.c.y, nd.c.z, nd.c.w};
+      float tl0, tl1, tr0, tr1;
       const bool hl = slab_hit(r, l0, h0, tl0, tl1);
       const bool hr = slab_hit(r, l1, h1, tr0, tr1);
+      uint32_t next;
       if (hl && hr) {
         const bool nl = tuple_lt(tl0, tl1, tr0, tr1);
         if (sp == Stack::kCap) return kTraceOverflow;
-        stk.store(sp++, make_uint2(nl ? c1 : c0, __float_as_uint(nl ? tr0 : tl0)));
-        node = nl ? c0 : c1;
+        stk.store(sp++, make_uint2(nl ? nd.d.y : nd.d.x, __float_as_uint(nl ? tr0 : tl0)));
+        next = nl ? nd.d.x : nd.d.y;
       } else if (hl || hr) {
-        node = hl ? c0 : c1;
+        next = hl ? nd.d.x : nd.d.y;
       } else {
         res_has = false;
         res_t = 0.f;
         res_tri = DMSH_U32MAX;
-        descend = false;
+        expand = false;
+        continue;
+      }
+      if (next & kLeafBit) {
+        leaf_test(pk, next & kIndexMask, org, dir, res_has, res_t, res_tri);
+        expand = false;
+      } else {
+        node = next;
       }
     } else {
       if (sp == 0) break;
@@ -147,23 +259,34 @@ __device__ __forceinline__ int trace(const Scene& sc, const float org[3], const 
         // near subtree done: prune the far child or descend into it
         if (res_has && __uint_as_float(f.y) > res_t) {
           --sp;
+          continue;
+        }
+        if (res_has) {
+          stk.store(sp - 1, make_uint2(kState1 | res_tri, __float_as_uint(res_t)));
         } else {
-          stk.store(sp - 1, make_uint2(kState1 | (res_has ? res_tri : kNoneTri),
-                                       __float_as_uint(res_t)));
+          --sp;  // combine(none, far) = far: no frame needed
+        }
+        if (f.x & kLeafBit) {
+          bool fh;
+          float ft;
+          uint32_t ftri;
+          leaf_test(pk, f.x & kIndexMask, org, dir, fh, ft, ftri);
+          res_has = fh;
+          res_t = ft;
+          res_tri = ftri;
+        } else {
           node = f.x;
-          descend = true;
+          expand = true;
         }
       } else {
         // far subtree done: combine with the stored near result
         --sp;
         const uint32_t rn_tri = f.x & ~kState1;
-        if (rn_tri != kNoneTri) {
-          const float rn_t = __uint_as_float(f.y);
-          if (!res_has || rn_t < res_t) {
-            res_has = true;
-            res_t = rn_t;
-            res_tri = rn_tri;
-          }
+        const float rn_t = __uint_as_float(f.y);
+        if (!res_has || rn_t < res_t) {
+          res_has = true;
+          res_t = rn_t;
+          res_tri = rn_tri;
         }
       }
     }
@@ -186,8 +309,9 @@ struct CastArgs {
   uint32_t* pix2tri;
   float* pix2t;
   uint32_t img_w, img_h, tiles_x, num_tiles;
-  uint32_t* ovf_count;
-  uint32_t* ovf_list;
+  uint32_t* counts;  // [0]: tier-2 queue length, [1]: tier-3 queue length
+  uint32_t* list2;
+  uint32_t* list3;
   uint2* deep_stack;
   uint32_t* err;
 };
@@ -197,8 +321,14 @@ __device__ __forceinline__ void write_result(const CastArgs& a, uint32_t ip, con
   if (a.pix2t) a.pix2t[ip] = h.has ? h.t : 0.f;
 }
 
+__device__ __forceinline__ void load_transform(const float* __restrict__ T, float m[16]) {
+#pragma unroll
+  for (int k = 0; k < 16; ++k) m[k] = T[k];
+}
+
+// Tier 1: one ray per lane, 8x8 pixel tiles per wave.
 template <int S, int WAVES>
-__global__ __launch_bounds__(64 * WAVES) void k_pix2tri(Scene sc, const float* __restrict__ T,
+__global__ __launch_bounds__(64 * WAVES) void k_pix2tri(Packed pk, const float* __restrict__ T,
                                                         CastArgs a) {
   __shared__ uint2 s_stack[S][64 * WAVES];
   const uint32_t tid = threadIdx.x, wave = tid >> 6, lane = tid & 63u;
@@ -208,46 +338,69 @@ __global__ __launch_bounds__(64 * WAVES) void k_pix2tri(Scene sc, const float* _
   const uint32_t ih = (tile / a.tiles_x) * 8u + (lane >> 3);
   if (iw >= a.img_w || ih >= a.img_h) return;
   float m[16];
-#pragma unroll
-  for (int k = 0; k < 16; ++k) m[k] = T[k];
+  load_transform(T, m);
   float org[3], dir[3];
   ray_for_pixel(iw, ih, a.img_w, a.img_h, m, org, dir);
   LdsStack<S> stk{&s_stack[0][tid], 64 * WAVES};
   Hit h;
-  const int st = trace(sc, org, dir, stk, h);
+  const int st = trace(pk, org, dir, stk, h);
   const uint32_t ip = ih * a.img_w + iw;
   if (st == kTraceOverflow) {
-    const uint32_t slot = atomicAdd(a.ovf_count, 1u);
-    a.ovf_list[slot] = ip;
+    a.list2[atomicAdd(a.counts + 0, 1u)] = ip;
     return;
   }
-  if (st == kTraceBadIndex) raise_error(a.err, DMSH_DEVERR_INDEX);
   write_result(a, ip, h);
 }
 
-// Re-runs the queued rays with a deep stack; a fixed grid of kDeepSlots
-// threads strides over the queue, each thread owning one stack slot.
-static __global__ __launch_bounds__(64) void k_pix2tri_deep(Scene sc, const float* __restrict__ T,
-                                                     CastArgs a) {
-  const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
-  const uint32_t nthr = gridDim.x * blockDim.x;
+// Tier 2: queued rays, 64 LDS frames per lane, grid-stride over the queue.
+static __global__ __launch_bounds__(64) void k_pix2tri_tier2(Packed pk,
+                                                             const float* __restrict__ T,
+                                                             CastArgs a) {
+  __shared__ uint2 s_stack[kTier2Frames][64];
   const uint32_t num_pix = a.img_w * a.img_h;
-  uint32_t count = *a.ovf_count;
+  uint32_t count = a.counts[0];
   if (count > num_pix) count = num_pix;
   float m[16];
-#pragma unroll
-  for (int k = 0; k < 16; ++k) m[k] = T[k];
-  GlobalStack stk{a.deep_stack + (size_t)gid * kDeepFrames};
-  for (uint32_t e = gid; e < count; e += nthr) {
-    const uint32_t ip = a.ovf_list[e];
+  load_transform(T, m);
+  LdsStack<kTier2Frames> stk{&s_stack[0][threadIdx.x], 64};
+  for (uint32_t e = blockIdx.x * 64u + threadIdx.x; e < count; e += gridDim.x * 64u) {
+    const uint32_t ip = a.list2[e];
     const uint32_t ih = ip / a.img_w, iw = ip - ih * a.img_w;
     float org[3], dir[3];
     ray_for_pixel(iw, ih, a.img_w, a.img_h, m, org, dir);
     Hit h;
-    const int st = trace(sc, org, dir, stk, h);
-    if (st == kTraceOverflow) raise_error(a.err, DMSH_DEVERR_STACK);
-    if (st == kTraceBadIndex) raise_error(a.err, DMSH_DEVERR_INDEX);
-    if (st != kTraceOk) h.has = false;
+    const int st = trace(pk, org, dir, stk, h);
+    if (st == kTraceOverflow) {
+      a.list3[atomicAdd(a.counts + 1, 1u)] = ip;
+      continue;
+    }
+    write_result(a, ip, h);
+  }
+}
+
+// Tier 3: a fixed grid of kDeepSlots threads, each owning a global stack slot.
+static __global__ __launch_bounds__(64) void k_pix2tri_tier3(Packed pk,
+                                                             const float* __restrict__ T,
+                                                             CastArgs a) {
+  const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t nthr = gridDim.x * blockDim.x;
+  const uint32_t num_pix = a.img_w * a.img_h;
+  uint32_t count = a.counts[1];
+  if (count > num_pix) count = num_pix;
+  float m[16];
+  load_transform(T, m);
+  GlobalStack stk{a.deep_stack + (size_t)gid * kDeepFrames};
+  for (uint32_t e = gid; e < count; e += nthr) {
+    const uint32_t ip = a.list3[e];
+    const uint32_t ih = ip / a.img_w, iw = ip - ih * a.img_w;
+    float org[3], dir[3];
+    ray_for_pixel(iw, ih, a.img_w, a.img_h, m, org, dir);
+    Hit h;
+    const int st = trace(pk, org, dir, stk, h);
+    if (st != kTraceOk) {
+      raise_error(a.err, DMSH_DEVERR_STACK);
+      h.has = false;
+    }
     write_result(a, ip, h);
   }
 }
@@ -255,6 +408,12 @@ static __global__ __launch_bounds__(64) void k_pix2tri_deep(Scene sc, const floa
 // ---------------------------------------------------------------------------
 // depth
 // ---------------------------------------------------------------------------
+
+struct TriMesh {
+  const uint32_t* __restrict__ tri2vtx;
+  const float* __restrict__ xyz;
+  uint32_t num_tri, num_vtx;
+};
 
 // Gauss-Jordan inverse with partial pivoting of a column-major 4x4 (pivot =
 // first row with the largest |a|); returns false when singular.
@@ -298,25 +457,29 @@ __device__ __forceinline__ bool mat4_inverse(const float* m, float* inv) {
   return true;
 }
 
+// One thread: inverse of the transform into inv[0..15], flag in inv[16].
+static __global__ void k_mat4_inverse(const float* __restrict__ T, float* __restrict__ inv,
+                                      uint32_t* __restrict__ err) {
+  float m[16], r[16];
+  for (int k = 0; k < 16; ++k) m[k] = T[k];
+  const bool ok = mat4_inverse(m, r);
+  for (int k = 0; k < 16; ++k) inv[k] = r[k];
+  inv[16] = ok ? 1.f : 0.f;
+  if (!ok) raise_error(err, DMSH_DEVERR_SINGULAR);
+}
+
 static __global__ __launch_bounds__(256) void k_pix2depth(float* __restrict__ depth,
-                                                   const uint32_t* __restrict__ pix2tri, Scene sc,
-                                                   uint32_t img_w, uint32_t img_h,
-                                                   const float* __restrict__ T,
-                                                   uint32_t* __restrict__ err) {
-  __shared__ float s_inv[16];
-  __shared__ int s_ok;
-  if (threadIdx.x == 0) {
-    float m[16];
-    for (int k = 0; k < 16; ++k) m[k] = T[k];
-    s_ok = mat4_inverse(m, s_inv) ? 1 : 0;
-    if (!s_ok && blockIdx.x == 0) raise_error(err, DMSH_DEVERR_SINGULAR);
-  }
-  __syncthreads();
+                                                          const uint32_t* __restrict__ pix2tri,
+                                                          TriMesh sc, uint32_t img_w,
+                                                          uint32_t img_h,
+                                                          const float* __restrict__ T,
+                                                          const float* __restrict__ inv,
+                                                          uint32_t* __restrict__ err) {
   const uint32_t ip = blockIdx.x * blockDim.x + threadIdx.x;
   if (ip >= img_w * img_h) return;
   const uint32_t tri = pix2tri[ip];
   float out = 0.f;
-  if (tri != DMSH_U32MAX && s_ok) {
+  if (tri != DMSH_U32MAX && inv[16] != 0.f) {
     if (tri >= sc.num_tri) {
       raise_error(err, DMSH_DEVERR_INDEX);
     } else {
@@ -325,9 +488,9 @@ static __global__ __launch_bounds__(256) void k_pix2depth(float* __restrict__ de
       if (i0 >= sc.num_vtx || i1 >= sc.num_vtx || i2 >= sc.num_vtx) {
         raise_error(err, DMSH_DEVERR_INDEX);
       } else {
-        float m[16];
-#pragma unroll
-        for (int k = 0; k < 16; ++k) m[k] = T[k];
+        float m[16], w2n[16];
+        load_transform(T, m);
+        load_transform(inv, w2n);
         const uint32_t ih = ip / img_w, iw = ip - ih * img_w;
         float org[3], dir[3], p0[3], p1[3], p2[3], t;
         ray_for_pixel(iw, ih, img_w, img_h, m, org, dir);
@@ -340,7 +503,7 @@ static __global__ __launch_bounds__(256) void k_pix2depth(float* __restrict__ de
         if (tri_intersect(p0, p1, p2, org, dir, false, t, nullptr)) {
           const float pos[3] = {t * dir[0] + org[0], t * dir[1] + org[1], t * dir[2] + org[2]};
           float ndc[3];
-          if (transform_homogeneous(s_inv, pos[0], pos[1], pos[2], ndc)) out = (ndc[2] + 1.f) * 0.5f;
+          if (transform_homogeneous(w2n, pos[0], pos[1], pos[2], ndc)) out = (ndc[2] + 1.f) * 0.5f;
         }
       }
     }

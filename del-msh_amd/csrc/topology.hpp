@@ -85,15 +85,19 @@ __device__ __forceinline__ uint32_t node_split(const KeyT* __restrict__ c, uint3
   return split;
 }
 
+// node2end (nullable): for internal node i, the end of its leaf range that is
+// not i itself (node i always sits at one end of its range).
 template <typename KeyT>
 __global__ __launch_bounds__(256) void k_topology(uint32_t n, uint32_t* __restrict__ bvhnodes,
                                                   const KeyT* __restrict__ codes,
-                                                  const uint32_t* __restrict__ idx2obj) {
+                                                  const uint32_t* __restrict__ idx2obj,
+                                                  uint32_t* __restrict__ node2end) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t nb = n - 1;
   if (i >= nb) return;
   uint32_t first, last;
   node_range<KeyT>(codes, n, i, first, last);
+  if (node2end) node2end[i] = (first == i) ? last : first;
   const uint32_t split = node_split<KeyT>(codes, first, last);
   if (first == split) {
     const uint32_t a = nb + split;
@@ -127,13 +131,13 @@ static __global__ void k_topology_single(uint32_t* __restrict__ bvhnodes,
 
 template <typename KeyT>
 inline int launch_topology(hipStream_t s, uint32_t n, uint32_t* bvhnodes, const KeyT* codes,
-                           const uint32_t* idx2obj) {
+                           const uint32_t* idx2obj, uint32_t* node2end = nullptr) {
   if (n == 0) return DMSH_OK;
   if (n == 1) {
     hipLaunchKernelGGL(k_topology_single, dim3(1), dim3(1), 0, s, bvhnodes, idx2obj);
   } else {
     hipLaunchKernelGGL(k_topology<KeyT>, dim3(ceil_div(n - 1, 256)), dim3(256), 0, s, n, bvhnodes,
-                       codes, idx2obj);
+                       codes, idx2obj, node2end);
   }
   return launch_status();
 }

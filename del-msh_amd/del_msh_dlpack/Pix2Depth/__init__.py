@@ -11,9 +11,11 @@ def update(pix2depth, pix2tri, tri2vtx, vtx2xyz, transform_ndc2world, stream_ptr
     dev = _lib.require_gpu(pix2depth, pix2tri, tri2vtx, vtx2xyz, transform_ndc2world)
     img_h, img_w = pix2depth.shape
     stream = stream_ptr or _lib.stream_of(dev)
+    ws = _lib.workspace(_lib.lib().dmsh_pix2depth_workspace_bytes(), dev)
     _lib.check(
         _lib.lib().dmsh_pix2depth(
             stream, _lib.ptr(pix2depth), _lib.ptr(pix2tri), tri2vtx.shape[0], _lib.ptr(tri2vtx),
-            vtx2xyz.shape[0], _lib.ptr(vtx2xyz), img_w, img_h, _lib.ptr(transform_ndc2world)),
+            vtx2xyz.shape[0], _lib.ptr(vtx2xyz), img_w, img_h, _lib.ptr(transform_ndc2world),
+            _lib.ptr(ws), ws.numel()),
         "dmsh_pix2depth",
     )

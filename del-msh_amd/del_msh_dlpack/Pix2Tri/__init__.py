@@ -14,7 +14,7 @@ def update_pix2tri(tri2vtx, vtx2xyz, bvhnodes, bvhnode2aabb, transform_ndc2world
     img_h, img_w = pix2tri.shape
     num_tri = tri2vtx.shape[0]
     assert bvhnodes.shape[0] == max(2 * num_tri - 1, 0)
-    ws = _lib.workspace(_lib.lib().dmsh_pix2tri_workspace_bytes(img_w, img_h), dev)
+    ws = _lib.workspace(_lib.lib().dmsh_pix2tri_workspace_bytes(num_tri, img_w, img_h), dev)
     stream = stream_ptr or _lib.stream_of(dev)
     _lib.check(
         _lib.lib().dmsh_pix2tri(

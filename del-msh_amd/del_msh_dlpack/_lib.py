@@ -51,7 +51,10 @@ _PROTOTYPES = {
         c_int,
         [c_void_p, c_u32, c_void_p, c_u32, c_void_p, c_void_p, c_void_p, c_size],
     ),
-    "dmsh_pix2tri_workspace_bytes": (c_size, [c_u32, c_u32]),
+    "dmsh_set_tuning": (c_int, [ctypes.c_char_p, c_int]),
+    "dmsh_get_tuning": (c_int, [ctypes.c_char_p]),
+    "dmsh_pix2tri_workspace_bytes": (c_size, [c_u32, c_u32, c_u32]),
+    "dmsh_pix2depth_workspace_bytes": (c_size, []),
     "dmsh_pix2tri": (
         c_int,
         [c_void_p, c_void_p, c_void_p, c_u32, c_void_p, c_u32, c_void_p, c_u32, c_u32, c_void_p,
@@ -59,7 +62,8 @@ _PROTOTYPES = {
     ),
     "dmsh_pix2depth": (
         c_int,
-        [c_void_p, c_void_p, c_void_p, c_u32, c_void_p, c_u32, c_void_p, c_u32, c_u32, c_void_p],
+        [c_void_p, c_void_p, c_void_p, c_u32, c_void_p, c_u32, c_void_p, c_u32, c_u32, c_void_p,
+         c_void_p, c_size],
     ),
     "dmsh_rasterized_edge_gradient_bwd": (
         c_int,

@@ -47,6 +47,13 @@ int dmsh_abi_version(void);
  * (synchronizes the device). */
 int dmsh_get_device_errors(uint32_t *bits_out);
 
+/* Process-wide launch tuning (no reference counterpart; set by the
+ * autotuner).  Keys: "pix2tri.stack" (LDS frames per lane: 16, 24, 32),
+ * "pix2tri.waves" (waves per workgroup: 1, 2, 4).  get returns -1 for an
+ * unknown key. */
+int dmsh_set_tuning(const char *key, int value);
+int dmsh_get_tuning(const char *key);
+
 /* ---------------------------------------------------------------------
  * Morton codes
  * Replaces kernel `vtx2morton` (del-msh-cuda-kernels/cuda/mortons.cu:7-36)
@@ -145,7 +152,8 @@ int dmsh_build_points(void *stream, uint32_t num_pnt, const float *vtx2xyz,
  * ray coefficient of the hit (0 on a miss); the ray runs from NDC z = +1
  * (t = 0) to NDC z = -1 (t = 1).
  * ------------------------------------------------------------------- */
-size_t dmsh_pix2tri_workspace_bytes(uint32_t img_w, uint32_t img_h);
+size_t dmsh_pix2tri_workspace_bytes(uint32_t num_tri, uint32_t img_w,
+                                    uint32_t img_h);
 int dmsh_pix2tri(void *stream, uint32_t *pix2tri, float *pix2t,
                  uint32_t num_tri, const uint32_t *tri2vtx, uint32_t num_vtx,
                  const float *vtx2xyz, uint32_t img_w, uint32_t img_h,
@@ -159,10 +167,12 @@ int dmsh_pix2tri(void *stream, uint32_t *pix2tri, float *pix2t,
  * inverse transform is computed on the device (no host round trip).
  * CPU counterpart del-msh-cpu/src/pix2depth.rs:107-146.
  * ------------------------------------------------------------------- */
+size_t dmsh_pix2depth_workspace_bytes(void);
 int dmsh_pix2depth(void *stream, float *pix2depth, const uint32_t *pix2tri,
                    uint32_t num_tri, const uint32_t *tri2vtx, uint32_t num_vtx,
                    const float *vtx2xyz, uint32_t img_w, uint32_t img_h,
-                   const float *transform_ndc2world);
+                   const float *transform_ndc2world, void *ws,
+                   size_t ws_bytes);
 
 /* ---------------------------------------------------------------------
  * Edge-gradient backward of the rasterized silhouette (accumulates into

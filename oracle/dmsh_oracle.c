@@ -632,6 +632,7 @@ typedef struct {
   const uint32_t *bn;
   const float *aabb;
   orc_trace_stats st;
+  int frames, max_frames; /* nesting of both-children-hit nodes on the path */
 } cast_ctx;
 
 typedef struct {
@@ -677,17 +678,21 @@ static int fir(cast_ctx *cx, size_t node, hit_rec *out) {
   const size_t nnear = near_is_left ? left : right;
   const size_t nfar = near_is_left ? right : left;
   hit_rec rn, rf;
+  const int both = hl && hr;
+  if (both && ++cx->frames > cx->max_frames) cx->max_frames = cx->frames;
   const int has_n = fir(cx, nnear, &rn);
   if (has_n) {
     float tf0, tf1;
     if (orc_ray_aabb(cx->aabb + nfar * 6, cx->org, cx->dir, &tf0, &tf1)) {
-      if (tf0 > rn.t) { *out = rn; return 1; }
+      if (tf0 > rn.t) { cx->frames -= both; *out = rn; return 1; }
     } else {
+      cx->frames -= both;
       *out = rn;
       return 1;
     }
   }
   const int has_f = fir(cx, nfar, &rf);
+  cx->frames -= both;
   if (has_n && has_f) { *out = (rn.t < rf.t) ? rn : rf; return 1; }
   if (has_n) { *out = rn; return 1; }
   if (has_f) { *out = rf; return 1; }
@@ -700,12 +705,20 @@ int orc_first_intersection_ray(const float org[3], const float dir[3],
                                const float *aabb, float *t, uint32_t *i_tri,
                                float bc[3], orc_trace_stats *stats) {
   if (num_node == 0) return 0;
-  cast_ctx cx = {org, dir, tri2vtx, vtx2xyz, bvhnodes, aabb, {0, 0}};
+  cast_ctx cx;
+  memset(&cx, 0, sizeof(cx));
+  cx.org = org;
+  cx.dir = dir;
+  cx.tri2vtx = tri2vtx;
+  cx.xyz = vtx2xyz;
+  cx.bn = bvhnodes;
+  cx.aabb = aabb;
   hit_rec h;
   const int hit = fir(&cx, 0, &h);
   if (stats) {
     stats->nodes_visited += cx.st.nodes_visited;
     stats->tris_tested += cx.st.tris_tested;
+    stats->frames_hist[cx.max_frames < 63 ? cx.max_frames : 63] += 1;
   }
   if (!hit) return 0;
   *t = h.t;
@@ -722,10 +735,11 @@ int orc_pix2tri(uint32_t *pix2tri, float *pix2t, size_t num_tri,
                 orc_trace_stats *stats) {
   const size_t num_node = num_tri ? 2 * num_tri - 1 : 0;
   const int64_t num_pix = (int64_t)img_w * (int64_t)img_h;
-  uint64_t nv = 0, nt = 0;
+  uint64_t nv = 0, nt = 0, hist[64];
+  memset(hist, 0, sizeof(hist));
 #ifdef _OPENMP
   const int nth = num_threads > 0 ? num_threads : 1;
-#pragma omp parallel for schedule(dynamic, 256) num_threads(nth) reduction(+ : nv, nt)
+#pragma omp parallel for schedule(dynamic, 256) num_threads(nth) reduction(+ : nv, nt, hist[:64])
 #endif
   for (int64_t ip = 0; ip < num_pix; ++ip) {
     const uint32_t ih = (uint32_t)(ip / img_w);
@@ -733,7 +747,8 @@ int orc_pix2tri(uint32_t *pix2tri, float *pix2t, size_t num_tri,
     float org[3], dir[3], t = 0.f;
     uint32_t tri = U32MAX;
     orc_ray_for_pixel(iw, ih, img_w, img_h, T, org, dir);
-    orc_trace_stats st = {0, 0};
+    orc_trace_stats st;
+    memset(&st, 0, sizeof(st));
     if (!orc_first_intersection_ray(org, dir, tri2vtx, vtx2xyz, bvhnodes, num_node,
                                     aabb, &t, &tri, NULL, &st)) {
       tri = U32MAX;
@@ -741,6 +756,7 @@ int orc_pix2tri(uint32_t *pix2tri, float *pix2t, size_t num_tri,
     }
     nv += st.nodes_visited;
     nt += st.tris_tested;
+    for (int k = 0; k < 64; ++k) hist[k] += st.frames_hist[k];
     pix2tri[ip] = tri;
     if (pix2t) pix2t[ip] = t;
   }
@@ -748,6 +764,7 @@ int orc_pix2tri(uint32_t *pix2tri, float *pix2t, size_t num_tri,
   if (stats) {
     stats->nodes_visited += nv;
     stats->tris_tested += nt;
+    for (int k = 0; k < 64; ++k) stats->frames_hist[k] += hist[k];
   }
   return 0;
 }

@@ -103,6 +103,9 @@ void orc_fit_into_unit_cube(size_t num_pnt, const float *vtx2xyz, float out[16])
 typedef struct {
   uint64_t nodes_visited; /* internal nodes expanded (both child boxes tested) */
   uint64_t tris_tested;   /* leaves reached (ray-triangle tests performed)     */
+  uint64_t frames_hist[64]; /* rays by the deepest nesting of nodes whose two
+                               children were both hit (clamped to 63): the
+                               frame-stack depth an iterative traversal needs */
 } orc_trace_stats;
 
 /* search_bvh3.rs:103-221 from the root with dis = +inf.

@@ -22,7 +22,8 @@ _sz = ctypes.c_size_t
 
 
 class TraceStats(ctypes.Structure):
-    _fields_ = [("nodes_visited", ctypes.c_uint64), ("tris_tested", ctypes.c_uint64)]
+    _fields_ = [("nodes_visited", ctypes.c_uint64), ("tris_tested", ctypes.c_uint64),
+                ("frames_hist", ctypes.c_uint64 * 64)]
 
 
 def lib():
@@ -216,7 +217,7 @@ def pix2tri(tri2vtx, vtx2xyz, bvhnodes, bvhnode2aabb, ndc2world_colmajor, img_w,
                       _f32(bvhnode2aabb).reshape(-1), img_w, img_h, _f32(ndc2world_colmajor),
                       num_threads, ctypes.byref(st))
     res = out.reshape(img_h, img_w)
-    stats = (int(st.nodes_visited), int(st.tris_tested))
+    stats = (int(st.nodes_visited), int(st.tris_tested), np.array(st.frames_hist[:], np.int64))
     if with_t:
         return res, tt.reshape(img_h, img_w), stats
     return res, None, stats

@@ -48,7 +48,11 @@ def test_host_side_validation_without_gpu():
     assert L.dmsh_status_string(3) == b"workspace too small"
     assert L.dmsh_argsort_workspace_bytes(1000, 4) > 8 * 1000
     assert L.dmsh_build_workspace_bytes(1 << 20, 63) > L.dmsh_build_workspace_bytes(1 << 20, 30)
-    assert L.dmsh_pix2tri_workspace_bytes(64, 64) >= 4 * 64 * 64
+    assert L.dmsh_pix2tri_workspace_bytes(1000, 64, 64) >= 4 * 64 * 64 + 64 * 999 + 48 * 1000
+    assert L.dmsh_set_tuning(b"pix2tri.stack", 24) == 0
+    assert L.dmsh_get_tuning(b"pix2tri.stack") == 24
+    assert L.dmsh_set_tuning(b"pix2tri.stack", 7) == 1
+    assert L.dmsh_get_tuning(b"no.such.key") == -1
     # invalid arguments are rejected before anything is enqueued
     assert L.dmsh_vtx2morton_u32(None, 10, None, 3, None, None) == 1
     assert L.dmsh_build_trimesh3(None, 10, None, 10, None, 31, None, None, None, 0) == 1
