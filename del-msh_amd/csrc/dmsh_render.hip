@@ -164,7 +164,7 @@ int dmsh_pix2depth(void* stream, float* pix2depth, const uint32_t* pix2tri, uint
   if (!err) return DMSH_ERR_HIP;
   hipStream_t s = as_stream(stream);
   float* inv = static_cast<float*>(ws);
-  hipLaunchKernelGGL(rc::k_mat4_inverse, dim3(1), dim3(1), 0, s, transform_ndc2world, inv, err);
+  hipLaunchKernelGGL(k_mat4_inverse, dim3(1), dim3(1), 0, s, transform_ndc2world, inv, err);
   int st = launch_status();
   if (st != DMSH_OK) return st;
   rc::TriMesh sc{tri2vtx, vtx2xyz, num_tri, num_vtx};

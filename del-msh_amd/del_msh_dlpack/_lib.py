@@ -70,6 +70,23 @@ _PROTOTYPES = {
         [c_void_p, c_void_p, c_u32, c_void_p, c_u32, c_void_p, c_void_p, c_u32, c_u32, c_void_p,
          c_u32, c_void_p, c_void_p],
     ),
+    "dmsh_contour_workspace_bytes": (c_size, [c_u32]),
+    "dmsh_contour_for_triangle_mesh": (
+        c_int,
+        [c_void_p, c_u32, c_void_p, c_u32, c_void_p, c_void_p, c_u32, c_void_p, c_void_p,
+         c_void_p, c_void_p, c_void_p, c_size],
+    ),
+    "dmsh_antialias_workspace_bytes": (c_size, [c_u32, c_u32]),
+    "dmsh_antialias_fwd": (
+        c_int,
+        [c_void_p, c_u32, c_void_p, c_u32, c_void_p, c_void_p, c_u32, c_u32, c_void_p, c_void_p,
+         c_void_p, c_void_p, c_size],
+    ),
+    "dmsh_antialias_bwd": (
+        c_int,
+        [c_void_p, c_u32, c_void_p, c_u32, c_void_p, c_void_p, c_void_p, c_u32, c_u32, c_void_p,
+         c_void_p, c_void_p],
+    ),
 }
 
 
@@ -117,6 +134,7 @@ DEVERR_NAMES = {
     0x4: "radix-sort look-back gave up",
     0x8: "broken tree in refit",
     0x10: "singular transform",
+    0x20: "antialias pixel walk longer than 2^26 steps",
 }
 
 

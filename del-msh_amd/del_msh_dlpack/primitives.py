@@ -137,3 +137,25 @@ def octahedral_sphere(level: int, radius: float = 1.0, noise_sigma: float = 0.0,
     flip = np.einsum("ij,ij->i", np.cross(b - a, c - a), a + b + c) < 0
     tri[flip] = tri[flip][:, [0, 2, 1]]
     return tri.astype(np.uint32), p.astype(np.float32)
+
+
+def edge2vtx_edge2tri(tri2vtx: np.ndarray):
+    """Undirected edges (i < j, sorted) of a closed triangle mesh and the two
+    triangles sharing each edge (inputs of the contour extraction)."""
+    t = tri2vtx.astype(np.int64)
+    ntri = t.shape[0]
+    e = np.concatenate([t[:, [0, 1]], t[:, [1, 2]], t[:, [2, 0]]])
+    owner = np.concatenate([np.arange(ntri)] * 3)
+    e.sort(axis=1)
+    key = e[:, 0] * (int(t.max()) + 1) + e[:, 1]
+    order = np.argsort(key, kind="stable")
+    key_s = key[order]
+    first = np.ones(len(key_s), bool)
+    first[1:] = key_s[1:] != key_s[:-1]
+    starts = np.nonzero(first)[0]
+    counts = np.diff(np.append(starts, len(key_s)))
+    if not np.all(counts == 2):
+        raise ValueError("mesh is not a closed 2-manifold")
+    edge2vtx = e[order[starts]].astype(np.uint32)
+    edge2tri = np.stack([owner[order[starts]], owner[order[starts + 1]]], 1).astype(np.uint32)
+    return edge2vtx, edge2tri

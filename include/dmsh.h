@@ -39,6 +39,7 @@ extern "C" {
 #define DMSH_DEVERR_SORT_PROGRESS 0x4u  /* radix-sort look-back gave up       */
 #define DMSH_DEVERR_REFIT 0x8u          /* refit climb found a broken tree    */
 #define DMSH_DEVERR_SINGULAR 0x10u      /* transform could not be inverted    */
+#define DMSH_DEVERR_WALK 0x20u          /* antialias pixel walk > 2^26 steps  */
 
 const char *dmsh_status_string(int status);
 int dmsh_abi_version(void);
@@ -191,6 +192,62 @@ int dmsh_rasterized_edge_gradient_bwd(void *stream, const uint32_t *tri2vtx,
                                       const uint32_t *pix2tri,
                                       uint32_t num_vdim, const float *pix2val,
                                       const float *dldw_pix2val);
+
+/* ---------------------------------------------------------------------
+ * Contour edges: the edges whose two triangles face opposite ways along the
+ * view ray through the edge midpoint, in edge order.  Replaces kernel
+ * `edge2vtx_contour_set_flag` (del-msh-cuda-kernels/cuda/edge2vtx.cu:28-75)
+ * plus the flag compaction, launched by edge2vtx_contour_for_triangle_mesh
+ * (del-msh-dlpack/src/edge2vtx.rs:19-115); CPU counterpart
+ * del-msh-cpu/src/edge2vtx.rs:191-242.  The inverse of transform_world2ndc
+ * (16 floats column-major) is computed on the device.  cedge2vtx needs room
+ * for num_edge pairs; the number written lands in *num_cedge (device
+ * memory), so reading it is the caller's one sync.
+ * ------------------------------------------------------------------- */
+size_t dmsh_contour_workspace_bytes(uint32_t num_edge);
+int dmsh_contour_for_triangle_mesh(void *stream, uint32_t num_tri,
+                                   const uint32_t *tri2vtx, uint32_t num_vtx,
+                                   const float *vtx2xyz,
+                                   const float *transform_world2ndc,
+                                   uint32_t num_edge, const uint32_t *edge2vtx,
+                                   const uint32_t *edge2tri, uint32_t *cedge2vtx,
+                                   uint32_t *num_cedge, void *ws,
+                                   size_t ws_bytes);
+
+/* ---------------------------------------------------------------------
+ * Anti-aliased silhouette, forward: blends pix2vin across every contour
+ * edge into pix2vout, which the caller has initialised (normally a copy of
+ * pix2vin); a pixel written by several edges keeps the value of the last
+ * write in serial (edge, step, pair) order, so the result is deterministic.
+ * Replaces kernel `antialias_fwd`
+ * (del-msh-cuda-kernels/cuda/differentiable_rasterizer.cu:69-123) launched
+ * by differentiable_antialias_fwd
+ * (del-msh-dlpack/src/differentiable_antialias.rs:103-182); CPU counterpart
+ * del-msh-cpu/src/antialias.rs:54-99.  transform_world2pix: 16 floats
+ * column-major.
+ * ------------------------------------------------------------------- */
+size_t dmsh_antialias_workspace_bytes(uint32_t img_w, uint32_t img_h);
+int dmsh_antialias_fwd(void *stream, uint32_t num_cedge,
+                       const uint32_t *cedge2vtx, uint32_t num_vtx,
+                       const float *vtx2xyz, const float *transform_world2pix,
+                       uint32_t img_w, uint32_t img_h, const uint32_t *pix2tri,
+                       const float *pix2vin, float *pix2vout, void *ws,
+                       size_t ws_bytes);
+
+/* ---------------------------------------------------------------------
+ * Anti-aliased silhouette, backward (accumulates into dldw_vtx2xyz).
+ * Replaces kernel `antialias_bwd`
+ * (del-msh-cuda-kernels/cuda/differentiable_rasterizer.cu:126-196) launched
+ * by differentiable_antialias_bwd
+ * (del-msh-dlpack/src/differentiable_antialias.rs:16-101); CPU counterpart
+ * del-msh-cpu/src/antialias.rs:102-159.
+ * ------------------------------------------------------------------- */
+int dmsh_antialias_bwd(void *stream, uint32_t num_cedge,
+                       const uint32_t *cedge2vtx, uint32_t num_vtx,
+                       const float *vtx2xyz, float *dldw_vtx2xyz,
+                       const float *transform_world2pix, uint32_t img_w,
+                       uint32_t img_h, const float *pix2val,
+                       const float *dldw_pix2val, const uint32_t *pix2tri);
 
 #ifdef __cplusplus
 }

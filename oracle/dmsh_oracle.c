@@ -912,3 +912,259 @@ void orc_edge_gradient_bwd(const uint32_t *tri2vtx, const float *xyz,
     }
   }
 }
+
+/* ======================================================================= */
+/* contour and anti-aliasing                                                */
+/* ======================================================================= */
+
+/* tri3::unit_normal: cross(p1 - p0, p2 - p0) scaled by 1 / |.| */
+static void tri_unit_normal(const float *p0, const float *p1, const float *p2, float n[3]) {
+  float e1[3], e2[3];
+  for (int i = 0; i < 3; ++i) {
+    e1[i] = p1[i] - p0[i];
+    e2[i] = p2[i] - p0[i];
+  }
+  cross3(e1, e2, n);
+  const float len = sqrtf(dot3(n, n));
+  const float inv = 1.f / len;
+  for (int i = 0; i < 3; ++i) n[i] = n[i] * inv;
+}
+
+/* mat4_col_major::ray_from_transform_world2ndc: the view ray through a world
+ * point -- project it to NDC, then take the NDC segment z = +1 -> -1 at that
+ * (x, y) back to world space. */
+static void ray_through_point(const float w2n[16], const float n2w[16], const float pos[3],
+                              float dir[3]) {
+  float ndc[3], q0[3], q1[3];
+  if (!orc_transform_homogeneous(w2n, pos, ndc)) ndc[0] = ndc[1] = ndc[2] = NAN;
+  const float a[3] = {ndc[0], ndc[1], 1.f}, b[3] = {ndc[0], ndc[1], -1.f};
+  if (!orc_transform_homogeneous(n2w, a, q0)) q0[0] = q0[1] = q0[2] = NAN;
+  if (!orc_transform_homogeneous(n2w, b, q1)) q1[0] = q1[1] = q1[2] = NAN;
+  for (int i = 0; i < 3; ++i) dir[i] = q1[i] - q0[i];
+}
+
+/* edge2vtx.rs:191-242 */
+long orc_contour_for_triangle_mesh(const uint32_t *tri2vtx, const float *xyz,
+                                   const float w2n[16], size_t num_edge,
+                                   const uint32_t *edge2vtx, const uint32_t *edge2tri,
+                                   uint32_t *cedge2vtx) {
+  float n2w[16];
+  if (!orc_mat4_inverse(w2n, n2w)) return -1;
+  long cnt = 0;
+  for (size_t e = 0; e < num_edge; ++e) {
+    const uint32_t i0 = edge2vtx[e * 2], i1 = edge2vtx[e * 2 + 1];
+    float mid[3], dir[3], n0[3], n1[3];
+    for (int i = 0; i < 3; ++i) mid[i] = (xyz[(size_t)i0 * 3 + i] + xyz[(size_t)i1 * 3 + i]) * 0.5f;
+    ray_through_point(w2n, n2w, mid, dir);
+    const uint32_t *ta = tri2vtx + (size_t)edge2tri[e * 2] * 3;
+    const uint32_t *tb = tri2vtx + (size_t)edge2tri[e * 2 + 1] * 3;
+    tri_unit_normal(xyz + (size_t)ta[0] * 3, xyz + (size_t)ta[1] * 3, xyz + (size_t)ta[2] * 3, n0);
+    tri_unit_normal(xyz + (size_t)tb[0] * 3, xyz + (size_t)tb[1] * 3, xyz + (size_t)tb[2] * 3, n1);
+    const int f0 = dot3(n0, dir) > 0.f;
+    const int f1 = dot3(n1, dir) > 0.f;
+    if (f0 == f1) continue;
+    cedge2vtx[cnt * 2] = i0;
+    cedge2vtx[cnt * 2 + 1] = i1;
+    ++cnt;
+  }
+  return cnt;
+}
+
+static inline float cross2(float ax, float ay, float bx, float by) { return ax * by - ay * bx; }
+
+/* parametric segment-segment intersection: rc = (a x s) / (r x s),
+ * re = (a x r) / (r x s) with r = c1 - c0, s = e1 - s0, a = s0 - c0 */
+int orc_intersection_edge2(const float c0[2], const float c1[2], const float s0[2],
+                           const float e1[2], float *rc, float *re) {
+  const float rx = c1[0] - c0[0], ry = c1[1] - c0[1];
+  const float sx = e1[0] - s0[0], sy = e1[1] - s0[1];
+  const float d = cross2(rx, ry, sx, sy);
+  if (d == 0.f) return 0;
+  const float ax = s0[0] - c0[0], ay = s0[1] - c0[1];
+  const float r0 = cross2(ax, ay, sx, sy) / d;
+  const float r1 = cross2(ax, ay, rx, ry) / d;
+  if (r0 < 0.f || r0 > 1.f || r1 < 0.f || r1 > 1.f) return 0;
+  *rc = r0;
+  *re = r1;
+  return 1;
+}
+
+/* d rc / d s0 = (dN/ds0 - rc dD/ds0) / D, d rc / d e1 likewise, with
+ * N = a x s, D = r x s (derived from the formula above) */
+void orc_dldw_intersection_edge2(const float c0[2], const float c1[2], const float s0[2],
+                                 const float e1[2], float dldrc, float dlds0[2],
+                                 float dlde1[2]) {
+  const float rx = c1[0] - c0[0], ry = c1[1] - c0[1];
+  const float sx = e1[0] - s0[0], sy = e1[1] - s0[1];
+  const float ax = s0[0] - c0[0], ay = s0[1] - c0[1];
+  const float d = cross2(rx, ry, sx, sy);
+  const float rc = cross2(ax, ay, sx, sy) / d;
+  const float g = dldrc / d;
+  dlds0[0] = ((sy + ay) - rc * ry) * g;
+  dlds0[1] = ((-sx - ax) + rc * rx) * g;
+  dlde1[0] = (-ay + rc * ry) * g;
+  dlde1[1] = (ax - rc * rx) * g;
+}
+
+/* one (pix0, c0, pix1, c1) neighbour pair, differentiable_rasterizer.cu:15-64 */
+typedef struct {
+  uint32_t p0, p1;
+  float c0[2], c1[2];
+} aa_pair;
+
+static void aa_pairs(uint32_t ipix, uint32_t img_w, int horizontal, aa_pair out[4]) {
+  const uint32_t iw = ipix % img_w, ih = ipix / img_w;
+  const uint32_t pix[5] = {ih * img_w + iw, ih * img_w + iw - 1u, ih * img_w + iw + 1u,
+                           (ih - 1u) * img_w + iw, (ih + 1u) * img_w + iw};
+  const float pos[5][2] = {{(float)iw + 0.5f, (float)ih + 0.5f},
+                           {(float)iw - 0.5f, (float)ih + 0.5f},
+                           {(float)iw + 1.5f, (float)ih + 0.5f},
+                           {(float)iw + 0.5f, (float)ih - 0.5f},
+                           {(float)iw + 0.5f, (float)ih + 1.5f}};
+  static const int hz[4][2] = {{0, 1}, {1, 0}, {0, 2}, {2, 0}};
+  static const int vt[4][2] = {{0, 3}, {3, 0}, {0, 4}, {4, 0}};
+  for (int k = 0; k < 4; ++k) {
+    const int a = horizontal ? hz[k][0] : vt[k][0], b = horizontal ? hz[k][1] : vt[k][1];
+    out[k].p0 = pix[a];
+    out[k].p1 = pix[b];
+    out[k].c0[0] = pos[a][0];
+    out[k].c0[1] = pos[a][1];
+    out[k].c1[0] = pos[b][0];
+    out[k].c1[1] = pos[b][1];
+  }
+}
+
+/* Walks the pixels of the projected edge q0 -> q1 as dda.h:6-63 does (one
+ * step past the end so the line stays connected) and calls fn on every
+ * foreground/background neighbour pair crossed by the edge.
+ * Two exits that change no result: the walk stops once it has left the image
+ * in its direction of travel (every later pixel is outside too), and edges
+ * whose projection is not finite are skipped (their walk would not end).
+ * A walk longer than ORC_WALK_LIMIT steps is abandoned; returns 0 then. */
+#define ORC_WALK_LIMIT (1u << 26)
+typedef void (*aa_visit_fn)(void *ctx, const aa_pair *pr, float rc);
+
+static int aa_walk(const float q0[2], const float q1[2], uint32_t img_w, uint32_t img_h,
+                   const uint32_t *pix2tri, aa_visit_fn fn, void *ctx) {
+  if (!(isfinite(q0[0]) && isfinite(q0[1]) && isfinite(q1[0]) && isfinite(q1[1]))) return 1;
+  const float v01x = q1[0] - q0[0], v01y = q1[1] - q0[1];
+  if (v01x == 0.f && v01y == 0.f) return 1;
+  const int horizontal = fabsf(v01x) < fabsf(v01y);
+  float x = q0[0], y = q0[1];
+  const float dx = q1[0] - q0[0], dy = q1[1] - q0[1];
+  const float step = fmax_sel(fabsf(dx), fabsf(dy));
+  const float slope_x = dx / step, slope_y = dy / step;
+  const uint32_t total = img_w * img_h;
+  const float fw = (float)img_w, fh = (float)img_h;
+  for (uint32_t k = 0; k < ORC_WALK_LIMIT; ++k) {
+    const float fx = floorf(x), fy = floorf(y);
+    if (fx >= 0.f && fy >= 0.f && fx < fw && fy < fh) {
+      aa_pair prs[4];
+      aa_pairs((uint32_t)fy * img_w + (uint32_t)fx, img_w, horizontal, prs);
+      for (int k = 0; k < 4; ++k) {
+        const aa_pair *pr = &prs[k];
+        if (pr->p0 >= total || pr->p1 >= total) continue;
+        if (pix2tri[pr->p0] == U32MAX || pix2tri[pr->p1] != U32MAX) continue;
+        float rc, re;
+        if (!orc_intersection_edge2(pr->c0, pr->c1, q1, q0, &rc, &re)) continue;
+        fn(ctx, pr, rc);
+      }
+    }
+    const float eps = 1.0e-3f;
+    if (fabsf(x - q0[0]) > fabsf(q1[0] - q0[0]) + eps) return 1;
+    if (fabsf(y - q0[1]) > fabsf(q1[1] - q0[1]) + eps) return 1;
+    if ((slope_x > 0.f && x >= fw) || (slope_x < 0.f && x < 0.f) || (slope_y > 0.f && y >= fh) ||
+        (slope_y < 0.f && y < 0.f))
+      return 1;
+    x = x + slope_x;
+    y = y + slope_y;
+  }
+  return 0;
+}
+
+typedef struct {
+  const float *vin;
+  float *vout;
+} aa_fwd_ctx;
+
+static void aa_fwd_visit(void *vctx, const aa_pair *pr, float rc) {
+  aa_fwd_ctx *c = (aa_fwd_ctx *)vctx;
+  if (rc < 0.5f) {
+    const float r0 = 0.5f + rc;
+    c->vout[pr->p0] = r0 * c->vin[pr->p0] + (1.f - r0) * c->vin[pr->p1];
+  } else {
+    const float r0 = rc - 0.5f;
+    c->vout[pr->p1] = r0 * c->vin[pr->p0] + (1.f - r0) * c->vin[pr->p1];
+  }
+}
+
+/* antialias.rs:54-99 */
+long orc_antialias(size_t num_cedge, const uint32_t *cedge2vtx, const float *xyz,
+                   const float m[16], uint32_t img_w, uint32_t img_h, const uint32_t *pix2tri,
+                   const float *pix2vin, float *pix2vout) {
+  aa_fwd_ctx c = {pix2vin, pix2vout};
+  long abandoned = 0;
+  for (size_t e = 0; e < num_cedge; ++e) {
+    float q0[2], q1[2];
+    project_xy(m, xyz + (size_t)cedge2vtx[e * 2] * 3, q0);
+    project_xy(m, xyz + (size_t)cedge2vtx[e * 2 + 1] * 3, q1);
+    abandoned += !aa_walk(q0, q1, img_w, img_h, pix2tri, aa_fwd_visit, &c);
+  }
+  return abandoned;
+}
+
+/* 2x3 xy-rows of the Jacobian of transform_homogeneous at p
+ * (mat4_col_major::jacobian_transform + to_mat2x3_col_major_xy):
+ * J[r][j] = (m[j*4+r] w - q_r m[j*4+3]) / w^2 */
+static void proj_jacobian_xy(const float m[16], const float *p, float J[2][3]) {
+  const float x = p[0], y = p[1], z = p[2];
+  const float w = ((m[3] * x + m[7] * y) + m[11] * z) + m[15];
+  const float inv_w2 = 1.f / (w * w);
+  for (int r = 0; r < 2; ++r) {
+    const float q = ((m[r] * x + m[4 + r] * y) + m[8 + r] * z) + m[12 + r];
+    for (int j = 0; j < 3; ++j) J[r][j] = (m[j * 4 + r] * w - q * m[j * 4 + 3]) * inv_w2;
+  }
+}
+
+typedef struct {
+  const float *val, *dldw;
+  const float *q0, *q1;
+  float J0[2][3], J1[2][3];
+  float *g0, *g1; /* dL/dp0, dL/dp1 accumulators (the vertex rows) */
+} aa_bwd_ctx;
+
+static void aa_bwd_visit(void *vctx, const aa_pair *pr, float rc) {
+  aa_bwd_ctx *c = (aa_bwd_ctx *)vctx;
+  const float dv = c->val[pr->p0] - c->val[pr->p1];
+  const float dldr0 = rc < 0.5f ? c->dldw[pr->p0] * dv : c->dldw[pr->p1] * dv;
+  float dldq1[2], dldq0[2];
+  orc_dldw_intersection_edge2(pr->c0, pr->c1, c->q1, c->q0, dldr0, dldq1, dldq0);
+  for (int j = 0; j < 3; ++j) {
+    c->g0[j] += c->J0[0][j] * dldq0[0] + c->J0[1][j] * dldq0[1];
+    c->g1[j] += c->J1[0][j] * dldq1[0] + c->J1[1][j] * dldq1[1];
+  }
+}
+
+/* antialias.rs:102-159 */
+long orc_bwd_antialias(size_t num_cedge, const uint32_t *cedge2vtx, const float *xyz,
+                       float *dldw_vtx2xyz, const float m[16], uint32_t img_w, uint32_t img_h,
+                       const float *pix2val, const float *dldw_pix2val, const uint32_t *pix2tri) {
+  long abandoned = 0;
+  for (size_t e = 0; e < num_cedge; ++e) {
+    const uint32_t i0 = cedge2vtx[e * 2], i1 = cedge2vtx[e * 2 + 1];
+    float q0[2], q1[2];
+    project_xy(m, xyz + (size_t)i0 * 3, q0);
+    project_xy(m, xyz + (size_t)i1 * 3, q1);
+    aa_bwd_ctx c;
+    c.val = pix2val;
+    c.dldw = dldw_pix2val;
+    c.q0 = q0;
+    c.q1 = q1;
+    proj_jacobian_xy(m, xyz + (size_t)i0 * 3, c.J0);
+    proj_jacobian_xy(m, xyz + (size_t)i1 * 3, c.J1);
+    c.g0 = dldw_vtx2xyz + (size_t)i0 * 3;
+    c.g1 = dldw_vtx2xyz + (size_t)i1 * 3;
+    abandoned += !aa_walk(q0, q1, img_w, img_h, pix2tri, aa_bwd_visit, &c);
+  }
+  return abandoned;
+}

@@ -128,6 +128,35 @@ int orc_pix2depth(float *pix2depth, const uint32_t *pix2tri,
                   const uint32_t *tri2vtx, const float *vtx2xyz,
                   uint32_t img_w, uint32_t img_h, const float ndc2world[16]);
 
+/* ---- contour and anti-aliasing (edge2vtx.rs, antialias.rs) ------------- */
+/* edge2vtx.rs:191-242: edges whose two triangles face opposite ways along
+ * the view ray through the edge midpoint.  Writes the selected edges' vertex
+ * pairs in edge order into cedge2vtx (capacity num_edge) and returns their
+ * count, or -1 when world2ndc is singular. */
+long orc_contour_for_triangle_mesh(const uint32_t *tri2vtx, const float *vtx2xyz,
+                                   const float world2ndc[16], size_t num_edge,
+                                   const uint32_t *edge2vtx, const uint32_t *edge2tri,
+                                   uint32_t *cedge2vtx);
+/* edge2::intersection_edge2: c0 + rc (c1 - c0) = s0 + re (e1 - s0); returns 1
+ * with (rc, re) when both lie in [0, 1] */
+int orc_intersection_edge2(const float c0[2], const float c1[2], const float s0[2],
+                           const float e1[2], float *rc, float *re);
+/* edge2::dldw_intersection_edge2 (dldre = 0): dL/ds0 and dL/de1 of dL/drc */
+void orc_dldw_intersection_edge2(const float c0[2], const float c1[2], const float s0[2],
+                                 const float e1[2], float dldrc, float dlds0[2],
+                                 float dlde1[2]);
+/* antialias.rs:54-99 (pixel walk of dda.h:6-63, neighbour pairs of
+ * differentiable_rasterizer.cu:15-64); pix2vout must start as a copy of pix2vin */
+long orc_antialias(size_t num_cedge, const uint32_t *cedge2vtx, const float *vtx2xyz,
+                   const float world2pix[16], uint32_t img_w, uint32_t img_h,
+                   const uint32_t *pix2tri, const float *pix2vin, float *pix2vout);
+/* antialias.rs:102-159 (accumulates into dldw_vtx2xyz).  Both return the
+ * number of edges whose pixel walk was abandoned after 2^26 steps. */
+long orc_bwd_antialias(size_t num_cedge, const uint32_t *cedge2vtx, const float *vtx2xyz,
+                        float *dldw_vtx2xyz, const float world2pix[16], uint32_t img_w,
+                        uint32_t img_h, const float *pix2val, const float *dldw_pix2val,
+                       const uint32_t *pix2tri);
+
 /* ---- differentiable rasterizer ----------------------------------------- */
 /* rasterized_edge_gradient.rs:434-603 (accumulates into dldw_vtx2xyz) */
 void orc_edge_gradient_bwd(const uint32_t *tri2vtx, const float *vtx2xyz,

@@ -278,3 +278,78 @@ def edge_gradient_bwd(tri2vtx, vtx2xyz, world2pix_colmajor, pix2tri_, pix2val, d
                                 _f32(world2pix_colmajor), w, h, _u32(pix2tri_).reshape(-1), nv,
                                 _f32(pix2val).reshape(-1), _f32(dldw_pix2val).reshape(-1))
     return out.reshape(-1, 3)
+
+
+# --- contour / anti-aliasing -----------------------------------------------------
+def _sig_aa():
+    L = lib()
+    if getattr(L, "_aa_ready", False):
+        return L
+    L.orc_contour_for_triangle_mesh.restype = ctypes.c_long
+    L.orc_contour_for_triangle_mesh.argtypes = [_u32p, _f32p, _f32p, _sz, _u32p, _u32p, _u32p]
+    L.orc_intersection_edge2.restype = ctypes.c_int
+    L.orc_intersection_edge2.argtypes = [_f32p, _f32p, _f32p, _f32p, ctypes.POINTER(ctypes.c_float),
+                                         ctypes.POINTER(ctypes.c_float)]
+    L.orc_dldw_intersection_edge2.restype = None
+    L.orc_dldw_intersection_edge2.argtypes = [_f32p, _f32p, _f32p, _f32p, ctypes.c_float, _f32p, _f32p]
+    L.orc_antialias.restype = ctypes.c_long
+    L.orc_antialias.argtypes = [_sz, _u32p, _f32p, _f32p, ctypes.c_uint32, ctypes.c_uint32, _u32p,
+                                _f32p, _f32p]
+    L.orc_bwd_antialias.restype = ctypes.c_long
+    L.orc_bwd_antialias.argtypes = [_sz, _u32p, _f32p, _f32p, _f32p, ctypes.c_uint32, ctypes.c_uint32,
+                                    _f32p, _f32p, _u32p]
+    L._aa_ready = True
+    return L
+
+
+def contour(tri2vtx, vtx2xyz, world2ndc_colmajor, edge2vtx, edge2tri):
+    L = _sig_aa()
+    e2v = _u32(edge2vtx)
+    out = np.zeros(e2v.size, np.uint32)
+    cnt = L.orc_contour_for_triangle_mesh(_u32(tri2vtx).reshape(-1), _f32(vtx2xyz).reshape(-1),
+                                          _f32(world2ndc_colmajor), e2v.shape[0], e2v.reshape(-1),
+                                          _u32(edge2tri).reshape(-1), out)
+    assert cnt >= 0
+    return out[: 2 * cnt].reshape(-1, 2)
+
+
+def intersection_edge2(c0, c1, s0, e1):
+    L = _sig_aa()
+    rc = ctypes.c_float(0)
+    re = ctypes.c_float(0)
+    ok = L.orc_intersection_edge2(_f32(c0), _f32(c1), _f32(s0), _f32(e1), ctypes.byref(rc), ctypes.byref(re))
+    return (float(rc.value), float(re.value)) if ok else None
+
+
+def dldw_intersection_edge2(c0, c1, s0, e1, dldrc):
+    L = _sig_aa()
+    a = np.zeros(2, np.float32)
+    b = np.zeros(2, np.float32)
+    L.orc_dldw_intersection_edge2(_f32(c0), _f32(c1), _f32(s0), _f32(e1), dldrc, a, b)
+    return a, b
+
+
+def antialias(cedge2vtx, vtx2xyz, world2pix_colmajor, pix2tri_, pix2vin):
+    L = _sig_aa()
+    h, w = pix2vin.shape
+    out = _f32(pix2vin).copy()
+    c = _u32(cedge2vtx)
+    n = L.orc_antialias(c.shape[0], c.reshape(-1), _f32(vtx2xyz).reshape(-1),
+                        _f32(world2pix_colmajor), w, h, _u32(pix2tri_).reshape(-1),
+                        _f32(pix2vin).reshape(-1), out.reshape(-1))
+    if n:
+        raise RuntimeError(f"{n} antialias pixel walks exceeded 2^26 steps")
+    return out
+
+
+def bwd_antialias(cedge2vtx, vtx2xyz, world2pix_colmajor, pix2val, dldw_pix2val, pix2tri_):
+    L = _sig_aa()
+    h, w = pix2val.shape
+    g = np.zeros(_f32(vtx2xyz).size, np.float32)
+    c = _u32(cedge2vtx)
+    n = L.orc_bwd_antialias(c.shape[0], c.reshape(-1), _f32(vtx2xyz).reshape(-1), g,
+                            _f32(world2pix_colmajor), w, h, _f32(pix2val).reshape(-1),
+                            _f32(dldw_pix2val).reshape(-1), _u32(pix2tri_).reshape(-1))
+    if n:
+        raise RuntimeError(f"{n} antialias pixel walks exceeded 2^26 steps")
+    return g.reshape(-1, 3)
