@@ -5,17 +5,24 @@ with exactly 2^21 triangles (level 9), vertices displaced radially by seeded
 N(0, 0.01^2) noise; LBVH build with 63-bit Morton codes (centroid, unit-cube
 fit, Morton, stable radix argsort, Karras topology, refit); perspective
 ray-cast of 2048 x 2048 rays (eye (0,0,3), target origin, fovy 45, near 1,
-far 5) producing the hit triangle and hit coefficient per pixel.
+far 5; the near plane maps to NDC z = +1 where pixel rays start) producing
+the hit triangle and hit coefficient per pixel.
 
 One step = one build + one ray-cast.  Inputs are resident in HBM before the
 timed region.  `value` is the build throughput in Mtri/s; the ray-cast
 throughput and the separate 2^22-point build / sort-alone figures are
-reported beside it.  Prints one JSON line.
+reported beside it.  Per-kernel launch times come from the library's HIP
+event timing on the launch stream.  Prints one JSON line.
+
+Other modes: --pmc-workload (the short run a rocprofv3 counter pass
+profiles) and --summarize-pmc (turns two counter passes into the traffic
+summary under profiles/, which the default run reads back).
 """
 
 from __future__ import annotations
 
 import argparse
+import csv
 import json
 import math
 import os
@@ -37,7 +44,22 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--img", type=int, default=2048)
     ap.add_argument("--level", type=int, default=9)
+    ap.add_argument("--pmc-workload", action="store_true",
+                    help="run only what a counter pass needs: 3 ray-casts of the bench workload "
+                         "and 3 device-to-device copies of 512 MiB (the calibration kernel)")
+    ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
+                    help="turn two rocprofv3 --pmc output folders (FETCH_SIZE, WRITE_SIZE) "
+                         "into the traffic summary text")
     return ap.parse_args()
+
+
+# kernels whose launch time the bench reads back from the library's event timing
+TIMED_KERNELS = ["k_pix2tri", "k_pix2tri_tier2", "k_pix2tri_tier3", "k_pack_inner", "k_pack_leaf",
+                 "k_build_init", "k_centroid_aabb", "k_morton_hist", "k_hist_all", "k_onesweep",
+                 "k_topology", "k_refit_local", "k_refit_top", "k_fill_u32"]
+TRAFFIC_SUMMARY = ROOT / "profiles" / "pmc_traffic_k_pix2tri_octa9_2048.txt"
+CALIB_BYTES = 512 << 20
+COPY_KERNEL = "__amd_rocclr_copyBuffer"
 
 
 def perspective_ndc2world(eye=(0.0, 0.0, 3.0), fovy=45.0, near=1.0, far=5.0):
@@ -53,14 +75,88 @@ def perspective_ndc2world(eye=(0.0, 0.0, 3.0), fovy=45.0, near=1.0, far=5.0):
     view[0, :3], view[1, :3], view[2, :3] = s, u, -f
     view[:3, 3] = -view[:3, :3] @ eye
     g = 1.0 / math.tan(math.radians(fovy) * 0.5)
+    # near plane -> NDC z = +1 (where pixel rays start), far plane -> -1
     proj = np.array([[g, 0, 0, 0], [0, g, 0, 0],
-                     [0, 0, (far + near) / (near - far), 2 * far * near / (near - far)],
+                     [0, 0, -(far + near) / (near - far), -2 * far * near / (near - far)],
                      [0, 0, -1, 0]])
     return np.linalg.inv(proj @ view).astype(np.float32)
 
 
+# ---------------------------------------------------------------------------
+# counter passes
+# ---------------------------------------------------------------------------
+def _read_counter_csv(folder: str, counter: str):
+    """{kernel name: [value per dispatch]} from a rocprofv3 counter_collection
+    CSV (values in KiB for FETCH_SIZE / WRITE_SIZE)."""
+    files = sorted(pathlib.Path(folder).rglob("*counter_collection.csv"))
+    if not files:
+        raise SystemExit(f"no counter_collection.csv under {folder}")
+    per = {}
+    for f in files:
+        with open(f, newline="") as fh:
+            for row in csv.DictReader(fh):
+                if row.get("Counter_Name") != counter:
+                    continue
+                name = row.get("Kernel_Name", "")
+                per.setdefault(name, []).append(float(row["Counter_Value"]))
+    return per
+
+
+def _match(per, short):
+    for name, vals in per.items():
+        if short in name:
+            return vals
+    return None
+
+
+def summarize_pmc(fetch_dir: str, write_dir: str) -> str:
+    fetch = _read_counter_csv(fetch_dir, "FETCH_SIZE")
+    write = _read_counter_csv(write_dir, "WRITE_SIZE")
+    cf, cw = _match(fetch, COPY_KERNEL), _match(write, COPY_KERNEL)
+    kf, kw = _match(fetch, "k_pix2tri<"), _match(write, "k_pix2tri<")
+    if not (cf and cw and kf and kw):
+        raise SystemExit("counter passes lack the copy kernel or k_pix2tri")
+    calib_kib = CALIB_BYTES / 1024.0
+    # the copy reads and writes CALIB_BYTES once: the factor maps counter KiB to bytes
+    f_fac = CALIB_BYTES / (sum(cf) / len(cf) * 1024.0)
+    w_fac = CALIB_BYTES / (sum(cw) / len(cw) * 1024.0)
+    kf_avg = sum(kf) / len(kf)
+    kw_avg = sum(kw) / len(kw)
+    traffic = kf_avg * 1024.0 * f_fac + kw_avg * 1024.0 * w_fac
+    lines = [
+        "HBM traffic of k_pix2tri on the bench workload (octahedral sphere level 9, 2^21 tris,",
+        "63-bit LBVH, perspective 2048x2048), from two rocprofv3 counter passes:",
+        "  cd /tmp && export TMPDIR=/tmp",
+        "  rocprofv3 --pmc FETCH_SIZE --output-format csv -d <fetch> -- python3 bench.py --pmc-workload",
+        "  rocprofv3 --pmc WRITE_SIZE --output-format csv -d <write> -- python3 bench.py --pmc-workload",
+        "  python3 bench.py --summarize-pmc <fetch> <write>",
+        f"calibration: {COPY_KERNEL} of {CALIB_BYTES} B (read once, written once)",
+        f"  FETCH_SIZE {sum(cf) / len(cf):.1f} KiB (expected {calib_kib:.1f}) -> factor {f_fac:.4f}",
+        f"  WRITE_SIZE {sum(cw) / len(cw):.1f} KiB (expected {calib_kib:.1f}) -> factor {w_fac:.4f}",
+        f"k_pix2tri: {len(kf)} dispatches, FETCH_SIZE {kf_avg:.1f} KiB, WRITE_SIZE {kw_avg:.1f} KiB",
+        f"traffic_bytes_per_launch {traffic:.6e}",
+    ]
+    return "\n".join(lines) + "\n"
+
+
+def read_traffic():
+    try:
+        for line in TRAFFIC_SUMMARY.read_text().splitlines():
+            if line.startswith("traffic_bytes_per_launch"):
+                return float(line.split()[1])
+    except OSError:
+        return None
+    return None
+
+
+# ---------------------------------------------------------------------------
+# main
+# ---------------------------------------------------------------------------
 def main():
     args = parse()
+    if args.summarize_pmc:
+        sys.stdout.write(summarize_pmc(*args.summarize_pmc))
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
               "(the batched/served paths shard meshes per GPU, which this build does not run)",
@@ -105,6 +201,18 @@ def main():
                                   bvhnodes.data_ptr(), aabb.data_ptr(), ws_cast.data_ptr(),
                                   ws_cast.numel()), "pix2tri")
 
+    if args.pmc_workload:
+        build()
+        for _ in range(3):
+            cast()
+        src = torch.empty(CALIB_BYTES, dtype=torch.uint8, device=dev).fill_(1)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"pmc_workload": "done", "casts": 3, "copies": 3}))
+        return
+
     for _ in range(args.warmup):
         build()
         cast()
@@ -121,6 +229,20 @@ def main():
         e2.record(stream)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
+    # the same steps again with the library's per-kernel events on (the event
+    # records add gaps between kernels, so this pass is kept out of `value`)
+    _lib.timing_enable(True)
+    for _ in range(args.steps):
+        build()
+        cast()
+    torch.cuda.synchronize()
+    _lib.timing_enable(False)
+    kernels = {}
+    for name in TIMED_KERNELS:
+        n, total, mx = _lib.timing_query(name)
+        if n:
+            kernels[name] = {"launches_per_step": round(n / args.steps, 3),
+                             "avg_ms": round(total / n, 5), "max_ms": round(mx, 5)}
     build_ms = [a.elapsed_time(b) for a, b, _ in ev]
     cast_ms = [b.elapsed_time(c) for _, b, c in ev]
     errs = _lib.device_errors()
@@ -170,12 +292,7 @@ def main():
     timed(psort, 2, lambda: keys.copy_(keys0))
     psort_ms = timed(psort, 10, lambda: keys.copy_(keys0))
 
-    # ---------------- roofline of the dominant kernel ----------------
-    # build: compulsory bytes per triangle (u64 codes), see DESIGN.md
-    build_bytes = num_tri * 12 + num_vtx * 12 + num_tri * 12 + num_node * 12 + num_node * 24
-    # ray-cast: traversal bytes from the oracle's count of expanded internal
-    # nodes (row 12 B + two child boxes 48 B) and tested leaves (row 12 B +
-    # indices 12 B + vertices 36 B), plus the root box and 8 B of output per ray
+    # ---------------- CPU baseline and traversal counts ----------------
     cast_counts = None
     cpu = None
     if not args.no_cpu_baseline:
@@ -205,26 +322,48 @@ def main():
             "raycast_mrays_per_s": round(num_pix / cpu_cast_s / 1e6, 3),
             "raycast_threads": threads,
         }
-    if cast_counts:
-        cast_bytes = num_pix * (24 + 8) + 60 * (cast_counts[0] + cast_counts[1])
+
+    # ---------------- roofline of the dominant kernel ----------------
+    # build: compulsory bytes per triangle with u64 codes
+    build_bytes = num_tri * 12 + num_vtx * 12 + num_tri * 12 + num_node * 12 + num_node * 24
+    stack = L.dmsh_get_tuning(b"pix2tri.stack")
+    k1 = kernels.get("k_pix2tri")
+    if cast_counts and k1:
+        # ray-cast, in the reference's layout: per ray the root box (24 B) and
+        # 8 B of output; per expanded internal node its row (12 B) and its two
+        # child boxes (48 B); per tested leaf the triangle's indices (12 B) and
+        # vertices (36 B)
+        cast_bytes = num_pix * (24 + 8) + 60 * cast_counts[0] + 48 * cast_counts[1]
+        complete = cast_counts[2] <= stack  # every ray finished in k_pix2tri
+        k_ms = k1["avg_ms"]
+        name = "k_pix2tri"
+        if not complete:
+            k_ms = sum(kernels[k]["avg_ms"] * kernels[k]["launches_per_step"]
+                       for k in ("k_pix2tri", "k_pix2tri_tier2", "k_pix2tri_tier3") if k in kernels)
+            name = "k_pix2tri + restart tiers"
+        roof = {
+            "bound": "hbm",
+            "kernel": name,
+            "achieved": round(cast_bytes / (k_ms * 1e-3) / 1e9, 1),
+            "peak": PEAK_HBM_GBS,
+            "unit": "GB/s",
+            "frac": round(cast_bytes / (k_ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4),
+            "traffic": read_traffic() if complete else None,
+            "algorithmic_bytes": cast_bytes,
+            "kernel_ms": k_ms,
+        }
     else:
-        cast_bytes = None
-    dominant = "raycast" if cast_avg >= build_avg else "build"
-    if dominant == "raycast" and cast_bytes:
-        achieved = cast_bytes / (cast_avg * 1e-3) / 1e9
-        kern = "k_pix2tri"
-    else:
-        achieved = build_bytes / (build_avg * 1e-3) / 1e9
-        kern = "dmsh_build_trimesh3 (13 kernels)"
-    roof = {
-        "bound": "hbm",
-        "kernel": kern,
-        "achieved": round(achieved, 1),
-        "peak": PEAK_HBM_GBS,
-        "unit": "GB/s",
-        "frac": round(achieved / PEAK_HBM_GBS, 4),
-        "traffic": None,
-    }
+        roof = {
+            "bound": "hbm",
+            "kernel": "dmsh_build_trimesh3 (all build kernels)",
+            "achieved": round(build_bytes / (build_avg * 1e-3) / 1e9, 1),
+            "peak": PEAK_HBM_GBS,
+            "unit": "GB/s",
+            "frac": round(build_bytes / (build_avg * 1e-3) / 1e9 / PEAK_HBM_GBS, 4),
+            "traffic": None,
+            "algorithmic_bytes": build_bytes,
+            "kernel_ms": build_avg,
+        }
     line = {
         "metric": "LBVH build Mtri/s + Mrays/s, batched/served vs plain kernel; p50/p99 latency",
         "value": round(build_mtri, 2),
@@ -246,6 +385,7 @@ def main():
             "rays": num_pix,
         },
         "build_ms": round(build_avg, 4),
+        "build_algorithmic_gbs": round(build_bytes / (build_avg * 1e-3) / 1e9, 1),
         "raycast_ms": round(cast_avg, 4),
         "raycast_mrays_per_s": round(cast_mrays, 2),
         "points_build_2p22_ms": round(pbuild_ms, 4),
@@ -253,10 +393,10 @@ def main():
         "sort_alone_u64_2p22_ms": round(psort_ms, 4),
         "sort_alone_melem_per_s": round(npnt / (psort_ms * 1e-3) / 1e6, 2),
         "raycast_traversal": ({"internal_expansions": cast_counts[0], "leaf_tests": cast_counts[1],
-                               "max_stack_frames": cast_counts[2],
-                               "lds_stack_frames": L.dmsh_get_tuning(b"pix2tri.stack"),
+                               "max_stack_frames": cast_counts[2], "lds_stack_frames": stack,
                                "waves_per_block": L.dmsh_get_tuning(b"pix2tri.waves")}
                               if cast_counts else None),
+        "kernels": kernels,
         "roofline": roof,
         "cpu_baseline": cpu,
     }

@@ -222,15 +222,15 @@ inline int build(hipStream_t s, uint32_t n, const uint32_t* tri2vtx, uint32_t nu
   const uint32_t kb = key_bits_of<KeyT>();
   const uint32_t P = sortk::num_passes(kb);
   int rc;
-  hipLaunchKernelGGL(k_build_init, dim3(std::min<uint32_t>(ceil_div(L.zero_words, 256), 2048)),
+  launch("k_build_init", k_build_init, dim3(std::min<uint32_t>(ceil_div(L.zero_words, 256), 2048)),
                      dim3(256), 0, s, L.keys6, L.zero_begin, L.zero_words);
   if ((rc = launch_status()) != DMSH_OK) return rc;
-  hipLaunchKernelGGL(k_centroid_aabb<TRI>, dim3(std::min<uint32_t>(ceil_div(n, 256), 1024)),
+  launch("k_centroid_aabb", k_centroid_aabb<TRI>, dim3(std::min<uint32_t>(ceil_div(n, 256), 1024)),
                      dim3(256), 0, s, n, tri2vtx, TRI ? num_vtx : n, xyz, L.cntr, L.keys6, err);
   if ((rc = launch_status()) != DMSH_OK) return rc;
   KeyT* codes = static_cast<KeyT*>(L.codes);
   const uint32_t mh_blocks = std::min<uint32_t>(sortk::num_tiles<KeyT>(n), 2048);
-  hipLaunchKernelGGL(k_morton_hist<KeyT>, dim3(mh_blocks), dim3(256), 0, s, n,
+  launch("k_morton_hist", k_morton_hist<KeyT>, dim3(mh_blocks), dim3(256), 0, s, n,
                      TRI ? L.cntr : xyz, L.keys6, codes, P, kb, L.sort.hist);
   if ((rc = launch_status()) != DMSH_OK) return rc;
   // P is even for 30- and 63-bit codes: the passes end back in `codes`.

@@ -49,22 +49,22 @@ int dmsh_contour_for_triangle_mesh(void* stream, uint32_t num_tri, const uint32_
   Arena a{static_cast<char*>(ws), ws_bytes, 0};
   ContourLayout L;
   if (!carve_contour(a, num_edge, L)) return DMSH_ERR_WORKSPACE;
-  hipLaunchKernelGGL(k_mat4_inverse, dim3(1), dim3(1), 0, s, transform_world2ndc, L.n2w, err);
+  launch("k_mat4_inverse", k_mat4_inverse, dim3(1), dim3(1), 0, s, transform_world2ndc, L.n2w, err);
   int st = launch_status();
   if (st != DMSH_OK) return st;
   const uint32_t nb = num_edge ? contour_blocks(num_edge) : 0;
   if (nb) {
     aa::ContourArgs ca{tri2vtx, num_tri, vtx2xyz, num_vtx, transform_world2ndc, L.n2w,
                        num_edge, edge2vtx, edge2tri, err};
-    hipLaunchKernelGGL(aa::k_contour_count, dim3(nb), dim3(256), 0, s, ca, L.flags,
+    launch("k_contour_count", aa::k_contour_count, dim3(nb), dim3(256), 0, s, ca, L.flags,
                        L.block_count);
     if ((st = launch_status()) != DMSH_OK) return st;
   }
   // also writes the total when there is no edge (0)
-  hipLaunchKernelGGL(aa::k_scan_counts, dim3(1), dim3(1024), 0, s, L.block_count, nb, num_cedge);
+  launch("k_scan_counts", aa::k_scan_counts, dim3(1), dim3(1024), 0, s, L.block_count, nb, num_cedge);
   if ((st = launch_status()) != DMSH_OK) return st;
   if (nb) {
-    hipLaunchKernelGGL(aa::k_contour_scatter, dim3(nb), dim3(256), 0, s, edge2vtx, num_edge,
+    launch("k_contour_scatter", aa::k_contour_scatter, dim3(nb), dim3(256), 0, s, edge2vtx, num_edge,
                        L.flags, L.block_count, cedge2vtx);
     st = launch_status();
   }
@@ -89,15 +89,15 @@ int dmsh_antialias_fwd(void* stream, uint32_t num_cedge, const uint32_t* cedge2v
   if (!err) return DMSH_ERR_HIP;
   hipStream_t s = as_stream(stream);
   auto* keys = static_cast<unsigned long long*>(ws);
-  hipLaunchKernelGGL(aa::k_zero_u64, dim3(std::min<uint32_t>(ceil_div(num_pix, 256), 4096)),
+  launch("k_zero_u64", aa::k_zero_u64, dim3(std::min<uint32_t>(ceil_div(num_pix, 256), 4096)),
                      dim3(256), 0, s, keys, num_pix);
   int st = launch_status();
   if (st != DMSH_OK) return st;
   aa::EdgeArgs ea{cedge2vtx, num_cedge, vtx2xyz, num_vtx, transform_world2pix,
                   img_w, img_h, pix2tri, err};
-  hipLaunchKernelGGL(aa::k_aa_keys, dim3(ceil_div(num_cedge, 64)), dim3(64), 0, s, ea, keys);
+  launch("k_aa_keys", aa::k_aa_keys, dim3(ceil_div(num_cedge, 64)), dim3(64), 0, s, ea, keys);
   if ((st = launch_status()) != DMSH_OK) return st;
-  hipLaunchKernelGGL(aa::k_aa_write, dim3(ceil_div(num_cedge, 64)), dim3(64), 0, s, ea, keys,
+  launch("k_aa_write", aa::k_aa_write, dim3(ceil_div(num_cedge, 64)), dim3(64), 0, s, ea, keys,
                      pix2vin, pix2vout);
   return launch_status();
 }
@@ -116,7 +116,7 @@ int dmsh_antialias_bwd(void* stream, uint32_t num_cedge, const uint32_t* cedge2v
   if (!err) return DMSH_ERR_HIP;
   aa::EdgeArgs ea{cedge2vtx, num_cedge, vtx2xyz, num_vtx, transform_world2pix,
                   img_w, img_h, pix2tri, err};
-  hipLaunchKernelGGL(aa::k_aa_bwd, dim3(ceil_div(num_cedge, 64)), dim3(64), 0, as_stream(stream),
+  launch("k_aa_bwd", aa::k_aa_bwd, dim3(ceil_div(num_cedge, 64)), dim3(64), 0, as_stream(stream),
                      ea, pix2val, dldw_pix2val, dldw_vtx2xyz);
   return launch_status();
 }

@@ -13,10 +13,10 @@ int dmsh_vtx2morton_u32(void* stream, uint32_t num_vtx, const float* vtx2co, uin
   if (!vtx2co || !transform_co2unit || !vtx2morton) return DMSH_ERR_INVALID;
   hipStream_t s = as_stream(stream);
   if (num_dim == 3) {
-    hipLaunchKernelGGL(k_vtx2morton3<uint32_t>, dim3(ceil_div(num_vtx, 256)), dim3(256), 0, s,
+    launch("k_vtx2morton3", k_vtx2morton3<uint32_t>, dim3(ceil_div(num_vtx, 256)), dim3(256), 0, s,
                        num_vtx, vtx2co, transform_co2unit, vtx2morton);
   } else if (num_dim == 2) {
-    hipLaunchKernelGGL(k_vtx2morton2, dim3(ceil_div(num_vtx, 256)), dim3(256), 0, s, num_vtx,
+    launch("k_vtx2morton2", k_vtx2morton2, dim3(ceil_div(num_vtx, 256)), dim3(256), 0, s, num_vtx,
                        vtx2co, transform_co2unit, vtx2morton);
   } else {
     return DMSH_ERR_INVALID;
@@ -28,7 +28,7 @@ int dmsh_vtx2morton_u64(void* stream, uint32_t num_vtx, const float* vtx2xyz,
                         const float* transform_co2unit, uint64_t* vtx2morton) {
   if (num_vtx == 0) return DMSH_OK;
   if (!vtx2xyz || !transform_co2unit || !vtx2morton) return DMSH_ERR_INVALID;
-  hipLaunchKernelGGL(k_vtx2morton3<uint64_t>, dim3(ceil_div(num_vtx, 256)), dim3(256), 0,
+  launch("k_vtx2morton3", k_vtx2morton3<uint64_t>, dim3(ceil_div(num_vtx, 256)), dim3(256), 0,
                      as_stream(stream), num_vtx, vtx2xyz, transform_co2unit, vtx2morton);
   return launch_status();
 }

@@ -197,6 +197,34 @@ namespace dmsh {
 // returns its device address for the current device, or nullptr on failure.
 uint32_t* device_error_word();
 
+// Opt-in kernel timing (dmsh_timing_enable): while on, a TimedScope records a
+// HIP event on the launch stream before and after the kernels it encloses,
+// filed under its label.  Off, it costs one relaxed atomic load.
+bool timing_on();
+void timing_mark(hipStream_t s, const char* label, bool begin);
+struct TimedScope {
+  hipStream_t s;
+  const char* label;
+  bool on;
+  TimedScope(hipStream_t s_, const char* label_) : s(s_), label(label_), on(timing_on()) {
+    if (on) timing_mark(s, label, true);
+  }
+  ~TimedScope() {
+    if (on) timing_mark(s, label, false);
+  }
+  TimedScope(const TimedScope&) = delete;
+  TimedScope& operator=(const TimedScope&) = delete;
+};
+
+// Every kernel launch of the library goes through here, filed under `label`
+// (the kernel's name) when timing is on.
+template <typename... KArgs, typename... Args>
+inline void launch(const char* label, void (*kernel)(KArgs...), dim3 grid, dim3 block,
+                   uint32_t shmem, hipStream_t s, Args&&... args) {
+  TimedScope ts(s, label);
+  hipLaunchKernelGGL(kernel, grid, block, shmem, s, static_cast<KArgs>(args)...);
+}
+
 __device__ __forceinline__ void raise_error(uint32_t* err, uint32_t bits) {
   if (err) atomicOr(err, bits);
 }

@@ -11,9 +11,11 @@ using namespace dmsh;
 
 namespace {
 
-// Defaults from a sweep on the 2^21-triangle / 2048^2-ray workload (MI355X):
-// 16 frames x 1 wave 7.38 ms, 24 x 2 7.89 ms, 32 x 4 11.2 ms.
-std::atomic<int> g_cast_stack{16};  // LDS frames per lane (tier 1)
+// Defaults from a sweep on the 2^21-triangle / 2048^2-ray workload (MI355X),
+// whose deepest ray needs 24 frames: 16 frames x 1 wave 3.36 ms (some rays
+// restart in tier 2), 24 x 1 3.37 ms (none do), 24 x 2 3.53 ms, 32 x 1
+// 3.98 ms, 32 x 4 5.12 ms.
+std::atomic<int> g_cast_stack{24};  // LDS frames per lane (tier 1)
 std::atomic<int> g_cast_waves{1};   // waves per workgroup (tier 1)
 
 struct CastLayout {
@@ -37,7 +39,7 @@ inline bool carve_cast(Arena& a, uint32_t num_tri, uint32_t num_pix, CastLayout&
 
 template <int S, int WAVES>
 void launch_tier1(hipStream_t s, const rc::Packed& pk, const float* T, const rc::CastArgs& a) {
-  hipLaunchKernelGGL((rc::k_pix2tri<S, WAVES>), dim3(ceil_div(a.num_tiles, WAVES)),
+  launch("k_pix2tri", (rc::k_pix2tri<S, WAVES>), dim3(ceil_div(a.num_tiles, WAVES)),
                      dim3(64 * WAVES), 0, s, pk, T, a);
 }
 
@@ -112,15 +114,15 @@ int dmsh_pix2tri(void* stream, uint32_t* pix2tri, float* pix2t, uint32_t num_tri
   uint32_t* err = device_error_word();
   if (!err) return DMSH_ERR_HIP;
   hipStream_t s = as_stream(stream);
-  int st = sortk::launch_fill(s, L.counts, 2, 0u);
+  int st = sortk::launch_fill(s, L.counts, 3, 0u);
   if (st != DMSH_OK) return st;
   if (num_tri > 1) {
-    hipLaunchKernelGGL(rc::k_pack_inner, dim3(ceil_div(num_tri - 1, 256)), dim3(256), 0, s,
-                       L.inner, bvhnodes, bvhnode2aabb, num_tri, err);
+    launch("k_pack_inner", rc::k_pack_inner, dim3(ceil_div(num_tri - 1, 256)), dim3(256), 0, s,
+                       L.inner, bvhnodes, bvhnode2aabb, num_tri, L.counts + 2, err);
     if ((st = launch_status()) != DMSH_OK) return st;
   }
   if (num_tri > 0) {
-    hipLaunchKernelGGL(rc::k_pack_leaf, dim3(ceil_div(num_tri, 256)), dim3(256), 0, s, L.leaf,
+    launch("k_pack_leaf", rc::k_pack_leaf, dim3(ceil_div(num_tri, 256)), dim3(256), 0, s, L.leaf,
                        bvhnodes, tri2vtx, vtx2xyz, num_tri, num_tri, num_vtx, err);
     if ((st = launch_status()) != DMSH_OK) return st;
   }
@@ -140,10 +142,10 @@ int dmsh_pix2tri(void* stream, uint32_t* pix2tri, float* pix2t, uint32_t num_tri
   if (!launch_tier1_any(g_cast_stack.load(), g_cast_waves.load(), s, pk, transform_ndc2world, a))
     return DMSH_ERR_INVALID;
   if ((st = launch_status()) != DMSH_OK) return st;
-  hipLaunchKernelGGL(rc::k_pix2tri_tier2, dim3(rc::kTier2Blocks), dim3(64), 0, s, pk,
+  launch("k_pix2tri_tier2", rc::k_pix2tri_tier2, dim3(rc::kTier2Blocks), dim3(64), 0, s, pk,
                      transform_ndc2world, a);
   if ((st = launch_status()) != DMSH_OK) return st;
-  hipLaunchKernelGGL(rc::k_pix2tri_tier3, dim3(rc::kDeepSlots / 64), dim3(64), 0, s, pk,
+  launch("k_pix2tri_tier3", rc::k_pix2tri_tier3, dim3(rc::kDeepSlots / 64), dim3(64), 0, s, pk,
                      transform_ndc2world, a);
   return launch_status();
 }
@@ -164,12 +166,37 @@ int dmsh_pix2depth(void* stream, float* pix2depth, const uint32_t* pix2tri, uint
   if (!err) return DMSH_ERR_HIP;
   hipStream_t s = as_stream(stream);
   float* inv = static_cast<float*>(ws);
-  hipLaunchKernelGGL(k_mat4_inverse, dim3(1), dim3(1), 0, s, transform_ndc2world, inv, err);
+  launch("k_mat4_inverse", k_mat4_inverse, dim3(1), dim3(1), 0, s, transform_ndc2world, inv, err);
   int st = launch_status();
   if (st != DMSH_OK) return st;
   rc::TriMesh sc{tri2vtx, vtx2xyz, num_tri, num_vtx};
-  hipLaunchKernelGGL(rc::k_pix2depth, dim3(ceil_div(num_pix, 256)), dim3(256), 0, s, pix2depth,
+  launch("k_pix2depth", rc::k_pix2depth, dim3(ceil_div(num_pix, 256)), dim3(256), 0, s, pix2depth,
                      pix2tri, sc, img_w, img_h, transform_ndc2world, inv, err);
+  return launch_status();
+}
+
+int dmsh_pix2depth_bwd_wrt_vtx2xyz(void* stream, float* dldw_vtx2xyz, const uint32_t* pix2tri,
+                                   uint32_t num_tri, const uint32_t* tri2vtx, uint32_t num_vtx,
+                                   const float* vtx2xyz, const float* dldw_pix2depth,
+                                   uint32_t img_w, uint32_t img_h,
+                                   const float* transform_ndc2world, void* ws, size_t ws_bytes) {
+  const uint64_t num_pix = (uint64_t)img_w * img_h;
+  if (num_pix == 0) return DMSH_OK;
+  if (num_pix >= (1ull << 32) || !dldw_vtx2xyz || !pix2tri || !dldw_pix2depth ||
+      !transform_ndc2world)
+    return DMSH_ERR_INVALID;
+  if (num_tri > 0 && (!tri2vtx || !vtx2xyz)) return DMSH_ERR_INVALID;
+  if (!ws || ws_bytes < dmsh_pix2depth_workspace_bytes()) return DMSH_ERR_WORKSPACE;
+  uint32_t* err = device_error_word();
+  if (!err) return DMSH_ERR_HIP;
+  hipStream_t s = as_stream(stream);
+  float* inv = static_cast<float*>(ws);
+  launch("k_mat4_inverse", k_mat4_inverse, dim3(1), dim3(1), 0, s, transform_ndc2world, inv, err);
+  int st = launch_status();
+  if (st != DMSH_OK) return st;
+  rc::TriMesh sc{tri2vtx, vtx2xyz, num_tri, num_vtx};
+  launch("k_pix2depth_bwd", rc::k_pix2depth_bwd, dim3(ceil_div(num_pix, 256)), dim3(256), 0, s,
+         dldw_vtx2xyz, pix2tri, sc, dldw_pix2depth, img_w, img_h, transform_ndc2world, inv, err);
   return launch_status();
 }
 
@@ -187,7 +214,7 @@ int dmsh_rasterized_edge_gradient_bwd(void* stream, const uint32_t* tri2vtx, uin
     return DMSH_ERR_INVALID;
   uint32_t* err = device_error_word();
   if (!err) return DMSH_ERR_HIP;
-  hipLaunchKernelGGL(eg::k_edge_grad_bwd, dim3(ceil_div(img_w, 64), ceil_div(img_h, 4)),
+  launch("k_edge_grad_bwd", eg::k_edge_grad_bwd, dim3(ceil_div(img_w, 64), ceil_div(img_h, 4)),
                      dim3(64, 4), 0, as_stream(stream), tri2vtx, num_tri, vtx2xyz, num_vtx,
                      dldw_vtx2xyz, transform_world2pix, img_w, img_h, pix2tri, num_vdim, pix2val,
                      dldw_pix2val, err);

@@ -1,5 +1,9 @@
 // dmsh_status.hip -- status strings and the sticky device error word.
+#include <atomic>
+#include <map>
 #include <mutex>
+#include <string>
+#include <vector>
 
 #include "dmsh_common.hpp"
 
@@ -24,9 +28,90 @@ uint32_t* device_error_word() {
   return g_err_ptr[dev];
 }
 
+// ---------------------------------------------------------------------------
+// kernel timing
+// ---------------------------------------------------------------------------
+namespace {
+struct EventPair {
+  hipEvent_t start, stop;
+};
+std::atomic<bool> g_timing{false};
+std::mutex g_timing_mutex;
+std::map<std::string, std::vector<hipEvent_t>> g_open;     // started, not stopped
+std::map<std::string, std::vector<EventPair>> g_recorded;  // complete launches
+
+void destroy_all_events() {
+  for (auto& kv : g_open)
+    for (hipEvent_t e : kv.second) (void)hipEventDestroy(e);
+  for (auto& kv : g_recorded)
+    for (EventPair& p : kv.second) {
+      (void)hipEventDestroy(p.start);
+      (void)hipEventDestroy(p.stop);
+    }
+  g_open.clear();
+  g_recorded.clear();
+}
+}  // namespace
+
+bool timing_on() { return g_timing.load(std::memory_order_relaxed); }
+
+void timing_mark(hipStream_t s, const char* label, bool begin) {
+  std::lock_guard<std::mutex> lock(g_timing_mutex);
+  if (begin) {
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return;
+    if (hipEventRecord(e, s) != hipSuccess) {
+      (void)hipEventDestroy(e);
+      return;
+    }
+    g_open[label].push_back(e);
+    return;
+  }
+  auto it = g_open.find(label);
+  if (it == g_open.end() || it->second.empty()) return;
+  hipEvent_t start = it->second.back();
+  it->second.pop_back();
+  hipEvent_t stop = nullptr;
+  if (hipEventCreate(&stop) != hipSuccess || hipEventRecord(stop, s) != hipSuccess) {
+    if (stop) (void)hipEventDestroy(stop);
+    (void)hipEventDestroy(start);
+    return;
+  }
+  g_recorded[label].push_back(EventPair{start, stop});
+}
+
 }  // namespace dmsh
 
 extern "C" {
+
+int dmsh_timing_enable(int on) {
+  std::lock_guard<std::mutex> lock(dmsh::g_timing_mutex);
+  if (on) dmsh::destroy_all_events();
+  dmsh::g_timing.store(on != 0);
+  return DMSH_OK;
+}
+
+int dmsh_timing_query(const char* label, uint32_t* launches, double* total_ms, double* max_ms) {
+  if (!label || !launches || !total_ms) return DMSH_ERR_INVALID;
+  std::lock_guard<std::mutex> lock(dmsh::g_timing_mutex);
+  *launches = 0;
+  *total_ms = 0.0;
+  if (max_ms) *max_ms = 0.0;
+  auto it = dmsh::g_recorded.find(label);
+  if (it == dmsh::g_recorded.end()) return DMSH_OK;
+  double total = 0.0, mx = 0.0;
+  for (const dmsh::EventPair& p : it->second) {
+    if (hipEventSynchronize(p.stop) != hipSuccess) return DMSH_ERR_HIP;
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, p.start, p.stop) != hipSuccess) return DMSH_ERR_HIP;
+    total += ms;
+    mx = ms > mx ? ms : mx;
+  }
+  *launches = (uint32_t)it->second.size();
+  *total_ms = total;
+  if (max_ms) *max_ms = mx;
+  return DMSH_OK;
+}
 
 const char* dmsh_status_string(int status) {
   switch (status) {

@@ -88,6 +88,7 @@ static __global__ __launch_bounds__(256) void k_pack_inner(InnerNode* __restrict
                                                            const uint32_t* __restrict__ bvhnodes,
                                                            const float* __restrict__ aabb,
                                                            uint32_t num_leaf,
+                                                           uint32_t* __restrict__ nonfinite,
                                                            uint32_t* __restrict__ err) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i + 1 >= num_leaf) return;
@@ -111,6 +112,11 @@ static __global__ __launch_bounds__(256) void k_pack_inner(InnerNode* __restrict
     nd.b = make_float4(b0[4], b0[5], b1[0], b1[1]);
     nd.c = make_float4(b1[2], b1[3], b1[4], b1[5]);
     nd.d = make_uint4(r0, r1, 0u, 0u);
+    bool fin = true;
+#pragma unroll
+    for (int k = 0; k < 6; ++k)
+      fin = fin && fabsf(b0[k]) < __builtin_inff() && fabsf(b1[k]) < __builtin_inff();
+    if (!fin) atomicOr(nonfinite, 1u);
   }
   inner[i] = nd;
 }
@@ -196,42 +202,53 @@ __device__ __forceinline__ void leaf_test(const Packed& pk, uint32_t slot, const
   tri_out = has ? __float_as_uint(lt.c.y) : DMSH_U32MAX;
 }
 
-template <class Stack>
-__device__ __forceinline__ int trace(const Packed& pk, const float org[3], const float dir[3],
-                                     Stack& stk, Hit& out) {
-  out.has = false;
-  out.t = 0.f;
-  out.tri = DMSH_U32MAX;
-  if (pk.num_leaf == 0) return kTraceOk;
-  RaySlab r;
-  make_slab(org, dir, r);
-  {
-    const float lo[3] = {pk.root_box[0], pk.root_box[1], pk.root_box[2]};
-    const float hi[3] = {pk.root_box[3], pk.root_box[4], pk.root_box[5]};
-    float ta, tb;
-    if (!slab_hit(r, lo, hi, ta, tb)) return kTraceOk;
-  }
+// Slab test for rays with finite origin and finite reciprocal direction on
+// every axis, against a finite box.  No t below can be NaN then, so min / max
+// instructions pick the same values as the compare-and-select of slab_hit up
+// to the sign of a zero, which no later comparison can see.
+__device__ __forceinline__ bool slab_hit_finite(const RaySlab& r, const float lo[3],
+                                                const float hi[3], float& tmin_out,
+                                                float& tmax_out) {
+  const float a0 = (lo[0] - r.o[0]) * r.inv[0], b0 = (hi[0] - r.o[0]) * r.inv[0];
+  const float a1 = (lo[1] - r.o[1]) * r.inv[1], b1 = (hi[1] - r.o[1]) * r.inv[1];
+  const float a2 = (lo[2] - r.o[2]) * r.inv[2], b2 = (hi[2] - r.o[2]) * r.inv[2];
+  const float tmin = fmaxf(fmaxf(fminf(a0, b0), fminf(a1, b1)), fminf(a2, b2));
+  const float tmax = fminf(fminf(fmaxf(a0, b0), fmaxf(a1, b1)), fmaxf(a2, b2));
+  tmin_out = tmin;
+  tmax_out = tmax;
+  return !(tmin > tmax) && !(tmax < 0.f);
+}
+
+constexpr uint32_t kNoTest = DMSH_U32MAX;
+
+// The traversal loop.  Each iteration either expands one internal node or
+// pops one frame, then runs the (single) leaf-test site if either step
+// produced a leaf, so a wave whose lanes diverge between the two steps pays
+// for the triangle test once.
+template <bool kFinite, class Stack>
+__device__ __forceinline__ int trace_loop(const Packed& pk, const RaySlab& r, const float org[3],
+                                          const float dir[3], Stack& stk, Hit& out) {
   bool res_has = false;
   float res_t = 0.f;
   uint32_t res_tri = DMSH_U32MAX;
-  if (pk.num_leaf == 1) {
-    leaf_test(pk, 0, org, dir, res_has, res_t, res_tri);
-    out.has = res_has;
-    out.t = res_t;
-    out.tri = res_tri;
-    return kTraceOk;
-  }
   uint32_t node = 0;
   int sp = 0;
   bool expand = true;
   for (;;) {
+    uint32_t test = kNoTest;
     if (expand) {
       const InnerNode nd = pk.inner[node];
       const float l0[3] = {nd.a.x, nd.a.y, nd.a.z}, h0[3] = {nd.a.w, nd.b.x, nd.b.y};
       const float l1[3] = {nd.b.z, nd.b.w, nd.c.x}, h1[3] = {nd.c.y, nd.c.z, nd.c.w};
       float tl0, tl1, tr0, tr1;
-      const bool hl = slab_hit(r, l0, h0, tl0, tl1);
-      const bool hr = slab_hit(r, l1, h1, tr0, tr1);
+      bool hl, hr;
+      if (kFinite) {
+        hl = slab_hit_finite(r, l0, h0, tl0, tl1);
+        hr = slab_hit_finite(r, l1, h1, tr0, tr1);
+      } else {
+        hl = slab_hit(r, l0, h0, tl0, tl1);
+        hr = slab_hit(r, l1, h1, tr0, tr1);
+      }
       uint32_t next;
       if (hl && hr) {
         const bool nl = tuple_lt(tl0, tl1, tr0, tr1);
@@ -248,7 +265,7 @@ __device__ __forceinline__ int trace(const Packed& pk, const float org[3], const
         continue;
       }
       if (next & kLeafBit) {
-        leaf_test(pk, next & kIndexMask, org, dir, res_has, res_t, res_tri);
+        test = next & kIndexMask;
         expand = false;
       } else {
         node = next;
@@ -268,13 +285,7 @@ __device__ __forceinline__ int trace(const Packed& pk, const float org[3], const
           --sp;  // combine(none, far) = far: no frame needed
         }
         if (f.x & kLeafBit) {
-          bool fh;
-          float ft;
-          uint32_t ftri;
-          leaf_test(pk, f.x & kIndexMask, org, dir, fh, ft, ftri);
-          res_has = fh;
-          res_t = ft;
-          res_tri = ftri;
+          test = f.x & kIndexMask;
         } else {
           node = f.x;
           expand = true;
@@ -291,11 +302,47 @@ __device__ __forceinline__ int trace(const Packed& pk, const float org[3], const
         }
       }
     }
+    if (test != kNoTest) leaf_test(pk, test, org, dir, res_has, res_t, res_tri);
   }
   out.has = res_has;
   out.t = res_t;
   out.tri = res_tri;
   return kTraceOk;
+}
+
+// boxes_finite: every packed child box is finite (checked while packing).
+template <class Stack>
+__device__ __forceinline__ int trace(const Packed& pk, const float org[3], const float dir[3],
+                                     Stack& stk, Hit& out, bool boxes_finite) {
+  out.has = false;
+  out.t = 0.f;
+  out.tri = DMSH_U32MAX;
+  if (pk.num_leaf == 0) return kTraceOk;
+  RaySlab r;
+  make_slab(org, dir, r);
+  const bool lane_finite = !r.par[0] && !r.par[1] && !r.par[2] &&
+                           fabsf(org[0]) < __builtin_inff() && fabsf(org[1]) < __builtin_inff() &&
+                           fabsf(org[2]) < __builtin_inff();
+  // one path for the whole wave, so the vote must come before any return
+  const bool all_finite = __all(boxes_finite && lane_finite);
+  {
+    const float lo[3] = {pk.root_box[0], pk.root_box[1], pk.root_box[2]};
+    const float hi[3] = {pk.root_box[3], pk.root_box[4], pk.root_box[5]};
+    float ta, tb;
+    if (!slab_hit(r, lo, hi, ta, tb)) return kTraceOk;
+  }
+  if (pk.num_leaf == 1) {
+    bool has;
+    float t;
+    uint32_t tri;
+    leaf_test(pk, 0, org, dir, has, t, tri);
+    out.has = has;
+    out.t = t;
+    out.tri = tri;
+    return kTraceOk;
+  }
+  if (all_finite) return trace_loop<true>(pk, r, org, dir, stk, out);
+  return trace_loop<false>(pk, r, org, dir, stk, out);
 }
 
 // Workgroups are dealt to the 8 XCDs round-robin; renumber them so that each
@@ -310,7 +357,8 @@ struct CastArgs {
   uint32_t* pix2tri;
   float* pix2t;
   uint32_t img_w, img_h, tiles_x, num_tiles;
-  uint32_t* counts;  // [0]: tier-2 queue length, [1]: tier-3 queue length
+  uint32_t* counts;  // [0]: tier-2 queue length, [1]: tier-3 queue length,
+                     // [2]: nonzero when a packed box is not finite
   uint32_t* list2;
   uint32_t* list3;
   uint2* deep_stack;
@@ -344,7 +392,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_pix2tri(Packed pk, const float* 
   ray_for_pixel(iw, ih, a.img_w, a.img_h, m, org, dir);
   LdsStack<S> stk{&s_stack[0][tid], 64 * WAVES};
   Hit h;
-  const int st = trace(pk, org, dir, stk, h);
+  const int st = trace(pk, org, dir, stk, h, a.counts[2] == 0u);
   const uint32_t ip = ih * a.img_w + iw;
   if (st == kTraceOverflow) {
     a.list2[atomicAdd(a.counts + 0, 1u)] = ip;
@@ -364,13 +412,14 @@ static __global__ __launch_bounds__(64) void k_pix2tri_tier2(Packed pk,
   float m[16];
   load_transform(T, m);
   LdsStack<kTier2Frames> stk{&s_stack[0][threadIdx.x], 64};
+  const bool boxes_finite = a.counts[2] == 0u;
   for (uint32_t e = blockIdx.x * 64u + threadIdx.x; e < count; e += gridDim.x * 64u) {
     const uint32_t ip = a.list2[e];
     const uint32_t ih = ip / a.img_w, iw = ip - ih * a.img_w;
     float org[3], dir[3];
     ray_for_pixel(iw, ih, a.img_w, a.img_h, m, org, dir);
     Hit h;
-    const int st = trace(pk, org, dir, stk, h);
+    const int st = trace(pk, org, dir, stk, h, boxes_finite);
     if (st == kTraceOverflow) {
       a.list3[atomicAdd(a.counts + 1, 1u)] = ip;
       continue;
@@ -391,13 +440,14 @@ static __global__ __launch_bounds__(64) void k_pix2tri_tier3(Packed pk,
   float m[16];
   load_transform(T, m);
   GlobalStack stk{a.deep_stack + (size_t)gid * kDeepFrames};
+  const bool boxes_finite = a.counts[2] == 0u;
   for (uint32_t e = gid; e < count; e += nthr) {
     const uint32_t ip = a.list3[e];
     const uint32_t ih = ip / a.img_w, iw = ip - ih * a.img_w;
     float org[3], dir[3];
     ray_for_pixel(iw, ih, a.img_w, a.img_h, m, org, dir);
     Hit h;
-    const int st = trace(pk, org, dir, stk, h);
+    const int st = trace(pk, org, dir, stk, h, boxes_finite);
     if (st != kTraceOk) {
       raise_error(a.err, DMSH_DEVERR_STACK);
       h.has = false;
@@ -457,6 +507,80 @@ static __global__ __launch_bounds__(256) void k_pix2depth(float* __restrict__ de
     }
   }
   depth[ip] = out;
+}
+
+// d transform_homogeneous(m, p) / d p, rows 0..2 (quotient rule):
+// J[r][j] = (m[j*4+r] w - x_r m[j*4+3]) / w^2, x_r the numerator of row r.
+__device__ __forceinline__ void jacobian_transform3(const float* m, const float p[3],
+                                                    float J[3][3]) {
+  const float x = p[0], y = p[1], z = p[2];
+  const float w = ((m[3] * x + m[7] * y) + m[11] * z) + m[15];
+  const float inv_w2 = 1.f / (w * w);
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    const float q = ((m[r] * x + m[4 + r] * y) + m[8 + r] * z) + m[12 + r];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) J[r][j] = (m[j * 4 + r] * w - q * m[j * 4 + 3]) * inv_w2;
+  }
+}
+
+// Depth backward, one thread per pixel: the line hit (t, bc) of the stored
+// triangle, q = sum bc_k p_k, dL/dq = J(world2ndc, q)^T (0, 0, g/2),
+// dL/dt = dir . dL/dq, and the hit's sensitivity to the triangle,
+// dL/dp_k = (dL/dt / (n . dir)) bc_k n with n = (p1 - p0) x (p2 - p0);
+// nine float atomics per covered pixel.
+static __global__ __launch_bounds__(256) void k_pix2depth_bwd(
+    float* __restrict__ dldw_xyz, const uint32_t* __restrict__ pix2tri, TriMesh sc,
+    const float* __restrict__ dldw_depth, uint32_t img_w, uint32_t img_h,
+    const float* __restrict__ T, const float* __restrict__ inv, uint32_t* __restrict__ err) {
+  const uint32_t ip = blockIdx.x * blockDim.x + threadIdx.x;
+  if (ip >= img_w * img_h) return;
+  const uint32_t tri = pix2tri[ip];
+  if (tri == DMSH_U32MAX || inv[16] == 0.f) return;
+  if (tri >= sc.num_tri) {
+    raise_error(err, DMSH_DEVERR_INDEX);
+    return;
+  }
+  const uint32_t* tv = sc.tri2vtx + (size_t)tri * 3;
+  const uint32_t iv[3] = {tv[0], tv[1], tv[2]};
+  if (iv[0] >= sc.num_vtx || iv[1] >= sc.num_vtx || iv[2] >= sc.num_vtx) {
+    raise_error(err, DMSH_DEVERR_INDEX);
+    return;
+  }
+  float m[16], w2n[16];
+  load_transform(T, m);
+  load_transform(inv, w2n);
+  const uint32_t ih = ip / img_w, iw = ip - ih * img_w;
+  float org[3], dir[3], p0[3], p1[3], p2[3], t, bc[3];
+  ray_for_pixel(iw, ih, img_w, img_h, m, org, dir);
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    p0[k] = sc.xyz[(size_t)iv[0] * 3 + k];
+    p1[k] = sc.xyz[(size_t)iv[1] * 3 + k];
+    p2[k] = sc.xyz[(size_t)iv[2] * 3 + k];
+  }
+  if (!tri_intersect(p0, p1, p2, org, dir, false, t, bc)) return;
+  float q[3], J[3][3], dldq[3], e1[3], e2[3], n[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) q[i] = (bc[0] * p0[i] + bc[1] * p1[i]) + bc[2] * p2[i];
+  jacobian_transform3(w2n, q, J);
+  const float half_g = 0.5f * dldw_depth[ip];
+#pragma unroll
+  for (int j = 0; j < 3; ++j) dldq[j] = (J[0][j] * 0.f + J[1][j] * 0.f) + J[2][j] * half_g;
+  const float dldt = dot3(dir, dldq);
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    e1[i] = p1[i] - p0[i];
+    e2[i] = p2[i] - p0[i];
+  }
+  cross3(e1, e2, n);
+  const float s = dldt / dot3(n, dir);
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const float sk = s * bc[k];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) unsafeAtomicAdd(dldw_xyz + (size_t)iv[k] * 3 + i, sk * n[i]);
+  }
 }
 
 }  // namespace rc

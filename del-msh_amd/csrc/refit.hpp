@@ -319,12 +319,12 @@ inline int launch_refit_two_phase(hipStream_t s, float* aabb, uint32_t* flags,
                                   float eps, uint32_t* err) {
   constexpr int kChunk = 2048, kThreads = 1024, kTopThreads = 1024;
   if (num_leaf == 0) return DMSH_OK;
-  hipLaunchKernelGGL((k_refit_local<TRI, kChunk, kThreads>), dim3(ceil_div(num_leaf, kChunk)),
+  launch("k_refit_local", (k_refit_local<TRI, kChunk, kThreads>), dim3(ceil_div(num_leaf, kChunk)),
                      dim3(kThreads), 0, s, aabb, num_leaf, bvhnodes, node2end, tri2vtx, num_vtx,
                      xyz, eps, start_count, starts, err);
   int rc = launch_status();
   if (rc != DMSH_OK || num_leaf <= 1) return rc;
-  hipLaunchKernelGGL(k_refit_top<kTopThreads>, dim3(1), dim3(kTopThreads), 0, s, aabb, flags,
+  launch("k_refit_top", k_refit_top<kTopThreads>, dim3(1), dim3(kTopThreads), 0, s, aabb, flags,
                      num_leaf, bvhnodes, start_count, starts, err);
   return launch_status();
 }
@@ -334,7 +334,7 @@ inline int launch_refit(hipStream_t s, float* aabb, uint32_t* flags, uint32_t nu
                         const uint32_t* bvhnodes, const uint32_t* tri2vtx, uint32_t num_vtx,
                         const float* xyz, float eps, uint32_t* err) {
   if (num_leaf == 0) return DMSH_OK;
-  hipLaunchKernelGGL(k_refit<TRI>, dim3(ceil_div(num_leaf, 256)), dim3(256), 0, s, aabb, flags,
+  launch("k_refit", k_refit<TRI>, dim3(ceil_div(num_leaf, 256)), dim3(256), 0, s, aabb, flags,
                      num_leaf, bvhnodes, tri2vtx, num_vtx, xyz, eps, err);
   return launch_status();
 }

@@ -274,7 +274,7 @@ inline bool carve(Arena& a, uint32_t n, uint32_t key_bits, Layout& L) {
 inline int launch_fill(hipStream_t s, uint32_t* p, size_t count, uint32_t value) {
   if (count == 0) return DMSH_OK;
   const uint32_t blocks = (uint32_t)std::min<size_t>(ceil_div(count, 256), 4096);
-  hipLaunchKernelGGL(k_fill_u32, dim3(blocks), dim3(256), 0, s, p, count, value);
+  launch("k_fill_u32", k_fill_u32, dim3(blocks), dim3(256), 0, s, p, count, value);
   return launch_status();
 }
 
@@ -303,11 +303,11 @@ inline int run_passes(hipStream_t s, uint32_t n, uint32_t key_bits, const Layout
     if (dbits > (uint32_t)kRadixBits) dbits = kRadixBits;
     uint32_t* st = L.status + (size_t)p * tiles * kRadix;
     if (p == 0) {
-      hipLaunchKernelGGL((k_onesweep<KeyT, ITEMS, true>), dim3(tiles), dim3(kThreads), 0, s, src_k,
+      launch("k_onesweep", (k_onesweep<KeyT, ITEMS, true>), dim3(tiles), dim3(kThreads), 0, s, src_k,
                          src_v, dk, dv, n, shift, dbits, L.hist + p * kRadix, st, L.tickets + p,
                          err);
     } else {
-      hipLaunchKernelGGL((k_onesweep<KeyT, ITEMS, false>), dim3(tiles), dim3(kThreads), 0, s,
+      launch("k_onesweep", (k_onesweep<KeyT, ITEMS, false>), dim3(tiles), dim3(kThreads), 0, s,
                          src_k, src_v, dk, dv, n, shift, dbits, L.hist + p * kRadix, st,
                          L.tickets + p, err);
     }
@@ -337,14 +337,14 @@ inline int argsort(hipStream_t s, uint32_t n, KeyT* keys_inout, uint32_t* idx_ou
   if (rc != DMSH_OK) return rc;
   const uint32_t P = num_passes(key_bits);
   const uint32_t hist_blocks = std::min<uint32_t>(num_tiles<KeyT>(n), 2048);
-  hipLaunchKernelGGL(k_hist_all<KeyT>, dim3(hist_blocks), dim3(256), 0, s, keys_inout, n, P,
+  launch("k_hist_all", k_hist_all<KeyT>, dim3(hist_blocks), dim3(256), 0, s, keys_inout, n, P,
                      key_bits, L.hist);
   if ((rc = launch_status()) != DMSH_OK) return rc;
   const KeyT* first_src = keys_inout;
   if (P % 2 == 1) {
     // An odd pass count would end in the scratch buffer: start from a copy.
     KeyT* alt_k = static_cast<KeyT*>(L.alt_keys);
-    hipLaunchKernelGGL(k_copy_keys<KeyT>, dim3(std::min<uint32_t>(ceil_div(n, 256), 4096)),
+    launch("k_copy_keys", k_copy_keys<KeyT>, dim3(std::min<uint32_t>(ceil_div(n, 256), 4096)),
                        dim3(256), 0, s, keys_inout, alt_k, n);
     if ((rc = launch_status()) != DMSH_OK) return rc;
     first_src = alt_k;

@@ -134,9 +134,9 @@ inline int launch_topology(hipStream_t s, uint32_t n, uint32_t* bvhnodes, const 
                            const uint32_t* idx2obj, uint32_t* node2end = nullptr) {
   if (n == 0) return DMSH_OK;
   if (n == 1) {
-    hipLaunchKernelGGL(k_topology_single, dim3(1), dim3(1), 0, s, bvhnodes, idx2obj);
+    launch("k_topology_single", k_topology_single, dim3(1), dim3(1), 0, s, bvhnodes, idx2obj);
   } else {
-    hipLaunchKernelGGL(k_topology<KeyT>, dim3(ceil_div(n - 1, 256)), dim3(256), 0, s, n, bvhnodes,
+    launch("k_topology", k_topology<KeyT>, dim3(ceil_div(n - 1, 256)), dim3(256), 0, s, n, bvhnodes,
                        codes, idx2obj, node2end);
   }
   return launch_status();

@@ -54,11 +54,47 @@ def from_fit_vtx2xyz_into_unit_cube(vtx2xyz: torch.Tensor) -> torch.Tensor:
 
 def perspective(fovy_deg: float, aspect: float, near: float, far: float,
                 device=None, dtype=torch.float32) -> torch.Tensor:
-    """OpenGL-style projection (camera looks down -z), NDC z in [-1, 1]."""
+    """Symmetric-frustum projection for a camera looking down -z, with the
+    depth convention of camera_perspective_blender: the near plane maps to
+    NDC z = +1 and the far plane to -1, so pixel rays (which start at NDC
+    z = +1) leave the camera and their first hit is the visible surface.
+    w = -z_camera > 0 in front of the camera."""
     f = 1.0 / math.tan(math.radians(fovy_deg) * 0.5)
     return _mat([[f / aspect, 0, 0, 0], [0, f, 0, 0],
-                 [0, 0, (far + near) / (near - far), 2.0 * far * near / (near - far)],
+                 [0, 0, -(far + near) / (near - far), -2.0 * far * near / (near - far)],
                  [0, 0, -1, 0]], device, dtype)
+
+
+def camera_perspective_blender(aspect: float, lens: float, near: float, far: float,
+                               proj_direction: bool, *, dtype=torch.float32,
+                               device=None) -> torch.Tensor:
+    """Blender-like projection (reference Mat44/torch.py:143-206): focal
+    length `lens` in mm on an 18 mm half-sensor; with proj_direction the
+    camera looks down -z and w = z_camera, the near plane maps to NDC z = +1.
+    Without it, the same frustum seen down +z (the matrix for (-far, -near)
+    followed by a flip of all three axes)."""
+    if not proj_direction:
+        t0 = camera_perspective_blender(aspect, lens, -far, -near, True, dtype=dtype,
+                                        device=device)
+        flip = torch.diag(torch.tensor([-1.0, -1.0, -1.0, 1.0], dtype=dtype, device=device))
+        return t0 @ flip
+    aspect = torch.as_tensor(aspect, dtype=dtype, device=device)
+    lens = torch.as_tensor(lens, dtype=dtype, device=device)
+    near = torch.as_tensor(near, dtype=dtype, device=device)
+    far = torch.as_tensor(far, dtype=dtype, device=device)
+    one = torch.ones((), dtype=dtype, device=device)
+    zero = torch.zeros((), dtype=dtype, device=device)
+    sx, sy = (one / aspect, one) if aspect < one else (one, aspect)
+    half = torch.atan2(torch.tensor(18.0, dtype=dtype, device=device), lens)
+    focus = torch.cos(half) / torch.sin(half)
+    zz = (near + far) / (near - far)
+    zw = 2.0 * far * near / (near - far)
+    return torch.stack([
+        torch.stack([-sx * focus, zero, zero, zero]),
+        torch.stack([zero, -sy * focus, zero, zero]),
+        torch.stack([zero, zero, zz, zw]),
+        torch.stack([zero, zero, one, zero]),
+    ])
 
 
 def look_at(eye, target, up, device=None, dtype=torch.float32) -> torch.Tensor:

@@ -70,6 +70,14 @@ _PROTOTYPES = {
         [c_void_p, c_void_p, c_u32, c_void_p, c_u32, c_void_p, c_void_p, c_u32, c_u32, c_void_p,
          c_u32, c_void_p, c_void_p],
     ),
+    "dmsh_pix2depth_bwd_wrt_vtx2xyz": (
+        c_int,
+        [c_void_p, c_void_p, c_void_p, c_u32, c_void_p, c_u32, c_void_p, c_void_p, c_u32, c_u32,
+         c_void_p, c_void_p, c_size],
+    ),
+    "dmsh_timing_enable": (c_int, [c_int]),
+    "dmsh_timing_query": (c_int, [ctypes.c_char_p, ctypes.POINTER(c_u32),
+                                  ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]),
     "dmsh_contour_workspace_bytes": (c_size, [c_u32]),
     "dmsh_contour_for_triangle_mesh": (
         c_int,
@@ -136,6 +144,21 @@ DEVERR_NAMES = {
     0x10: "singular transform",
     0x20: "antialias pixel walk longer than 2^26 steps",
 }
+
+
+def timing_enable(on: bool) -> None:
+    """Start (clearing earlier records) or stop per-kernel HIP-event timing."""
+    check(lib().dmsh_timing_enable(1 if on else 0), "dmsh_timing_enable")
+
+
+def timing_query(label: str):
+    """(launches, total_ms, max_ms) recorded for one kernel name."""
+    n = c_u32(0)
+    total = ctypes.c_double(0.0)
+    mx = ctypes.c_double(0.0)
+    check(lib().dmsh_timing_query(label.encode(), ctypes.byref(n), ctypes.byref(total),
+                                  ctypes.byref(mx)), "dmsh_timing_query")
+    return int(n.value), float(total.value), float(mx.value)
 
 
 def raise_on_device_errors() -> None:

@@ -55,6 +55,17 @@ int dmsh_get_device_errors(uint32_t *bits_out);
 int dmsh_set_tuning(const char *key, int value);
 int dmsh_get_tuning(const char *key);
 
+/* Kernel timing with HIP events on the launch stream (no reference
+ * counterpart; used by benchmarks and the autotuner).  enable(1) clears what
+ * was recorded and starts recording, enable(0) stops.  While on, each kernel
+ * launch of the entry points below is filed under the kernel's name
+ * ("k_pix2tri", "k_onesweep", ...).  query synchronizes the recorded events
+ * of a label and returns the launch count, the summed and the longest
+ * duration (max_ms may be NULL).  Keep it off under graph capture. */
+int dmsh_timing_enable(int on);
+int dmsh_timing_query(const char *label, uint32_t *launches, double *total_ms,
+                      double *max_ms);
+
 /* ---------------------------------------------------------------------
  * Morton codes
  * Replaces kernel `vtx2morton` (del-msh-cuda-kernels/cuda/mortons.cu:7-36)
@@ -174,6 +185,24 @@ int dmsh_pix2depth(void *stream, float *pix2depth, const uint32_t *pix2tri,
                    const float *vtx2xyz, uint32_t img_w, uint32_t img_h,
                    const float *transform_ndc2world, void *ws,
                    size_t ws_bytes);
+
+/* ---------------------------------------------------------------------
+ * Depth backward (accumulates dL/d(vertex) into dldw_vtx2xyz).  Replaces
+ * kernel `bwd_wrt_vtx2xyz` (del-msh-cuda-kernels/cuda/pix2depth.cu:49-97)
+ * launched by pix2depth_bwd_wrt_vtx2xyz
+ * (del-msh-dlpack/src/pix2depth.rs:101-190); CPU counterpart
+ * del-msh-cpu/src/pix2depth.rs:27-56 driven by
+ * del-msh-cpu/src/trimesh3_raycast.rs:25-71.  Same workspace as
+ * dmsh_pix2depth (the inverse transform, computed on the device).
+ * ------------------------------------------------------------------- */
+int dmsh_pix2depth_bwd_wrt_vtx2xyz(void *stream, float *dldw_vtx2xyz,
+                                   const uint32_t *pix2tri, uint32_t num_tri,
+                                   const uint32_t *tri2vtx, uint32_t num_vtx,
+                                   const float *vtx2xyz,
+                                   const float *dldw_pix2depth, uint32_t img_w,
+                                   uint32_t img_h,
+                                   const float *transform_ndc2world, void *ws,
+                                   size_t ws_bytes);
 
 /* ---------------------------------------------------------------------
  * Edge-gradient backward of the rasterized silhouette (accumulates into

@@ -796,6 +796,65 @@ int orc_pix2depth(float *pix2depth, const uint32_t *pix2tri,
   return 0;
 }
 
+/* mat4_col_major::jacobian_transform: d transform_homogeneous(m, p) / d p,
+ * J[r][j] = (m[j*4+r] w - x_r m[j*4+3]) / w^2 with x_r the homogeneous
+ * numerator of row r (quotient rule). */
+static void jacobian_transform3(const float m[16], const float *p, float J[3][3]) {
+  const float x = p[0], y = p[1], z = p[2];
+  const float w = ((m[3] * x + m[7] * y) + m[11] * z) + m[15];
+  const float inv_w2 = 1.f / (w * w);
+  for (int r = 0; r < 3; ++r) {
+    const float q = ((m[r] * x + m[4 + r] * y) + m[8 + r] * z) + m[12 + r];
+    for (int j = 0; j < 3; ++j) J[r][j] = (m[j * 4 + r] * w - q * m[j * 4 + 3]) * inv_w2;
+  }
+}
+
+/* pix2depth.rs:27-56 (Depth::bwd) as driven by trimesh3_raycast.rs:25-71
+ * (bwd_continuous).  Per covered pixel: the line hit (t, bc) of the stored
+ * triangle, q = bc0 p0 + bc1 p1 + bc2 p2, dL/dq = J(world2ndc, q)^T (0,0,g/2),
+ * dL/dt = dir . dL/dq, and then
+ * F6 tri3::intersection_against_line_bwd_wrt_tri with dL/du = dL/dv = 0:
+ *    dL/dp_k = (dL/dt / (n . dir)) bc_k n,  n = (p1 - p0) x (p2 - p0)
+ *    (moving p_k by d moves the plane at the hit point by bc_k d, and only
+ *    its normal component shifts the hit along the line).
+ * Accumulates into dldw_vtx2xyz; returns -1 when ndc2world is singular. */
+int orc_pix2depth_bwd(float *dldw_vtx2xyz, const uint32_t *pix2tri, const uint32_t *tri2vtx,
+                      const float *vtx2xyz, const float *dldw_pix2depth, uint32_t img_w,
+                      uint32_t img_h, const float T[16]) {
+  float w2n[16];
+  if (!orc_mat4_inverse(T, w2n)) return -1;
+  const size_t num_pix = (size_t)img_w * img_h;
+  for (size_t ip = 0; ip < num_pix; ++ip) {
+    const uint32_t tri = pix2tri[ip];
+    if (tri == U32MAX) continue;
+    const uint32_t iw = (uint32_t)(ip % img_w), ih = (uint32_t)(ip / img_w);
+    float org[3], dir[3], t, bc[3];
+    orc_ray_for_pixel(iw, ih, img_w, img_h, T, org, dir);
+    const uint32_t *v = tri2vtx + (size_t)tri * 3;
+    const float *p0 = vtx2xyz + (size_t)v[0] * 3;
+    const float *p1 = vtx2xyz + (size_t)v[1] * 3;
+    const float *p2 = vtx2xyz + (size_t)v[2] * 3;
+    if (!orc_line_tri(p0, p1, p2, org, dir, &t, bc)) continue; /* the reference unwraps */
+    float q[3], J[3][3], dldq[3], e1[3], e2[3], n[3];
+    for (int i = 0; i < 3; ++i) q[i] = (bc[0] * p0[i] + bc[1] * p1[i]) + bc[2] * p2[i];
+    jacobian_transform3(w2n, q, J);
+    const float half_g = 0.5f * dldw_pix2depth[ip];
+    for (int j = 0; j < 3; ++j) dldq[j] = (J[0][j] * 0.f + J[1][j] * 0.f) + J[2][j] * half_g;
+    const float dldt = dot3(dir, dldq);
+    for (int i = 0; i < 3; ++i) {
+      e1[i] = p1[i] - p0[i];
+      e2[i] = p2[i] - p0[i];
+    }
+    cross3(e1, e2, n);
+    const float s = dldt / dot3(n, dir);
+    for (int k = 0; k < 3; ++k) {
+      const float sk = s * bc[k];
+      for (int i = 0; i < 3; ++i) dldw_vtx2xyz[(size_t)v[k] * 3 + i] += sk * n[i];
+    }
+  }
+  return 0;
+}
+
 /* ======================================================================= */
 /* edge-gradient backward (rasterized_edge_gradient.rs)                     */
 /* ======================================================================= */

@@ -127,6 +127,11 @@ int orc_pix2tri(uint32_t *pix2tri, float *pix2t, size_t num_tri,
 int orc_pix2depth(float *pix2depth, const uint32_t *pix2tri,
                   const uint32_t *tri2vtx, const float *vtx2xyz,
                   uint32_t img_w, uint32_t img_h, const float ndc2world[16]);
+/* pix2depth.rs:27-56 + trimesh3_raycast.rs:25-71 (accumulates) */
+int orc_pix2depth_bwd(float *dldw_vtx2xyz, const uint32_t *pix2tri,
+                      const uint32_t *tri2vtx, const float *vtx2xyz,
+                      const float *dldw_pix2depth, uint32_t img_w, uint32_t img_h,
+                      const float ndc2world[16]);
 
 /* ---- contour and anti-aliasing (edge2vtx.rs, antialias.rs) ------------- */
 /* edge2vtx.rs:191-242: edges whose two triangles face opposite ways along

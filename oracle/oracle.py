@@ -73,6 +73,8 @@ def lib():
                  ctypes.c_uint32, _f32p, ctypes.c_int, ctypes.POINTER(TraceStats)]),
             "orc_pix2depth": (ctypes.c_int, [_f32p, _u32p, _u32p, _f32p, ctypes.c_uint32,
                                              ctypes.c_uint32, _f32p]),
+            "orc_pix2depth_bwd": (ctypes.c_int, [_f32p, _u32p, _u32p, _f32p, _f32p,
+                                                 ctypes.c_uint32, ctypes.c_uint32, _f32p]),
             "orc_edge_gradient_bwd": (
                 None,
                 [_u32p, _f32p, _f32p, _f32p, ctypes.c_uint32, ctypes.c_uint32, _u32p,
@@ -269,6 +271,16 @@ def pix2depth(pix2tri_, tri2vtx, vtx2xyz, ndc2world_colmajor, img_w, img_h):
                              _f32(vtx2xyz).reshape(-1), img_w, img_h, _f32(ndc2world_colmajor))
     assert rc == 0
     return out.reshape(img_h, img_w)
+
+
+def pix2depth_bwd(pix2tri_, tri2vtx, vtx2xyz, dldw_pix2depth, ndc2world_colmajor):
+    h, w = pix2tri_.shape
+    out = np.zeros(_f32(vtx2xyz).size, np.float32)
+    rc = lib().orc_pix2depth_bwd(out, _u32(pix2tri_).reshape(-1), _u32(tri2vtx).reshape(-1),
+                                 _f32(vtx2xyz).reshape(-1), _f32(dldw_pix2depth).reshape(-1),
+                                 w, h, _f32(ndc2world_colmajor))
+    assert rc == 0
+    return out.reshape(-1, 3)
 
 
 def edge_gradient_bwd(tri2vtx, vtx2xyz, world2pix_colmajor, pix2tri_, pix2val, dldw_pix2val):

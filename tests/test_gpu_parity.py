@@ -200,11 +200,13 @@ def test_pix2tri(gpu, oracle, mesh, camera):
     no_device_errors()
 
 
-def test_pix2tri_deep_stack_path(gpu, oracle):
+@pytest.mark.parametrize("camera", ["ortho", "persp"])
+def test_pix2tri_deep_stack_path(gpu, oracle, camera):
     """A run of equal Morton codes whose triangles are stacked along the view
     ray, listed far to near, makes the near child the rest of the run at every
     level: the traversal needs ~K frames, more than the LDS stack holds, so the
-    rays go through the global-stack kernel."""
+    rays go through the global-stack kernel.  The orthographic camera has
+    parallel axes (general slab test), the perspective one none."""
     K = 80
     quad = []
     vtx = []
@@ -221,13 +223,36 @@ def test_pix2tri_deep_stack_path(gpu, oracle):
     tri = np.array(quad, np.uint32)
     vtx = np.array(vtx, np.float32)
     bn, aabb = oracle.build_trimesh3(tri, vtx)
-    n2w = np.diag([1.0, 1.0, 1.0, 1.0]).astype(np.float32)
+    if camera == "ortho":
+        n2w = np.diag([1.0, 1.0, 1.0, 1.0]).astype(np.float32)
+    else:
+        n2w = _perspective(eye=(0.0, 0.0, 2.0), fovy=20.0)[0]
     pix, t = Pix2Tri.by_raycasting_with_coeff(G(tri, gpu), G(vtx, gpu), G(bn, gpu), G(aabb, gpu),
                                               G(n2w, gpu), (32, 32))
     wpix, wt, _ = oracle.pix2tri(tri, vtx, bn, aabb, n2w.T.reshape(-1), 32, 32)
     assert np.array_equal(H(pix), wpix)
     assert bits_equal(H(t), wt)
     assert (wpix == K - 1).sum() > 0  # the nearest sheet is what the rays see
+    no_device_errors()
+
+
+def test_pix2tri_nonfinite_vertices(gpu, oracle):
+    """NaN / infinite vertices give non-finite boxes, which switch the
+    traversal to the general slab test; results still match bit for bit."""
+    tri, vtx = primitives.torus_zup(0.6, 0.2, 32, 16)
+    vtx = vtx.copy()
+    vtx[5] = np.nan
+    vtx[17] = [np.inf, 0.1, 0.0]
+    vtx[40, 2] = -np.inf
+    bn, aabb = oracle.build_trimesh3(tri, vtx)
+    assert not np.isfinite(aabb).all()
+    n2w = _perspective(eye=(0.2, 0.3, 2.5))[0]
+    W, H_ = 96, 64
+    pix, t = Pix2Tri.by_raycasting_with_coeff(G(tri, gpu), G(vtx, gpu), G(bn, gpu), G(aabb, gpu),
+                                              G(n2w, gpu), (W, H_))
+    wpix, wt, _ = oracle.pix2tri(tri, vtx, bn, aabb, n2w.T.reshape(-1), W, H_, num_threads=8)
+    assert np.array_equal(H(pix), wpix)
+    assert bits_equal(H(t), wt)
     no_device_errors()
 
 

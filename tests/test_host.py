@@ -1,0 +1,59 @@
+"""Host-side pieces that need no GPU: camera matrices and the library's
+timing registry."""
+
+import math
+
+import numpy as np
+import torch
+
+import del_msh_dlpack.Mat44.torch as Mat44
+from del_msh_dlpack import _lib
+
+
+def _ndc(m, p):
+    q = m.double().numpy() @ np.append(np.asarray(p, np.float64), 1.0)
+    return q[:3] / q[3]
+
+
+def test_camera_perspective_blender_depth_convention():
+    """Reference test_pix2depth.py:23 uses (1, 30, 2, 6, True): the camera
+    looks down -z, z = -near maps to NDC z = +1 and z = -far to -1."""
+    m = Mat44.camera_perspective_blender(1.0, 30.0, 2.0, 6.0, True)
+    assert math.isclose(_ndc(m, (0, 0, -2.0))[2], 1.0, abs_tol=1e-6)
+    assert math.isclose(_ndc(m, (0, 0, -6.0))[2], -1.0, abs_tol=1e-6)
+    # focus = cot(atan2(18, lens)) = lens / 18; x_ndc = focus * x / |z|
+    p = _ndc(m, (0.3, -0.2, -4.0))
+    assert math.isclose(p[0], 30.0 / 18.0 * 0.3 / 4.0, rel_tol=1e-5)
+    assert math.isclose(p[1], 30.0 / 18.0 * -0.2 / 4.0, rel_tol=1e-5)
+    # aspect > 1 scales y by the aspect, aspect < 1 scales x by 1/aspect
+    m2 = Mat44.camera_perspective_blender(2.0, 30.0, 2.0, 6.0, True)
+    assert math.isclose(_ndc(m2, (0, 0.3, -4.0))[1], 2.0 * 30.0 / 18.0 * 0.3 / 4.0, rel_tol=1e-5)
+    m3 = Mat44.camera_perspective_blender(0.5, 30.0, 2.0, 6.0, True)
+    assert math.isclose(_ndc(m3, (0.3, 0, -4.0))[0], 2.0 * 30.0 / 18.0 * 0.3 / 4.0, rel_tol=1e-5)
+
+
+def test_camera_perspective_blender_other_direction():
+    """Without proj_direction: the matrix for (-far, -near) times diag(-1,-1,-1,1),
+    worked by hand for (1, 30, 2, 6) to [[f,0,0,0],[0,f,0,0],[0,0,-2,-6],[0,0,-1,0]]
+    with f = 30/18: still looking down -z, depth reversed (near -> -1)."""
+    m = Mat44.camera_perspective_blender(1.0, 30.0, 2.0, 6.0, False)
+    f = 30.0 / 18.0
+    want = np.array([[f, 0, 0, 0], [0, f, 0, 0], [0, 0, -2, -6], [0, 0, -1, 0]])
+    assert np.allclose(m.double().numpy(), want, rtol=1e-6, atol=1e-6)
+    assert math.isclose(_ndc(m, (0, 0, -2.0))[2], -1.0, abs_tol=1e-6)
+    assert math.isclose(_ndc(m, (0, 0, -6.0))[2], 1.0, abs_tol=1e-6)
+
+
+def test_perspective_near_plane_is_ndc_plus_one():
+    m = Mat44.perspective(45.0, 1.0, 1.0, 5.0, dtype=torch.float64)
+    assert math.isclose(_ndc(m, (0, 0, -1.0))[2], 1.0, abs_tol=1e-12)
+    assert math.isclose(_ndc(m, (0, 0, -5.0))[2], -1.0, abs_tol=1e-12)
+    f = 1.0 / math.tan(math.radians(22.5))
+    assert math.isclose(_ndc(m, (0.5, 0, -2.0))[0], f * 0.25, rel_tol=1e-12)
+
+
+def test_timing_registry_without_launches():
+    _lib.timing_enable(True)
+    assert _lib.timing_query("k_pix2tri") == (0, 0.0, 0.0)
+    _lib.timing_enable(False)
+    assert _lib.timing_query("no_such_kernel") == (0, 0.0, 0.0)
