@@ -161,9 +161,8 @@ struct BuildLayout {
   uint32_t* keys6;
   sortk::Layout sort;
   uint32_t* flags;
-  uint32_t* start_count;
-  uint32_t* starts;
-  uint32_t* node2end;
+  uint32_t* treelet_count;
+  uint4* treelets;
   float* cntr;
   void* codes;
   uint32_t* idx;
@@ -184,13 +183,12 @@ inline bool carve(Arena& a, uint32_t n, bool tri, BuildLayout& L) {
   a.off += sizeof(uint32_t) * status_words;  // contiguous after the tickets
   L.flags = L.sort.status + status_words;
   a.off += sizeof(uint32_t) * (n > 1 ? n - 1 : 1);
-  L.start_count = L.flags + (n > 1 ? n - 1 : 1);
+  L.treelet_count = L.flags + (n > 1 ? n - 1 : 1);
   a.off += sizeof(uint32_t);
   L.zero_begin = L.sort.hist;
-  L.zero_words = (size_t)(L.start_count - L.sort.hist) + 1;
+  L.zero_words = (size_t)(L.treelet_count - L.sort.hist) + 1;
   L.sort.zero_words = 0;
-  L.starts = a.take<uint32_t>(2 * (size_t)n);
-  L.node2end = a.take<uint32_t>(n > 1 ? n - 1 : 1);
+  L.treelets = a.take<uint4>(n);
   L.cntr = tri ? a.take<float>((size_t)n * 3) : nullptr;
   L.codes = a.take<KeyT>(n);
   L.idx = a.take<uint32_t>(n);
@@ -236,10 +234,11 @@ inline int build(hipStream_t s, uint32_t n, const uint32_t* tri2vtx, uint32_t nu
   // P is even for 30- and 63-bit codes: the passes end back in `codes`.
   rc = sortk::run_passes<KeyT>(s, n, kb, L.sort, codes, codes, L.idx, err);
   if (rc != DMSH_OK) return rc;
-  rc = launch_topology<KeyT>(s, n, bvhnodes, codes, L.idx, L.node2end);
+  rc = launch_topology<KeyT>(s, n, bvhnodes, codes, L.idx,
+                            TreeletSink{kTreeletLeaves, L.treelet_count, L.treelets, n});
   if (rc != DMSH_OK) return rc;
-  return launch_refit_two_phase<TRI>(s, aabb, L.flags, L.start_count, L.starts, n, bvhnodes,
-                                     L.node2end, tri2vtx, num_vtx, xyz, 0.f, err);
+  return launch_refit_treelets<TRI>(s, aabb, L.flags, L.treelet_count, L.treelets, n, bvhnodes,
+                                    tri2vtx, num_vtx, xyz, 0.f, err);
 }
 
 }  // namespace bld

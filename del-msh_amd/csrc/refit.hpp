@@ -17,6 +17,8 @@
 // enough on a weakly ordered multi-L2 part.
 #pragma once
 
+#include <algorithm>
+
 #include "dmsh_common.hpp"
 
 namespace dmsh {
@@ -154,120 +156,108 @@ __device__ __forceinline__ bool leaf_box(const uint32_t* __restrict__ bvhnodes, 
 }
 
 // ---------------------------------------------------------------------------
-// Two-kernel refit for trees whose internal-node ranges are known (node2end
-// from the topology kernel).
+// Treelet refit for LBVHs built here (the topology kernel lists the
+// treelets: subtrees of at most K leaves whose parent has more than K).
 //
-// k_refit_local: workgroup b owns leaves [c0, c1) = [b*CHUNK, (b+1)*CHUNK).
-//   An internal node is local to the chunk iff its leaf range lies inside it;
-//   a local node's id is then also in [c0, c1), so the chunk's node rows,
-//   range ends, arrival counters and boxes all fit in LDS.  Phase 1 loads
-//   them with coalesced reads and builds the leaf boxes; phase 2 climbs from
-//   every leaf through local nodes with LDS counters (second arrival merges,
-//   left child first); phase 3 writes the local internal boxes back coalesced.
-//   A climb that reaches a parent outside the chunk records its node in a
-//   start list and stops.
-// k_refit_top: a single workgroup climbs from the recorded starts through the
-//   remaining (chunk-crossing) nodes with global arrival counters.  All of
-//   those climbs run on one CU, so workgroup-scope acq_rel RMWs order the box
-//   stores and loads (one L1, no cross-XCD traffic) and no L2 write-back is
-//   needed.
-template <bool TRI, int CHUNK, int T>
-__global__ __launch_bounds__(T) void k_refit_local(
+// k_refit_treelet: a persistent grid; each workgroup takes whole treelets.
+//   A treelet over leaves [f, l] keeps its internal nodes at ids inside
+//   [f, l], so its node rows, arrival counters and boxes fit in LDS slots
+//   (id - f).  Phase 1 builds the leaf boxes (stored to HBM at once, in leaf
+//   order) and loads the node rows with coalesced reads; phase 2 climbs from
+//   every leaf with LDS counters (the second arrival merges, left child
+//   first) up to the treelet root; phase 3 writes the internal boxes back in
+//   id order.  Every node is written once and no global atomics are used.
+// k_refit_top: one workgroup climbs from the treelet roots through the nodes
+//   above them (at most ~2n/K of them for a balanced tree).  Those climbs all
+//   run on one CU, so workgroup-scope acq_rel RMWs on the global counters
+//   order the box stores and loads (one L1, no cross-XCD traffic) and no L2
+//   write-back is needed.
+template <bool TRI, int K, int T>
+__global__ __launch_bounds__(T) void k_refit_treelet(
     float* __restrict__ aabb, uint32_t num_leaf, const uint32_t* __restrict__ bvhnodes,
-    const uint32_t* __restrict__ node2end, const uint32_t* __restrict__ tri2vtx, uint32_t num_vtx,
-    const float* __restrict__ xyz, float eps, uint32_t* __restrict__ start_count,
-    uint32_t* __restrict__ starts, uint32_t* __restrict__ err) {
-  constexpr int LPT = CHUNK / T;
-  __shared__ Box s_leaf[CHUNK];
-  __shared__ Box s_inner[CHUNK];
-  __shared__ uint32_t s_flag[CHUNK];
-  __shared__ uint32_t s_end[CHUNK];
-  __shared__ uint2 s_child[CHUNK];
-  __shared__ uint32_t s_parent[CHUNK];    // parent of internal node c0 + j
-  __shared__ uint32_t s_lparent[CHUNK];   // parent of leaf c0 + j
+    const uint32_t* __restrict__ tri2vtx, uint32_t num_vtx, const float* __restrict__ xyz,
+    float eps, const uint32_t* __restrict__ treelet_count, const uint4* __restrict__ treelets,
+    uint32_t* __restrict__ err) {
+  constexpr int LPT = K / T;
+  __shared__ Box s_leaf[K];
+  __shared__ Box s_inner[K];
+  __shared__ uint32_t s_flag[K];
+  __shared__ uint2 s_child[K];
+  __shared__ uint32_t s_parent[K];
   const uint32_t nb = num_leaf - 1;
-  const uint32_t c0 = blockIdx.x * (uint32_t)CHUNK;
-  const uint32_t c1 = (c0 + CHUNK < num_leaf) ? c0 + CHUNK : num_leaf;
-  // phase 1
-#pragma unroll
-  for (int r = 0; r < LPT; ++r) {
-    const uint32_t j = (uint32_t)r * T + threadIdx.x;
-    const uint32_t id = c0 + j;
-    s_flag[j] = 0u;
-    if (id < nb) {
-      const uint32_t* row = bvhnodes + (size_t)id * 3;
-      s_parent[j] = row[0];
-      s_child[j] = make_uint2(row[1], row[2]);
-      s_end[j] = node2end[id];
+  uint32_t count = *treelet_count;
+  if (count > num_leaf) count = num_leaf;
+  for (uint32_t t = blockIdx.x; t < count; t += gridDim.x) {
+    const uint4 tl = treelets[t];
+    const uint32_t root = tl.x, f = tl.y, l = tl.z;
+    if (f > l || l >= num_leaf || l - f >= (uint32_t)K) {  // uniform over the workgroup
+      if (threadIdx.x == 0) raise_error(err, DMSH_DEVERR_REFIT);
+      continue;
     }
-    if (id < num_leaf) {
+    const uint32_t m = l - f + 1;
+    uint32_t lpar[LPT];
+    // phase 1
+#pragma unroll
+    for (int r = 0; r < LPT; ++r) {
+      const uint32_t j = (uint32_t)r * T + threadIdx.x;
+      lpar[r] = DMSH_U32MAX - 1;  // no climb
+      if (j >= m) continue;
+      const uint32_t id = f + j;
+      s_flag[j] = 0u;
+      if (id < nb) {
+        const uint32_t* row = bvhnodes + (size_t)id * 3;
+        s_parent[j] = row[0];
+        s_child[j] = make_uint2(row[1], row[2]);
+      }
       Box b;
       uint32_t p;
       if (leaf_box<TRI>(bvhnodes, num_leaf, id, tri2vtx, num_vtx, xyz, eps, b, p, err)) {
         s_leaf[j] = b;
-        s_lparent[j] = p;
         box_store(aabb, nb + id, b);
-      } else {
-        s_lparent[j] = DMSH_U32MAX - 1;  // marks a bad leaf: no climb
+        if (nb + id != root) lpar[r] = p;
       }
     }
-  }
-  __syncthreads();
-  // phase 2
+    __syncthreads();
+    // phase 2
 #pragma unroll
-  for (int r = 0; r < LPT; ++r) {
-    const uint32_t j = (uint32_t)r * T + threadIdx.x;
-    const uint32_t k = c0 + j;
-    if (k >= num_leaf) continue;
-    uint32_t parent = s_lparent[j];
-    if (parent == DMSH_U32MAX - 1) continue;
-    uint32_t node = nb + k;
-    Box mine = s_leaf[j];
-    for (uint32_t steps = 0;; ++steps) {
-      if (parent == DMSH_U32MAX) break;  // reached the root
-      if (parent >= nb) {
-        raise_error(err, DMSH_DEVERR_REFIT);
-        break;
+    for (int r = 0; r < LPT; ++r) {
+      const uint32_t j = (uint32_t)r * T + threadIdx.x;
+      uint32_t parent = lpar[r];
+      if (parent == DMSH_U32MAX - 1) continue;
+      uint32_t node = nb + f + j;
+      Box mine = s_leaf[j];
+      for (uint32_t steps = 0;; ++steps) {
+        if (parent < f || parent > l || parent >= nb || steps >= (uint32_t)K) {
+          raise_error(err, DMSH_DEVERR_REFIT);
+          break;
+        }
+        const uint32_t pj = parent - f;
+        const uint32_t old =
+            __hip_atomic_fetch_add(&s_flag[pj], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
+        if (old == 0) break;
+        const uint2 ch = s_child[pj];
+        const uint32_t sib = (ch.x == node) ? ch.y : ch.x;
+        const uint32_t si = (sib >= nb) ? sib - nb - f : sib - f;
+        if ((ch.x != node && ch.y != node) || si >= m) {
+          raise_error(err, DMSH_DEVERR_REFIT);
+          break;
+        }
+        const Box sb = (sib >= nb) ? s_leaf[si] : s_inner[si];
+        mine = (ch.x == node) ? box_merge(mine, sb) : box_merge(sb, mine);
+        s_inner[pj] = mine;
+        if (parent == root) break;
+        node = parent;
+        parent = s_parent[pj];
       }
-      const bool in_chunk = parent >= c0 && parent < c1;
-      uint32_t lo = 0, hi = 0;
-      if (in_chunk) {
-        const uint32_t e = s_end[parent - c0];
-        lo = parent < e ? parent : e;
-        hi = parent < e ? e : parent;
-      }
-      if (!in_chunk || lo < c0 || hi >= c1) {
-        starts[atomicAdd(start_count, 1u)] = node;
-        break;
-      }
-      if (steps >= (uint32_t)CHUNK) {
-        raise_error(err, DMSH_DEVERR_REFIT);
-        break;
-      }
-      const uint32_t pj = parent - c0;
-      const uint32_t old =
-          __hip_atomic_fetch_add(&s_flag[pj], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
-      if (old == 0) break;
-      const uint2 ch = s_child[pj];
-      const uint32_t sib = (ch.x == node) ? ch.y : ch.x;
-      const uint32_t si = (sib >= nb) ? sib - nb - c0 : sib - c0;
-      if ((ch.x != node && ch.y != node) || si >= (uint32_t)CHUNK) {
-        raise_error(err, DMSH_DEVERR_REFIT);
-        break;
-      }
-      const Box sb = (sib >= nb) ? s_leaf[si] : s_inner[si];
-      mine = (ch.x == node) ? box_merge(mine, sb) : box_merge(sb, mine);
-      s_inner[pj] = mine;
-      node = parent;
-      parent = s_parent[pj];
     }
-  }
-  __syncthreads();
-  // phase 3: nodes that received both arrivals were computed here
+    __syncthreads();
+    // phase 3: the nodes that received both arrivals are this treelet's
 #pragma unroll
-  for (int r = 0; r < LPT; ++r) {
-    const uint32_t j = (uint32_t)r * T + threadIdx.x;
-    if (c0 + j < nb && s_flag[j] >= 2u) box_store(aabb, c0 + j, s_inner[j]);
+    for (int r = 0; r < LPT; ++r) {
+      const uint32_t j = (uint32_t)r * T + threadIdx.x;
+      if (j < m && f + j < nb && s_flag[j] >= 2u) box_store(aabb, f + j, s_inner[j]);
+    }
+    __syncthreads();  // LDS is reused by the next treelet
   }
 }
 
@@ -275,14 +265,14 @@ template <int T>
 __global__ __launch_bounds__(T) void k_refit_top(float* __restrict__ aabb,
                                                  uint32_t* __restrict__ flags, uint32_t num_leaf,
                                                  const uint32_t* __restrict__ bvhnodes,
-                                                 const uint32_t* __restrict__ start_count,
-                                                 const uint32_t* __restrict__ starts,
+                                                 const uint32_t* __restrict__ treelet_count,
+                                                 const uint4* __restrict__ treelets,
                                                  uint32_t* __restrict__ err) {
   const uint32_t nb = num_leaf - 1;
-  uint32_t count = *start_count;
-  if (count > 2 * num_leaf) count = 2 * num_leaf;
+  uint32_t count = *treelet_count;
+  if (count > num_leaf) count = num_leaf;
   for (uint32_t e = threadIdx.x; e < count; e += T) {
-    uint32_t node = starts[e];
+    uint32_t node = treelets[e].x;
     if (node > 2 * nb) {
       raise_error(err, DMSH_DEVERR_REFIT);
       continue;
@@ -311,21 +301,28 @@ __global__ __launch_bounds__(T) void k_refit_top(float* __restrict__ aabb,
   }
 }
 
+constexpr uint32_t kTreeletLeaves = 1024;  // K
+constexpr int kTreeletThreads = 256;
+constexpr uint32_t kTreeletGrid = 1024;    // persistent workgroups
+
+// flags: num_leaf - 1 zeroed counters; treelets as listed by the topology
+// kernel with k = kTreeletLeaves.
 template <bool TRI>
-inline int launch_refit_two_phase(hipStream_t s, float* aabb, uint32_t* flags,
-                                  uint32_t* start_count, uint32_t* starts, uint32_t num_leaf,
-                                  const uint32_t* bvhnodes, const uint32_t* node2end,
-                                  const uint32_t* tri2vtx, uint32_t num_vtx, const float* xyz,
-                                  float eps, uint32_t* err) {
-  constexpr int kChunk = 2048, kThreads = 1024, kTopThreads = 1024;
+inline int launch_refit_treelets(hipStream_t s, float* aabb, uint32_t* flags,
+                                 const uint32_t* treelet_count, const uint4* treelets,
+                                 uint32_t num_leaf, const uint32_t* bvhnodes,
+                                 const uint32_t* tri2vtx, uint32_t num_vtx, const float* xyz,
+                                 float eps, uint32_t* err) {
+  constexpr int kTopThreads = 1024;
   if (num_leaf == 0) return DMSH_OK;
-  launch("k_refit_local", (k_refit_local<TRI, kChunk, kThreads>), dim3(ceil_div(num_leaf, kChunk)),
-                     dim3(kThreads), 0, s, aabb, num_leaf, bvhnodes, node2end, tri2vtx, num_vtx,
-                     xyz, eps, start_count, starts, err);
+  const uint32_t grid = std::min<uint32_t>(kTreeletGrid, num_leaf);
+  launch("k_refit_treelet", (k_refit_treelet<TRI, (int)kTreeletLeaves, kTreeletThreads>),
+         dim3(grid), dim3(kTreeletThreads), 0, s, aabb, num_leaf, bvhnodes, tri2vtx, num_vtx, xyz,
+         eps, treelet_count, treelets, err);
   int rc = launch_status();
-  if (rc != DMSH_OK || num_leaf <= 1) return rc;
+  if (rc != DMSH_OK || num_leaf <= kTreeletLeaves) return rc;
   launch("k_refit_top", k_refit_top<kTopThreads>, dim3(1), dim3(kTopThreads), 0, s, aabb, flags,
-                     num_leaf, bvhnodes, start_count, starts, err);
+         num_leaf, bvhnodes, treelet_count, treelets, err);
   return launch_status();
 }
 

@@ -85,20 +85,45 @@ __device__ __forceinline__ uint32_t node_split(const KeyT* __restrict__ c, uint3
   return split;
 }
 
-// node2end (nullable): for internal node i, the end of its leaf range that is
-// not i itself (node i always sits at one end of its range).
+// Treelets for the refit (optional): a treelet is a subtree of at most k
+// leaves whose parent has more than k (or the whole tree when n <= k).  A
+// treelet covering leaves [f, l] holds exactly the internal nodes of its
+// subtree, and every one of them has an id inside [f, l] (a Karras node sits
+// at one end of its own range), so one workgroup can refit it in LDS.
+// Entries are (node id, f, l, 0), appended in any order; at most n of them.
+struct TreeletSink {
+  uint32_t k;
+  uint32_t* count;  // zeroed by the caller
+  uint4* list;      // nullptr: no treelets wanted
+  uint32_t cap;
+};
+
+__device__ __forceinline__ void emit_treelet(const TreeletSink& ts, uint32_t node, uint32_t f,
+                                             uint32_t l) {
+  const uint32_t slot = atomicAdd(ts.count, 1u);
+  if (slot < ts.cap) ts.list[slot] = make_uint4(node, f, l, 0u);
+}
+
 template <typename KeyT>
 __global__ __launch_bounds__(256) void k_topology(uint32_t n, uint32_t* __restrict__ bvhnodes,
                                                   const KeyT* __restrict__ codes,
                                                   const uint32_t* __restrict__ idx2obj,
-                                                  uint32_t* __restrict__ node2end) {
+                                                  TreeletSink ts) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t nb = n - 1;
   if (i >= nb) return;
   uint32_t first, last;
   node_range<KeyT>(codes, n, i, first, last);
-  if (node2end) node2end[i] = (first == i) ? last : first;
   const uint32_t split = node_split<KeyT>(codes, first, last);
+  if (ts.list) {
+    if (last - first + 1 > ts.k) {
+      if (split - first + 1 <= ts.k) emit_treelet(ts, first == split ? nb + split : split, first, split);
+      if (last - split <= ts.k)
+        emit_treelet(ts, last == split + 1 ? nb + split + 1 : split + 1, split + 1, last);
+    } else if (i == 0) {
+      emit_treelet(ts, 0u, 0u, n - 1);
+    }
+  }
   if (first == split) {
     const uint32_t a = nb + split;
     bvhnodes[(size_t)i * 3 + 1] = a;
@@ -123,21 +148,23 @@ __global__ __launch_bounds__(256) void k_topology(uint32_t n, uint32_t* __restri
 }
 
 static __global__ void k_topology_single(uint32_t* __restrict__ bvhnodes,
-                                  const uint32_t* __restrict__ idx2obj) {
+                                         const uint32_t* __restrict__ idx2obj, TreeletSink ts) {
   bvhnodes[0] = DMSH_U32MAX;
   bvhnodes[1] = idx2obj[0];
   bvhnodes[2] = DMSH_U32MAX;
+  if (ts.list) emit_treelet(ts, 0u, 0u, 0u);
 }
 
 template <typename KeyT>
 inline int launch_topology(hipStream_t s, uint32_t n, uint32_t* bvhnodes, const KeyT* codes,
-                           const uint32_t* idx2obj, uint32_t* node2end = nullptr) {
+                           const uint32_t* idx2obj, TreeletSink ts = TreeletSink{0, nullptr,
+                                                                                 nullptr, 0}) {
   if (n == 0) return DMSH_OK;
   if (n == 1) {
-    launch("k_topology_single", k_topology_single, dim3(1), dim3(1), 0, s, bvhnodes, idx2obj);
+    launch("k_topology_single", k_topology_single, dim3(1), dim3(1), 0, s, bvhnodes, idx2obj, ts);
   } else {
     launch("k_topology", k_topology<KeyT>, dim3(ceil_div(n - 1, 256)), dim3(256), 0, s, n, bvhnodes,
-                       codes, idx2obj, node2end);
+           codes, idx2obj, ts);
   }
   return launch_status();
 }
