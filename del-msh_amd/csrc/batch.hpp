@@ -1,0 +1,503 @@
+// batch.hpp -- LBVH build and ray-cast of many meshes of uneven size, each
+// stage one launch over all meshes (CSR offsets), no per-mesh launches.
+//
+// Meshes are concatenated: triangles of mesh m are [mesh2tri[m], mesh2tri[m+1])
+// with vertex indices local to the mesh's vertices [mesh2vtx[m], mesh2vtx[m+1]).
+// Each mesh's tree is exactly the tree the single-mesh build gives it (30-bit
+// codes): per-mesh centroid boxes and unit-cube fits, then one stable radix
+// sort of composite keys (mesh << 30 | code) -- stable by (mesh, code, index),
+// i.e. every mesh's segment is its own stable sort -- then the Karras
+// topology and the treelet refit run on each segment with local indices.
+// Output rows of mesh m start at node_off[m] = sum_{j<m} (2 N_j - 1).
+//
+// The ray-cast hands 8x8 pixel tiles of all meshes to a persistent grid of
+// waves through per-XCD counters (each XCD starts on its own contiguous run of
+// tiles and steals from the others when it runs out), so meshes of every size
+// share the chip evenly and neighbouring tiles stay in one L2.
+#pragma once
+
+#include "build.hpp"
+#include "raycast.hpp"
+
+namespace dmsh {
+namespace bat {
+
+constexpr uint32_t kMeshKeyShift = 30;
+constexpr uint32_t kXcds = 8;
+
+// The segment holding element x (x < off[M]): the last m with off[m] <= x,
+// which skips empty segments (repeated offsets).
+__device__ __forceinline__ uint32_t find_segment(const uint32_t* __restrict__ off, uint32_t M,
+                                                 uint32_t x) {
+  uint32_t lo = 0, hi = M;  // off[lo] <= x < off[hi]
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (off[mid] <= x) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// Exclusive scan over meshes of value(m) into out[0..M] (out[M] = total), by
+// one workgroup of 1024 threads.
+template <class F>
+__device__ __forceinline__ void scan_meshes(uint32_t M, uint32_t* __restrict__ out, F value) {
+  __shared__ uint32_t s_w[16];
+  __shared__ uint32_t s_carry;
+  if (threadIdx.x == 0) s_carry = 0;
+  __syncthreads();
+  for (uint32_t base = 0; base <= M; base += 1024) {
+    const uint32_t i = base + threadIdx.x;
+    const uint32_t v = i < M ? value(i) : 0u;
+    uint32_t inc = v;
+    const uint32_t lane = threadIdx.x & 63u;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const uint32_t u = __shfl_up(inc, d, 64);
+      if (lane >= (uint32_t)d) inc += u;
+    }
+    if (lane == 63u) s_w[threadIdx.x >> 6] = inc;
+    __syncthreads();
+    uint32_t pre = s_carry;
+    for (uint32_t w = 0; w < (threadIdx.x >> 6); ++w) pre += s_w[w];
+    if (i <= M) out[i] = pre + inc - v;
+    __syncthreads();
+    if (threadIdx.x == 1023) s_carry = pre + inc;
+    __syncthreads();
+  }
+}
+
+__device__ __forceinline__ uint32_t seg_len(const uint32_t* __restrict__ off, uint32_t m) {
+  return off[m + 1] - off[m];
+}
+
+// An offset array fits its data: first entry 0, non-decreasing, last entry
+// `total`.  Otherwise *bad is set (every later kernel of the call then does
+// nothing) and DMSH_DEVERR_INDEX is raised.
+__device__ __forceinline__ void check_offsets(const uint32_t* __restrict__ off, uint32_t M,
+                                              uint32_t total, uint32_t* __restrict__ bad,
+                                              uint32_t* __restrict__ err) {
+  bool ok = true;
+  for (uint32_t m = threadIdx.x; m < M; m += blockDim.x) ok = ok && off[m + 1] >= off[m];
+  if (threadIdx.x == 0) ok = ok && off[0] == 0u && off[M] == total;
+  if (!ok) {
+    atomicOr(bad, 1u);
+    raise_error(err, DMSH_DEVERR_INDEX);
+  }
+}
+
+// inner_off[m] = sum max(N_j - 1, 0); node_off[m] = sum max(2 N_j - 1, 0).
+static __global__ __launch_bounds__(1024) void k_batch_offsets(
+    uint32_t M, uint32_t total_tri, uint32_t total_vtx, const uint32_t* __restrict__ mesh2tri,
+    const uint32_t* __restrict__ mesh2vtx, uint32_t* __restrict__ inner_off,
+    uint32_t* __restrict__ node_off, uint32_t* __restrict__ bad, uint32_t* __restrict__ err) {
+  check_offsets(mesh2tri, M, total_tri, bad, err);
+  check_offsets(mesh2vtx, M, total_vtx, bad, err);
+  scan_meshes(M, inner_off, [&](uint32_t m) {
+    const uint32_t n = mesh2tri[m + 1] - mesh2tri[m];
+    return n > 0 ? n - 1 : 0u;
+  });
+  if (node_off) {
+    scan_meshes(M, node_off, [&](uint32_t m) {
+      const uint32_t n = mesh2tri[m + 1] - mesh2tri[m];
+      return n > 0 ? 2 * n - 1 : 0u;
+    });
+  }
+}
+
+// keys6 of every mesh (min keys 0xFFFFFFFF, max keys 0), then zero words.
+static __global__ __launch_bounds__(256) void k_batch_init(uint32_t M, uint32_t* __restrict__ keys6,
+                                                           uint32_t* __restrict__ zero,
+                                                           size_t zero_count) {
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  const size_t i0 = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  for (size_t i = i0; i < (size_t)M * 6; i += stride) keys6[i] = (i % 6 < 3) ? 0xFFFFFFFFu : 0u;
+  for (size_t i = i0; i < zero_count; i += stride) zero[i] = 0u;
+}
+
+// Triangle centroids ((0 + p0 + p1 + p2) * (1/3)) and per-mesh centroid
+// boxes through order keys.  A wave whose triangles all lie in one mesh
+// reduces first and issues 6 atomics; a wave across a mesh boundary lets each
+// lane issue its own.
+static __global__ __launch_bounds__(256) void k_batch_centroid(
+    const uint32_t* __restrict__ bad, uint32_t total_tri, uint32_t M,
+    const uint32_t* __restrict__ mesh2tri, const uint32_t* __restrict__ mesh2vtx,
+    const uint32_t* __restrict__ tri2vtx, const float* __restrict__ xyz, float* __restrict__ cntr,
+    uint32_t* __restrict__ keys6, uint32_t* __restrict__ err) {
+  if (*bad) return;
+  const uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t base = blockIdx.x * blockDim.x; base < total_tri; base += stride) {
+    const uint32_t t = base + threadIdx.x;
+    const bool valid = t < total_tri;
+    uint32_t m = 0;
+    uint32_t kmin[3] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu}, kmax[3] = {0u, 0u, 0u};
+    if (valid) {
+      m = find_segment(mesh2tri, M, t);
+      const uint32_t vb = mesh2vtx[m], nv = mesh2vtx[m + 1] - vb;
+      const uint32_t* tv = tri2vtx + (size_t)t * 3;
+      uint32_t v[3] = {tv[0], tv[1], tv[2]};
+      float c[3] = {0.f, 0.f, 0.f};
+      if (v[0] >= nv || v[1] >= nv || v[2] >= nv) {
+        raise_error(err, DMSH_DEVERR_INDEX);
+      } else {
+        const float ratio = 1.f / 3.f;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          float cog = 0.f;
+          cog += xyz[((size_t)vb + v[0]) * 3 + d];
+          cog += xyz[((size_t)vb + v[1]) * 3 + d];
+          cog += xyz[((size_t)vb + v[2]) * 3 + d];
+          c[d] = cog * ratio;
+        }
+      }
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        cntr[(size_t)t * 3 + d] = c[d];
+        kmin[d] = kmax[d] = f32_order_key(c[d]);
+      }
+    }
+    const uint32_t m0 = __builtin_amdgcn_readfirstlane(m);
+    if (__all(!valid || m == m0)) {
+#pragma unroll
+      for (int off = 32; off >= 1; off >>= 1) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          const uint32_t a = __shfl_xor(kmin[d], off, 64), b = __shfl_xor(kmax[d], off, 64);
+          kmin[d] = a < kmin[d] ? a : kmin[d];
+          kmax[d] = b > kmax[d] ? b : kmax[d];
+        }
+      }
+      if ((threadIdx.x & 63u) == 0 && base + (threadIdx.x & ~63u) < total_tri) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          atomicMin(keys6 + (size_t)m0 * 6 + d, kmin[d]);
+          atomicMax(keys6 + (size_t)m0 * 6 + 3 + d, kmax[d]);
+        }
+      }
+    } else if (valid) {
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        atomicMin(keys6 + (size_t)m * 6 + d, kmin[d]);
+        atomicMax(keys6 + (size_t)m * 6 + 3 + d, kmax[d]);
+      }
+    }
+  }
+}
+
+// Composite key (mesh << 30 | 30-bit code under the mesh's unit-cube fit) and
+// the digit histograms of every sort pass.
+static __global__ __launch_bounds__(256) void k_batch_morton_hist(
+    const uint32_t* __restrict__ bad, uint32_t total_tri, uint32_t M,
+    const uint32_t* __restrict__ mesh2tri, const float* __restrict__ cntr,
+    const uint32_t* __restrict__ keys6, uint64_t* __restrict__ codes, uint32_t num_passes,
+    uint32_t key_bits, uint32_t* __restrict__ hist) {
+  if (*bad) return;  // the sort then orders whatever the keys hold, in bounds
+  __shared__ uint32_t s_h[sortk::kMaxPasses][sortk::kRadix];
+  for (uint32_t e = threadIdx.x; e < sortk::kMaxPasses * sortk::kRadix; e += blockDim.x)
+    (&s_h[0][0])[e] = 0;
+  __syncthreads();
+  for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < total_tri;
+       t += gridDim.x * blockDim.x) {
+    const uint32_t m = find_segment(mesh2tri, M, t);
+    float T[16];
+    bld::fit_transform(keys6 + (size_t)m * 6, T);
+    const float* p = cntr + (size_t)t * 3;
+    const uint32_t code = morton3_transformed<uint32_t>(T, p[0], p[1], p[2]);
+    const uint64_t key = ((uint64_t)m << kMeshKeyShift) | code;
+    codes[t] = key;
+    sortk::hist_accumulate(s_h, key, num_passes, key_bits);
+  }
+  __syncthreads();
+  sortk::hist_flush(s_h, num_passes, hist);
+}
+
+// Internal node g of all meshes (g < inner_off[M]).
+static __global__ __launch_bounds__(256) void k_batch_topology(
+    const uint32_t* __restrict__ bad, uint32_t total_tri, uint32_t M,
+    const uint32_t* __restrict__ mesh2tri, const uint32_t* __restrict__ inner_off,
+    const uint32_t* __restrict__ node_off, uint32_t* __restrict__ bvhnodes,
+    const uint64_t* __restrict__ codes, const uint32_t* __restrict__ idx, TreeletSink ts) {
+  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (*bad || g >= total_tri || g >= inner_off[M]) return;
+  const uint32_t m = find_segment(inner_off, M, g);
+  const uint32_t tb = mesh2tri[m], n = mesh2tri[m + 1] - tb;
+  topology_node<uint64_t>(n, g - inner_off[m], bvhnodes + (size_t)node_off[m] * 3, codes + tb,
+                          idx + tb, tb, ts, m, tb);
+}
+
+// Meshes of one triangle: the single leaf row and its treelet.
+static __global__ __launch_bounds__(256) void k_batch_singles(
+    const uint32_t* __restrict__ bad, uint32_t M, const uint32_t* __restrict__ mesh2tri,
+    const uint32_t* __restrict__ node_off, uint32_t* __restrict__ bvhnodes,
+    const uint32_t* __restrict__ idx, TreeletSink ts) {
+  const uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (*bad || m >= M) return;
+  const uint32_t tb = mesh2tri[m];
+  if (mesh2tri[m + 1] - tb != 1) return;
+  topology_single(bvhnodes + (size_t)node_off[m] * 3, idx[tb] - tb, ts, m, tb);
+}
+
+struct BatchTrees {
+  const uint32_t* bad;  // nonzero: the offsets were rejected, do nothing
+  uint32_t M;
+  const uint32_t* mesh2tri;
+  const uint32_t* mesh2vtx;
+  const uint32_t* node_off;
+  float* aabb;
+  const uint32_t* bvhnodes;
+  const uint32_t* tri2vtx;
+  const float* xyz;
+};
+
+__device__ __forceinline__ TreeView tree_of(const BatchTrees& b, uint32_t m) {
+  const uint32_t tb = b.mesh2tri[m], vb = b.mesh2vtx[m];
+  return TreeView{b.aabb + (size_t)b.node_off[m] * 6, b.bvhnodes + (size_t)b.node_off[m] * 3,
+                  b.mesh2tri[m + 1] - tb, b.tri2vtx + (size_t)tb * 3, b.mesh2vtx[m + 1] - vb,
+                  b.xyz + (size_t)vb * 3};
+}
+
+template <int K, int T>
+__global__ __launch_bounds__(T) void k_batch_refit_treelet(BatchTrees b, uint32_t total_tri,
+                                                           const uint32_t* __restrict__ count_p,
+                                                           const uint4* __restrict__ treelets,
+                                                           uint32_t* __restrict__ err) {
+  __shared__ TreeletLds<K> s;
+  if (*b.bad) return;
+  uint32_t count = *count_p;
+  if (count > total_tri) count = total_tri;
+  for (uint32_t t = blockIdx.x; t < count; t += gridDim.x) {
+    const uint4 tl = treelets[t];
+    if (tl.w >= b.M) {
+      if (threadIdx.x == 0) raise_error(err, DMSH_DEVERR_REFIT);
+      continue;
+    }
+    refit_treelet<true, K, T>(tree_of(b, tl.w), tl.x, tl.y, tl.z, 0.f, s, err);
+  }
+}
+
+// One workgroup per mesh with more than K triangles (grid-stride over meshes).
+template <int K, int T>
+__global__ __launch_bounds__(T) void k_batch_refit_top(BatchTrees b,
+                                                       const uint32_t* __restrict__ inner_off,
+                                                       uint32_t* __restrict__ flags,
+                                                       const uint32_t* __restrict__ root_count,
+                                                       const uint32_t* __restrict__ roots,
+                                                       uint32_t* __restrict__ err) {
+  if (*b.bad) return;
+  for (uint32_t m = blockIdx.x; m < b.M; m += gridDim.x) {
+    const uint32_t n = seg_len(b.mesh2tri, m);
+    if (n <= (uint32_t)K) continue;
+    uint32_t count = root_count[m];
+    if (count > n) count = n;
+    refit_top<T>(tree_of(b, m), flags + inner_off[m], roots + b.mesh2tri[m], 1u, count, err);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// ray-cast
+// ---------------------------------------------------------------------------
+
+// inner_off as for the build, tile_off[m] = sum of 8x8 tiles of the images
+// before m.  Rejects (sets *bad, raises DMSH_DEVERR_INDEX) offsets that do not
+// fit their arrays, a node offset that is not the build's, and an image
+// whose pixel count is not a multiple of its width.
+static __global__ __launch_bounds__(1024) void k_batch_cast_offsets(
+    uint32_t M, uint32_t total_tri, uint32_t total_vtx, uint32_t total_pix,
+    const uint32_t* __restrict__ mesh2tri, const uint32_t* __restrict__ mesh2vtx,
+    const uint32_t* __restrict__ node_off, const uint32_t* __restrict__ pix_off,
+    const uint32_t* __restrict__ widths, uint32_t* __restrict__ inner_off,
+    uint32_t* __restrict__ tile_off, uint32_t* __restrict__ bad, uint32_t* __restrict__ err) {
+  check_offsets(mesh2tri, M, total_tri, bad, err);
+  check_offsets(mesh2vtx, M, total_vtx, bad, err);
+  check_offsets(pix_off, M, total_pix, bad, err);
+  for (uint32_t m = threadIdx.x; m < M; m += 1024) {
+    const uint32_t n = mesh2tri[m + 1] - mesh2tri[m];
+    const uint32_t p = pix_off[m + 1] - pix_off[m], w = widths[m];
+    const bool rows_ok = node_off[m + 1] - node_off[m] == (n > 0 ? 2 * n - 1 : 0u);
+    const bool img_ok = p == 0 || (w != 0 && p % w == 0);
+    if (!rows_ok || !img_ok) {
+      atomicOr(bad, 1u);
+      raise_error(err, DMSH_DEVERR_INDEX);
+    }
+  }
+  if (threadIdx.x == 0 && node_off[0] != 0u) {
+    atomicOr(bad, 1u);
+    raise_error(err, DMSH_DEVERR_INDEX);
+  }
+  scan_meshes(M, inner_off, [&](uint32_t m) {
+    const uint32_t n = mesh2tri[m + 1] - mesh2tri[m];
+    return n > 0 ? n - 1 : 0u;
+  });
+  scan_meshes(M, tile_off, [&](uint32_t m) {
+    const uint32_t p = pix_off[m + 1] - pix_off[m], w = widths[m];
+    if (p == 0 || w == 0 || p % w != 0) return 0u;
+    return ceil_div_u32(w, 8u) * ceil_div_u32(p / w, 8u);
+  });
+}
+
+static __global__ __launch_bounds__(256) void k_batch_pack_inner(
+    BatchTrees b, uint32_t total_tri, const uint32_t* __restrict__ inner_off,
+    rc::InnerNode* __restrict__ inner, uint32_t* __restrict__ nonfinite,
+    uint32_t* __restrict__ err) {
+  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (*b.bad || g >= total_tri || g >= inner_off[b.M]) return;
+  const uint32_t m = find_segment(inner_off, b.M, g);
+  const TreeView v = tree_of(b, m);
+  rc::pack_inner(inner + inner_off[m], v.bvhnodes, v.aabb, v.num_leaf, g - inner_off[m], nonfinite,
+                 err);
+}
+
+static __global__ __launch_bounds__(256) void k_batch_pack_leaf(BatchTrees b, uint32_t total_tri,
+                                                                rc::LeafTri* __restrict__ leaf,
+                                                                uint32_t* __restrict__ err) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (*b.bad || t >= total_tri) return;
+  const uint32_t m = find_segment(b.mesh2tri, b.M, t);
+  const TreeView v = tree_of(b, m);
+  const uint32_t tb = b.mesh2tri[m];
+  rc::pack_leaf(leaf + tb, v.bvhnodes, v.tri2vtx, v.xyz, v.num_leaf, v.num_leaf, v.num_vtx, t - tb,
+                err);
+}
+
+struct BatchCast {
+  uint32_t M;
+  const uint32_t* mesh2tri;
+  const uint32_t* node_off;
+  const uint32_t* inner_off;
+  const uint32_t* tile_off;
+  const uint32_t* pix_off;
+  const uint32_t* widths;
+  const rc::InnerNode* inner;
+  const rc::LeafTri* leaf;
+  const float* aabb;
+  const float* transforms;  // [M][16] ndc2world, column-major
+  uint32_t* pix2tri;
+  float* pix2t;
+  uint32_t* counts;  // [0] tier-2 queue, [1] tier-3 queue, [2] non-finite box flag,
+                     // [3] offsets rejected, [16 + 16 x] tile counter of XCD x
+  uint32_t* list2;
+  uint32_t* list3;
+  uint2* deep_stack;
+  uint32_t* err;
+};
+
+__device__ __forceinline__ rc::Packed packed_of(const BatchCast& a, uint32_t m) {
+  const uint32_t tb = a.mesh2tri[m];
+  return rc::Packed{a.inner + a.inner_off[m], a.leaf + tb, a.aabb + (size_t)a.node_off[m] * 6,
+                    a.mesh2tri[m + 1] - tb};
+}
+
+__device__ __forceinline__ void batch_write(const BatchCast& a, uint32_t gp, const rc::Hit& h) {
+  a.pix2tri[gp] = h.has ? h.tri : DMSH_U32MAX;
+  if (a.pix2t) a.pix2t[gp] = h.has ? h.t : 0.f;
+}
+
+// Next tile for the calling wave (the same value in every lane), or
+// num_tiles when none is left.
+__device__ __forceinline__ uint32_t next_tile(const BatchCast& a, uint32_t num_tiles,
+                                              uint32_t& xcd) {
+  const uint32_t per = (num_tiles + kXcds - 1) / kXcds;
+  for (uint32_t tries = 0; tries < kXcds; ++tries) {
+    const uint32_t lo = xcd * per;
+    const uint32_t hi = (lo + per < num_tiles) ? lo + per : num_tiles;
+    uint32_t t = 0;
+    if ((threadIdx.x & 63u) == 0) t = atomicAdd(a.counts + 16 + 16 * xcd, 1u);
+    t = __builtin_amdgcn_readfirstlane(t) + lo;
+    if (t < hi) return t;
+    xcd = (xcd + 1) % kXcds;
+  }
+  return num_tiles;
+}
+
+// Tier 1 over all meshes: persistent single-wave workgroups.
+template <int S>
+__global__ __launch_bounds__(64) void k_batch_pix2tri(BatchCast a) {
+  __shared__ uint2 s_stack[S][64];
+  const uint32_t lane = threadIdx.x;
+  if (a.counts[3]) return;  // offsets rejected
+  const uint32_t num_tiles = a.tile_off[a.M];
+  const bool boxes_finite = a.counts[2] == 0u;
+  uint32_t xcd = blockIdx.x % kXcds;
+  for (;;) {
+    const uint32_t tile = next_tile(a, num_tiles, xcd);
+    if (tile >= num_tiles) break;
+    const uint32_t m = find_segment(a.tile_off, a.M, tile);
+    const uint32_t w = a.widths[m], p0 = a.pix_off[m], h = (a.pix_off[m + 1] - p0) / w;
+    const uint32_t tiles_x = ceil_div_u32(w, 8u), lt = tile - a.tile_off[m];
+    const uint32_t iw = (lt % tiles_x) * 8u + (lane & 7u), ih = (lt / tiles_x) * 8u + (lane >> 3);
+    if (iw < w && ih < h) {
+      float T[16];
+      rc::load_transform(a.transforms + (size_t)m * 16, T);
+      float org[3], dir[3];
+      ray_for_pixel(iw, ih, w, h, T, org, dir);
+      rc::LdsStack<S> stk{&s_stack[0][lane], 64};
+      rc::Hit hit;
+      const int st = rc::trace(packed_of(a, m), org, dir, stk, hit, boxes_finite);
+      const uint32_t gp = p0 + ih * w + iw;
+      if (st == rc::kTraceOverflow) a.list2[atomicAdd(a.counts + 0, 1u)] = gp;
+      else batch_write(a, gp, hit);
+    }
+  }
+}
+
+__device__ __forceinline__ void batch_pixel(const BatchCast& a, uint32_t gp, uint32_t& m,
+                                            uint32_t& iw, uint32_t& ih, uint32_t& w,
+                                            uint32_t& h) {
+  m = find_segment(a.pix_off, a.M, gp);
+  w = a.widths[m];
+  const uint32_t p0 = a.pix_off[m];
+  h = (a.pix_off[m + 1] - p0) / w;
+  ih = (gp - p0) / w;
+  iw = gp - p0 - ih * w;
+}
+
+static __global__ __launch_bounds__(64) void k_batch_pix2tri_tier2(BatchCast a,
+                                                                   uint32_t total_pix) {
+  __shared__ uint2 s_stack[rc::kTier2Frames][64];
+  if (a.counts[3]) return;
+  uint32_t count = a.counts[0];
+  if (count > total_pix) count = total_pix;
+  const bool boxes_finite = a.counts[2] == 0u;
+  rc::LdsStack<rc::kTier2Frames> stk{&s_stack[0][threadIdx.x], 64};
+  for (uint32_t e = blockIdx.x * 64u + threadIdx.x; e < count; e += gridDim.x * 64u) {
+    const uint32_t gp = a.list2[e];
+    uint32_t m, iw, ih, w, h;
+    batch_pixel(a, gp, m, iw, ih, w, h);
+    float T[16], org[3], dir[3];
+    rc::load_transform(a.transforms + (size_t)m * 16, T);
+    ray_for_pixel(iw, ih, w, h, T, org, dir);
+    rc::Hit hit;
+    const int st = rc::trace(packed_of(a, m), org, dir, stk, hit, boxes_finite);
+    if (st == rc::kTraceOverflow) a.list3[atomicAdd(a.counts + 1, 1u)] = gp;
+    else batch_write(a, gp, hit);
+  }
+}
+
+static __global__ __launch_bounds__(64) void k_batch_pix2tri_tier3(BatchCast a,
+                                                                   uint32_t total_pix) {
+  const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t nthr = gridDim.x * blockDim.x;
+  if (a.counts[3]) return;
+  uint32_t count = a.counts[1];
+  if (count > total_pix) count = total_pix;
+  const bool boxes_finite = a.counts[2] == 0u;
+  rc::GlobalStack stk{a.deep_stack + (size_t)gid * rc::kDeepFrames};
+  for (uint32_t e = gid; e < count; e += nthr) {
+    const uint32_t gp = a.list3[e];
+    uint32_t m, iw, ih, w, h;
+    batch_pixel(a, gp, m, iw, ih, w, h);
+    float T[16], org[3], dir[3];
+    rc::load_transform(a.transforms + (size_t)m * 16, T);
+    ray_for_pixel(iw, ih, w, h, T, org, dir);
+    rc::Hit hit;
+    const int st = rc::trace(packed_of(a, m), org, dir, stk, hit, boxes_finite);
+    if (st != rc::kTraceOk) {
+      raise_error(a.err, DMSH_DEVERR_STACK);
+      hit.has = false;
+    }
+    batch_write(a, gp, hit);
+  }
+}
+
+}  // namespace bat
+}  // namespace dmsh

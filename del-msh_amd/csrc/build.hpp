@@ -235,7 +235,8 @@ inline int build(hipStream_t s, uint32_t n, const uint32_t* tri2vtx, uint32_t nu
   rc = sortk::run_passes<KeyT>(s, n, kb, L.sort, codes, codes, L.idx, err);
   if (rc != DMSH_OK) return rc;
   rc = launch_topology<KeyT>(s, n, bvhnodes, codes, L.idx,
-                            TreeletSink{kTreeletLeaves, L.treelet_count, L.treelets, n});
+                            TreeletSink{kTreeletLeaves, L.treelet_count, L.treelets, n, nullptr,
+                                        nullptr});
   if (rc != DMSH_OK) return rc;
   return launch_refit_treelets<TRI>(s, aabb, L.flags, L.treelet_count, L.treelets, n, bvhnodes,
                                     tri2vtx, num_vtx, xyz, 0.f, err);

@@ -1,0 +1,290 @@
+// dmsh_batch.hip -- C ABI: LBVH build and ray-cast of many meshes per launch.
+#include <algorithm>
+#include <atomic>
+#include <cstring>
+
+#include "batch.hpp"
+
+using namespace dmsh;
+
+namespace {
+
+constexpr uint32_t kCastCounts = 256;  // see bat::BatchCast::counts
+
+std::atomic<int> g_batch_stack{24};      // LDS frames per lane of the batched tier 1
+std::atomic<int> g_batch_wgs_per_cu{16};  // persistent single-wave workgroups per CU
+std::atomic<int> g_batch_treelet_grid{1024};
+
+// key bits of the composite (mesh << 30 | code): 30 + the bits of M - 1
+inline uint32_t batch_key_bits(uint32_t M) {
+  uint32_t mb = 0;
+  while (mb < 34 && ((uint64_t)1 << mb) < M) ++mb;
+  return bat::kMeshKeyShift + mb;
+}
+
+struct BatchBuildLayout {
+  uint32_t* keys6;  // [M][6]
+  sortk::Layout sort;
+  uint32_t* flags;          // [total_tri] top-climb counters (>= internal nodes)
+  uint32_t* treelet_count;  // 1
+  uint32_t* root_count;     // [M]
+  uint32_t* bad;            // 1: offsets rejected
+  uint32_t* zero_begin;
+  size_t zero_words;
+  uint32_t* inner_off;  // [M + 1]
+  uint4* treelets;      // [total_tri]
+  uint32_t* roots;      // [total_tri]
+  float* cntr;          // [total_tri][3]
+  uint64_t* codes;
+  uint32_t* idx;
+};
+
+inline bool carve_batch_build(Arena& a, uint32_t M, uint32_t n, BatchBuildLayout& L) {
+  const uint32_t kb = batch_key_bits(M);
+  const size_t P = sortk::num_passes(kb);
+  const size_t nn = n ? n : 1;
+  L.keys6 = a.take<uint32_t>((size_t)M * 6);
+  L.sort.hist = a.take<uint32_t>(sortk::kMaxPasses * sortk::kRadix + sortk::kMaxPasses);
+  L.sort.tickets = L.sort.hist + sortk::kMaxPasses * sortk::kRadix;
+  L.sort.status = L.sort.tickets + sortk::kMaxPasses;
+  const size_t status_words = P * (size_t)sortk::num_tiles<uint64_t>(n) * sortk::kRadix;
+  a.off += sizeof(uint32_t) * status_words;
+  L.flags = L.sort.status + status_words;
+  a.off += sizeof(uint32_t) * nn;
+  L.treelet_count = L.flags + nn;
+  a.off += sizeof(uint32_t);
+  L.root_count = L.treelet_count + 1;
+  a.off += sizeof(uint32_t) * M;
+  L.bad = L.root_count + M;
+  a.off += sizeof(uint32_t);
+  L.zero_begin = L.sort.hist;
+  L.zero_words = (size_t)(L.bad + 1 - L.sort.hist);
+  L.sort.zero_words = 0;
+  L.inner_off = a.take<uint32_t>((size_t)M + 1);
+  L.treelets = a.take<uint4>(nn);
+  L.roots = a.take<uint32_t>(nn);
+  L.cntr = a.take<float>(nn * 3);
+  L.codes = a.take<uint64_t>(nn);
+  L.idx = a.take<uint32_t>(nn);
+  L.sort.alt_keys = a.take<uint64_t>(nn);
+  L.sort.alt_vals = a.take<uint32_t>(nn);
+  return a.ok();
+}
+
+struct BatchCastLayout {
+  uint32_t* counts;  // [kCastCounts]
+  uint32_t* inner_off;
+  uint32_t* tile_off;
+  uint32_t* list2;
+  uint32_t* list3;
+  uint2* deep;
+  rc::InnerNode* inner;
+  rc::LeafTri* leaf;
+};
+
+inline bool carve_batch_cast(Arena& a, uint32_t M, uint32_t n, uint32_t num_pix,
+                             BatchCastLayout& L) {
+  L.counts = a.take<uint32_t>(kCastCounts);
+  L.inner_off = a.take<uint32_t>((size_t)M + 1);
+  L.tile_off = a.take<uint32_t>((size_t)M + 1);
+  L.list2 = a.take<uint32_t>(num_pix ? num_pix : 1);
+  L.list3 = a.take<uint32_t>(num_pix ? num_pix : 1);
+  L.deep = a.take<uint2>((size_t)rc::kDeepSlots * rc::kDeepFrames);
+  L.inner = a.take<rc::InnerNode>(n ? n : 1);
+  L.leaf = a.take<rc::LeafTri>(n ? n : 1);
+  return a.ok();
+}
+
+template <int S>
+void launch_batch_tier1(hipStream_t s, uint32_t grid, const bat::BatchCast& a) {
+  launch("k_batch_pix2tri", bat::k_batch_pix2tri<S>, dim3(grid), dim3(64), 0, s, a);
+}
+
+int device_cus() {
+  int dev = 0, cus = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 256;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+    return 256;
+  return cus > 0 ? cus : 256;
+}
+
+}  // namespace
+
+extern "C" {
+
+int dmsh_batch_set_tuning(const char* key, int value) {
+  if (!key) return DMSH_ERR_INVALID;
+  if (std::strcmp(key, "batch.stack") == 0) {
+    if (value != 16 && value != 24 && value != 32) return DMSH_ERR_INVALID;
+    g_batch_stack.store(value);
+    return DMSH_OK;
+  }
+  if (std::strcmp(key, "batch.wgs_per_cu") == 0) {
+    if (value < 1 || value > 64) return DMSH_ERR_INVALID;
+    g_batch_wgs_per_cu.store(value);
+    return DMSH_OK;
+  }
+  if (std::strcmp(key, "batch.treelet_grid") == 0) {
+    if (value < 1 || value > 65536) return DMSH_ERR_INVALID;
+    g_batch_treelet_grid.store(value);
+    return DMSH_OK;
+  }
+  return DMSH_ERR_INVALID;
+}
+
+int dmsh_batch_get_tuning(const char* key) {
+  if (!key) return -1;
+  if (std::strcmp(key, "batch.stack") == 0) return g_batch_stack.load();
+  if (std::strcmp(key, "batch.wgs_per_cu") == 0) return g_batch_wgs_per_cu.load();
+  if (std::strcmp(key, "batch.treelet_grid") == 0) return g_batch_treelet_grid.load();
+  return -1;
+}
+
+size_t dmsh_batch_build_workspace_bytes(uint32_t num_mesh, uint32_t total_tri) {
+  Arena a{nullptr, ~(size_t)0, 0};
+  BatchBuildLayout L;
+  carve_batch_build(a, num_mesh, total_tri, L);
+  return a.off + 256;
+}
+
+int dmsh_batch_build_trimesh3(void* stream, uint32_t num_mesh, uint32_t total_tri,
+                              uint32_t total_vtx, const uint32_t* mesh2tri,
+                              const uint32_t* mesh2vtx, const uint32_t* tri2vtx,
+                              const float* vtx2xyz, uint32_t* mesh2node, uint32_t* bvhnodes,
+                              float* bvhnode2aabb, void* ws, size_t ws_bytes) {
+  if (num_mesh == 0 || !mesh2tri || !mesh2vtx || !mesh2node) return DMSH_ERR_INVALID;
+  if (total_tri >= (1u << 30)) return DMSH_ERR_INVALID;
+  if (total_tri > 0 && (!tri2vtx || !vtx2xyz || !bvhnodes || !bvhnode2aabb || total_vtx == 0))
+    return DMSH_ERR_INVALID;
+  if (!ws || ws_bytes < dmsh_batch_build_workspace_bytes(num_mesh, total_tri))
+    return DMSH_ERR_WORKSPACE;
+  Arena ar{static_cast<char*>(ws), ws_bytes, 0};
+  BatchBuildLayout L;
+  if (!carve_batch_build(ar, num_mesh, total_tri, L)) return DMSH_ERR_WORKSPACE;
+  uint32_t* err = device_error_word();
+  if (!err) return DMSH_ERR_HIP;
+  hipStream_t s = as_stream(stream);
+  const uint32_t M = num_mesh, n = total_tri;
+  const uint32_t kb = batch_key_bits(M);
+  const uint32_t P = sortk::num_passes(kb);
+  int rc;
+  launch("k_batch_init", bat::k_batch_init, dim3(std::min<uint32_t>(ceil_div(L.zero_words, 256), 2048)),
+         dim3(256), 0, s, M, L.keys6, L.zero_begin, L.zero_words);
+  if ((rc = launch_status()) != DMSH_OK) return rc;
+  launch("k_batch_offsets", bat::k_batch_offsets, dim3(1), dim3(1024), 0, s, M, n, total_vtx,
+         mesh2tri, mesh2vtx, L.inner_off, mesh2node, L.bad, err);
+  if ((rc = launch_status()) != DMSH_OK) return rc;
+  if (n == 0) return DMSH_OK;
+  launch("k_batch_centroid", bat::k_batch_centroid, dim3(std::min<uint32_t>(ceil_div(n, 256), 4096)),
+         dim3(256), 0, s, L.bad, n, M, mesh2tri, mesh2vtx, tri2vtx, vtx2xyz, L.cntr, L.keys6, err);
+  if ((rc = launch_status()) != DMSH_OK) return rc;
+  // With an even number of passes the sort returns to the buffer it starts
+  // from; with an odd number it starts from the alternate key buffer.
+  uint64_t* morton_out = (P % 2 == 0) ? L.codes : static_cast<uint64_t*>(L.sort.alt_keys);
+  launch("k_morton_hist", bat::k_batch_morton_hist,
+         dim3(std::min<uint32_t>(sortk::num_tiles<uint64_t>(n), 2048)), dim3(256), 0, s, L.bad, n,
+         M, mesh2tri, L.cntr, L.keys6, morton_out, P, kb, L.sort.hist);
+  if ((rc = launch_status()) != DMSH_OK) return rc;
+  rc = sortk::run_passes<uint64_t>(s, n, kb, L.sort, morton_out, L.codes, L.idx, err);
+  if (rc != DMSH_OK) return rc;
+  const TreeletSink ts{kTreeletLeaves, L.treelet_count, L.treelets, n, L.root_count, L.roots};
+  launch("k_topology", bat::k_batch_topology, dim3(ceil_div(n, 256)), dim3(256), 0, s, L.bad, n,
+         M, mesh2tri, L.inner_off, mesh2node, bvhnodes, L.codes, L.idx, ts);
+  if ((rc = launch_status()) != DMSH_OK) return rc;
+  launch("k_batch_singles", bat::k_batch_singles, dim3(ceil_div(M, 256)), dim3(256), 0, s, L.bad,
+         M, mesh2tri, mesh2node, bvhnodes, L.idx, ts);
+  if ((rc = launch_status()) != DMSH_OK) return rc;
+  const bat::BatchTrees bt{L.bad,    M,        mesh2tri, mesh2vtx, mesh2node,
+                           bvhnode2aabb, bvhnodes, tri2vtx, vtx2xyz};
+  const uint32_t tgrid = std::min<uint32_t>((uint32_t)g_batch_treelet_grid.load(), n);
+  launch("k_refit_treelet", bat::k_batch_refit_treelet<(int)kTreeletLeaves, kTreeletThreads>,
+         dim3(tgrid), dim3(kTreeletThreads), 0, s, bt, n, L.treelet_count, L.treelets, err);
+  if ((rc = launch_status()) != DMSH_OK) return rc;
+  launch("k_refit_top", bat::k_batch_refit_top<(int)kTreeletLeaves, 256>,
+         dim3(std::min<uint32_t>(M, 1024)), dim3(256), 0, s, bt, L.inner_off, L.flags,
+         L.root_count, L.roots, err);
+  return launch_status();
+}
+
+size_t dmsh_batch_pix2tri_workspace_bytes(uint32_t num_mesh, uint32_t total_tri,
+                                          uint32_t total_pix) {
+  Arena a{nullptr, ~(size_t)0, 0};
+  BatchCastLayout L;
+  carve_batch_cast(a, num_mesh, total_tri, total_pix, L);
+  return a.off + 256;
+}
+
+int dmsh_batch_pix2tri(void* stream, uint32_t num_mesh, uint32_t total_tri, uint32_t total_vtx,
+                       uint32_t total_pix, const uint32_t* mesh2tri, const uint32_t* mesh2vtx,
+                       const uint32_t* tri2vtx, const float* vtx2xyz, const uint32_t* mesh2node,
+                       const uint32_t* bvhnodes, const float* bvhnode2aabb,
+                       const uint32_t* mesh2pix, const uint32_t* mesh2width,
+                       const float* transforms_ndc2world, uint32_t* pix2tri, float* pix2t,
+                       void* ws, size_t ws_bytes) {
+  if (num_mesh == 0 || !mesh2tri || !mesh2vtx || !mesh2node || !mesh2pix || !mesh2width ||
+      !transforms_ndc2world)
+    return DMSH_ERR_INVALID;
+  if (total_tri >= rc::kNoneTri || total_pix == 0) return total_pix == 0 ? DMSH_OK : DMSH_ERR_INVALID;
+  if (!pix2tri) return DMSH_ERR_INVALID;
+  if (total_tri > 0 && (!tri2vtx || !vtx2xyz || !bvhnodes || !bvhnode2aabb))
+    return DMSH_ERR_INVALID;
+  if (!ws || ws_bytes < dmsh_batch_pix2tri_workspace_bytes(num_mesh, total_tri, total_pix))
+    return DMSH_ERR_WORKSPACE;
+  Arena ar{static_cast<char*>(ws), ws_bytes, 0};
+  BatchCastLayout L;
+  if (!carve_batch_cast(ar, num_mesh, total_tri, total_pix, L)) return DMSH_ERR_WORKSPACE;
+  uint32_t* err = device_error_word();
+  if (!err) return DMSH_ERR_HIP;
+  hipStream_t s = as_stream(stream);
+  const uint32_t M = num_mesh, n = total_tri;
+  int rc = sortk::launch_fill(s, L.counts, kCastCounts, 0u);
+  if (rc != DMSH_OK) return rc;
+  launch("k_batch_cast_offsets", bat::k_batch_cast_offsets, dim3(1), dim3(1024), 0, s, M, n,
+         total_vtx, total_pix, mesh2tri, mesh2vtx, mesh2node, mesh2pix, mesh2width, L.inner_off,
+         L.tile_off, L.counts + 3, err);
+  if ((rc = launch_status()) != DMSH_OK) return rc;
+  const bat::BatchTrees bt{L.counts + 3, M, mesh2tri, mesh2vtx, mesh2node,
+                           const_cast<float*>(bvhnode2aabb), bvhnodes, tri2vtx, vtx2xyz};
+  if (n > 0) {
+    launch("k_pack_inner", bat::k_batch_pack_inner, dim3(ceil_div(n, 256)), dim3(256), 0, s, bt,
+           n, L.inner_off, L.inner, L.counts + 2, err);
+    if ((rc = launch_status()) != DMSH_OK) return rc;
+    launch("k_pack_leaf", bat::k_batch_pack_leaf, dim3(ceil_div(n, 256)), dim3(256), 0, s, bt, n,
+           L.leaf, err);
+    if ((rc = launch_status()) != DMSH_OK) return rc;
+  }
+  bat::BatchCast a;
+  a.M = M;
+  a.mesh2tri = mesh2tri;
+  a.node_off = mesh2node;
+  a.inner_off = L.inner_off;
+  a.tile_off = L.tile_off;
+  a.pix_off = mesh2pix;
+  a.widths = mesh2width;
+  a.inner = L.inner;
+  a.leaf = L.leaf;
+  a.aabb = bvhnode2aabb;
+  a.transforms = transforms_ndc2world;
+  a.pix2tri = pix2tri;
+  a.pix2t = pix2t;
+  a.counts = L.counts;
+  a.list2 = L.list2;
+  a.list3 = L.list3;
+  a.deep_stack = L.deep;
+  a.err = err;
+  const uint32_t grid = (uint32_t)device_cus() * (uint32_t)g_batch_wgs_per_cu.load();
+  switch (g_batch_stack.load()) {
+    case 16: launch_batch_tier1<16>(s, grid, a); break;
+    case 32: launch_batch_tier1<32>(s, grid, a); break;
+    default: launch_batch_tier1<24>(s, grid, a); break;
+  }
+  if ((rc = launch_status()) != DMSH_OK) return rc;
+  launch("k_batch_pix2tri_tier2", bat::k_batch_pix2tri_tier2, dim3(rc::kTier2Blocks), dim3(64), 0,
+         s, a, total_pix);
+  if ((rc = launch_status()) != DMSH_OK) return rc;
+  launch("k_batch_pix2tri_tier3", bat::k_batch_pix2tri_tier3, dim3(rc::kDeepSlots / 64), dim3(64),
+         0, s, a, total_pix);
+  return launch_status();
+}
+
+}  // extern "C"

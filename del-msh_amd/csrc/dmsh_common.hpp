@@ -38,6 +38,9 @@ inline int launch_status() {
 }
 
 inline uint32_t ceil_div(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
+__host__ __device__ __forceinline__ uint32_t ceil_div_u32(uint32_t a, uint32_t b) {
+  return a / b + (a % b != 0u);
+}
 
 inline size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
 

@@ -84,14 +84,12 @@ __device__ __forceinline__ uint32_t child_ref(const uint32_t* __restrict__ bvhno
   return leaf ? (kLeafBit | (c - nb)) : c;
 }
 
-static __global__ __launch_bounds__(256) void k_pack_inner(InnerNode* __restrict__ inner,
-                                                           const uint32_t* __restrict__ bvhnodes,
-                                                           const float* __restrict__ aabb,
-                                                           uint32_t num_leaf,
-                                                           uint32_t* __restrict__ nonfinite,
-                                                           uint32_t* __restrict__ err) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i + 1 >= num_leaf) return;
+// Internal node i (i + 1 < num_leaf) of one tree into inner[i].
+__device__ __forceinline__ void pack_inner(InnerNode* __restrict__ inner,
+                                           const uint32_t* __restrict__ bvhnodes,
+                                           const float* __restrict__ aabb, uint32_t num_leaf,
+                                           uint32_t i, uint32_t* __restrict__ nonfinite,
+                                           uint32_t* __restrict__ err) {
   const uint32_t* row = bvhnodes + (size_t)i * 3;
   const uint32_t c0 = row[1], c1 = row[2];
   bool bad = (c1 == DMSH_U32MAX);
@@ -121,15 +119,24 @@ static __global__ __launch_bounds__(256) void k_pack_inner(InnerNode* __restrict
   inner[i] = nd;
 }
 
-static __global__ __launch_bounds__(256) void k_pack_leaf(LeafTri* __restrict__ leaf,
-                                                          const uint32_t* __restrict__ bvhnodes,
-                                                          const uint32_t* __restrict__ tri2vtx,
-                                                          const float* __restrict__ xyz,
-                                                          uint32_t num_leaf, uint32_t num_tri,
-                                                          uint32_t num_vtx,
-                                                          uint32_t* __restrict__ err) {
-  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= num_leaf) return;
+static __global__ __launch_bounds__(256) void k_pack_inner(InnerNode* __restrict__ inner,
+                                                           const uint32_t* __restrict__ bvhnodes,
+                                                           const float* __restrict__ aabb,
+                                                           uint32_t num_leaf,
+                                                           uint32_t* __restrict__ nonfinite,
+                                                           uint32_t* __restrict__ err) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i + 1 >= num_leaf) return;
+  pack_inner(inner, bvhnodes, aabb, num_leaf, i, nonfinite, err);
+}
+
+// Leaf k of one tree into leaf[k].
+__device__ __forceinline__ void pack_leaf(LeafTri* __restrict__ leaf,
+                                          const uint32_t* __restrict__ bvhnodes,
+                                          const uint32_t* __restrict__ tri2vtx,
+                                          const float* __restrict__ xyz, uint32_t num_leaf,
+                                          uint32_t num_tri, uint32_t num_vtx, uint32_t k,
+                                          uint32_t* __restrict__ err) {
   const uint32_t* row = bvhnodes + (size_t)(num_leaf - 1 + k) * 3;
   const uint32_t tri = row[1];
   LeafTri lt;
@@ -157,6 +164,18 @@ static __global__ __launch_bounds__(256) void k_pack_leaf(LeafTri* __restrict__ 
     lt.c = make_float4(p2[2], __uint_as_float(tri), 0.f, 0.f);
   }
   leaf[k] = lt;
+}
+
+static __global__ __launch_bounds__(256) void k_pack_leaf(LeafTri* __restrict__ leaf,
+                                                          const uint32_t* __restrict__ bvhnodes,
+                                                          const uint32_t* __restrict__ tri2vtx,
+                                                          const float* __restrict__ xyz,
+                                                          uint32_t num_leaf, uint32_t num_tri,
+                                                          uint32_t num_vtx,
+                                                          uint32_t* __restrict__ err) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= num_leaf) return;
+  pack_leaf(leaf, bvhnodes, tri2vtx, xyz, num_leaf, num_tri, num_vtx, k, err);
 }
 
 // ---------------------------------------------------------------------------

@@ -1,13 +1,14 @@
 // refit.hpp -- bottom-up AABB refit of an LBVH on gfx950.
 //
-// One thread per leaf (leaves are rows num_leaf-1 .. 2 num_leaf-2, in sorted
-// Morton order, so neighbouring threads touch neighbouring geometry).  The
-// leaf box is set from the first vertex and grown by the others
-// (del-msh-cpu/src/vtx2xyz.rs:39-61 with margin eps); a thread then climbs to
-// the parent and takes the parent's arrival counter.  The first arrival stops;
-// the second merges the two children in fixed (left, right) order
-// (bvhnode2aabb3.rs:136-141) and continues upward, so the result does not
-// depend on which child arrives last.
+// Generic refit (any tree in the reference layout, k_refit): one thread per
+// leaf (leaves are rows num_leaf-1 .. 2 num_leaf-2, in sorted Morton order,
+// so neighbouring threads touch neighbouring geometry).  The leaf box is set
+// from the first vertex and grown by the others (del-msh-cpu/src/
+// vtx2xyz.rs:39-61 with margin eps); a thread then climbs to the parent and
+// takes the parent's arrival counter.  The first arrival stops; the second
+// merges the two children in fixed (left, right) order (bvhnode2aabb3.rs:
+// 136-141) and continues upward, so the result does not depend on which child
+// arrives last.
 //
 // Memory ordering: the counter update is an agent-scope acq_rel RMW.  Its
 // release half publishes the arriving thread's box stores (issued earlier in
@@ -15,6 +16,9 @@
 // published by the first arrival, visible to the second.  The reference CUDA
 // kernel fences only after the RMW (bvhnode2aabb.cu:50-55), which is not
 // enough on a weakly ordered multi-L2 part.
+//
+// Trees built here take the treelet refit below instead (same merge order,
+// same results, no agent-scope atomics).
 #pragma once
 
 #include <algorithm>
@@ -126,6 +130,17 @@ __global__ __launch_bounds__(256) void k_refit(float* __restrict__ aabb,
   }
 }
 
+// One tree (one mesh of a batch): node rows and boxes, and the leaves'
+// geometry (triangles over vertices, or points).
+struct TreeView {
+  float* aabb;
+  const uint32_t* bvhnodes;
+  uint32_t num_leaf;
+  const uint32_t* tri2vtx;
+  uint32_t num_vtx;
+  const float* xyz;
+};
+
 // Leaf box of leaf k; returns false (and raises) on a bad index.
 template <bool TRI>
 __device__ __forceinline__ bool leaf_box(const uint32_t* __restrict__ bvhnodes, uint32_t num_leaf,
@@ -159,105 +174,156 @@ __device__ __forceinline__ bool leaf_box(const uint32_t* __restrict__ bvhnodes, 
 // Treelet refit for LBVHs built here (the topology kernel lists the
 // treelets: subtrees of at most K leaves whose parent has more than K).
 //
-// k_refit_treelet: a persistent grid; each workgroup takes whole treelets.
-//   A treelet over leaves [f, l] keeps its internal nodes at ids inside
-//   [f, l], so its node rows, arrival counters and boxes fit in LDS slots
-//   (id - f).  Phase 1 builds the leaf boxes (stored to HBM at once, in leaf
-//   order) and loads the node rows with coalesced reads; phase 2 climbs from
-//   every leaf with LDS counters (the second arrival merges, left child
-//   first) up to the treelet root; phase 3 writes the internal boxes back in
-//   id order.  Every node is written once and no global atomics are used.
-// k_refit_top: one workgroup climbs from the treelet roots through the nodes
-//   above them (at most ~2n/K of them for a balanced tree).  Those climbs all
-//   run on one CU, so workgroup-scope acq_rel RMWs on the global counters
-//   order the box stores and loads (one L1, no cross-XCD traffic) and no L2
-//   write-back is needed.
+// refit_treelet: one workgroup, one treelet.  A treelet over leaves [f, l]
+//   keeps its internal nodes at ids inside [f, l], so its node rows, arrival
+//   counters and boxes fit in LDS slots (id - f).  Phase 1 builds the leaf
+//   boxes (stored to HBM at once, in leaf order) and loads the node rows with
+//   coalesced reads; phase 2 climbs from every leaf with LDS counters (the
+//   second arrival merges, left child first) up to the treelet root; phase 3
+//   writes the internal boxes back in id order.  Every node is written once
+//   and no global atomics are used.
+// refit_top: one workgroup climbs from a tree's treelet roots through the
+//   nodes above them (at most ~2n/K of them for a balanced tree).  Those
+//   climbs all run on one CU, so workgroup-scope acq_rel RMWs on the global
+//   counters order the box stores and loads (one L1, no cross-XCD traffic) and
+//   no L2 write-back is needed.
+template <int K>
+struct TreeletLds {
+  Box leaf[K];
+  Box inner[K];
+  uint32_t flag[K];
+  uint2 child[K];
+  uint32_t parent[K];
+};
+
+template <bool TRI, int K, int T>
+__device__ __forceinline__ void refit_treelet(const TreeView& v, uint32_t root, uint32_t f,
+                                              uint32_t l, float eps, TreeletLds<K>& s,
+                                              uint32_t* __restrict__ err) {
+  constexpr int LPT = K / T;
+  const uint32_t num_leaf = v.num_leaf;
+  const uint32_t nb = num_leaf - 1;
+  if (f > l || l >= num_leaf || l - f >= (uint32_t)K) {  // uniform over the workgroup
+    if (threadIdx.x == 0) raise_error(err, DMSH_DEVERR_REFIT);
+    return;
+  }
+  const uint32_t m = l - f + 1;
+  uint32_t lpar[LPT];
+  // phase 1
+#pragma unroll
+  for (int r = 0; r < LPT; ++r) {
+    const uint32_t j = (uint32_t)r * T + threadIdx.x;
+    lpar[r] = DMSH_U32MAX - 1;  // no climb
+    if (j >= m) continue;
+    const uint32_t id = f + j;
+    s.flag[j] = 0u;
+    if (id < nb) {
+      const uint32_t* row = v.bvhnodes + (size_t)id * 3;
+      s.parent[j] = row[0];
+      s.child[j] = make_uint2(row[1], row[2]);
+    }
+    Box b;
+    uint32_t p;
+    if (leaf_box<TRI>(v.bvhnodes, num_leaf, id, v.tri2vtx, v.num_vtx, v.xyz, eps, b, p, err)) {
+      s.leaf[j] = b;
+      box_store(v.aabb, nb + id, b);
+      if (nb + id != root) lpar[r] = p;
+    }
+  }
+  __syncthreads();
+  // phase 2
+#pragma unroll
+  for (int r = 0; r < LPT; ++r) {
+    const uint32_t j = (uint32_t)r * T + threadIdx.x;
+    uint32_t parent = lpar[r];
+    if (parent == DMSH_U32MAX - 1) continue;
+    uint32_t node = nb + f + j;
+    Box mine = s.leaf[j];
+    for (uint32_t steps = 0;; ++steps) {
+      if (parent < f || parent > l || parent >= nb || steps >= (uint32_t)K) {
+        raise_error(err, DMSH_DEVERR_REFIT);
+        break;
+      }
+      const uint32_t pj = parent - f;
+      const uint32_t old =
+          __hip_atomic_fetch_add(&s.flag[pj], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
+      if (old == 0) break;
+      const uint2 ch = s.child[pj];
+      const uint32_t sib = (ch.x == node) ? ch.y : ch.x;
+      const uint32_t si = (sib >= nb) ? sib - nb - f : sib - f;
+      if ((ch.x != node && ch.y != node) || si >= m) {
+        raise_error(err, DMSH_DEVERR_REFIT);
+        break;
+      }
+      const Box sb = (sib >= nb) ? s.leaf[si] : s.inner[si];
+      mine = (ch.x == node) ? box_merge(mine, sb) : box_merge(sb, mine);
+      s.inner[pj] = mine;
+      if (parent == root) break;
+      node = parent;
+      parent = s.parent[pj];
+    }
+  }
+  __syncthreads();
+  // phase 3: the nodes that received both arrivals are this treelet's
+#pragma unroll
+  for (int r = 0; r < LPT; ++r) {
+    const uint32_t j = (uint32_t)r * T + threadIdx.x;
+    if (j < m && f + j < nb && s.flag[j] >= 2u) box_store(v.aabb, f + j, s.inner[j]);
+  }
+  __syncthreads();  // LDS is reused by the next treelet
+}
+
+// Climbs from roots[e * stride], e < count, through the tree's top nodes.
+template <int T>
+__device__ __forceinline__ void refit_top(const TreeView& v, uint32_t* __restrict__ flags,
+                                          const uint32_t* __restrict__ roots, uint32_t stride,
+                                          uint32_t count, uint32_t* __restrict__ err) {
+  const uint32_t nb = v.num_leaf - 1;
+  for (uint32_t e = threadIdx.x; e < count; e += T) {
+    uint32_t node = roots[(size_t)e * stride];
+    if (node > 2 * nb) {
+      raise_error(err, DMSH_DEVERR_REFIT);
+      continue;
+    }
+    Box mine = box_load(v.aabb, node);
+    uint32_t parent = v.bvhnodes[(size_t)node * 3];
+    for (uint32_t steps = 0; parent != DMSH_U32MAX; ++steps) {
+      if (parent >= nb || steps >= nb) {
+        raise_error(err, DMSH_DEVERR_REFIT);
+        break;
+      }
+      const uint32_t old = __hip_atomic_fetch_add(flags + parent, 1u, __ATOMIC_ACQ_REL,
+                                                  __HIP_MEMORY_SCOPE_WORKGROUP);
+      if (old == 0) break;
+      const uint32_t* prow = v.bvhnodes + (size_t)parent * 3;
+      const uint32_t a = prow[1], b = prow[2];
+      if (a > 2 * nb || b > 2 * nb || (a != node && b != node)) {
+        raise_error(err, DMSH_DEVERR_REFIT);
+        break;
+      }
+      mine = (a == node) ? box_merge(mine, box_load(v.aabb, b))
+                         : box_merge(box_load(v.aabb, a), mine);
+      box_store(v.aabb, parent, mine);
+      node = parent;
+      parent = prow[0];
+    }
+  }
+}
+
+// persistent grid over the treelets of one tree
 template <bool TRI, int K, int T>
 __global__ __launch_bounds__(T) void k_refit_treelet(
     float* __restrict__ aabb, uint32_t num_leaf, const uint32_t* __restrict__ bvhnodes,
     const uint32_t* __restrict__ tri2vtx, uint32_t num_vtx, const float* __restrict__ xyz,
     float eps, const uint32_t* __restrict__ treelet_count, const uint4* __restrict__ treelets,
     uint32_t* __restrict__ err) {
-  constexpr int LPT = K / T;
-  __shared__ Box s_leaf[K];
-  __shared__ Box s_inner[K];
-  __shared__ uint32_t s_flag[K];
-  __shared__ uint2 s_child[K];
-  __shared__ uint32_t s_parent[K];
-  const uint32_t nb = num_leaf - 1;
+  __shared__ TreeletLds<K> s;
+  const TreeView v{aabb, bvhnodes, num_leaf, tri2vtx, num_vtx, xyz};
   uint32_t count = *treelet_count;
   if (count > num_leaf) count = num_leaf;
   for (uint32_t t = blockIdx.x; t < count; t += gridDim.x) {
     const uint4 tl = treelets[t];
-    const uint32_t root = tl.x, f = tl.y, l = tl.z;
-    if (f > l || l >= num_leaf || l - f >= (uint32_t)K) {  // uniform over the workgroup
-      if (threadIdx.x == 0) raise_error(err, DMSH_DEVERR_REFIT);
-      continue;
-    }
-    const uint32_t m = l - f + 1;
-    uint32_t lpar[LPT];
-    // phase 1
-#pragma unroll
-    for (int r = 0; r < LPT; ++r) {
-      const uint32_t j = (uint32_t)r * T + threadIdx.x;
-      lpar[r] = DMSH_U32MAX - 1;  // no climb
-      if (j >= m) continue;
-      const uint32_t id = f + j;
-      s_flag[j] = 0u;
-      if (id < nb) {
-        const uint32_t* row = bvhnodes + (size_t)id * 3;
-        s_parent[j] = row[0];
-        s_child[j] = make_uint2(row[1], row[2]);
-      }
-      Box b;
-      uint32_t p;
-      if (leaf_box<TRI>(bvhnodes, num_leaf, id, tri2vtx, num_vtx, xyz, eps, b, p, err)) {
-        s_leaf[j] = b;
-        box_store(aabb, nb + id, b);
-        if (nb + id != root) lpar[r] = p;
-      }
-    }
-    __syncthreads();
-    // phase 2
-#pragma unroll
-    for (int r = 0; r < LPT; ++r) {
-      const uint32_t j = (uint32_t)r * T + threadIdx.x;
-      uint32_t parent = lpar[r];
-      if (parent == DMSH_U32MAX - 1) continue;
-      uint32_t node = nb + f + j;
-      Box mine = s_leaf[j];
-      for (uint32_t steps = 0;; ++steps) {
-        if (parent < f || parent > l || parent >= nb || steps >= (uint32_t)K) {
-          raise_error(err, DMSH_DEVERR_REFIT);
-          break;
-        }
-        const uint32_t pj = parent - f;
-        const uint32_t old =
-            __hip_atomic_fetch_add(&s_flag[pj], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
-        if (old == 0) break;
-        const uint2 ch = s_child[pj];
-        const uint32_t sib = (ch.x == node) ? ch.y : ch.x;
-        const uint32_t si = (sib >= nb) ? sib - nb - f : sib - f;
-        if ((ch.x != node && ch.y != node) || si >= m) {
-          raise_error(err, DMSH_DEVERR_REFIT);
-          break;
-        }
-        const Box sb = (sib >= nb) ? s_leaf[si] : s_inner[si];
-        mine = (ch.x == node) ? box_merge(mine, sb) : box_merge(sb, mine);
-        s_inner[pj] = mine;
-        if (parent == root) break;
-        node = parent;
-        parent = s_parent[pj];
-      }
-    }
-    __syncthreads();
-    // phase 3: the nodes that received both arrivals are this treelet's
-#pragma unroll
-    for (int r = 0; r < LPT; ++r) {
-      const uint32_t j = (uint32_t)r * T + threadIdx.x;
-      if (j < m && f + j < nb && s_flag[j] >= 2u) box_store(aabb, f + j, s_inner[j]);
-    }
-    __syncthreads();  // LDS is reused by the next treelet
+    refit_treelet<TRI, K, T>(v, tl.x, tl.y, tl.z, eps, s, err);
   }
 }
 
@@ -268,42 +334,16 @@ __global__ __launch_bounds__(T) void k_refit_top(float* __restrict__ aabb,
                                                  const uint32_t* __restrict__ treelet_count,
                                                  const uint4* __restrict__ treelets,
                                                  uint32_t* __restrict__ err) {
-  const uint32_t nb = num_leaf - 1;
+  const TreeView v{aabb, bvhnodes, num_leaf, nullptr, 0u, nullptr};
   uint32_t count = *treelet_count;
   if (count > num_leaf) count = num_leaf;
-  for (uint32_t e = threadIdx.x; e < count; e += T) {
-    uint32_t node = treelets[e].x;
-    if (node > 2 * nb) {
-      raise_error(err, DMSH_DEVERR_REFIT);
-      continue;
-    }
-    Box mine = box_load(aabb, node);
-    uint32_t parent = bvhnodes[(size_t)node * 3];
-    for (uint32_t steps = 0; parent != DMSH_U32MAX; ++steps) {
-      if (parent >= nb || steps >= nb) {
-        raise_error(err, DMSH_DEVERR_REFIT);
-        break;
-      }
-      const uint32_t old = __hip_atomic_fetch_add(flags + parent, 1u, __ATOMIC_ACQ_REL,
-                                                  __HIP_MEMORY_SCOPE_WORKGROUP);
-      if (old == 0) break;
-      const uint32_t* prow = bvhnodes + (size_t)parent * 3;
-      const uint32_t a = prow[1], b = prow[2];
-      if (a > 2 * nb || b > 2 * nb || (a != node && b != node)) {
-        raise_error(err, DMSH_DEVERR_REFIT);
-        break;
-      }
-      mine = (a == node) ? box_merge(mine, box_load(aabb, b)) : box_merge(box_load(aabb, a), mine);
-      box_store(aabb, parent, mine);
-      node = parent;
-      parent = prow[0];
-    }
-  }
+  refit_top<T>(v, flags, reinterpret_cast<const uint32_t*>(treelets), 4u, count, err);
 }
 
 constexpr uint32_t kTreeletLeaves = 1024;  // K
 constexpr int kTreeletThreads = 256;
 constexpr uint32_t kTreeletGrid = 1024;    // persistent workgroups
+constexpr int kTopThreads = 1024;
 
 // flags: num_leaf - 1 zeroed counters; treelets as listed by the topology
 // kernel with k = kTreeletLeaves.
@@ -313,7 +353,6 @@ inline int launch_refit_treelets(hipStream_t s, float* aabb, uint32_t* flags,
                                  uint32_t num_leaf, const uint32_t* bvhnodes,
                                  const uint32_t* tri2vtx, uint32_t num_vtx, const float* xyz,
                                  float eps, uint32_t* err) {
-  constexpr int kTopThreads = 1024;
   if (num_leaf == 0) return DMSH_OK;
   const uint32_t grid = std::min<uint32_t>(kTreeletGrid, num_leaf);
   launch("k_refit_treelet", (k_refit_treelet<TRI, (int)kTreeletLeaves, kTreeletThreads>),
@@ -332,7 +371,7 @@ inline int launch_refit(hipStream_t s, float* aabb, uint32_t* flags, uint32_t nu
                         const float* xyz, float eps, uint32_t* err) {
   if (num_leaf == 0) return DMSH_OK;
   launch("k_refit", k_refit<TRI>, dim3(ceil_div(num_leaf, 256)), dim3(256), 0, s, aabb, flags,
-                     num_leaf, bvhnodes, tri2vtx, num_vtx, xyz, eps, err);
+         num_leaf, bvhnodes, tri2vtx, num_vtx, xyz, eps, err);
   return launch_status();
 }
 

@@ -90,45 +90,57 @@ __device__ __forceinline__ uint32_t node_split(const KeyT* __restrict__ c, uint3
 // treelet covering leaves [f, l] holds exactly the internal nodes of its
 // subtree, and every one of them has an id inside [f, l] (a Karras node sits
 // at one end of its own range), so one workgroup can refit it in LDS.
-// Entries are (node id, f, l, 0), appended in any order; at most n of them.
+// Entries are (node id, f, l, mesh), appended in any order; at most n of
+// them.  Batched builds also append each treelet root to its mesh's own list
+// (mesh_roots + root_base, counted in mesh_root_count[mesh]) for the top climb.
 struct TreeletSink {
   uint32_t k;
   uint32_t* count;  // zeroed by the caller
   uint4* list;      // nullptr: no treelets wanted
   uint32_t cap;
+  uint32_t* mesh_root_count;  // batched only, zeroed by the caller
+  uint32_t* mesh_roots;
 };
 
 __device__ __forceinline__ void emit_treelet(const TreeletSink& ts, uint32_t node, uint32_t f,
-                                             uint32_t l) {
+                                             uint32_t l, uint32_t mesh, uint32_t root_base) {
   const uint32_t slot = atomicAdd(ts.count, 1u);
-  if (slot < ts.cap) ts.list[slot] = make_uint4(node, f, l, 0u);
+  if (slot < ts.cap) ts.list[slot] = make_uint4(node, f, l, mesh);
+  if (ts.mesh_roots) {
+    const uint32_t r = atomicAdd(ts.mesh_root_count + mesh, 1u);
+    ts.mesh_roots[root_base + r] = node;  // at most one root per leaf of the mesh
+  }
 }
 
+// Internal node i of a tree over n sorted codes: range, split, child links
+// (bvhnodes_morton.rs:18-49) and the treelets it starts.  idx2obj[k] - obj_sub
+// is the object of sorted position k.
 template <typename KeyT>
-__global__ __launch_bounds__(256) void k_topology(uint32_t n, uint32_t* __restrict__ bvhnodes,
-                                                  const KeyT* __restrict__ codes,
-                                                  const uint32_t* __restrict__ idx2obj,
-                                                  TreeletSink ts) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void topology_node(uint32_t n, uint32_t i, uint32_t* __restrict__ bvhnodes,
+                                              const KeyT* __restrict__ codes,
+                                              const uint32_t* __restrict__ idx2obj,
+                                              uint32_t obj_sub, const TreeletSink& ts,
+                                              uint32_t mesh, uint32_t root_base) {
   const uint32_t nb = n - 1;
-  if (i >= nb) return;
   uint32_t first, last;
   node_range<KeyT>(codes, n, i, first, last);
   const uint32_t split = node_split<KeyT>(codes, first, last);
   if (ts.list) {
     if (last - first + 1 > ts.k) {
-      if (split - first + 1 <= ts.k) emit_treelet(ts, first == split ? nb + split : split, first, split);
+      if (split - first + 1 <= ts.k)
+        emit_treelet(ts, first == split ? nb + split : split, first, split, mesh, root_base);
       if (last - split <= ts.k)
-        emit_treelet(ts, last == split + 1 ? nb + split + 1 : split + 1, split + 1, last);
+        emit_treelet(ts, last == split + 1 ? nb + split + 1 : split + 1, split + 1, last, mesh,
+                     root_base);
     } else if (i == 0) {
-      emit_treelet(ts, 0u, 0u, n - 1);
+      emit_treelet(ts, 0u, 0u, n - 1, mesh, root_base);
     }
   }
   if (first == split) {
     const uint32_t a = nb + split;
     bvhnodes[(size_t)i * 3 + 1] = a;
     bvhnodes[(size_t)a * 3 + 0] = i;
-    bvhnodes[(size_t)a * 3 + 1] = idx2obj[split];
+    bvhnodes[(size_t)a * 3 + 1] = idx2obj[split] - obj_sub;
     bvhnodes[(size_t)a * 3 + 2] = DMSH_U32MAX;
   } else {
     bvhnodes[(size_t)i * 3 + 1] = split;
@@ -138,7 +150,7 @@ __global__ __launch_bounds__(256) void k_topology(uint32_t n, uint32_t* __restri
     const uint32_t b = nb + split + 1;
     bvhnodes[(size_t)i * 3 + 2] = b;
     bvhnodes[(size_t)b * 3 + 0] = i;
-    bvhnodes[(size_t)b * 3 + 1] = idx2obj[split + 1];
+    bvhnodes[(size_t)b * 3 + 1] = idx2obj[split + 1] - obj_sub;
     bvhnodes[(size_t)b * 3 + 2] = DMSH_U32MAX;
   } else {
     bvhnodes[(size_t)i * 3 + 2] = split + 1;
@@ -147,18 +159,35 @@ __global__ __launch_bounds__(256) void k_topology(uint32_t n, uint32_t* __restri
   if (i == 0) bvhnodes[0] = DMSH_U32MAX;
 }
 
+// A tree of one object: the single leaf row (MAX, obj, MAX) and one treelet.
+__device__ __forceinline__ void topology_single(uint32_t* __restrict__ bvhnodes, uint32_t obj,
+                                                const TreeletSink& ts, uint32_t mesh,
+                                                uint32_t root_base) {
+  bvhnodes[0] = DMSH_U32MAX;
+  bvhnodes[1] = obj;
+  bvhnodes[2] = DMSH_U32MAX;
+  if (ts.list) emit_treelet(ts, 0u, 0u, 0u, mesh, root_base);
+}
+
+template <typename KeyT>
+__global__ __launch_bounds__(256) void k_topology(uint32_t n, uint32_t* __restrict__ bvhnodes,
+                                                  const KeyT* __restrict__ codes,
+                                                  const uint32_t* __restrict__ idx2obj,
+                                                  TreeletSink ts) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n - 1) return;
+  topology_node<KeyT>(n, i, bvhnodes, codes, idx2obj, 0u, ts, 0u, 0u);
+}
+
 static __global__ void k_topology_single(uint32_t* __restrict__ bvhnodes,
                                          const uint32_t* __restrict__ idx2obj, TreeletSink ts) {
-  bvhnodes[0] = DMSH_U32MAX;
-  bvhnodes[1] = idx2obj[0];
-  bvhnodes[2] = DMSH_U32MAX;
-  if (ts.list) emit_treelet(ts, 0u, 0u, 0u);
+  topology_single(bvhnodes, idx2obj[0], ts, 0u, 0u);
 }
 
 template <typename KeyT>
 inline int launch_topology(hipStream_t s, uint32_t n, uint32_t* bvhnodes, const KeyT* codes,
-                           const uint32_t* idx2obj, TreeletSink ts = TreeletSink{0, nullptr,
-                                                                                 nullptr, 0}) {
+                           const uint32_t* idx2obj,
+                           TreeletSink ts = TreeletSink{0, nullptr, nullptr, 0, nullptr, nullptr}) {
   if (n == 0) return DMSH_OK;
   if (n == 1) {
     launch("k_topology_single", k_topology_single, dim3(1), dim3(1), 0, s, bvhnodes, idx2obj, ts);

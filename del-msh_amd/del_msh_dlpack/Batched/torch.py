@@ -1,0 +1,158 @@
+"""Batched build / ray-cast over a CSR batch of triangle meshes.
+
+A batch is given by host-side offsets (numpy or CPU tensors, num_mesh + 1
+entries, first entry 0): triangles [mesh2tri[m], mesh2tri[m+1]) of tri2vtx
+with vertex indices local to vertices [mesh2vtx[m], mesh2vtx[m+1]) of vtx2xyz,
+and for the ray-cast pixels [mesh2pix[m], mesh2pix[m+1]) of an image mesh2width[m]
+wide.  Bulk arrays live on the GPU.  Mesh m's rows of bvhnodes / bvhnode2aabb
+start at mesh2node[m] = sum_{j<m} (2 N_j - 1).
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from .. import _lib
+
+
+def _host_u32(a) -> np.ndarray:
+    if isinstance(a, torch.Tensor):
+        a = a.detach().cpu().numpy()
+    return np.ascontiguousarray(np.asarray(a, dtype=np.uint32))
+
+
+def node_offsets(mesh2tri) -> np.ndarray:
+    n = np.diff(_host_u32(mesh2tri).astype(np.int64))
+    return np.concatenate([[0], np.cumsum(np.where(n > 0, 2 * n - 1, 0))]).astype(np.uint32)
+
+
+class Pipeline:
+    """Build and ray-cast of one batch with its workspaces allocated once, so
+    that build(), cast() and run() allocate nothing; capture() records run()
+    into a HIP graph that replay() launches."""
+
+    def __init__(self, mesh2tri, mesh2vtx, tri2vtx: torch.Tensor, vtx2xyz: torch.Tensor,
+                 mesh2pix=None, mesh2width=None, ndc2world: torch.Tensor | None = None,
+                 with_coeff: bool = True):
+        dev = tri2vtx.device
+        self.device = dev
+        _lib.require_gpu(tri2vtx, vtx2xyz)
+        self.mesh2tri_h = _host_u32(mesh2tri)
+        self.mesh2vtx_h = _host_u32(mesh2vtx)
+        self.num_mesh = len(self.mesh2tri_h) - 1
+        if self.num_mesh < 1 or len(self.mesh2vtx_h) != self.num_mesh + 1:
+            raise ValueError("mesh2tri / mesh2vtx need num_mesh + 1 >= 2 entries")
+        self.total_tri = int(self.mesh2tri_h[-1])
+        self.total_vtx = int(self.mesh2vtx_h[-1])
+        if tri2vtx.shape != (self.total_tri, 3) or tri2vtx.dtype != torch.uint32:
+            raise ValueError("tri2vtx must be (mesh2tri[-1], 3) uint32")
+        if vtx2xyz.shape != (self.total_vtx, 3) or vtx2xyz.dtype != torch.float32:
+            raise ValueError("vtx2xyz must be (mesh2vtx[-1], 3) float32")
+        self.tri2vtx = tri2vtx.contiguous()
+        self.vtx2xyz = vtx2xyz.contiguous()
+        self.mesh2tri = torch.from_numpy(self.mesh2tri_h).to(dev)
+        self.mesh2vtx = torch.from_numpy(self.mesh2vtx_h).to(dev)
+        self.mesh2node_h = node_offsets(self.mesh2tri_h)
+        self.num_rows = int(self.mesh2node_h[-1])
+        self.mesh2node = torch.empty(self.num_mesh + 1, dtype=torch.uint32, device=dev)
+        self.bvhnodes = torch.empty((max(self.num_rows, 1), 3), dtype=torch.uint32, device=dev)
+        self.bvhnode2aabb = torch.empty((max(self.num_rows, 1), 6), dtype=torch.float32, device=dev)
+        L = _lib.lib()
+        self.ws_build = _lib.workspace(
+            L.dmsh_batch_build_workspace_bytes(self.num_mesh, self.total_tri), dev)
+        self.has_cast = mesh2pix is not None
+        if self.has_cast:
+            self.mesh2pix_h = _host_u32(mesh2pix)
+            self.mesh2width_h = _host_u32(mesh2width)
+            if len(self.mesh2pix_h) != self.num_mesh + 1 or len(self.mesh2width_h) != self.num_mesh:
+                raise ValueError("mesh2pix needs num_mesh + 1 entries, mesh2width num_mesh")
+            self.total_pix = int(self.mesh2pix_h[-1])
+            self.mesh2pix = torch.from_numpy(self.mesh2pix_h).to(dev)
+            self.mesh2width = torch.from_numpy(self.mesh2width_h).to(dev)
+            if ndc2world is None or tuple(ndc2world.shape) != (self.num_mesh, 4, 4):
+                raise ValueError("ndc2world must be (num_mesh, 4, 4)")
+            # column-major per mesh, as the single-mesh entry points take it
+            self.transforms = ndc2world.to(dev, torch.float32).transpose(1, 2).contiguous()
+            self.pix2tri = torch.empty(max(self.total_pix, 1), dtype=torch.uint32, device=dev)
+            self.pix2t = (torch.empty(max(self.total_pix, 1), dtype=torch.float32, device=dev)
+                          if with_coeff else None)
+            self.ws_cast = _lib.workspace(
+                L.dmsh_batch_pix2tri_workspace_bytes(self.num_mesh, self.total_tri,
+                                                     self.total_pix), dev)
+        self.graph = None
+
+    def build(self):
+        L = _lib.lib()
+        _lib.check(L.dmsh_batch_build_trimesh3(
+            _lib.stream_of(self.device), self.num_mesh, self.total_tri, self.total_vtx,
+            _lib.ptr(self.mesh2tri), _lib.ptr(self.mesh2vtx), _lib.ptr(self.tri2vtx),
+            _lib.ptr(self.vtx2xyz), _lib.ptr(self.mesh2node), _lib.ptr(self.bvhnodes),
+            _lib.ptr(self.bvhnode2aabb), _lib.ptr(self.ws_build), self.ws_build.numel()),
+            "dmsh_batch_build_trimesh3")
+
+    def cast(self):
+        if not self.has_cast:
+            raise ValueError("no images were given")
+        L = _lib.lib()
+        _lib.check(L.dmsh_batch_pix2tri(
+            _lib.stream_of(self.device), self.num_mesh, self.total_tri, self.total_vtx,
+            self.total_pix,
+            _lib.ptr(self.mesh2tri), _lib.ptr(self.mesh2vtx), _lib.ptr(self.tri2vtx),
+            _lib.ptr(self.vtx2xyz), _lib.ptr(self.mesh2node), _lib.ptr(self.bvhnodes),
+            _lib.ptr(self.bvhnode2aabb), _lib.ptr(self.mesh2pix), _lib.ptr(self.mesh2width),
+            _lib.ptr(self.transforms), _lib.ptr(self.pix2tri),
+            _lib.ptr(self.pix2t) if self.pix2t is not None else None,
+            _lib.ptr(self.ws_cast), self.ws_cast.numel()),
+            "dmsh_batch_pix2tri")
+
+    def run(self):
+        self.build()
+        if self.has_cast:
+            self.cast()
+
+    def capture(self):
+        """Record run() into a HIP graph (warm it up eagerly first)."""
+        self.run()
+        torch.cuda.synchronize(self.device)
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            self.run()
+        self.graph = g
+        return g
+
+    def replay(self):
+        if self.graph is None:
+            raise RuntimeError("capture() first")
+        self.graph.replay()
+
+    # views -------------------------------------------------------------
+    def mesh_tree(self, m: int):
+        a, b = int(self.mesh2node_h[m]), int(self.mesh2node_h[m + 1])
+        return self.bvhnodes[a:b], self.bvhnode2aabb[a:b]
+
+    def mesh_image(self, m: int):
+        a, b = int(self.mesh2pix_h[m]), int(self.mesh2pix_h[m + 1])
+        w = int(self.mesh2width_h[m])
+        h = (b - a) // w if w else 0
+        pix = self.pix2tri[a:b].view(h, w)
+        t = self.pix2t[a:b].view(h, w) if self.pix2t is not None else None
+        return pix, t
+
+
+def make_bvhnodes_bvhnode2aabb(mesh2tri, mesh2vtx, tri2vtx: torch.Tensor,
+                               vtx2xyz: torch.Tensor):
+    """(mesh2node [host u32, M+1], bvhnodes [R,3] u32, bvhnode2aabb [R,6] f32)."""
+    p = Pipeline(mesh2tri, mesh2vtx, tri2vtx, vtx2xyz)
+    p.build()
+    return p.mesh2node_h, p.bvhnodes[:p.num_rows], p.bvhnode2aabb[:p.num_rows]
+
+
+def by_raycasting(mesh2tri, mesh2vtx, tri2vtx: torch.Tensor, vtx2xyz: torch.Tensor,
+                  mesh2pix, mesh2width, ndc2world: torch.Tensor, with_coeff: bool = True):
+    """Build and cast: (pix2tri [P] u32, pix2t [P] f32 or None), flat over all
+    images in mesh order."""
+    p = Pipeline(mesh2tri, mesh2vtx, tri2vtx, vtx2xyz, mesh2pix, mesh2width, ndc2world,
+                 with_coeff)
+    p.run()
+    return p.pix2tri[:p.total_pix], (p.pix2t[:p.total_pix] if p.pix2t is not None else None)

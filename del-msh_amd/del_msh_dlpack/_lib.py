@@ -75,6 +75,20 @@ _PROTOTYPES = {
         [c_void_p, c_void_p, c_void_p, c_u32, c_void_p, c_u32, c_void_p, c_void_p, c_u32, c_u32,
          c_void_p, c_void_p, c_size],
     ),
+    "dmsh_batch_set_tuning": (c_int, [ctypes.c_char_p, c_int]),
+    "dmsh_batch_get_tuning": (c_int, [ctypes.c_char_p]),
+    "dmsh_batch_build_workspace_bytes": (c_size, [c_u32, c_u32]),
+    "dmsh_batch_build_trimesh3": (
+        c_int,
+        [c_void_p, c_u32, c_u32, c_u32, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+         c_void_p, c_void_p, c_void_p, c_size],
+    ),
+    "dmsh_batch_pix2tri_workspace_bytes": (c_size, [c_u32, c_u32, c_u32]),
+    "dmsh_batch_pix2tri": (
+        c_int,
+        [c_void_p, c_u32, c_u32, c_u32, c_u32, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+         c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_size],
+    ),
     "dmsh_timing_enable": (c_int, [c_int]),
     "dmsh_timing_query": (c_int, [ctypes.c_char_p, ctypes.POINTER(c_u32),
                                   ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]),

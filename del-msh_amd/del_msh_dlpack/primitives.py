@@ -159,3 +159,80 @@ def edge2vtx_edge2tri(tri2vtx: np.ndarray):
     edge2vtx = e[order[starts]].astype(np.uint32)
     edge2tri = np.stack([owner[order[starts]], owner[order[starts + 1]]], 1).astype(np.uint32)
     return edge2vtx, edge2tri
+
+
+def _divisions_for(kind: int, target: float):
+    """Generator divisions whose triangle count is near `target`:
+    sphere 2 n_lat (n_long - 1) with n_lat = 2 (n_long - 1); torus 2 a b with
+    a = 2 b; cylinder 2 c (l + 1) with c ~ sqrt(target)."""
+    if kind == 0:
+        k = max(1, int(round(np.sqrt(target / 4.0))))
+        return (k + 1, 2 * k) if k >= 2 else (2, 3)
+    if kind == 1:
+        b = max(3, int(round(np.sqrt(target / 4.0))))
+        return (2 * b, b)
+    c = max(3, int(round(np.sqrt(target))))
+    ln = max(1, int(round(target / (2.0 * c))) - 1)
+    return (c, ln)
+
+
+def _quaternion_matrix(q):
+    w, x, y, z = q / np.linalg.norm(q)
+    return np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - w * z), 2 * (x * z + w * y)],
+                     [2 * (x * y + w * z), 1 - 2 * (x * x + z * z), 2 * (y * z - w * x)],
+                     [2 * (x * z - w * y), 2 * (y * z + w * x), 1 - 2 * (x * x + y * y)]])
+
+
+def batch_meshes(num_mesh: int, seed: int = 3, tri_range=(256, 262144),
+                 pix_range=(64 * 64, 512 * 512)):
+    """A batch of sphere / torus / closed-cylinder meshes, concatenated in CSR
+    form, each with its own orthographic image.
+
+    Per mesh (seeded): the generator type uniform over the three; a triangle
+    target log-uniform in tri_range mapped to the nearest divisions; a uniform
+    scale in [0.5, 2] and a rotation from a normalised 4D Gaussian quaternion
+    (applied in f64, rounded once to f32); a square image whose pixel count is
+    log-uniform in pix_range, looking down -z through a box 1.1x the mesh's
+    bounding radius R (ndc2world = diag(1.1 R, 1.1 R, 2.2 R, 1)).
+
+    Returns a dict: mesh2tri, mesh2vtx, mesh2pix (u32 [M+1]), tri2vtx (u32
+    [T,3], vertex indices local to each mesh), vtx2xyz (f32 [V,3]),
+    mesh2width (u32 [M]), ndc2world (f32 [M,4,4], row-major), kinds (u8 [M])."""
+    rng = np.random.default_rng(seed)
+    tris, vtxs, widths, cams, kinds = [], [], [], [], []
+    lo, hi = np.log(tri_range[0]), np.log(tri_range[1])
+    plo, phi = np.log(pix_range[0]), np.log(pix_range[1])
+    for _ in range(num_mesh):
+        kind = int(rng.integers(3))
+        target = float(np.exp(rng.uniform(lo, hi)))
+        scale = float(rng.uniform(0.5, 2.0))
+        rot = _quaternion_matrix(rng.normal(size=4))
+        side = int(round(np.sqrt(np.exp(rng.uniform(plo, phi)))))
+        a, b = _divisions_for(kind, target)
+        if kind == 0:
+            tri, vtx = sphere_yup(1.0, a, b)
+        elif kind == 1:
+            tri, vtx = torus_zup(0.7, 0.3, a, b)
+        else:
+            tri, vtx = cylinder_closed_end_yup(0.5, 1.4, a, b)
+        v = (vtx.astype(np.float64) @ rot.T) * scale
+        r = 1.1 * float(np.sqrt((v * v).sum(axis=1)).max())
+        tris.append(tri.astype(np.uint32))
+        vtxs.append(v.astype(np.float32))
+        widths.append(side)
+        cams.append(np.diag([r, r, 2.0 * r, 1.0]).astype(np.float32))
+        kinds.append(kind)
+    ntri = np.array([t.shape[0] for t in tris], np.int64)
+    nvtx = np.array([v.shape[0] for v in vtxs], np.int64)
+    npix = np.array(widths, np.int64) ** 2
+    csr = lambda c: np.concatenate([[0], np.cumsum(c)]).astype(np.uint32)
+    return {
+        "mesh2tri": csr(ntri),
+        "mesh2vtx": csr(nvtx),
+        "mesh2pix": csr(npix),
+        "tri2vtx": np.concatenate(tris) if tris else np.zeros((0, 3), np.uint32),
+        "vtx2xyz": np.concatenate(vtxs) if vtxs else np.zeros((0, 3), np.float32),
+        "mesh2width": np.array(widths, np.uint32),
+        "ndc2world": np.stack(cams) if cams else np.zeros((0, 4, 4), np.float32),
+        "kinds": np.array(kinds, np.uint8),
+    }

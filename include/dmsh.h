@@ -278,6 +278,53 @@ int dmsh_antialias_bwd(void *stream, uint32_t num_cedge,
                        uint32_t img_h, const float *pix2val,
                        const float *dldw_pix2val, const uint32_t *pix2tri);
 
+/* ---------------------------------------------------------------------
+ * Batched build and ray-cast: many triangle meshes of uneven size, each
+ * stage one launch over all of them.  No reference counterpart (the
+ * reference builds and casts one mesh per call,
+ * del-msh-dlpack/del_msh_dlpack/TriMesh3/torch.py:233-285 and
+ * Pix2Tri/torch.py:8-61); per mesh the results equal what
+ * dmsh_build_trimesh3 (30-bit codes) and dmsh_pix2tri return for that mesh.
+ *
+ * Meshes are concatenated in CSR form (all offsets device arrays of
+ * num_mesh + 1 u32, first entry 0): triangles [mesh2tri[m], mesh2tri[m+1])
+ * with vertex indices local to vertices [mesh2vtx[m], mesh2vtx[m+1]).
+ * Empty meshes are allowed.  total_tri < 2^30.  Offsets that do not fit
+ * the totals (first entry 0, non-decreasing, last entry = total), a
+ * mesh2node other than the build's, or an image whose pixel count is not a
+ * multiple of its width set DMSH_DEVERR_INDEX and the call then writes
+ * nothing past the offsets check.
+ * Build output: mesh2node[m] (written, num_mesh + 1 entries) = the first
+ * row of mesh m in bvhnodes / bvhnode2aabb, which hold
+ * sum_m (2 N_m - 1) rows in the single-mesh layout with local indices.
+ * Ray-cast: image of mesh m = pixels [mesh2pix[m], mesh2pix[m+1]) of
+ * pix2tri / pix2t (nullable), mesh2width[m] pixels wide, row 0 at the top,
+ * cast with transforms_ndc2world + 16 m (column-major).
+ * Tuning keys (results do not depend on them): "batch.stack" (16, 24, 32),
+ * "batch.wgs_per_cu" (persistent ray-cast workgroups per CU, 1..64),
+ * "batch.treelet_grid" (refit workgroups, 1..65536).
+ * ------------------------------------------------------------------- */
+int dmsh_batch_set_tuning(const char *key, int value);
+int dmsh_batch_get_tuning(const char *key);
+size_t dmsh_batch_build_workspace_bytes(uint32_t num_mesh, uint32_t total_tri);
+int dmsh_batch_build_trimesh3(void *stream, uint32_t num_mesh, uint32_t total_tri,
+                              uint32_t total_vtx, const uint32_t *mesh2tri,
+                              const uint32_t *mesh2vtx, const uint32_t *tri2vtx,
+                              const float *vtx2xyz, uint32_t *mesh2node,
+                              uint32_t *bvhnodes, float *bvhnode2aabb, void *ws,
+                              size_t ws_bytes);
+size_t dmsh_batch_pix2tri_workspace_bytes(uint32_t num_mesh, uint32_t total_tri,
+                                          uint32_t total_pix);
+int dmsh_batch_pix2tri(void *stream, uint32_t num_mesh, uint32_t total_tri,
+                       uint32_t total_vtx, uint32_t total_pix,
+                       const uint32_t *mesh2tri,
+                       const uint32_t *mesh2vtx, const uint32_t *tri2vtx,
+                       const float *vtx2xyz, const uint32_t *mesh2node,
+                       const uint32_t *bvhnodes, const float *bvhnode2aabb,
+                       const uint32_t *mesh2pix, const uint32_t *mesh2width,
+                       const float *transforms_ndc2world, uint32_t *pix2tri,
+                       float *pix2t, void *ws, size_t ws_bytes);
+
 #ifdef __cplusplus
 }
 #endif

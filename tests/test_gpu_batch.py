@@ -1,0 +1,163 @@
+"""GPU parity for the batched build and ray-cast: every mesh of a batch gets
+exactly the tree and image the single-mesh path (and the oracle) gives it."""
+
+import numpy as np
+import pytest
+import torch
+
+from del_msh_dlpack import _lib, primitives
+import del_msh_dlpack.Batched.torch as Batched
+
+pytestmark = pytest.mark.gpu
+
+U32MAX = 0xFFFFFFFF
+
+
+def G(a, dev):
+    return torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+
+
+def H(t):
+    return t.detach().cpu().numpy()
+
+
+def no_device_errors():
+    bits = _lib.device_errors()
+    assert bits == 0, f"device error bits 0x{bits:x}"
+
+
+def bits_equal(a, b):
+    a = np.ascontiguousarray(a)
+    b = np.ascontiguousarray(b)
+    return a.shape == b.shape and np.array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
+def _edge_batch():
+    """Generated meshes plus the edge cases: an empty mesh, one and two
+    triangles, and one just over the 1024-leaf treelet size."""
+    b = primitives.batch_meshes(14, seed=11, tri_range=(6, 6000), pix_range=(16 * 16, 96 * 96))
+    tri_list, vtx_list, w_list, cam_list = [], [], [], []
+    for m in range(14):
+        t0, t1 = b["mesh2tri"][m], b["mesh2tri"][m + 1]
+        v0, v1 = b["mesh2vtx"][m], b["mesh2vtx"][m + 1]
+        tri_list.append(b["tri2vtx"][t0:t1])
+        vtx_list.append(b["vtx2xyz"][v0:v1])
+        w_list.append(int(b["mesh2width"][m]))
+        cam_list.append(b["ndc2world"][m])
+    one = (np.array([[0, 1, 2]], np.uint32), np.array([[-.5, -.5, 0], [.6, -.4, .1], [0, .7, -.1]], np.float32))
+    two = (np.array([[0, 1, 2], [0, 2, 3]], np.uint32),
+           np.array([[-.5, -.5, 0], [.5, -.5, .1], [.5, .5, 0], [-.5, .5, -.1]], np.float32))
+    tri_k, vtx_k = primitives.torus_zup(0.7, 0.3, 25, 21)  # 1050 triangles
+    empty = (np.zeros((0, 3), np.uint32), np.zeros((0, 3), np.float32))
+    for k, (t, v) in enumerate([empty, one, two, (tri_k, vtx_k), empty]):
+        pos = [3, 0, 7, 9, 16][k]
+        tri_list.insert(pos, t)
+        vtx_list.insert(pos, v)
+        w_list.insert(pos, 24 + 8 * k)
+        cam_list.insert(pos, np.diag([1.2, 1.2, 2.0, 1.0]).astype(np.float32))
+    csr = lambda c: np.concatenate([[0], np.cumsum(c)]).astype(np.uint32)
+    return {
+        "tris": tri_list, "vtxs": vtx_list,
+        "mesh2tri": csr([len(t) for t in tri_list]),
+        "mesh2vtx": csr([len(v) for v in vtx_list]),
+        "mesh2pix": csr([w * w for w in w_list]),
+        "tri2vtx": np.concatenate(tri_list), "vtx2xyz": np.concatenate(vtx_list),
+        "mesh2width": np.array(w_list, np.uint32), "ndc2world": np.stack(cam_list),
+    }
+
+
+def _pipeline(b, gpu, with_coeff=True):
+    return Batched.Pipeline(b["mesh2tri"], b["mesh2vtx"], G(b["tri2vtx"], gpu),
+                            G(b["vtx2xyz"], gpu), b["mesh2pix"], b["mesh2width"],
+                            G(b["ndc2world"], gpu), with_coeff)
+
+
+def _check_against_oracle(oracle, p, b, meshes):
+    for m in meshes:
+        tri, vtx = b["tris"][m], b["vtxs"][m]
+        bn, ab = p.mesh_tree(m)
+        pix, t = p.mesh_image(m)
+        w = int(b["mesh2width"][m])
+        if len(tri) == 0:
+            assert bn.shape[0] == 0
+            assert np.all(H(pix) == U32MAX)
+            continue
+        wbn, wab = oracle.build_trimesh3(tri, vtx)
+        assert np.array_equal(H(bn), wbn), f"mesh {m} topology"
+        assert bits_equal(H(ab), wab), f"mesh {m} boxes"
+        wpix, wt, _ = oracle.pix2tri(tri, vtx, wbn, wab, oracle.colmajor(b["ndc2world"][m]), w, w,
+                                     num_threads=8)
+        assert np.array_equal(H(pix), wpix), f"mesh {m} pix2tri"
+        assert bits_equal(H(t), wt), f"mesh {m} t"
+
+
+def test_batch_build_and_cast_match_single_meshes(gpu, oracle):
+    b = _edge_batch()
+    p = _pipeline(b, gpu)
+    p.run()
+    torch.cuda.synchronize()
+    assert np.array_equal(H(p.mesh2node), p.mesh2node_h)
+    _check_against_oracle(oracle, p, b, range(len(b["tris"])))
+    no_device_errors()
+
+
+def test_batch_results_do_not_depend_on_tuning(gpu):
+    b = _edge_batch()
+    p = _pipeline(b, gpu)
+    p.run()
+    ref = (H(p.bvhnodes).copy(), H(p.bvhnode2aabb).copy(), H(p.pix2tri).copy(), H(p.pix2t).copy())
+    L = _lib.lib()
+    saved = {k: L.dmsh_batch_get_tuning(k) for k in (b"batch.stack", b"batch.wgs_per_cu",
+                                                      b"batch.treelet_grid")}
+    try:
+        for stack, wgs, tg in ((16, 1, 3), (32, 64, 4096)):
+            assert L.dmsh_batch_set_tuning(b"batch.stack", stack) == 0
+            assert L.dmsh_batch_set_tuning(b"batch.wgs_per_cu", wgs) == 0
+            assert L.dmsh_batch_set_tuning(b"batch.treelet_grid", tg) == 0
+            p.run()
+            got = (H(p.bvhnodes), H(p.bvhnode2aabb), H(p.pix2tri), H(p.pix2t))
+            assert np.array_equal(got[0], ref[0]) and bits_equal(got[1], ref[1])
+            assert np.array_equal(got[2], ref[2]) and bits_equal(got[3], ref[3])
+    finally:
+        for k, v in saved.items():
+            L.dmsh_batch_set_tuning(k, v)
+    no_device_errors()
+
+
+def test_batch_graph_replays_match_eager(gpu):
+    b = _edge_batch()
+    p = _pipeline(b, gpu)
+    p.run()
+    torch.cuda.synchronize()
+    ref = (H(p.bvhnodes).copy(), H(p.bvhnode2aabb).copy(), H(p.pix2tri).copy(), H(p.pix2t).copy())
+    p.capture()
+    for k in range(3):
+        # eager work and a sync between replays
+        p.bvhnodes.fill_(0)
+        p.pix2tri.fill_(7)
+        torch.cuda.synchronize()
+        p.replay()
+        torch.cuda.synchronize()
+        assert np.array_equal(H(p.bvhnodes), ref[0]), f"replay {k}"
+        assert bits_equal(H(p.bvhnode2aabb), ref[1])
+        assert np.array_equal(H(p.pix2tri), ref[2])
+        assert bits_equal(H(p.pix2t), ref[3])
+    no_device_errors()
+
+
+def test_batch_generated_sample(gpu, oracle):
+    """300 generated meshes (256..65536 triangles); 16 of them checked
+    against the oracle, all of them for device errors."""
+    b = primitives.batch_meshes(300, seed=3, tri_range=(256, 65536),
+                                pix_range=(64 * 64, 160 * 160))
+    b["tris"] = [b["tri2vtx"][b["mesh2tri"][m]:b["mesh2tri"][m + 1]] for m in range(300)]
+    b["vtxs"] = [b["vtx2xyz"][b["mesh2vtx"][m]:b["mesh2vtx"][m + 1]] for m in range(300)]
+    p = _pipeline(b, gpu)
+    p.run()
+    torch.cuda.synchronize()
+    no_device_errors()
+    rng = np.random.default_rng(0)
+    sizes = np.diff(b["mesh2tri"].astype(np.int64))
+    pick = set(rng.choice(300, size=12, replace=False).tolist())
+    pick |= {int(np.argmax(sizes)), int(np.argmin(sizes)), 0, 299}
+    _check_against_oracle(oracle, p, b, sorted(pick))
