@@ -50,6 +50,9 @@ def parse():
     ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
                     help="turn two rocprofv3 --pmc output folders (FETCH_SIZE, WRITE_SIZE) "
                          "into the traffic summary text")
+    ap.add_argument("--batch-meshes", type=int, default=4096,
+                    help="meshes of the batched workload (0 skips it)")
+    ap.add_argument("--batch-reps", type=int, default=3)
     return ap.parse_args()
 
 
@@ -147,6 +150,110 @@ def read_traffic():
     except OSError:
         return None
     return None
+
+
+# ---------------------------------------------------------------------------
+# batched workload: many meshes per launch vs one mesh per launch
+# ---------------------------------------------------------------------------
+def run_batched(num_mesh: int, reps: int):
+    """Seeded batch of sphere/torus/cylinder meshes (log-uniform 256..262144
+    triangles, scale U[0.5,2], random rotation, one orthographic image per
+    mesh with log-uniform 64^2..512^2 pixels, seed 3).  Times the batched
+    build and ray-cast (autotuned, eager and replayed from a HIP graph)
+    against the plain single-mesh entry points launched once per mesh on the
+    same data, and checks that both give identical trees and images."""
+    import numpy as np
+    import torch
+
+    from del_msh_dlpack import _lib, primitives
+    import del_msh_dlpack.Batched.torch as Batched
+
+    dev = torch.device("cuda:0")
+    L = _lib.lib()
+    t0 = time.perf_counter()
+    b = primitives.batch_meshes(num_mesh, seed=3)
+    gen_s = time.perf_counter() - t0
+    p = Batched.Pipeline(b["mesh2tri"], b["mesh2vtx"], torch.from_numpy(b["tri2vtx"]).to(dev),
+                         torch.from_numpy(b["vtx2xyz"]).to(dev), b["mesh2pix"], b["mesh2width"],
+                         torch.from_numpy(b["ndc2world"]).to(dev), with_coeff=True)
+    tuned = Batched.autotune(p, reps=1)
+    build_ms = Batched._time_ms(p.build, reps)
+    cast_ms = Batched._time_ms(p.cast, reps)
+    p.capture()
+    graph_ms = Batched._time_ms(p.replay, reps)
+    errs = _lib.device_errors()
+    if errs:
+        raise RuntimeError(f"device error bits 0x{errs:x} in the batched run")
+
+    # plain: the single-mesh entry points, one mesh per call, same data
+    stream = torch.cuda.current_stream(dev).cuda_stream
+    m2t, m2v, m2n, m2p = b["mesh2tri"], b["mesh2vtx"], p.mesh2node_h, b["mesh2pix"]
+    ntri = np.diff(m2t.astype(np.int64))
+    nmax = int(ntri.max())
+    widths = b["mesh2width"]
+    ws_b = _lib.workspace(L.dmsh_build_workspace_bytes(nmax, 30), dev)
+    pmax = int(np.diff(m2p.astype(np.int64)).max())
+    ws_c = _lib.workspace(L.dmsh_pix2tri_workspace_bytes(nmax, int(widths.max()),
+                                                         int(widths.max())), dev)
+    del pmax
+    nodes = torch.empty_like(p.bvhnodes)
+    aabb = torch.empty_like(p.bvhnode2aabb)
+    pix = torch.empty_like(p.pix2tri)
+    pt = torch.empty_like(p.pix2t)
+    tri_ptr, xyz_ptr = p.tri2vtx.data_ptr(), p.vtx2xyz.data_ptr()
+    tr_ptr = p.transforms.data_ptr()
+
+    def plain_build():
+        for m in range(num_mesh):
+            n = int(ntri[m])
+            if n == 0:
+                continue
+            _lib.check(L.dmsh_build_trimesh3(
+                stream, n, tri_ptr + int(m2t[m]) * 12, int(m2v[m + 1] - m2v[m]),
+                xyz_ptr + int(m2v[m]) * 12, 30, nodes.data_ptr() + int(m2n[m]) * 12,
+                aabb.data_ptr() + int(m2n[m]) * 24, ws_b.data_ptr(), ws_b.numel()), "build")
+
+    def plain_cast():
+        for m in range(num_mesh):
+            w = int(widths[m])
+            if w == 0:
+                continue
+            _lib.check(L.dmsh_pix2tri(
+                stream, pix.data_ptr() + int(m2p[m]) * 4, pt.data_ptr() + int(m2p[m]) * 4,
+                int(ntri[m]), tri_ptr + int(m2t[m]) * 12, int(m2v[m + 1] - m2v[m]),
+                xyz_ptr + int(m2v[m]) * 12, w, int(m2p[m + 1] - m2p[m]) // w,
+                tr_ptr + m * 64, nodes.data_ptr() + int(m2n[m]) * 12,
+                aabb.data_ptr() + int(m2n[m]) * 24, ws_c.data_ptr(), ws_c.numel()), "cast")
+
+    plain_build_ms = Batched._time_ms(plain_build, 1)
+    plain_cast_ms = Batched._time_ms(plain_cast, 1)
+    same = (torch.equal(nodes[:p.num_rows], p.bvhnodes[:p.num_rows])
+            and torch.equal(aabb[:p.num_rows].view(torch.int32),
+                            p.bvhnode2aabb[:p.num_rows].view(torch.int32))
+            and torch.equal(pix, p.pix2tri) and torch.equal(pt.view(torch.int32),
+                                                            p.pix2t.view(torch.int32)))
+    errs = _lib.device_errors()
+    if errs:
+        raise RuntimeError(f"device error bits 0x{errs:x} in the plain loop")
+    tri, px = p.total_tri, p.total_pix
+    return {
+        "workload": (f"{num_mesh} meshes (sphere/torus/cylinder, 256..262144 tris log-uniform, "
+                     f"scale U[0.5,2], random rotation, seed 3), 30-bit codes, per-mesh "
+                     f"orthographic images 64^2..512^2 log-uniform"),
+        "num_mesh": num_mesh, "total_tri": tri, "total_rays": px,
+        "generate_s": round(gen_s, 2),
+        "build_ms": round(build_ms, 3), "build_mtri_per_s": round(tri / build_ms / 1e3, 1),
+        "cast_ms": round(cast_ms, 3), "cast_mrays_per_s": round(px / cast_ms / 1e3, 1),
+        "graph_build_cast_ms": round(graph_ms, 3),
+        "plain_build_ms": round(plain_build_ms, 3),
+        "plain_build_mtri_per_s": round(tri / plain_build_ms / 1e3, 1),
+        "plain_cast_ms": round(plain_cast_ms, 3),
+        "plain_cast_mrays_per_s": round(px / plain_cast_ms / 1e3, 1),
+        "speedup_build": round(plain_build_ms / build_ms, 2),
+        "speedup_cast": round(plain_cast_ms / cast_ms, 2),
+        "plain_equals_batched": bool(same),
+        "autotuned": {k: v for k, v in tuned.items() if k != "trials"},
+    }
 
 
 # ---------------------------------------------------------------------------
@@ -323,6 +430,9 @@ def main():
             "raycast_threads": threads,
         }
 
+    # ---------------- batched workload ----------------
+    batched = run_batched(args.batch_meshes, args.batch_reps) if args.batch_meshes > 0 else None
+
     # ---------------- roofline of the dominant kernel ----------------
     # build: compulsory bytes per triangle with u64 codes
     build_bytes = num_tri * 12 + num_vtx * 12 + num_tri * 12 + num_node * 12 + num_node * 24
@@ -397,6 +507,7 @@ def main():
                                "waves_per_block": L.dmsh_get_tuning(b"pix2tri.waves")}
                               if cast_counts else None),
         "kernels": kernels,
+        "batched": batched,
         "roofline": roof,
         "cpu_baseline": cpu,
     }

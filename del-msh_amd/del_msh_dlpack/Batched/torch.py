@@ -156,3 +156,53 @@ def by_raycasting(mesh2tri, mesh2vtx, tri2vtx: torch.Tensor, vtx2xyz: torch.Tens
                  with_coeff)
     p.run()
     return p.pix2tri[:p.total_pix], (p.pix2t[:p.total_pix] if p.pix2t is not None else None)
+
+
+def _time_ms(fn, reps: int) -> float:
+    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    fn()
+    torch.cuda.synchronize()
+    start.record()
+    for _ in range(reps):
+        fn()
+    stop.record()
+    torch.cuda.synchronize()
+    return start.elapsed_time(stop) / reps
+
+
+def autotune(p: Pipeline, reps: int = 2, stacks=(16, 24, 32), wgs_per_cu=(4, 8, 16, 32),
+             treelet_grids=(512, 1024, 2048)):
+    """Times the launch parameters of the batched kernels on this batch and
+    keeps the fastest (results do not depend on them).  The ray-cast is tuned
+    over (LDS stack frames, persistent workgroups per CU), the build over the
+    refit grid.  Returns {"cast": {...}, "build": {...}, "trials": [...]}."""
+    L = _lib.lib()
+    trials = []
+    best_cast = None
+    if p.has_cast:
+        p.build()
+        for s in stacks:
+            for w in wgs_per_cu:
+                _lib.check(L.dmsh_batch_set_tuning(b"batch.stack", s), "tuning")
+                _lib.check(L.dmsh_batch_set_tuning(b"batch.wgs_per_cu", w), "tuning")
+                ms = _time_ms(p.cast, reps)
+                trials.append({"stage": "cast", "batch.stack": s, "batch.wgs_per_cu": w,
+                               "ms": round(ms, 4)})
+                if best_cast is None or ms < best_cast[0]:
+                    best_cast = (ms, s, w)
+        _lib.check(L.dmsh_batch_set_tuning(b"batch.stack", best_cast[1]), "tuning")
+        _lib.check(L.dmsh_batch_set_tuning(b"batch.wgs_per_cu", best_cast[2]), "tuning")
+    best_build = None
+    for g in treelet_grids:
+        _lib.check(L.dmsh_batch_set_tuning(b"batch.treelet_grid", g), "tuning")
+        ms = _time_ms(p.build, reps)
+        trials.append({"stage": "build", "batch.treelet_grid": g, "ms": round(ms, 4)})
+        if best_build is None or ms < best_build[0]:
+            best_build = (ms, g)
+    _lib.check(L.dmsh_batch_set_tuning(b"batch.treelet_grid", best_build[1]), "tuning")
+    out = {"build": {"batch.treelet_grid": best_build[1], "ms": round(best_build[0], 4)},
+           "trials": trials}
+    if best_cast:
+        out["cast"] = {"batch.stack": best_cast[1], "batch.wgs_per_cu": best_cast[2],
+                       "ms": round(best_cast[0], 4)}
+    return out
