@@ -53,6 +53,11 @@ def parse():
     ap.add_argument("--batch-meshes", type=int, default=4096,
                     help="meshes of the batched workload (0 skips it)")
     ap.add_argument("--batch-reps", type=int, default=3)
+    ap.add_argument("--serve-requests", type=int, default=2000,
+                    help="requests per arrival rate of the served workload (0 skips it)")
+    ap.add_argument("--serve-rates", default="1000,4000,20000",
+                    help="arrival rates, requests/s")
+    ap.add_argument("--serve-budget-mib", type=int, default=4096)
     return ap.parse_args()
 
 
@@ -257,6 +262,46 @@ def run_batched(num_mesh: int, reps: int):
 
 
 # ---------------------------------------------------------------------------
+# served workload: continuous batching vs one request at a time
+# ---------------------------------------------------------------------------
+def run_served(num_req: int, rates, budget_bytes: int):
+    """Poisson request streams (mesh id Zipf(1.1) over 10k ids, 1k..1M
+    triangles with radial noise, eye on the sphere of radius 3, viewport
+    256^2..1024^2 log-uniform, seed 7) served by the continuous-batching
+    server and by the per-request baseline (single-mesh entry points), both
+    building trees on first sight into a cache of `budget_bytes`."""
+    import copy
+
+    import del_msh_dlpack.Serve.torch as Serve
+
+    cat = Serve.MeshCatalog(tri_range=(1000, 1_000_000), seed=7)
+    out = {"workload": (f"{num_req} requests per rate, mesh id Zipf(1.1) over 10000 ids, "
+                        f"1k..1M tris, perspective 256^2..1024^2, cache {budget_bytes >> 20} MiB"),
+           "rates": {}}
+    for rate in rates:
+        reqs = Serve.request_stream(num_req, rate=rate, num_ids=10_000, seed=7)
+        t0 = time.perf_counter()
+        for mid in sorted({r.mesh_id for r in reqs}):
+            cat.pinned(mid)  # host-side meshes exist (pinned) before the stream starts
+        gen_s = time.perf_counter() - t0
+        served = copy.deepcopy(reqs)
+        srv = Serve.Server(cat, budget_bytes)
+        steps = srv.run(served)
+        s_stats = Serve.latency_stats(served)
+        s_stats.update({"steps": steps, "cache_hits": srv.cache.hits,
+                        "cache_misses": srv.cache.misses, "evictions": srv.cache.evicted})
+        plain = copy.deepcopy(reqs)
+        psrv = Serve.PlainServer(cat, budget_bytes)
+        psrv.run(plain)
+        p_stats = Serve.latency_stats(plain)
+        p_stats.update({"cache_hits": psrv.cache.hits, "cache_misses": psrv.cache.misses})
+        out["rates"][str(rate)] = {"served": s_stats, "plain": p_stats,
+                                   "distinct_meshes": len({r.mesh_id for r in reqs}),
+                                   "host_generate_s": round(gen_s, 2)}
+    return out
+
+
+# ---------------------------------------------------------------------------
 # main
 # ---------------------------------------------------------------------------
 def main():
@@ -432,6 +477,8 @@ def main():
 
     # ---------------- batched workload ----------------
     batched = run_batched(args.batch_meshes, args.batch_reps) if args.batch_meshes > 0 else None
+    served = (run_served(args.serve_requests, [float(x) for x in args.serve_rates.split(",")],
+                         args.serve_budget_mib << 20) if args.serve_requests > 0 else None)
 
     # ---------------- roofline of the dominant kernel ----------------
     # build: compulsory bytes per triangle with u64 codes
@@ -508,6 +555,7 @@ def main():
                               if cast_counts else None),
         "kernels": kernels,
         "batched": batched,
+        "served": served,
         "roofline": roof,
         "cpu_baseline": cpu,
     }

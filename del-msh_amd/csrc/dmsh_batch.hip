@@ -4,6 +4,7 @@
 #include <cstring>
 
 #include "batch.hpp"
+#include "serve.hpp"
 
 using namespace dmsh;
 
@@ -98,6 +99,54 @@ inline bool carve_batch_cast(Arena& a, uint32_t M, uint32_t n, uint32_t num_pix,
 template <int S>
 void launch_batch_tier1(hipStream_t s, uint32_t grid, const bat::BatchCast& a) {
   launch("k_batch_pix2tri", bat::k_batch_pix2tri<S>, dim3(grid), dim3(64), 0, s, a);
+}
+
+// Offsets of a batch of built trees (as for the batched cast, without images).
+__global__ __launch_bounds__(1024) void k_pack_offsets(uint32_t M, uint32_t total_tri,
+                                                       uint32_t total_vtx,
+                                                       const uint32_t* __restrict__ mesh2tri,
+                                                       const uint32_t* __restrict__ mesh2vtx,
+                                                       const uint32_t* __restrict__ node_off,
+                                                       uint32_t* __restrict__ inner_off,
+                                                       uint32_t* __restrict__ bad,
+                                                       uint32_t* __restrict__ err) {
+  bat::check_offsets(mesh2tri, M, total_tri, bad, err);
+  bat::check_offsets(mesh2vtx, M, total_vtx, bad, err);
+  for (uint32_t m = threadIdx.x; m < M; m += 1024) {
+    const uint32_t n = mesh2tri[m + 1] - mesh2tri[m];
+    if (node_off[m + 1] - node_off[m] != (n > 0 ? 2 * n - 1 : 0u) || (m == 0 && node_off[0] != 0u)) {
+      atomicOr(bad, 1u);
+      raise_error(err, DMSH_DEVERR_INDEX);
+    }
+  }
+  bat::scan_meshes(M, inner_off, [&](uint32_t m) {
+    const uint32_t n = mesh2tri[m + 1] - mesh2tri[m];
+    return n > 0 ? n - 1 : 0u;
+  });
+}
+
+struct ReqLayout {
+  uint32_t* counts;
+  uint32_t* tile_off;
+  uint32_t* pix_off;
+  uint32_t* list2;
+  uint32_t* list3;
+  uint2* deep;
+};
+
+inline bool carve_req(Arena& a, uint32_t R, uint32_t num_pix, ReqLayout& L) {
+  L.counts = a.take<uint32_t>(kCastCounts);
+  L.tile_off = a.take<uint32_t>((size_t)R + 1);
+  L.pix_off = a.take<uint32_t>((size_t)R + 1);
+  L.list2 = a.take<uint32_t>(num_pix ? num_pix : 1);
+  L.list3 = a.take<uint32_t>(num_pix ? num_pix : 1);
+  L.deep = a.take<uint2>((size_t)rc::kDeepSlots * rc::kDeepFrames);
+  return a.ok();
+}
+
+template <int S>
+void launch_req_tier1(hipStream_t s, uint32_t grid, const srv::ReqCast& a) {
+  launch("k_req_pix2tri", srv::k_req_pix2tri<S>, dim3(grid), dim3(64), 0, s, a);
 }
 
 int device_cus() {
@@ -284,6 +333,93 @@ int dmsh_batch_pix2tri(void* stream, uint32_t num_mesh, uint32_t total_tri, uint
   if ((rc = launch_status()) != DMSH_OK) return rc;
   launch("k_batch_pix2tri_tier3", bat::k_batch_pix2tri_tier3, dim3(rc::kDeepSlots / 64), dim3(64),
          0, s, a, total_pix);
+  return launch_status();
+}
+
+size_t dmsh_packed_bytes(uint32_t num_tri) { return srv::packed_bytes(num_tri); }
+
+size_t dmsh_batch_pack_workspace_bytes(uint32_t num_mesh) {
+  Arena a{nullptr, ~(size_t)0, 0};
+  a.take<uint32_t>((size_t)num_mesh + 1);
+  a.take<uint32_t>(64);
+  return a.off + 256;
+}
+
+int dmsh_batch_pack(void* stream, uint32_t num_mesh, uint32_t total_tri, uint32_t total_vtx,
+                    const uint32_t* mesh2tri, const uint32_t* mesh2vtx, const uint32_t* tri2vtx,
+                    const float* vtx2xyz, const uint32_t* mesh2node, const uint32_t* bvhnodes,
+                    const float* bvhnode2aabb, const uint64_t* packed, uint32_t* nonfinite,
+                    void* ws, size_t ws_bytes) {
+  if (num_mesh == 0 || !mesh2tri || !mesh2vtx || !mesh2node || !packed || !nonfinite)
+    return DMSH_ERR_INVALID;
+  if (total_tri >= rc::kNoneTri) return DMSH_ERR_INVALID;
+  if (total_tri > 0 && (!tri2vtx || !vtx2xyz || !bvhnodes || !bvhnode2aabb))
+    return DMSH_ERR_INVALID;
+  if (!ws || ws_bytes < dmsh_batch_pack_workspace_bytes(num_mesh)) return DMSH_ERR_WORKSPACE;
+  Arena ar{static_cast<char*>(ws), ws_bytes, 0};
+  uint32_t* inner_off = ar.take<uint32_t>((size_t)num_mesh + 1);
+  uint32_t* bad = ar.take<uint32_t>(64);
+  if (!ar.ok()) return DMSH_ERR_WORKSPACE;
+  uint32_t* err = device_error_word();
+  if (!err) return DMSH_ERR_HIP;
+  hipStream_t s = as_stream(stream);
+  int rc = sortk::launch_fill(s, bad, 1, 0u);
+  if (rc != DMSH_OK) return rc;
+  if ((rc = sortk::launch_fill(s, nonfinite, num_mesh, 0u)) != DMSH_OK) return rc;
+  launch("k_pack_offsets", k_pack_offsets, dim3(1), dim3(1024), 0, s, num_mesh, total_tri,
+         total_vtx, mesh2tri, mesh2vtx, mesh2node, inner_off, bad, err);
+  if ((rc = launch_status()) != DMSH_OK) return rc;
+  if (total_tri == 0) return DMSH_OK;
+  const bat::BatchTrees bt{bad, num_mesh, mesh2tri, mesh2vtx, mesh2node,
+                           const_cast<float*>(bvhnode2aabb), bvhnodes, tri2vtx, vtx2xyz};
+  launch("k_pack_to_buffers", srv::k_pack_to_buffers, dim3(ceil_div(total_tri, 256)), dim3(256),
+         0, s, bt, total_tri, inner_off, packed, nonfinite, err);
+  if ((rc = launch_status()) != DMSH_OK) return rc;
+  launch("k_pack_flags", srv::k_pack_flags, dim3(ceil_div(num_mesh, 256)), dim3(256), 0, s,
+         num_mesh, mesh2tri, packed, nonfinite, bad);
+  return launch_status();
+}
+
+size_t dmsh_cast_requests_workspace_bytes(uint32_t num_req, uint32_t total_pix) {
+  Arena a{nullptr, ~(size_t)0, 0};
+  ReqLayout L;
+  carve_req(a, num_req, total_pix, L);
+  return a.off + 256;
+}
+
+int dmsh_cast_requests(void* stream, uint32_t num_req, const dmsh_cast_request* requests,
+                       uint32_t total_pix, uint32_t* pix2tri, float* pix2t, void* ws,
+                       size_t ws_bytes) {
+  if (num_req == 0 || total_pix == 0) return DMSH_OK;
+  if (!requests || !pix2tri) return DMSH_ERR_INVALID;
+  if (!ws || ws_bytes < dmsh_cast_requests_workspace_bytes(num_req, total_pix))
+    return DMSH_ERR_WORKSPACE;
+  Arena ar{static_cast<char*>(ws), ws_bytes, 0};
+  ReqLayout L;
+  if (!carve_req(ar, num_req, total_pix, L)) return DMSH_ERR_WORKSPACE;
+  uint32_t* err = device_error_word();
+  if (!err) return DMSH_ERR_HIP;
+  hipStream_t s = as_stream(stream);
+  int rc = sortk::launch_fill(s, L.counts, kCastCounts, 0u);
+  if (rc != DMSH_OK) return rc;
+  const srv::Request* req = reinterpret_cast<const srv::Request*>(requests);
+  launch("k_req_offsets", srv::k_req_offsets, dim3(1), dim3(1024), 0, s, num_req, total_pix, req,
+         L.tile_off, L.pix_off, L.counts + 3, err);
+  if ((rc = launch_status()) != DMSH_OK) return rc;
+  srv::ReqCast a{num_req, req, L.tile_off, L.pix_off, pix2tri, pix2t, L.counts,
+                 L.list2, L.list3, L.deep, err};
+  const uint32_t grid = (uint32_t)device_cus() * (uint32_t)g_batch_wgs_per_cu.load();
+  switch (g_batch_stack.load()) {
+    case 16: launch_req_tier1<16>(s, grid, a); break;
+    case 32: launch_req_tier1<32>(s, grid, a); break;
+    default: launch_req_tier1<24>(s, grid, a); break;
+  }
+  if ((rc = launch_status()) != DMSH_OK) return rc;
+  launch("k_req_pix2tri_tier2", srv::k_req_pix2tri_tier2, dim3(rc::kTier2Blocks), dim3(64), 0, s,
+         a, total_pix);
+  if ((rc = launch_status()) != DMSH_OK) return rc;
+  launch("k_req_pix2tri_tier3", srv::k_req_pix2tri_tier3, dim3(rc::kDeepSlots / 64), dim3(64), 0,
+         s, a, total_pix);
   return launch_status();
 }
 

@@ -89,6 +89,16 @@ _PROTOTYPES = {
         [c_void_p, c_u32, c_u32, c_u32, c_u32, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
          c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_size],
     ),
+    "dmsh_packed_bytes": (c_size, [c_u32]),
+    "dmsh_batch_pack_workspace_bytes": (c_size, [c_u32]),
+    "dmsh_batch_pack": (
+        c_int,
+        [c_void_p, c_u32, c_u32, c_u32, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+         c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_size],
+    ),
+    "dmsh_cast_requests_workspace_bytes": (c_size, [c_u32, c_u32]),
+    "dmsh_cast_requests": (
+        c_int, [c_void_p, c_u32, c_void_p, c_u32, c_void_p, c_void_p, c_void_p, c_size]),
     "dmsh_timing_enable": (c_int, [c_int]),
     "dmsh_timing_query": (c_int, [ctypes.c_char_p, ctypes.POINTER(c_u32),
                                   ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]),

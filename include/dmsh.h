@@ -325,6 +325,49 @@ int dmsh_batch_pix2tri(void *stream, uint32_t num_mesh, uint32_t total_tri,
                        const float *transforms_ndc2world, uint32_t *pix2tri,
                        float *pix2t, void *ws, size_t ws_bytes);
 
+/* ---------------------------------------------------------------------
+ * Serving: cached trees and one ray-cast launch over many requests.  No
+ * reference counterpart (the serving loop that drives these lives in
+ * del_msh_dlpack/Serve).
+ *
+ * dmsh_batch_pack packs the trees of a CSR batch (as built by
+ * dmsh_batch_build_trimesh3, same offsets) into one self-contained buffer
+ * per mesh of dmsh_packed_bytes(N_m) bytes at device address packed[m];
+ * nonfinite[m] (written) is nonzero when a box of that tree is not finite.
+ * A packed tree is all a ray-cast reads, so it is what a cache keeps.
+ *
+ * dmsh_cast_requests ray-casts requests[0..num_req) (a device array) into
+ * pix2tri / pix2t (nullable): request r fills pixels [pix_start,
+ * pix_start + img_w img_h) with pix_start the running sum of the earlier
+ * requests' pixel counts (so total_pix is the sum of all of them); per
+ * pixel the result equals dmsh_pix2tri on the request's mesh and camera.
+ * Tuning keys as for the batched ray-cast.
+ * ------------------------------------------------------------------- */
+typedef struct dmsh_cast_request {
+  uint64_t packed;    /* device address of a tree packed by dmsh_batch_pack */
+  uint32_t num_tri;   /* triangles of that tree                             */
+  uint32_t nonfinite; /* as dmsh_batch_pack wrote it for the tree            */
+  uint32_t img_w, img_h;
+  uint32_t pix_start;
+  uint32_t reserved;
+  float ndc2world[16]; /* column-major */
+} dmsh_cast_request;   /* 96 bytes */
+
+size_t dmsh_packed_bytes(uint32_t num_tri);
+size_t dmsh_batch_pack_workspace_bytes(uint32_t num_mesh);
+int dmsh_batch_pack(void *stream, uint32_t num_mesh, uint32_t total_tri,
+                    uint32_t total_vtx, const uint32_t *mesh2tri,
+                    const uint32_t *mesh2vtx, const uint32_t *tri2vtx,
+                    const float *vtx2xyz, const uint32_t *mesh2node,
+                    const uint32_t *bvhnodes, const float *bvhnode2aabb,
+                    const uint64_t *packed, uint32_t *nonfinite, void *ws,
+                    size_t ws_bytes);
+size_t dmsh_cast_requests_workspace_bytes(uint32_t num_req, uint32_t total_pix);
+int dmsh_cast_requests(void *stream, uint32_t num_req,
+                       const dmsh_cast_request *requests, uint32_t total_pix,
+                       uint32_t *pix2tri, float *pix2t, void *ws,
+                       size_t ws_bytes);
+
 #ifdef __cplusplus
 }
 #endif

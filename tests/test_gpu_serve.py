@@ -1,0 +1,62 @@
+"""GPU: the serving loop's images equal what each request gets alone from the
+single-mesh entry points, with trees evicted and rebuilt along the way."""
+
+import copy
+
+import numpy as np
+import pytest
+import torch
+
+from del_msh_dlpack import _lib
+import del_msh_dlpack.Serve.torch as Serve
+
+pytestmark = pytest.mark.gpu
+
+
+def H(t):
+    return t.detach().cpu().numpy()
+
+
+def test_served_images_equal_each_request_alone(gpu):
+    cat = Serve.MeshCatalog(tri_range=(200, 20000), seed=2)
+    reqs = Serve.request_stream(80, rate=5000.0, num_ids=40, side_range=(24, 96), seed=5)
+    alone = copy.deepcopy(reqs)
+    # room for a handful of trees only: the mix evicts and rebuilds
+    srv = Serve.Server(cat, budget_bytes=3 << 20, device=gpu, max_step_requests=16,
+                       max_step_pixels=1 << 15)
+    steps = srv.run(reqs, keep_results=True)
+    torch.cuda.synchronize()
+    assert steps > 1
+    assert srv.cache.evicted > 0 and srv.cache.hits > 0
+    assert all(r.done >= r.arrival for r in reqs)
+    Serve.PlainServer(cat, budget_bytes=1 << 30, device=gpu).run(alone, keep_results=True)
+    for a, b in zip(reqs, alone):
+        pa, ta = a.result
+        pb, tb = b.result
+        assert np.array_equal(H(pa), H(pb)), a.rid
+        assert np.array_equal(H(ta).view(np.uint32), H(tb).view(np.uint32)), a.rid
+        assert (H(pa) != 0xFFFFFFFF).any(), a.rid  # the camera sees the mesh
+    bits = _lib.device_errors()
+    assert bits == 0, hex(bits)
+
+
+def test_cast_requests_rejects_bad_offsets(gpu):
+    """A request whose pixel offset is not the running sum is rejected on the
+    device: nothing is written and DMSH_DEVERR_INDEX is raised."""
+    cat = Serve.MeshCatalog(tri_range=(200, 400), seed=2)
+    srv = Serve.Server(cat, budget_bytes=1 << 30, device=gpu)
+    reqs = Serve.request_stream(2, rate=1e6, num_ids=3, side_range=(16, 16), seed=1)
+    srv.run(reqs)  # builds the trees
+    _lib.device_errors()
+    recs = np.zeros(2, Serve.REQ_DTYPE)
+    for k, r in enumerate(reqs):
+        buf, n, nf = srv.cache.entries[r.mesh_id].value
+        recs[k] = (buf.data_ptr(), n, 0, 16, 16, 0, 0, r.ndc2world.T.reshape(-1))  # both start at 0
+    d_req = torch.from_numpy(recs.view(np.uint8)).to(gpu)
+    pix = torch.full((512,), 7, dtype=torch.uint32, device=gpu)
+    L = _lib.lib()
+    ws = _lib.workspace(L.dmsh_cast_requests_workspace_bytes(2, 512), gpu)
+    _lib.check(L.dmsh_cast_requests(torch.cuda.current_stream().cuda_stream, 2, d_req.data_ptr(),
+                                    512, pix.data_ptr(), None, ws.data_ptr(), ws.numel()), "cast")
+    assert _lib.device_errors() & 0x1
+    assert np.all(H(pix) == 7)
