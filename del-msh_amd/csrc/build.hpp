@@ -5,7 +5,10 @@
 //   centroid    triangle centroids ((0 + p0 + p1 + p2) * (1/3), as
 //               del-msh-cpu/src/elem2center.rs:8-37) and the bounding box of
 //               the centroids, reduced through order-preserving integer keys so
-//               the box does not depend on reduction order
+//               the box does not depend on reduction order; also each
+//               triangle's box (the refit's leaf box, in triangle order, so the
+//               refit reads one 24-byte record per leaf instead of gathering
+//               the indices and three vertices)
 //   morton      unit-cube fit of that box (Mat44/torch.py:120-140 formula),
 //               Morton code per element, and the digit histograms of every
 //               radix pass (the sort's histogram read is fused in here)
@@ -38,6 +41,7 @@ __global__ __launch_bounds__(256) void k_centroid_aabb(uint32_t n,
                                                        uint32_t num_vtx,
                                                        const float* __restrict__ xyz,
                                                        float* __restrict__ cntr,
+                                                       float* __restrict__ tri_box,
                                                        uint32_t* __restrict__ keys6,
                                                        uint32_t* __restrict__ err) {
   __shared__ uint32_t s_red[4][6];
@@ -62,6 +66,11 @@ __global__ __launch_bounds__(256) void k_centroid_aabb(uint32_t n,
         c[d] = cog * ratio;
         cntr[(size_t)i * 3 + d] = c[d];
       }
+      Box b;
+      box_set(b, xyz + (size_t)v[0] * 3, 0.f);
+      box_add(b, xyz + (size_t)v[1] * 3, 0.f);
+      box_add(b, xyz + (size_t)v[2] * 3, 0.f);
+      box_store(tri_box, i, b);
     } else {
 #pragma unroll
       for (int d = 0; d < 3; ++d) c[d] = xyz[(size_t)i * 3 + d];
@@ -163,7 +172,12 @@ struct BuildLayout {
   uint32_t* flags;
   uint32_t* treelet_count;
   uint4* treelets;
+  uint32_t* top_count;
+  uint32_t* treelet_next;
+  uint2* top_list;
+  uint32_t* node2slot;
   float* cntr;
+  float* tri_box;
   void* codes;
   uint32_t* idx;
   uint32_t* zero_begin;
@@ -184,12 +198,17 @@ inline bool carve(Arena& a, uint32_t n, bool tri, BuildLayout& L) {
   L.flags = L.sort.status + status_words;
   a.off += sizeof(uint32_t) * (n > 1 ? n - 1 : 1);
   L.treelet_count = L.flags + (n > 1 ? n - 1 : 1);
-  a.off += sizeof(uint32_t);
+  L.top_count = L.treelet_count + 1;
+  L.treelet_next = L.top_count + 1;
+  a.off += 3 * sizeof(uint32_t);
   L.zero_begin = L.sort.hist;
-  L.zero_words = (size_t)(L.treelet_count - L.sort.hist) + 1;
+  L.zero_words = (size_t)(L.treelet_next - L.sort.hist) + 1;
   L.sort.zero_words = 0;
   L.treelets = a.take<uint4>(n);
+  L.top_list = a.take<uint2>(kTopSlots);
+  L.node2slot = a.take<uint32_t>(n);
   L.cntr = tri ? a.take<float>((size_t)n * 3) : nullptr;
+  L.tri_box = tri ? a.take<float>((size_t)n * 6) : nullptr;
   L.codes = a.take<KeyT>(n);
   L.idx = a.take<uint32_t>(n);
   L.sort.alt_keys = a.take<KeyT>(n);
@@ -224,7 +243,8 @@ inline int build(hipStream_t s, uint32_t n, const uint32_t* tri2vtx, uint32_t nu
                      dim3(256), 0, s, L.keys6, L.zero_begin, L.zero_words);
   if ((rc = launch_status()) != DMSH_OK) return rc;
   launch("k_centroid_aabb", k_centroid_aabb<TRI>, dim3(std::min<uint32_t>(ceil_div(n, 256), 1024)),
-                     dim3(256), 0, s, n, tri2vtx, TRI ? num_vtx : n, xyz, L.cntr, L.keys6, err);
+                     dim3(256), 0, s, n, tri2vtx, TRI ? num_vtx : n, xyz, L.cntr, L.tri_box,
+                     L.keys6, err);
   if ((rc = launch_status()) != DMSH_OK) return rc;
   KeyT* codes = static_cast<KeyT*>(L.codes);
   const uint32_t mh_blocks = std::min<uint32_t>(sortk::num_tiles<KeyT>(n), 2048);
@@ -236,10 +256,12 @@ inline int build(hipStream_t s, uint32_t n, const uint32_t* tri2vtx, uint32_t nu
   if (rc != DMSH_OK) return rc;
   rc = launch_topology<KeyT>(s, n, bvhnodes, codes, L.idx,
                             TreeletSink{kTreeletLeaves, L.treelet_count, L.treelets, n, nullptr,
-                                        nullptr});
+                                        nullptr, L.top_count, L.top_list, (uint32_t)kTopSlots,
+                                        L.node2slot});
   if (rc != DMSH_OK) return rc;
   return launch_refit_treelets<TRI>(s, aabb, L.flags, L.treelet_count, L.treelets, n, bvhnodes,
-                                    tri2vtx, num_vtx, xyz, 0.f, err);
+                                    tri2vtx, num_vtx, xyz, 0.f, err, L.top_count, L.top_list,
+                                    L.node2slot, L.treelet_next, L.tri_box);
 }
 
 }  // namespace bld

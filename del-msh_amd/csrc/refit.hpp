@@ -139,21 +139,27 @@ struct TreeView {
   const uint32_t* tri2vtx;
   uint32_t num_vtx;
   const float* xyz;
+  const float* tri_box = nullptr;  // per-triangle boxes in triangle order, if built
 };
 
-// Leaf box of leaf k; returns false (and raises) on a bad index.
+// Leaf box of leaf k; returns false (and raises) on a bad index.  With
+// tri_box (the triangle boxes the build's centroid pass wrote, eps = 0) the
+// box is one 24-byte read.
 template <bool TRI>
 __device__ __forceinline__ bool leaf_box(const uint32_t* __restrict__ bvhnodes, uint32_t num_leaf,
                                          uint32_t k, const uint32_t* __restrict__ tri2vtx,
                                          uint32_t num_vtx, const float* __restrict__ xyz, float eps,
-                                         Box& box, uint32_t& parent, uint32_t* err) {
+                                         Box& box, uint32_t& parent, uint32_t* err,
+                                         const float* __restrict__ tri_box = nullptr) {
   const uint32_t* row = bvhnodes + (size_t)(num_leaf - 1 + k) * 3;
   const uint32_t p = row[0], obj = row[1], tag = row[2];
   if (tag != DMSH_U32MAX || obj >= (TRI ? num_leaf : num_vtx)) {
     raise_error(err, DMSH_DEVERR_INDEX);
     return false;
   }
-  if (TRI) {
+  if (TRI && tri_box) {
+    box = box_load(tri_box, obj);
+  } else if (TRI) {
     const uint32_t* tv = tri2vtx + (size_t)obj * 3;
     const uint32_t i0 = tv[0], i1 = tv[1], i2 = tv[2];
     if (i0 >= num_vtx || i1 >= num_vtx || i2 >= num_vtx) {
@@ -224,7 +230,8 @@ __device__ __forceinline__ void refit_treelet(const TreeView& v, uint32_t root, 
     }
     Box b;
     uint32_t p;
-    if (leaf_box<TRI>(v.bvhnodes, num_leaf, id, v.tri2vtx, v.num_vtx, v.xyz, eps, b, p, err)) {
+    if (leaf_box<TRI>(v.bvhnodes, num_leaf, id, v.tri2vtx, v.num_vtx, v.xyz, eps, b, p, err,
+                      v.tri_box)) {
       s.leaf[j] = b;
       box_store(v.aabb, nb + id, b);
       if (nb + id != root) lpar[r] = p;
@@ -310,18 +317,34 @@ __device__ __forceinline__ void refit_top(const TreeView& v, uint32_t* __restric
   }
 }
 
-// persistent grid over the treelets of one tree
+// Persistent grid over the treelets of one tree.  With a zeroed `next`
+// counter the workgroups take treelets from it one at a time (treelet sizes
+// vary from 1 to K leaves, so a fixed deal would leave workgroups idle);
+// without it, treelet t goes to workgroup t mod grid.
 template <bool TRI, int K, int T>
 __global__ __launch_bounds__(T) void k_refit_treelet(
     float* __restrict__ aabb, uint32_t num_leaf, const uint32_t* __restrict__ bvhnodes,
     const uint32_t* __restrict__ tri2vtx, uint32_t num_vtx, const float* __restrict__ xyz,
     float eps, const uint32_t* __restrict__ treelet_count, const uint4* __restrict__ treelets,
-    uint32_t* __restrict__ err) {
+    uint32_t* __restrict__ next, const float* __restrict__ tri_box, uint32_t* __restrict__ err) {
   __shared__ TreeletLds<K> s;
-  const TreeView v{aabb, bvhnodes, num_leaf, tri2vtx, num_vtx, xyz};
+  __shared__ uint32_t s_t;
+  const TreeView v{aabb, bvhnodes, num_leaf, tri2vtx, num_vtx, xyz, tri_box};
   uint32_t count = *treelet_count;
   if (count > num_leaf) count = num_leaf;
-  for (uint32_t t = blockIdx.x; t < count; t += gridDim.x) {
+  if (!next) {
+    for (uint32_t t = blockIdx.x; t < count; t += gridDim.x) {
+      const uint4 tl = treelets[t];
+      refit_treelet<TRI, K, T>(v, tl.x, tl.y, tl.z, eps, s, err);
+    }
+    return;
+  }
+  for (;;) {
+    if (threadIdx.x == 0) s_t = atomicAdd(next, 1u);
+    __syncthreads();
+    const uint32_t t = s_t;
+    __syncthreads();  // every thread has read s_t before it is reused
+    if (t >= count) break;
     const uint4 tl = treelets[t];
     refit_treelet<TRI, K, T>(v, tl.x, tl.y, tl.z, eps, s, err);
   }
@@ -340,28 +363,163 @@ __global__ __launch_bounds__(T) void k_refit_top(float* __restrict__ aabb,
   refit_top<T>(v, flags, reinterpret_cast<const uint32_t*>(treelets), 4u, count, err);
 }
 
+// The nodes above the treelets, merged in LDS by one workgroup.  The topology
+// kernel lists them (top_list, node2slot; see TreeletSink); slot e of this
+// kernel is top_list[e], owned by thread e mod T.
+//   Phase 1 reads each slot's node row, maps its top children to their slots
+//   (kept in registers) and reads the boxes of its other children -- treelet
+//   roots or leaves, which the treelet pass wrote.  A node with no top child
+//   is finished at once; a node with one keeps the other child's box in its
+//   slot until the top child is done.
+//   Phase 2 runs in rounds: in round t a thread finishes each of its pending
+//   nodes whose top children were finished in an earlier round (done[] < t),
+//   merging children in (left, right) order.  A round ends at a barrier, so a
+//   box is read only after the round that wrote it; the number of rounds is
+//   the height of the top tree, and no atomics are needed.
+//   Phase 3 writes the boxes out.
+// More than C top nodes (an unbalanced tree) fall back to the climb through
+// global counters.
+constexpr uint32_t kTopRef = 0x80000000u;
+constexpr uint8_t kTopPending = 0xFFu;
+
+template <int C, int T>
+__global__ __launch_bounds__(T) void k_refit_top_lds(
+    float* __restrict__ aabb, uint32_t* __restrict__ flags, uint32_t num_leaf,
+    const uint32_t* __restrict__ bvhnodes, const uint32_t* __restrict__ top_count,
+    const uint2* __restrict__ top_list, const uint32_t* __restrict__ node2slot,
+    const uint32_t* __restrict__ treelet_count, const uint4* __restrict__ treelets,
+    uint32_t* __restrict__ err) {
+  constexpr int R = (C + T - 1) / T;  // slots per thread
+  const TreeView v{aabb, bvhnodes, num_leaf, nullptr, 0u, nullptr};
+  const uint32_t count = *top_count;
+  if (count > (uint32_t)C) {
+    uint32_t nt = *treelet_count;
+    if (nt > num_leaf) nt = num_leaf;
+    refit_top<T>(v, flags, reinterpret_cast<const uint32_t*>(treelets), 4u, nt, err);
+    return;
+  }
+  __shared__ Box s_box[C];
+  __shared__ uint8_t s_done[C];  // round that finished the slot, or kTopPending
+  const uint32_t nb = num_leaf - 1;
+  // phase 1, stage by stage over this thread's slots e = r T + tid
+  uint2 rec[R];
+  uint2 kid[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const uint32_t e = (uint32_t)r * T + threadIdx.x;
+    rec[r] = e < count ? top_list[e] : make_uint2(DMSH_U32MAX, 0u);
+  }
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    kid[r] = make_uint2(0u, 0u);
+    if (rec[r].x < nb) {
+      const uint32_t* pr = bvhnodes + (size_t)rec[r].x * 3;
+      kid[r] = make_uint2(pr[1], pr[2]);
+    }
+  }
+  bool bad = false;
+  uint32_t pending = 0u;  // bit r: slot r T + tid still to merge
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const uint32_t e = (uint32_t)r * T + threadIdx.x;
+    if (e >= count) continue;
+    const bool ta = rec[r].y & 1u, tb = rec[r].y & 2u;
+    const uint32_t a = kid[r].x, b = kid[r].y;
+    bool ok = rec[r].x < nb && (ta ? a < nb : a <= 2 * nb) && (tb ? b < nb : b <= 2 * nb);
+    if (ok && ta) {
+      kid[r].x = node2slot[a];
+      ok = kid[r].x < count;
+      kid[r].x |= kTopRef;
+    }
+    if (ok && tb) {
+      kid[r].y = node2slot[b];
+      ok = kid[r].y < count;
+      kid[r].y |= kTopRef;
+    }
+    if (!ok) {
+      bad = true;
+      continue;
+    }
+    if (!ta && !tb) {
+      s_box[e] = box_merge(box_load(aabb, a), box_load(aabb, b));
+      s_done[e] = 0u;
+    } else {
+      if (!ta || !tb) s_box[e] = box_load(aabb, ta ? b : a);  // the non-top child
+      s_done[e] = kTopPending;
+      pending |= 1u << r;
+    }
+  }
+  if (__syncthreads_or(bad)) {
+    if (threadIdx.x == 0) raise_error(err, DMSH_DEVERR_REFIT);
+    return;
+  }
+  // phase 2
+  for (uint32_t t = 1; __syncthreads_or(pending != 0u); ++t) {
+    if (t > count || t >= kTopPending) {
+      // a top tree higher than the done[] rounds can count (or a corrupt
+      // one): redo the nodes above the treelets with the global climb
+      uint32_t nt = *treelet_count;
+      if (nt > num_leaf) nt = num_leaf;
+      refit_top<T>(v, flags, reinterpret_cast<const uint32_t*>(treelets), 4u, nt, err);
+      return;
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      if (!(pending & (1u << r))) continue;
+      const uint32_t e = (uint32_t)r * T + threadIdx.x;
+      const uint2 k = kid[r];
+      const uint32_t ia = (k.x & kTopRef) ? k.x & ~kTopRef : e;
+      const uint32_t ib = (k.y & kTopRef) ? k.y & ~kTopRef : e;
+      const bool ra = ia == e || s_done[ia] < t;
+      const bool rb = ib == e || s_done[ib] < t;
+      if (!(ra && rb)) continue;
+      s_box[e] = box_merge(s_box[ia], s_box[ib]);
+      s_done[e] = (uint8_t)t;
+      pending &= ~(1u << r);
+    }
+  }
+  // phase 3 (the loop's last barrier orders it after every merge)
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const uint32_t e = (uint32_t)r * T + threadIdx.x;
+    if (e < count) box_store(aabb, rec[r].x, s_box[e]);
+  }
+}
+
 constexpr uint32_t kTreeletLeaves = 1024;  // K
 constexpr int kTreeletThreads = 256;
 constexpr uint32_t kTreeletGrid = 1024;    // persistent workgroups
 constexpr int kTopThreads = 1024;
+constexpr int kTopSlots = 6400;  // 25 B each: 156 KB of LDS
 
-// flags: num_leaf - 1 zeroed counters; treelets as listed by the topology
+// flags: num_leaf - 1 zeroed counters; treelets (and, when top_count is not
+// null, the top nodes with top_cap = kTopSlots) as listed by the topology
 // kernel with k = kTreeletLeaves.
 template <bool TRI>
 inline int launch_refit_treelets(hipStream_t s, float* aabb, uint32_t* flags,
                                  const uint32_t* treelet_count, const uint4* treelets,
                                  uint32_t num_leaf, const uint32_t* bvhnodes,
                                  const uint32_t* tri2vtx, uint32_t num_vtx, const float* xyz,
-                                 float eps, uint32_t* err) {
+                                 float eps, uint32_t* err, const uint32_t* top_count = nullptr,
+                                 const uint2* top_list = nullptr,
+                                 const uint32_t* node2slot = nullptr,
+                                 uint32_t* treelet_next = nullptr,
+                                 const float* tri_box = nullptr) {
   if (num_leaf == 0) return DMSH_OK;
   const uint32_t grid = std::min<uint32_t>(kTreeletGrid, num_leaf);
   launch("k_refit_treelet", (k_refit_treelet<TRI, (int)kTreeletLeaves, kTreeletThreads>),
          dim3(grid), dim3(kTreeletThreads), 0, s, aabb, num_leaf, bvhnodes, tri2vtx, num_vtx, xyz,
-         eps, treelet_count, treelets, err);
+         eps, treelet_count, treelets, treelet_next, tri_box, err);
   int rc = launch_status();
   if (rc != DMSH_OK || num_leaf <= kTreeletLeaves) return rc;
-  launch("k_refit_top", k_refit_top<kTopThreads>, dim3(1), dim3(kTopThreads), 0, s, aabb, flags,
-         num_leaf, bvhnodes, treelet_count, treelets, err);
+  if (top_count) {
+    launch("k_refit_top", k_refit_top_lds<kTopSlots, kTopThreads>, dim3(1), dim3(kTopThreads), 0,
+           s, aabb, flags, num_leaf, bvhnodes, top_count, top_list, node2slot, treelet_count,
+           treelets, err);
+  } else {
+    launch("k_refit_top", k_refit_top<kTopThreads>, dim3(1), dim3(kTopThreads), 0, s, aabb,
+           flags, num_leaf, bvhnodes, treelet_count, treelets, err);
+  }
   return launch_status();
 }
 

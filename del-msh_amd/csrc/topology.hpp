@@ -93,6 +93,10 @@ __device__ __forceinline__ uint32_t node_split(const KeyT* __restrict__ c, uint3
 // Entries are (node id, f, l, mesh), appended in any order; at most n of
 // them.  Batched builds also append each treelet root to its mesh's own list
 // (mesh_roots + root_base, counted in mesh_root_count[mesh]) for the top climb.
+// Optionally the nodes above the treelets ("top" nodes, range > k) are
+// listed too: top_list[slot] = (node, bit 0: left child is top, bit 1: right
+// child is top) and node2slot[node] = slot, for slots < top_cap; top_count
+// counts all of them, so a count above top_cap says the list is incomplete.
 struct TreeletSink {
   uint32_t k;
   uint32_t* count;  // zeroed by the caller
@@ -100,6 +104,10 @@ struct TreeletSink {
   uint32_t cap;
   uint32_t* mesh_root_count;  // batched only, zeroed by the caller
   uint32_t* mesh_roots;
+  uint32_t* top_count = nullptr;  // zeroed by the caller; nullptr: no top list
+  uint2* top_list = nullptr;
+  uint32_t top_cap = 0;
+  uint32_t* node2slot = nullptr;
 };
 
 __device__ __forceinline__ void emit_treelet(const TreeletSink& ts, uint32_t node, uint32_t f,
@@ -127,6 +135,14 @@ __device__ __forceinline__ void topology_node(uint32_t n, uint32_t i, uint32_t* 
   const uint32_t split = node_split<KeyT>(codes, first, last);
   if (ts.list) {
     if (last - first + 1 > ts.k) {
+      if (ts.top_count) {
+        const uint32_t slot = atomicAdd(ts.top_count, 1u);
+        if (slot < ts.top_cap) {
+          ts.top_list[slot] = make_uint2(i, (split - first + 1 > ts.k ? 1u : 0u) |
+                                                (last - split > ts.k ? 2u : 0u));
+          ts.node2slot[i] = slot;
+        }
+      }
       if (split - first + 1 <= ts.k)
         emit_treelet(ts, first == split ? nb + split : split, first, split, mesh, root_base);
       if (last - split <= ts.k)
