@@ -29,10 +29,11 @@ constexpr uint32_t kFlagAgg = 1u << 30;
 constexpr uint32_t kFlagPrefix = 2u << 30;
 constexpr uint32_t kValueMask = (1u << 30) - 1u;
 constexpr uint32_t kSpinLimit = 1u << 22;
+constexpr int kLookBack = 8;
 
 template <typename KeyT>
 struct Tune {
-  static constexpr int kItems = 8;
+  static constexpr int kItems = 16;
 };
 
 __device__ __forceinline__ uint32_t wave_inclusive_scan(uint32_t v) {
@@ -179,25 +180,40 @@ __global__ __launch_bounds__(256) void k_onesweep(
                      __HIP_MEMORY_SCOPE_AGENT);
   const uint32_t excl = block_exclusive_scan(cnt, s_wsum);
   const uint32_t goff = block_exclusive_scan(hist[d], s_wsum);
-  // Decoupled look-back over the preceding tiles for digit d.
+  // Decoupled look-back over the preceding tiles for digit d, kLookBack
+  // status words per round trip: tiles that start together all publish their
+  // counts at about the same time and then walk back over each other, so a
+  // walk of k tiles costs k / kLookBack memory latencies instead of k.
   uint32_t prefix = 0;
   if (tile > 0) {
-    uint32_t tt = tile - 1, spins = 0;
-    for (;;) {
-      const uint32_t v = __hip_atomic_load(status + (size_t)tt * kRadix + d, __ATOMIC_RELAXED,
-                                           __HIP_MEMORY_SCOPE_AGENT);
-      const uint32_t flag = v & ~kValueMask;
-      if (flag == 0u) {
+    uint32_t tt = tile - 1, spins = 0;  // next tile to read
+    bool done = false;
+    while (!done) {
+      uint32_t v[kLookBack];
+#pragma unroll
+      for (int b = 0; b < kLookBack; ++b)
+        v[b] = (uint32_t)b <= tt ? __hip_atomic_load(status + (size_t)(tt - b) * kRadix + d,
+                                                     __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                                 : 0u;
+      uint32_t used = 0;  // words consumed this round, in tile order
+#pragma unroll
+      for (int b = 0; b < kLookBack; ++b) {
+        if (done || used != (uint32_t)b) continue;
+        const uint32_t flag = v[b] & ~kValueMask;
+        if (flag == 0u) continue;  // not published yet: read again from this tile
+        prefix += v[b] & kValueMask;
+        ++used;
+        if (flag == kFlagPrefix || tt == (uint32_t)b) done = true;
+      }
+      if (done) break;
+      if (used < (uint32_t)kLookBack) {
         if (++spins > kSpinLimit) {
           raise_error(err, DMSH_DEVERR_SORT_PROGRESS);
           break;
         }
         __builtin_amdgcn_s_sleep(1);
-        continue;
       }
-      prefix += v & kValueMask;
-      if (flag == kFlagPrefix || tt == 0) break;
-      --tt;
+      tt -= used;
     }
     __hip_atomic_store(my_status, kFlagPrefix | (prefix + cnt), __ATOMIC_RELAXED,
                        __HIP_MEMORY_SCOPE_AGENT);
