@@ -1,6 +1,6 @@
-"""Mid-level contour extraction (reference: del_msh_dlpack/Edge2Vtx/__init__.py,
-native function edge2vtx_contour_for_triangle_mesh at
-del-msh-dlpack/src/edge2vtx.rs:19-115)."""
+"""Mid-level edge functions (reference: native functions
+edge2vtx_contour_for_triangle_mesh and edge2vtx_from_vtx2vtx at
+del-msh-dlpack/src/edge2vtx.rs:19-115 and 117-174)."""
 
 import torch as _torch
 
@@ -30,3 +30,13 @@ def contour_for_triangle_mesh(tri2vtx, vtx2xyz, transform_world2ndc, edge2vtx, e
     )
     n = int(count.view(_torch.int32).cpu()[0])
     return cedge2vtx[:n].clone()
+
+
+def from_vtx2vtx(vtx2idx_offset, idx2vtx, edge2vtx, stream_ptr=0):
+    """edge2vtx[idx] = (i, idx2vtx[idx]) for every entry idx of vertex i (in place)."""
+    vtx2idx_offset, idx2vtx, edge2vtx = map(as_tensor, (vtx2idx_offset, idx2vtx, edge2vtx))
+    dev = _lib.require_gpu(vtx2idx_offset, idx2vtx, edge2vtx)
+    stream = stream_ptr or _lib.stream_of(dev)
+    _lib.check(_lib.lib().dmsh_edge2vtx_from_vtx2vtx(
+        stream, vtx2idx_offset.shape[0] - 1, _lib.ptr(vtx2idx_offset), idx2vtx.shape[0],
+        _lib.ptr(idx2vtx), _lib.ptr(edge2vtx)), "dmsh_edge2vtx_from_vtx2vtx")

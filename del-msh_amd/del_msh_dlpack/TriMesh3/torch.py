@@ -92,3 +92,39 @@ def make_bvhnode2aabb_from_bvhnodes(tri2vtx: torch.Tensor, vtx2xyz: torch.Tensor
     native(tri2vtx, vtx2xyz.detach(), vtx2xyz1, bvhnodes, bvhnode2aabb,
            _lib.stream_of(bvhnodes.device))
     return bvhnode2aabb
+
+
+def make_edge2vtx(tri2vtx: torch.Tensor, num_vtx: int) -> torch.Tensor:
+    """Edges of a triangle mesh from its vertex-to-vertex adjacency
+    (TriMesh3/torch.py:288-302 in the reference): each mesh edge appears once
+    from each end, (i, j) and (j, i)."""
+    assert tri2vtx.ndim == 2 and tri2vtx.shape[1] == 3
+    assert tri2vtx.is_contiguous()
+    assert tri2vtx.dtype == torch.uint32
+    from ..Vtx2Vtx.torch import from_uniform_mesh
+    from ..Edge2Vtx.torch import from_vtx2vtx
+
+    vtx2idx_offset, idx2vtx = from_uniform_mesh(tri2vtx, num_vtx, False)
+    edge2vtx = torch.empty((idx2vtx.shape[0], 2), dtype=torch.uint32, device=tri2vtx.device)
+    from_vtx2vtx(vtx2idx_offset, idx2vtx, edge2vtx)
+    return edge2vtx
+
+
+def make_edge2tri(tri2vtx: torch.Tensor, num_vtx: int, edge2vtx: torch.Tensor) -> torch.Tensor:
+    """The two triangles of each edge (TriMesh3/torch.py:305-325 in the
+    reference).  Like the reference, the output starts uninitialised: an edge
+    with fewer than two triangles leaves the missing entries as they were."""
+    assert tri2vtx.ndim == 2 and tri2vtx.shape[1] == 3
+    assert tri2vtx.is_contiguous()
+    assert tri2vtx.dtype == torch.uint32
+    assert edge2vtx.ndim == 2 and edge2vtx.shape[1] == 2
+    assert edge2vtx.is_contiguous()
+    assert edge2vtx.dtype == torch.uint32
+    assert tri2vtx.device == edge2vtx.device
+    from ..Vtx2Elem.torch import from_uniform_mesh
+    from ..Edge2Elem.torch import from_edge2vtx_of_tri2vtx_with_vtx2vtx
+
+    vtx2jdx_offset, jdx2tri = from_uniform_mesh(tri2vtx, num_vtx)
+    edge2tri = torch.empty((edge2vtx.shape[0], 2), dtype=torch.uint32, device=tri2vtx.device)
+    from_edge2vtx_of_tri2vtx_with_vtx2vtx(edge2vtx, tri2vtx, vtx2jdx_offset, jdx2tri, edge2tri)
+    return edge2tri

@@ -119,6 +119,26 @@ _PROTOTYPES = {
         [c_void_p, c_u32, c_void_p, c_u32, c_void_p, c_void_p, c_void_p, c_u32, c_u32, c_void_p,
          c_void_p, c_void_p],
     ),
+    "dmsh_exclusive_scan_workspace_bytes": (c_size, [c_u32]),
+    "dmsh_exclusive_scan_u32": (c_int, [c_void_p, c_u32, c_void_p, c_void_p, c_void_p, c_size]),
+    "dmsh_vtx2elem_workspace_bytes": (c_size, [c_u32, c_u32, c_u32]),
+    "dmsh_vtx2elem_from_uniform_mesh": (
+        c_int, [c_void_p, c_u32, c_u32, c_void_p, c_u32, c_void_p, c_void_p, c_void_p, c_size]),
+    "dmsh_vtx2vtx_workspace_bytes": (c_size, [c_u32, c_u32, c_u32, c_int]),
+    "dmsh_vtx2vtx_count": (
+        c_int, [c_void_p, c_u32, c_u32, c_void_p, c_u32, c_int, c_void_p, c_void_p, c_size]),
+    "dmsh_vtx2vtx_fill": (
+        c_int, [c_void_p, c_u32, c_u32, c_u32, c_int, c_void_p, c_void_p, c_void_p, c_size]),
+    "dmsh_edge2vtx_from_vtx2vtx": (c_int, [c_void_p, c_u32, c_void_p, c_u32, c_void_p, c_void_p]),
+    "dmsh_edge2elem_from_edge2vtx_of_tri2vtx": (
+        c_int,
+        [c_void_p, c_u32, c_void_p, c_u32, c_void_p, c_u32, c_void_p, c_u32, c_void_p, c_void_p]),
+    "dmsh_laplacian_smoothing": (
+        c_int,
+        [c_void_p, c_u32, c_void_p, c_u32, c_void_p, c_f32, c_u32, c_void_p, c_void_p, c_u32,
+         c_void_p]),
+    "dmsh_multiply_graph_laplacian": (
+        c_int, [c_void_p, c_u32, c_void_p, c_u32, c_void_p, c_u32, c_void_p, c_void_p]),
 }
 
 

@@ -368,6 +368,59 @@ int dmsh_cast_requests(void *stream, uint32_t num_req,
                        uint32_t *pix2tri, float *pix2t, void *ws,
                        size_t ws_bytes);
 
+/* ---------------------------------------------------------------------
+ * Mesh topology: offset arrays (vtx2idx[num_vtx + 1] over a flat idx2*).
+ * Replaces the launches of del-msh-cudarc-sys/src/vtx2elem.rs:2-68
+ * (uniform_mesh.cu vtx2nelem / fill_idx2elem + offset_array::sort),
+ * del-msh-cudarc-sys/src/vtx2vtx.rs:2-84 (vtx2vtx.cu
+ * vtx2nvtx_from_uniform_mesh / idx2vtx_from_vtx2buff_for_uniform_mesh),
+ * del-msh-dlpack/src/edge2vtx.rs:117-174 (edge2vtx.cu edge2vtx_from_vtx2vtx),
+ * del-msh-dlpack/src/edge2elem.rs:17-83 (edge2elem.cu), and
+ * del-msh-dlpack/src/vtx2vtx.rs:23-190 (vtx2vtx.cu laplacian_smoothing /
+ * multiply_graph_laplacian).  Results equal del-msh-cpu's: elements
+ * ascending per vertex (vtx2elem.rs:58-96), neighbours in first-appearance
+ * order (vtx2vtx.rs:11-76), edge2tri as edge2elem.rs:20-70 (entries with no
+ * second triangle are left untouched), Laplacian sweeps in f32 in the CPU's
+ * operation order (vtx2vtx.rs:210-280).
+ *
+ * Vertex-to-vertex is two calls on one workspace: dmsh_vtx2vtx_count writes
+ * vtx2idx (the caller reads vtx2idx[num_vtx] to size idx2vtx, as the
+ * reference does) and keeps the lists in ws; dmsh_vtx2vtx_fill, with the
+ * same sizes, is_self and workspace, writes idx2vtx.
+ * Out-of-range indices set DMSH_DEVERR_INDEX.
+ * ------------------------------------------------------------------- */
+size_t dmsh_exclusive_scan_workspace_bytes(uint32_t n);
+/* out[0..n] = exclusive prefix sums of in[0..n), out[n] = total; in may alias out */
+int dmsh_exclusive_scan_u32(void *stream, uint32_t n, const uint32_t *in, uint32_t *out,
+                            void *ws, size_t ws_bytes);
+size_t dmsh_vtx2elem_workspace_bytes(uint32_t num_elem, uint32_t num_node, uint32_t num_vtx);
+int dmsh_vtx2elem_from_uniform_mesh(void *stream, uint32_t num_elem, uint32_t num_node,
+                                    const uint32_t *elem2vtx, uint32_t num_vtx,
+                                    uint32_t *vtx2idx, uint32_t *idx2elem, void *ws,
+                                    size_t ws_bytes);
+size_t dmsh_vtx2vtx_workspace_bytes(uint32_t num_elem, uint32_t num_node, uint32_t num_vtx,
+                                    int is_self);
+int dmsh_vtx2vtx_count(void *stream, uint32_t num_elem, uint32_t num_node,
+                       const uint32_t *elem2vtx, uint32_t num_vtx, int is_self,
+                       uint32_t *vtx2idx, void *ws, size_t ws_bytes);
+int dmsh_vtx2vtx_fill(void *stream, uint32_t num_elem, uint32_t num_node, uint32_t num_vtx,
+                      int is_self, const uint32_t *vtx2idx, uint32_t *idx2vtx, void *ws,
+                      size_t ws_bytes);
+int dmsh_edge2vtx_from_vtx2vtx(void *stream, uint32_t num_vtx, const uint32_t *vtx2idx,
+                               uint32_t num_idx, const uint32_t *idx2vtx, uint32_t *edge2vtx);
+int dmsh_edge2elem_from_edge2vtx_of_tri2vtx(void *stream, uint32_t num_edge,
+                                            const uint32_t *edge2vtx, uint32_t num_tri,
+                                            const uint32_t *tri2vtx, uint32_t num_vtx,
+                                            const uint32_t *vtx2idx, uint32_t num_idx,
+                                            const uint32_t *idx2tri, uint32_t *edge2tri);
+int dmsh_laplacian_smoothing(void *stream, uint32_t num_vtx, const uint32_t *vtx2idx,
+                             uint32_t num_idx, const uint32_t *idx2vtx, float lambda,
+                             uint32_t num_dim, float *vtx2lhs, const float *vtx2rhs,
+                             uint32_t num_iter, float *vtx2lhstmp);
+int dmsh_multiply_graph_laplacian(void *stream, uint32_t num_vtx, const uint32_t *vtx2idx,
+                                  uint32_t num_idx, const uint32_t *idx2vtx, uint32_t num_dim,
+                                  const float *vtx2rhs, float *vtx2lhs);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1227,3 +1227,129 @@ long orc_bwd_antialias(size_t num_cedge, const uint32_t *cedge2vtx, const float 
   }
   return abandoned;
 }
+
+/* ========================================================================
+ * Mesh topology (vertex-to-element, vertex-to-vertex, edges)
+ * ===================================================================== */
+
+/* vtx2elem.rs:58-96 */
+int orc_vtx2elem(size_t num_elem, size_t num_node, const uint32_t *elem2vtx, size_t num_vtx,
+                 uint32_t *vtx2idx, uint32_t *idx2elem) {
+  for (size_t v = 0; v <= num_vtx; ++v) vtx2idx[v] = 0;
+  for (size_t e = 0; e < num_elem; ++e)
+    for (size_t k = 0; k < num_node; ++k) {
+      const uint32_t v = elem2vtx[e * num_node + k];
+      if (v >= num_vtx) return -1;
+      vtx2idx[v + 1] += 1;
+    }
+  for (size_t v = 0; v < num_vtx; ++v) vtx2idx[v + 1] += vtx2idx[v];
+  for (size_t e = 0; e < num_elem; ++e)
+    for (size_t k = 0; k < num_node; ++k) {
+      const uint32_t v = elem2vtx[e * num_node + k];
+      idx2elem[vtx2idx[v]] = (uint32_t)e;
+      vtx2idx[v] += 1;
+    }
+  for (size_t v = num_vtx; v >= 1; --v) vtx2idx[v] = vtx2idx[v - 1];
+  vtx2idx[0] = 0;
+  return 0;
+}
+
+/* vtx2vtx.rs:11-76 (a per-vertex "last visited by" flag removes repeats) */
+size_t orc_vtx2vtx(size_t num_elem, size_t num_node, const uint32_t *elem2vtx, size_t num_vtx,
+                   const uint32_t *vtx2jdx, const uint32_t *jdx2elem, int is_self,
+                   uint32_t *vtx2idx, uint32_t *idx2vtx) {
+  (void)num_elem;
+  size_t *flag = (size_t *)malloc(sizeof(size_t) * (num_vtx ? num_vtx : 1));
+  for (size_t v = 0; v < num_vtx; ++v) flag[v] = (size_t)-1;
+  vtx2idx[0] = 0;
+  size_t cnt = 0;
+  for (size_t i = 0; i < num_vtx; ++i) {
+    if (!is_self) flag[i] = i;
+    for (uint32_t jdx = vtx2jdx[i]; jdx < vtx2jdx[i + 1]; ++jdx) {
+      const size_t e = jdx2elem[jdx];
+      for (size_t k = 0; k < num_node; ++k) {
+        const uint32_t j = elem2vtx[e * num_node + k];
+        if (flag[j] == i) continue;
+        flag[j] = i;
+        if (idx2vtx) idx2vtx[cnt] = j;
+        ++cnt;
+      }
+    }
+    vtx2idx[i + 1] = (uint32_t)cnt;
+  }
+  free(flag);
+  return cnt;
+}
+
+/* edge2vtx.rs:16-36 */
+void orc_edge2vtx_from_vtx2vtx(size_t num_vtx, const uint32_t *vtx2idx, const uint32_t *idx2vtx,
+                               uint32_t *edge2vtx) {
+  for (size_t i = 0; i < num_vtx; ++i)
+    for (uint32_t idx = vtx2idx[i]; idx < vtx2idx[i + 1]; ++idx) {
+      edge2vtx[(size_t)idx * 2] = (uint32_t)i;
+      edge2vtx[(size_t)idx * 2 + 1] = idx2vtx[idx];
+    }
+}
+
+/* edge2elem.rs:20-70 */
+int orc_edge2elem(size_t num_edge, const uint32_t *edge2vtx, const uint32_t *tri2vtx,
+                  const uint32_t *vtx2idx, const uint32_t *idx2tri, uint32_t *edge2tri) {
+  for (size_t e = 0; e < num_edge; ++e) {
+    const uint32_t i0 = edge2vtx[e * 2], i1 = edge2vtx[e * 2 + 1];
+    int cnt = 0;
+    for (uint32_t idx = vtx2idx[i0]; idx < vtx2idx[i0 + 1]; ++idx) {
+      const uint32_t t = idx2tri[idx];
+      const uint32_t j0 = tri2vtx[(size_t)t * 3], j1 = tri2vtx[(size_t)t * 3 + 1],
+                     j2 = tri2vtx[(size_t)t * 3 + 2];
+      int adjacent;
+      if (i0 == j0 && i0 != j1 && i0 != j2) adjacent = (j1 == i1) || (j2 == i1);
+      else if (i0 != j0 && i0 == j1 && i0 != j2) adjacent = (j2 == i1) || (j0 == i1);
+      else if (i0 != j0 && i0 != j1 && i0 == j2) adjacent = (j0 == i1) || (j1 == i1);
+      else return -1;
+      if (!adjacent) continue;
+      edge2tri[e * 2 + (size_t)cnt] = t;
+      if (++cnt == 2) break;
+    }
+  }
+  return 0;
+}
+
+/* vtx2vtx.rs:226-240: one Jacobi update of vertex i from prev */
+static void lap_update(size_t i, const uint32_t *vtx2idx, const uint32_t *idx2vtx, float lambda,
+                       size_t num_dim, const float *vtx2rhs, const float *prev, float *next) {
+  float buff[64];
+  for (size_t d = 0; d < num_dim; ++d) buff[d] = vtx2rhs[i * num_dim + d];
+  for (uint32_t idx = vtx2idx[i]; idx < vtx2idx[i + 1]; ++idx) {
+    const size_t j = idx2vtx[idx];
+    for (size_t d = 0; d < num_dim; ++d) buff[d] += lambda * prev[j * num_dim + d];
+  }
+  const float valence = (float)(vtx2idx[i + 1] - vtx2idx[i]);
+  const float inv_dia = 1.f / (1.f + lambda * valence);
+  for (size_t d = 0; d < num_dim; ++d) next[i * num_dim + d] = buff[d] * inv_dia;
+}
+
+/* vtx2vtx.rs:210-252 (num_dim <= 64) */
+void orc_laplacian_smoothing(size_t num_vtx, const uint32_t *vtx2idx, const uint32_t *idx2vtx,
+                             float lambda, size_t num_dim, float *vtx2lhs, const float *vtx2rhs,
+                             size_t num_iter, float *vtx2lhstmp) {
+  for (size_t it = 0; it < num_iter; ++it) {
+    for (size_t i = 0; i < num_vtx; ++i)
+      lap_update(i, vtx2idx, idx2vtx, lambda, num_dim, vtx2rhs, vtx2lhs, vtx2lhstmp);
+    for (size_t i = 0; i < num_vtx; ++i)
+      lap_update(i, vtx2idx, idx2vtx, lambda, num_dim, vtx2rhs, vtx2lhstmp, vtx2lhs);
+  }
+}
+
+/* vtx2vtx.rs:254-280 */
+void orc_multiply_graph_laplacian(size_t num_vtx, const uint32_t *vtx2idx,
+                                  const uint32_t *idx2vtx, size_t num_dim, const float *vtx2rhs,
+                                  float *vtx2lhs) {
+  for (size_t i = 0; i < num_vtx; ++i) {
+    const float valence = (float)(vtx2idx[i + 1] - vtx2idx[i]);
+    for (size_t d = 0; d < num_dim; ++d) vtx2lhs[i * num_dim + d] = valence * vtx2rhs[i * num_dim + d];
+    for (uint32_t idx = vtx2idx[i]; idx < vtx2idx[i + 1]; ++idx) {
+      const size_t j = idx2vtx[idx];
+      for (size_t d = 0; d < num_dim; ++d) vtx2lhs[i * num_dim + d] -= vtx2rhs[j * num_dim + d];
+    }
+  }
+}

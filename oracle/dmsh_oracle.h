@@ -170,6 +170,38 @@ void orc_edge_gradient_bwd(const uint32_t *tri2vtx, const float *vtx2xyz,
                            const uint32_t *pix2tri, uint32_t num_vdim,
                            const float *pix2val, const float *dldw_pix2val);
 
+/* ---- mesh topology ------------------------------------------------------- */
+/* vtx2elem.rs:58-96: elements around each vertex (offsets vtx2idx[num_vtx+1],
+ * idx2elem[num_elem*num_node], elements ascending per vertex).  Returns 0, or
+ * -1 on a vertex index >= num_vtx (the reference asserts). */
+int orc_vtx2elem(size_t num_elem, size_t num_node, const uint32_t *elem2vtx, size_t num_vtx,
+                 uint32_t *vtx2idx, uint32_t *idx2elem);
+/* vtx2vtx.rs:11-76 from_uniform_mesh_with_vtx2elem: vertices around each
+ * vertex in first-appearance order over the vertex's elements (ascending) and
+ * their nodes; is_self keeps the vertex itself.  Pass idx2vtx = NULL to count
+ * only.  Returns the number of entries (vtx2idx[num_vtx]). */
+size_t orc_vtx2vtx(size_t num_elem, size_t num_node, const uint32_t *elem2vtx, size_t num_vtx,
+                   const uint32_t *vtx2jdx, const uint32_t *jdx2elem, int is_self,
+                   uint32_t *vtx2idx, uint32_t *idx2vtx);
+/* edge2vtx.rs:16-36 from_vtx2vtx: edge (i, j) for every entry j of vertex i */
+void orc_edge2vtx_from_vtx2vtx(size_t num_vtx, const uint32_t *vtx2idx, const uint32_t *idx2vtx,
+                               uint32_t *edge2vtx);
+/* edge2elem.rs:20-70: the first two triangles around edge2vtx[e][0] (in
+ * vtx2idx/idx2tri order) that also hold edge2vtx[e][1]; entries not found
+ * are left as they are.  Returns 0, or -1 where a listed triangle does not
+ * hold the edge's first vertex exactly once (the reference panics). */
+int orc_edge2elem(size_t num_edge, const uint32_t *edge2vtx, const uint32_t *tri2vtx,
+                  const uint32_t *vtx2idx, const uint32_t *idx2tri, uint32_t *edge2tri);
+/* vtx2vtx.rs:210-252: num_iter Jacobi sweeps of [I + lambda L] x = rhs,
+ * each sweep vtx2lhs -> vtx2lhstmp -> vtx2lhs */
+void orc_laplacian_smoothing(size_t num_vtx, const uint32_t *vtx2idx, const uint32_t *idx2vtx,
+                             float lambda, size_t num_dim, float *vtx2lhs, const float *vtx2rhs,
+                             size_t num_iter, float *vtx2lhstmp);
+/* vtx2vtx.rs:254-280: lhs = L rhs, L = [.., -1, .., valence, .., -1, ..] */
+void orc_multiply_graph_laplacian(size_t num_vtx, const uint32_t *vtx2idx,
+                                  const uint32_t *idx2vtx, size_t num_dim, const float *vtx2rhs,
+                                  float *vtx2lhs);
+
 #ifdef __cplusplus
 }
 #endif

@@ -365,3 +365,91 @@ def bwd_antialias(cedge2vtx, vtx2xyz, world2pix_colmajor, pix2val, dldw_pix2val,
     if n:
         raise RuntimeError(f"{n} antialias pixel walks exceeded 2^26 steps")
     return g.reshape(-1, 3)
+
+
+# ---------------------------------------------------------------------------
+# mesh topology
+# ---------------------------------------------------------------------------
+def _sig_topo():
+    L = lib()
+    if getattr(L, "_topo_ready", False):
+        return L
+    _u32p_or_null = ctypes.c_void_p
+    L.orc_vtx2elem.restype = ctypes.c_int
+    L.orc_vtx2elem.argtypes = [_sz, _sz, _u32p, _sz, _u32p, _u32p]
+    L.orc_vtx2vtx.restype = _sz
+    L.orc_vtx2vtx.argtypes = [_sz, _sz, _u32p, _sz, _u32p, _u32p, ctypes.c_int, _u32p,
+                              _u32p_or_null]
+    L.orc_edge2vtx_from_vtx2vtx.restype = None
+    L.orc_edge2vtx_from_vtx2vtx.argtypes = [_sz, _u32p, _u32p, _u32p]
+    L.orc_edge2elem.restype = ctypes.c_int
+    L.orc_edge2elem.argtypes = [_sz, _u32p, _u32p, _u32p, _u32p, _u32p]
+    L.orc_laplacian_smoothing.restype = None
+    L.orc_laplacian_smoothing.argtypes = [_sz, _u32p, _u32p, ctypes.c_float, _sz, _f32p, _f32p, _sz,
+                                          _f32p]
+    L.orc_multiply_graph_laplacian.restype = None
+    L.orc_multiply_graph_laplacian.argtypes = [_sz, _u32p, _u32p, _sz, _f32p, _f32p]
+    L._topo_ready = True
+    return L
+
+
+def vtx2elem(elem2vtx, num_vtx):
+    """(vtx2idx [num_vtx+1], idx2elem [num_elem*num_node]) of vtx2elem.rs:58-96."""
+    L = _sig_topo()
+    e = _u32(elem2vtx)
+    vtx2idx = np.zeros(num_vtx + 1, np.uint32)
+    idx2elem = np.zeros(e.size, np.uint32)
+    if L.orc_vtx2elem(e.shape[0], e.shape[1], e.reshape(-1), num_vtx, vtx2idx, idx2elem) != 0:
+        raise ValueError("vertex index out of range")
+    return vtx2idx, idx2elem
+
+
+def vtx2vtx(elem2vtx, num_vtx, is_self):
+    """(vtx2idx, idx2vtx) of vtx2vtx.rs:78-102 from_uniform_mesh."""
+    L = _sig_topo()
+    e = _u32(elem2vtx)
+    vtx2jdx, jdx2elem = vtx2elem(e, num_vtx)
+    vtx2idx = np.zeros(num_vtx + 1, np.uint32)
+    n = L.orc_vtx2vtx(e.shape[0], e.shape[1], e.reshape(-1), num_vtx, vtx2jdx, jdx2elem,
+                      int(bool(is_self)), vtx2idx, None)
+    idx2vtx = np.zeros(max(n, 1), np.uint32)
+    L.orc_vtx2vtx(e.shape[0], e.shape[1], e.reshape(-1), num_vtx, vtx2jdx, jdx2elem,
+                  int(bool(is_self)), vtx2idx, idx2vtx.ctypes.data)
+    return vtx2idx, idx2vtx[:n]
+
+
+def edge2vtx_from_vtx2vtx(vtx2idx, idx2vtx):
+    L = _sig_topo()
+    out = np.zeros((idx2vtx.shape[0], 2), np.uint32)
+    L.orc_edge2vtx_from_vtx2vtx(vtx2idx.shape[0] - 1, _u32(vtx2idx), _u32(idx2vtx), out.reshape(-1))
+    return out
+
+
+def edge2elem(edge2vtx, tri2vtx, vtx2idx, idx2tri, fill=0xFFFFFFFF):
+    """edge2elem.rs:20-70; entries not found keep `fill`."""
+    L = _sig_topo()
+    e2v = _u32(edge2vtx)
+    out = np.full((e2v.shape[0], 2), fill, np.uint32)
+    rc = L.orc_edge2elem(e2v.shape[0], e2v.reshape(-1), _u32(tri2vtx).reshape(-1), _u32(vtx2idx),
+                         _u32(idx2tri), out.reshape(-1))
+    if rc != 0:
+        raise ValueError("a triangle holds an edge's first vertex more than once")
+    return out
+
+
+def laplacian_smoothing(vtx2idx, idx2vtx, lam, vtx2lhs, vtx2rhs, num_iter):
+    L = _sig_topo()
+    lhs = _f32(vtx2lhs).copy()
+    tmp = np.zeros_like(lhs)
+    L.orc_laplacian_smoothing(vtx2idx.shape[0] - 1, _u32(vtx2idx), _u32(idx2vtx), lam, lhs.shape[1],
+                              lhs.reshape(-1), _f32(vtx2rhs).reshape(-1), num_iter, tmp.reshape(-1))
+    return lhs
+
+
+def multiply_graph_laplacian(vtx2idx, idx2vtx, vtx2rhs):
+    L = _sig_topo()
+    rhs = _f32(vtx2rhs)
+    out = np.zeros_like(rhs)
+    L.orc_multiply_graph_laplacian(vtx2idx.shape[0] - 1, _u32(vtx2idx), _u32(idx2vtx), rhs.shape[1],
+                                   rhs.reshape(-1), out.reshape(-1))
+    return out
