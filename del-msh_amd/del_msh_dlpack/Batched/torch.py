@@ -111,6 +111,14 @@ class Pipeline:
         if self.has_cast:
             self.cast()
 
+    def set_vertices(self, vtx2xyz: torch.Tensor):
+        """New vertex positions (same shape) for the next build; a graph
+        captured earlier keeps reading the old tensor."""
+        if vtx2xyz.shape != self.vtx2xyz.shape or vtx2xyz.dtype != torch.float32:
+            raise ValueError("vtx2xyz must keep its shape and dtype")
+        _lib.require_gpu(vtx2xyz)
+        self.vtx2xyz = vtx2xyz.detach().contiguous()
+
     def capture(self):
         """Record run() into a HIP graph (warm it up eagerly first)."""
         self.run()
@@ -206,3 +214,147 @@ def autotune(p: Pipeline, reps: int = 2, stacks=(16, 24, 32), wgs_per_cu=(4, 8, 
         out["cast"] = {"batch.stack": best_cast[1], "batch.wgs_per_cu": best_cast[2],
                        "ms": round(best_cast[0], 4)}
     return out
+
+
+# ---------------------------------------------------------------------------
+# differentiable anti-aliased silhouettes of a batch
+# ---------------------------------------------------------------------------
+class SilhouetteBatch:
+    """Anti-aliased silhouette images of a CSR mesh batch, one image per mesh
+    (occupancy of the ray-cast, blended across the contour edges as
+    DifferentiableAntialias does), and dL/d(vertex positions) of them.
+
+    The "hit buffers" -- the batch's BVHs, pix2tri, the occupancy and the
+    contour edges -- come from hits(): one batched build and one batched
+    ray-cast for all meshes, then one contour extraction per mesh.  The mesh
+    edges are built once on the GPU (TriMesh3.make_edge2vtx / make_edge2tri)
+    and kept one direction per edge, so each silhouette edge is walked once
+    and the gradient is that of the forward image.
+
+    No reference counterpart (the reference renders and differentiates one
+    mesh per call)."""
+
+    def __init__(self, mesh2tri, mesh2vtx, tri2vtx: torch.Tensor, vtx2xyz: torch.Tensor,
+                 mesh2pix, mesh2width, ndc2world: torch.Tensor):
+        from ..Mat44.torch import from_transform_ndc2pix
+        from ..TriMesh3.torch import make_edge2tri, make_edge2vtx
+
+        self.pipe = Pipeline(mesh2tri, mesh2vtx, tri2vtx, vtx2xyz, mesh2pix, mesh2width, ndc2world,
+                             with_coeff=False)
+        p = self.pipe
+        dev = p.device
+        self.num_vtx = p.total_vtx
+        n2w = ndc2world.detach().cpu().to(torch.float64)
+        self.world2ndc, self.world2pix, self.edges, self.shapes = [], [], [], []
+        for m in range(p.num_mesh):
+            a, b = int(p.mesh2pix_h[m]), int(p.mesh2pix_h[m + 1])
+            w = int(p.mesh2width_h[m])
+            h = (b - a) // w if w else 0
+            self.shapes.append((h, w))
+            w2n = torch.linalg.inv(n2w[m])
+            self.world2ndc.append(w2n.to(torch.float32).contiguous().to(dev))
+            self.world2pix.append((from_transform_ndc2pix((w, h), dtype=torch.float64) @ w2n)
+                                  .to(torch.float32).contiguous().to(dev))
+            t0, t1 = int(p.mesh2tri_h[m]), int(p.mesh2tri_h[m + 1])
+            v0, v1 = int(p.mesh2vtx_h[m]), int(p.mesh2vtx_h[m + 1])
+            tri = p.tri2vtx[t0:t1]
+            if t1 > t0:
+                e2v = make_edge2vtx(tri, v1 - v0)
+                e2t = make_edge2tri(tri, v1 - v0, e2v)
+                keep = e2v[:, 0].to(torch.int64) < e2v[:, 1].to(torch.int64)
+                rows = lambda a: a.view(torch.int32)[keep].view(torch.uint32).contiguous()
+                self.edges.append((rows(e2v), rows(e2t)))
+            else:
+                z = torch.zeros((0, 2), dtype=torch.uint32, device=dev)
+                self.edges.append((z, z))
+
+    def _slices(self, m: int):
+        p = self.pipe
+        return ((int(p.mesh2tri_h[m]), int(p.mesh2tri_h[m + 1])),
+                (int(p.mesh2vtx_h[m]), int(p.mesh2vtx_h[m + 1])),
+                (int(p.mesh2pix_h[m]), int(p.mesh2pix_h[m + 1])))
+
+    def hits(self, vtx2xyz: torch.Tensor) -> dict:
+        """BVHs + pix2tri (batched), occupancy, contour edges per mesh."""
+        from ..Edge2Vtx.torch import contour_for_triangle_mesh
+
+        p = self.pipe
+        p.set_vertices(vtx2xyz)
+        p.run()
+        pix2tri = p.pix2tri[: p.total_pix].clone()
+        occ = (pix2tri != 0xFFFFFFFF).to(torch.float32)
+        v = p.vtx2xyz
+        cedges = []
+        for m in range(p.num_mesh):
+            (t0, t1), (v0, v1), _ = self._slices(m)
+            e2v, e2t = self.edges[m]
+            if t1 == t0 or e2v.shape[0] == 0:
+                cedges.append(e2v)
+                continue
+            cedges.append(contour_for_triangle_mesh(p.tri2vtx[t0:t1], v[v0:v1], self.world2ndc[m],
+                                                    e2v, e2t))
+        return {"pix2tri": pix2tri, "occ": occ, "cedges": cedges}
+
+    @staticmethod
+    def hit_bytes(hits: dict) -> int:
+        return (hits["pix2tri"].numel() * 4 + hits["occ"].numel() * 4 +
+                sum(c.numel() * 4 for c in hits["cedges"]))
+
+    def images(self, vtx2xyz: torch.Tensor, hits: dict) -> torch.Tensor:
+        from ..DifferentiableAntialias.torch import antialias
+
+        p = self.pipe
+        out = hits["occ"].clone()
+        v = vtx2xyz.detach()
+        for m in range(p.num_mesh):
+            _, (v0, v1), (a, b) = self._slices(m)
+            c = hits["cedges"][m]
+            if c.shape[0] == 0 or b == a:
+                continue
+            h, w = self.shapes[m]
+            out[a:b] = antialias(c, v[v0:v1], self.world2pix[m], hits["pix2tri"][a:b].view(h, w),
+                                 hits["occ"][a:b].view(h, w)).view(-1)
+        return out
+
+    def vertex_gradient(self, vtx2xyz: torch.Tensor, hits: dict,
+                        dldw: torch.Tensor) -> torch.Tensor:
+        from ..DifferentiableAntialias.torch import bwd_antialias
+
+        p = self.pipe
+        v = vtx2xyz.detach()
+        grad = torch.zeros_like(v)
+        for m in range(p.num_mesh):
+            _, (v0, v1), (a, b) = self._slices(m)
+            c = hits["cedges"][m]
+            if c.shape[0] == 0 or b == a:
+                continue
+            h, w = self.shapes[m]
+            grad[v0:v1] = bwd_antialias(c, v[v0:v1], self.world2pix[m],
+                                        hits["occ"][a:b].view(h, w), dldw[a:b].view(h, w),
+                                        hits["pix2tri"][a:b].view(h, w))
+        return grad
+
+
+class AntialiasedSilhouette(torch.autograd.Function):
+    """images = AntialiasedSilhouette.apply(vtx2xyz, batch, recompute):
+    the concatenated silhouette images of a SilhouetteBatch.  With
+    recompute=True nothing but the vertices is kept between forward and
+    backward: the backward pass rebuilds the BVHs, re-casts pix2tri and
+    re-extracts the contours (bit-identical, the build and the cast are
+    deterministic) before the antialias backward."""
+
+    @staticmethod
+    def forward(ctx, vtx2xyz, batch: SilhouetteBatch, recompute: bool = True):
+        hits = batch.hits(vtx2xyz)
+        out = batch.images(vtx2xyz, hits)
+        ctx.batch = batch
+        ctx.hits = None if recompute else hits
+        ctx.save_for_backward(vtx2xyz)
+        return out
+
+    @staticmethod
+    def backward(ctx, dldw):
+        (vtx2xyz,) = ctx.saved_tensors
+        hits = ctx.hits if ctx.hits is not None else ctx.batch.hits(vtx2xyz)
+        ctx.hits = None
+        return ctx.batch.vertex_gradient(vtx2xyz, hits, dldw.contiguous()), None, None
