@@ -58,6 +58,8 @@ def parse():
     ap.add_argument("--serve-rates", default="1000,4000,20000",
                     help="arrival rates, requests/s")
     ap.add_argument("--serve-budget-mib", type=int, default=4096)
+    ap.add_argument("--diff-reps", type=int, default=5,
+                    help="repetitions of the differentiation workload (0 skips it)")
     return ap.parse_args()
 
 
@@ -264,6 +266,137 @@ def run_batched(num_mesh: int, reps: int):
 # ---------------------------------------------------------------------------
 # served workload: continuous batching vs one request at a time
 # ---------------------------------------------------------------------------
+def run_diff(reps: int):
+    """Differentiation workload.  (1) A 50,000-triangle sphere (sphere_yup(1,
+    126, 200)) seen from (0, 0, 3) at 512^2, dL/dpixel ~ N(0, 1) (seed 4):
+    forward = LBVH build + ray-cast + contour + anti-aliased silhouette,
+    backward = antialias backward; the edge-gradient backward of the same
+    image timed beside it.  (2) A batch of 64 meshes (5k..200k triangles
+    log-uniform, seed 6, 512^2 each) through Batched.AntialiasedSilhouette,
+    with the hit buffers kept from forward or recomputed in backward."""
+    import numpy as np
+    import torch
+
+    from del_msh_dlpack import _lib, primitives
+    import del_msh_dlpack.Batched.torch as Batched
+    import del_msh_dlpack.DifferentiableAntialias.torch as DA
+    import del_msh_dlpack.Edge2Vtx.torch as Edge2Vtx
+    import del_msh_dlpack.Mat44.torch as Mat44
+    import del_msh_dlpack.Pix2Tri.torch as Pix2Tri
+    import del_msh_dlpack.RasterizedEdgeGradient.torch as REG
+    import del_msh_dlpack.TriMesh3.torch as TriMesh3
+
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream(dev)
+    W = 512
+    tri, vtx = primitives.sphere_yup(1.0, 126, 200)
+    w2c = Mat44.look_at((0.0, 0.0, 3.0), (0, 0, 0), (0, 1, 0), dtype=torch.float64)
+    w2n = Mat44.perspective(45.0, 1.0, 1.0, 5.0, dtype=torch.float64) @ w2c
+    w2p = (Mat44.from_transform_ndc2pix((W, W), dtype=torch.float64) @ w2n).float().to(dev)
+    n2w = torch.linalg.inv(w2n).float().contiguous().to(dev)
+    w2n = w2n.float().contiguous().to(dev)
+    t = torch.from_numpy(tri).to(dev)
+    v0 = torch.from_numpy(vtx).to(dev)
+    e2v = TriMesh3.make_edge2vtx(t, vtx.shape[0])
+    e2t = TriMesh3.make_edge2tri(t, vtx.shape[0], e2v)
+    one = e2v[:, 0].to(torch.int64) < e2v[:, 1].to(torch.int64)
+    e2v = e2v.view(torch.int32)[one].view(torch.uint32).contiguous()
+    e2t = e2t.view(torch.int32)[one].view(torch.uint32).contiguous()
+    dldw = torch.from_numpy(np.random.default_rng(4).standard_normal((W, W)).astype(np.float32)).to(dev)
+
+    def forward(v):
+        bn, ab = TriMesh3.make_bvhnodes_bvhnode2aabb(t, v)
+        pix = Pix2Tri.by_raycasting(t, v, bn, ab, n2w, (W, W))
+        cedge = Edge2Vtx.contour_for_triangle_mesh(t, v, w2n, e2v, e2t)
+        occ = (pix != 0xFFFFFFFF).to(torch.float32)
+        return DA.DifferentiableAntialiasFunction.apply(cedge, v, w2p, pix, occ), pix, occ, cedge
+
+    def timed(fn, n):
+        ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+              for _ in range(n)]
+        for a, b in ev:
+            a.record(stream)
+            fn()
+            b.record(stream)
+        torch.cuda.synchronize()
+        return float(np.median([a.elapsed_time(b) for a, b in ev]))
+
+    state = {}
+
+    def fwd():
+        v = v0.clone().requires_grad_(True)
+        out = forward(v)
+        state["v"], state["out"] = v, out
+
+    def bwd():
+        (state["out"][0] * dldw).sum().backward()
+
+    for _ in range(2):
+        fwd()
+        bwd()
+    f_ms, b_ms = [], []
+    for _ in range(reps):
+        f_ms.append(timed(fwd, 1))
+        b_ms.append(timed(bwd, 1))
+    _, pix, occ, cedge = state["out"]
+    occ3 = occ[..., None].contiguous()
+    dl3 = dldw[..., None].contiguous()
+    eg_ms = timed(lambda: REG.bwd(t, v0, w2p, pix, occ3, dl3), max(reps, 5))
+    _lib.timing_enable(True)
+    for _ in range(3):
+        REG.bwd(t, v0, w2p, pix, occ3, dl3)
+        DA.bwd_antialias(cedge, v0, w2p, occ, dldw, pix)
+    torch.cuda.synchronize()
+    _lib.timing_enable(False)
+    kern = {}
+    for name in ("k_edge_grad_bwd", "k_aa_bwd"):
+        n, total, _ = _lib.timing_query(name)
+        if n:
+            kern[name] = round(total / n, 5)
+    ph = pix.cpu().numpy()
+    active = int((ph[:, 1:] != ph[:, :-1]).sum() + (ph[1:, :] != ph[:-1, :]).sum())
+    # edge-gradient basis: 12 B per pixel, 12 B per vertex, 84 B per active pair
+    eg_bytes = W * W * 12 + vtx.shape[0] * 12 + active * 84
+    single = {
+        "workload": "sphere_yup(1,126,200): 50000 tris, perspective eye (0,0,3) fovy 45, 512^2, "
+                    "dL/dpixel N(0,1) seed 4",
+        "forward_ms": round(float(np.median(f_ms)), 4),
+        "backward_ms": round(float(np.median(b_ms)), 4),
+        "contour_edges": int(cedge.shape[0]),
+        "edge_grad_bwd_ms": round(eg_ms, 4),
+        "kernels_ms": kern,
+        "edge_grad_active_pairs": active,
+    }
+    if "k_edge_grad_bwd" in kern:
+        single["edge_grad_bwd_algorithmic_gbs"] = round(eg_bytes / (kern["k_edge_grad_bwd"] * 1e-3) / 1e9, 1)
+
+    b = primitives.batch_meshes(64, seed=6, tri_range=(5000, 200000), pix_range=(W * W, W * W))
+    sb = Batched.SilhouetteBatch(b["mesh2tri"], b["mesh2vtx"], torch.from_numpy(b["tri2vtx"]).to(dev),
+                                 torch.from_numpy(b["vtx2xyz"]).to(dev), b["mesh2pix"], b["mesh2width"],
+                                 torch.from_numpy(b["ndc2world"]))
+    bdl = torch.from_numpy(np.random.default_rng(4).standard_normal(int(b["mesh2pix"][-1]))
+                           .astype(np.float32)).to(dev)
+    bv0 = torch.from_numpy(b["vtx2xyz"]).to(dev)
+    batch = {"workload": "64 meshes (sphere/torus/cylinder, 5k..200k tris log-uniform, seed 6), 512^2 each",
+             "total_tri": int(b["mesh2tri"][-1])}
+    for recompute in (False, True):
+        def step():
+            v = bv0.clone().requires_grad_(True)
+            img = Batched.AntialiasedSilhouette.apply(v, sb, recompute)
+            (img * bdl).sum().backward()
+
+        step()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            step()
+        torch.cuda.synchronize()
+        batch["recompute" if recompute else "stored"] = {
+            "fwd_bwd_ms": round((time.perf_counter() - t0) * 1e3 / reps, 3)}
+    batch["hit_buffer_bytes_not_kept"] = Batched.SilhouetteBatch.hit_bytes(sb.hits(bv0))
+    return {"single": single, "batch64": batch}
+
+
 def run_served(num_req: int, rates, budget_bytes: int):
     """Poisson request streams (mesh id Zipf(1.1) over 10k ids, 1k..1M
     triangles with radial noise, eye on the sphere of radius 3, viewport
@@ -479,6 +612,7 @@ def main():
     batched = run_batched(args.batch_meshes, args.batch_reps) if args.batch_meshes > 0 else None
     served = (run_served(args.serve_requests, [float(x) for x in args.serve_rates.split(",")],
                          args.serve_budget_mib << 20) if args.serve_requests > 0 else None)
+    diff = run_diff(args.diff_reps) if args.diff_reps > 0 else None
 
     # ---------------- roofline of the dominant kernel ----------------
     # build: compulsory bytes per triangle with u64 codes
@@ -556,6 +690,7 @@ def main():
         "kernels": kernels,
         "batched": batched,
         "served": served,
+        "differentiation": diff,
         "roofline": roof,
         "cpu_baseline": cpu,
     }
