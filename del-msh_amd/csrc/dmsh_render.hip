@@ -200,6 +200,46 @@ int dmsh_pix2depth_bwd_wrt_vtx2xyz(void* stream, float* dldw_vtx2xyz, const uint
   return launch_status();
 }
 
+int dmsh_interpolate(void* stream, const uint32_t* pix2tri, uint32_t img_w, uint32_t img_h,
+                     uint32_t num_tri, const uint32_t* tri2vtx, uint32_t num_vtx,
+                     const float* vtx2xyz, uint32_t num_vdim, const float* vtx2val,
+                     const float* transform_ndc2world, float* pix2val) {
+  const uint64_t num_pix = (uint64_t)img_w * img_h;
+  if (num_pix == 0) return DMSH_OK;
+  if (num_pix >= (1ull << 32) || num_vdim == 0 || !pix2tri || !transform_ndc2world || !pix2val)
+    return DMSH_ERR_INVALID;
+  if (num_tri > 0 && (!tri2vtx || !vtx2xyz)) return DMSH_ERR_INVALID;
+  if (num_vtx > 0 && !vtx2val) return DMSH_ERR_INVALID;
+  uint32_t* err = device_error_word();
+  if (!err) return DMSH_ERR_HIP;
+  rc::TriMesh sc{tri2vtx, vtx2xyz, num_tri, num_vtx};
+  launch("k_interpolate", rc::k_interpolate, dim3(ceil_div(num_pix, 256)), dim3(256), 0,
+         as_stream(stream), pix2val, pix2tri, sc, num_vdim, vtx2val, img_w, img_h,
+         transform_ndc2world, err);
+  return launch_status();
+}
+
+int dmsh_interpolate_bwd(void* stream, const uint32_t* pix2tri, uint32_t img_w, uint32_t img_h,
+                         uint32_t num_tri, const uint32_t* tri2vtx, uint32_t num_vtx,
+                         const float* vtx2xyz, uint32_t num_vdim, const float* vtx2val,
+                         const float* transform_ndc2world, const float* dldw_pix2val,
+                         float* dldw_vtx2xyz, float* dldw_vtx2val) {
+  const uint64_t num_pix = (uint64_t)img_w * img_h;
+  if (num_pix == 0) return DMSH_OK;
+  if (num_pix >= (1ull << 32) || num_vdim == 0 || !pix2tri || !transform_ndc2world ||
+      !dldw_pix2val)
+    return DMSH_ERR_INVALID;
+  if (num_tri > 0 && (!tri2vtx || !vtx2xyz)) return DMSH_ERR_INVALID;
+  if (num_vtx > 0 && (!vtx2val || !dldw_vtx2xyz || !dldw_vtx2val)) return DMSH_ERR_INVALID;
+  uint32_t* err = device_error_word();
+  if (!err) return DMSH_ERR_HIP;
+  rc::TriMesh sc{tri2vtx, vtx2xyz, num_tri, num_vtx};
+  launch("k_interpolate_bwd", rc::k_interpolate_bwd, dim3(ceil_div(num_pix, 256)), dim3(256), 0,
+         as_stream(stream), dldw_vtx2xyz, dldw_vtx2val, pix2tri, sc, num_vdim, vtx2val,
+         dldw_pix2val, img_w, img_h, transform_ndc2world, err);
+  return launch_status();
+}
+
 int dmsh_rasterized_edge_gradient_bwd(void* stream, const uint32_t* tri2vtx, uint32_t num_tri,
                                       const float* vtx2xyz, uint32_t num_vtx,
                                       float* dldw_vtx2xyz, const float* transform_world2pix,

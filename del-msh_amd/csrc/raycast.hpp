@@ -602,5 +602,134 @@ static __global__ __launch_bounds__(256) void k_pix2depth_bwd(
   }
 }
 
+// ---------------------------------------------------------------------------
+// attribute interpolation (pix2tri.rs:176-304)
+// ---------------------------------------------------------------------------
+// Barycentric coordinates of the line (org, dir) against the plane of the
+// triangle: Moller-Trumbore's u, v with no range or epsilon test; also the
+// reciprocal determinant and the two cross products the backward reuses.
+__device__ __forceinline__ void plane_line_bc(const float p0[3], const float p1[3],
+                                              const float p2[3], const float org[3],
+                                              const float dir[3], float& u, float& v,
+                                              float& inv_det, float pvec[3], float e1xd[3]) {
+  float e1[3], e2[3], tvec[3], qvec[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    e1[i] = p1[i] - p0[i];
+    e2[i] = p2[i] - p0[i];
+    tvec[i] = org[i] - p0[i];
+  }
+  cross3(dir, e2, pvec);
+  const float det = dot3(e1, pvec);
+  inv_det = 1.f / det;
+  u = inv_det * dot3(tvec, pvec);
+  cross3(tvec, e1, qvec);
+  v = inv_det * dot3(dir, qvec);
+  cross3(e1, dir, e1xd);
+}
+
+// One thread per pixel: pix2val = bc0 val[i0] + bc1 val[i1] + bc2 val[i2] on
+// covered pixels, 0 on the background.
+static __global__ __launch_bounds__(256) void k_interpolate(
+    float* __restrict__ pix2val, const uint32_t* __restrict__ pix2tri, TriMesh sc,
+    uint32_t num_vdim, const float* __restrict__ vtx2val, uint32_t img_w, uint32_t img_h,
+    const float* __restrict__ T, uint32_t* __restrict__ err) {
+  const uint32_t ip = blockIdx.x * blockDim.x + threadIdx.x;
+  if (ip >= img_w * img_h) return;
+  float* out = pix2val + (size_t)ip * num_vdim;
+  const uint32_t tri = pix2tri[ip];
+  uint32_t iv[3] = {0u, 0u, 0u};
+  bool ok = tri != DMSH_U32MAX;
+  if (ok && tri >= sc.num_tri) {
+    raise_error(err, DMSH_DEVERR_INDEX);
+    ok = false;
+  }
+  if (ok) {
+    const uint32_t* tv = sc.tri2vtx + (size_t)tri * 3;
+    iv[0] = tv[0];
+    iv[1] = tv[1];
+    iv[2] = tv[2];
+    if (iv[0] >= sc.num_vtx || iv[1] >= sc.num_vtx || iv[2] >= sc.num_vtx) {
+      raise_error(err, DMSH_DEVERR_INDEX);
+      ok = false;
+    }
+  }
+  if (!ok) {
+    for (uint32_t d = 0; d < num_vdim; ++d) out[d] = 0.f;
+    return;
+  }
+  float m[16];
+  load_transform(T, m);
+  const uint32_t ih = ip / img_w, iw = ip - ih * img_w;
+  float org[3], dir[3], p[3][3], u, v, inv_det, pvec[3], c[3];
+  ray_for_pixel(iw, ih, img_w, img_h, m, org, dir);
+#pragma unroll
+  for (int k = 0; k < 3; ++k)
+#pragma unroll
+    for (int i = 0; i < 3; ++i) p[k][i] = sc.xyz[(size_t)iv[k] * 3 + i];
+  plane_line_bc(p[0], p[1], p[2], org, dir, u, v, inv_det, pvec, c);
+  const float bc0 = 1.f - u - v;
+  for (uint32_t d = 0; d < num_vdim; ++d)
+    out[d] = (bc0 * vtx2val[(size_t)iv[0] * num_vdim + d] +
+              u * vtx2val[(size_t)iv[1] * num_vdim + d]) +
+             v * vtx2val[(size_t)iv[2] * num_vdim + d];
+}
+
+// Backward, one thread per covered pixel.  With gu = dL/dbc1 - dL/dbc0 and
+// gv = dL/dbc2 - dL/dbc0 (dL/dt = 0), implicit differentiation of
+// u e1 + v e2 - t dir = org - p0 gives dL/dp0 = -bc0 lam, dL/dp1 = -u lam,
+// dL/dp2 = -v lam with lam = (gu (dir x e2) + gv (e1 x dir)) / det; and
+// dL/dval[ik] = dL/dpix bc_k.  Float atomics accumulate both.
+static __global__ __launch_bounds__(256) void k_interpolate_bwd(
+    float* __restrict__ dldw_xyz, float* __restrict__ dldw_val,
+    const uint32_t* __restrict__ pix2tri, TriMesh sc, uint32_t num_vdim,
+    const float* __restrict__ vtx2val, const float* __restrict__ dldw_pix2val, uint32_t img_w,
+    uint32_t img_h, const float* __restrict__ T, uint32_t* __restrict__ err) {
+  const uint32_t ip = blockIdx.x * blockDim.x + threadIdx.x;
+  if (ip >= img_w * img_h) return;
+  const uint32_t tri = pix2tri[ip];
+  if (tri == DMSH_U32MAX) return;
+  if (tri >= sc.num_tri) {
+    raise_error(err, DMSH_DEVERR_INDEX);
+    return;
+  }
+  const uint32_t* tv = sc.tri2vtx + (size_t)tri * 3;
+  const uint32_t iv[3] = {tv[0], tv[1], tv[2]};
+  if (iv[0] >= sc.num_vtx || iv[1] >= sc.num_vtx || iv[2] >= sc.num_vtx) {
+    raise_error(err, DMSH_DEVERR_INDEX);
+    return;
+  }
+  const float* g = dldw_pix2val + (size_t)ip * num_vdim;
+  float gb[3] = {0.f, 0.f, 0.f};
+  for (uint32_t d = 0; d < num_vdim; ++d) {
+    const float gd = g[d];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) gb[k] += vtx2val[(size_t)iv[k] * num_vdim + d] * gd;
+  }
+  const float gu = gb[1] - gb[0], gv = gb[2] - gb[0];
+  float m[16];
+  load_transform(T, m);
+  const uint32_t ih = ip / img_w, iw = ip - ih * img_w;
+  float org[3], dir[3], p[3][3], u, v, inv_det, pvec[3], c[3];
+  ray_for_pixel(iw, ih, img_w, img_h, m, org, dir);
+#pragma unroll
+  for (int k = 0; k < 3; ++k)
+#pragma unroll
+    for (int i = 0; i < 3; ++i) p[k][i] = sc.xyz[(size_t)iv[k] * 3 + i];
+  plane_line_bc(p[0], p[1], p[2], org, dir, u, v, inv_det, pvec, c);
+  const float bc[3] = {1.f - u - v, u, v};
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    const float lam = (gu * pvec[i] + gv * c[i]) * inv_det;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) unsafeAtomicAdd(dldw_xyz + (size_t)iv[k] * 3 + i, -(bc[k] * lam));
+  }
+  for (uint32_t d = 0; d < num_vdim; ++d) {
+    const float gd = g[d];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) unsafeAtomicAdd(dldw_val + (size_t)iv[k] * num_vdim + d, gd * bc[k]);
+  }
+}
+
 }  // namespace rc
 }  // namespace dmsh

@@ -205,6 +205,28 @@ int dmsh_pix2depth_bwd_wrt_vtx2xyz(void *stream, float *dldw_vtx2xyz,
                                    size_t ws_bytes);
 
 /* ---------------------------------------------------------------------
+ * Attribute interpolation over pix2tri and its backward.  Replaces
+ * pix2tri.cu:78-182 (interpolate / interpolate_bwd) as launched by
+ * del-msh-dlpack/src/pix2tri.rs:98-271; results follow del-msh-cpu/src/
+ * pix2tri.rs:176-304: per covered pixel the barycentric coordinates of the
+ * pixel's line against the plane of its triangle (u, v of the
+ * Moller-Trumbore form, no range test), pix2val = bc0 val0 + bc1 val1 +
+ * bc2 val2 ([H][W][num_vdim], 0 on the background).  The backward
+ * accumulates into dldw_vtx2xyz [num_vtx][3] and dldw_vtx2val
+ * [num_vtx][num_vdim] (zero them first) with float atomics.
+ * transform_ndc2world: 16 floats column-major.
+ * ------------------------------------------------------------------- */
+int dmsh_interpolate(void *stream, const uint32_t *pix2tri, uint32_t img_w, uint32_t img_h,
+                     uint32_t num_tri, const uint32_t *tri2vtx, uint32_t num_vtx,
+                     const float *vtx2xyz, uint32_t num_vdim, const float *vtx2val,
+                     const float *transform_ndc2world, float *pix2val);
+int dmsh_interpolate_bwd(void *stream, const uint32_t *pix2tri, uint32_t img_w, uint32_t img_h,
+                         uint32_t num_tri, const uint32_t *tri2vtx, uint32_t num_vtx,
+                         const float *vtx2xyz, uint32_t num_vdim, const float *vtx2val,
+                         const float *transform_ndc2world, const float *dldw_pix2val,
+                         float *dldw_vtx2xyz, float *dldw_vtx2val);
+
+/* ---------------------------------------------------------------------
  * Edge-gradient backward of the rasterized silhouette (accumulates into
  * dldw_vtx2xyz).  Replaces kernels `bwd_hedge` / `bwd_vedge`
  * (del-msh-cuda-kernels/cuda/rasterized_edge_gradient.cu:387-520) launched

@@ -1353,3 +1353,99 @@ void orc_multiply_graph_laplacian(size_t num_vtx, const uint32_t *vtx2idx,
     }
   }
 }
+
+/* ========================================================================
+ * Attribute interpolation over pix2tri
+ * ===================================================================== */
+
+/* F7: barycentric coordinates of the line (org, dir) against the plane of
+ * (p0, p1, p2) -- Moller-Trumbore's u, v with no range or epsilon test
+ * (tri3::intersection_plane_of_tri3_against_line as used at
+ * pix2tri.rs:176-229).  Also returns inv_det and the two cross products the
+ * backward reuses. */
+static void plane_line_bc(const float p0[3], const float p1[3], const float p2[3],
+                          const float org[3], const float dir[3], float *u_out, float *v_out,
+                          float *inv_det_out, float pvec[3], float e1xd[3]) {
+  float e1[3], e2[3], tvec[3], qvec[3];
+  for (int i = 0; i < 3; ++i) {
+    e1[i] = p1[i] - p0[i];
+    e2[i] = p2[i] - p0[i];
+    tvec[i] = org[i] - p0[i];
+  }
+  cross3(dir, e2, pvec);
+  const float det = dot3(e1, pvec);
+  const float inv_det = 1.f / det;
+  *u_out = inv_det * dot3(tvec, pvec);
+  cross3(tvec, e1, qvec);
+  *v_out = inv_det * dot3(dir, qvec);
+  *inv_det_out = inv_det;
+  cross3(e1, dir, e1xd);
+}
+
+/* pix2tri.rs:176-229: pix2val[p] = bc0 val[i0] + bc1 val[i1] + bc2 val[i2]
+ * for covered pixels, 0 elsewhere */
+int orc_interpolate(float *pix2val, const uint32_t *pix2tri, uint32_t img_w, uint32_t img_h,
+                    const uint32_t *tri2vtx, size_t num_tri, const float *vtx2xyz,
+                    size_t num_vtx, uint32_t num_vdim, const float *vtx2val, const float T[16]) {
+  const size_t np = (size_t)img_w * img_h;
+  int bad = 0;
+#pragma omp parallel for schedule(static) reduction(| : bad)
+  for (size_t ip = 0; ip < np; ++ip) {
+    float *out = pix2val + ip * num_vdim;
+    const uint32_t it = pix2tri[ip];
+    if (it == U32MAX) {
+      for (uint32_t d = 0; d < num_vdim; ++d) out[d] = 0.f;
+      continue;
+    }
+    if (it >= num_tri) { bad = 1; continue; }
+    const uint32_t i0 = tri2vtx[(size_t)it * 3], i1 = tri2vtx[(size_t)it * 3 + 1],
+                   i2 = tri2vtx[(size_t)it * 3 + 2];
+    if (i0 >= num_vtx || i1 >= num_vtx || i2 >= num_vtx) { bad = 1; continue; }
+    float org[3], dir[3], u, v, inv_det, pvec[3], c[3];
+    orc_ray_for_pixel((uint32_t)(ip % img_w), (uint32_t)(ip / img_w), img_w, img_h, T, org, dir);
+    plane_line_bc(vtx2xyz + (size_t)i0 * 3, vtx2xyz + (size_t)i1 * 3, vtx2xyz + (size_t)i2 * 3,
+                  org, dir, &u, &v, &inv_det, pvec, c);
+    const float bc0 = 1.f - u - v;
+    for (uint32_t d = 0; d < num_vdim; ++d)
+      out[d] = (bc0 * vtx2val[(size_t)i0 * num_vdim + d] + u * vtx2val[(size_t)i1 * num_vdim + d]) +
+               v * vtx2val[(size_t)i2 * num_vdim + d];
+  }
+  return bad ? -1 : 0;
+}
+
+/* pix2tri.rs:231-304 (serial; accumulates).  F8: with gu = dL/du, gv = dL/dv
+ * (dL/dbc1 - dL/dbc0, dL/dbc2 - dL/dbc0) and dL/dt = 0, the gradient of the
+ * plane intersection with respect to the corners is dL/dp0 = -bc0 lam,
+ * dL/dp1 = -u lam, dL/dp2 = -v lam, lam = (gu (dir x e2) + gv (e1 x dir)) / det
+ * (implicit differentiation of u e1 + v e2 - t dir = org - p0). */
+int orc_interpolate_bwd(float *dldw_vtx2xyz, float *dldw_vtx2val, const uint32_t *pix2tri,
+                        uint32_t img_w, uint32_t img_h, const uint32_t *tri2vtx, size_t num_tri,
+                        const float *vtx2xyz, size_t num_vtx, uint32_t num_vdim,
+                        const float *vtx2val, const float T[16], const float *dldw_pix2val) {
+  const size_t np = (size_t)img_w * img_h;
+  for (size_t ip = 0; ip < np; ++ip) {
+    const uint32_t it = pix2tri[ip];
+    if (it == U32MAX) continue;
+    if (it >= num_tri) return -1;
+    const uint32_t iv[3] = {tri2vtx[(size_t)it * 3], tri2vtx[(size_t)it * 3 + 1],
+                            tri2vtx[(size_t)it * 3 + 2]};
+    if (iv[0] >= num_vtx || iv[1] >= num_vtx || iv[2] >= num_vtx) return -1;
+    const float *g = dldw_pix2val + ip * num_vdim;
+    float gb[3] = {0.f, 0.f, 0.f};
+    for (uint32_t d = 0; d < num_vdim; ++d)
+      for (int k = 0; k < 3; ++k) gb[k] += vtx2val[(size_t)iv[k] * num_vdim + d] * g[d];
+    const float gu = gb[1] - gb[0], gv = gb[2] - gb[0];
+    float org[3], dir[3], u, v, inv_det, pvec[3], c[3];
+    orc_ray_for_pixel((uint32_t)(ip % img_w), (uint32_t)(ip / img_w), img_w, img_h, T, org, dir);
+    plane_line_bc(vtx2xyz + (size_t)iv[0] * 3, vtx2xyz + (size_t)iv[1] * 3,
+                  vtx2xyz + (size_t)iv[2] * 3, org, dir, &u, &v, &inv_det, pvec, c);
+    const float bc[3] = {1.f - u - v, u, v};
+    for (int i = 0; i < 3; ++i) {
+      const float lam = (gu * pvec[i] + gv * c[i]) * inv_det;
+      for (int k = 0; k < 3; ++k) dldw_vtx2xyz[(size_t)iv[k] * 3 + i] += -(bc[k] * lam);
+    }
+    for (uint32_t d = 0; d < num_vdim; ++d)
+      for (int k = 0; k < 3; ++k) dldw_vtx2val[(size_t)iv[k] * num_vdim + d] += g[d] * bc[k];
+  }
+  return 0;
+}

@@ -170,6 +170,19 @@ void orc_edge_gradient_bwd(const uint32_t *tri2vtx, const float *vtx2xyz,
                            const uint32_t *pix2tri, uint32_t num_vdim,
                            const float *pix2val, const float *dldw_pix2val);
 
+/* ---- attribute interpolation ------------------------------------------- */
+/* pix2tri.rs:176-229 (F7 plane barycentrics); returns -1 on a bad index */
+int orc_interpolate(float *pix2val, const uint32_t *pix2tri, uint32_t img_w, uint32_t img_h,
+                    const uint32_t *tri2vtx, size_t num_tri, const float *vtx2xyz,
+                    size_t num_vtx, uint32_t num_vdim, const float *vtx2val,
+                    const float ndc2world[16]);
+/* pix2tri.rs:231-304 (F8; accumulates) */
+int orc_interpolate_bwd(float *dldw_vtx2xyz, float *dldw_vtx2val, const uint32_t *pix2tri,
+                        uint32_t img_w, uint32_t img_h, const uint32_t *tri2vtx, size_t num_tri,
+                        const float *vtx2xyz, size_t num_vtx, uint32_t num_vdim,
+                        const float *vtx2val, const float ndc2world[16],
+                        const float *dldw_pix2val);
+
 /* ---- mesh topology ------------------------------------------------------- */
 /* vtx2elem.rs:58-96: elements around each vertex (offsets vtx2idx[num_vtx+1],
  * idx2elem[num_elem*num_node], elements ascending per vertex).  Returns 0, or

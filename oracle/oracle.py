@@ -453,3 +453,52 @@ def multiply_graph_laplacian(vtx2idx, idx2vtx, vtx2rhs):
     L.orc_multiply_graph_laplacian(vtx2idx.shape[0] - 1, _u32(vtx2idx), _u32(idx2vtx), rhs.shape[1],
                                    rhs.reshape(-1), out.reshape(-1))
     return out
+
+
+# ---------------------------------------------------------------------------
+# attribute interpolation
+# ---------------------------------------------------------------------------
+def _sig_interp():
+    L = lib()
+    if getattr(L, "_interp_ready", False):
+        return L
+    L.orc_interpolate.restype = ctypes.c_int
+    L.orc_interpolate.argtypes = [_f32p, _u32p, ctypes.c_uint32, ctypes.c_uint32, _u32p, _sz, _f32p,
+                                  _sz, ctypes.c_uint32, _f32p, _f32p]
+    L.orc_interpolate_bwd.restype = ctypes.c_int
+    L.orc_interpolate_bwd.argtypes = [_f32p, _f32p, _u32p, ctypes.c_uint32, ctypes.c_uint32, _u32p,
+                                      _sz, _f32p, _sz, ctypes.c_uint32, _f32p, _f32p, _f32p]
+    L._interp_ready = True
+    return L
+
+
+def interpolate(pix2tri_, tri2vtx, vtx2xyz, vtx2val, ndc2world_colmajor):
+    """pix2tri.rs:176-229 -> pix2val [H, W, num_vdim]."""
+    L = _sig_interp()
+    h, w = pix2tri_.shape
+    val = _f32(vtx2val)
+    nv = val.shape[1]
+    out = np.zeros((h, w, nv), np.float32)
+    t = _u32(tri2vtx)
+    x = _f32(vtx2xyz)
+    if L.orc_interpolate(out.reshape(-1), _u32(pix2tri_).reshape(-1), w, h, t.reshape(-1), t.shape[0],
+                         x.reshape(-1), x.shape[0], nv, val.reshape(-1), _f32(ndc2world_colmajor)):
+        raise ValueError("index out of range")
+    return out
+
+
+def interpolate_bwd(pix2tri_, tri2vtx, vtx2xyz, vtx2val, ndc2world_colmajor, dldw_pix2val):
+    """pix2tri.rs:231-304 -> (dL/dvtx2xyz, dL/dvtx2val)."""
+    L = _sig_interp()
+    h, w = pix2tri_.shape
+    val = _f32(vtx2val)
+    nv = val.shape[1]
+    t = _u32(tri2vtx)
+    x = _f32(vtx2xyz)
+    gx = np.zeros(x.shape, np.float32)
+    gv = np.zeros(val.shape, np.float32)
+    if L.orc_interpolate_bwd(gx.reshape(-1), gv.reshape(-1), _u32(pix2tri_).reshape(-1), w, h,
+                             t.reshape(-1), t.shape[0], x.reshape(-1), x.shape[0], nv, val.reshape(-1),
+                             _f32(ndc2world_colmajor), _f32(dldw_pix2val).reshape(-1)):
+        raise ValueError("index out of range")
+    return gx, gv
