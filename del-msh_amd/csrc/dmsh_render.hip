@@ -261,4 +261,56 @@ int dmsh_rasterized_edge_gradient_bwd(void* stream, const uint32_t* tri2vtx, uin
   return launch_status();
 }
 
+
+int dmsh_edge_gradient_and_type(void* stream, const uint32_t* tri2vtx, uint32_t num_tri,
+                                const float* vtx2xyz, uint32_t num_vtx,
+                                const float* transform_world2pix, uint32_t img_w, uint32_t img_h,
+                                const uint32_t* pix2tri, uint32_t num_vdim, const float* pix2val,
+                                const float* dldw_pix2val, uint8_t* hedge2type,
+                                float* hedge2dldr, uint8_t* vedge2type, float* vedge2dldr) {
+  const uint64_t num_pix = (uint64_t)img_w * img_h;
+  if (img_w < 2 || img_h < 2 || num_pix >= (1ull << 32) || num_vdim == 0) return DMSH_ERR_INVALID;
+  if (!transform_world2pix || !pix2tri || !pix2val || !dldw_pix2val || !hedge2type ||
+      !hedge2dldr || !vedge2type || !vedge2dldr)
+    return DMSH_ERR_INVALID;
+  if (num_tri > 0 && (!tri2vtx || !vtx2xyz)) return DMSH_ERR_INVALID;
+  uint32_t* err = device_error_word();
+  if (!err) return DMSH_ERR_HIP;
+  launch("k_edge_type_grad", eg::k_edge_type_grad, dim3(ceil_div(img_w, 64), ceil_div(img_h, 4)),
+         dim3(64, 4), 0, as_stream(stream), tri2vtx, num_tri, vtx2xyz, num_vtx,
+         transform_world2pix, img_w, img_h, pix2tri, num_vdim, pix2val, dldw_pix2val, hedge2type,
+         hedge2dldr, vedge2type, vedge2dldr, err);
+  return launch_status();
+}
+
+int dmsh_smooth_gradient(void* stream, uint32_t img_w, uint32_t img_h, const uint8_t* hedge2type,
+                         const uint8_t* vedge2type, uint32_t num_iter, float* hedge2dldr,
+                         float* vedge2dldr) {
+  if (img_w < 2 || img_h < 2 || (uint64_t)img_w * img_h >= (1ull << 31)) return DMSH_ERR_INVALID;
+  if (!hedge2type || !vedge2type || !hedge2dldr || !vedge2dldr) return DMSH_ERR_INVALID;
+  const uint32_t cells = (img_h - 1) * img_w + img_h * (img_w - 1);
+  hipStream_t s = as_stream(stream);
+  for (uint32_t it = 0; it < num_iter; ++it) {
+    for (uint32_t color = 0; color < 2; ++color) {
+      launch("k_smooth_red_black", eg::k_smooth_red_black, dim3(ceil_div(cells, 256)), dim3(256),
+             0, s, img_w, img_h, hedge2type, hedge2dldr, vedge2type, vedge2dldr, color);
+      int rc = launch_status();
+      if (rc != DMSH_OK) return rc;
+    }
+  }
+  return DMSH_OK;
+}
+
+int dmsh_interpolate_staggered_grid(void* stream, uint32_t img_w, uint32_t img_h,
+                                    const float* hedge2vy, const float* vedge2vx,
+                                    uint32_t num_vtx, const float* vtx2xy, float* vtx2velo) {
+  if (num_vtx == 0) return DMSH_OK;
+  if (img_w < 2 || img_h < 2 || !hedge2vy || !vedge2vx || !vtx2xy || !vtx2velo)
+    return DMSH_ERR_INVALID;
+  launch("k_interpolate_staggered", eg::k_interpolate_staggered, dim3(ceil_div(num_vtx, 256)),
+         dim3(256), 0, as_stream(stream), img_w, img_h, hedge2vy, vedge2vx, num_vtx, vtx2xy,
+         vtx2velo);
+  return launch_status();
+}
+
 }  // extern "C"

@@ -145,5 +145,178 @@ static __global__ __launch_bounds__(256) void k_edge_grad_bwd(
   }
 }
 
+// ---------------------------------------------------------------------------
+// Smoothed variant on the staggered grid of pixel edges
+// (rasterized_edge_gradient.rs:77-432): hedge (ih, iw) lies between pixels
+// (iw, ih) and (iw, ih + 1), [H-1][W]; vedge (ih, iw) between (iw, ih) and
+// (iw + 1, ih), [H][W-1].  Type 0: same triangle; 1: shared edge; 2 / 3: the
+// first / second pixel's triangle is in front; 4: intersection.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint8_t pair_type(const Ctx& c, uint32_t t0, uint32_t t1,
+                                             const float pc0[2], const float pc1[2]) {
+  if (t0 == t1) return 0;
+  float b[3];
+  const int h0 = bary(c, pc0, t1, b);
+  const bool in0 = inside(h0, b);
+  const int h1 = bary(c, pc1, t0, b);
+  const bool in1 = inside(h1, b);
+  if (h0 < 0 || h1 < 0) {
+    raise_error(c.err, DMSH_DEVERR_INDEX);
+    return 0;
+  }
+  if (!in0 && !in1) return 1;
+  if (in0 && !in1) return 2;
+  if (!in0 && in1) return 3;
+  return 4;
+}
+
+__device__ __forceinline__ float pair_dldr(uint32_t nv, const float* __restrict__ pix2val,
+                                           const float* __restrict__ dpix, size_t ip0,
+                                           size_t ip1) {
+  float r = 0.f;
+  for (uint32_t k = 0; k < nv; ++k) {
+    const float v0 = pix2val[ip0 * nv + k], v1 = pix2val[ip1 * nv + k];
+    const float g0 = dpix[ip0 * nv + k], g1 = dpix[ip1 * nv + k];
+    r += (g0 + g1) * 0.5f * (v0 - v1);
+  }
+  return r;
+}
+
+// block (64, 4): the thread of pixel (iw, ih) does the hedge below it and
+// the vedge to its right.
+static __global__ __launch_bounds__(256) void k_edge_type_grad(
+    const uint32_t* __restrict__ tri2vtx, uint32_t num_tri, const float* __restrict__ xyz,
+    uint32_t num_vtx, const float* __restrict__ T, uint32_t img_w, uint32_t img_h,
+    const uint32_t* __restrict__ pix2tri, uint32_t nv, const float* __restrict__ pix2val,
+    const float* __restrict__ dpix, uint8_t* __restrict__ htype, float* __restrict__ hdldr,
+    uint8_t* __restrict__ vtype, float* __restrict__ vdldr, uint32_t* err) {
+  const uint32_t iw = blockIdx.x * 64u + threadIdx.x;
+  const uint32_t ih = blockIdx.y * 4u + threadIdx.y;
+  if (iw >= img_w || ih >= img_h) return;
+  Ctx c;
+  c.tri2vtx = tri2vtx;
+  c.xyz = xyz;
+  c.num_tri = num_tri;
+  c.num_vtx = num_vtx;
+  c.err = err;
+#pragma unroll
+  for (int k = 0; k < 16; ++k) c.m[k] = T[k];
+  const size_t ip = (size_t)ih * img_w + iw;
+  const float pc[2] = {(float)iw + 0.5f, (float)ih + 0.5f};
+  if (ih + 1 < img_h) {
+    const float pcs[2] = {(float)iw + 0.5f, (float)(ih + 1) + 0.5f};
+    const size_t e = (size_t)ih * img_w + iw;
+    htype[e] = pair_type(c, pix2tri[ip], pix2tri[ip + img_w], pc, pcs);
+    hdldr[e] = pair_dldr(nv, pix2val, dpix, ip, ip + img_w);
+  }
+  if (iw + 1 < img_w) {
+    const float pce[2] = {(float)(iw + 1) + 0.5f, (float)ih + 0.5f};
+    const size_t e = (size_t)ih * (img_w - 1) + iw;
+    vtype[e] = pair_type(c, pix2tri[ip], pix2tri[ip + 1], pc, pce);
+    vdldr[e] = pair_dldr(nv, pix2val, dpix, ip, ip + 1);
+  }
+}
+
+__device__ __forceinline__ bool fixed_type(uint8_t t) { return t == 2 || t == 3; }
+
+// One red-black half-step over both grids (they do not read each other's
+// values, only types): cells with (iw + ih) & 1 == color and a free type take
+// the average of their admitted neighbours, summed north, south, west, east
+// (hedges) or west, east, north, south (vedges), as
+// rasterized_edge_gradient.cu:139-275.  Neighbours have the other colour, so
+// no cell reads a value written in the same launch.
+static __global__ __launch_bounds__(256) void k_smooth_red_black(
+    uint32_t W, uint32_t H, const uint8_t* __restrict__ ht, float* __restrict__ hv,
+    const uint8_t* __restrict__ vt, float* __restrict__ vv, uint32_t color) {
+  const uint32_t nh = (H - 1) * W, nvx = H * (W - 1);
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nh) {
+    const uint32_t iw = i % W, ih = i / W;
+    if (((iw + ih) & 1u) != color || fixed_type(ht[i])) return;
+    float s = 0.f;
+    uint32_t n = 0;
+    if (ih > 0 && ht[i - W] != 2) { s += hv[i - W]; ++n; }
+    if (ih + 1 < H - 1 && ht[i + W] != 3) { s += hv[i + W]; ++n; }
+    if (iw > 0 && !fixed_type(vt[ih * (W - 1) + iw - 1]) &&
+        !fixed_type(vt[(ih + 1) * (W - 1) + iw - 1])) {
+      s += hv[i - 1];
+      ++n;
+    }
+    if (iw + 1 < W && !fixed_type(vt[ih * (W - 1) + iw]) &&
+        !fixed_type(vt[(ih + 1) * (W - 1) + iw])) {
+      s += hv[i + 1];
+      ++n;
+    }
+    if (n) hv[i] = s / (float)n;
+    return;
+  }
+  const uint32_t j = i - nh;
+  if (j >= nvx) return;
+  const uint32_t w = W - 1;
+  const uint32_t iw = j % w, ih = j / w;
+  if (((iw + ih) & 1u) != color || fixed_type(vt[j])) return;
+  float s = 0.f;
+  uint32_t n = 0;
+  if (iw > 0 && vt[j - 1] != 2) { s += vv[j - 1]; ++n; }
+  if (iw + 1 < w && vt[j + 1] != 3) { s += vv[j + 1]; ++n; }
+  if (ih > 0 && !fixed_type(ht[(ih - 1) * W + iw]) && !fixed_type(ht[(ih - 1) * W + iw + 1])) {
+    s += vv[j - w];
+    ++n;
+  }
+  if (ih + 1 < H && !fixed_type(ht[ih * W + iw]) && !fixed_type(ht[ih * W + iw + 1])) {
+    s += vv[j + w];
+    ++n;
+  }
+  if (n) vv[j] = s / (float)n;
+}
+
+// Rust's `f32 as i32`: NaN -> 0, saturating
+__device__ __forceinline__ int f2i_sat(float x) {
+  if (x != x) return 0;
+  if (x >= 2147483647.f) return 2147483647;
+  if (x <= -2147483648.f) return -2147483647 - 1;
+  return (int)x;
+}
+
+__device__ __forceinline__ int clampi(int x, int lo, int hi) {
+  return x < lo ? lo : (x > hi ? hi : x);
+}
+__device__ __forceinline__ float clamp01(float x) { return x < 0.f ? 0.f : (x > 1.f ? 1.f : x); }
+
+// One thread per vertex: x-velocity bilinear from the vedges (sampled at
+// (iw + 1, ih + 0.5)), y-velocity from the hedges (at (iw + 0.5, ih + 1)).
+static __global__ __launch_bounds__(256) void k_interpolate_staggered(
+    uint32_t W, uint32_t H, const float* __restrict__ hvy, const float* __restrict__ vvx,
+    uint32_t num_vtx, const float* __restrict__ vtx2xy, float* __restrict__ vtx2velo) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= num_vtx) return;
+  const float px = vtx2xy[(size_t)i * 2], py = vtx2xy[(size_t)i * 2 + 1];
+  {
+    const float gx = px - 1.0f, gy = py - 0.5f;
+    const int ix0 = clampi(f2i_sat(floorf(gx)), 0, (int)W - 2);
+    const int iy0 = clampi(f2i_sat(floorf(gy)), 0, (int)H - 2);
+    const int ix1 = ix0 + 1 < (int)W - 2 ? ix0 + 1 : (int)W - 2;
+    const int iy1 = iy0 + 1 < (int)H - 1 ? iy0 + 1 : (int)H - 1;
+    const float tx = clamp01(gx - (float)ix0), ty = clamp01(gy - (float)iy0);
+    const int w = (int)W - 1;
+    vtx2velo[(size_t)i * 2] = (((1.f - tx) * (1.f - ty) * vvx[iy0 * w + ix0] +
+                                tx * (1.f - ty) * vvx[iy0 * w + ix1]) +
+                               (1.f - tx) * ty * vvx[iy1 * w + ix0]) +
+                              tx * ty * vvx[iy1 * w + ix1];
+  }
+  {
+    const float gx = px - 0.5f, gy = py - 1.0f;
+    const int ix0 = clampi(f2i_sat(floorf(gx)), 0, (int)W - 2);
+    const int iy0 = clampi(f2i_sat(floorf(gy)), 0, (int)H - 2);
+    const int ix1 = ix0 + 1 < (int)W - 1 ? ix0 + 1 : (int)W - 1;
+    const int iy1 = iy0 + 1 < (int)H - 2 ? iy0 + 1 : (int)H - 2;
+    const float tx = clamp01(gx - (float)ix0), ty = clamp01(gy - (float)iy0);
+    vtx2velo[(size_t)i * 2 + 1] = (((1.f - tx) * (1.f - ty) * hvy[iy0 * W + ix0] +
+                                    tx * (1.f - ty) * hvy[iy0 * W + ix1]) +
+                                   (1.f - tx) * ty * hvy[iy1 * W + ix0]) +
+                                  tx * ty * hvy[iy1 * W + ix1];
+  }
+}
+
 }  // namespace eg
 }  // namespace dmsh

@@ -125,6 +125,13 @@ _PROTOTYPES = {
     "dmsh_interpolate_bwd": (
         c_int, [c_void_p, c_void_p, c_u32, c_u32, c_u32, c_void_p, c_u32, c_void_p, c_u32, c_void_p,
                 c_void_p, c_void_p, c_void_p, c_void_p]),
+    "dmsh_edge_gradient_and_type": (
+        c_int, [c_void_p, c_void_p, c_u32, c_void_p, c_u32, c_void_p, c_u32, c_u32, c_void_p, c_u32,
+                c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "dmsh_smooth_gradient": (
+        c_int, [c_void_p, c_u32, c_u32, c_void_p, c_void_p, c_u32, c_void_p, c_void_p]),
+    "dmsh_interpolate_staggered_grid": (
+        c_int, [c_void_p, c_u32, c_u32, c_void_p, c_void_p, c_u32, c_void_p, c_void_p]),
     "dmsh_exclusive_scan_workspace_bytes": (c_size, [c_u32]),
     "dmsh_exclusive_scan_u32": (c_int, [c_void_p, c_u32, c_void_p, c_void_p, c_void_p, c_size]),
     "dmsh_vtx2elem_workspace_bytes": (c_size, [c_u32, c_u32, c_u32]),

@@ -205,6 +205,36 @@ int dmsh_pix2depth_bwd_wrt_vtx2xyz(void *stream, float *dldw_vtx2xyz,
                                    size_t ws_bytes);
 
 /* ---------------------------------------------------------------------
+ * Smoothed edge gradient (the backward of RasterizedEdgeGradient.
+ * AutogradWithSmooth, del_msh_dlpack/RasterizedEdgeGradient/torch.py:
+ * 237-274).  Replaces rasterized_edge_gradient.cu:43-326
+ * (hedge/vedge_gradient_and_type, smooth_hedge/vedge_red_black,
+ * interpolate_staggered_grid) as launched by del-msh-dlpack/src/
+ * rasterized_edge_gradient.rs:153-450.  Pixel-edge grids: hedge (ih, iw)
+ * between pixels (iw, ih) and (iw, ih+1), [H-1][W]; vedge (ih, iw) between
+ * (iw, ih) and (iw+1, ih), [H][W-1]; types 0 same triangle, 1 shared edge,
+ * 2 / 3 first / second pixel's triangle in front, 4 intersection;
+ * dldr = sum_k (g0 + g1) * 0.5 * (v0 - v1) (rasterized_edge_gradient.rs:
+ * 77-200).  dmsh_smooth_gradient runs num_iter red-black sweeps of both
+ * grids in place (cells of type 2 / 3 stay fixed) as the reference GPU path
+ * does; del-msh-cpu's serial sweeps (rasterized_edge_gradient.rs:250-432)
+ * converge to the same fixed point.  img_w, img_h >= 2.
+ * ------------------------------------------------------------------- */
+int dmsh_edge_gradient_and_type(void *stream, const uint32_t *tri2vtx, uint32_t num_tri,
+                                const float *vtx2xyz, uint32_t num_vtx,
+                                const float *transform_world2pix, uint32_t img_w, uint32_t img_h,
+                                const uint32_t *pix2tri, uint32_t num_vdim, const float *pix2val,
+                                const float *dldw_pix2val, uint8_t *hedge2type,
+                                float *hedge2dldr, uint8_t *vedge2type, float *vedge2dldr);
+int dmsh_smooth_gradient(void *stream, uint32_t img_w, uint32_t img_h, const uint8_t *hedge2type,
+                         const uint8_t *vedge2type, uint32_t num_iter, float *hedge2dldr,
+                         float *vedge2dldr);
+/* rasterized_edge_gradient.rs:202-248: vtx2velo [num_vtx][2] */
+int dmsh_interpolate_staggered_grid(void *stream, uint32_t img_w, uint32_t img_h,
+                                    const float *hedge2vy, const float *vedge2vx,
+                                    uint32_t num_vtx, const float *vtx2xy, float *vtx2velo);
+
+/* ---------------------------------------------------------------------
  * Attribute interpolation over pix2tri and its backward.  Replaces
  * pix2tri.cu:78-182 (interpolate / interpolate_bwd) as launched by
  * del-msh-dlpack/src/pix2tri.rs:98-271; results follow del-msh-cpu/src/

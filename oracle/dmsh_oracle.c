@@ -1449,3 +1449,192 @@ int orc_interpolate_bwd(float *dldw_vtx2xyz, float *dldw_vtx2val, const uint32_t
   }
   return 0;
 }
+
+/* ========================================================================
+ * Smoothed edge gradient on the staggered pixel-edge grid
+ * ===================================================================== */
+
+static uint8_t edge_type(const uint32_t *tri2vtx, const float *xyz, const float m[16],
+                         uint32_t itri0, uint32_t itri1, const float pc0[2], const float pc1[2]) {
+  if (itri0 == itri1) return 0;
+  float b[3];
+  const int in0 = fn_inside(fn_barycentric(tri2vtx, xyz, pc0, itri1, m, b), b);
+  const int in1 = fn_inside(fn_barycentric(tri2vtx, xyz, pc1, itri0, m, b), b);
+  if (!in0 && !in1) return 1; /* shared edge */
+  if (in0 && !in1) return 2;  /* tri0 in front */
+  if (!in0 && in1) return 3;  /* tri1 in front */
+  return 4;                   /* intersection */
+}
+
+static float edge_dldr(uint32_t nv, const float *pix2val, const float *dpix, size_t ip0,
+                       size_t ip1) {
+  float r = 0.f;
+  for (uint32_t k = 0; k < nv; ++k) {
+    const float v0 = pix2val[ip0 * nv + k], v1 = pix2val[ip1 * nv + k];
+    const float d0 = dpix[ip0 * nv + k], d1 = dpix[ip1 * nv + k];
+    r += (d0 + d1) * 0.5f * (v0 - v1);
+  }
+  return r;
+}
+
+/* rasterized_edge_gradient.rs:77-200: hedge (ih0, iw) between pixels
+ * (iw, ih0) and (iw, ih0+1), [H-1][W]; vedge (ih, iw0) between (iw0, ih) and
+ * (iw0+1, ih), [H][W-1]. */
+void orc_edge_gradient_and_type(const uint32_t *tri2vtx, const float *xyz, const float m[16],
+                                uint32_t img_w, uint32_t img_h, const uint32_t *pix2tri,
+                                uint32_t nv, const float *pix2val, const float *dpix,
+                                uint8_t *hedge2type, float *hedge2dldr, uint8_t *vedge2type,
+                                float *vedge2dldr) {
+  for (uint32_t iw = 0; iw < img_w; ++iw)
+    for (uint32_t ih0 = 0; ih0 + 1 < img_h; ++ih0) {
+      const size_t ip0 = (size_t)ih0 * img_w + iw, ip1 = ip0 + img_w;
+      const float pc0[2] = {(float)iw + 0.5f, (float)ih0 + 0.5f};
+      const float pc1[2] = {(float)iw + 0.5f, (float)(ih0 + 1) + 0.5f};
+      const size_t e = (size_t)ih0 * img_w + iw;
+      hedge2type[e] = edge_type(tri2vtx, xyz, m, pix2tri[ip0], pix2tri[ip1], pc0, pc1);
+      hedge2dldr[e] = edge_dldr(nv, pix2val, dpix, ip0, ip1);
+    }
+  for (uint32_t iw0 = 0; iw0 + 1 < img_w; ++iw0)
+    for (uint32_t ih = 0; ih < img_h; ++ih) {
+      const size_t ip0 = (size_t)ih * img_w + iw0, ip1 = ip0 + 1;
+      const float pc0[2] = {(float)iw0 + 0.5f, (float)ih + 0.5f};
+      const float pc1[2] = {(float)(iw0 + 1) + 0.5f, (float)ih + 0.5f};
+      const size_t e = (size_t)ih * (img_w - 1) + iw0;
+      vedge2type[e] = edge_type(tri2vtx, xyz, m, pix2tri[ip0], pix2tri[ip1], pc0, pc1);
+      vedge2dldr[e] = edge_dldr(nv, pix2val, dpix, ip0, ip1);
+    }
+}
+
+static int fixed_type(uint8_t t) { return t == 2 || t == 3; }
+
+/* one hedge cell update; returns 1 and the new value when it has neighbours */
+static int hedge_avg(uint32_t W, uint32_t H, const uint8_t *ht, const float *hv, const uint8_t *vt,
+                     uint32_t ih, uint32_t iw, float *out) {
+  float s = 0.f;
+  uint32_t n = 0;
+  if (ih > 0 && ht[(size_t)(ih - 1) * W + iw] != 2) { s += hv[(size_t)(ih - 1) * W + iw]; ++n; }
+  if (ih + 1 < H - 1 && ht[(size_t)(ih + 1) * W + iw] != 3) { s += hv[(size_t)(ih + 1) * W + iw]; ++n; }
+  if (iw > 0 && !fixed_type(vt[(size_t)ih * (W - 1) + iw - 1]) &&
+      !fixed_type(vt[(size_t)(ih + 1) * (W - 1) + iw - 1])) {
+    s += hv[(size_t)ih * W + iw - 1];
+    ++n;
+  }
+  if (iw + 1 < W && !fixed_type(vt[(size_t)ih * (W - 1) + iw]) &&
+      !fixed_type(vt[(size_t)(ih + 1) * (W - 1) + iw])) {
+    s += hv[(size_t)ih * W + iw + 1];
+    ++n;
+  }
+  if (!n) return 0;
+  *out = s / (float)n;
+  return 1;
+}
+
+static int vedge_avg(uint32_t W, uint32_t H, const uint8_t *vt, const float *vv, const uint8_t *ht,
+                     uint32_t ih, uint32_t iw, float *out) {
+  const uint32_t w = W - 1;
+  float s = 0.f;
+  uint32_t n = 0;
+  if (iw > 0 && vt[(size_t)ih * w + iw - 1] != 2) { s += vv[(size_t)ih * w + iw - 1]; ++n; }
+  if (iw + 1 < w && vt[(size_t)ih * w + iw + 1] != 3) { s += vv[(size_t)ih * w + iw + 1]; ++n; }
+  if (ih > 0 && !fixed_type(ht[(size_t)(ih - 1) * W + iw]) &&
+      !fixed_type(ht[(size_t)(ih - 1) * W + iw + 1])) {
+    s += vv[(size_t)(ih - 1) * w + iw];
+    ++n;
+  }
+  if (ih + 1 < H && !fixed_type(ht[(size_t)ih * W + iw]) && !fixed_type(ht[(size_t)ih * W + iw + 1])) {
+    s += vv[(size_t)(ih + 1) * w + iw];
+    ++n;
+  }
+  if (!n) return 0;
+  *out = s / (float)n;
+  return 1;
+}
+
+/* rasterized_edge_gradient.rs:250-432: in-place sweeps in index order
+ * (Gauss-Seidel), hedges then vedges, cells of type 2 / 3 held fixed */
+void orc_smooth_gradient_serial(uint32_t W, uint32_t H, const uint8_t *ht, const uint8_t *vt,
+                                uint32_t num_iter, float *hv, float *vv) {
+  for (uint32_t it = 0; it < num_iter; ++it)
+    for (uint32_t ih = 0; ih + 1 < H; ++ih)
+      for (uint32_t iw = 0; iw < W; ++iw) {
+        if (fixed_type(ht[(size_t)ih * W + iw])) continue;
+        float v;
+        if (hedge_avg(W, H, ht, hv, vt, ih, iw, &v)) hv[(size_t)ih * W + iw] = v;
+      }
+  for (uint32_t it = 0; it < num_iter; ++it)
+    for (uint32_t ih = 0; ih < H; ++ih)
+      for (uint32_t iw = 0; iw + 1 < W; ++iw) {
+        if (fixed_type(vt[(size_t)ih * (W - 1) + iw])) continue;
+        float v;
+        if (vedge_avg(W, H, vt, vv, ht, ih, iw, &v)) vv[(size_t)ih * (W - 1) + iw] = v;
+      }
+}
+
+/* the GPU path of the reference (rasterized_edge_gradient.cu:139-275 as
+ * launched at del-msh-dlpack/src/rasterized_edge_gradient.rs:336-380): per
+ * iteration the cells with (iw + ih) even, then odd, each from the values
+ * before that half-step (a cell's neighbours have the other colour) */
+void orc_smooth_gradient_red_black(uint32_t W, uint32_t H, const uint8_t *ht, const uint8_t *vt,
+                                   uint32_t num_iter, float *hv, float *vv) {
+  for (uint32_t it = 0; it < num_iter; ++it)
+    for (uint32_t color = 0; color < 2; ++color)
+      for (uint32_t ih = 0; ih + 1 < H; ++ih)
+        for (uint32_t iw = 0; iw < W; ++iw) {
+          if (((iw + ih) & 1u) != color || fixed_type(ht[(size_t)ih * W + iw])) continue;
+          float v;
+          if (hedge_avg(W, H, ht, hv, vt, ih, iw, &v)) hv[(size_t)ih * W + iw] = v;
+        }
+  for (uint32_t it = 0; it < num_iter; ++it)
+    for (uint32_t color = 0; color < 2; ++color)
+      for (uint32_t ih = 0; ih < H; ++ih)
+        for (uint32_t iw = 0; iw + 1 < W; ++iw) {
+          if (((iw + ih) & 1u) != color || fixed_type(vt[(size_t)ih * (W - 1) + iw])) continue;
+          float v;
+          if (vedge_avg(W, H, vt, vv, ht, ih, iw, &v)) vv[(size_t)ih * (W - 1) + iw] = v;
+        }
+}
+
+static inline int clampi(int x, int lo, int hi) { return x < lo ? lo : (x > hi ? hi : x); }
+/* Rust's `f32 as i32`: NaN -> 0, saturating */
+static inline int f2i_sat(float x) {
+  if (x != x) return 0;
+  if (x >= 2147483647.f) return 2147483647;
+  if (x <= -2147483648.f) return (-2147483647 - 1);
+  return (int)x;
+}
+static inline float clampf01(float x) { return x < 0.f ? 0.f : (x > 1.f ? 1.f : x); }
+
+/* rasterized_edge_gradient.rs:202-248: x-velocity bilinear from vedges at
+ * (iw0 + 1, ih + 0.5), y-velocity from hedges at (iw + 0.5, ih0 + 1) */
+void orc_interpolate_staggered_grid(uint32_t W, uint32_t H, const float *hedge2vy,
+                                    const float *vedge2vx, size_t num_vtx, const float *vtx2xy,
+                                    float *vtx2velo) {
+  for (size_t i = 0; i < num_vtx; ++i) {
+    const float px = vtx2xy[i * 2], py = vtx2xy[i * 2 + 1];
+    {
+      const float gx = px - 1.0f, gy = py - 0.5f;
+      const int ix0 = clampi(f2i_sat(floorf(gx)), 0, (int)W - 2);
+      const int iy0 = clampi(f2i_sat(floorf(gy)), 0, (int)H - 2);
+      const int ix1 = ix0 + 1 < (int)W - 2 ? ix0 + 1 : (int)W - 2;
+      const int iy1 = iy0 + 1 < (int)H - 1 ? iy0 + 1 : (int)H - 1;
+      const float tx = clampf01(gx - (float)ix0), ty = clampf01(gy - (float)iy0);
+      const int w = (int)W - 1;
+      vtx2velo[i * 2] = (((1.f - tx) * (1.f - ty) * vedge2vx[iy0 * w + ix0] +
+                          tx * (1.f - ty) * vedge2vx[iy0 * w + ix1]) +
+                         (1.f - tx) * ty * vedge2vx[iy1 * w + ix0]) +
+                        tx * ty * vedge2vx[iy1 * w + ix1];
+    }
+    {
+      const float gx = px - 0.5f, gy = py - 1.0f;
+      const int ix0 = clampi(f2i_sat(floorf(gx)), 0, (int)W - 2);
+      const int iy0 = clampi(f2i_sat(floorf(gy)), 0, (int)H - 2);
+      const int ix1 = ix0 + 1 < (int)W - 1 ? ix0 + 1 : (int)W - 1;
+      const int iy1 = iy0 + 1 < (int)H - 2 ? iy0 + 1 : (int)H - 2;
+      const float tx = clampf01(gx - (float)ix0), ty = clampf01(gy - (float)iy0);
+      vtx2velo[i * 2 + 1] = (((1.f - tx) * (1.f - ty) * hedge2vy[iy0 * W + ix0] +
+                              tx * (1.f - ty) * hedge2vy[iy0 * W + ix1]) +
+                             (1.f - tx) * ty * hedge2vy[iy1 * W + ix0]) +
+                            tx * ty * hedge2vy[iy1 * W + ix1];
+    }
+  }
+}

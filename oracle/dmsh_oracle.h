@@ -183,6 +183,28 @@ int orc_interpolate_bwd(float *dldw_vtx2xyz, float *dldw_vtx2val, const uint32_t
                         const float *vtx2val, const float ndc2world[16],
                         const float *dldw_pix2val);
 
+/* ---- smoothed edge gradient (staggered pixel-edge grid) ----------------- */
+/* rasterized_edge_gradient.rs:77-200; types 0 same triangle, 1 shared edge,
+ * 2 / 3 the first / second pixel's triangle in front, 4 intersection */
+void orc_edge_gradient_and_type(const uint32_t *tri2vtx, const float *vtx2xyz,
+                                const float world2pix[16], uint32_t img_w, uint32_t img_h,
+                                const uint32_t *pix2tri, uint32_t num_vdim,
+                                const float *pix2val, const float *dldw_pix2val,
+                                uint8_t *hedge2type, float *hedge2dldr, uint8_t *vedge2type,
+                                float *vedge2dldr);
+/* rasterized_edge_gradient.rs:250-432 (serial, del-msh-cpu) */
+void orc_smooth_gradient_serial(uint32_t img_w, uint32_t img_h, const uint8_t *hedge2type,
+                                const uint8_t *vedge2type, uint32_t num_iter, float *hedge2dldr,
+                                float *vedge2dldr);
+/* rasterized_edge_gradient.cu:139-275 (red-black, the reference GPU path) */
+void orc_smooth_gradient_red_black(uint32_t img_w, uint32_t img_h, const uint8_t *hedge2type,
+                                   const uint8_t *vedge2type, uint32_t num_iter,
+                                   float *hedge2dldr, float *vedge2dldr);
+/* rasterized_edge_gradient.rs:202-248 (img_w, img_h >= 2) */
+void orc_interpolate_staggered_grid(uint32_t img_w, uint32_t img_h, const float *hedge2vy,
+                                    const float *vedge2vx, size_t num_vtx, const float *vtx2xy,
+                                    float *vtx2velo);
+
 /* ---- mesh topology ------------------------------------------------------- */
 /* vtx2elem.rs:58-96: elements around each vertex (offsets vtx2idx[num_vtx+1],
  * idx2elem[num_elem*num_node], elements ascending per vertex).  Returns 0, or

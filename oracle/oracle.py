@@ -502,3 +502,70 @@ def interpolate_bwd(pix2tri_, tri2vtx, vtx2xyz, vtx2val, ndc2world_colmajor, dld
                              _f32(ndc2world_colmajor), _f32(dldw_pix2val).reshape(-1)):
         raise ValueError("index out of range")
     return gx, gv
+
+
+# ---------------------------------------------------------------------------
+# smoothed edge gradient
+# ---------------------------------------------------------------------------
+_u8p = np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS")
+
+
+def _sig_smooth():
+    L = lib()
+    if getattr(L, "_smooth_ready", False):
+        return L
+    L.orc_edge_gradient_and_type.restype = None
+    L.orc_edge_gradient_and_type.argtypes = [_u32p, _f32p, _f32p, ctypes.c_uint32, ctypes.c_uint32,
+                                             _u32p, ctypes.c_uint32, _f32p, _f32p, _u8p, _f32p,
+                                             _u8p, _f32p]
+    for name in ("orc_smooth_gradient_serial", "orc_smooth_gradient_red_black"):
+        fn = getattr(L, name)
+        fn.restype = None
+        fn.argtypes = [ctypes.c_uint32, ctypes.c_uint32, _u8p, _u8p, ctypes.c_uint32, _f32p, _f32p]
+    L.orc_interpolate_staggered_grid.restype = None
+    L.orc_interpolate_staggered_grid.argtypes = [ctypes.c_uint32, ctypes.c_uint32, _f32p, _f32p, _sz,
+                                                 _f32p, _f32p]
+    L._smooth_ready = True
+    return L
+
+
+def edge_gradient_and_type(tri2vtx, vtx2xyz, world2pix_colmajor, pix2tri_, pix2val, dldw_pix2val):
+    """-> hedge2type [H-1,W] u8, hedge2dldr [H-1,W], vedge2type [H,W-1] u8, vedge2dldr [H,W-1]."""
+    L = _sig_smooth()
+    h, w = pix2tri_.shape
+    pv = _f32(pix2val)
+    nv = pv.shape[2]
+    ht = np.zeros((h - 1, w), np.uint8)
+    hd = np.zeros((h - 1, w), np.float32)
+    vt = np.zeros((h, w - 1), np.uint8)
+    vd = np.zeros((h, w - 1), np.float32)
+    L.orc_edge_gradient_and_type(_u32(tri2vtx).reshape(-1), _f32(vtx2xyz).reshape(-1),
+                                 _f32(world2pix_colmajor), w, h, _u32(pix2tri_).reshape(-1), nv,
+                                 pv.reshape(-1), _f32(dldw_pix2val).reshape(-1), ht.reshape(-1),
+                                 hd.reshape(-1), vt.reshape(-1), vd.reshape(-1))
+    return ht, hd, vt, vd
+
+
+def smooth_gradient(hedge2type, vedge2type, num_iter, hedge2dldr, vedge2dldr, red_black=True):
+    """Returns smoothed copies (red-black as the reference GPU path, or the
+    serial sweeps of del-msh-cpu)."""
+    L = _sig_smooth()
+    h = hedge2type.shape[0] + 1
+    w = hedge2type.shape[1]
+    hd = _f32(hedge2dldr).copy()
+    vd = _f32(vedge2dldr).copy()
+    fn = L.orc_smooth_gradient_red_black if red_black else L.orc_smooth_gradient_serial
+    fn(w, h, np.ascontiguousarray(hedge2type, np.uint8).reshape(-1),
+       np.ascontiguousarray(vedge2type, np.uint8).reshape(-1), num_iter, hd.reshape(-1), vd.reshape(-1))
+    return hd, vd
+
+
+def interpolate_staggered_grid(hedge2vy, vedge2vx, vtx2xy):
+    L = _sig_smooth()
+    h = hedge2vy.shape[0] + 1
+    w = hedge2vy.shape[1]
+    xy = _f32(vtx2xy)
+    out = np.zeros_like(xy)
+    L.orc_interpolate_staggered_grid(w, h, _f32(hedge2vy).reshape(-1), _f32(vedge2vx).reshape(-1),
+                                     xy.shape[0], xy.reshape(-1), out.reshape(-1))
+    return out
