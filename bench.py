@@ -342,17 +342,27 @@ def run_diff(reps: int):
     occ3 = occ[..., None].contiguous()
     dl3 = dldw[..., None].contiguous()
     eg_ms = timed(lambda: REG.bwd(t, v0, w2p, pix, occ3, dl3), max(reps, 5))
+
+    def smooth_bwd():
+        v = v0.clone().requires_grad_(True)
+        out = REG.AutogradWithSmooth.apply(t, v, w2p, pix, occ3)
+        (out * dl3).sum().backward()
+
+    smooth_bwd()
+    smooth_ms = timed(smooth_bwd, max(reps, 5))
     _lib.timing_enable(True)
     for _ in range(3):
         REG.bwd(t, v0, w2p, pix, occ3, dl3)
         DA.bwd_antialias(cedge, v0, w2p, occ, dldw, pix)
+    smooth_bwd()
     torch.cuda.synchronize()
     _lib.timing_enable(False)
     kern = {}
-    for name in ("k_edge_grad_bwd", "k_aa_bwd"):
+    for name in ("k_edge_grad_bwd", "k_aa_bwd", "k_edge_type_grad", "k_smooth_links",
+                 "k_smooth_tiled", "k_interpolate_staggered"):
         n, total, _ = _lib.timing_query(name)
         if n:
-            kern[name] = round(total / n, 5)
+            kern[name] = {"avg_ms": round(total / n, 5), "launches": n}
     ph = pix.cpu().numpy()
     active = int((ph[:, 1:] != ph[:, :-1]).sum() + (ph[1:, :] != ph[:-1, :]).sum())
     # edge-gradient basis: 12 B per pixel, 12 B per vertex, 84 B per active pair
@@ -364,11 +374,13 @@ def run_diff(reps: int):
         "backward_ms": round(float(np.median(b_ms)), 4),
         "contour_edges": int(cedge.shape[0]),
         "edge_grad_bwd_ms": round(eg_ms, 4),
+        "smoothed_edge_grad_bwd_ms": round(smooth_ms, 4),
         "kernels_ms": kern,
         "edge_grad_active_pairs": active,
     }
     if "k_edge_grad_bwd" in kern:
-        single["edge_grad_bwd_algorithmic_gbs"] = round(eg_bytes / (kern["k_edge_grad_bwd"] * 1e-3) / 1e9, 1)
+        single["edge_grad_bwd_algorithmic_gbs"] = round(
+            eg_bytes / (kern["k_edge_grad_bwd"]["avg_ms"] * 1e-3) / 1e9, 1)
 
     b = primitives.batch_meshes(64, seed=6, tri_range=(5000, 200000), pix_range=(W * W, W * W))
     sb = Batched.SilhouetteBatch(b["mesh2tri"], b["mesh2vtx"], torch.from_numpy(b["tri2vtx"]).to(dev),

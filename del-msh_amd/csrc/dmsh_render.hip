@@ -283,20 +283,77 @@ int dmsh_edge_gradient_and_type(void* stream, const uint32_t* tri2vtx, uint32_t 
   return launch_status();
 }
 
+namespace {
+struct SmoothGrids {
+  uint32_t tiles_h, tiles_v, nh, nv;
+};
+inline SmoothGrids smooth_grids(uint32_t W, uint32_t H) {
+  SmoothGrids g;
+  g.nh = (H - 1) * W;
+  g.nv = H * (W - 1);
+  g.tiles_h = ceil_div(H - 1, eg::kSmoothTR) * ceil_div(W, eg::kSmoothTC);
+  g.tiles_v = ceil_div(H, eg::kSmoothTR) * ceil_div(W - 1, eg::kSmoothTC);
+  return g;
+}
+}  // namespace
+
+size_t dmsh_smooth_gradient_workspace_bytes(uint32_t img_w, uint32_t img_h) {
+  if (img_w < 2 || img_h < 2) return 256;
+  const SmoothGrids g = smooth_grids(img_w, img_h);
+  Arena a{nullptr, ~(size_t)0, 0};
+  a.take<float>(g.nh);
+  a.take<float>(g.nv);
+  a.take<uint8_t>(g.nh);
+  a.take<uint8_t>(g.nv);
+  return a.off + 256;
+}
+
 int dmsh_smooth_gradient(void* stream, uint32_t img_w, uint32_t img_h, const uint8_t* hedge2type,
                          const uint8_t* vedge2type, uint32_t num_iter, float* hedge2dldr,
-                         float* vedge2dldr) {
+                         float* vedge2dldr, void* ws, size_t ws_bytes) {
   if (img_w < 2 || img_h < 2 || (uint64_t)img_w * img_h >= (1ull << 31)) return DMSH_ERR_INVALID;
   if (!hedge2type || !vedge2type || !hedge2dldr || !vedge2dldr) return DMSH_ERR_INVALID;
-  const uint32_t cells = (img_h - 1) * img_w + img_h * (img_w - 1);
+  if (num_iter == 0) return DMSH_OK;
+  if (!ws || ws_bytes < dmsh_smooth_gradient_workspace_bytes(img_w, img_h))
+    return DMSH_ERR_WORKSPACE;
+  const SmoothGrids g = smooth_grids(img_w, img_h);
+  Arena a{static_cast<char*>(ws), ws_bytes, 0};
+  float* h2 = a.take<float>(g.nh);
+  float* v2 = a.take<float>(g.nv);
+  uint8_t* hm = a.take<uint8_t>(g.nh);
+  uint8_t* vm = a.take<uint8_t>(g.nv);
+  if (!a.ok()) return DMSH_ERR_WORKSPACE;
   hipStream_t s = as_stream(stream);
-  for (uint32_t it = 0; it < num_iter; ++it) {
-    for (uint32_t color = 0; color < 2; ++color) {
-      launch("k_smooth_red_black", eg::k_smooth_red_black, dim3(ceil_div(cells, 256)), dim3(256),
-             0, s, img_w, img_h, hedge2type, hedge2dldr, vedge2type, vedge2dldr, color);
-      int rc = launch_status();
-      if (rc != DMSH_OK) return rc;
-    }
+  int rc;
+  launch("k_smooth_links", eg::k_smooth_links,
+         dim3(std::min<uint32_t>(ceil_div(g.nh + g.nv, 256), 4096)), dim3(256), 0, s, img_w,
+         img_h, hedge2type, vedge2type, hm, vm);
+  if ((rc = launch_status()) != DMSH_OK) return rc;
+  const uint32_t steps = 2u * num_iter;
+  const uint32_t launches = ceil_div(steps, (uint32_t)eg::kSmoothK);
+  // launches alternate buffers and must end in the caller's arrays
+  float *hs = hedge2dldr, *vs = vedge2dldr, *hd = h2, *vd = v2;
+  if (launches % 2 == 1) {
+    launch("k_copy_f32", eg::k_copy_f32, dim3(std::min<uint32_t>(ceil_div(g.nh, 256), 2048)),
+           dim3(256), 0, s, hedge2dldr, h2, g.nh);
+    if ((rc = launch_status()) != DMSH_OK) return rc;
+    launch("k_copy_f32", eg::k_copy_f32, dim3(std::min<uint32_t>(ceil_div(g.nv, 256), 2048)),
+           dim3(256), 0, s, vedge2dldr, v2, g.nv);
+    if ((rc = launch_status()) != DMSH_OK) return rc;
+    hs = h2;
+    vs = v2;
+    hd = hedge2dldr;
+    vd = vedge2dldr;
+  }
+  for (uint32_t step0 = 0; step0 < steps; step0 += eg::kSmoothK) {
+    const uint32_t n = std::min<uint32_t>(eg::kSmoothK, steps - step0);
+    launch("k_smooth_tiled",
+           (eg::k_smooth_tiled<eg::kSmoothK, eg::kSmoothTR, eg::kSmoothTC, eg::kSmoothThreads>),
+           dim3(g.tiles_h + g.tiles_v), dim3(eg::kSmoothThreads), 0, s, img_w, img_h, hm, vm, hs,
+           hd, vs, vd, g.tiles_h, step0, n);
+    if ((rc = launch_status()) != DMSH_OK) return rc;
+    std::swap(hs, hd);
+    std::swap(vs, vd);
   }
   return DMSH_OK;
 }

@@ -219,55 +219,167 @@ static __global__ __launch_bounds__(256) void k_edge_type_grad(
 
 __device__ __forceinline__ bool fixed_type(uint8_t t) { return t == 2 || t == 3; }
 
-// One red-black half-step over both grids (they do not read each other's
-// values, only types): cells with (iw + ih) & 1 == color and a free type take
-// the average of their admitted neighbours, summed north, south, west, east
-// (hedges) or west, east, north, south (vedges), as
-// rasterized_edge_gradient.cu:139-275.  Neighbours have the other colour, so
-// no cell reads a value written in the same launch.
-static __global__ __launch_bounds__(256) void k_smooth_red_black(
-    uint32_t W, uint32_t H, const uint8_t* __restrict__ ht, float* __restrict__ hv,
-    const uint8_t* __restrict__ vt, float* __restrict__ vv, uint32_t color) {
-  const uint32_t nh = (H - 1) * W, nvx = H * (W - 1);
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < nh) {
-    const uint32_t iw = i % W, ih = i / W;
-    if (((iw + ih) & 1u) != color || fixed_type(ht[i])) return;
-    float s = 0.f;
-    uint32_t n = 0;
-    if (ih > 0 && ht[i - W] != 2) { s += hv[i - W]; ++n; }
-    if (ih + 1 < H - 1 && ht[i + W] != 3) { s += hv[i + W]; ++n; }
-    if (iw > 0 && !fixed_type(vt[ih * (W - 1) + iw - 1]) &&
-        !fixed_type(vt[(ih + 1) * (W - 1) + iw - 1])) {
-      s += hv[i - 1];
-      ++n;
+// Red-black smoothing of both grids (rasterized_edge_gradient.cu:139-275):
+// per iteration the cells with (iw + ih) even, then odd, each taking the
+// average of its admitted neighbours, summed north, south, west, east
+// (hedges) or west, east, north, south (vedges); cells of type 2 / 3 stay
+// fixed.  The two grids do not read each other's values, only types.
+// It runs K half-steps per launch: a workgroup loads a tile of one grid with
+// a halo of K cells into LDS, together with each cell's links (which
+// neighbours enter its average, in summation order; worked out from the
+// types once per call by k_smooth_links), runs the half-steps in LDS and
+// writes the tile back to the other buffer.  After s half-steps every cell at
+// least s cells inside the loaded region holds exactly the value the global
+// iteration gives, so the tile itself (K cells inside) is exact after up to
+// K half-steps.  Launches alternate between two buffers because tiles read
+// each other's cells as halo.
+constexpr int kSmoothK = 16;
+constexpr int kSmoothTR = 32;
+constexpr int kSmoothTC = 64;
+constexpr int kSmoothThreads = 1024;
+constexpr uint8_t kLinkFixed = 0x10;
+
+__device__ __forceinline__ int clampi(int x, int lo, int hi) {
+  return x < lo ? lo : (x > hi ? hi : x);
+}
+
+// link bits of hedge (r, c): bit0 north, bit1 south, bit2 west, bit3 east
+__device__ __forceinline__ uint8_t hedge_links(uint32_t W, uint32_t H, const uint8_t* ht,
+                                               const uint8_t* vt, int r, int c) {
+  const int R = (int)H - 1, C = (int)W;
+  if (fixed_type(ht[r * C + c])) return kLinkFixed;
+  uint8_t m = 0;
+  const int vw = (int)W - 1;
+  if (r > 0 && ht[(r - 1) * C + c] != 2) m |= 1;
+  if (r + 1 < R && ht[(r + 1) * C + c] != 3) m |= 2;
+  if (c > 0 && !fixed_type(vt[r * vw + c - 1]) && !fixed_type(vt[(r + 1) * vw + c - 1])) m |= 4;
+  if (c + 1 < C && !fixed_type(vt[r * vw + c]) && !fixed_type(vt[(r + 1) * vw + c])) m |= 8;
+  return m;
+}
+
+// link bits of vedge (r, c): bit0 west, bit1 east, bit2 north, bit3 south
+__device__ __forceinline__ uint8_t vedge_links(uint32_t W, uint32_t H, const uint8_t* ht,
+                                               const uint8_t* vt, int r, int c) {
+  const int R = (int)H, C = (int)W - 1;
+  if (fixed_type(vt[r * C + c])) return kLinkFixed;
+  uint8_t m = 0;
+  if (c > 0 && vt[r * C + c - 1] != 2) m |= 1;
+  if (c + 1 < C && vt[r * C + c + 1] != 3) m |= 2;
+  if (r > 0 && !fixed_type(ht[(r - 1) * (int)W + c]) && !fixed_type(ht[(r - 1) * (int)W + c + 1]))
+    m |= 4;
+  if (r + 1 < R && !fixed_type(ht[r * (int)W + c]) && !fixed_type(ht[r * (int)W + c + 1])) m |= 8;
+  return m;
+}
+
+// links of every hedge (first nh cells) and vedge (next nv cells)
+static __global__ __launch_bounds__(256) void k_smooth_links(
+    uint32_t W, uint32_t H, const uint8_t* __restrict__ ht, const uint8_t* __restrict__ vt,
+    uint8_t* __restrict__ hm, uint8_t* __restrict__ vm) {
+  const uint32_t nh = (H - 1) * W, n = nh + H * (W - 1);
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    if (i < nh) {
+      hm[i] = hedge_links(W, H, ht, vt, (int)(i / W), (int)(i % W));
+    } else {
+      const uint32_t j = i - nh;
+      vm[j] = vedge_links(W, H, ht, vt, (int)(j / (W - 1)), (int)(j % (W - 1)));
     }
-    if (iw + 1 < W && !fixed_type(vt[ih * (W - 1) + iw]) &&
-        !fixed_type(vt[(ih + 1) * (W - 1) + iw])) {
-      s += hv[i + 1];
-      ++n;
+  }
+}
+
+// blockIdx.x < tiles_h: a tile of the hedge grid, else of the vedge grid.
+// src / dst: the two buffers of each grid (cells outside the grid load as 0);
+// hm / vm: the links.  Every thread owns the same NPT cells of each colour in
+// every half-step; their LDS index and links stay in registers (links 0:
+// the cell keeps its value, as fixed cells, cells outside the grid and the
+// border ring of the tile do).  All reads of a half-step are issued before
+// its writes: a cell's neighbours have the other colour, so no cell read in
+// a half-step is written in it.  The LDS array has a pad of CC + 1 cells on
+// either side so that the border ring's neighbour reads stay inside it.  The
+// four neighbour values are read unconditionally and a disabled link adds
+// +0, which leaves the running sum (it starts at +0 and can never be -0)
+// exactly as skipping the term would.
+template <int K, int TR, int TC, int T>
+__global__ __launch_bounds__(T) void k_smooth_tiled(
+    uint32_t W, uint32_t H, const uint8_t* __restrict__ hm, const uint8_t* __restrict__ vm,
+    const float* __restrict__ hsrc, float* __restrict__ hdst, const float* __restrict__ vsrc,
+    float* __restrict__ vdst, uint32_t tiles_h, uint32_t step0, uint32_t nsteps) {
+  constexpr int RR = TR + 2 * K, CC = TC + 2 * K, HALF = RR * (CC / 2);
+  constexpr int NPT = HALF / T, NL = RR * CC / T, PAD = CC + 1;
+  static_assert(CC % 2 == 0 && HALF % T == 0, "colour pairs split evenly over the threads");
+  static_assert(RR * CC <= 65536, "LDS index packed in 16 bits");
+  __shared__ float sv[RR * CC + 2 * PAD];
+  const bool hedge = blockIdx.x < tiles_h;
+  const int R = hedge ? (int)H - 1 : (int)H;
+  const int C = hedge ? (int)W : (int)W - 1;
+  const int tcols = (C + TC - 1) / TC;
+  const int t = hedge ? (int)blockIdx.x : (int)(blockIdx.x - tiles_h);
+  const int r0 = (t / tcols) * TR - K, c0 = (t % tcols) * TC - K;
+  const float* src = hedge ? hsrc : vsrc;
+  float* dst = hedge ? hdst : vdst;
+  const uint8_t* msk = hedge ? hm : vm;
+  // LDS offsets of the four links in summation order: hedges N, S, W, E;
+  // vedges W, E, N, S
+  const int o0 = hedge ? -CC : -1, o1 = hedge ? CC : 1, o2 = hedge ? -1 : -CC,
+            o3 = hedge ? 1 : CC;
+  float v[NL];
+#pragma unroll
+  for (int k = 0; k < NL; ++k) {
+    const int i = (int)threadIdx.x + k * T;
+    const int r = r0 + i / CC, c = c0 + i % CC;
+    const float x = src[clampi(r, 0, R - 1) * C + clampi(c, 0, C - 1)];
+    v[k] = (r >= 0 && c >= 0 && r < R && c < C) ? x : 0.f;
+  }
+  uint32_t code[2][NPT];
+#pragma unroll
+  for (int color = 0; color < 2; ++color) {
+#pragma unroll
+    for (int k = 0; k < NPT; ++k) {
+      const int j = (int)threadIdx.x + k * T;
+      const int lr = j / (CC / 2);
+      const int lc = 2 * (j % (CC / 2)) + ((color + r0 + c0 + lr) & 1);
+      const int r = r0 + lr, c = c0 + lc;
+      const uint32_t m = msk[clampi(r, 0, R - 1) * C + clampi(c, 0, C - 1)];
+      const bool live = lr >= 1 && lc >= 1 && lr < RR - 1 && lc < CC - 1 && r >= 0 && c >= 0 &&
+                        r < R && c < C && !(m & kLinkFixed);
+      code[color][k] = (uint32_t)(lr * CC + lc) | ((live ? m & 15u : 0u) << 16);
     }
-    if (n) hv[i] = s / (float)n;
-    return;
   }
-  const uint32_t j = i - nh;
-  if (j >= nvx) return;
-  const uint32_t w = W - 1;
-  const uint32_t iw = j % w, ih = j / w;
-  if (((iw + ih) & 1u) != color || fixed_type(vt[j])) return;
-  float s = 0.f;
-  uint32_t n = 0;
-  if (iw > 0 && vt[j - 1] != 2) { s += vv[j - 1]; ++n; }
-  if (iw + 1 < w && vt[j + 1] != 3) { s += vv[j + 1]; ++n; }
-  if (ih > 0 && !fixed_type(ht[(ih - 1) * W + iw]) && !fixed_type(ht[(ih - 1) * W + iw + 1])) {
-    s += vv[j - w];
-    ++n;
+#pragma unroll
+  for (int k = 0; k < NL; ++k) sv[PAD + (int)threadIdx.x + k * T] = v[k];
+  __syncthreads();
+  for (uint32_t s = 0; s < nsteps; ++s) {
+    const bool odd = (step0 + s) & 1u;
+    float a[NPT][4];
+#pragma unroll
+    for (int k = 0; k < NPT; ++k) {
+      const int i = PAD + (int)((odd ? code[1][k] : code[0][k]) & 0xffffu);
+      a[k][0] = sv[i + o0];
+      a[k][1] = sv[i + o1];
+      a[k][2] = sv[i + o2];
+      a[k][3] = sv[i + o3];
+    }
+#pragma unroll
+    for (int k = 0; k < NPT; ++k) {
+      const uint32_t cd = odd ? code[1][k] : code[0][k];
+      const uint32_t m = cd >> 16;
+      const float acc = (((0.f + ((m & 1u) ? a[k][0] : 0.f)) + ((m & 2u) ? a[k][1] : 0.f)) +
+                         ((m & 4u) ? a[k][2] : 0.f)) +
+                        ((m & 8u) ? a[k][3] : 0.f);
+      if (m) sv[PAD + (int)(cd & 0xffffu)] = acc / (float)__builtin_popcount(m);
+    }
+    __syncthreads();
   }
-  if (ih + 1 < H && !fixed_type(ht[ih * W + iw]) && !fixed_type(ht[ih * W + iw + 1])) {
-    s += vv[j + w];
-    ++n;
+  for (int i = threadIdx.x; i < TR * TC; i += T) {
+    const int lr = K + i / TC, lc = K + i % TC;
+    const int r = r0 + lr, c = c0 + lc;
+    if (r < R && c < C) dst[r * C + c] = sv[PAD + lr * CC + lc];
   }
-  if (n) vv[j] = s / (float)n;
+}
+
+static __global__ __launch_bounds__(256) void k_copy_f32(const float* __restrict__ a,
+                                                        float* __restrict__ b, uint32_t n) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    b[i] = a[i];
 }
 
 // Rust's `f32 as i32`: NaN -> 0, saturating
@@ -278,9 +390,6 @@ __device__ __forceinline__ int f2i_sat(float x) {
   return (int)x;
 }
 
-__device__ __forceinline__ int clampi(int x, int lo, int hi) {
-  return x < lo ? lo : (x > hi ? hi : x);
-}
 __device__ __forceinline__ float clamp01(float x) { return x < 0.f ? 0.f : (x > 1.f ? 1.f : x); }
 
 // One thread per vertex: x-velocity bilinear from the vedges (sampled at

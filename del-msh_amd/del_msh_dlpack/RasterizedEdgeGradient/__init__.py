@@ -49,8 +49,10 @@ def smooth_gradient(hedge2type, hedge2dldr, vedge2type, vedge2dldr, num_iter, st
     ht, hd, vt, vd = args
     img_h, img_w = ht.shape[0] + 1, ht.shape[1]
     stream = stream_ptr or _lib.stream_of(dev)
-    _lib.check(_lib.lib().dmsh_smooth_gradient(stream, img_w, img_h, _lib.ptr(ht), _lib.ptr(vt),
-                                               int(num_iter), _lib.ptr(hd), _lib.ptr(vd)),
+    L = _lib.lib()
+    ws = _lib.workspace(L.dmsh_smooth_gradient_workspace_bytes(img_w, img_h), dev)
+    _lib.check(L.dmsh_smooth_gradient(stream, img_w, img_h, _lib.ptr(ht), _lib.ptr(vt), int(num_iter),
+                                      _lib.ptr(hd), _lib.ptr(vd), _lib.ptr(ws), ws.numel()),
                "dmsh_smooth_gradient")
 
 

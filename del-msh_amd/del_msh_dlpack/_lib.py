@@ -128,8 +128,10 @@ _PROTOTYPES = {
     "dmsh_edge_gradient_and_type": (
         c_int, [c_void_p, c_void_p, c_u32, c_void_p, c_u32, c_void_p, c_u32, c_u32, c_void_p, c_u32,
                 c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "dmsh_smooth_gradient_workspace_bytes": (c_size, [c_u32, c_u32]),
     "dmsh_smooth_gradient": (
-        c_int, [c_void_p, c_u32, c_u32, c_void_p, c_void_p, c_u32, c_void_p, c_void_p]),
+        c_int, [c_void_p, c_u32, c_u32, c_void_p, c_void_p, c_u32, c_void_p, c_void_p, c_void_p,
+                c_size]),
     "dmsh_interpolate_staggered_grid": (
         c_int, [c_void_p, c_u32, c_u32, c_void_p, c_void_p, c_u32, c_void_p, c_void_p]),
     "dmsh_exclusive_scan_workspace_bytes": (c_size, [c_u32]),

@@ -218,7 +218,11 @@ int dmsh_pix2depth_bwd_wrt_vtx2xyz(void *stream, float *dldw_vtx2xyz,
  * 77-200).  dmsh_smooth_gradient runs num_iter red-black sweeps of both
  * grids in place (cells of type 2 / 3 stay fixed) as the reference GPU path
  * does; del-msh-cpu's serial sweeps (rasterized_edge_gradient.rs:250-432)
- * converge to the same fixed point.  img_w, img_h >= 2.
+ * converge to the same fixed point.  img_w, img_h >= 2.  The smoothing
+ * needs a device workspace of dmsh_smooth_gradient_workspace_bytes (a second
+ * buffer of each grid and the per-cell links); it runs 16 half-sweeps per
+ * launch in LDS tiles and gives bit for bit the values of the sweeps run one
+ * colour at a time over the whole grid.
  * ------------------------------------------------------------------- */
 int dmsh_edge_gradient_and_type(void *stream, const uint32_t *tri2vtx, uint32_t num_tri,
                                 const float *vtx2xyz, uint32_t num_vtx,
@@ -226,9 +230,10 @@ int dmsh_edge_gradient_and_type(void *stream, const uint32_t *tri2vtx, uint32_t 
                                 const uint32_t *pix2tri, uint32_t num_vdim, const float *pix2val,
                                 const float *dldw_pix2val, uint8_t *hedge2type,
                                 float *hedge2dldr, uint8_t *vedge2type, float *vedge2dldr);
+size_t dmsh_smooth_gradient_workspace_bytes(uint32_t img_w, uint32_t img_h);
 int dmsh_smooth_gradient(void *stream, uint32_t img_w, uint32_t img_h, const uint8_t *hedge2type,
                          const uint8_t *vedge2type, uint32_t num_iter, float *hedge2dldr,
-                         float *vedge2dldr);
+                         float *vedge2dldr, void *ws, size_t ws_bytes);
 /* rasterized_edge_gradient.rs:202-248: vtx2velo [num_vtx][2] */
 int dmsh_interpolate_staggered_grid(void *stream, uint32_t img_w, uint32_t img_h,
                                     const float *hedge2vy, const float *vedge2vx,
