@@ -286,28 +286,69 @@ static __global__ __launch_bounds__(256) void k_smooth_links(
   }
 }
 
+// One cell's update in a half-step: the LDS addresses of its four terms in
+// summation order (a disabled link reads the word that holds +0), where the
+// result goes (cells that keep their value write to a sink word of their own)
+// and 1 / (number of links) in binary64.
+struct SmoothCell {
+  uint32_t a0, a1, a2, a3, out;
+  double inv;
+};
+
+// acc / n for n in 1..4, correctly rounded to binary32: for n = 1, 2, 4 the
+// binary64 product is exact; for n = 3, acc / 3 is never a binary32 midpoint
+// and lies at least 2^-26 (relative) from every one, while RN64(acc *
+// RN64(1/3)) is within 2^-52 of it, so the one rounding to binary32 lands on
+// the correctly rounded quotient (subnormal results included: they are
+// multiples of 2^-149 / 3 off the 2^-150 midpoint grid).
+__device__ __forceinline__ float smooth_div(float acc, double inv) {
+  return (float)((double)acc * inv);
+}
+
+template <int NPT>
+__device__ __forceinline__ void smooth_half_step(float* sv, const SmoothCell (&cell)[NPT]) {
+  float a[NPT][4];
+#pragma unroll
+  for (int k = 0; k < NPT; ++k) {
+    a[k][0] = sv[cell[k].a0];
+    a[k][1] = sv[cell[k].a1];
+    a[k][2] = sv[cell[k].a2];
+    a[k][3] = sv[cell[k].a3];
+  }
+#pragma unroll
+  for (int k = 0; k < NPT; ++k) {
+    const float acc = (((0.f + a[k][0]) + a[k][1]) + a[k][2]) + a[k][3];
+    sv[cell[k].out] = smooth_div(acc, cell[k].inv);
+  }
+}
+
 // blockIdx.x < tiles_h: a tile of the hedge grid, else of the vedge grid.
 // src / dst: the two buffers of each grid (cells outside the grid load as 0);
-// hm / vm: the links.  Every thread owns the same NPT cells of each colour in
-// every half-step; their LDS index and links stay in registers (links 0:
-// the cell keeps its value, as fixed cells, cells outside the grid and the
-// border ring of the tile do).  All reads of a half-step are issued before
-// its writes: a cell's neighbours have the other colour, so no cell read in
-// a half-step is written in it.  The LDS array has a pad of CC + 1 cells on
-// either side so that the border ring's neighbour reads stay inside it.  The
-// four neighbour values are read unconditionally and a disabled link adds
-// +0, which leaves the running sum (it starts at +0 and can never be -0)
-// exactly as skipping the term would.
+// hm / vm: the links.  The tile sits in LDS as two arrays [RR][CC / 2], one
+// per colour ((r + c) & 1 of the global cell): cell (lr, lc) is entry
+// (lr, lc >> 1) of its colour's array.  Its north and south neighbours are
+// entries (lr -/+ 1, lc >> 1) of the other array, and its west and east ones
+// (lr, (lc >> 1) - 1 + p) and (lr, (lc >> 1) + p) with p = lc & 1, so the
+// threads of a wave, which own consecutive entries, read consecutive words.
+// After the arrays come the +0 word and 2 * RR * (CC / 2) sink words.  Every
+// thread owns the same NPT entries of each colour in every half-step and
+// keeps their SmoothCell in registers; a cell that keeps its value (fixed,
+// outside the grid, no links, or on the border ring of the tile) reads +0 and
+// writes its sink word.  All reads of a half-step are issued before its
+// writes: a cell's neighbours have the other colour, so no cell read in a
+// half-step is written in it.  Adding the +0 of a disabled link leaves the
+// running sum (it starts at +0 and can never be -0) exactly as skipping the
+// term would.
 template <int K, int TR, int TC, int T>
 __global__ __launch_bounds__(T) void k_smooth_tiled(
     uint32_t W, uint32_t H, const uint8_t* __restrict__ hm, const uint8_t* __restrict__ vm,
     const float* __restrict__ hsrc, float* __restrict__ hdst, const float* __restrict__ vsrc,
     float* __restrict__ vdst, uint32_t tiles_h, uint32_t step0, uint32_t nsteps) {
-  constexpr int RR = TR + 2 * K, CC = TC + 2 * K, HALF = RR * (CC / 2);
-  constexpr int NPT = HALF / T, NL = RR * CC / T, PAD = CC + 1;
-  static_assert(CC % 2 == 0 && HALF % T == 0, "colour pairs split evenly over the threads");
-  static_assert(RR * CC <= 65536, "LDS index packed in 16 bits");
-  __shared__ float sv[RR * CC + 2 * PAD];
+  constexpr int RR = TR + 2 * K, CC = TC + 2 * K, HC = CC / 2, SZ = RR * HC;
+  constexpr int NPT = SZ / T, NL = RR * CC / T, ZERO = 2 * SZ, SINK = 2 * SZ + 1;
+  static_assert(CC % 2 == 0 && SZ % T == 0 && (RR * CC) % T == 0,
+                "colour entries split evenly over the threads");
+  __shared__ float sv[4 * SZ + 1];
   const bool hedge = blockIdx.x < tiles_h;
   const int R = hedge ? (int)H - 1 : (int)H;
   const int C = hedge ? (int)W : (int)W - 1;
@@ -317,10 +358,6 @@ __global__ __launch_bounds__(T) void k_smooth_tiled(
   const float* src = hedge ? hsrc : vsrc;
   float* dst = hedge ? hdst : vdst;
   const uint8_t* msk = hedge ? hm : vm;
-  // LDS offsets of the four links in summation order: hedges N, S, W, E;
-  // vedges W, E, N, S
-  const int o0 = hedge ? -CC : -1, o1 = hedge ? CC : 1, o2 = hedge ? -1 : -CC,
-            o3 = hedge ? 1 : CC;
   float v[NL];
 #pragma unroll
   for (int k = 0; k < NL; ++k) {
@@ -329,50 +366,63 @@ __global__ __launch_bounds__(T) void k_smooth_tiled(
     const float x = src[clampi(r, 0, R - 1) * C + clampi(c, 0, C - 1)];
     v[k] = (r >= 0 && c >= 0 && r < R && c < C) ? x : 0.f;
   }
-  uint32_t code[2][NPT];
+  SmoothCell cell[2][NPT];
 #pragma unroll
   for (int color = 0; color < 2; ++color) {
 #pragma unroll
     for (int k = 0; k < NPT; ++k) {
       const int j = (int)threadIdx.x + k * T;
-      const int lr = j / (CC / 2);
-      const int lc = 2 * (j % (CC / 2)) + ((color + r0 + c0 + lr) & 1);
+      const int lr = j / HC, h = j % HC;
+      const int p = (color + r0 + c0 + lr) & 1;
+      const int lc = 2 * h + p;
       const int r = r0 + lr, c = c0 + lc;
       const uint32_t m = msk[clampi(r, 0, R - 1) * C + clampi(c, 0, C - 1)];
       const bool live = lr >= 1 && lc >= 1 && lr < RR - 1 && lc < CC - 1 && r >= 0 && c >= 0 &&
-                        r < R && c < C && !(m & kLinkFixed);
-      code[color][k] = (uint32_t)(lr * CC + lc) | ((live ? m & 15u : 0u) << 16);
+                        r < R && c < C && !(m & kLinkFixed) && (m & 15u);
+      const uint32_t own = (uint32_t)(color * SZ + j);
+      // the same entry of the other colour's array, then its four neighbours
+      const int b = (int)own + (color ? -SZ : SZ);
+      const int ns0 = b - HC, ns1 = b + HC, we0 = b + p - 1, we1 = b + p;
+      const int t0 = hedge ? ns0 : we0, t1 = hedge ? ns1 : we1;
+      const int t2 = hedge ? we0 : ns0, t3 = hedge ? we1 : ns1;
+      SmoothCell& e = cell[color][k];
+      e.a0 = (live && (m & 1u)) ? (uint32_t)t0 : (uint32_t)ZERO;
+      e.a1 = (live && (m & 2u)) ? (uint32_t)t1 : (uint32_t)ZERO;
+      e.a2 = (live && (m & 4u)) ? (uint32_t)t2 : (uint32_t)ZERO;
+      e.a3 = (live && (m & 8u)) ? (uint32_t)t3 : (uint32_t)ZERO;
+      e.out = live ? own : (uint32_t)SINK + own;
+      const uint32_t n = __builtin_popcount(m & 15u);
+      e.inv = n == 2 ? 0.5 : (n == 3 ? 1.0 / 3.0 : (n == 4 ? 0.25 : 1.0));
     }
   }
 #pragma unroll
-  for (int k = 0; k < NL; ++k) sv[PAD + (int)threadIdx.x + k * T] = v[k];
+  for (int k = 0; k < NL; ++k) {
+    const int i = (int)threadIdx.x + k * T;
+    const int lr = i / CC, lc = i % CC;
+    sv[((r0 + c0 + lr + lc) & 1) * SZ + lr * HC + (lc >> 1)] = v[k];
+  }
+  if (threadIdx.x == 0) sv[ZERO] = 0.f;
   __syncthreads();
-  for (uint32_t s = 0; s < nsteps; ++s) {
-    const bool odd = (step0 + s) & 1u;
-    float a[NPT][4];
-#pragma unroll
-    for (int k = 0; k < NPT; ++k) {
-      const int i = PAD + (int)((odd ? code[1][k] : code[0][k]) & 0xffffu);
-      a[k][0] = sv[i + o0];
-      a[k][1] = sv[i + o1];
-      a[k][2] = sv[i + o2];
-      a[k][3] = sv[i + o3];
-    }
-#pragma unroll
-    for (int k = 0; k < NPT; ++k) {
-      const uint32_t cd = odd ? code[1][k] : code[0][k];
-      const uint32_t m = cd >> 16;
-      const float acc = (((0.f + ((m & 1u) ? a[k][0] : 0.f)) + ((m & 2u) ? a[k][1] : 0.f)) +
-                         ((m & 4u) ? a[k][2] : 0.f)) +
-                        ((m & 8u) ? a[k][3] : 0.f);
-      if (m) sv[PAD + (int)(cd & 0xffffu)] = acc / (float)__builtin_popcount(m);
-    }
+  uint32_t s = 0;
+  if ((step0 & 1u) && s < nsteps) {
+    smooth_half_step(sv, cell[1]);
+    __syncthreads();
+    ++s;
+  }
+  for (; s + 2 <= nsteps; s += 2) {
+    smooth_half_step(sv, cell[0]);
+    __syncthreads();
+    smooth_half_step(sv, cell[1]);
+    __syncthreads();
+  }
+  if (s < nsteps) {
+    smooth_half_step(sv, cell[0]);
     __syncthreads();
   }
   for (int i = threadIdx.x; i < TR * TC; i += T) {
     const int lr = K + i / TC, lc = K + i % TC;
     const int r = r0 + lr, c = c0 + lc;
-    if (r < R && c < C) dst[r * C + c] = sv[PAD + lr * CC + lc];
+    if (r < R && c < C) dst[r * C + c] = sv[((r + c) & 1) * SZ + lr * HC + (lc >> 1)];
   }
 }
 
