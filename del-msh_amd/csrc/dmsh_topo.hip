@@ -149,7 +149,8 @@ int dmsh_vtx2vtx_count(void* stream, uint32_t num_elem, uint32_t num_node,
   hipStream_t s = as_stream(stream);
   int rc = vtx2elem(s, num_elem, num_node, elem2vtx, num_vtx, L.vtx2jdx, L.jdx2elem, L.ve, err);
   if (rc != DMSH_OK) return rc;
-  launch("k_vtx2vtx_count", topo::k_vtx2vtx_count, dim3(ceil_div(num_vtx, 256)), dim3(256), 0, s,
+  launch("k_vtx2vtx_count", topo::k_vtx2vtx_count, dim3(ceil_div(num_vtx, topo::kV2VWaves)),
+         dim3(256), 0, s,
          num_vtx, elem2vtx, num_elem, num_node, (uint32_t)(is_self != 0), L.vtx2jdx, L.jdx2elem,
          L.buff, L.counts, err);
   if ((rc = launch_status()) != DMSH_OK) return rc;

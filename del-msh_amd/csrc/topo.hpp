@@ -165,42 +165,76 @@ static __global__ __launch_bounds__(256) void k_idx2elem(const uint32_t* __restr
 // ---------------------------------------------------------------------------
 // Vertex i's unique neighbours go to buff[vtx2jdx[i] * m ...] (m = num_node,
 // or num_node - 1 without the vertex itself: the most one element adds).
+// One wave per vertex (four per workgroup of 256).  The candidates, element
+// jdx2elem[jdx] node k for jdx in [j0, j1) and k in [0, num_node), are taken
+// 64 at a time in that order, one per lane.  A lane's candidate is new if it
+// is not the vertex itself (unless is_self), not in the list so far and not
+// held by a lower lane of the same chunk; new ones are appended in lane order,
+// which keeps the list in order of first appearance.  The first kV2VLocal
+// entries of the list are also kept in LDS, where the lanes' membership scans
+// read them as broadcasts; longer lists are scanned in buff beyond that.
+constexpr uint32_t kV2VLocal = 1024;
+constexpr uint32_t kV2VWaves = 4;
+
 static __global__ __launch_bounds__(256) void k_vtx2vtx_count(
     uint32_t num_vtx, const uint32_t* __restrict__ elem2vtx, uint32_t num_elem, uint32_t num_node,
     uint32_t is_self, const uint32_t* __restrict__ vtx2jdx, const uint32_t* __restrict__ jdx2elem,
     uint32_t* __restrict__ buff, uint32_t* __restrict__ counts, uint32_t* __restrict__ err) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  __shared__ uint32_t loc[kV2VWaves * kV2VLocal];
+  const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
+  const uint32_t i = blockIdx.x * kV2VWaves + w;
   if (i >= num_vtx) return;
+  uint32_t* sl = loc + w * kV2VLocal;
   const uint32_t m = is_self ? num_node : num_node - 1u;
   const uint32_t j0 = vtx2jdx[i], j1 = vtx2jdx[i + 1];
-  const size_t cap = (size_t)(j1 - j0) * m;
+  const uint32_t cap = (j1 - j0) * m;
+  const uint32_t ncand = (j1 - j0) * num_node;
   uint32_t* b = buff + (size_t)j0 * m;
+  const uint64_t below = (1ull << lane) - 1ull;
   uint32_t cnt = 0;
-  for (uint32_t jdx = j0; jdx < j1; ++jdx) {
-    const uint32_t e = jdx2elem[jdx];
-    if (e >= num_elem) {
-      raise_error(err, DMSH_DEVERR_INDEX);
-      continue;
-    }
-    for (uint32_t k = 0; k < num_node; ++k) {
-      const uint32_t j = elem2vtx[(size_t)e * num_node + k];
-      if (!is_self && j == i) continue;
-      bool seen = false;
-      for (uint32_t q = 0; q < cnt; ++q) {
-        if (b[q] == j) {
-          seen = true;
-          break;
-        }
-      }
-      if (seen) continue;
-      if (cnt >= cap) {  // only an element holding i more than its nodes allow
+  for (uint32_t base = 0; base < ncand; base += 64) {
+    const uint32_t c = base + lane;
+    bool valid = c < ncand;
+    uint32_t cand = 0xffffffffu;
+    if (valid) {
+      const uint32_t e = jdx2elem[j0 + c / num_node];
+      if (e >= num_elem) {
         raise_error(err, DMSH_DEVERR_INDEX);
-        continue;
+        valid = false;
+      } else {
+        cand = elem2vtx[(size_t)e * num_node + c % num_node];
+        valid = is_self || cand != i;
       }
-      b[cnt++] = j;
     }
+    bool dup = false;
+    const uint32_t nl = cnt < kV2VLocal ? cnt : kV2VLocal;
+    for (uint32_t q = 0; q < nl; ++q) dup |= sl[q] == cand;
+    if (cnt > kV2VLocal) {
+      __threadfence();  // the wave's own earlier stores to b, seen by all its lanes
+      for (uint32_t q = kV2VLocal; q < cnt; ++q) dup |= b[q] == cand;
+    }
+    const uint32_t nc = ncand - base < 64u ? ncand - base : 64u;
+    for (uint32_t l = 0; l + 1 < nc; ++l) {
+      const uint32_t x = (uint32_t)__shfl((int)cand, (int)l);
+      dup |= l < lane && x == cand;
+    }
+    // an invalid lane holds 0xffffffff, which a valid lane equals only for an
+    // out-of-range index, one the vertex-key pass has already flagged
+    const bool fresh = valid && !dup;
+    const uint64_t mask = __ballot(fresh);
+    const uint32_t pos = cnt + (uint32_t)__popcll(mask & below);
+    if (fresh) {
+      if (pos >= cap) {  // only an element holding i more than its nodes allow
+        raise_error(err, DMSH_DEVERR_INDEX);
+      } else {
+        if (pos < kV2VLocal) sl[pos] = cand;
+        b[pos] = cand;
+      }
+    }
+    cnt += (uint32_t)__popcll(mask);
+    if (cnt > cap) cnt = cap;
   }
-  counts[i] = cnt;
+  if (lane == 0) counts[i] = cnt;
 }
 
 static __global__ __launch_bounds__(256) void k_vtx2vtx_fill(uint32_t num_vtx, uint32_t m,

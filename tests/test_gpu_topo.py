@@ -44,10 +44,19 @@ def _case(name):
     if name == "quads":
         rng = np.random.default_rng(7)
         return rng.integers(0, 5000, (20000, 4)).astype(np.uint32), 5003
+    if name == "fan":  # hub with 3000 neighbours: lists longer than a wave's LDS share
+        n = 3000
+        k = np.arange(n, dtype=np.uint32)
+        return np.stack([np.zeros(n, np.uint32), 1 + k, 1 + (k + 1) % n], axis=1), n + 1
+    if name == "hubs":  # a few vertices in hundreds of random triangles, with repeats
+        rng = np.random.default_rng(11)
+        tri = rng.integers(0, 400, (4000, 3)).astype(np.uint32)
+        tri[::3, 0] = rng.integers(0, 3, tri[::3].shape[0])
+        return tri, 401
     raise KeyError(name)
 
 
-@pytest.mark.parametrize("name", ["sphere", "torus", "octa7", "ragged", "quads"])
+@pytest.mark.parametrize("name", ["sphere", "torus", "octa7", "ragged", "quads", "fan", "hubs"])
 def test_vtx2elem(gpu, oracle, name):
     elem, nv = _case(name)
     offs, idx2elem = Vtx2Elem.from_uniform_mesh(G(elem, gpu), nv)
@@ -58,7 +67,7 @@ def test_vtx2elem(gpu, oracle, name):
 
 
 @pytest.mark.parametrize("is_self", [False, True])
-@pytest.mark.parametrize("name", ["sphere", "torus", "octa7", "ragged", "quads"])
+@pytest.mark.parametrize("name", ["sphere", "torus", "octa7", "ragged", "quads", "fan", "hubs"])
 def test_vtx2vtx(gpu, oracle, name, is_self):
     elem, nv = _case(name)
     offs, idx2vtx = Vtx2Vtx.from_uniform_mesh(G(elem, gpu), nv, is_self)
