@@ -115,27 +115,52 @@ static __global__ __launch_bounds__(256) void k_batch_init(uint32_t M, uint32_t*
   for (size_t i = i0; i < zero_count; i += stride) zero[i] = 0u;
 }
 
+constexpr uint32_t kCentroidChunks = 16;  // 64-triangle chunks per wave
+
+__device__ __forceinline__ void flush_box(uint32_t* __restrict__ keys6, uint32_t m,
+                                          const uint32_t (&kmin)[3], const uint32_t (&kmax)[3]) {
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    atomicMin(keys6 + (size_t)m * 6 + d, kmin[d]);
+    atomicMax(keys6 + (size_t)m * 6 + 3 + d, kmax[d]);
+  }
+}
+
 // Triangle centroids ((0 + p0 + p1 + p2) * (1/3)) and per-mesh centroid
-// boxes through order keys.  A wave whose triangles all lie in one mesh
-// reduces first and issues 6 atomics; a wave across a mesh boundary lets each
-// lane issue its own.
+// boxes through order keys.  Each wave takes kCentroidChunks consecutive
+// chunks of 64 triangles.  Triangles are ordered by mesh, so the lanes of a
+// chunk form runs of one mesh; a segmented min/max over the lanes leaves each
+// run's box in its first lane.  The chunk's last run is carried into the next
+// chunk (it may continue there), every other run, and the carry where the
+// next chunk starts with another mesh or the wave ends, goes out as six
+// atomics.  So the atomics per mesh are bounded by the waves that touch it.
 static __global__ __launch_bounds__(256) void k_batch_centroid(
     const uint32_t* __restrict__ bad, uint32_t total_tri, uint32_t M,
     const uint32_t* __restrict__ mesh2tri, const uint32_t* __restrict__ mesh2vtx,
     const uint32_t* __restrict__ tri2vtx, const float* __restrict__ xyz, float* __restrict__ cntr,
     uint32_t* __restrict__ keys6, uint32_t* __restrict__ err) {
   if (*bad) return;
-  const uint32_t stride = gridDim.x * blockDim.x;
-  for (uint32_t base = blockIdx.x * blockDim.x; base < total_tri; base += stride) {
-    const uint32_t t = base + threadIdx.x;
+  constexpr uint32_t kNone = 0xFFFFFFFFu;
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t wave = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const uint32_t w0 = wave * 64u * kCentroidChunks;
+  if (w0 >= total_tri) return;
+  uint32_t m = find_segment(mesh2tri, M, w0 + lane < total_tri ? w0 + lane : total_tri - 1);
+  uint32_t carry = kNone;  // wave-uniform: the mesh of the carried run
+  uint32_t cmin[3] = {kNone, kNone, kNone}, cmax[3] = {0u, 0u, 0u};
+  for (uint32_t ch = 0; ch < kCentroidChunks; ++ch) {
+    const uint32_t base = w0 + ch * 64u;
+    if (base >= total_tri) break;
+    const uint32_t t = base + lane;
     const bool valid = t < total_tri;
-    uint32_t m = 0;
-    uint32_t kmin[3] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu}, kmax[3] = {0u, 0u, 0u};
+    uint32_t mm = kNone;
+    uint32_t kmin[3] = {kNone, kNone, kNone}, kmax[3] = {0u, 0u, 0u};
     if (valid) {
-      m = find_segment(mesh2tri, M, t);
+      while (mesh2tri[m + 1] <= t) ++m;
+      mm = m;
       const uint32_t vb = mesh2vtx[m], nv = mesh2vtx[m + 1] - vb;
       const uint32_t* tv = tri2vtx + (size_t)t * 3;
-      uint32_t v[3] = {tv[0], tv[1], tv[2]};
+      const uint32_t v[3] = {tv[0], tv[1], tv[2]};
       float c[3] = {0.f, 0.f, 0.f};
       if (v[0] >= nv || v[1] >= nv || v[2] >= nv) {
         raise_error(err, DMSH_DEVERR_INDEX);
@@ -156,32 +181,47 @@ static __global__ __launch_bounds__(256) void k_batch_centroid(
         kmin[d] = kmax[d] = f32_order_key(c[d]);
       }
     }
-    const uint32_t m0 = __builtin_amdgcn_readfirstlane(m);
-    if (__all(!valid || m == m0)) {
+    // segmented reduction towards each run's first lane (invalid lanes, at
+    // the end, form a run of kNone)
 #pragma unroll
-      for (int off = 32; off >= 1; off >>= 1) {
+    for (uint32_t off = 1; off < 64; off <<= 1) {
+      const uint32_t om = (uint32_t)__shfl_down((int)mm, off, 64);
+      const bool same = lane + off < 64 && om == mm;
 #pragma unroll
-        for (int d = 0; d < 3; ++d) {
-          const uint32_t a = __shfl_xor(kmin[d], off, 64), b = __shfl_xor(kmax[d], off, 64);
+      for (int d = 0; d < 3; ++d) {
+        const uint32_t a = (uint32_t)__shfl_down((int)kmin[d], off, 64);
+        const uint32_t b = (uint32_t)__shfl_down((int)kmax[d], off, 64);
+        if (same) {
           kmin[d] = a < kmin[d] ? a : kmin[d];
           kmax[d] = b > kmax[d] ? b : kmax[d];
         }
       }
-      if ((threadIdx.x & 63u) == 0 && base + (threadIdx.x & ~63u) < total_tri) {
+    }
+    const uint32_t prev = (uint32_t)__shfl_up((int)mm, 1, 64);
+    const bool head = mm != kNone && (lane == 0 || prev != mm);
+    const uint64_t heads = __ballot(head);
+    const uint32_t last = 63u - (uint32_t)__clzll(heads);  // first lane of the last run
+    const uint32_t first_m = (uint32_t)__shfl((int)mm, 0, 64);
+    if (carry != kNone && carry == first_m) {  // the carried run continues here
+      if (lane == 0) {
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
-          atomicMin(keys6 + (size_t)m0 * 6 + d, kmin[d]);
-          atomicMax(keys6 + (size_t)m0 * 6 + 3 + d, kmax[d]);
+          kmin[d] = cmin[d] < kmin[d] ? cmin[d] : kmin[d];
+          kmax[d] = cmax[d] > kmax[d] ? cmax[d] : kmax[d];
         }
       }
-    } else if (valid) {
+    } else if (carry != kNone && lane == 0) {
+      flush_box(keys6, carry, cmin, cmax);
+    }
+    if (head && lane != last) flush_box(keys6, mm, kmin, kmax);
+    carry = (uint32_t)__shfl((int)mm, (int)last, 64);
 #pragma unroll
-      for (int d = 0; d < 3; ++d) {
-        atomicMin(keys6 + (size_t)m * 6 + d, kmin[d]);
-        atomicMax(keys6 + (size_t)m * 6 + 3 + d, kmax[d]);
-      }
+    for (int d = 0; d < 3; ++d) {
+      cmin[d] = (uint32_t)__shfl((int)kmin[d], (int)last, 64);
+      cmax[d] = (uint32_t)__shfl((int)kmax[d], (int)last, 64);
     }
   }
+  if (carry != kNone && lane == 0) flush_box(keys6, carry, cmin, cmax);
 }
 
 // Composite key (mesh << 30 | 30-bit code under the mesh's unit-cube fit) and

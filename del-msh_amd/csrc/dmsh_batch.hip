@@ -224,8 +224,8 @@ int dmsh_batch_build_trimesh3(void* stream, uint32_t num_mesh, uint32_t total_tr
          mesh2tri, mesh2vtx, L.inner_off, mesh2node, L.bad, err);
   if ((rc = launch_status()) != DMSH_OK) return rc;
   if (n == 0) return DMSH_OK;
-  launch("k_batch_centroid", bat::k_batch_centroid, dim3(std::min<uint32_t>(ceil_div(n, 256), 4096)),
-         dim3(256), 0, s, L.bad, n, M, mesh2tri, mesh2vtx, tri2vtx, vtx2xyz, L.cntr, L.keys6, err);
+  launch("k_batch_centroid", bat::k_batch_centroid,
+         dim3(ceil_div(n, 4u * 64u * bat::kCentroidChunks)), dim3(256), 0, s, L.bad, n, M, mesh2tri, mesh2vtx, tri2vtx, vtx2xyz, L.cntr, L.keys6, err);
   if ((rc = launch_status()) != DMSH_OK) return rc;
   // With an even number of passes the sort returns to the buffer it starts
   // from; with an odd number it starts from the alternate key buffer.
