@@ -77,6 +77,27 @@ def test_smooth_gradient_red_black(gpu, oracle, num_iter):
     assert np.array_equal(H(vd), want_v)
 
 
+@pytest.mark.parametrize("num_iter", [8, 16, 23])
+@pytest.mark.parametrize("shape", [(2, 2), (77, 301), (515, 33)])
+def test_smooth_gradient_random_grids(gpu, oracle, shape, num_iter):
+    """Random types (fixed cells anywhere), -0 values, image sizes that are
+    not tile multiples, and launch counts of both parities; compared bit for
+    bit, so a -0 turned into +0 fails."""
+    W, Hh = shape
+    rng = np.random.default_rng(W * 1000 + Hh + num_iter)
+    ht = rng.integers(0, 5, (Hh - 1, W)).astype(np.uint8)
+    vt = rng.integers(0, 5, (Hh, W - 1)).astype(np.uint8)
+    hd = rng.standard_normal((Hh - 1, W)).astype(np.float32)
+    vd = rng.standard_normal((Hh, W - 1)).astype(np.float32)
+    hd[rng.random(hd.shape) < 0.1] = -0.0
+    vd[rng.random(vd.shape) < 0.1] = -0.0
+    want_h, want_v = oracle.smooth_gradient(ht, vt, num_iter, hd, vd)
+    gh, gv = G(hd, gpu), G(vd, gpu)
+    REG.smooth_gradient(G(ht, gpu), G(vt, gpu), num_iter, gh, gv)
+    assert np.array_equal(H(gh).view(np.uint32), want_h.view(np.uint32))
+    assert np.array_equal(H(gv).view(np.uint32), want_v.view(np.uint32))
+
+
 def test_interpolate_staggered(gpu, oracle):
     rng = np.random.default_rng(4)
     Hh, W = 37, 53
