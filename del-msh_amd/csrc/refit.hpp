@@ -487,7 +487,7 @@ __global__ __launch_bounds__(T) void k_refit_top_lds(
 }
 
 constexpr uint32_t kTreeletLeaves = 1024;  // K
-constexpr int kTreeletThreads = 256;
+constexpr int kTreeletThreads = 512;
 constexpr uint32_t kTreeletGrid = 1024;    // persistent workgroups
 constexpr int kTopThreads = 1024;
 constexpr int kTopSlots = 6400;  // 25 B each: 156 KB of LDS
