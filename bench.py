@@ -600,11 +600,16 @@ def main():
         aabb_h = aabb.cpu().numpy()
         threads = min(16, os.cpu_count() or 1)
         tc = time.perf_counter()
-        wpix, _, (n_exp, n_leaf, fhist) = oracle.pix2tri(tri, vtx, nodes_h, aabb_h, n2w.T.reshape(-1),
-                                                         W, H, num_threads=threads, with_t=False)
+        wpix, _, _ = oracle.pix2tri(tri, vtx, nodes_h, aabb_h, n2w.T.reshape(-1), W, H,
+                                    num_threads=threads, with_t=False)
         cpu_cast_s = time.perf_counter() - tc
-        cast_counts = (n_exp, n_leaf, int(np.nonzero(fhist)[0].max()))
         mism = int(np.count_nonzero(wpix != pix2tri.cpu().numpy()))
+        # the work of the search the kernel runs (one best hit, far children
+        # on a stack), which equals del-msh-cpu's recursion in result
+        fpix, _, (n_exp, n_leaf, fhist) = oracle.pix2tri_flat(
+            tri, vtx, nodes_h, aabb_h, n2w.T.reshape(-1), W, H, num_threads=threads, with_t=False)
+        cast_counts = (n_exp, n_leaf, int(np.nonzero(fhist)[0].max()))
+        mism += int(np.count_nonzero(fpix != wpix))
         tb = time.perf_counter()
         oracle.build_trimesh3(tri, vtx, morton_bits=63)
         cpu_build_s = time.perf_counter() - tb

@@ -273,7 +273,7 @@ int dmsh_batch_pix2tri(void* stream, uint32_t num_mesh, uint32_t total_tri, uint
   if (num_mesh == 0 || !mesh2tri || !mesh2vtx || !mesh2node || !mesh2pix || !mesh2width ||
       !transforms_ndc2world)
     return DMSH_ERR_INVALID;
-  if (total_tri >= rc::kNoneTri || total_pix == 0) return total_pix == 0 ? DMSH_OK : DMSH_ERR_INVALID;
+  if (total_tri >= rc::kMaxTri || total_pix == 0) return total_pix == 0 ? DMSH_OK : DMSH_ERR_INVALID;
   if (!pix2tri) return DMSH_ERR_INVALID;
   if (total_tri > 0 && (!tri2vtx || !vtx2xyz || !bvhnodes || !bvhnode2aabb))
     return DMSH_ERR_INVALID;
@@ -352,7 +352,7 @@ int dmsh_batch_pack(void* stream, uint32_t num_mesh, uint32_t total_tri, uint32_
                     void* ws, size_t ws_bytes) {
   if (num_mesh == 0 || !mesh2tri || !mesh2vtx || !mesh2node || !packed || !nonfinite)
     return DMSH_ERR_INVALID;
-  if (total_tri >= rc::kNoneTri) return DMSH_ERR_INVALID;
+  if (total_tri >= rc::kMaxTri) return DMSH_ERR_INVALID;
   if (total_tri > 0 && (!tri2vtx || !vtx2xyz || !bvhnodes || !bvhnode2aabb))
     return DMSH_ERR_INVALID;
   if (!ws || ws_bytes < dmsh_batch_pack_workspace_bytes(num_mesh)) return DMSH_ERR_WORKSPACE;

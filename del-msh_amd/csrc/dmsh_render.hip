@@ -104,7 +104,7 @@ int dmsh_pix2tri(void* stream, uint32_t* pix2tri, float* pix2t, uint32_t num_tri
   const uint64_t num_pix = (uint64_t)img_w * img_h;
   if (num_pix == 0) return DMSH_OK;
   if (num_pix >= (1ull << 32) || !pix2tri || !transform_ndc2world) return DMSH_ERR_INVALID;
-  if (num_tri >= rc::kNoneTri) return DMSH_ERR_INVALID;
+  if (num_tri >= rc::kMaxTri) return DMSH_ERR_INVALID;
   if (num_tri > 0 && (!tri2vtx || !vtx2xyz || !bvhnodes || !bvhnode2aabb)) return DMSH_ERR_INVALID;
   if (!ws || ws_bytes < dmsh_pix2tri_workspace_bytes(num_tri, img_w, img_h))
     return DMSH_ERR_WORKSPACE;

@@ -9,11 +9,16 @@
 //            child is near); rn = F(near); if rn and the far entry t > rn.t
 //            (or far missed) -> rn; else rf = F(far) and the result is
 //            rn if rn.t < rf.t else rf.
-// Executed as a loop over an explicit frame stack.  A frame is pushed only
-// where both children are hit (a single hit child is a tail call) and holds
-// 8 bytes: before the near subtree returns it stores (far child, far entry t);
-// after the decision to descend far it stores the near result (tri, t); a far
-// descent with no near result keeps no frame.
+// Executed as a flat closest-hit search with the same result: one best hit,
+// a stack of 8-byte frames (far child, far entry t) pushed where both children
+// are hit (a single hit child is a tail call), a popped far child skipped when
+// its entry t exceeds the best t so far, and a triangle hit replacing the best
+// when its t <= the best t.  Children are visited near first, so of two hits
+// the later one lies in the far subtree of their lowest common ancestor,
+// where F's combine keeps far on equal t; a skipped subtree cannot hold a hit
+// with t <= the final one.  (The oracle checks this search against the
+// recursion on scenes of exact ties: duplicated triangles, rays through grid
+// vertices, coplanar soups.)
 //
 // Layout: before traversal the BVH is packed into
 //   inner[i] (64 B, internal node i): both child boxes + both child refs
@@ -38,8 +43,7 @@ namespace rc {
 
 constexpr uint32_t kLeafBit = 0x40000000u;   // child ref: leaf slot
 constexpr uint32_t kIndexMask = 0x3FFFFFFFu;
-constexpr uint32_t kState1 = 0x80000000u;    // frame holds a near result
-constexpr uint32_t kNoneTri = 0x3FFFFFFFu;   // frame: near result is none
+constexpr uint32_t kMaxTri = 0x3FFFFFFFu;    // child refs hold 30-bit indices
 constexpr int kTier2Frames = 64;
 constexpr uint32_t kDeepFrames = 4096;
 constexpr uint32_t kDeepSlots = 256;
@@ -240,10 +244,12 @@ __device__ __forceinline__ bool slab_hit_finite(const RaySlab& r, const float lo
 
 constexpr uint32_t kNoTest = DMSH_U32MAX;
 
-// The traversal loop.  Each iteration either expands one internal node or
-// pops one frame, then runs the (single) leaf-test site if either step
-// produced a leaf, so a wave whose lanes diverge between the two steps pays
-// for the triangle test once.
+// The traversal loop.  An iteration first pops, if the lane has no node to
+// expand, the next far child not skipped by the best hit (ending the lane's
+// search when the stack runs out); then expands the lane's node, if any; then
+// runs the (single) leaf-test site if the pop or the expansion produced a
+// leaf, so a wave whose lanes diverge between these steps pays for the
+// triangle test once.
 template <bool kFinite, class Stack>
 __device__ __forceinline__ int trace_loop(const Packed& pk, const RaySlab& r, const float org[3],
                                           const float dir[3], Stack& stk, Hit& out) {
@@ -255,6 +261,24 @@ __device__ __forceinline__ int trace_loop(const Packed& pk, const RaySlab& r, co
   bool expand = true;
   for (;;) {
     uint32_t test = kNoTest;
+    if (!expand) {
+      uint2 f = make_uint2(0u, 0u);
+      bool found = false;
+      while (sp > 0) {
+        f = stk.load(--sp);
+        if (!(res_has && __uint_as_float(f.y) > res_t)) {
+          found = true;
+          break;
+        }
+      }
+      if (!found) break;
+      if (f.x & kLeafBit) {
+        test = f.x & kIndexMask;
+      } else {
+        node = f.x;
+        expand = true;
+      }
+    }
     if (expand) {
       const InnerNode nd = pk.inner[node];
       const float l0[3] = {nd.a.x, nd.a.y, nd.a.z}, h0[3] = {nd.a.w, nd.b.x, nd.b.y};
@@ -268,7 +292,7 @@ __device__ __forceinline__ int trace_loop(const Packed& pk, const RaySlab& r, co
         hl = slab_hit(r, l0, h0, tl0, tl1);
         hr = slab_hit(r, l1, h1, tr0, tr1);
       }
-      uint32_t next;
+      uint32_t next = kNoTest;
       if (hl && hr) {
         const bool nl = tuple_lt(tl0, tl1, tr0, tr1);
         if (sp == Stack::kCap) return kTraceOverflow;
@@ -276,52 +300,27 @@ __device__ __forceinline__ int trace_loop(const Packed& pk, const RaySlab& r, co
         next = nl ? nd.d.x : nd.d.y;
       } else if (hl || hr) {
         next = hl ? nd.d.x : nd.d.y;
-      } else {
-        res_has = false;
-        res_t = 0.f;
-        res_tri = DMSH_U32MAX;
-        expand = false;
-        continue;
       }
-      if (next & kLeafBit) {
+      if (next == kNoTest) {
+        expand = false;  // both children missed
+      } else if (next & kLeafBit) {
         test = next & kIndexMask;
         expand = false;
       } else {
         node = next;
       }
-    } else {
-      if (sp == 0) break;
-      const uint2 f = stk.load(sp - 1);
-      if (!(f.x & kState1)) {
-        // near subtree done: prune the far child or descend into it
-        if (res_has && __uint_as_float(f.y) > res_t) {
-          --sp;
-          continue;
-        }
-        if (res_has) {
-          stk.store(sp - 1, make_uint2(kState1 | res_tri, __float_as_uint(res_t)));
-        } else {
-          --sp;  // combine(none, far) = far: no frame needed
-        }
-        if (f.x & kLeafBit) {
-          test = f.x & kIndexMask;
-        } else {
-          node = f.x;
-          expand = true;
-        }
-      } else {
-        // far subtree done: combine with the stored near result
-        --sp;
-        const uint32_t rn_tri = f.x & ~kState1;
-        const float rn_t = __uint_as_float(f.y);
-        if (!res_has || rn_t < res_t) {
-          res_has = true;
-          res_t = rn_t;
-          res_tri = rn_tri;
-        }
+    }
+    if (test != kNoTest) {
+      bool has;
+      float t;
+      uint32_t tri;
+      leaf_test(pk, test, org, dir, has, t, tri);
+      if (has && (!res_has || t <= res_t)) {
+        res_has = true;
+        res_t = t;
+        res_tri = tri;
       }
     }
-    if (test != kNoTest) leaf_test(pk, test, org, dir, res_has, res_t, res_tri);
   }
   out.has = res_has;
   out.t = res_t;

@@ -127,7 +127,7 @@ static __global__ __launch_bounds__(1024) void k_req_offsets(uint32_t R, uint32_
     const Request& q = req[r];
     // a running sum that wrapped past 2^32 shows up as a decrease
     const bool ok = q.img_w > 0 && q.img_h > 0 && q.pix_start == pix_off[r] &&
-                    (uint64_t)q.img_w * q.img_h < (1ull << 32) && q.num_tri < rc::kNoneTri &&
+                    (uint64_t)q.img_w * q.img_h < (1ull << 32) && q.num_tri < rc::kMaxTri &&
                     pix_off[r + 1] > pix_off[r] && tile_off[r + 1] > tile_off[r];
     if (!ok) {
       atomicOr(bad, 1u);

@@ -126,6 +126,8 @@ class Server:
         self.max_step_requests = max_step_requests
         self.max_step_pixels = max_step_pixels
         self.stream = torch.cuda.current_stream(self.device)
+        # uploads of new meshes run here, beside the previous step's casts
+        self.copy_stream = torch.cuda.Stream(device=self.device)
 
     def tree_bytes(self, mid: int) -> int:
         return int(self.L.dmsh_packed_bytes(self.catalog.num_tri(mid)))
@@ -142,14 +144,24 @@ class Server:
         m2t = np.concatenate([[0], np.cumsum(ntri)]).astype(np.uint32)
         m2v = np.concatenate([[0], np.cumsum(nvtx)]).astype(np.uint32)
         rows = np.concatenate([[0], np.cumsum(2 * ntri - 1)]).astype(np.uint32)
-        # the step's meshes, concatenated on the device straight from pinned memory
-        tri = torch.empty((int(m2t[-1]), 3), dtype=torch.uint32, device=dev)
-        vtx = torch.empty((int(m2v[-1]), 3), dtype=torch.float32, device=dev)
-        for k, (t, v) in enumerate(meshes):
-            tri[int(m2t[k]):int(m2t[k + 1])].copy_(t, non_blocking=True)
-            vtx[int(m2v[k]):int(m2v[k + 1])].copy_(v, non_blocking=True)
-        d_m2t = _upload(m2t, dev)
-        d_m2v = _upload(m2v, dev)
+        # the step's meshes, concatenated on the device straight from pinned
+        # memory on the copy stream, so the transfer overlaps the casts already
+        # queued on the compute stream; the compute stream waits for it, and
+        # record_stream keeps the staging memory from reuse until the build and
+        # pack that read it have run
+        with torch.cuda.stream(self.copy_stream):
+            tri = torch.empty((int(m2t[-1]), 3), dtype=torch.uint32, device=dev)
+            vtx = torch.empty((int(m2v[-1]), 3), dtype=torch.float32, device=dev)
+            for k, (t, v) in enumerate(meshes):
+                tri[int(m2t[k]):int(m2t[k + 1])].copy_(t, non_blocking=True)
+                vtx[int(m2v[k]):int(m2v[k + 1])].copy_(v, non_blocking=True)
+            d_m2t = _upload(m2t, dev)
+            d_m2v = _upload(m2v, dev)
+            uploaded = torch.cuda.Event()
+            uploaded.record(self.copy_stream)
+        self.stream.wait_event(uploaded)
+        for t in (tri, vtx, d_m2t, d_m2v):
+            t.record_stream(self.stream)
         m2n = torch.empty(len(new_ids) + 1, dtype=torch.uint32, device=dev)
         nodes = torch.empty((int(rows[-1]), 3), dtype=torch.uint32, device=dev)
         aabb = torch.empty((int(rows[-1]), 6), dtype=torch.float32, device=dev)

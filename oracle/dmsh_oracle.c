@@ -727,6 +727,118 @@ int orc_first_intersection_ray(const float org[3], const float dir[3],
   return 1;
 }
 
+/* The same search as fir() without per-subtree results: one best hit, a
+ * stack of far children with their entry t, a far child skipped when its
+ * entry t exceeds the best t so far, and a hit replacing the best when its
+ * t <= the best t.  Children are visited near first, so of two hits the later
+ * one lies in the far subtree of their lowest common ancestor, where fir()'s
+ * combine keeps far on equal t; and a skipped subtree cannot hold a hit with
+ * t <= the final one.  Returns -1 if the stack (cap frames) overflows. */
+static int fir_flat(cast_ctx *cx, hit_rec *out, size_t *stk_node, float *stk_t, int cap) {
+  float t0, t1;
+  if (!orc_ray_aabb(cx->aabb, cx->org, cx->dir, &t0, &t1)) return 0;
+  const uint32_t *bn = cx->bn;
+  int has = 0, sp = 0;
+  size_t node = 0;
+  for (;;) {
+    if (bn[node * 3 + 2] == U32MAX) {
+      const uint32_t tri = bn[node * 3 + 1];
+      const uint32_t *v = cx->tri2vtx + (size_t)tri * 3;
+      float t, bc[3];
+      cx->st.tris_tested++;
+      if (orc_ray_tri(cx->xyz + (size_t)v[0] * 3, cx->xyz + (size_t)v[1] * 3,
+                      cx->xyz + (size_t)v[2] * 3, cx->org, cx->dir, &t, bc) &&
+          t < INFINITY && (!has || t <= out->t)) {
+        has = 1;
+        out->t = t;
+        out->tri = tri;
+        memcpy(out->bc, bc, sizeof(bc));
+      }
+    } else {
+      cx->st.nodes_visited++;
+      const size_t left = bn[node * 3 + 1], right = bn[node * 3 + 2];
+      float tl0, tl1, tr0, tr1;
+      const int hl = orc_ray_aabb(cx->aabb + left * 6, cx->org, cx->dir, &tl0, &tl1);
+      const int hr = orc_ray_aabb(cx->aabb + right * 6, cx->org, cx->dir, &tr0, &tr1);
+      if (hl && hr) {
+        const int nl = tuple_lt(tl0, tl1, tr0, tr1);
+        if (sp == cap) return -1;
+        stk_node[sp] = nl ? right : left;
+        stk_t[sp] = nl ? tr0 : tl0;
+        ++sp;
+        if (sp > cx->max_frames) cx->max_frames = sp;
+        node = nl ? left : right;
+        continue;
+      }
+      if (hl || hr) {
+        node = hl ? left : right;
+        continue;
+      }
+    }
+    for (;;) {  /* pop the next far child that may still hold a hit */
+      if (sp == 0) return has;
+      --sp;
+      if (!(has && stk_t[sp] > out->t)) break;
+    }
+    node = stk_node[sp];
+  }
+}
+
+int orc_pix2tri_flat(uint32_t *pix2tri, float *pix2t, size_t num_tri, const uint32_t *tri2vtx,
+                     const float *vtx2xyz, const uint32_t *bvhnodes, const float *aabb,
+                     uint32_t img_w, uint32_t img_h, const float T[16], int num_threads,
+                     orc_trace_stats *stats) {
+  enum { CAP = 4096 };
+  const int64_t num_pix = (int64_t)img_w * (int64_t)img_h;
+  uint64_t nv = 0, nt = 0, hist[64];
+  memset(hist, 0, sizeof(hist));
+  int rc = 0;
+#ifdef _OPENMP
+  const int nth = num_threads > 0 ? num_threads : 1;
+#pragma omp parallel num_threads(nth) reduction(+ : nv, nt, hist[:64]) reduction(min : rc)
+#endif
+  {
+    size_t *stk_node = (size_t *)malloc(sizeof(size_t) * CAP);
+    float *stk_t = (float *)malloc(sizeof(float) * CAP);
+    if (!stk_node || !stk_t) rc = -1;
+#ifdef _OPENMP
+#pragma omp for schedule(dynamic, 256)
+#endif
+    for (int64_t ip = 0; ip < num_pix; ++ip) {
+      const uint32_t ih = (uint32_t)(ip / img_w);
+      const uint32_t iw = (uint32_t)(ip - (int64_t)ih * img_w);
+      cast_ctx cx;
+      memset(&cx, 0, sizeof(cx));
+      float org[3], dir[3];
+      orc_ray_for_pixel(iw, ih, img_w, img_h, T, org, dir);
+      cx.org = org;
+      cx.dir = dir;
+      cx.tri2vtx = tri2vtx;
+      cx.xyz = vtx2xyz;
+      cx.bn = bvhnodes;
+      cx.aabb = aabb;
+      hit_rec h;
+      int hit = 0;
+      if (num_tri && stk_node && stk_t) hit = fir_flat(&cx, &h, stk_node, stk_t, CAP);
+      if (hit < 0) rc = -1;
+      pix2tri[ip] = hit > 0 ? h.tri : U32MAX;
+      if (pix2t) pix2t[ip] = hit > 0 ? h.t : 0.f;
+      nv += cx.st.nodes_visited;
+      nt += cx.st.tris_tested;
+      hist[cx.max_frames < 63 ? cx.max_frames : 63] += 1;
+    }
+    free(stk_node);
+    free(stk_t);
+  }
+  (void)num_threads;
+  if (stats) {
+    stats->nodes_visited += nv;
+    stats->tris_tested += nt;
+    for (int k = 0; k < 64; ++k) stats->frames_hist[k] += hist[k];
+  }
+  return rc;
+}
+
 /* pix2tri.rs:3-48 (rayon par_iter over pixels -> OpenMP parallel for) */
 int orc_pix2tri(uint32_t *pix2tri, float *pix2t, size_t num_tri,
                 const uint32_t *tri2vtx, const float *vtx2xyz,

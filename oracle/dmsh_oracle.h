@@ -123,6 +123,13 @@ int orc_pix2tri(uint32_t *pix2tri, float *pix2t, size_t num_tri,
                 const uint32_t *bvhnodes, const float *bvhnode2aabb,
                 uint32_t img_w, uint32_t img_h, const float ndc2world[16],
                 int num_threads, orc_trace_stats *stats);
+/* the same image by a flat closest-hit search (one best hit, far children on
+ * a stack, ties to the later hit); equal to orc_pix2tri by construction, and
+ * checked to be so */
+int orc_pix2tri_flat(uint32_t *pix2tri, float *pix2t, size_t num_tri, const uint32_t *tri2vtx,
+                     const float *vtx2xyz, const uint32_t *bvhnodes, const float *aabb,
+                     uint32_t img_w, uint32_t img_h, const float T[16], int num_threads,
+                     orc_trace_stats *stats);
 /* pix2depth.rs:107-146 */
 int orc_pix2depth(float *pix2depth, const uint32_t *pix2tri,
                   const uint32_t *tri2vtx, const float *vtx2xyz,

@@ -71,6 +71,10 @@ def lib():
                 ctypes.c_int,
                 [_u32p, ctypes.c_void_p, _sz, _u32p, _f32p, _u32p, _f32p, ctypes.c_uint32,
                  ctypes.c_uint32, _f32p, ctypes.c_int, ctypes.POINTER(TraceStats)]),
+            "orc_pix2tri_flat": (
+                ctypes.c_int,
+                [_u32p, ctypes.c_void_p, _sz, _u32p, _f32p, _u32p, _f32p, ctypes.c_uint32,
+                 ctypes.c_uint32, _f32p, ctypes.c_int, ctypes.POINTER(TraceStats)]),
             "orc_pix2depth": (ctypes.c_int, [_f32p, _u32p, _u32p, _f32p, ctypes.c_uint32,
                                              ctypes.c_uint32, _f32p]),
             "orc_pix2depth_bwd": (ctypes.c_int, [_f32p, _u32p, _u32p, _f32p, _f32p,
@@ -223,6 +227,25 @@ def pix2tri(tri2vtx, vtx2xyz, bvhnodes, bvhnode2aabb, ndc2world_colmajor, img_w,
     if with_t:
         return res, tt.reshape(img_h, img_w), stats
     return res, None, stats
+
+
+def pix2tri_flat(tri2vtx, vtx2xyz, bvhnodes, bvhnode2aabb, ndc2world_colmajor, img_w, img_h,
+                 num_threads=1, with_t=True):
+    """The image by the flat closest-hit search the GPU kernels run; returns
+    (pix2tri, pix2t or None, (internal nodes expanded, triangles tested,
+    histogram of the deepest stack per ray))."""
+    t = _u32(tri2vtx)
+    out = np.empty(img_w * img_h, np.uint32)
+    tt = np.empty(img_w * img_h, np.float32) if with_t else None
+    st = TraceStats()
+    rc = lib().orc_pix2tri_flat(out, tt.ctypes.data if with_t else None, t.shape[0], t.reshape(-1),
+                                _f32(vtx2xyz).reshape(-1), _u32(bvhnodes).reshape(-1),
+                                _f32(bvhnode2aabb).reshape(-1), img_w, img_h,
+                                _f32(ndc2world_colmajor), num_threads, ctypes.byref(st))
+    if rc != 0:
+        raise RuntimeError("flat search stack overflow")
+    stats = (int(st.nodes_visited), int(st.tris_tested), np.array(st.frames_hist[:], np.int64))
+    return (out.reshape(img_h, img_w), tt.reshape(img_h, img_w) if with_t else None, stats)
 
 
 def first_intersection_ray(org, dirv, tri2vtx, vtx2xyz, bvhnodes, bvhnode2aabb):

@@ -200,6 +200,52 @@ def test_pix2tri(gpu, oracle, mesh, camera):
     no_device_errors()
 
 
+def _pixel_grid(n):
+    """z = 0 grid with its vertices under the pixel centres of an n x n
+    orthographic image: rays through vertices and edges tie between the
+    triangles around them."""
+    x = ((2.0 * (np.arange(n) + 0.5) / n) - 1.0).astype(np.float32)
+    xx, yy = np.meshgrid(x, x)
+    vtx = np.stack([xx.ravel(), yy.ravel(), np.zeros(n * n, np.float32)], 1).astype(np.float32)
+    tri = []
+    for j in range(n - 1):
+        for i in range(n - 1):
+            a, b, c, d = j * n + i, j * n + i + 1, (j + 1) * n + i, (j + 1) * n + i + 1
+            tri += [[a, b, d], [a, d, c]]
+    return np.array(tri, np.uint32), vtx
+
+
+@pytest.mark.parametrize("scene", ["sphere_twice", "grid_twice", "coplanar_soup"])
+def test_pix2tri_exact_ties(gpu, oracle, scene):
+    """Scenes where hits tie exactly (every triangle duplicated, rays through
+    grid vertices, overlapping coplanar triangles): the winner of each tie is
+    fixed by the reference's recursion, and the flat search must pick it."""
+    img = 64
+    if scene == "sphere_twice":
+        tri, vtx = MESHES["sphere_c1"]()
+        tri = np.concatenate([tri, tri])
+        n2w = _perspective()[0]
+    elif scene == "grid_twice":
+        tri, vtx = _pixel_grid(img)
+        tri = np.concatenate([tri[::-1], tri])
+        n2w = np.eye(4, dtype=np.float32)
+    else:
+        rng = np.random.default_rng(5)
+        soup = rng.uniform(-1, 1, (800, 3, 3)).astype(np.float32)
+        soup[:, :, 2] = np.round(soup[:, :, 2] * 2) / 4
+        vtx = soup.reshape(-1, 3)
+        tri = np.arange(vtx.shape[0], dtype=np.uint32).reshape(-1, 3)
+        n2w = np.eye(4, dtype=np.float32)
+    bn, aabb = oracle.build_trimesh3(tri, vtx)
+    pix, t = Pix2Tri.by_raycasting_with_coeff(G(tri, gpu), G(vtx, gpu), G(bn, gpu), G(aabb, gpu),
+                                              G(n2w, gpu), (img, img))
+    wpix, wt, _ = oracle.pix2tri(tri, vtx, bn, aabb, n2w.T.reshape(-1), img, img, num_threads=8)
+    assert (wpix != U32MAX).sum() > img
+    assert np.array_equal(H(pix), wpix)
+    assert bits_equal(H(t), wt)
+    no_device_errors()
+
+
 @pytest.mark.parametrize("camera", ["ortho", "persp"])
 def test_pix2tri_deep_stack_path(gpu, oracle, camera):
     """A run of equal Morton codes whose triangles are stacked along the view
