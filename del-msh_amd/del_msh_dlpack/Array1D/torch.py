@@ -26,3 +26,38 @@ def argsort(idx2val: torch.Tensor, key_bits: int | None = None):
 
     native(jdx2val, jdx2idx, _lib.stream_of(device), key_bits)
     return jdx2idx, jdx2val
+
+
+def has_duplicate_in_sorted_array(idx2val: torch.Tensor) -> bool:
+    """Whether a sorted uint32 array holds a duplicate (reference:
+    del_msh_dlpack/Array1D/torch.py:77-101)."""
+    n = idx2val.shape[0]
+    assert idx2val.shape == (n,) and idx2val.dtype == torch.uint32
+    from . import has_duplicate_sorted_array
+
+    return has_duplicate_sorted_array(idx2val, _lib.stream_of(_lib.require_gpu(idx2val)))
+
+
+def unique_for_sorted_array(idx2val: torch.Tensor):
+    """Unique values of a sorted uint32 array (reference:
+    del_msh_dlpack/Array1D/torch.py:104-148).
+
+    Returns:
+        idx2jdx: (num_idx,) uint32 -- group of each element
+        jdx2val: (num_jdx,) uint32 -- the unique values
+        jdx2idx_offset: (num_jdx+1,) uint32 -- first element of each group
+    """
+    num_idx = idx2val.shape[0]
+    device = idx2val.device
+    assert idx2val.shape == (num_idx,) and idx2val.dtype == torch.uint32
+    stream = _lib.stream_of(_lib.require_gpu(idx2val))
+    from . import unique_for_sorted_array as native_unique
+    from . import unique_jdx2val_jdx2idx
+
+    idx2jdx = torch.empty((num_idx,), dtype=torch.uint32, device=device)
+    native_unique(idx2val, idx2jdx, stream)
+    num_jdx = int(idx2jdx[-1:].view(torch.int32).item()) + 1 if num_idx else 0
+    jdx2val = torch.empty((num_jdx,), dtype=torch.uint32, device=device)
+    jdx2idx_offset = torch.empty((num_jdx + 1,), dtype=torch.uint32, device=device)
+    unique_jdx2val_jdx2idx(idx2val, idx2jdx, jdx2val, jdx2idx_offset, stream)
+    return idx2jdx, jdx2val, jdx2idx_offset

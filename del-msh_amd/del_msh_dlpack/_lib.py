@@ -154,6 +154,24 @@ _PROTOTYPES = {
          c_void_p]),
     "dmsh_multiply_graph_laplacian": (
         c_int, [c_void_p, c_u32, c_void_p, c_u32, c_void_p, c_u32, c_void_p, c_void_p]),
+    "dmsh_octree_bnodes_workspace_bytes": (c_size, [c_u32]),
+    "dmsh_octree_bnodes": (
+        c_int, [c_void_p, c_u32, c_void_p, c_u32, c_u32, c_void_p, c_void_p, c_void_p, c_void_p,
+                c_void_p, c_size]),
+    "dmsh_octree_make_tree": (
+        c_int, [c_void_p, c_u32, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_u32, c_u32,
+                c_u32, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "dmsh_octree_aggregate_workspace_bytes": (c_size, [c_u32]),
+    "dmsh_octree_aggregate": (
+        c_int, [c_void_p, c_u32, c_u32, c_void_p, c_void_p, c_u32, c_u32, c_void_p, c_void_p,
+                c_void_p, c_size]),
+    "dmsh_has_duplicate_sorted_u32": (c_int, [c_void_p, c_u32, c_void_p, c_void_p]),
+    "dmsh_unique_for_sorted_workspace_bytes": (c_size, [c_u32]),
+    "dmsh_unique_for_sorted_u32": (c_int, [c_void_p, c_u32, c_void_p, c_void_p, c_void_p, c_size]),
+    "dmsh_unique_jdx2val_jdx2idx": (
+        c_int, [c_void_p, c_u32, c_void_p, c_void_p, c_u32, c_void_p, c_void_p]),
+    "dmsh_offset_array_aggregate": (
+        c_int, [c_void_p, c_u32, c_void_p, c_u32, c_void_p, c_u32, c_u32, c_void_p, c_void_p]),
 }
 
 

@@ -478,6 +478,66 @@ int dmsh_multiply_graph_laplacian(void *stream, uint32_t num_vtx, const uint32_t
                                   uint32_t num_idx, const uint32_t *idx2vtx, uint32_t num_dim,
                                   const float *vtx2rhs, float *vtx2lhs);
 
+/* ---------------------------------------------------------------------
+ * Quad/oct-tree from sorted unique Morton codes.  Replaces
+ * quad_oct_tree.cu:28-233 (binary_radix_tree_and_depth,
+ * bnode2isonode_and_idx2bnode, make_tree_from_binary_radix_tree, aggregate)
+ * as launched by del-msh-dlpack/src/quad_oct_tree.rs; results follow
+ * del-msh-cpu/src/quad_oct_tree.rs:1-230.  num_dim 2 or 3, num_dim *
+ * max_depth <= 32 (max_depth is 32 / num_dim by default in the reference).
+ * dmsh_octree_bnodes writes bnodes [num_idx-1][3], bnode2depth,
+ * bnode2onode [num_idx-1] (the caller reads bnode2onode[num_idx-2] + 1 =
+ * num_onode, as the reference does) and idx2bnode [num_idx];
+ * dmsh_octree_make_tree fills onodes [num_onode][1+2^num_dim] (entry 0 the
+ * parent, then the children; a point child is idx + num_onode; empty slots
+ * 0xFFFFFFFF), onode2depth, onode2center [num_onode][num_dim], idx2onode,
+ * idx2center [num_idx][num_dim].  dmsh_octree_aggregate writes every
+ * node's sum of idx2val over its points, added in ascending idx order
+ * (del-msh-cpu's order, so the same bits; the reference GPU adds with
+ * atomics in no fixed order).
+ * ------------------------------------------------------------------- */
+size_t dmsh_octree_bnodes_workspace_bytes(uint32_t num_idx);
+int dmsh_octree_bnodes(void *stream, uint32_t num_idx, const uint32_t *idx2morton,
+                       uint32_t num_dim, uint32_t max_depth, uint32_t *bnodes,
+                       uint32_t *bnode2depth, uint32_t *bnode2onode, uint32_t *idx2bnode, void *ws,
+                       size_t ws_bytes);
+int dmsh_octree_make_tree(void *stream, uint32_t num_idx, const uint32_t *bnodes,
+                          const uint32_t *bnode2onode, const uint32_t *bnode2depth,
+                          const uint32_t *idx2bnode, const uint32_t *idx2morton, uint32_t num_dim,
+                          uint32_t max_depth, uint32_t num_onode, uint32_t *onodes,
+                          uint32_t *onode2depth, float *onode2center, uint32_t *idx2onode,
+                          float *idx2center);
+size_t dmsh_octree_aggregate_workspace_bytes(uint32_t num_onode);
+int dmsh_octree_aggregate(void *stream, uint32_t num_idx, uint32_t num_vdim, const float *idx2val,
+                          const uint32_t *idx2onode, uint32_t num_link, uint32_t num_onode,
+                          const uint32_t *onodes, float *onode2aggval, void *ws,
+                          size_t ws_bytes);
+
+/* ---------------------------------------------------------------------
+ * Sorted-array and offset-array helpers.  Replace del-msh-dlpack/src/
+ * array1d.rs:156-300 (array1d_has_duplicate_sorted_array,
+ * array1d_unique_for_sorted_array, array1d_unique_jdx2val_jdx2idx; GPU side
+ * in the external del-cudarc) and offset_array.rs (offset_array_aggregate);
+ * results follow del-msh-cpu/src/array1d.rs:1-15 and map_idx.rs:83-115.
+ * dmsh_has_duplicate_sorted_u32 sets the device word *flag to 0 or 1.
+ * dmsh_unique_for_sorted_u32: idx2jdx[i] = number of value changes in
+ * idx2val[0..i].  dmsh_unique_jdx2val_jdx2idx: jdx2idx_offset [num_jdx+1],
+ * jdx2val [num_jdx].  dmsh_offset_array_aggregate adds, per idx, the rows
+ * kdx2val[jdx2kdx[j]] for j in [off[idx], off[idx+1]) in j order onto
+ * idx2aggval (the reference's wrapper zero-fills it first).
+ * ------------------------------------------------------------------- */
+int dmsh_has_duplicate_sorted_u32(void *stream, uint32_t n, const uint32_t *idx2val,
+                                  uint32_t *flag);
+size_t dmsh_unique_for_sorted_workspace_bytes(uint32_t n);
+int dmsh_unique_for_sorted_u32(void *stream, uint32_t n, const uint32_t *idx2val,
+                               uint32_t *idx2jdx, void *ws, size_t ws_bytes);
+int dmsh_unique_jdx2val_jdx2idx(void *stream, uint32_t n, const uint32_t *idx2val,
+                                const uint32_t *idx2jdx, uint32_t num_jdx, uint32_t *jdx2val,
+                                uint32_t *jdx2idx_offset);
+int dmsh_offset_array_aggregate(void *stream, uint32_t num_idx, const uint32_t *idx2jdx_offset,
+                                uint32_t num_jdx, const uint32_t *jdx2kdx, uint32_t num_kdx,
+                                uint32_t num_dim, const float *kdx2val, float *idx2aggval);
+
 #ifdef __cplusplus
 }
 #endif

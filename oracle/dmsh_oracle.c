@@ -1750,3 +1750,194 @@ void orc_interpolate_staggered_grid(uint32_t W, uint32_t H, const float *hedge2v
     }
   }
 }
+
+
+/* =========================================================================
+ * Quad/oct-tree (del-msh-cpu/src/quad_oct_tree.rs) and sorted-array helpers
+ * (array1d.rs:1-15, map_idx.rs:83-115, offset_array aggregate)
+ * ========================================================================= */
+
+/* quad_oct_tree.rs:1-49 binary_radix_tree_and_depth and :77-109
+ * bnode2onode_and_idx2bnode (idx2bnode must come filled with U32MAX, as the
+ * reference's wrapper does) */
+int orc_octree_bnodes(size_t num_idx, const uint32_t *idx2morton, uint32_t num_dim,
+                      uint32_t max_depth, uint32_t *bnodes, uint32_t *bnode2depth,
+                      uint32_t *bnode2onode, uint32_t *idx2bnode) {
+  if (num_idx < 2) return -1;
+  const size_t num_bnode = num_idx - 1;
+  bnodes[0] = U32MAX;
+  for (size_t ib = 0; ib < num_bnode; ++ib) {
+    size_t r0, r1;
+    orc_range_u32(idx2morton, num_idx, ib, &r0, &r1);
+    const size_t sp = orc_split_u32(idx2morton, r0, r1);
+    if (sp == (size_t)-1) return -2;
+    if (r0 == sp) {
+      bnodes[ib * 3 + 1] = (uint32_t)(num_bnode + sp);
+    } else {
+      bnodes[ib * 3 + 1] = (uint32_t)sp;
+      bnodes[sp * 3] = (uint32_t)ib;
+    }
+    if (r1 == sp + 1) {
+      bnodes[ib * 3 + 2] = (uint32_t)(num_bnode + sp + 1);
+    } else {
+      bnodes[ib * 3 + 2] = (uint32_t)(sp + 1);
+      bnodes[(sp + 1) * 3] = (uint32_t)ib;
+    }
+    const uint32_t delta = (uint32_t)clz32(idx2morton[sp] ^ idx2morton[sp + 1]);
+    const uint32_t offset = 32u - num_dim * max_depth;
+    bnode2depth[ib] = (delta - offset) / num_dim;
+  }
+  uint32_t *isonode = (uint32_t *)calloc(num_bnode, sizeof(uint32_t));
+  if (!isonode) return -3;
+  for (size_t ib = 0; ib < num_bnode; ++ib) {
+    if (bnodes[ib * 3 + 1] >= num_bnode) idx2bnode[bnodes[ib * 3 + 1] - num_bnode] = (uint32_t)ib;
+    if (bnodes[ib * 3 + 2] >= num_bnode) idx2bnode[bnodes[ib * 3 + 2] - num_bnode] = (uint32_t)ib;
+    if (ib == 0) continue;
+    const uint32_t p = bnodes[ib * 3];
+    if (bnode2depth[ib] != bnode2depth[p]) isonode[ib - 1] = 1;
+  }
+  bnode2onode[0] = 0;
+  for (size_t ib = 1; ib < num_bnode; ++ib) bnode2onode[ib] = bnode2onode[ib - 1] + isonode[ib - 1];
+  free(isonode);
+  return 0;
+}
+
+/* quad_oct_tree.rs:51-63 */
+static void morton2center(uint32_t morton, uint32_t num_dim, uint32_t depth, uint32_t max_depth,
+                          float *center) {
+  const uint32_t sh = (max_depth - depth) * num_dim;
+  uint32_t key = sh < 32u ? morton >> sh : 0u;
+  for (uint32_t d = 0; d < num_dim; ++d) center[d] = 0.5f;
+  for (uint32_t k = 0; k < depth; ++k)
+    for (uint32_t d = 0; d < num_dim; ++d) {
+      const uint32_t j = num_dim - d - 1;
+      center[j] += (float)(key & 1u);
+      center[j] *= 0.5f;
+      key >>= 1;
+    }
+}
+
+void orc_morton2center(uint32_t morton, uint32_t num_dim, uint32_t depth, uint32_t max_depth,
+                       float *center) {
+  morton2center(morton, num_dim, depth, max_depth, center);
+}
+
+/* quad_oct_tree.rs:111-227 make_tree_from_binary_radix_tree (onodes come
+ * filled with U32MAX and onode2depth with 0, as the reference's wrapper
+ * does); returns -1 on a duplicated child (the reference's assert) */
+int orc_octree_make_tree(size_t num_vtx, const uint32_t *bnodes, const uint32_t *bnode2onode,
+                         const uint32_t *bnode2depth, const uint32_t *idx2bnode,
+                         const uint32_t *idx2morton, size_t num_onode, uint32_t max_depth,
+                         uint32_t num_dim, uint32_t *onodes, uint32_t *onode2depth,
+                         float *onode2center, uint32_t *idx2onode, float *idx2center) {
+  const uint32_t num_child = 1u << num_dim;
+  const size_t nlink = num_child + 1;
+  for (size_t i = 0; i < num_vtx; ++i) {
+    {
+      size_t cur = idx2bnode[i];
+      uint32_t op, dp;
+      for (;;) {
+        if (cur == 0 || bnode2onode[cur] != bnode2onode[cur - 1]) {
+          op = bnode2onode[cur];
+          dp = bnode2depth[cur];
+          break;
+        }
+        cur = bnodes[cur * 3];
+      }
+      if (dp >= max_depth) return -2;
+      const uint32_t key = idx2morton[i];
+      const uint32_t c = (key >> ((max_depth - 1 - dp) * num_dim)) & (num_child - 1);
+      if (onodes[op * nlink + 1 + c] != U32MAX) return -1;
+      onodes[op * nlink + 1 + c] = (uint32_t)(i + num_onode);
+      idx2onode[i] = op;
+      float center[3];
+      morton2center(key, num_dim, max_depth, max_depth, center);
+      for (uint32_t d = 0; d < num_dim; ++d) idx2center[i * num_dim + d] = center[d];
+    }
+    if (i + 1 < num_vtx) {
+      const size_t ib = i;
+      if (ib == 0) {
+        for (uint32_t d = 0; d < num_dim; ++d) onode2center[d] = 0.5f;
+        continue;
+      }
+      if (bnode2onode[ib - 1] == bnode2onode[ib]) continue;
+      size_t par = bnodes[ib * 3];
+      uint32_t op, dp;
+      for (;;) {
+        if (par == 0 || bnode2onode[par] != bnode2onode[par - 1]) {
+          op = bnode2onode[par];
+          dp = bnode2depth[par];
+          break;
+        }
+        par = bnodes[par * 3];
+      }
+      if (dp >= max_depth) return -2;
+      const uint32_t morton = idx2morton[ib];
+      const uint32_t c = (morton >> ((max_depth - 1 - dp) * num_dim)) & (num_child - 1);
+      if (onodes[op * nlink + 1 + c] != U32MAX) return -1;
+      const uint32_t o = bnode2onode[ib];
+      onodes[op * nlink + 1 + c] = o;
+      onodes[(size_t)o * nlink] = op;
+      const uint32_t depth = bnode2depth[ib];
+      float center[3];
+      morton2center(morton, num_dim, depth, max_depth, center);
+      for (uint32_t d = 0; d < num_dim; ++d) onode2center[(size_t)o * num_dim + d] = center[d];
+      onode2depth[o] = depth;
+    }
+  }
+  return 0;
+}
+
+/* quad_oct_tree.rs:420-451 aggregate: per point, climb and add (parent link
+ * = entry 0 of num_link links) */
+void orc_octree_aggregate(size_t num_idx, uint32_t num_vdim, const float *idx2val,
+                          const uint32_t *idx2onode, uint32_t num_link, size_t num_onode,
+                          const uint32_t *onodes, float *onode2aggval) {
+  memset(onode2aggval, 0, sizeof(float) * num_onode * num_vdim);
+  for (size_t idx = 0; idx < num_idx; ++idx) {
+    uint32_t o = idx2onode[idx];
+    while (o != U32MAX && o < num_onode) {
+      for (uint32_t d = 0; d < num_vdim; ++d)
+        onode2aggval[(size_t)o * num_vdim + d] += idx2val[idx * num_vdim + d];
+      o = onodes[(size_t)o * num_link];
+    }
+  }
+}
+
+/* array1d.rs:1-15 */
+void orc_unique_for_sorted(size_t n, const uint32_t *val, uint32_t *idx2jdx) {
+  if (n == 0) return;
+  idx2jdx[0] = 0;
+  for (size_t i = 0; i + 1 < n; ++i) idx2jdx[i + 1] = val[i] == val[i + 1] ? 0u : 1u;
+  for (size_t i = 0; i + 1 < n; ++i) idx2jdx[i + 1] += idx2jdx[i];
+}
+
+/* map_idx.rs:83-115 inverse (bisection), then jdx2val[j] = val[offset[j]] */
+void orc_unique_jdx2val_jdx2idx(size_t n, const uint32_t *val, const uint32_t *idx2jdx,
+                                size_t num_jdx, uint32_t *jdx2val, uint32_t *jdx2idx_offset) {
+  for (size_t j = 0; j <= num_jdx; ++j) {
+    size_t i0 = 0, i2 = n;
+    uint32_t res;
+    for (;;) {
+      if (i2 - i0 == 1) {
+        res = (size_t)idx2jdx[i0] >= j ? (uint32_t)i0 : (uint32_t)i2;
+        break;
+      }
+      const size_t i1 = (i0 + i2) / 2;
+      if ((size_t)idx2jdx[i1] < j) i0 = i1;
+      else i2 = i1;
+    }
+    jdx2idx_offset[j] = res;
+  }
+  for (size_t j = 0; j < num_jdx; ++j) jdx2val[j] = val[jdx2idx_offset[j]];
+}
+
+/* del-msh-dlpack/src/offset_array.rs (CPU branch): accumulates */
+void orc_offset_array_aggregate(size_t num_idx, const uint32_t *off, const uint32_t *jdx2kdx,
+                                uint32_t num_dim, const float *kdx2val, float *idx2aggval) {
+  for (size_t i = 0; i < num_idx; ++i)
+    for (uint32_t j = off[i]; j < off[i + 1]; ++j) {
+      const size_t k = jdx2kdx[j];
+      for (uint32_t d = 0; d < num_dim; ++d) idx2aggval[i * num_dim + d] += kdx2val[k * num_dim + d];
+    }
+}

@@ -244,6 +244,26 @@ void orc_multiply_graph_laplacian(size_t num_vtx, const uint32_t *vtx2idx,
                                   const uint32_t *idx2vtx, size_t num_dim, const float *vtx2rhs,
                                   float *vtx2lhs);
 
+/* quad/oct-tree (del-msh-cpu/src/quad_oct_tree.rs) and sorted-array helpers */
+int orc_octree_bnodes(size_t num_idx, const uint32_t *idx2morton, uint32_t num_dim,
+                      uint32_t max_depth, uint32_t *bnodes, uint32_t *bnode2depth,
+                      uint32_t *bnode2onode, uint32_t *idx2bnode);
+void orc_morton2center(uint32_t morton, uint32_t num_dim, uint32_t depth, uint32_t max_depth,
+                       float *center);
+int orc_octree_make_tree(size_t num_vtx, const uint32_t *bnodes, const uint32_t *bnode2onode,
+                         const uint32_t *bnode2depth, const uint32_t *idx2bnode,
+                         const uint32_t *idx2morton, size_t num_onode, uint32_t max_depth,
+                         uint32_t num_dim, uint32_t *onodes, uint32_t *onode2depth,
+                         float *onode2center, uint32_t *idx2onode, float *idx2center);
+void orc_octree_aggregate(size_t num_idx, uint32_t num_vdim, const float *idx2val,
+                          const uint32_t *idx2onode, uint32_t num_link, size_t num_onode,
+                          const uint32_t *onodes, float *onode2aggval);
+void orc_unique_for_sorted(size_t n, const uint32_t *val, uint32_t *idx2jdx);
+void orc_unique_jdx2val_jdx2idx(size_t n, const uint32_t *val, const uint32_t *idx2jdx,
+                                size_t num_jdx, uint32_t *jdx2val, uint32_t *jdx2idx_offset);
+void orc_offset_array_aggregate(size_t num_idx, const uint32_t *off, const uint32_t *jdx2kdx,
+                                uint32_t num_dim, const float *kdx2val, float *idx2aggval);
+
 #ifdef __cplusplus
 }
 #endif

@@ -592,3 +592,110 @@ def interpolate_staggered_grid(hedge2vy, vedge2vx, vtx2xy):
     L.orc_interpolate_staggered_grid(w, h, _f32(hedge2vy).reshape(-1), _f32(vedge2vx).reshape(-1),
                                      xy.shape[0], xy.reshape(-1), out.reshape(-1))
     return out
+
+
+# ---------------------------------------------------------------------------
+# quad/oct-tree and sorted-array helpers
+# ---------------------------------------------------------------------------
+def _sig_octree():
+    L = lib()
+    if getattr(L, "_octree_ready", False):
+        return L
+    u32 = ctypes.c_uint32
+    L.orc_octree_bnodes.restype = ctypes.c_int
+    L.orc_octree_bnodes.argtypes = [_sz, _u32p, u32, u32, _u32p, _u32p, _u32p, _u32p]
+    L.orc_morton2center.restype = None
+    L.orc_morton2center.argtypes = [u32, u32, u32, u32, _f32p]
+    L.orc_octree_make_tree.restype = ctypes.c_int
+    L.orc_octree_make_tree.argtypes = [_sz, _u32p, _u32p, _u32p, _u32p, _u32p, _sz, u32, u32,
+                                       _u32p, _u32p, _f32p, _u32p, _f32p]
+    L.orc_octree_aggregate.restype = None
+    L.orc_octree_aggregate.argtypes = [_sz, u32, _f32p, _u32p, u32, _sz, _u32p, _f32p]
+    L.orc_unique_for_sorted.restype = None
+    L.orc_unique_for_sorted.argtypes = [_sz, _u32p, _u32p]
+    L.orc_unique_jdx2val_jdx2idx.restype = None
+    L.orc_unique_jdx2val_jdx2idx.argtypes = [_sz, _u32p, _u32p, _sz, _u32p, _u32p]
+    L.orc_offset_array_aggregate.restype = None
+    L.orc_offset_array_aggregate.argtypes = [_sz, _u32p, _u32p, u32, _f32p, _f32p]
+    L._octree_ready = True
+    return L
+
+
+def octree_bnodes(idx2morton, num_dim, max_depth=None):
+    """(bnodes [n-1,3], bnode2depth, bnode2onode, idx2bnode) of
+    quad_oct_tree.rs:1-109."""
+    L = _sig_octree()
+    if max_depth is None:
+        max_depth = 32 // num_dim
+    m = _u32(idx2morton)
+    n = m.shape[0]
+    bnodes = np.zeros((n - 1, 3), np.uint32)
+    depth = np.zeros(n - 1, np.uint32)
+    b2o = np.zeros(n - 1, np.uint32)
+    i2b = np.full(n, 0xFFFFFFFF, np.uint32)
+    if L.orc_octree_bnodes(n, m, num_dim, max_depth, bnodes.reshape(-1), depth, b2o, i2b) != 0:
+        raise ValueError("binary radix tree failed")
+    return bnodes, depth, b2o, i2b
+
+
+def morton2center(morton, num_dim, depth, max_depth):
+    c = np.zeros(3, np.float32)
+    _sig_octree().orc_morton2center(int(morton), num_dim, depth, max_depth, c)
+    return c[:num_dim]
+
+
+def octree_make_tree(bnodes, bnode2onode, bnode2depth, idx2bnode, idx2morton, num_dim,
+                     max_depth=None):
+    """(onodes, onode2depth, onode2center, idx2onode, idx2center) of
+    quad_oct_tree.rs:111-227."""
+    L = _sig_octree()
+    if max_depth is None:
+        max_depth = 32 // num_dim
+    n = _u32(idx2morton).shape[0]
+    num_onode = int(bnode2onode[-1]) + 1
+    nlink = 1 + 2 ** num_dim
+    onodes = np.full((num_onode, nlink), 0xFFFFFFFF, np.uint32)
+    o2d = np.zeros(num_onode, np.uint32)
+    o2c = np.zeros((num_onode, num_dim), np.float32)
+    i2o = np.zeros(n, np.uint32)
+    i2c = np.zeros((n, num_dim), np.float32)
+    rc = L.orc_octree_make_tree(n, _u32(bnodes).reshape(-1), _u32(bnode2onode), _u32(bnode2depth),
+                                _u32(idx2bnode), _u32(idx2morton), num_onode, max_depth, num_dim,
+                                onodes.reshape(-1), o2d, o2c.reshape(-1), i2o, i2c.reshape(-1))
+    if rc != 0:
+        raise ValueError("octree construction failed (duplicated child)")
+    return onodes, o2d, o2c, i2o, i2c
+
+
+def octree_aggregate(idx2val, idx2onode, onodes):
+    L = _sig_octree()
+    v = _f32(idx2val)
+    on = _u32(onodes)
+    out = np.zeros((on.shape[0], v.shape[1]), np.float32)
+    L.orc_octree_aggregate(v.shape[0], v.shape[1], v.reshape(-1), _u32(idx2onode), on.shape[1],
+                           on.shape[0], on.reshape(-1), out.reshape(-1))
+    return out
+
+
+def unique_for_sorted(idx2val):
+    """(idx2jdx, jdx2val, jdx2idx_offset) of array1d.rs:1-15 + map_idx.rs:83-115."""
+    L = _sig_octree()
+    v = _u32(idx2val)
+    n = v.shape[0]
+    idx2jdx = np.zeros(n, np.uint32)
+    L.orc_unique_for_sorted(n, v, idx2jdx)
+    num_jdx = int(idx2jdx[-1]) + 1 if n else 0
+    jdx2val = np.zeros(num_jdx, np.uint32)
+    off = np.zeros(num_jdx + 1, np.uint32)
+    L.orc_unique_jdx2val_jdx2idx(n, v, idx2jdx, num_jdx, jdx2val, off)
+    return idx2jdx, jdx2val, off
+
+
+def offset_array_aggregate(idx2jdx_offset, jdx2kdx, kdx2val):
+    L = _sig_octree()
+    off = _u32(idx2jdx_offset)
+    v = _f32(kdx2val)
+    out = np.zeros((off.shape[0] - 1, v.shape[1]), np.float32)
+    L.orc_offset_array_aggregate(off.shape[0] - 1, off, _u32(jdx2kdx), v.shape[1], v.reshape(-1),
+                                 out.reshape(-1))
+    return out
