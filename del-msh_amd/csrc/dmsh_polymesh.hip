@@ -1,0 +1,92 @@
+// dmsh_polymesh.hip -- C ABI: mixed polyhedron meshes in CSR form (element
+// centroids, element-BVH refit, point location, interpolation at points).
+#include <algorithm>
+
+#include "dmsh.h"
+#include "polymesh.hpp"
+#include "refit.hpp"
+#include "sort.hpp"
+
+using namespace dmsh;
+
+namespace {
+inline dim3 grid_for(uint64_t n) { return dim3((uint32_t)std::max<uint64_t>(1, ceil_div(n, 256))); }
+}  // namespace
+
+extern "C" {
+
+int dmsh_polyhedron_elem2center(void* stream, uint32_t num_elem, const uint32_t* elem2idx_offset,
+                                uint32_t num_idx, const uint32_t* idx2vtx, uint32_t num_vtx,
+                                uint32_t num_dim, const float* vtx2xyz, float* elem2center) {
+  if (num_elem == 0) return DMSH_OK;
+  if (num_dim == 0 || num_dim > 4 || !elem2idx_offset || !elem2center ||
+      (num_idx > 0 && (!idx2vtx || !vtx2xyz)))
+    return DMSH_ERR_INVALID;
+  launch("k_elem2center", poly::k_elem2center, grid_for(num_elem), dim3(256), 0,
+         as_stream(stream), num_elem, elem2idx_offset, num_idx, idx2vtx, num_vtx, num_dim,
+         vtx2xyz, elem2center, device_error_word());
+  return launch_status();
+}
+
+size_t dmsh_polyhedron_refit_workspace_bytes(uint32_t num_elem) {
+  return sizeof(uint32_t) * (size_t)(num_elem > 1 ? num_elem - 1 : 1) + 256;
+}
+
+int dmsh_bvhnode2aabb_polyhedron(void* stream, uint32_t num_bvhnode, const uint32_t* bvhnodes,
+                                 uint32_t num_elem, const uint32_t* elem2idx_offset,
+                                 uint32_t num_idx, const uint32_t* idx2vtx, uint32_t num_vtx,
+                                 const float* vtx2xyz, float* bvhnode2aabb, void* ws,
+                                 size_t ws_bytes) {
+  if (num_elem == 0) return num_bvhnode == 0 ? DMSH_OK : DMSH_ERR_INVALID;
+  if (num_elem >= (1u << 31) || num_bvhnode != 2 * num_elem - 1) return DMSH_ERR_INVALID;
+  if (!bvhnodes || !elem2idx_offset || !idx2vtx || !vtx2xyz || !bvhnode2aabb)
+    return DMSH_ERR_INVALID;
+  if (!ws || ws_bytes < dmsh_polyhedron_refit_workspace_bytes(num_elem))
+    return DMSH_ERR_WORKSPACE;
+  hipStream_t s = as_stream(stream);
+  uint32_t* flags = static_cast<uint32_t*>(ws);
+  int rc = sortk::launch_fill(s, flags, num_elem > 1 ? num_elem - 1 : 1, 0u);
+  if (rc != DMSH_OK) return rc;
+  launch("k_refit_csr", k_refit_csr, grid_for(num_elem), dim3(256), 0, s, bvhnode2aabb, flags,
+         num_elem, bvhnodes, elem2idx_offset, num_idx, idx2vtx, num_vtx, vtx2xyz,
+         device_error_word());
+  return launch_status();
+}
+
+int dmsh_polyhedron_search_elem_contain_points(void* stream, uint32_t num_elem,
+                                               const uint32_t* elem2idx_offset, uint32_t num_idx,
+                                               const uint32_t* idx2vtx, uint32_t num_vtx,
+                                               const float* vtx2xyz, const uint32_t* bvhnodes,
+                                               const float* bvhnode2aabb, uint32_t num_wtx,
+                                               const float* wtx2xyz, uint32_t* wtx2elem,
+                                               float* wtx2param) {
+  if (num_wtx == 0) return DMSH_OK;
+  if (!wtx2xyz || !wtx2elem || !wtx2param || !elem2idx_offset ||
+      (num_elem > 0 && (!bvhnodes || !bvhnode2aabb || !idx2vtx || !vtx2xyz)))
+    return DMSH_ERR_INVALID;
+  if (num_elem >= (1u << 31)) return DMSH_ERR_INVALID;
+  const poly::MeshView m{num_elem, elem2idx_offset, num_idx, idx2vtx, num_vtx, vtx2xyz};
+  launch("k_search_elem_contain_points", poly::k_search_elem_contain_points, grid_for(num_wtx),
+         dim3(256), 0, as_stream(stream), m, bvhnodes, bvhnode2aabb, num_wtx, wtx2xyz, wtx2elem,
+         wtx2param, device_error_word());
+  return launch_status();
+}
+
+int dmsh_polyhedron_interpolate_at_points(void* stream, uint32_t num_wtx, uint32_t num_elem,
+                                          const uint32_t* elem2idx_offset, uint32_t num_idx,
+                                          const uint32_t* idx2vtx, uint32_t num_vtx,
+                                          uint32_t num_vdim, const float* vtx2val,
+                                          const uint32_t* wtx2elem, const float* wtx2param,
+                                          float* wtx2val) {
+  if (num_wtx == 0 || num_vdim == 0) return DMSH_OK;
+  if (!wtx2elem || !wtx2param || !wtx2val || !elem2idx_offset ||
+      (num_elem > 0 && (!idx2vtx || !vtx2val)))
+    return DMSH_ERR_INVALID;
+  const poly::MeshView m{num_elem, elem2idx_offset, num_idx, idx2vtx, num_vtx, nullptr};
+  launch("k_interpolate_at_points", poly::k_interpolate_at_points, grid_for(num_wtx), dim3(256),
+         0, as_stream(stream), m, num_wtx, num_vdim, vtx2val, wtx2elem, wtx2param, wtx2val,
+         device_error_word());
+  return launch_status();
+}
+
+}  // extern "C"

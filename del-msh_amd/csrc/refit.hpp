@@ -66,40 +66,12 @@ __device__ __forceinline__ void box_store(float* __restrict__ aabb, uint32_t nod
   p[2] = make_float2(b.v[4], b.v[5]);
 }
 
-// TRI: leaves index triangles of tri2vtx; otherwise leaves index points.
-template <bool TRI>
-__global__ __launch_bounds__(256) void k_refit(float* __restrict__ aabb,
-                                               uint32_t* __restrict__ flags, uint32_t num_leaf,
-                                               const uint32_t* __restrict__ bvhnodes,
-                                               const uint32_t* __restrict__ tri2vtx,
-                                               uint32_t num_vtx, const float* __restrict__ xyz,
-                                               float eps, uint32_t* __restrict__ err) {
-  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= num_leaf) return;
-  const uint32_t nb = num_leaf - 1;
-  uint32_t node = nb + k;
-  const uint32_t* row = bvhnodes + (size_t)node * 3;
-  const uint32_t parent0 = row[0], obj = row[1], tag = row[2];
-  if (tag != DMSH_U32MAX || obj >= (TRI ? num_leaf : num_vtx)) {
-    raise_error(err, DMSH_DEVERR_INDEX);
-    return;
-  }
-  Box mine;
-  if (TRI) {
-    const uint32_t* tv = tri2vtx + (size_t)obj * 3;
-    const uint32_t i0 = tv[0], i1 = tv[1], i2 = tv[2];
-    if (i0 >= num_vtx || i1 >= num_vtx || i2 >= num_vtx) {
-      raise_error(err, DMSH_DEVERR_INDEX);
-      return;
-    }
-    box_set(mine, xyz + (size_t)i0 * 3, eps);
-    box_add(mine, xyz + (size_t)i1 * 3, eps);
-    box_add(mine, xyz + (size_t)i2 * 3, eps);
-  } else {
-    box_set(mine, xyz + (size_t)obj * 3, 0.f);
-  }
-  box_store(aabb, node, mine);
-  uint32_t parent = parent0;
+// From leaf row `node` (whose box `mine` is stored), climb through the
+// parents' arrival counters; the second arrival merges in (left, right) order.
+__device__ __forceinline__ void refit_climb(float* __restrict__ aabb, uint32_t* __restrict__ flags,
+                                            uint32_t nb, const uint32_t* __restrict__ bvhnodes,
+                                            uint32_t node, uint32_t parent, Box mine,
+                                            uint32_t* __restrict__ err) {
   // A climb visits at most nb internal nodes; the bound keeps a corrupt
   // (cyclic) parent chain from looping forever.
   for (uint32_t steps = 0; parent != DMSH_U32MAX; ++steps) {
@@ -128,6 +100,83 @@ __global__ __launch_bounds__(256) void k_refit(float* __restrict__ aabb,
     node = parent;
     parent = prow[0];
   }
+}
+
+// TRI: leaves index triangles of tri2vtx; otherwise leaves index points.
+template <bool TRI>
+__global__ __launch_bounds__(256) void k_refit(float* __restrict__ aabb,
+                                               uint32_t* __restrict__ flags, uint32_t num_leaf,
+                                               const uint32_t* __restrict__ bvhnodes,
+                                               const uint32_t* __restrict__ tri2vtx,
+                                               uint32_t num_vtx, const float* __restrict__ xyz,
+                                               float eps, uint32_t* __restrict__ err) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= num_leaf) return;
+  const uint32_t nb = num_leaf - 1;
+  const uint32_t node = nb + k;
+  const uint32_t* row = bvhnodes + (size_t)node * 3;
+  const uint32_t parent0 = row[0], obj = row[1], tag = row[2];
+  if (tag != DMSH_U32MAX || obj >= (TRI ? num_leaf : num_vtx)) {
+    raise_error(err, DMSH_DEVERR_INDEX);
+    return;
+  }
+  Box mine;
+  if (TRI) {
+    const uint32_t* tv = tri2vtx + (size_t)obj * 3;
+    const uint32_t i0 = tv[0], i1 = tv[1], i2 = tv[2];
+    if (i0 >= num_vtx || i1 >= num_vtx || i2 >= num_vtx) {
+      raise_error(err, DMSH_DEVERR_INDEX);
+      return;
+    }
+    box_set(mine, xyz + (size_t)i0 * 3, eps);
+    box_add(mine, xyz + (size_t)i1 * 3, eps);
+    box_add(mine, xyz + (size_t)i2 * 3, eps);
+  } else {
+    box_set(mine, xyz + (size_t)obj * 3, 0.f);
+  }
+  box_store(aabb, node, mine);
+  refit_climb(aabb, flags, nb, bvhnodes, node, parent0, mine, err);
+}
+
+// Leaves index elements of a CSR polygon / polyhedron mesh: the box of the
+// element's vertices, first vertex first (bvhnode2aabb3.rs:211-228 with
+// margin 0; bvhnode2aabb.cu:67-122).  An empty element or an index out of
+// range sets DMSH_DEVERR_INDEX.
+static __global__ __launch_bounds__(256) void k_refit_csr(float* __restrict__ aabb,
+                                                   uint32_t* __restrict__ flags, uint32_t num_elem,
+                                                   const uint32_t* __restrict__ bvhnodes,
+                                                   const uint32_t* __restrict__ elem2idx,
+                                                   uint32_t num_idx,
+                                                   const uint32_t* __restrict__ idx2vtx,
+                                                   uint32_t num_vtx, const float* __restrict__ xyz,
+                                                   uint32_t* __restrict__ err) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= num_elem) return;
+  const uint32_t nb = num_elem - 1;
+  const uint32_t node = nb + k;
+  const uint32_t* row = bvhnodes + (size_t)node * 3;
+  const uint32_t parent0 = row[0], obj = row[1], tag = row[2];
+  if (tag != DMSH_U32MAX || obj >= num_elem) {
+    raise_error(err, DMSH_DEVERR_INDEX);
+    return;
+  }
+  const uint32_t i0 = elem2idx[obj], i1 = elem2idx[obj + 1];
+  if (i0 >= i1 || i1 > num_idx) {
+    raise_error(err, DMSH_DEVERR_INDEX);
+    return;
+  }
+  Box mine;
+  for (uint32_t i = i0; i < i1; ++i) {
+    const uint32_t v = idx2vtx[i];
+    if (v >= num_vtx) {
+      raise_error(err, DMSH_DEVERR_INDEX);
+      return;
+    }
+    if (i == i0) box_set(mine, xyz + (size_t)v * 3, 0.f);
+    else box_add(mine, xyz + (size_t)v * 3, 0.f);
+  }
+  box_store(aabb, node, mine);
+  refit_climb(aabb, flags, nb, bvhnodes, node, parent0, mine, err);
 }
 
 // One tree (one mesh of a batch): node rows and boxes, and the leaves'

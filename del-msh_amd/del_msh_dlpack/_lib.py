@@ -170,6 +170,18 @@ _PROTOTYPES = {
     "dmsh_unique_for_sorted_u32": (c_int, [c_void_p, c_u32, c_void_p, c_void_p, c_void_p, c_size]),
     "dmsh_unique_jdx2val_jdx2idx": (
         c_int, [c_void_p, c_u32, c_void_p, c_void_p, c_u32, c_void_p, c_void_p]),
+    "dmsh_polyhedron_elem2center": (
+        c_int, [c_void_p, c_u32, c_void_p, c_u32, c_void_p, c_u32, c_u32, c_void_p, c_void_p]),
+    "dmsh_polyhedron_refit_workspace_bytes": (c_size, [c_u32]),
+    "dmsh_bvhnode2aabb_polyhedron": (
+        c_int, [c_void_p, c_u32, c_void_p, c_u32, c_void_p, c_u32, c_void_p, c_u32, c_void_p,
+                c_void_p, c_void_p, c_size]),
+    "dmsh_polyhedron_search_elem_contain_points": (
+        c_int, [c_void_p, c_u32, c_void_p, c_u32, c_void_p, c_u32, c_void_p, c_void_p, c_void_p,
+                c_u32, c_void_p, c_void_p, c_void_p]),
+    "dmsh_polyhedron_interpolate_at_points": (
+        c_int, [c_void_p, c_u32, c_u32, c_void_p, c_u32, c_void_p, c_u32, c_u32, c_void_p,
+                c_void_p, c_void_p, c_void_p]),
     "dmsh_offset_array_aggregate": (
         c_int, [c_void_p, c_u32, c_void_p, c_u32, c_void_p, c_u32, c_u32, c_void_p, c_void_p]),
 }

@@ -236,3 +236,64 @@ def batch_meshes(num_mesh: int, seed: int = 3, tri_range=(256, 262144),
         "ndc2world": np.stack(cams) if cams else np.zeros((0, 4, 4), np.float32),
         "kinds": np.array(kinds, np.uint8),
     }
+
+
+def mixed_hex_grid(nx: int, ny: int, nz: int, seed: int = 0, jitter: float = 0.0,
+                   kinds=(0, 1, 2, 3)):
+    """A mixed tet / pyramid / prism / hex mesh of the box [0, nx] x [0, ny] x
+    [0, nz] in CSR form: each unit cube, with a kind drawn from `kinds`
+    (seeded), becomes one hex (0), two prisms (1), six pyramids around the
+    cube's centre (2) or six tets around its main diagonal (3).  With jitter
+    > 0 the grid vertices move by U(-jitter, jitter) per axis (hexes then
+    become trilinear, and cubes of different kinds no longer tile exactly).
+    Node orders are those of the shape functions of the reference's
+    polyhedron_mesh.cu.  Returns (elem2idx_offset u32, idx2vtx u32,
+    vtx2xyz f32 [V, 3])."""
+    rng = np.random.default_rng(seed)
+    gx, gy, gz = np.meshgrid(np.arange(nx + 1), np.arange(ny + 1), np.arange(nz + 1),
+                             indexing="ij")
+    vtx = np.stack([gx.ravel(), gy.ravel(), gz.ravel()], 1).astype(np.float64)
+    if jitter > 0:
+        vtx = vtx + rng.uniform(-jitter, jitter, vtx.shape)
+
+    def vid(i, j, k):
+        return (i * (ny + 1) + j) * (nz + 1) + k
+
+    extra = []
+    elems = []
+    cube_kind = rng.choice(np.asarray(kinds), size=(nx, ny, nz))
+    for i in range(nx):
+        for j in range(ny):
+            for k in range(nz):
+                c = {(a, b, d): vid(i + a, j + b, k + d) for a in (0, 1) for b in (0, 1)
+                     for d in (0, 1)}
+                kind = int(cube_kind[i, j, k])
+                if kind == 0:
+                    elems.append([c[0, 0, 0], c[1, 0, 0], c[1, 1, 0], c[0, 1, 0],
+                                  c[0, 0, 1], c[1, 0, 1], c[1, 1, 1], c[0, 1, 1]])
+                elif kind == 1:
+                    elems.append([c[0, 0, 0], c[1, 0, 0], c[0, 1, 0], c[0, 0, 1], c[1, 0, 1],
+                                  c[0, 1, 1]])
+                    elems.append([c[1, 1, 0], c[0, 1, 0], c[1, 0, 0], c[1, 1, 1], c[0, 1, 1],
+                                  c[1, 0, 1]])
+                elif kind == 2:
+                    ctr = len(vtx) + len(extra)
+                    extra.append(np.mean([vtx[v] for v in c.values()], axis=0))
+                    for face in ((c[0, 0, 0], c[1, 0, 0], c[1, 1, 0], c[0, 1, 0]),
+                                 (c[0, 0, 1], c[1, 0, 1], c[1, 1, 1], c[0, 1, 1]),
+                                 (c[0, 0, 0], c[1, 0, 0], c[1, 0, 1], c[0, 0, 1]),
+                                 (c[0, 1, 0], c[1, 1, 0], c[1, 1, 1], c[0, 1, 1]),
+                                 (c[0, 0, 0], c[0, 1, 0], c[0, 1, 1], c[0, 0, 1]),
+                                 (c[1, 0, 0], c[1, 1, 0], c[1, 1, 1], c[1, 0, 1])):
+                        elems.append(list(face) + [ctr])
+                else:
+                    a, h = c[0, 0, 0], c[1, 1, 1]
+                    ring = (c[1, 0, 0], c[1, 1, 0], c[0, 1, 0], c[0, 1, 1], c[0, 0, 1], c[1, 0, 1])
+                    for m in range(6):
+                        elems.append([a, ring[m], ring[(m + 1) % 6], h])
+    if extra:
+        vtx = np.concatenate([vtx, np.array(extra)], 0)
+    off = np.zeros(len(elems) + 1, np.uint32)
+    off[1:] = np.cumsum([len(e) for e in elems])
+    idx2vtx = np.concatenate([np.asarray(e, np.uint32) for e in elems])
+    return off, idx2vtx, vtx.astype(np.float32)

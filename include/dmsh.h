@@ -538,6 +538,43 @@ int dmsh_offset_array_aggregate(void *stream, uint32_t num_idx, const uint32_t *
                                 uint32_t num_jdx, const uint32_t *jdx2kdx, uint32_t num_kdx,
                                 uint32_t num_dim, const float *kdx2val, float *idx2aggval);
 
+/* ---------------------------------------------------------------------
+ * Mixed polyhedron meshes in CSR form (elem2idx_offset [num_elem+1],
+ * idx2vtx [num_idx]; tet 4, pyramid 5, prism 6, hex 8 nodes).  Replaces
+ * elem2center.cu (from_polygon_mesh_as_points), bvhnode2aabb.cu:67-122
+ * (from_polyhedron_mesh) and polyhedron_mesh.cu:49-177
+ * (search_elem_contain_points, interpolate_values_at_points) as launched by
+ * del-msh-dlpack/src/polyhedron_mesh.rs; results follow del-msh-cpu/src/
+ * elem2center.rs:59-89, bvhnode2aabb3.rs:197-259 and polyhedron_mesh.rs:
+ * 79-155,470-575,868-907.  Point location walks the element BVH left child
+ * first and reports the first element containing the point (wtx2elem
+ * 0xFFFFFFFF and zero parameters if none).  The pyramid / prism / hex
+ * parametric coordinates are those of a fixed Newton iteration (del-geo's
+ * own solver is not available), shared with the test oracle.
+ * ------------------------------------------------------------------- */
+int dmsh_polyhedron_elem2center(void *stream, uint32_t num_elem, const uint32_t *elem2idx_offset,
+                                uint32_t num_idx, const uint32_t *idx2vtx, uint32_t num_vtx,
+                                uint32_t num_dim, const float *vtx2xyz, float *elem2center);
+size_t dmsh_polyhedron_refit_workspace_bytes(uint32_t num_elem);
+int dmsh_bvhnode2aabb_polyhedron(void *stream, uint32_t num_bvhnode, const uint32_t *bvhnodes,
+                                 uint32_t num_elem, const uint32_t *elem2idx_offset,
+                                 uint32_t num_idx, const uint32_t *idx2vtx, uint32_t num_vtx,
+                                 const float *vtx2xyz, float *bvhnode2aabb, void *ws,
+                                 size_t ws_bytes);
+int dmsh_polyhedron_search_elem_contain_points(void *stream, uint32_t num_elem,
+                                               const uint32_t *elem2idx_offset, uint32_t num_idx,
+                                               const uint32_t *idx2vtx, uint32_t num_vtx,
+                                               const float *vtx2xyz, const uint32_t *bvhnodes,
+                                               const float *bvhnode2aabb, uint32_t num_wtx,
+                                               const float *wtx2xyz, uint32_t *wtx2elem,
+                                               float *wtx2param);
+int dmsh_polyhedron_interpolate_at_points(void *stream, uint32_t num_wtx, uint32_t num_elem,
+                                          const uint32_t *elem2idx_offset, uint32_t num_idx,
+                                          const uint32_t *idx2vtx, uint32_t num_vtx,
+                                          uint32_t num_vdim, const float *vtx2val,
+                                          const uint32_t *wtx2elem, const float *wtx2param,
+                                          float *wtx2val);
+
 #ifdef __cplusplus
 }
 #endif

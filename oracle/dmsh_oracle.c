@@ -1941,3 +1941,241 @@ void orc_offset_array_aggregate(size_t num_idx, const uint32_t *off, const uint3
       for (uint32_t d = 0; d < num_dim; ++d) idx2aggval[i * num_dim + d] += kdx2val[k * num_dim + d];
     }
 }
+
+/* =========================================================================
+ * Mixed polyhedron meshes in CSR form (del-msh-cpu/src/polyhedron_mesh.rs,
+ * elem2center.rs, bvhnode2aabb3.rs).  The pyramid / prism / hex parametric
+ * coordinates follow the written rule of the kernels (del-geo's solver is
+ * not vendored): Newton's method from a fixed start, Cramer's rule, at most
+ * 20 steps, converged when max |d| < 1e-6, inside with a 1e-5 margin.
+ * ========================================================================= */
+
+/* elem2center.rs:59-89 */
+void orc_polyhedron_elem2center(size_t num_elem, const uint32_t *off, const uint32_t *idx2vtx,
+                                uint32_t num_dim, const float *xyz, float *cog) {
+  for (size_t e = 0; e < num_elem; ++e) {
+    float acc[4] = {0.f, 0.f, 0.f, 0.f};
+    for (uint32_t i = off[e]; i < off[e + 1]; ++i)
+      for (uint32_t d = 0; d < num_dim; ++d) acc[d] += xyz[(size_t)idx2vtx[i] * num_dim + d];
+    const uint32_t n = off[e + 1] - off[e];
+    const float ratio = n == 0 ? 0.f : 1.f / (float)n;
+    for (uint32_t d = 0; d < num_dim; ++d) cog[e * num_dim + d] = acc[d] * ratio;
+  }
+}
+
+/* bvhnode2aabb3.rs:197-259 (recursive, children first, margin 0) */
+static void poly_refit_rec(float *aabb, size_t node, const uint32_t *bn, const uint32_t *off,
+                           const uint32_t *idx2vtx, const float *xyz) {
+  float *b = aabb + node * 6;
+  if (bn[node * 3 + 2] == U32MAX) {
+    const uint32_t e = bn[node * 3 + 1];
+    for (uint32_t i = off[e]; i < off[e + 1]; ++i) {
+      const float *p = xyz + (size_t)idx2vtx[i] * 3;
+      for (int d = 0; d < 3; ++d) {
+        if (i == off[e]) {
+          b[d] = p[d];
+          b[d + 3] = p[d];
+        } else {
+          b[d] = fmin_sel(b[d], p[d]);
+          b[d + 3] = fmax_sel(b[d + 3], p[d]);
+        }
+      }
+    }
+    return;
+  }
+  const size_t c0 = bn[node * 3 + 1], c1 = bn[node * 3 + 2];
+  poly_refit_rec(aabb, c0, bn, off, idx2vtx, xyz);
+  poly_refit_rec(aabb, c1, bn, off, idx2vtx, xyz);
+  for (int d = 0; d < 3; ++d) {
+    b[d] = fmin_sel(aabb[c0 * 6 + d], aabb[c1 * 6 + d]);
+    b[d + 3] = fmax_sel(aabb[c0 * 6 + d + 3], aabb[c1 * 6 + d + 3]);
+  }
+}
+
+void orc_polyhedron_refit(float *aabb, const uint32_t *bvhnodes, size_t num_elem,
+                          const uint32_t *off, const uint32_t *idx2vtx, const float *xyz) {
+  if (num_elem) poly_refit_rec(aabb, 0, bvhnodes, off, idx2vtx, xyz);
+}
+
+typedef struct { float x, y, z; } v3f;
+static inline v3f v3sub(v3f a, v3f b) { v3f r = {a.x - b.x, a.y - b.y, a.z - b.z}; return r; }
+static inline v3f v3neg(v3f a) { v3f r = {-a.x, -a.y, -a.z}; return r; }
+/* a . (b x c) */
+static inline float v3det(v3f a, v3f b, v3f c) {
+  const float cx = b.y * c.z - b.z * c.y;
+  const float cy = b.z * c.x - b.x * c.z;
+  const float cz = b.x * c.y - b.y * c.x;
+  return (a.x * cx + a.y * cy) + a.z * cz;
+}
+
+/* polyhedron_mesh.rs:89-100 with barycentrics by volume ratios */
+static int tet_param(const v3f *p, float pco[3]) {
+  const v3f e1 = v3sub(p[1], p[0]), e2 = v3sub(p[2], p[0]), e3 = v3sub(p[3], p[0]);
+  const float vol = v3det(e1, e2, e3);
+  if (vol == 0.f) return 0;
+  const v3f o = v3neg(p[0]);
+  const float b0 = v3det(p[1], p[2], p[3]) / vol;
+  const float b1 = v3det(o, e2, e3) / vol;
+  const float b2 = v3det(e1, o, e3) / vol;
+  if (!(b0 > 0.f && b1 > 0.f && b2 > 0.f && ((1.f - b0) - b1) - b2 > 0.f)) return 0;
+  pco[0] = b0;
+  pco[1] = b1;
+  pco[2] = b2;
+  return 1;
+}
+
+/* shape functions with the operation order of polyhedron_mesh.cu:134-163 */
+void orc_shape(int nn, float r, float s, float t, float *N, float *dr, float *ds, float *dt) {
+  if (nn == 5) {
+    const float rm = 1.f - r, sm = 1.f - s, tm = 1.f - t;
+    N[0] = (rm * sm) * tm; N[1] = (r * sm) * tm; N[2] = (r * s) * tm; N[3] = (rm * s) * tm;
+    N[4] = t;
+    if (!dr) return;
+    dr[0] = -(sm * tm); dr[1] = sm * tm; dr[2] = s * tm; dr[3] = -(s * tm); dr[4] = 0.f;
+    ds[0] = -(rm * tm); ds[1] = -(r * tm); ds[2] = r * tm; ds[3] = rm * tm; ds[4] = 0.f;
+    dt[0] = -(rm * sm); dt[1] = -(r * sm); dt[2] = -(r * s); dt[3] = -(rm * s); dt[4] = 1.f;
+  } else if (nn == 6) {
+    const float rs = (1.f - r) - s, tm = 1.f - t;
+    N[0] = rs * tm; N[1] = r * tm; N[2] = s * tm; N[3] = rs * t; N[4] = r * t; N[5] = s * t;
+    if (!dr) return;
+    dr[0] = -tm; dr[1] = tm; dr[2] = 0.f; dr[3] = -t; dr[4] = t; dr[5] = 0.f;
+    ds[0] = -tm; ds[1] = 0.f; ds[2] = tm; ds[3] = -t; ds[4] = 0.f; ds[5] = t;
+    dt[0] = -rs; dt[1] = -r; dt[2] = -s; dt[3] = rs; dt[4] = r; dt[5] = s;
+  } else {
+    static const float sr[8] = {-1.f, 1.f, 1.f, -1.f, -1.f, 1.f, 1.f, -1.f};
+    static const float ss[8] = {-1.f, -1.f, 1.f, 1.f, -1.f, -1.f, 1.f, 1.f};
+    static const float st[8] = {-1.f, -1.f, -1.f, -1.f, 1.f, 1.f, 1.f, 1.f};
+    const float e = 0.125f;
+    for (int i = 0; i < 8; ++i) {
+      const float fr = sr[i] > 0.f ? 1.f + r : 1.f - r;
+      const float fs = ss[i] > 0.f ? 1.f + s : 1.f - s;
+      const float ft = st[i] > 0.f ? 1.f + t : 1.f - t;
+      N[i] = ((e * fr) * fs) * ft;
+      if (dr) {
+        dr[i] = ((e * sr[i]) * fs) * ft;
+        ds[i] = ((e * fr) * ss[i]) * ft;
+        dt[i] = ((e * fr) * fs) * st[i];
+      }
+    }
+  }
+}
+
+static int newton_param(int nn, const v3f *p, float r, float s, float t, float pco[3]) {
+  int converged = 0;
+  for (int it = 0; it < 20; ++it) {
+    float N[8], dr[8], ds[8], dt[8];
+    orc_shape(nn, r, s, t, N, dr, ds, dt);
+    v3f x = {0.f, 0.f, 0.f}, jr = {0.f, 0.f, 0.f}, js = {0.f, 0.f, 0.f}, jt = {0.f, 0.f, 0.f};
+    for (int i = 0; i < nn; ++i) {
+      x.x = x.x + N[i] * p[i].x; x.y = x.y + N[i] * p[i].y; x.z = x.z + N[i] * p[i].z;
+      jr.x = jr.x + dr[i] * p[i].x; jr.y = jr.y + dr[i] * p[i].y; jr.z = jr.z + dr[i] * p[i].z;
+      js.x = js.x + ds[i] * p[i].x; js.y = js.y + ds[i] * p[i].y; js.z = js.z + ds[i] * p[i].z;
+      jt.x = jt.x + dt[i] * p[i].x; jt.y = jt.y + dt[i] * p[i].y; jt.z = jt.z + dt[i] * p[i].z;
+    }
+    const float d = v3det(jr, js, jt);
+    if (!(d != 0.f) || !(fabsf(d) < INFINITY)) return 0;
+    const v3f f = v3neg(x);
+    const float dr0 = v3det(f, js, jt) / d;
+    const float ds0 = v3det(jr, f, jt) / d;
+    const float dt0 = v3det(jr, js, f) / d;
+    r = r + dr0;
+    s = s + ds0;
+    t = t + dt0;
+    const float m = fmaxf(fmaxf(fabsf(dr0), fabsf(ds0)), fabsf(dt0));
+    if (!(m == m)) return 0;
+    if (m < 1e-6f) { converged = 1; break; }
+  }
+  if (!converged) return 0;
+  const float lo = -1e-5f, hi = 1.f + 1e-5f;
+  int inside;
+  if (nn == 5) inside = r >= lo && r <= hi && s >= lo && s <= hi && t >= lo && t <= hi;
+  else if (nn == 6) inside = r >= lo && s >= lo && r + s <= hi && t >= lo && t <= hi;
+  else inside = r >= -hi && r <= hi && s >= -hi && s <= hi && t >= -hi && t <= hi;
+  if (!inside) return 0;
+  pco[0] = r;
+  pco[1] = s;
+  pco[2] = t;
+  return 1;
+}
+
+/* polyhedron_mesh.rs:79-130: the element's vertices shifted by -q */
+int orc_param_coord(const float *q, const uint32_t *node2vtx, uint32_t n, const float *xyz,
+                    float pco[3]) {
+  v3f p[8];
+  if (n != 4 && n != 5 && n != 6 && n != 8) return 0;
+  for (uint32_t k = 0; k < n; ++k) {
+    const float *x = xyz + (size_t)node2vtx[k] * 3;
+    p[k].x = x[0] - q[0];
+    p[k].y = x[1] - q[1];
+    p[k].z = x[2] - q[2];
+  }
+  if (n == 4) return tet_param(p, pco);
+  if (n == 5) return newton_param(5, p, 0.5f, 0.5f, 0.25f, pco);
+  if (n == 6) return newton_param(6, p, 1.f / 3.f, 1.f / 3.f, 0.5f, pco);
+  return newton_param(8, p, 0.f, 0.f, 0.f, pco);
+}
+
+/* polyhedron_mesh.rs:470-522 recursion */
+static void poly_search_rec(const float *q, const uint32_t *bn, const float *aabb,
+                            const uint32_t *off, const uint32_t *idx2vtx, const float *xyz,
+                            size_t node, uint32_t *best, float *pco) {
+  if (*best != U32MAX) return;
+  const float *b = aabb + node * 6;
+  if (!(b[0] <= q[0] && q[0] <= b[3] && b[1] <= q[1] && q[1] <= b[4] && b[2] <= q[2] &&
+        q[2] <= b[5]))
+    return;
+  if (bn[node * 3 + 2] == U32MAX) {
+    const uint32_t e = bn[node * 3 + 1];
+    float c[3];
+    if (orc_param_coord(q, idx2vtx + off[e], off[e + 1] - off[e], xyz, c)) {
+      *best = e;
+      pco[0] = c[0];
+      pco[1] = c[1];
+      pco[2] = c[2];
+    }
+    return;
+  }
+  poly_search_rec(q, bn, aabb, off, idx2vtx, xyz, bn[node * 3 + 1], best, pco);
+  poly_search_rec(q, bn, aabb, off, idx2vtx, xyz, bn[node * 3 + 2], best, pco);
+}
+
+/* polyhedron_mesh.rs:524-575 */
+void orc_polyhedron_search(size_t num_elem, const uint32_t *off, const uint32_t *idx2vtx,
+                           const float *xyz, const uint32_t *bvhnodes, const float *aabb,
+                           size_t num_wtx, const float *wtx2xyz, uint32_t *wtx2elem,
+                           float *wtx2param) {
+  for (size_t w = 0; w < num_wtx; ++w) {
+    uint32_t best = U32MAX;
+    float pco[3] = {0.f, 0.f, 0.f};
+    if (num_elem) poly_search_rec(wtx2xyz + w * 3, bvhnodes, aabb, off, idx2vtx, xyz, 0, &best, pco);
+    wtx2elem[w] = best;
+    wtx2param[w * 3] = pco[0];
+    wtx2param[w * 3 + 1] = pco[1];
+    wtx2param[w * 3 + 2] = pco[2];
+  }
+}
+
+/* polyhedron_mesh.rs:868-907 with polyhedron_mesh.cu:127-176's shape
+ * functions; accumulates onto wtx2val (zeroed by the caller) */
+void orc_polyhedron_interpolate(size_t num_wtx, const uint32_t *off, const uint32_t *idx2vtx,
+                                uint32_t num_vdim, const float *vtx2val, const uint32_t *wtx2elem,
+                                const float *wtx2param, float *wtx2val) {
+  for (size_t w = 0; w < num_wtx; ++w) {
+    const uint32_t e = wtx2elem[w];
+    if (e == U32MAX) continue;
+    const uint32_t n = off[e + 1] - off[e];
+    const float r = wtx2param[w * 3], s = wtx2param[w * 3 + 1], t = wtx2param[w * 3 + 2];
+    float sf[8];
+    if (n == 4) {
+      sf[0] = r; sf[1] = s; sf[2] = t; sf[3] = ((1.f - r) - s) - t;
+    } else if (n == 5 || n == 6 || n == 8) {
+      orc_shape((int)n, r, s, t, sf, NULL, NULL, NULL);
+    } else {
+      continue;
+    }
+    for (uint32_t k = 0; k < n; ++k) {
+      const float *val = vtx2val + (size_t)idx2vtx[off[e] + k] * num_vdim;
+      for (uint32_t d = 0; d < num_vdim; ++d) wtx2val[w * num_vdim + d] += sf[k] * val[d];
+    }
+  }
+}

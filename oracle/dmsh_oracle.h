@@ -264,6 +264,23 @@ void orc_unique_jdx2val_jdx2idx(size_t n, const uint32_t *val, const uint32_t *i
 void orc_offset_array_aggregate(size_t num_idx, const uint32_t *off, const uint32_t *jdx2kdx,
                                 uint32_t num_dim, const float *kdx2val, float *idx2aggval);
 
+/* mixed polyhedron meshes in CSR form (polyhedron_mesh.rs, elem2center.rs,
+ * bvhnode2aabb3.rs) */
+void orc_polyhedron_elem2center(size_t num_elem, const uint32_t *off, const uint32_t *idx2vtx,
+                                uint32_t num_dim, const float *xyz, float *cog);
+void orc_polyhedron_refit(float *aabb, const uint32_t *bvhnodes, size_t num_elem,
+                          const uint32_t *off, const uint32_t *idx2vtx, const float *xyz);
+void orc_shape(int nn, float r, float s, float t, float *N, float *dr, float *ds, float *dt);
+int orc_param_coord(const float *q, const uint32_t *node2vtx, uint32_t n, const float *xyz,
+                    float pco[3]);
+void orc_polyhedron_search(size_t num_elem, const uint32_t *off, const uint32_t *idx2vtx,
+                           const float *xyz, const uint32_t *bvhnodes, const float *aabb,
+                           size_t num_wtx, const float *wtx2xyz, uint32_t *wtx2elem,
+                           float *wtx2param);
+void orc_polyhedron_interpolate(size_t num_wtx, const uint32_t *off, const uint32_t *idx2vtx,
+                                uint32_t num_vdim, const float *vtx2val, const uint32_t *wtx2elem,
+                                const float *wtx2param, float *wtx2val);
+
 #ifdef __cplusplus
 }
 #endif

@@ -699,3 +699,83 @@ def offset_array_aggregate(idx2jdx_offset, jdx2kdx, kdx2val):
     L.orc_offset_array_aggregate(off.shape[0] - 1, off, _u32(jdx2kdx), v.shape[1], v.reshape(-1),
                                  out.reshape(-1))
     return out
+
+
+# ---------------------------------------------------------------------------
+# mixed polyhedron meshes (CSR)
+# ---------------------------------------------------------------------------
+def _sig_poly():
+    L = lib()
+    if getattr(L, "_poly_ready", False):
+        return L
+    u32 = ctypes.c_uint32
+    L.orc_polyhedron_elem2center.restype = None
+    L.orc_polyhedron_elem2center.argtypes = [_sz, _u32p, _u32p, u32, _f32p, _f32p]
+    L.orc_polyhedron_refit.restype = None
+    L.orc_polyhedron_refit.argtypes = [_f32p, _u32p, _sz, _u32p, _u32p, _f32p]
+    L.orc_shape.restype = None
+    L.orc_shape.argtypes = [ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_float, _f32p,
+                            ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+    L.orc_param_coord.restype = ctypes.c_int
+    L.orc_param_coord.argtypes = [_f32p, _u32p, u32, _f32p, _f32p]
+    L.orc_polyhedron_search.restype = None
+    L.orc_polyhedron_search.argtypes = [_sz, _u32p, _u32p, _f32p, _u32p, _f32p, _sz, _f32p, _u32p,
+                                        _f32p]
+    L.orc_polyhedron_interpolate.restype = None
+    L.orc_polyhedron_interpolate.argtypes = [_sz, _u32p, _u32p, u32, _f32p, _u32p, _f32p, _f32p]
+    L._poly_ready = True
+    return L
+
+
+def polyhedron_elem2center(elem2idx_offset, idx2vtx, vtx2xyz):
+    L = _sig_poly()
+    off, iv, x = _u32(elem2idx_offset), _u32(idx2vtx), _f32(vtx2xyz)
+    out = np.zeros((off.shape[0] - 1, x.shape[1]), np.float32)
+    L.orc_polyhedron_elem2center(off.shape[0] - 1, off, iv, x.shape[1], x.reshape(-1),
+                                 out.reshape(-1))
+    return out
+
+
+def polyhedron_refit(bvhnodes, elem2idx_offset, idx2vtx, vtx2xyz):
+    L = _sig_poly()
+    bn = _u32(bvhnodes)
+    out = np.zeros((bn.shape[0], 6), np.float32)
+    off = _u32(elem2idx_offset)
+    L.orc_polyhedron_refit(out.reshape(-1), bn.reshape(-1), off.shape[0] - 1, off, _u32(idx2vtx),
+                           _f32(vtx2xyz).reshape(-1))
+    return out
+
+
+def shape_functions(num_node, r, s, t):
+    N = np.zeros(8, np.float32)
+    _sig_poly().orc_shape(num_node, r, s, t, N, None, None, None)
+    return N[:num_node]
+
+
+def param_coord(q, node2vtx, vtx2xyz):
+    p = np.zeros(3, np.float32)
+    ok = _sig_poly().orc_param_coord(_f32(q), _u32(node2vtx), len(node2vtx),
+                                     _f32(vtx2xyz).reshape(-1), p)
+    return p if ok else None
+
+
+def polyhedron_search(elem2idx_offset, idx2vtx, vtx2xyz, bvhnodes, bvhnode2aabb, wtx2xyz):
+    L = _sig_poly()
+    off = _u32(elem2idx_offset)
+    w = _f32(wtx2xyz)
+    elem = np.zeros(w.shape[0], np.uint32)
+    par = np.zeros((w.shape[0], 3), np.float32)
+    L.orc_polyhedron_search(off.shape[0] - 1, off, _u32(idx2vtx), _f32(vtx2xyz).reshape(-1),
+                            _u32(bvhnodes).reshape(-1), _f32(bvhnode2aabb).reshape(-1),
+                            w.shape[0], w.reshape(-1), elem, par.reshape(-1))
+    return elem, par
+
+
+def polyhedron_interpolate(elem2idx_offset, idx2vtx, vtx2val, wtx2elem, wtx2param):
+    L = _sig_poly()
+    v = _f32(vtx2val)
+    e = _u32(wtx2elem)
+    out = np.zeros((e.shape[0], v.shape[1]), np.float32)
+    L.orc_polyhedron_interpolate(e.shape[0], _u32(elem2idx_offset), _u32(idx2vtx), v.shape[1],
+                                 v.reshape(-1), e, _f32(wtx2param).reshape(-1), out.reshape(-1))
+    return out
