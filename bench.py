@@ -60,6 +60,8 @@ def parse():
     ap.add_argument("--serve-budget-mib", type=int, default=4096)
     ap.add_argument("--diff-reps", type=int, default=5,
                     help="repetitions of the differentiation workload (0 skips it)")
+    ap.add_argument("--tree-reps", type=int, default=5,
+                    help="repetitions of the octree and point-location workloads (0 skips them)")
     return ap.parse_args()
 
 
@@ -266,6 +268,104 @@ def run_batched(num_mesh: int, reps: int):
 # ---------------------------------------------------------------------------
 # served workload: continuous batching vs one request at a time
 # ---------------------------------------------------------------------------
+def run_trees(reps: int):
+    """Other users of the Morton / LBVH machinery: a 3D octree over 2^22
+    uniform points (codes, sort, unique, tree, aggregate of 3 values per
+    point), and point location + interpolation of 2^22 query points in a
+    mixed tet / pyramid / prism / hex grid of 48^3 jittered cubes (element
+    BVH build included).  Medians over `reps` runs, HIP events."""
+    import numpy as np
+    import torch
+
+    import del_msh_dlpack.Array1D.torch as Array1D
+    import del_msh_dlpack.Mortons.torch as Mortons
+    import del_msh_dlpack.OffsetArray.torch as OffsetArray
+    import del_msh_dlpack.PolyhedronMesh.torch as PolyhedronMesh
+    import del_msh_dlpack.QuadOctTree.torch as QuadOctTree
+    from del_msh_dlpack import primitives
+
+    dev = torch.device("cuda", 0)
+
+    def timed(fn, n):
+        ev = []
+        for _ in range(n):
+            a = torch.cuda.Event(enable_timing=True)
+            b = torch.cuda.Event(enable_timing=True)
+            a.record()
+            fn()
+            b.record()
+            ev.append((a, b))
+        torch.cuda.synchronize()
+        return float(np.median([a.elapsed_time(b) for a, b in ev]))
+
+    n = 1 << 22
+    g = torch.Generator(device="cpu").manual_seed(0)
+    vtx2co = torch.rand((n, 3), generator=g).to(dev)
+    eye = torch.eye(4, device=dev)
+    vals = torch.ones((n, 3), dtype=torch.float32, device=dev)
+    state = {}
+
+    def codes():
+        m = Mortons.make_vtx2morton_from_vtx2co(vtx2co, eye)
+        state["jdx2vtx"], state["sorted"] = Array1D.argsort(m)
+        _, state["uniq"], state["off"] = Array1D.unique_for_sorted_array(state["sorted"])
+
+    def tree():
+        t = QuadOctTree.QuadOctTree()
+        t.construct_from_idx2morton(state["uniq"], 3)
+        state["tree"] = t
+
+    def agg():
+        idx2agg = OffsetArray.aggregate(state["off"], state["jdx2vtx"], vals)
+        state["agg"] = state["tree"].aggregate(idx2agg)
+
+    for f in (codes, tree, agg):
+        f()
+    torch.cuda.synchronize()
+    codes_ms, tree_ms, agg_ms = timed(codes, reps), timed(tree, reps), timed(agg, reps)
+    octree = {
+        "workload": "2^22 points uniform in [0,1]^3 (seed 0), 30-bit codes, max depth 10",
+        "points": n, "unique_codes": int(state["uniq"].shape[0]),
+        "onodes": int(state["tree"].onodes.shape[0]),
+        "codes_sort_unique_ms": round(codes_ms, 4), "tree_ms": round(tree_ms, 4),
+        "aggregate_ms": round(agg_ms, 4),
+        "mpts_per_s": round(n / ((codes_ms + tree_ms) * 1e-3) / 1e6, 1),
+        "root_sum_ok": bool(state["agg"][0, 0].item() == n),
+    }
+
+    off, iv, x = primitives.mixed_hex_grid(48, 48, 48, seed=1, jitter=0.15)
+    d_off, d_iv, d_x = (torch.from_numpy(a).to(dev) for a in (off, iv, x))
+    q = torch.from_numpy(np.random.default_rng(2).uniform(0.0, 48.0, (n, 3)).astype(np.float32)).to(dev)
+    v = torch.from_numpy(x[:, :2].copy()).to(dev)
+
+    def build():
+        state["bvh"] = PolyhedronMesh.make_bvhnodes_bvhnode2aabb(d_off, d_iv, d_x)
+
+    def search():
+        bn, ab = state["bvh"]
+        state["found"] = PolyhedronMesh.search_elem_contain_points(d_off, d_iv, d_x, bn, ab, q)
+
+    def interp():
+        e, p = state["found"]
+        state["val"] = PolyhedronMesh.interpolate_values_at_points(d_off, d_iv, v, e, p)
+
+    for f in (build, search, interp):
+        f()
+    torch.cuda.synchronize()
+    build_ms, search_ms, interp_ms = timed(build, reps), timed(search, reps), timed(interp, reps)
+    e = state["found"][0]
+    located = {
+        "workload": (f"mixed grid 48^3 cubes (hex / 2 prisms / 6 pyramids / 6 tets per cube, "
+                     f"seed 1, jitter 0.15): {off.size - 1} elements; {n} query points U[0,48]^3"),
+        "elements": int(off.size - 1), "queries": n,
+        "found_fraction": round(float((e != 0xFFFFFFFF).float().mean().item()), 4),
+        "bvh_build_ms": round(build_ms, 4), "search_ms": round(search_ms, 4),
+        "interpolate_ms": round(interp_ms, 4),
+        "mqueries_per_s": round(n / (search_ms * 1e-3) / 1e6, 1),
+    }
+    return {"octree": octree, "point_location": located}
+
+
 def run_diff(reps: int):
     """Differentiation workload.  (1) A 50,000-triangle sphere (sphere_yup(1,
     126, 200)) seen from (0, 0, 3) at 512^2, dL/dpixel ~ N(0, 1) (seed 4):
@@ -630,6 +730,7 @@ def main():
     served = (run_served(args.serve_requests, [float(x) for x in args.serve_rates.split(",")],
                          args.serve_budget_mib << 20) if args.serve_requests > 0 else None)
     diff = run_diff(args.diff_reps) if args.diff_reps > 0 else None
+    trees = run_trees(args.tree_reps) if args.tree_reps > 0 else None
 
     # ---------------- roofline of the dominant kernel ----------------
     # build: compulsory bytes per triangle with u64 codes
@@ -708,6 +809,7 @@ def main():
         "batched": batched,
         "served": served,
         "differentiation": diff,
+        "trees": trees,
         "roofline": roof,
         "cpu_baseline": cpu,
     }

@@ -75,10 +75,7 @@ int dmsh_octree_make_tree(void* stream, uint32_t num_idx, const uint32_t* bnodes
 }
 
 size_t dmsh_octree_aggregate_workspace_bytes(uint32_t num_onode) {
-  Arena a{nullptr, ~(size_t)0, 0};
-  a.take<uint32_t>(num_onode ? num_onode : 1);
-  a.take<uint32_t>(num_onode ? num_onode : 1);
-  return a.off + 256;
+  return sizeof(uint32_t) * (size_t)(num_onode ? num_onode : 1) + 256;
 }
 
 int dmsh_octree_aggregate(void* stream, uint32_t num_idx, uint32_t num_vdim, const float* idx2val,
@@ -86,26 +83,23 @@ int dmsh_octree_aggregate(void* stream, uint32_t num_idx, uint32_t num_vdim, con
                           const uint32_t* onodes, float* onode2aggval, void* ws,
                           size_t ws_bytes) {
   if (num_onode == 0 || num_vdim == 0) return DMSH_OK;
-  if (num_link == 0 || !idx2onode || !onodes || !onode2aggval || (num_idx > 0 && !idx2val))
+  if (num_link < 2 || !idx2onode || !onodes || !onode2aggval || (num_idx > 0 && !idx2val))
     return DMSH_ERR_INVALID;
   if (!ws || ws_bytes < dmsh_octree_aggregate_workspace_bytes(num_onode))
     return DMSH_ERR_WORKSPACE;
-  Arena a{static_cast<char*>(ws), ws_bytes, 0};
-  uint32_t* lo = a.take<uint32_t>(num_onode);
-  uint32_t* hi = a.take<uint32_t>(num_onode);
-  if (!a.ok()) return DMSH_ERR_WORKSPACE;
   hipStream_t s = as_stream(stream);
-  int rc = sortk::launch_fill(s, lo, num_onode, DMSH_U32MAX);
-  if (rc != DMSH_OK) return rc;
-  if ((rc = sortk::launch_fill(s, hi, num_onode, 0u)) != DMSH_OK) return rc;
-  if (num_idx > 0) {
-    launch("k_onode_range", oct::k_onode_range, grid_for(num_idx), dim3(256), 0, s, num_idx,
-           idx2onode, onodes, num_link, num_onode, lo, hi, device_error_word());
-    if ((rc = launch_status()) != DMSH_OK) return rc;
+  uint32_t* hops = static_cast<uint32_t*>(ws);
+  uint32_t* err = device_error_word();
+  launch("k_onode_hops", oct::k_onode_hops, grid_for(num_onode), dim3(256), 0, s, num_onode,
+         num_link, onodes, hops, err);
+  int rc = launch_status();
+  for (int level = (int)oct::kMaxHops; level >= 0 && rc == DMSH_OK; --level) {
+    launch("k_onode_sum_level", oct::k_onode_sum_level, grid_for(num_onode), dim3(256), 0, s,
+           num_onode, num_vdim, num_link, onodes, idx2val, num_idx, hops, (uint32_t)level,
+           onode2aggval, err);
+    rc = launch_status();
   }
-  launch("k_onode_aggregate", oct::k_onode_aggregate, grid_for((uint64_t)num_onode * num_vdim),
-         dim3(256), 0, s, num_onode, num_vdim, idx2val, lo, hi, onode2aggval);
-  return launch_status();
+  return rc;
 }
 
 int dmsh_has_duplicate_sorted_u32(void* stream, uint32_t n, const uint32_t* idx2val,

@@ -12,15 +12,16 @@
 //                     numbers the tree nodes (onodes);
 //   k_octree_make     children / parent links, depths and cell centres of
 //                     the onodes and of the points;
-//   k_onode_range +
-//   k_onode_aggregate per-node sums of per-point values.  A node is a Morton
-//                     cell, and the points of a cell are contiguous in sorted
-//                     order, so each node sums its run of points in ascending
-//                     order: the order del-msh-cpu's per-point climb adds
-//                     them (quad_oct_tree.rs:420-451), hence the same bits.
-//                     The parent link is entry 0 of the node's 1 + 2^num_dim
-//                     links (the reference GPU kernel's num_link; the CPU one
-//                     reads entry i*9, right only in 3D).
+//   k_onode_hops +
+//   k_onode_sum_level per-node sums of per-point values, each node adding its
+//                     children in slot order (a fixed order; del-msh-cpu adds
+//                     point by point, quad_oct_tree.rs:420-451, and the
+//                     reference GPU kernel with atomics, so neither order is
+//                     kept: sums of exactly representable partial sums agree
+//                     bit for bit, others to rounding).  The parent link is
+//                     entry 0 of the node's 1 + 2^num_dim links (the
+//                     reference GPU kernel's num_link; the CPU one reads
+//                     entry i*9, right only in 3D).
 // Codes must be unique (the reference asserts it when it builds the tree);
 // walks that leave the tree set DMSH_DEVERR_INDEX instead of looping.
 #pragma once
@@ -174,44 +175,60 @@ __global__ __launch_bounds__(256) void k_octree_make(
   onode2depth[o] = depth;
 }
 
-// the first and last point under every onode (lo starts at 0xFFFFFFFF, hi at 0)
-__global__ __launch_bounds__(256) void k_onode_range(uint32_t num_idx,
-                                                     const uint32_t* __restrict__ idx2onode,
-                                                     const uint32_t* __restrict__ onodes,
-                                                     uint32_t num_link, uint32_t num_onode,
-                                                     uint32_t* __restrict__ lo,
-                                                     uint32_t* __restrict__ hi,
-                                                     uint32_t* __restrict__ err) {
-  const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= num_idx) return;
-  uint32_t o = idx2onode[idx];
-  for (uint32_t steps = 0; o != DMSH_U32MAX; ++steps) {
-    if (o >= num_onode || steps >= num_onode) {
+constexpr uint32_t kMaxHops = 16;  // node depth grows on every hop, up to 32 / num_dim
+
+// hops[o] = number of parent links from onode o to the root
+__global__ __launch_bounds__(256) void k_onode_hops(uint32_t num_onode, uint32_t num_link,
+                                                    const uint32_t* __restrict__ onodes,
+                                                    uint32_t* __restrict__ hops,
+                                                    uint32_t* __restrict__ err) {
+  const uint32_t o = blockIdx.x * blockDim.x + threadIdx.x;
+  if (o >= num_onode) return;
+  uint32_t h = 0, p = onodes[(size_t)o * num_link];
+  while (p != DMSH_U32MAX) {
+    if (p >= num_onode || h >= kMaxHops) {
       raise_error(err, DMSH_DEVERR_INDEX);
-      return;
+      h = DMSH_U32MAX;
+      break;
     }
-    atomicMin(lo + o, idx);
-    atomicMax(hi + o, idx);
-    o = onodes[(size_t)o * num_link];
+    ++h;
+    p = onodes[(size_t)p * num_link];
   }
+  hops[o] = h;
 }
 
-// one thread per (onode, value dimension): the run [lo, hi] summed in order
-__global__ __launch_bounds__(256) void k_onode_aggregate(uint32_t num_onode, uint32_t num_vdim,
+// Per-node sums in a fixed order: a node adds its children in slot order (a
+// point child contributes its idx2val row, a node child its own sum),
+// starting from 0.  Launched once per level, deepest first; a node's
+// children are one hop deeper, so their sums are final when it runs.
+__global__ __launch_bounds__(256) void k_onode_sum_level(uint32_t num_onode, uint32_t num_vdim,
+                                                         uint32_t num_link,
+                                                         const uint32_t* __restrict__ onodes,
                                                          const float* __restrict__ idx2val,
-                                                         const uint32_t* __restrict__ lo,
-                                                         const uint32_t* __restrict__ hi,
-                                                         float* __restrict__ onode2aggval) {
-  const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (uint64_t)num_onode * num_vdim) return;
-  const uint32_t o = (uint32_t)(t / num_vdim), d = (uint32_t)(t % num_vdim);
-  const uint32_t a = lo[o], b = hi[o];
-  float acc = 0.f;
-  if (a <= b) {
-    const float* v = idx2val + (size_t)a * num_vdim + d;
-    for (uint32_t k = a; k <= b; ++k, v += num_vdim) acc += *v;
+                                                         uint32_t num_idx,
+                                                         const uint32_t* __restrict__ hops,
+                                                         uint32_t level, float* __restrict__ out,
+                                                         uint32_t* __restrict__ err) {
+  const uint32_t o = blockIdx.x * blockDim.x + threadIdx.x;
+  if (o >= num_onode || hops[o] != level) return;
+  for (uint32_t d = 0; d < num_vdim; ++d) {
+    float acc = 0.f;
+    for (uint32_t c = 1; c < num_link; ++c) {
+      const uint32_t ch = onodes[(size_t)o * num_link + c];
+      if (ch == DMSH_U32MAX) continue;
+      if (ch >= num_onode) {
+        const uint32_t idx = ch - num_onode;
+        if (idx >= num_idx) {
+          raise_error(err, DMSH_DEVERR_INDEX);
+          continue;
+        }
+        acc += idx2val[(size_t)idx * num_vdim + d];
+      } else {
+        acc += out[(size_t)ch * num_vdim + d];
+      }
+    }
+    out[(size_t)o * num_vdim + d] = acc;
   }
-  onode2aggval[t] = acc;
 }
 
 // ---------------------------------------------------------------------------

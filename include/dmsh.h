@@ -492,9 +492,10 @@ int dmsh_multiply_graph_laplacian(void *stream, uint32_t num_vtx, const uint32_t
  * parent, then the children; a point child is idx + num_onode; empty slots
  * 0xFFFFFFFF), onode2depth, onode2center [num_onode][num_dim], idx2onode,
  * idx2center [num_idx][num_dim].  dmsh_octree_aggregate writes every
- * node's sum of idx2val over its points, added in ascending idx order
- * (del-msh-cpu's order, so the same bits; the reference GPU adds with
- * atomics in no fixed order).
+ * node's sum of idx2val over its points, each node adding its children in
+ * slot order (a fixed order, so the same bits on every run; del-msh-cpu adds
+ * point by point and the reference GPU with atomics, so sums agree with
+ * theirs bit for bit when the partial sums are exact, else to rounding).
  * ------------------------------------------------------------------- */
 size_t dmsh_octree_bnodes_workspace_bytes(uint32_t num_idx);
 int dmsh_octree_bnodes(void *stream, uint32_t num_idx, const uint32_t *idx2morton,

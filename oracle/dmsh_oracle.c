@@ -2179,3 +2179,28 @@ void orc_polyhedron_interpolate(size_t num_wtx, const uint32_t *off, const uint3
     }
   }
 }
+
+/* The kernels' order for the octree aggregate: each node adds its children
+ * in slot order (point child: its value row; node child: its sum), from 0. */
+static void oct_sum_rec(uint32_t o, size_t num_onode, uint32_t num_link, const uint32_t *onodes,
+                        uint32_t num_vdim, const float *idx2val, float *out) {
+  for (uint32_t c = 1; c < num_link; ++c) {
+    const uint32_t ch = onodes[(size_t)o * num_link + c];
+    if (ch != U32MAX && ch < num_onode) oct_sum_rec(ch, num_onode, num_link, onodes, num_vdim, idx2val, out);
+  }
+  for (uint32_t d = 0; d < num_vdim; ++d) {
+    float acc = 0.f;
+    for (uint32_t c = 1; c < num_link; ++c) {
+      const uint32_t ch = onodes[(size_t)o * num_link + c];
+      if (ch == U32MAX) continue;
+      acc += ch >= num_onode ? idx2val[(size_t)(ch - num_onode) * num_vdim + d]
+                             : out[(size_t)ch * num_vdim + d];
+    }
+    out[(size_t)o * num_vdim + d] = acc;
+  }
+}
+
+void orc_octree_aggregate_tree(size_t num_onode, uint32_t num_link, const uint32_t *onodes,
+                               uint32_t num_vdim, const float *idx2val, float *out) {
+  if (num_onode) oct_sum_rec(0, num_onode, num_link, onodes, num_vdim, idx2val, out);
+}

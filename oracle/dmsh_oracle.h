@@ -258,6 +258,8 @@ int orc_octree_make_tree(size_t num_vtx, const uint32_t *bnodes, const uint32_t 
 void orc_octree_aggregate(size_t num_idx, uint32_t num_vdim, const float *idx2val,
                           const uint32_t *idx2onode, uint32_t num_link, size_t num_onode,
                           const uint32_t *onodes, float *onode2aggval);
+void orc_octree_aggregate_tree(size_t num_onode, uint32_t num_link, const uint32_t *onodes,
+                               uint32_t num_vdim, const float *idx2val, float *out);
 void orc_unique_for_sorted(size_t n, const uint32_t *val, uint32_t *idx2jdx);
 void orc_unique_jdx2val_jdx2idx(size_t n, const uint32_t *val, const uint32_t *idx2jdx,
                                 size_t num_jdx, uint32_t *jdx2val, uint32_t *jdx2idx_offset);

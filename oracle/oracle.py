@@ -611,6 +611,8 @@ def _sig_octree():
                                        _u32p, _u32p, _f32p, _u32p, _f32p]
     L.orc_octree_aggregate.restype = None
     L.orc_octree_aggregate.argtypes = [_sz, u32, _f32p, _u32p, u32, _sz, _u32p, _f32p]
+    L.orc_octree_aggregate_tree.restype = None
+    L.orc_octree_aggregate_tree.argtypes = [_sz, u32, _u32p, u32, _f32p, _f32p]
     L.orc_unique_for_sorted.restype = None
     L.orc_unique_for_sorted.argtypes = [_sz, _u32p, _u32p]
     L.orc_unique_jdx2val_jdx2idx.restype = None
@@ -674,6 +676,17 @@ def octree_aggregate(idx2val, idx2onode, onodes):
     out = np.zeros((on.shape[0], v.shape[1]), np.float32)
     L.orc_octree_aggregate(v.shape[0], v.shape[1], v.reshape(-1), _u32(idx2onode), on.shape[1],
                            on.shape[0], on.reshape(-1), out.reshape(-1))
+    return out
+
+
+def octree_aggregate_tree(idx2val, onodes):
+    """The kernels' summation order: each node adds its children in slot order."""
+    L = _sig_octree()
+    v = _f32(idx2val)
+    on = _u32(onodes)
+    out = np.zeros((on.shape[0], v.shape[1]), np.float32)
+    L.orc_octree_aggregate_tree(on.shape[0], on.shape[1], on.reshape(-1), v.shape[1], v.reshape(-1),
+                                out.reshape(-1))
     return out
 
 

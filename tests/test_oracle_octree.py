@@ -131,6 +131,13 @@ def test_octree_aggregate_closed_forms(oracle, num_dim):
     v = rng.standard_normal((n, 3)).astype(np.float32)
     agg = oracle.octree_aggregate(v, i2o, on)
     assert np.allclose(agg[0], v.astype(np.float64).sum(0), rtol=1e-4, atol=1e-3)
+    # the kernels' slot order: the same sums as the point-by-point order when
+    # every partial sum is exact, and within rounding otherwise
+    tree = oracle.octree_aggregate_tree(v, on)
+    assert np.allclose(tree, agg, rtol=1e-4, atol=1e-4)
+    ints = rng.integers(-8, 9, (n, 2)).astype(np.float32)
+    assert np.array_equal(oracle.octree_aggregate_tree(ints, on),
+                          oracle.octree_aggregate(ints, i2o, on))
 
 
 def test_unique_and_offset_aggregate(oracle):
