@@ -287,37 +287,69 @@ __device__ __forceinline__ void refit_treelet(const TreeView& v, uint32_t root, 
     }
   }
   __syncthreads();
-  // phase 2
+  // phase 2: a thread's LPT climbs advance in lock-step, one level per round
+  // (a round costs as much as the longest of them, not their sum); the
+  // parent's static links are read before its arrival counter, so a level
+  // costs the counter and the sibling's box.  Two climbs of one thread that
+  // meet at a node arrive in program order; the box each reads was stored in
+  // an earlier round.
+  uint32_t cnode[LPT], cpar[LPT];
+  Box mine[LPT];
+  bool act[LPT];
 #pragma unroll
   for (int r = 0; r < LPT; ++r) {
     const uint32_t j = (uint32_t)r * T + threadIdx.x;
-    uint32_t parent = lpar[r];
-    if (parent == DMSH_U32MAX - 1) continue;
-    uint32_t node = nb + f + j;
-    Box mine = s.leaf[j];
-    for (uint32_t steps = 0;; ++steps) {
+    cpar[r] = lpar[r];
+    act[r] = cpar[r] != DMSH_U32MAX - 1;
+    cnode[r] = nb + f + j;
+    if (act[r]) mine[r] = s.leaf[j];
+  }
+  for (uint32_t steps = 0;; ++steps) {
+    uint2 ch[LPT];
+    uint32_t gp[LPT], old[LPT];
+#pragma unroll
+    for (int r = 0; r < LPT; ++r) {
+      if (!act[r]) continue;
+      const uint32_t parent = cpar[r];
       if (parent < f || parent > l || parent >= nb || steps >= (uint32_t)K) {
         raise_error(err, DMSH_DEVERR_REFIT);
-        break;
+        act[r] = false;
+        continue;
       }
       const uint32_t pj = parent - f;
-      const uint32_t old =
+      ch[r] = s.child[pj];
+      gp[r] = s.parent[pj];
+      old[r] =
           __hip_atomic_fetch_add(&s.flag[pj], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
-      if (old == 0) break;
-      const uint2 ch = s.child[pj];
-      const uint32_t sib = (ch.x == node) ? ch.y : ch.x;
+    }
+    bool more = false;
+#pragma unroll
+    for (int r = 0; r < LPT; ++r) {
+      if (!act[r]) continue;
+      if (old[r] == 0) {  // the sibling subtree finishes this node
+        act[r] = false;
+        continue;
+      }
+      const uint32_t parent = cpar[r], pj = parent - f, node = cnode[r];
+      const uint32_t sib = (ch[r].x == node) ? ch[r].y : ch[r].x;
       const uint32_t si = (sib >= nb) ? sib - nb - f : sib - f;
-      if ((ch.x != node && ch.y != node) || si >= m) {
+      if ((ch[r].x != node && ch[r].y != node) || si >= m) {
         raise_error(err, DMSH_DEVERR_REFIT);
-        break;
+        act[r] = false;
+        continue;
       }
       const Box sb = (sib >= nb) ? s.leaf[si] : s.inner[si];
-      mine = (ch.x == node) ? box_merge(mine, sb) : box_merge(sb, mine);
-      s.inner[pj] = mine;
-      if (parent == root) break;
-      node = parent;
-      parent = s.parent[pj];
+      mine[r] = (ch[r].x == node) ? box_merge(mine[r], sb) : box_merge(sb, mine[r]);
+      s.inner[pj] = mine[r];
+      if (parent == root) {
+        act[r] = false;
+        continue;
+      }
+      cnode[r] = parent;
+      cpar[r] = gp[r];
+      more = true;
     }
+    if (!more) break;
   }
   __syncthreads();
   // phase 3: the nodes that received both arrivals are this treelet's
@@ -536,7 +568,7 @@ __global__ __launch_bounds__(T) void k_refit_top_lds(
 }
 
 constexpr uint32_t kTreeletLeaves = 1024;  // K
-constexpr int kTreeletThreads = 512;
+constexpr int kTreeletThreads = 1024;
 constexpr uint32_t kTreeletGrid = 1024;    // persistent workgroups
 constexpr int kTopThreads = 1024;
 constexpr int kTopSlots = 6400;  // 25 B each: 156 KB of LDS
