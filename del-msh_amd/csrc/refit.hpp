@@ -463,6 +463,131 @@ __global__ __launch_bounds__(T) void k_refit_top(float* __restrict__ aabb,
 constexpr uint32_t kTopRef = 0x80000000u;
 constexpr uint8_t kTopPending = 0xFFu;
 
+// Top nodes merged by climbs through LDS arrival counters (up to C slots):
+// slot e keeps its box, its parent's slot (16 bits) and its top children's
+// slots (16 bits each, 0xFFFF for a child below the top, whose box the
+// slot's own entry holds until the merge), and one byte of a packed counter
+// word.  A slot whose children are both below the top is finished in the
+// load pass and starts a climb; at each parent the arrival that completes
+// its top children merges (left, right) and climbs on, so a chain costs one
+// LDS round trip per level instead of one barrier per level.
+constexpr int kTopClimbSlots = 4096;
+
+template <int C, int T>
+__device__ __forceinline__ void top_climb(float* __restrict__ aabb, uint32_t num_leaf,
+                                          const uint32_t* __restrict__ bvhnodes, uint32_t count,
+                                          const uint2* __restrict__ top_list,
+                                          const uint32_t* __restrict__ node2slot,
+                                          unsigned char* __restrict__ smem,
+                                          uint32_t* __restrict__ err) {
+  constexpr int R = (C + T - 1) / T;
+  constexpr uint32_t kNone = 0xFFFFu;
+  Box* s_box = reinterpret_cast<Box*>(smem);
+  uint16_t* s_par = reinterpret_cast<uint16_t*>(smem + (size_t)C * sizeof(Box));
+  uint32_t* s_kid = reinterpret_cast<uint32_t*>(smem + (size_t)C * (sizeof(Box) + 2));
+  uint32_t* s_cnt = reinterpret_cast<uint32_t*>(smem + (size_t)C * (sizeof(Box) + 6));
+  const uint32_t nb = num_leaf - 1;
+  for (uint32_t w = threadIdx.x; w < (count + 3) / 4; w += T) s_cnt[w] = 0u;
+  // load pass, stage by stage over this thread's slots e = r T + tid, so
+  // each stage's loads are in flight together: the records, then the node
+  // rows, then the children's / parent's slots and the boxes of children
+  // below the top
+  uint2 rec[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const uint32_t e = (uint32_t)r * T + threadIdx.x;
+    rec[r] = e < count ? top_list[e] : make_uint2(DMSH_U32MAX, 0u);
+  }
+  uint32_t rp[R], ra[R], rb[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const bool ok = rec[r].x < nb;
+    const uint32_t* row = bvhnodes + (size_t)(ok ? rec[r].x : 0u) * 3;
+    rp[r] = ok ? row[0] : DMSH_U32MAX;
+    ra[r] = ok ? row[1] : 0u;
+    rb[r] = ok ? row[2] : 0u;
+  }
+  uint32_t sa[R], sb[R], sp[R];
+  bool bad = false;
+  uint32_t start = 0u;
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const uint32_t e = (uint32_t)r * T + threadIdx.x;
+    sa[r] = sb[r] = sp[r] = kNone;
+    if (e >= count) continue;
+    if (rec[r].x >= nb) {
+      bad = true;
+      continue;
+    }
+    const bool ta = rec[r].y & 1u, tb = rec[r].y & 2u;
+    const uint32_t a = ra[r], b = rb[r], par = rp[r];
+    if ((ta ? a >= nb : a > 2 * nb) || (tb ? b >= nb : b > 2 * nb) ||
+        (par != DMSH_U32MAX && par >= nb)) {
+      bad = true;
+      continue;
+    }
+    if (ta) sa[r] = node2slot[a];
+    if (tb) sb[r] = node2slot[b];
+    if (par != DMSH_U32MAX) sp[r] = node2slot[par];
+  }
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const uint32_t e = (uint32_t)r * T + threadIdx.x;
+    if (e >= count || rec[r].x >= nb) continue;
+    const bool ta = rec[r].y & 1u, tb = rec[r].y & 2u;
+    if (!ta && !tb) {
+      s_box[e] = box_merge(box_load(aabb, ra[r]), box_load(aabb, rb[r]));
+      start |= 1u << r;
+    } else if (!ta || !tb) {
+      s_box[e] = box_load(aabb, ta ? rb[r] : ra[r]);  // the child below the top
+    }
+    if ((ta && sa[r] >= count) || (tb && sb[r] >= count) ||
+        (rp[r] != DMSH_U32MAX && sp[r] >= count)) {
+      bad = true;
+      continue;
+    }
+    s_kid[e] = sa[r] | (sb[r] << 16);
+    s_par[e] = (uint16_t)sp[r];
+  }
+  if (__syncthreads_or(bad)) {
+    if (threadIdx.x == 0) raise_error(err, DMSH_DEVERR_REFIT);
+    return;
+  }
+  // climbs; the next parent's static slot and links are read while the
+  // current parent's counter is in flight
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    if (!(start & (1u << r))) continue;
+    uint32_t p = s_par[(uint32_t)r * T + threadIdx.x];
+    for (uint32_t steps = 0; p != kNone; ++steps) {
+      if (steps >= count) {
+        raise_error(err, DMSH_DEVERR_REFIT);
+        break;
+      }
+      const uint32_t kid = s_kid[p];
+      const uint32_t pnext = s_par[p];
+      const uint32_t la = kid & kNone, lb = kid >> 16;
+      const uint32_t need = (la != kNone ? 1u : 0u) + (lb != kNone ? 1u : 0u);
+      const uint32_t sh = 8u * (p & 3u);
+      const uint32_t old = (__hip_atomic_fetch_add(s_cnt + (p >> 2), 1u << sh, __ATOMIC_ACQ_REL,
+                                                   __HIP_MEMORY_SCOPE_WORKGROUP) >>
+                            sh) &
+                           0xFFu;
+      if (old + 1u < need) break;  // the other top child finishes this node
+      const Box A = la != kNone ? s_box[la] : s_box[p];
+      const Box B = lb != kNone ? s_box[lb] : s_box[p];
+      s_box[p] = box_merge(A, B);
+      p = pnext;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const uint32_t e = (uint32_t)r * T + threadIdx.x;
+    if (e < count) box_store(aabb, rec[r].x, s_box[e]);
+  }
+}
+
 template <int C, int T>
 __global__ __launch_bounds__(T) void k_refit_top_lds(
     float* __restrict__ aabb, uint32_t* __restrict__ flags, uint32_t num_leaf,
@@ -471,16 +596,23 @@ __global__ __launch_bounds__(T) void k_refit_top_lds(
     const uint32_t* __restrict__ treelet_count, const uint4* __restrict__ treelets,
     uint32_t* __restrict__ err) {
   constexpr int R = (C + T - 1) / T;  // slots per thread
+  static_assert((size_t)kTopClimbSlots * (sizeof(Box) + 7) <= (size_t)C * (sizeof(Box) + 1),
+                "the climb layout fits in the round layout's LDS");
+  __shared__ __align__(16) unsigned char smem[(size_t)C * (sizeof(Box) + 1)];
   const TreeView v{aabb, bvhnodes, num_leaf, nullptr, 0u, nullptr};
   const uint32_t count = *top_count;
+  if (count <= (uint32_t)kTopClimbSlots) {
+    top_climb<kTopClimbSlots, T>(aabb, num_leaf, bvhnodes, count, top_list, node2slot, smem, err);
+    return;
+  }
   if (count > (uint32_t)C) {
     uint32_t nt = *treelet_count;
     if (nt > num_leaf) nt = num_leaf;
     refit_top<T>(v, flags, reinterpret_cast<const uint32_t*>(treelets), 4u, nt, err);
     return;
   }
-  __shared__ Box s_box[C];
-  __shared__ uint8_t s_done[C];  // round that finished the slot, or kTopPending
+  Box* s_box = reinterpret_cast<Box*>(smem);
+  uint8_t* s_done = smem + (size_t)C * sizeof(Box);  // round that finished the slot, or kTopPending
   const uint32_t nb = num_leaf - 1;
   // phase 1, stage by stage over this thread's slots e = r T + tid
   uint2 rec[R];
