@@ -12,7 +12,7 @@ namespace {
 
 constexpr uint32_t kCastCounts = 256;  // see bat::BatchCast::counts
 
-std::atomic<int> g_batch_stack{24};      // LDS frames per lane of the batched tier 1
+std::atomic<int> g_batch_stack{20};      // LDS frames per lane of the batched tier 1
 std::atomic<int> g_batch_wgs_per_cu{16};  // persistent single-wave workgroups per CU
 std::atomic<int> g_batch_treelet_grid{1024};
 
@@ -164,7 +164,7 @@ extern "C" {
 int dmsh_batch_set_tuning(const char* key, int value) {
   if (!key) return DMSH_ERR_INVALID;
   if (std::strcmp(key, "batch.stack") == 0) {
-    if (value != 16 && value != 24 && value != 32) return DMSH_ERR_INVALID;
+    if (value != 16 && value != 20 && value != 24 && value != 32) return DMSH_ERR_INVALID;
     g_batch_stack.store(value);
     return DMSH_OK;
   }
@@ -324,6 +324,7 @@ int dmsh_batch_pix2tri(void* stream, uint32_t num_mesh, uint32_t total_tri, uint
   const uint32_t grid = (uint32_t)device_cus() * (uint32_t)g_batch_wgs_per_cu.load();
   switch (g_batch_stack.load()) {
     case 16: launch_batch_tier1<16>(s, grid, a); break;
+    case 20: launch_batch_tier1<20>(s, grid, a); break;
     case 32: launch_batch_tier1<32>(s, grid, a); break;
     default: launch_batch_tier1<24>(s, grid, a); break;
   }
@@ -411,6 +412,7 @@ int dmsh_cast_requests(void* stream, uint32_t num_req, const dmsh_cast_request* 
   const uint32_t grid = (uint32_t)device_cus() * (uint32_t)g_batch_wgs_per_cu.load();
   switch (g_batch_stack.load()) {
     case 16: launch_req_tier1<16>(s, grid, a); break;
+    case 20: launch_req_tier1<20>(s, grid, a); break;
     case 32: launch_req_tier1<32>(s, grid, a); break;
     default: launch_req_tier1<24>(s, grid, a); break;
   }

@@ -15,7 +15,7 @@ namespace {
 // whose deepest ray needs 24 frames: 16 frames x 1 wave 3.36 ms (some rays
 // restart in tier 2), 24 x 1 3.37 ms (none do), 24 x 2 3.53 ms, 32 x 1
 // 3.98 ms, 32 x 4 5.12 ms.
-std::atomic<int> g_cast_stack{24};  // LDS frames per lane (tier 1)
+std::atomic<int> g_cast_stack{20};  // LDS frames per lane (tier 1)
 std::atomic<int> g_cast_waves{1};   // waves per workgroup (tier 1)
 
 struct CastLayout {
@@ -57,7 +57,9 @@ bool launch_tier1_waves(int waves, hipStream_t s, const rc::Packed& pk, const fl
 bool launch_tier1_any(int stack, int waves, hipStream_t s, const rc::Packed& pk, const float* T,
                       const rc::CastArgs& a) {
   switch (stack) {
+    case 12: return launch_tier1_waves<12>(waves, s, pk, T, a);
     case 16: return launch_tier1_waves<16>(waves, s, pk, T, a);
+    case 20: return launch_tier1_waves<20>(waves, s, pk, T, a);
     case 24: return launch_tier1_waves<24>(waves, s, pk, T, a);
     case 32: return launch_tier1_waves<32>(waves, s, pk, T, a);
     default: return false;
@@ -71,7 +73,8 @@ extern "C" {
 int dmsh_set_tuning(const char* key, int value) {
   if (!key) return DMSH_ERR_INVALID;
   if (std::strcmp(key, "pix2tri.stack") == 0) {
-    if (value != 16 && value != 24 && value != 32) return DMSH_ERR_INVALID;
+    if (value != 12 && value != 16 && value != 20 && value != 24 && value != 32)
+      return DMSH_ERR_INVALID;
     g_cast_stack.store(value);
     return DMSH_OK;
   }

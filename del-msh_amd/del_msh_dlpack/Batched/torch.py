@@ -178,7 +178,7 @@ def _time_ms(fn, reps: int) -> float:
     return start.elapsed_time(stop) / reps
 
 
-def autotune(p: Pipeline, reps: int = 2, stacks=(16, 24, 32), wgs_per_cu=(4, 8, 16, 32),
+def autotune(p: Pipeline, reps: int = 2, stacks=(16, 20, 24, 32), wgs_per_cu=(4, 8, 16, 32),
              treelet_grids=(512, 1024, 2048)):
     """Times the launch parameters of the batched kernels on this batch and
     keeps the fastest (results do not depend on them).  The ray-cast is tuned

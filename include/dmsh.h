@@ -49,8 +49,8 @@ int dmsh_abi_version(void);
 int dmsh_get_device_errors(uint32_t *bits_out);
 
 /* Process-wide launch tuning (no reference counterpart; set by the
- * autotuner).  Keys: "pix2tri.stack" (LDS frames per lane: 16, 24, 32),
- * "pix2tri.waves" (waves per workgroup: 1, 2, 4).  get returns -1 for an
+ * autotuner).  Keys: "pix2tri.stack" (LDS frames per lane: 12, 16, 20, 24,
+ * 32; default 20), "pix2tri.waves" (waves per workgroup: 1, 2, 4).  get returns -1 for an
  * unknown key. */
 int dmsh_set_tuning(const char *key, int value);
 int dmsh_get_tuning(const char *key);
@@ -357,7 +357,8 @@ int dmsh_antialias_bwd(void *stream, uint32_t num_cedge,
  * Ray-cast: image of mesh m = pixels [mesh2pix[m], mesh2pix[m+1]) of
  * pix2tri / pix2t (nullable), mesh2width[m] pixels wide, row 0 at the top,
  * cast with transforms_ndc2world + 16 m (column-major).
- * Tuning keys (results do not depend on them): "batch.stack" (16, 24, 32),
+ * Tuning keys (results do not depend on them): "batch.stack" (16, 20, 24,
+ * 32; default 20),
  * "batch.wgs_per_cu" (persistent ray-cast workgroups per CU, 1..64),
  * "batch.treelet_grid" (refit workgroups, 1..65536).
  * ------------------------------------------------------------------- */
