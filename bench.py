@@ -651,6 +651,55 @@ def main():
     build_mtri = num_tri / (build_avg * 1e-3) / 1e6
     cast_mrays = num_pix / (cast_avg * 1e-3) / 1e6
 
+    # ---------------- standalone refit of the built tree ----------------
+    # dmsh_bvhnode2aabb_trimesh3 on the tree just built (the reference's
+    # bvhnode2aabb call, as an optimisation loop makes it after moving the
+    # vertices): leaf ranges from the rows, treelet list, treelets, top.
+    r_aabb = torch.empty_like(aabb)
+    ws_refit = _lib.workspace(L.dmsh_refit_workspace_bytes(num_tri), dev)
+
+    def refit():
+        _lib.check(L.dmsh_bvhnode2aabb_trimesh3(sp, r_aabb.data_ptr(), num_node, bvhnodes.data_ptr(),
+                                                num_tri, t_tri.data_ptr(), num_vtx,
+                                                t_vtx.data_ptr(), 0.0, ws_refit.data_ptr(),
+                                                ws_refit.numel()), "refit")
+
+    for _ in range(3):
+        refit()
+    r_ev = []
+    for _ in range(20):
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        refit()
+        b.record(stream)
+        r_ev.append((a, b))
+    torch.cuda.synchronize()
+    refit_ms = float(np.median([a.elapsed_time(b) for a, b in r_ev]))
+    _lib.timing_enable(True)
+    for _ in range(10):
+        refit()
+    torch.cuda.synchronize()
+    _lib.timing_enable(False)
+    refit_kernels = {}
+    for name in ("k_leaf_ranges", "k_list_treelets", "k_refit_treelet", "k_refit_top", "k_refit"):
+        n, total, _ = _lib.timing_query(name)
+        if n:
+            refit_kernels[name] = round(total / n, 5)
+    refit_same = bool(torch.equal(r_aabb.view(torch.int32), aabb.view(torch.int32)))
+    # algorithmic bytes per triangle (refit basis): read tri2vtx 12 + vertices
+    # 12 V/N ~ 6 + node rows 12 (2N-1)/N ~ 24; write boxes 24 (2N-1)/N ~ 48;
+    # arrival counters 8 => 98 B
+    refit_bytes = 98 * num_tri
+    refit_line = {
+        "entry": "dmsh_bvhnode2aabb_trimesh3 (stored tree, treelet path)",
+        "ms": round(refit_ms, 4),
+        "mtri_per_s": round(num_tri / (refit_ms * 1e-3) / 1e6, 1),
+        "algorithmic_gbs": round(refit_bytes / (refit_ms * 1e-3) / 1e9, 1),
+        "frac_of_hbm_peak": round(refit_bytes / (refit_ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4),
+        "kernels_avg_ms": refit_kernels,
+        "equals_build_boxes": refit_same,
+    }
+
     # ---------------- secondary: 2^22-point build and sort alone ----------------
     rng = np.random.default_rng(2)
     pts = torch.from_numpy(rng.random((1 << 22, 3), dtype=np.float32)).to(dev)
@@ -797,6 +846,7 @@ def main():
         "build_algorithmic_gbs": round(build_bytes / (build_avg * 1e-3) / 1e9, 1),
         "raycast_ms": round(cast_avg, 4),
         "raycast_mrays_per_s": round(cast_mrays, 2),
+        "refit_standalone": refit_line,
         "points_build_2p22_ms": round(pbuild_ms, 4),
         "points_build_mpts_per_s": round(npnt / (pbuild_ms * 1e-3) / 1e6, 2),
         "sort_alone_u64_2p22_ms": round(psort_ms, 4),

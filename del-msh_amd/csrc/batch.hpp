@@ -311,7 +311,7 @@ __global__ __launch_bounds__(T) void k_batch_refit_treelet(BatchTrees b, uint32_
       if (threadIdx.x == 0) raise_error(err, DMSH_DEVERR_REFIT);
       continue;
     }
-    refit_treelet<true, K, T>(tree_of(b, tl.w), tl.x, tl.y, tl.z, 0.f, s, err);
+    refit_treelet<kLeafTri, K, T>(tree_of(b, tl.w), tl.x, tl.y, tl.z, 0.f, s, err);
   }
 }
 

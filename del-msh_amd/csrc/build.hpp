@@ -259,9 +259,10 @@ inline int build(hipStream_t s, uint32_t n, const uint32_t* tri2vtx, uint32_t nu
                                         nullptr, L.top_count, L.top_list, (uint32_t)kTopSlots,
                                         L.node2slot});
   if (rc != DMSH_OK) return rc;
-  return launch_refit_treelets<TRI>(s, aabb, L.flags, L.treelet_count, L.treelets, n, bvhnodes,
-                                    tri2vtx, num_vtx, xyz, 0.f, err, L.top_count, L.top_list,
-                                    L.node2slot, L.treelet_next, L.tri_box);
+  const TreeView v{aabb, bvhnodes, n, tri2vtx, num_vtx, xyz, L.tri_box};
+  return launch_refit_treelets<TRI ? kLeafTri : kLeafPoint>(
+      s, v, L.flags, L.treelet_count, L.treelets, 0.f, err, L.top_count, L.top_list, L.node2slot,
+      L.treelet_next);
 }
 
 }  // namespace bld

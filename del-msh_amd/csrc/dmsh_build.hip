@@ -65,7 +65,7 @@ int dmsh_bvhnodes_from_sorted_mortons_u64(void* stream, uint32_t n, uint32_t* bv
 }
 
 size_t dmsh_refit_workspace_bytes(uint32_t num_leaf) {
-  return align_up(sizeof(uint32_t) * (num_leaf > 1 ? num_leaf - 1 : 1), 256);
+  return standalone_workspace_bytes(num_leaf);
 }
 
 static int refit_common(void* stream, bool tri, float* aabb, uint32_t num_bvhnode,
@@ -79,11 +79,12 @@ static int refit_common(void* stream, bool tri, float* aabb, uint32_t num_bvhnod
   uint32_t* err = device_error_word();
   if (!err) return DMSH_ERR_HIP;
   hipStream_t s = as_stream(stream);
-  uint32_t* flags = static_cast<uint32_t*>(ws);
-  int rc = sortk::launch_fill(s, flags, num_leaf > 1 ? num_leaf - 1 : 1, 0u);
-  if (rc != DMSH_OK) return rc;
-  if (tri) return launch_refit<true>(s, aabb, flags, num_leaf, bvhnodes, tri2vtx, num_vtx, xyz, eps, err);
-  return launch_refit<false>(s, aabb, flags, num_leaf, bvhnodes, nullptr, num_leaf, xyz, 0.f, err);
+  if (tri) {
+    const TreeView v{aabb, bvhnodes, num_leaf, tri2vtx, num_vtx, xyz};
+    return launch_refit_standalone<kLeafTri>(s, v, eps, ws, ws_bytes, err);
+  }
+  const TreeView v{aabb, bvhnodes, num_leaf, nullptr, num_leaf, xyz};
+  return launch_refit_standalone<kLeafPoint>(s, v, 0.f, ws, ws_bytes, err);
 }
 
 int dmsh_bvhnode2aabb_trimesh3(void* stream, float* bvhnode2aabb, uint32_t num_bvhnode,

@@ -29,7 +29,7 @@ int dmsh_polyhedron_elem2center(void* stream, uint32_t num_elem, const uint32_t*
 }
 
 size_t dmsh_polyhedron_refit_workspace_bytes(uint32_t num_elem) {
-  return sizeof(uint32_t) * (size_t)(num_elem > 1 ? num_elem - 1 : 1) + 256;
+  return standalone_workspace_bytes(num_elem);
 }
 
 int dmsh_bvhnode2aabb_polyhedron(void* stream, uint32_t num_bvhnode, const uint32_t* bvhnodes,
@@ -43,14 +43,11 @@ int dmsh_bvhnode2aabb_polyhedron(void* stream, uint32_t num_bvhnode, const uint3
     return DMSH_ERR_INVALID;
   if (!ws || ws_bytes < dmsh_polyhedron_refit_workspace_bytes(num_elem))
     return DMSH_ERR_WORKSPACE;
-  hipStream_t s = as_stream(stream);
-  uint32_t* flags = static_cast<uint32_t*>(ws);
-  int rc = sortk::launch_fill(s, flags, num_elem > 1 ? num_elem - 1 : 1, 0u);
-  if (rc != DMSH_OK) return rc;
-  launch("k_refit_csr", k_refit_csr, grid_for(num_elem), dim3(256), 0, s, bvhnode2aabb, flags,
-         num_elem, bvhnodes, elem2idx_offset, num_idx, idx2vtx, num_vtx, vtx2xyz,
-         device_error_word());
-  return launch_status();
+  uint32_t* err = device_error_word();
+  if (!err) return DMSH_ERR_HIP;
+  const TreeView v{bvhnode2aabb, bvhnodes, num_elem, idx2vtx, num_vtx, vtx2xyz, nullptr,
+                   elem2idx_offset, num_idx};
+  return launch_refit_standalone<kLeafElem>(as_stream(stream), v, 0.f, ws, ws_bytes, err);
 }
 
 int dmsh_polyhedron_search_elem_contain_points(void* stream, uint32_t num_elem,

@@ -18,7 +18,9 @@
 // enough on a weakly ordered multi-L2 part.
 //
 // Trees built here take the treelet refit below instead (same merge order,
-// same results, no agent-scope atomics).
+// same results, no agent-scope atomics), and so does the standalone refit of
+// any tree whose node ranges pass the LBVH-layout check (k_leaf_ranges,
+// k_list_treelets); the generic climb is its fallback for other trees.
 #pragma once
 
 #include <algorithm>
@@ -102,85 +104,13 @@ __device__ __forceinline__ void refit_climb(float* __restrict__ aabb, uint32_t* 
   }
 }
 
-// TRI: leaves index triangles of tri2vtx; otherwise leaves index points.
-template <bool TRI>
-__global__ __launch_bounds__(256) void k_refit(float* __restrict__ aabb,
-                                               uint32_t* __restrict__ flags, uint32_t num_leaf,
-                                               const uint32_t* __restrict__ bvhnodes,
-                                               const uint32_t* __restrict__ tri2vtx,
-                                               uint32_t num_vtx, const float* __restrict__ xyz,
-                                               float eps, uint32_t* __restrict__ err) {
-  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= num_leaf) return;
-  const uint32_t nb = num_leaf - 1;
-  const uint32_t node = nb + k;
-  const uint32_t* row = bvhnodes + (size_t)node * 3;
-  const uint32_t parent0 = row[0], obj = row[1], tag = row[2];
-  if (tag != DMSH_U32MAX || obj >= (TRI ? num_leaf : num_vtx)) {
-    raise_error(err, DMSH_DEVERR_INDEX);
-    return;
-  }
-  Box mine;
-  if (TRI) {
-    const uint32_t* tv = tri2vtx + (size_t)obj * 3;
-    const uint32_t i0 = tv[0], i1 = tv[1], i2 = tv[2];
-    if (i0 >= num_vtx || i1 >= num_vtx || i2 >= num_vtx) {
-      raise_error(err, DMSH_DEVERR_INDEX);
-      return;
-    }
-    box_set(mine, xyz + (size_t)i0 * 3, eps);
-    box_add(mine, xyz + (size_t)i1 * 3, eps);
-    box_add(mine, xyz + (size_t)i2 * 3, eps);
-  } else {
-    box_set(mine, xyz + (size_t)obj * 3, 0.f);
-  }
-  box_store(aabb, node, mine);
-  refit_climb(aabb, flags, nb, bvhnodes, node, parent0, mine, err);
-}
-
-// Leaves index elements of a CSR polygon / polyhedron mesh: the box of the
-// element's vertices, first vertex first (bvhnode2aabb3.rs:211-228 with
-// margin 0; bvhnode2aabb.cu:67-122).  An empty element or an index out of
-// range sets DMSH_DEVERR_INDEX.
-static __global__ __launch_bounds__(256) void k_refit_csr(float* __restrict__ aabb,
-                                                   uint32_t* __restrict__ flags, uint32_t num_elem,
-                                                   const uint32_t* __restrict__ bvhnodes,
-                                                   const uint32_t* __restrict__ elem2idx,
-                                                   uint32_t num_idx,
-                                                   const uint32_t* __restrict__ idx2vtx,
-                                                   uint32_t num_vtx, const float* __restrict__ xyz,
-                                                   uint32_t* __restrict__ err) {
-  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= num_elem) return;
-  const uint32_t nb = num_elem - 1;
-  const uint32_t node = nb + k;
-  const uint32_t* row = bvhnodes + (size_t)node * 3;
-  const uint32_t parent0 = row[0], obj = row[1], tag = row[2];
-  if (tag != DMSH_U32MAX || obj >= num_elem) {
-    raise_error(err, DMSH_DEVERR_INDEX);
-    return;
-  }
-  const uint32_t i0 = elem2idx[obj], i1 = elem2idx[obj + 1];
-  if (i0 >= i1 || i1 > num_idx) {
-    raise_error(err, DMSH_DEVERR_INDEX);
-    return;
-  }
-  Box mine;
-  for (uint32_t i = i0; i < i1; ++i) {
-    const uint32_t v = idx2vtx[i];
-    if (v >= num_vtx) {
-      raise_error(err, DMSH_DEVERR_INDEX);
-      return;
-    }
-    if (i == i0) box_set(mine, xyz + (size_t)v * 3, 0.f);
-    else box_add(mine, xyz + (size_t)v * 3, 0.f);
-  }
-  box_store(aabb, node, mine);
-  refit_climb(aabb, flags, nb, bvhnodes, node, parent0, mine, err);
-}
+// What a leaf's object is, and so how its box is made.
+constexpr int kLeafPoint = 0;  // point obj of xyz (margin 0)
+constexpr int kLeafTri = 1;    // triangle obj of tri2vtx (margin eps)
+constexpr int kLeafElem = 2;   // element obj of a CSR mesh: elem2idx over idx2vtx (margin 0)
 
 // One tree (one mesh of a batch): node rows and boxes, and the leaves'
-// geometry (triangles over vertices, or points).
+// geometry.  For kLeafElem, tri2vtx is the CSR vertex list idx2vtx.
 struct TreeView {
   float* aabb;
   const uint32_t* bvhnodes;
@@ -189,40 +119,76 @@ struct TreeView {
   uint32_t num_vtx;
   const float* xyz;
   const float* tri_box = nullptr;  // per-triangle boxes in triangle order, if built
+  const uint32_t* elem2idx = nullptr;  // kLeafElem: num_leaf + 1 offsets into tri2vtx
+  uint32_t num_idx = 0;
 };
 
-// Leaf box of leaf k; returns false (and raises) on a bad index.  With
-// tri_box (the triangle boxes the build's centroid pass wrote, eps = 0) the
-// box is one 24-byte read.
-template <bool TRI>
-__device__ __forceinline__ bool leaf_box(const uint32_t* __restrict__ bvhnodes, uint32_t num_leaf,
-                                         uint32_t k, const uint32_t* __restrict__ tri2vtx,
-                                         uint32_t num_vtx, const float* __restrict__ xyz, float eps,
-                                         Box& box, uint32_t& parent, uint32_t* err,
-                                         const float* __restrict__ tri_box = nullptr) {
-  const uint32_t* row = bvhnodes + (size_t)(num_leaf - 1 + k) * 3;
+// Leaf box of leaf k and its parent; returns false (and raises) on a bad
+// index.  Points: the point.  Triangles: the box of the three corners grown
+// by eps (vtx2xyz.rs:39-61), or, with tri_box (the triangle boxes the
+// build's centroid pass wrote, eps = 0), one 24-byte read.  Elements: the box
+// of the element's vertices, first vertex first (bvhnode2aabb3.rs:211-228
+// with margin 0; bvhnode2aabb.cu:67-122); an empty element is an index error.
+template <int LK>
+__device__ __forceinline__ bool leaf_box(const TreeView& v, uint32_t k, float eps, Box& box,
+                                         uint32_t& parent, uint32_t* err) {
+  const uint32_t* row = v.bvhnodes + (size_t)(v.num_leaf - 1 + k) * 3;
   const uint32_t p = row[0], obj = row[1], tag = row[2];
-  if (tag != DMSH_U32MAX || obj >= (TRI ? num_leaf : num_vtx)) {
+  if (tag != DMSH_U32MAX || obj >= (LK == kLeafPoint ? v.num_vtx : v.num_leaf)) {
     raise_error(err, DMSH_DEVERR_INDEX);
     return false;
   }
-  if (TRI && tri_box) {
-    box = box_load(tri_box, obj);
-  } else if (TRI) {
-    const uint32_t* tv = tri2vtx + (size_t)obj * 3;
+  if (LK == kLeafTri && v.tri_box) {
+    box = box_load(v.tri_box, obj);
+  } else if (LK == kLeafTri) {
+    const uint32_t* tv = v.tri2vtx + (size_t)obj * 3;
     const uint32_t i0 = tv[0], i1 = tv[1], i2 = tv[2];
-    if (i0 >= num_vtx || i1 >= num_vtx || i2 >= num_vtx) {
+    if (i0 >= v.num_vtx || i1 >= v.num_vtx || i2 >= v.num_vtx) {
       raise_error(err, DMSH_DEVERR_INDEX);
       return false;
     }
-    box_set(box, xyz + (size_t)i0 * 3, eps);
-    box_add(box, xyz + (size_t)i1 * 3, eps);
-    box_add(box, xyz + (size_t)i2 * 3, eps);
+    box_set(box, v.xyz + (size_t)i0 * 3, eps);
+    box_add(box, v.xyz + (size_t)i1 * 3, eps);
+    box_add(box, v.xyz + (size_t)i2 * 3, eps);
+  } else if (LK == kLeafElem) {
+    const uint32_t i0 = v.elem2idx[obj], i1 = v.elem2idx[obj + 1];
+    if (i0 >= i1 || i1 > v.num_idx) {
+      raise_error(err, DMSH_DEVERR_INDEX);
+      return false;
+    }
+    for (uint32_t i = i0; i < i1; ++i) {
+      const uint32_t q = v.tri2vtx[i];
+      if (q >= v.num_vtx) {
+        raise_error(err, DMSH_DEVERR_INDEX);
+        return false;
+      }
+      if (i == i0) box_set(box, v.xyz + (size_t)q * 3, 0.f);
+      else box_add(box, v.xyz + (size_t)q * 3, 0.f);
+    }
   } else {
-    box_set(box, xyz + (size_t)obj * 3, 0.f);
+    box_set(box, v.xyz + (size_t)obj * 3, 0.f);
   }
   parent = p;
   return true;
+}
+
+// The generic climb, one thread per leaf (grid-stride).  With `gate`, the
+// kernel runs only if *gate is nonzero: the standalone refit sets it for a
+// tree that fails the LBVH-layout check.
+template <int LK>
+__global__ __launch_bounds__(256) void k_refit(TreeView v, uint32_t* __restrict__ flags, float eps,
+                                               const uint32_t* __restrict__ gate,
+                                               uint32_t* __restrict__ err) {
+  if (gate && *gate == 0u) return;
+  const uint32_t nb = v.num_leaf - 1;
+  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < v.num_leaf;
+       k += gridDim.x * blockDim.x) {
+    Box mine;
+    uint32_t parent;
+    if (!leaf_box<LK>(v, k, eps, mine, parent, err)) continue;
+    box_store(v.aabb, nb + k, mine);
+    refit_climb(v.aabb, flags, nb, v.bvhnodes, nb + k, parent, mine, err);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -251,7 +217,7 @@ struct TreeletLds {
   uint32_t parent[K];
 };
 
-template <bool TRI, int K, int T>
+template <int LK, int K, int T>
 __device__ __forceinline__ void refit_treelet(const TreeView& v, uint32_t root, uint32_t f,
                                               uint32_t l, float eps, TreeletLds<K>& s,
                                               uint32_t* __restrict__ err) {
@@ -279,8 +245,7 @@ __device__ __forceinline__ void refit_treelet(const TreeView& v, uint32_t root, 
     }
     Box b;
     uint32_t p;
-    if (leaf_box<TRI>(v.bvhnodes, num_leaf, id, v.tri2vtx, v.num_vtx, v.xyz, eps, b, p, err,
-                      v.tri_box)) {
+    if (leaf_box<LK>(v, id, eps, b, p, err)) {
       s.leaf[j] = b;
       box_store(v.aabb, nb + id, b);
       if (nb + id != root) lpar[r] = p;
@@ -401,22 +366,24 @@ __device__ __forceinline__ void refit_top(const TreeView& v, uint32_t* __restric
 // Persistent grid over the treelets of one tree.  With a zeroed `next`
 // counter the workgroups take treelets from it one at a time (treelet sizes
 // vary from 1 to K leaves, so a fixed deal would leave workgroups idle);
-// without it, treelet t goes to workgroup t mod grid.
-template <bool TRI, int K, int T>
-__global__ __launch_bounds__(T) void k_refit_treelet(
-    float* __restrict__ aabb, uint32_t num_leaf, const uint32_t* __restrict__ bvhnodes,
-    const uint32_t* __restrict__ tri2vtx, uint32_t num_vtx, const float* __restrict__ xyz,
-    float eps, const uint32_t* __restrict__ treelet_count, const uint4* __restrict__ treelets,
-    uint32_t* __restrict__ next, const float* __restrict__ tri_box, uint32_t* __restrict__ err) {
+// without it, treelet t goes to workgroup t mod grid.  A nonzero *skip (the
+// standalone refit's layout check failed) makes the kernel do nothing.
+template <int LK, int K, int T>
+__global__ __launch_bounds__(T) void k_refit_treelet(TreeView v, float eps,
+                                                     const uint32_t* __restrict__ treelet_count,
+                                                     const uint4* __restrict__ treelets,
+                                                     uint32_t* __restrict__ next,
+                                                     const uint32_t* __restrict__ skip,
+                                                     uint32_t* __restrict__ err) {
   __shared__ TreeletLds<K> s;
   __shared__ uint32_t s_t;
-  const TreeView v{aabb, bvhnodes, num_leaf, tri2vtx, num_vtx, xyz, tri_box};
+  if (skip && *skip) return;
   uint32_t count = *treelet_count;
-  if (count > num_leaf) count = num_leaf;
+  if (count > v.num_leaf) count = v.num_leaf;
   if (!next) {
     for (uint32_t t = blockIdx.x; t < count; t += gridDim.x) {
       const uint4 tl = treelets[t];
-      refit_treelet<TRI, K, T>(v, tl.x, tl.y, tl.z, eps, s, err);
+      refit_treelet<LK, K, T>(v, tl.x, tl.y, tl.z, eps, s, err);
     }
     return;
   }
@@ -427,7 +394,7 @@ __global__ __launch_bounds__(T) void k_refit_treelet(
     __syncthreads();  // every thread has read s_t before it is reused
     if (t >= count) break;
     const uint4 tl = treelets[t];
-    refit_treelet<TRI, K, T>(v, tl.x, tl.y, tl.z, eps, s, err);
+    refit_treelet<LK, K, T>(v, tl.x, tl.y, tl.z, eps, s, err);
   }
 }
 
@@ -594,11 +561,12 @@ __global__ __launch_bounds__(T) void k_refit_top_lds(
     const uint32_t* __restrict__ bvhnodes, const uint32_t* __restrict__ top_count,
     const uint2* __restrict__ top_list, const uint32_t* __restrict__ node2slot,
     const uint32_t* __restrict__ treelet_count, const uint4* __restrict__ treelets,
-    uint32_t* __restrict__ err) {
+    const uint32_t* __restrict__ skip, uint32_t* __restrict__ err) {
   constexpr int R = (C + T - 1) / T;  // slots per thread
   static_assert((size_t)kTopClimbSlots * (sizeof(Box) + 7) <= (size_t)C * (sizeof(Box) + 1),
                 "the climb layout fits in the round layout's LDS");
   __shared__ __align__(16) unsigned char smem[(size_t)C * (sizeof(Box) + 1)];
+  if (skip && *skip) return;
   const TreeView v{aabb, bvhnodes, num_leaf, nullptr, 0u, nullptr};
   const uint32_t count = *top_count;
   if (count <= (uint32_t)kTopClimbSlots) {
@@ -707,42 +675,237 @@ constexpr int kTopSlots = 6400;  // 25 B each: 156 KB of LDS
 
 // flags: num_leaf - 1 zeroed counters; treelets (and, when top_count is not
 // null, the top nodes with top_cap = kTopSlots) as listed by the topology
-// kernel with k = kTreeletLeaves.
-template <bool TRI>
-inline int launch_refit_treelets(hipStream_t s, float* aabb, uint32_t* flags,
-                                 const uint32_t* treelet_count, const uint4* treelets,
-                                 uint32_t num_leaf, const uint32_t* bvhnodes,
-                                 const uint32_t* tri2vtx, uint32_t num_vtx, const float* xyz,
-                                 float eps, uint32_t* err, const uint32_t* top_count = nullptr,
+// kernel (or k_list_treelets) with k = kTreeletLeaves.  A nonzero *skip makes
+// both kernels do nothing.
+template <int LK>
+inline int launch_refit_treelets(hipStream_t s, const TreeView& v, uint32_t* flags,
+                                 const uint32_t* treelet_count, const uint4* treelets, float eps,
+                                 uint32_t* err, const uint32_t* top_count = nullptr,
                                  const uint2* top_list = nullptr,
                                  const uint32_t* node2slot = nullptr,
                                  uint32_t* treelet_next = nullptr,
-                                 const float* tri_box = nullptr) {
+                                 const uint32_t* skip = nullptr) {
+  const uint32_t num_leaf = v.num_leaf;
   if (num_leaf == 0) return DMSH_OK;
   const uint32_t grid = std::min<uint32_t>(kTreeletGrid, num_leaf);
-  launch("k_refit_treelet", (k_refit_treelet<TRI, (int)kTreeletLeaves, kTreeletThreads>),
-         dim3(grid), dim3(kTreeletThreads), 0, s, aabb, num_leaf, bvhnodes, tri2vtx, num_vtx, xyz,
-         eps, treelet_count, treelets, treelet_next, tri_box, err);
+  launch("k_refit_treelet", (k_refit_treelet<LK, (int)kTreeletLeaves, kTreeletThreads>),
+         dim3(grid), dim3(kTreeletThreads), 0, s, v, eps, treelet_count, treelets, treelet_next,
+         skip, err);
   int rc = launch_status();
   if (rc != DMSH_OK || num_leaf <= kTreeletLeaves) return rc;
   if (top_count) {
     launch("k_refit_top", k_refit_top_lds<kTopSlots, kTopThreads>, dim3(1), dim3(kTopThreads), 0,
-           s, aabb, flags, num_leaf, bvhnodes, top_count, top_list, node2slot, treelet_count,
-           treelets, err);
+           s, v.aabb, flags, num_leaf, v.bvhnodes, top_count, top_list, node2slot, treelet_count,
+           treelets, skip, err);
   } else {
-    launch("k_refit_top", k_refit_top<kTopThreads>, dim3(1), dim3(kTopThreads), 0, s, aabb,
-           flags, num_leaf, bvhnodes, treelet_count, treelets, err);
+    launch("k_refit_top", k_refit_top<kTopThreads>, dim3(1), dim3(kTopThreads), 0, s, v.aabb,
+           flags, num_leaf, v.bvhnodes, treelet_count, treelets, err);
   }
   return launch_status();
 }
 
-template <bool TRI>
-inline int launch_refit(hipStream_t s, float* aabb, uint32_t* flags, uint32_t num_leaf,
-                        const uint32_t* bvhnodes, const uint32_t* tri2vtx, uint32_t num_vtx,
-                        const float* xyz, float eps, uint32_t* err) {
-  if (num_leaf == 0) return DMSH_OK;
-  launch("k_refit", k_refit<TRI>, dim3(ceil_div(num_leaf, 256)), dim3(256), 0, s, aabb, flags,
-         num_leaf, bvhnodes, tri2vtx, num_vtx, xyz, eps, err);
+// ---------------------------------------------------------------------------
+// Standalone refit of a tree given only its rows (bvhnode2aabb3.rs:68-142,
+// bvhnode2aabb.cu:6-122): the treelet kernels need each internal node's leaf
+// range [first, last], which the topology kernel knows during a build but a
+// stored tree does not carry.
+//
+// k_leaf_ranges: the first leaf of a node is the first leaf of its left
+//   child, so leaf k is the first leaf of every node on the run of left-child
+//   links directly above it (and the last leaf of the run of right-child
+//   links).  Every internal node lies on exactly one run of each kind, so one
+//   walk per leaf writes each first[] and last[] entry once, O(n) in all.
+// k_list_treelets: per internal node i with children a, b, checks the layout
+//   the treelet kernels rely on -- first[i] = first(a), last[i] = last(b),
+//   last(a) + 1 = first(b) (so a subtree's leaves are exactly [first, last])
+//   and first[i] <= i <= last[i] (so its internal ids lie in that range, as in
+//   every Karras tree) -- and lists the treelets and top nodes as the
+//   topology kernel does.  One failed check sets *bad: the treelet kernels
+//   then skip and the generic climb (gated on *bad) refits the tree instead.
+struct StandaloneLayout {
+  uint32_t* flags;     // num_leaf - 1 arrival counters
+  uint32_t* first;     // num_leaf - 1
+  uint32_t* last;      // num_leaf - 1
+  uint32_t* node2slot; // num_leaf - 1
+  uint32_t* counters;  // treelet count, top count, next treelet, bad
+  uint4* treelets;     // num_leaf
+  uint2* top_list;     // kTopSlots
+};
+
+inline bool carve_standalone(Arena& a, uint32_t n, StandaloneLayout& L) {
+  const size_t nb = n > 1 ? n - 1 : 1;
+  L.flags = a.take<uint32_t>(nb);
+  L.first = a.take<uint32_t>(nb);
+  L.last = a.take<uint32_t>(nb);
+  L.node2slot = a.take<uint32_t>(nb);
+  L.counters = a.take<uint32_t>(4);
+  L.treelets = a.take<uint4>(n ? n : 1);
+  L.top_list = a.take<uint2>(kTopSlots);
+  return a.ok();
+}
+
+inline size_t standalone_workspace_bytes(uint32_t n) {
+  Arena a{nullptr, ~(size_t)0, 0};
+  StandaloneLayout L;
+  carve_standalone(a, n, L);
+  return a.off + 256;
+}
+
+// Also zeroes the arrival counters and the four list counters (the next
+// kernel is the first to use them).  A thread walks kRangeWalks leaves in
+// lock-step, so their dependent row loads overlap.  A walk stops at a link
+// out of range; the entries it leaves unwritten fail k_list_treelets' checks
+// or the treelet kernels' own bounds checks.
+constexpr int kRangeWalks = 4;
+constexpr int kListThreads = 1024;
+constexpr uint32_t kListGrid = 256;
+
+static __global__ __launch_bounds__(256) void k_leaf_ranges(uint32_t num_leaf,
+                                                            const uint32_t* __restrict__ bvhnodes,
+                                                            uint32_t* __restrict__ first,
+                                                            uint32_t* __restrict__ last,
+                                                            uint32_t* __restrict__ flags,
+                                                            uint32_t* __restrict__ counters) {
+  const uint32_t nb = num_leaf - 1;
+  const uint32_t k0 = blockIdx.x * (256u * kRangeWalks) + threadIdx.x;
+  if (k0 == 0) {
+#pragma unroll
+    for (int c = 0; c < 4; ++c) counters[c] = 0u;
+  }
+  uint32_t node[kRangeWalks], p[kRangeWalks];
+  int side[kRangeWalks];
+#pragma unroll
+  for (int r = 0; r < kRangeWalks; ++r) {
+    const uint32_t k = k0 + 256u * r;
+    node[r] = nb + k;
+    p[r] = DMSH_U32MAX;
+    if (k < nb) flags[k] = 0u;
+    if (k < num_leaf) p[r] = bvhnodes[(size_t)node[r] * 3];
+  }
+  // the side of its parent a leaf is on fixes which run it starts
+#pragma unroll
+  for (int r = 0; r < kRangeWalks; ++r) {
+    side[r] = 0;
+    if (p[r] < nb) side[r] = bvhnodes[(size_t)p[r] * 3 + 1] == node[r] ? 1 : 2;
+  }
+  for (uint32_t steps = 0; steps < nb; ++steps) {
+    uint32_t row[kRangeWalks][2];
+#pragma unroll
+    for (int r = 0; r < kRangeWalks; ++r) {
+      if (!side[r]) continue;
+      const uint32_t* prow = bvhnodes + (size_t)p[r] * 3;
+      row[r][0] = prow[0];
+      row[r][1] = prow[side[r]];
+    }
+    bool more = false;
+#pragma unroll
+    for (int r = 0; r < kRangeWalks; ++r) {
+      if (!side[r]) continue;
+      if (row[r][1] != node[r]) {
+        side[r] = 0;
+        continue;
+      }
+      (side[r] == 1 ? first : last)[p[r]] = k0 + 256u * r;
+      node[r] = p[r];
+      p[r] = row[r][0];
+      if (p[r] >= nb) side[r] = 0;
+      more |= side[r] != 0;
+    }
+    if (!more) break;
+  }
+}
+
+// One workgroup of T threads takes T internal nodes per round (grid-stride);
+// its treelets and top nodes get their list slots from one 64-bit atomic on
+// (top count, treelet count) per round instead of one atomic per entry.
+template <int T>
+__global__ __launch_bounds__(T) void k_list_treelets(
+    uint32_t num_leaf, const uint32_t* __restrict__ bvhnodes, const uint32_t* __restrict__ first,
+    const uint32_t* __restrict__ last, uint32_t* __restrict__ counters,
+    uint4* __restrict__ treelets, uint2* __restrict__ top_list, uint32_t* __restrict__ node2slot) {
+  __shared__ uint32_t s_cnt[2];  // treelets, tops of this round
+  __shared__ unsigned long long s_base;
+  const uint32_t nb = num_leaf - 1;
+  const uint32_t K = kTreeletLeaves;
+  for (uint32_t base = blockIdx.x * T; base < nb; base += gridDim.x * T) {
+    const uint32_t i = base + threadIdx.x;
+    uint32_t n_tl = 0, n_top = 0;
+    uint4 tl[2];
+    uint2 top = make_uint2(0u, 0u);
+    if (i < nb) {
+      const uint32_t* row = bvhnodes + (size_t)i * 3;
+      const uint32_t p = row[0], a = row[1], b = row[2];
+      const uint32_t f = first[i], l = last[i];
+      bool ok = a <= 2 * nb && b <= 2 * nb && (i == 0) == (p == DMSH_U32MAX);
+      uint32_t fa = 0, la = 0, fb = 0, lb = 0;
+      if (ok) {
+        fa = a >= nb ? a - nb : first[a];
+        la = a >= nb ? a - nb : last[a];
+        fb = b >= nb ? b - nb : first[b];
+        lb = b >= nb ? b - nb : last[b];
+        ok = f == fa && l == lb && fa <= la && la < nb && la + 1u == fb && fb <= lb && f <= i &&
+             i <= l && l <= nb && (i != 0 || (f == 0 && l == nb));
+      }
+      if (!ok) {
+        atomicOr(counters + 3, 1u);
+      } else if (l - f + 1 > K) {
+        n_top = 1;
+        top = make_uint2(i, (la - fa + 1 > K ? 1u : 0u) | (lb - fb + 1 > K ? 2u : 0u));
+        if (la - fa + 1 <= K) tl[n_tl++] = make_uint4(a, fa, la, 0u);
+        if (lb - fb + 1 <= K) tl[n_tl++] = make_uint4(b, fb, lb, 0u);
+      } else if (i == 0) {
+        tl[n_tl++] = make_uint4(0u, 0u, nb, 0u);
+      }
+    }
+    if (threadIdx.x == 0) s_cnt[0] = s_cnt[1] = 0u;
+    __syncthreads();
+    const uint32_t o_tl = n_tl ? atomicAdd(&s_cnt[0], n_tl) : 0u;
+    const uint32_t o_top = n_top ? atomicAdd(&s_cnt[1], 1u) : 0u;
+    __syncthreads();
+    if (threadIdx.x == 0 && (s_cnt[0] | s_cnt[1]))
+      s_base = atomicAdd(reinterpret_cast<unsigned long long*>(counters),
+                         ((unsigned long long)s_cnt[1] << 32) | s_cnt[0]);
+    __syncthreads();
+    const uint32_t b_tl = (uint32_t)s_base, b_top = (uint32_t)(s_base >> 32);
+    for (uint32_t e = 0; e < n_tl; ++e)
+      if (b_tl + o_tl + e < num_leaf) treelets[b_tl + o_tl + e] = tl[e];
+    if (n_top && b_top + o_top < (uint32_t)kTopSlots) {
+      top_list[b_top + o_top] = top;
+      node2slot[i] = b_top + o_top;
+    }
+    __syncthreads();  // s_cnt and s_base are reset next round
+  }
+}
+
+// ws: standalone_workspace_bytes(v.num_leaf).  Five launches: ranges, list,
+// treelets, top (more than kTreeletLeaves leaves), gated generic climb.
+template <int LK>
+inline int launch_refit_standalone(hipStream_t s, const TreeView& v, float eps, void* ws,
+                                   size_t ws_bytes, uint32_t* err) {
+  const uint32_t n = v.num_leaf;
+  if (n == 0) return DMSH_OK;
+  Arena a{static_cast<char*>(ws), ws_bytes, 0};
+  StandaloneLayout L;
+  if (!carve_standalone(a, n, L)) return DMSH_ERR_WORKSPACE;
+  if (n == 1) {  // one leaf, no internal node: the generic kernel stores its box
+    launch("k_refit", k_refit<LK>, dim3(1), dim3(256), 0, s, v, L.flags, eps,
+           (const uint32_t*)nullptr, err);
+    return launch_status();
+  }
+  launch("k_leaf_ranges", k_leaf_ranges, dim3(ceil_div(n, 256 * kRangeWalks)), dim3(256), 0, s, n,
+         v.bvhnodes, L.first, L.last, L.flags, L.counters);
+  int rc = launch_status();
+  if (rc != DMSH_OK) return rc;
+  launch("k_list_treelets", k_list_treelets<kListThreads>,
+         dim3(std::min<uint32_t>(ceil_div(n - 1, kListThreads), kListGrid)), dim3(kListThreads), 0,
+         s, n, v.bvhnodes, L.first, L.last, L.counters, L.treelets, L.top_list, L.node2slot);
+  if ((rc = launch_status()) != DMSH_OK) return rc;
+  rc = launch_refit_treelets<LK>(s, v, L.flags, L.counters + 0, L.treelets, eps, err,
+                                 L.counters + 1, L.top_list, L.node2slot, L.counters + 2,
+                                 L.counters + 3);
+  if (rc != DMSH_OK) return rc;
+  launch("k_refit", k_refit<LK>, dim3(std::min<uint32_t>(ceil_div(n, 256), 1024)), dim3(256), 0,
+         s, v, L.flags, eps, (const uint32_t*)(L.counters + 3), err);
   return launch_status();
 }
 

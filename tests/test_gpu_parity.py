@@ -157,6 +157,105 @@ def test_refit_standalone_matches_oracle(gpu, oracle):
     assert bits_equal(H(aabb), waabb)
 
 
+def relabel_internal(bn, seed):
+    """The same tree with its internal ids (root 0 kept) randomly permuted:
+    leaf rows stay at n-1.., but internal ids no longer lie in their own leaf
+    ranges, so the standalone refit must take its generic fallback."""
+    nb = bn.shape[0] // 2
+    perm = np.concatenate([[0], 1 + np.random.default_rng(seed).permutation(nb - 1)])
+    idmap = np.arange(bn.shape[0], dtype=np.int64)
+    idmap[:nb] = perm
+
+    def f(x):
+        x = x.astype(np.int64)
+        return np.where(x < idmap.size, idmap[np.minimum(x, idmap.size - 1)], x).astype(np.uint32)
+
+    out = bn.copy()
+    out[perm] = np.stack([f(bn[:nb, c]) for c in range(3)], 1)
+    out[nb:, 0] = f(bn[nb:, 0])
+    return out
+
+
+def swap_leaves(bn, k1, k2):
+    """The tree with leaves k1 and k2 exchanged (rows and parent links):
+    subtree leaf sets are no longer contiguous ranges."""
+    bn = bn.copy()
+    nb = bn.shape[0] // 2
+    r1, r2 = nb + k1, nb + k2
+    p1, p2 = int(bn[r1, 0]), int(bn[r2, 0])
+    assert p1 != p2
+    bn[p1, 1:] = np.where(bn[p1, 1:] == r1, r2, bn[p1, 1:])
+    bn[p2, 1:] = np.where(bn[p2, 1:] == r2, r1, bn[p2, 1:])
+    bn[[r1, r2]] = bn[[r2, r1]]
+    return bn
+
+
+def refit_tri_abi(gpu, tri, vtx, bn, eps):
+    L = _lib.lib()
+    t_tri, t_vtx, t_bn = G(tri, gpu), G(vtx, gpu), G(bn, gpu)
+    aabb = torch.empty((bn.shape[0], 6), dtype=torch.float32, device=gpu)
+    ws = _lib.workspace(L.dmsh_refit_workspace_bytes(tri.shape[0]), gpu)
+    _lib.check(L.dmsh_bvhnode2aabb_trimesh3(
+        _lib.stream_of(gpu), _lib.ptr(aabb), bn.shape[0], _lib.ptr(t_bn), tri.shape[0],
+        _lib.ptr(t_tri), vtx.shape[0], _lib.ptr(t_vtx), eps, _lib.ptr(ws), ws.numel()),
+        "dmsh_bvhnode2aabb_trimesh3")
+    return H(aabb)
+
+
+def refit_points_abi(gpu, pts, bn):
+    L = _lib.lib()
+    t_pts, t_bn = G(pts, gpu), G(bn, gpu)
+    aabb = torch.empty((bn.shape[0], 6), dtype=torch.float32, device=gpu)
+    ws = _lib.workspace(L.dmsh_refit_workspace_bytes(pts.shape[0]), gpu)
+    _lib.check(L.dmsh_bvhnode2aabb_points(
+        _lib.stream_of(gpu), _lib.ptr(aabb), bn.shape[0], _lib.ptr(t_bn), pts.shape[0],
+        _lib.ptr(t_pts), _lib.ptr(ws), ws.numel()), "dmsh_bvhnode2aabb_points")
+    return H(aabb)
+
+
+@pytest.mark.parametrize("eps", [0.0, 0.01])
+def test_refit_standalone_large(gpu, oracle, eps):
+    """131,072 triangles (128 treelets and the top) through the treelet path
+    of the standalone refit, margin eps included."""
+    tri, vtx = primitives.octahedral_sphere(7, noise_sigma=0.01, seed=5)
+    wbn, _ = oracle.build_trimesh3(tri, vtx, morton_bits=63)
+    got = refit_tri_abi(gpu, tri, vtx, wbn, eps)
+    assert bits_equal(got, oracle.refit_trimesh3(wbn, tri, vtx, eps))
+    no_device_errors()
+
+
+@pytest.mark.parametrize("layout", ["relabelled", "swapped"])
+@pytest.mark.parametrize("mesh", ["torus", "octa6_noise"])
+def test_refit_standalone_non_lbvh(gpu, oracle, layout, mesh):
+    """Trees whose rows fail the LBVH-layout check take the generic climb
+    and still match the oracle's recursion bit for bit, with no device error."""
+    tri, vtx = MESHES[mesh]()
+    wbn, _ = oracle.build_trimesh3(tri, vtx)
+    n = tri.shape[0]
+    bn = relabel_internal(wbn, 3) if layout == "relabelled" else swap_leaves(wbn, 3, n - 5)
+    got = refit_tri_abi(gpu, tri, vtx, bn, 0.0)
+    assert bits_equal(got, oracle.refit_trimesh3(bn, tri, vtx, 0.0))
+    no_device_errors()
+
+
+@pytest.mark.parametrize("layout", ["lbvh", "relabelled"])
+def test_refit_standalone_points(gpu, oracle, layout):
+    pts = np.random.default_rng(33).random((70_000, 3), dtype=np.float32)
+    wbn, _ = oracle.build_points(pts, morton_bits=30)
+    bn = wbn if layout == "lbvh" else relabel_internal(wbn, 4)
+    assert bits_equal(refit_points_abi(gpu, pts, bn), oracle.refit_points(bn, pts))
+    no_device_errors()
+
+
+def test_refit_standalone_tiny(gpu, oracle):
+    """One, two and three leaves (no treelet above the root)."""
+    for n in (1, 2, 3):
+        pts = np.random.default_rng(n).random((n, 3), dtype=np.float32)
+        wbn, _ = oracle.build_points(pts, morton_bits=30)
+        assert bits_equal(refit_points_abi(gpu, pts, wbn), oracle.refit_points(wbn, pts))
+    no_device_errors()
+
+
 @pytest.mark.parametrize("bits", [30, 63])
 def test_build_points(gpu, oracle, bits):
     rng = np.random.default_rng(31)

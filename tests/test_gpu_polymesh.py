@@ -103,3 +103,20 @@ def test_to_polyhedral_mesh(gpu):
     want_iv = np.concatenate([a.reshape(-1) for a in arrs])
     want_off = np.concatenate([[0], np.cumsum([4] * 3 + [6] * 2 + [8] * 4)]).astype(np.uint32)
     assert np.array_equal(H(off), want_off) and np.array_equal(H(iv), want_iv)
+
+
+@pytest.mark.parametrize("layout", ["lbvh", "relabelled"])
+def test_refit_stored_tree(gpu, oracle, layout):
+    """The element-box refit of a stored tree: the LBVH built from the
+    element centres (treelet path), and the same tree with its internal ids
+    permuted (generic fallback); both bit-exact against the oracle."""
+    from test_gpu_parity import relabel_internal
+    import del_msh_dlpack.PolyhedronMesh as PM
+
+    off, iv, x = primitives.mixed_hex_grid(20, 16, 12, seed=5, jitter=0.2)
+    w_bn, _ = oracle.build_points(oracle.polyhedron_elem2center(off, iv, x))
+    bn = w_bn if layout == "lbvh" else relabel_internal(w_bn, 8)
+    aabb = torch.empty((bn.shape[0], 6), dtype=torch.float32, device=gpu)
+    PM.bvhnode2aabb_from_bvhnodes(G(off, gpu), G(iv, gpu), G(x, gpu), G(bn, gpu), aabb)
+    assert bits_equal(H(aabb), oracle.polyhedron_refit(bn, off, iv, x))
+    assert _lib.device_errors() == 0
