@@ -189,11 +189,10 @@ inline bool carve(Arena& a, uint32_t n, bool tri, BuildLayout& L) {
   const uint32_t kb = key_bits_of<KeyT>();
   L.keys6 = a.take<uint32_t>(64);
   // hist, tickets, status and the refit counters are one contiguous zero run
-  const size_t P = sortk::num_passes(kb);
-  L.sort.hist = a.take<uint32_t>(sortk::kMaxPasses * sortk::kRadix + sortk::kMaxPasses);
+  L.sort.hist = a.take<uint32_t>(sortk::kMaxPasses * sortk::kRadix + sortk::kTicketWords);
   L.sort.tickets = L.sort.hist + sortk::kMaxPasses * sortk::kRadix;
-  const size_t status_words = P * (size_t)sortk::num_tiles<KeyT>(n) * sortk::kRadix;
-  L.sort.status = L.sort.tickets + sortk::kMaxPasses;
+  const size_t status_words = sortk::status_words<KeyT>(n, kb);
+  L.sort.status = L.sort.tickets + sortk::kTicketWords;
   a.off += sizeof(uint32_t) * status_words;  // contiguous after the tickets
   L.flags = L.sort.status + status_words;
   a.off += sizeof(uint32_t) * (n > 1 ? n - 1 : 1);

@@ -42,13 +42,12 @@ struct BatchBuildLayout {
 
 inline bool carve_batch_build(Arena& a, uint32_t M, uint32_t n, BatchBuildLayout& L) {
   const uint32_t kb = batch_key_bits(M);
-  const size_t P = sortk::num_passes(kb);
   const size_t nn = n ? n : 1;
   L.keys6 = a.take<uint32_t>((size_t)M * 6);
-  L.sort.hist = a.take<uint32_t>(sortk::kMaxPasses * sortk::kRadix + sortk::kMaxPasses);
+  L.sort.hist = a.take<uint32_t>(sortk::kMaxPasses * sortk::kRadix + sortk::kTicketWords);
   L.sort.tickets = L.sort.hist + sortk::kMaxPasses * sortk::kRadix;
-  L.sort.status = L.sort.tickets + sortk::kMaxPasses;
-  const size_t status_words = P * (size_t)sortk::num_tiles<uint64_t>(n) * sortk::kRadix;
+  L.sort.status = L.sort.tickets + sortk::kTicketWords;
+  const size_t status_words = sortk::status_words<uint64_t>(n, kb);
   a.off += sizeof(uint32_t) * status_words;
   L.flags = L.sort.status + status_words;
   a.off += sizeof(uint32_t) * nn;
@@ -227,9 +226,12 @@ int dmsh_batch_build_trimesh3(void* stream, uint32_t num_mesh, uint32_t total_tr
   launch("k_batch_centroid", bat::k_batch_centroid,
          dim3(ceil_div(n, 4u * 64u * bat::kCentroidChunks)), dim3(256), 0, s, L.bad, n, M, mesh2tri, mesh2vtx, tri2vtx, vtx2xyz, L.cntr, L.keys6, err);
   if ((rc = launch_status()) != DMSH_OK) return rc;
-  // With an even number of passes the sort returns to the buffer it starts
-  // from; with an odd number it starts from the alternate key buffer.
-  uint64_t* morton_out = (P % 2 == 0) ? L.codes : static_cast<uint64_t*>(L.sort.alt_keys);
+  // With an even number of passes (or the top-digit passes and the finish)
+  // the sort returns to the buffer it starts from; with an odd number of
+  // plain passes it starts from the alternate key buffer.
+  uint64_t* morton_out = (P % 2 == 0 || sortk::uses_finish(P))
+                             ? L.codes
+                             : static_cast<uint64_t*>(L.sort.alt_keys);
   launch("k_morton_hist", bat::k_batch_morton_hist,
          dim3(std::min<uint32_t>(sortk::num_tiles<uint64_t>(n), 2048)), dim3(256), 0, s, L.bad, n,
          M, mesh2tri, L.cntr, L.keys6, morton_out, P, kb, L.sort.hist);

@@ -756,7 +756,7 @@ inline size_t standalone_workspace_bytes(uint32_t n) {
 // lock-step, so their dependent row loads overlap.  A walk stops at a link
 // out of range; the entries it leaves unwritten fail k_list_treelets' checks
 // or the treelet kernels' own bounds checks.
-constexpr int kRangeWalks = 4;
+constexpr int kRangeWalks = 1;
 constexpr int kListThreads = 1024;
 constexpr uint32_t kListGrid = 256;
 

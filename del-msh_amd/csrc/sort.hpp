@@ -14,6 +14,16 @@
 // in element order, waves and tiles are prefixed in order.  The look-back
 // spin is bounded; a block that gives up raises DMSH_DEVERR_SORT_PROGRESS and
 // still exits, so a launch always drains.
+//
+// Keys of five or more digits (63-bit Morton codes, batch keys) take
+// kTopPasses passes over their top digits only, which leaves them grouped by
+// top bits with each group in element order; k_sort_finish then orders every
+// group in LDS by the whole key, ties by position (a group is a handful of
+// elements for Morton codes of spread-out geometry).  The result is the same
+// stable order as the full LSD sort.  A group too long for the finish's LDS
+// window (heavily repeated top bits) sets a gate word instead, and the full
+// set of digit passes, launched behind it and idle otherwise, sorts the
+// top-sorted sequence again from the lowest digit.
 #pragma once
 
 #include "dmsh_common.hpp"
@@ -30,6 +40,22 @@ constexpr uint32_t kFlagPrefix = 2u << 30;
 constexpr uint32_t kValueMask = (1u << 30) - 1u;
 constexpr uint32_t kSpinLimit = 1u << 22;
 constexpr int kLookBack = 8;
+constexpr uint32_t kTopPasses = 3;      // global passes before the finish
+constexpr uint32_t kFinishHalo = 1024;  // LDS window margin of the finish
+// One ticket per launched pass (kTopPasses + kMaxPasses at most); the last
+// word is the finish's gate.
+constexpr uint32_t kTicketWords = 2 * kMaxPasses;
+
+// Keys of five or more digits take the top passes and the finish (with four,
+// three passes, the finish and the idle gated passes cost about as much as
+// four plain passes).
+inline bool uses_finish(uint32_t num_passes) { return num_passes > kTopPasses + 1; }
+
+// Passes that take a status region: every digit once, plus the top passes
+// when the finish is used.
+inline uint32_t status_passes(uint32_t num_passes) {
+  return uses_finish(num_passes) ? kTopPasses + num_passes : num_passes;
+}
 
 template <typename KeyT>
 struct Tune {
@@ -58,7 +84,7 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* s
   return base + inc - v;
 }
 
-__device__ __forceinline__ uint32_t pass_bits(uint32_t key_bits, uint32_t p) {
+__host__ __device__ __forceinline__ uint32_t pass_bits(uint32_t key_bits, uint32_t p) {
   const uint32_t lo = p * kRadixBits;
   const uint32_t rem = key_bits > lo ? key_bits - lo : 0u;
   return rem < (uint32_t)kRadixBits ? rem : (uint32_t)kRadixBits;
@@ -114,13 +140,16 @@ __global__ __launch_bounds__(256) void k_hist_all(const KeyT* __restrict__ keys,
   hist_flush(s_h, num_passes, hist);
 }
 
-// One digit pass.  IOTA: values are the element indices (first pass).
+// One digit pass.  IOTA: values are the element indices (first pass).  With
+// `gate`, the pass runs only if *gate is nonzero.
 template <typename KeyT, int ITEMS, bool IOTA>
 __global__ __launch_bounds__(256) void k_onesweep(
     const KeyT* __restrict__ kin, const uint32_t* __restrict__ vin, KeyT* __restrict__ kout,
     uint32_t* __restrict__ vout, uint32_t n, uint32_t shift, uint32_t dbits,
     const uint32_t* __restrict__ hist, uint32_t* __restrict__ status,
-    uint32_t* __restrict__ ticket, uint32_t* __restrict__ err) {
+    uint32_t* __restrict__ ticket, const uint32_t* __restrict__ gate,
+    uint32_t* __restrict__ err) {
+  if (gate && *gate == 0u) return;
   constexpr int kTile = kThreads * ITEMS;
   __shared__ uint32_t s_wcnt[4][kRadix];
   __shared__ uint32_t s_excl[kRadix];
@@ -246,6 +275,125 @@ __global__ __launch_bounds__(256) void k_onesweep(
   }
 }
 
+// The finish.  Input: keys (and element indices) sorted stably by their bits
+// at and above `shift` (masked to the sort's key bits).  Each block loads
+// kFinishHalo keys either side of its 4096 elements into LDS and marks the
+// group starts (equal bits above `shift` form a group) in an LDS bit set;
+// each element finds the ends of its group from that set, at most
+// kFinishHalo - 1 slots down and kFinishHalo up, and lands at group start +
+// the number of group members whose masked key is smaller, or equal and
+// earlier.  An element whose group end lies beyond that span sets *gate (the
+// full passes then redo the sort) and writes nothing.
+constexpr int kFinishThreads = 1024;
+constexpr int kFinishItems = 4;
+
+template <typename KeyT>
+__global__ __launch_bounds__(kFinishThreads) void k_sort_finish(const KeyT* __restrict__ kin,
+                                                     const uint32_t* __restrict__ vin,
+                                                     KeyT* __restrict__ kout,
+                                                     uint32_t* __restrict__ vout, uint32_t n,
+                                                     uint32_t shift, KeyT kmask,
+                                                     uint32_t* __restrict__ gate) {
+  constexpr uint32_t kTW = (uint32_t)kFinishThreads * kFinishItems;
+  constexpr uint32_t kG = kFinishHalo;
+  constexpr uint32_t kWin = kTW + 2 * kG;
+  constexpr int kLoads = (int)(kWin / kFinishThreads);
+  static_assert(kWin % kFinishThreads == 0, "the window is whole rounds of the block");
+  __shared__ KeyT s_k[kWin];  // keys as stored (the mask applies when compared)
+  const int64_t w0 = (int64_t)blockIdx.x * kTW - kG;  // element index of slot 0
+  const uint32_t lo = w0 < 0 ? (uint32_t)(-w0) : 0u;  // first slot holding an element
+  const int64_t rem = (int64_t)n - w0;
+  const uint32_t hi = rem < (int64_t)kWin ? (uint32_t)rem : kWin;  // one past the last
+  // every load of the window and of this thread's values is issued up front
+  KeyT kr[kLoads];
+#pragma unroll
+  for (int q = 0; q < kLoads; ++q) {
+    const uint32_t j = (uint32_t)q * kFinishThreads + threadIdx.x;
+    kr[q] = (j >= lo && j < hi) ? kin[w0 + j] : (KeyT)0;
+  }
+  uint32_t vr[kFinishItems];
+#pragma unroll
+  for (int r = 0; r < kFinishItems; ++r) {
+    const uint32_t li = kG + (uint32_t)r * kFinishThreads + threadIdx.x;
+    vr[r] = li < hi ? vin[w0 + li] : 0u;
+  }
+#pragma unroll
+  for (int q = 0; q < kLoads; ++q) s_k[(uint32_t)q * kFinishThreads + threadIdx.x] = kr[q];
+  // group starts as a bit set over the window: slot j starts a group if it is
+  // the first element of the sort or its top bits differ from slot j - 1's;
+  // slot hi is marked when the sort ends there (kWin / 32 + 1 words)
+  __shared__ uint32_t s_start[kWin / 32 + 1];
+  const uint32_t lane = threadIdx.x & 63u;
+  if (threadIdx.x == 0) s_start[kWin / 32] = (w0 + (int64_t)kWin == (int64_t)n) ? 1u : 0u;
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < kLoads; ++q) {
+    const uint32_t j = (uint32_t)q * kFinishThreads + threadIdx.x;
+    bool st;
+    if (j < lo || j > hi) st = false;
+    else if (j == lo) st = w0 + (int64_t)j == 0;
+    else if (j == hi) st = true;  // hi < kWin here: the sort ends at slot hi
+    else st = ((s_k[j] & kmask) >> shift) != ((s_k[j - 1] & kmask) >> shift);
+    const uint64_t m = __ballot(st);
+    if (lane == 0) {
+      s_start[j >> 5] = (uint32_t)m;
+      s_start[(j >> 5) + 1] = (uint32_t)(m >> 32);
+    }
+  }
+  __syncthreads();
+  bool big = false;
+#pragma unroll
+  for (int r = 0; r < kFinishItems; ++r) {
+    const uint32_t li = kG + (uint32_t)r * kFinishThreads + threadIdx.x;
+    if (li < hi) {
+      const KeyT full = s_k[li];
+      const KeyT key = full & kmask;
+      // a: the last group start <= li, looked for down to li - (kG - 1)
+      uint32_t w = li >> 5;
+      uint32_t bits = s_start[w] & (0xFFFFFFFFu >> (31u - (li & 31u)));
+      const uint32_t wlo = (li - (kG - 1)) >> 5;
+      while (bits == 0u && w > wlo) bits = s_start[--w];
+      const bool found_a = bits != 0u;
+      const uint32_t a = found_a ? w * 32u + 31u - (uint32_t)__clz(bits) : 0u;
+      // b: the first group start > li (or the end of the sort), up to li + kG
+      w = (li + 1) >> 5;
+      bits = s_start[w] & (0xFFFFFFFFu << ((li + 1) & 31u));
+      const uint32_t whi = (li + kG) >> 5;
+      while (bits == 0u && w < whi) bits = s_start[++w];
+      const bool found_b = bits != 0u;
+      const uint32_t b = found_b ? w * 32u + (uint32_t)__ffs(bits) - 1u : 0u;
+      if (!found_a || !found_b || a + (kG - 1) < li || b > li + kG) {
+        big = true;  // a group end lies outside the searched span
+      } else {
+        uint32_t rank = 0;
+#pragma unroll 4
+        for (uint32_t m = a; m < b; ++m) {
+          const KeyT km = s_k[m] & kmask;
+          rank += (km < key || (km == key && m < li)) ? 1u : 0u;
+        }
+        const size_t dst = (size_t)(w0 + a) + rank;
+        kout[dst] = full;
+        vout[dst] = vr[r];
+      }
+    }
+  }
+  if (big) atomicOr(gate, 1u);
+}
+
+// Copies (keys, values) a -> b when *gate is nonzero.
+template <typename KeyT>
+__global__ __launch_bounds__(256) void k_copy_pairs(const uint32_t* __restrict__ gate,
+                                                    const KeyT* __restrict__ ka,
+                                                    const uint32_t* __restrict__ va,
+                                                    KeyT* __restrict__ kb,
+                                                    uint32_t* __restrict__ vb, uint32_t n) {
+  if (*gate == 0u) return;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    kb[i] = ka[i];
+    vb[i] = va[i];
+  }
+}
+
 template <typename KeyT>
 inline uint32_t num_tiles(uint32_t n) {
   return ceil_div(n, (uint64_t)kThreads * Tune<KeyT>::kItems);
@@ -253,22 +401,28 @@ inline uint32_t num_tiles(uint32_t n) {
 
 inline uint32_t num_passes(uint32_t key_bits) { return ceil_div(key_bits, kRadixBits); }
 
-// Workspace: hist[8][256] + tickets[8] + status[P][tiles][256] + alt keys + alt values.
+// Status words of every pass that may run (see status_passes).
+template <typename KeyT>
+inline size_t status_words(uint32_t n, uint32_t key_bits) {
+  return (size_t)status_passes(num_passes(key_bits)) * num_tiles<KeyT>(n) * kRadix;
+}
+
+// Workspace: hist[8][256] + tickets/gate[16] + status[passes][tiles][256] +
+// alt keys + alt values.
 template <typename KeyT>
 inline size_t workspace_bytes(uint32_t n, uint32_t key_bits) {
-  const size_t P = num_passes(key_bits);
   Arena a{nullptr, 0, 0};
-  a.take<uint32_t>(kMaxPasses * kRadix + kMaxPasses);
-  a.take<uint32_t>(P * (size_t)num_tiles<KeyT>(n) * kRadix);
+  a.take<uint32_t>(kMaxPasses * kRadix + kTicketWords);
+  a.take<uint32_t>(status_words<KeyT>(n, key_bits));
   a.take<KeyT>(n);
   a.take<uint32_t>(n);
   return a.off + 256;
 }
 
 struct Layout {
-  uint32_t* hist;     // [kMaxPasses][kRadix], then tickets[kMaxPasses]
-  uint32_t* tickets;
-  uint32_t* status;   // [P][tiles][kRadix]
+  uint32_t* hist;     // [kMaxPasses][kRadix], then tickets[kTicketWords]
+  uint32_t* tickets;  // the last word is the finish's gate
+  uint32_t* status;   // [status_passes][tiles][kRadix]
   size_t zero_words;  // hist + tickets + status are contiguous from `hist`
   void* alt_keys;
   uint32_t* alt_vals;
@@ -276,12 +430,11 @@ struct Layout {
 
 template <typename KeyT>
 inline bool carve(Arena& a, uint32_t n, uint32_t key_bits, Layout& L) {
-  const size_t P = num_passes(key_bits);
-  L.hist = a.take<uint32_t>(kMaxPasses * kRadix + kMaxPasses);
+  L.hist = a.take<uint32_t>(kMaxPasses * kRadix + kTicketWords);
   L.tickets = L.hist + kMaxPasses * kRadix;
-  const size_t status_words = P * (size_t)num_tiles<KeyT>(n) * kRadix;
-  L.status = a.take<uint32_t>(status_words);
-  L.zero_words = (size_t)(L.status - L.hist) + status_words;
+  const size_t sw = status_words<KeyT>(n, key_bits);
+  L.status = a.take<uint32_t>(sw);
+  L.zero_words = (size_t)(L.status - L.hist) + sw;
   L.alt_keys = a.take<KeyT>(n);
   L.alt_vals = a.take<uint32_t>(n);
   return a.ok();
@@ -294,45 +447,92 @@ inline int launch_fill(hipStream_t s, uint32_t* p, size_t count, uint32_t value)
   return launch_status();
 }
 
-// Runs the digit passes over keys already in `src_keys`.  The histogram of
-// every pass must already be in L.hist (zeroed + accumulated) and L.status /
-// L.tickets must be zero.  The sorted keys and permutation end in
-// (dst_keys, dst_vals); `src_keys` may alias dst_keys only when the number of
-// passes is even (the caller arranges this, see argsort()).
+// One digit pass over (sk, sv) into (dk, dv); `slot` picks its status region
+// and ticket.  iota: the values are the element indices.
+template <typename KeyT>
+inline int launch_pass(hipStream_t s, uint32_t n, uint32_t key_bits, const Layout& L,
+                       uint32_t digit, uint32_t slot, bool iota, const KeyT* sk,
+                       const uint32_t* sv, KeyT* dk, uint32_t* dv, const uint32_t* gate,
+                       uint32_t* err) {
+  constexpr int ITEMS = Tune<KeyT>::kItems;
+  const uint32_t tiles = num_tiles<KeyT>(n);
+  const uint32_t shift = digit * kRadixBits;
+  const uint32_t dbits = pass_bits(key_bits, digit);
+  uint32_t* st = L.status + (size_t)slot * tiles * kRadix;
+  const char* label = gate ? "k_onesweep_gated" : "k_onesweep";
+  if (iota) {
+    launch(label, (k_onesweep<KeyT, ITEMS, true>), dim3(tiles), dim3(kThreads), 0, s, sk, sv, dk,
+           dv, n, shift, dbits, L.hist + digit * kRadix, st, L.tickets + slot, gate, err);
+  } else {
+    launch(label, (k_onesweep<KeyT, ITEMS, false>), dim3(tiles), dim3(kThreads), 0, s, sk, sv, dk,
+           dv, n, shift, dbits, L.hist + digit * kRadix, st, L.tickets + slot, gate, err);
+  }
+  return launch_status();
+}
+
+// Sorts the keys in `first_src` (values: the element indices).  The histogram
+// of every digit must already be in L.hist and L.tickets / L.status must be
+// zero.  The sorted keys and permutation end in (dst_keys, dst_vals).
+//   Up to four digits: one pass per digit, alternating so that the last lands
+//   in dst; `first_src` may alias dst_keys only when the number of passes is
+//   even.
+//   Five or more (uses_finish): top passes first_src -> alt -> dst -> alt, the finish
+//   alt -> dst, then the gated full passes alt -> dst -> ... (and a gated copy
+//   back to dst when their number is even).  `first_src` must not be the
+//   alternate key buffer.
 template <typename KeyT>
 inline int run_passes(hipStream_t s, uint32_t n, uint32_t key_bits, const Layout& L,
                       const KeyT* first_src, KeyT* dst_keys, uint32_t* dst_vals, uint32_t* err) {
-  constexpr int ITEMS = Tune<KeyT>::kItems;
   const uint32_t P = num_passes(key_bits);
-  const uint32_t tiles = num_tiles<KeyT>(n);
   KeyT* alt_k = static_cast<KeyT*>(L.alt_keys);
   uint32_t* alt_v = L.alt_vals;
-  // Destination of pass p: the user buffers for the last pass, alternating back.
-  const KeyT* src_k = first_src;
-  const uint32_t* src_v = nullptr;
-  for (uint32_t p = 0; p < P; ++p) {
-    const bool to_user = ((P - 1 - p) % 2) == 0;
-    KeyT* dk = to_user ? dst_keys : alt_k;
-    uint32_t* dv = to_user ? dst_vals : alt_v;
-    const uint32_t shift = p * kRadixBits;
-    uint32_t dbits = key_bits - shift;
-    if (dbits > (uint32_t)kRadixBits) dbits = kRadixBits;
-    uint32_t* st = L.status + (size_t)p * tiles * kRadix;
-    if (p == 0) {
-      launch("k_onesweep", (k_onesweep<KeyT, ITEMS, true>), dim3(tiles), dim3(kThreads), 0, s, src_k,
-                         src_v, dk, dv, n, shift, dbits, L.hist + p * kRadix, st, L.tickets + p,
-                         err);
-    } else {
-      launch("k_onesweep", (k_onesweep<KeyT, ITEMS, false>), dim3(tiles), dim3(kThreads), 0, s,
-                         src_k, src_v, dk, dv, n, shift, dbits, L.hist + p * kRadix, st,
-                         L.tickets + p, err);
+  int rc;
+  if (!uses_finish(P)) {
+    const KeyT* src_k = first_src;
+    const uint32_t* src_v = nullptr;
+    for (uint32_t p = 0; p < P; ++p) {
+      const bool to_user = ((P - 1 - p) % 2) == 0;
+      KeyT* dk = to_user ? dst_keys : alt_k;
+      uint32_t* dv = to_user ? dst_vals : alt_v;
+      rc = launch_pass<KeyT>(s, n, key_bits, L, p, p, p == 0, src_k, src_v, dk, dv, nullptr, err);
+      if (rc != DMSH_OK) return rc;
+      src_k = dk;
+      src_v = dv;
     }
-    int rc = launch_status();
-    if (rc != DMSH_OK) return rc;
-    src_k = dk;
-    src_v = dv;
+    return DMSH_OK;
   }
-  return DMSH_OK;
+  if (first_src == alt_k) return DMSH_ERR_INVALID;
+  const uint32_t d0 = P - kTopPasses;
+  if ((rc = launch_pass<KeyT>(s, n, key_bits, L, d0, 0, true, first_src, nullptr, alt_k, alt_v,
+                              nullptr, err)) != DMSH_OK)
+    return rc;
+  if ((rc = launch_pass<KeyT>(s, n, key_bits, L, d0 + 1, 1, false, alt_k, alt_v, dst_keys,
+                              dst_vals, nullptr, err)) != DMSH_OK)
+    return rc;
+  if ((rc = launch_pass<KeyT>(s, n, key_bits, L, d0 + 2, 2, false, dst_keys, dst_vals, alt_k,
+                              alt_v, nullptr, err)) != DMSH_OK)
+    return rc;
+  uint32_t* gate = L.tickets + kTicketWords - 1;
+  const KeyT kmask = key_bits >= 8 * sizeof(KeyT) ? ~(KeyT)0 : (((KeyT)1 << key_bits) - 1);
+  launch("k_sort_finish", k_sort_finish<KeyT>,
+         dim3(ceil_div(n, (uint64_t)kFinishThreads * kFinishItems)), dim3(kFinishThreads), 0, s,
+         alt_k, alt_v,
+         dst_keys, dst_vals, n, d0 * kRadixBits, kmask, gate);
+  if ((rc = launch_status()) != DMSH_OK) return rc;
+  for (uint32_t p = 0; p < P; ++p) {
+    const bool from_alt = p % 2 == 0;
+    rc = launch_pass<KeyT>(s, n, key_bits, L, p, kTopPasses + p, false,
+                           from_alt ? alt_k : dst_keys, from_alt ? alt_v : dst_vals,
+                           from_alt ? dst_keys : alt_k, from_alt ? dst_vals : alt_v, gate, err);
+    if (rc != DMSH_OK) return rc;
+  }
+  if (P % 2 == 0) {
+    launch("k_copy_pairs", k_copy_pairs<KeyT>, dim3(std::min<uint32_t>(ceil_div(n, 256), 2048)),
+           dim3(256), 0, s, (const uint32_t*)gate, (const KeyT*)alt_k, (const uint32_t*)alt_v,
+           dst_keys, dst_vals, n);
+    rc = launch_status();
+  }
+  return rc;
 }
 
 // Full argsort: keys_inout (sorted on return), idx_out (permutation).
@@ -357,7 +557,7 @@ inline int argsort(hipStream_t s, uint32_t n, KeyT* keys_inout, uint32_t* idx_ou
                      key_bits, L.hist);
   if ((rc = launch_status()) != DMSH_OK) return rc;
   const KeyT* first_src = keys_inout;
-  if (P % 2 == 1) {
+  if (P % 2 == 1 && !uses_finish(P)) {
     // An odd pass count would end in the scratch buffer: start from a copy.
     KeyT* alt_k = static_cast<KeyT*>(L.alt_keys);
     launch("k_copy_keys", k_copy_keys<KeyT>, dim3(std::min<uint32_t>(ceil_div(n, 256), 4096)),

@@ -91,6 +91,30 @@ def test_argsort_u32(gpu, oracle, n, spread):
     no_device_errors()
 
 
+@pytest.mark.parametrize("key_bits", [40, 48, 56, 63, 64])
+@pytest.mark.parametrize("groups", [None, 4000, 300])
+def test_argsort_u64_grouped(gpu, oracle, key_bits, groups):
+    """Keys of five or more digits: top-digit passes, then the in-LDS finish
+    per group of equal top bits.  groups=None: random keys (tiny groups);
+    4000: ~250 keys per group with repeated low parts; 300: ~3300 per group,
+    beyond the finish's window, so the gated full passes run."""
+    n = 1_000_003
+    rng = np.random.default_rng(key_bits * 7 + (groups or 1))
+    mask = np.uint64((1 << key_bits) - 1) if key_bits < 64 else np.uint64(0xFFFFFFFFFFFFFFFF)
+    keys = rng.integers(0, 1 << 63, n, dtype=np.uint64) * np.uint64(2) + rng.integers(0, 2, n, dtype=np.uint64)
+    if groups:
+        shift = np.uint64(8 * (-(-key_bits // 8) - 3))
+        top = rng.integers(0, groups, n, dtype=np.uint64) << shift
+        low = rng.integers(0, 50, n, dtype=np.uint64)  # repeats inside a group
+        keys = top | low
+    keys &= mask
+    idx, sk = Array1D.argsort(G(keys, gpu), key_bits=key_bits)
+    widx, wsk = oracle.argsort(keys)
+    assert np.array_equal(H(sk), wsk)
+    assert np.array_equal(H(idx), widx)
+    no_device_errors()
+
+
 @pytest.mark.parametrize("n", [3, 4096, 300_001])
 @pytest.mark.parametrize("key_bits", [63, 64, 17])
 def test_argsort_u64(gpu, oracle, n, key_bits):
