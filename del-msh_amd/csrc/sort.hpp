@@ -2,7 +2,7 @@
 //
 // Design (one pass per 8-bit digit, Onesweep-style):
 //   * k_hist_all reads the keys once and builds the digit histogram of every
-//     pass (LDS counters, then one global atomic per bin and block).
+//     pass it runs (LDS counters, then one global atomic per bin and block).
 //   * k_onesweep, one launch per digit: a block takes the next tile from an
 //     atomic ticket (so tiles are started in order), ranks its elements with
 //     wave64 ballots (one ballot per digit bit gives each lane the mask of
@@ -49,7 +49,16 @@ constexpr uint32_t kTicketWords = 2 * kMaxPasses;
 // Keys of five or more digits take the top passes and the finish (with four,
 // three passes, the finish and the idle gated passes cost about as much as
 // four plain passes).
-inline bool uses_finish(uint32_t num_passes) { return num_passes > kTopPasses + 1; }
+__host__ __device__ inline bool uses_finish(uint32_t num_passes) {
+  return num_passes > kTopPasses + 1;
+}
+
+// The first digit whose histogram the key producers accumulate: with the
+// finish, only the top digits' (the gated full passes build the others'
+// histograms themselves, k_hist_low).
+__host__ __device__ inline uint32_t first_hist_pass(uint32_t num_passes) {
+  return uses_finish(num_passes) ? num_passes - kTopPasses : 0u;
+}
 
 // Passes that take a status region: every digit once, plus the top passes
 // when the finish is used.
@@ -109,11 +118,12 @@ __global__ __launch_bounds__(256) void k_copy_keys(const KeyT* __restrict__ a, K
     b[i] = a[i];
 }
 
-// Accumulates the histograms of all passes into the LDS bins of this block.
+// Accumulates the histograms of passes [first_hist_pass, num_passes) into the
+// LDS bins of this block.
 template <typename KeyT>
 __device__ __forceinline__ void hist_accumulate(uint32_t (*s_h)[kRadix], KeyT k,
                                                 uint32_t num_passes, uint32_t key_bits) {
-  for (uint32_t p = 0; p < num_passes; ++p) {
+  for (uint32_t p = first_hist_pass(num_passes); p < num_passes; ++p) {
     const uint32_t mask = (1u << pass_bits(key_bits, p)) - 1u;
     atomicAdd(&s_h[p][digit_of(k, p * kRadixBits, mask)], 1u);
   }
@@ -121,7 +131,8 @@ __device__ __forceinline__ void hist_accumulate(uint32_t (*s_h)[kRadix], KeyT k,
 
 __device__ __forceinline__ void hist_flush(uint32_t (*s_h)[kRadix], uint32_t num_passes,
                                            uint32_t* __restrict__ hist) {
-  for (uint32_t e = threadIdx.x; e < num_passes * kRadix; e += blockDim.x) {
+  for (uint32_t e = first_hist_pass(num_passes) * kRadix + threadIdx.x; e < num_passes * kRadix;
+       e += blockDim.x) {
     const uint32_t v = (&s_h[0][0])[e];
     if (v) atomicAdd(hist + e, v);
   }
@@ -138,6 +149,29 @@ __global__ __launch_bounds__(256) void k_hist_all(const KeyT* __restrict__ keys,
     hist_accumulate(s_h, keys[i], num_passes, key_bits);
   __syncthreads();
   hist_flush(s_h, num_passes, hist);
+}
+
+// Histograms of digits [0, num_low) when *gate is set (the gated full passes
+// of the finish path; the keys may be in any order).
+template <typename KeyT>
+__global__ __launch_bounds__(256) void k_hist_low(const uint32_t* __restrict__ gate,
+                                                  const KeyT* __restrict__ keys, uint32_t n,
+                                                  uint32_t num_low, uint32_t key_bits,
+                                                  uint32_t* __restrict__ hist) {
+  if (*gate == 0u) return;
+  __shared__ uint32_t s_h[kMaxPasses][kRadix];
+  for (uint32_t e = threadIdx.x; e < kMaxPasses * kRadix; e += blockDim.x) (&s_h[0][0])[e] = 0;
+  __syncthreads();
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const KeyT k = keys[i];
+    for (uint32_t p = 0; p < num_low; ++p)
+      atomicAdd(&s_h[p][digit_of(k, p * kRadixBits, (1u << pass_bits(key_bits, p)) - 1u)], 1u);
+  }
+  __syncthreads();
+  for (uint32_t e = threadIdx.x; e < num_low * kRadix; e += blockDim.x) {
+    const uint32_t v = (&s_h[0][0])[e];
+    if (v) atomicAdd(hist + e, v);
+  }
 }
 
 // One digit pass.  IOTA: values are the element indices (first pass).  With
@@ -470,9 +504,9 @@ inline int launch_pass(hipStream_t s, uint32_t n, uint32_t key_bits, const Layou
   return launch_status();
 }
 
-// Sorts the keys in `first_src` (values: the element indices).  The histogram
-// of every digit must already be in L.hist and L.tickets / L.status must be
-// zero.  The sorted keys and permutation end in (dst_keys, dst_vals).
+// Sorts the keys in `first_src` (values: the element indices).  The histograms
+// of digits [first_hist_pass, P) must already be in L.hist (the others zero)
+// and L.tickets / L.status must be zero.  The sorted keys and permutation end in (dst_keys, dst_vals).
 //   Up to four digits: one pass per digit, alternating so that the last lands
 //   in dst; `first_src` may alias dst_keys only when the number of passes is
 //   even.
@@ -518,6 +552,10 @@ inline int run_passes(hipStream_t s, uint32_t n, uint32_t key_bits, const Layout
          dim3(ceil_div(n, (uint64_t)kFinishThreads * kFinishItems)), dim3(kFinishThreads), 0, s,
          alt_k, alt_v,
          dst_keys, dst_vals, n, d0 * kRadixBits, kmask, gate);
+  if ((rc = launch_status()) != DMSH_OK) return rc;
+  launch("k_hist_low", k_hist_low<KeyT>, dim3(std::min<uint32_t>(num_tiles<KeyT>(n), 2048)),
+         dim3(256), 0, s, (const uint32_t*)gate, (const KeyT*)alt_k, n, first_hist_pass(P),
+         key_bits, L.hist);
   if ((rc = launch_status()) != DMSH_OK) return rc;
   for (uint32_t p = 0; p < P; ++p) {
     const bool from_alt = p % 2 == 0;
