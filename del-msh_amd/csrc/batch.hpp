@@ -245,10 +245,10 @@ static __global__ __launch_bounds__(256) void k_batch_morton_hist(
     const uint32_t code = morton3_transformed<uint32_t>(T, p[0], p[1], p[2]);
     const uint64_t key = ((uint64_t)m << kMeshKeyShift) | code;
     codes[t] = key;
-    sortk::hist_accumulate(s_h, key, num_passes, key_bits);
+    sortk::hist_accumulate(s_h, key, 0u, num_passes, key_bits);
   }
   __syncthreads();
-  sortk::hist_flush(s_h, num_passes, hist);
+  sortk::hist_flush(s_h, 0u, num_passes, hist);
 }
 
 // Internal node g of all meshes (g < inner_off[M]).

@@ -10,8 +10,8 @@
 //               refit reads one 24-byte record per leaf instead of gathering
 //               the indices and three vertices)
 //   morton      unit-cube fit of that box (Mat44/torch.py:120-140 formula),
-//               Morton code per element, and the digit histograms of every
-//               radix pass (the sort's histogram read is fused in here)
+//               Morton code per element, and the digit histograms of the
+//               radix passes (the sort's histogram read is fused in here)
 //   sort        onesweep digit passes (sort.hpp), values = element index
 //   topology    Karras tree (topology.hpp)
 //   refit       bottom-up boxes (refit.hpp)
@@ -142,7 +142,7 @@ __device__ __forceinline__ void fit_transform(const uint32_t* __restrict__ keys6
 template <typename KeyT>
 __global__ __launch_bounds__(256) void k_morton_hist(uint32_t n, const float* __restrict__ pts,
                                                      const uint32_t* __restrict__ keys6,
-                                                     KeyT* __restrict__ codes,
+                                                     KeyT* __restrict__ codes, uint32_t first,
                                                      uint32_t num_passes, uint32_t key_bits,
                                                      uint32_t* __restrict__ hist) {
   __shared__ uint32_t s_h[sortk::kMaxPasses][sortk::kRadix];
@@ -155,10 +155,10 @@ __global__ __launch_bounds__(256) void k_morton_hist(uint32_t n, const float* __
     const float* p = pts + (size_t)i * 3;
     const KeyT code = morton3_transformed<KeyT>(T, p[0], p[1], p[2]);
     codes[i] = code;
-    sortk::hist_accumulate(s_h, code, num_passes, key_bits);
+    sortk::hist_accumulate(s_h, code, first, num_passes, key_bits);
   }
   __syncthreads();
-  sortk::hist_flush(s_h, num_passes, hist);
+  sortk::hist_flush(s_h, first, num_passes, hist);
 }
 
 template <typename KeyT>
@@ -248,10 +248,10 @@ inline int build(hipStream_t s, uint32_t n, const uint32_t* tri2vtx, uint32_t nu
   KeyT* codes = static_cast<KeyT*>(L.codes);
   const uint32_t mh_blocks = std::min<uint32_t>(sortk::num_tiles<KeyT>(n), 2048);
   launch("k_morton_hist", k_morton_hist<KeyT>, dim3(mh_blocks), dim3(256), 0, s, n,
-                     TRI ? L.cntr : xyz, L.keys6, codes, P, kb, L.sort.hist);
+         TRI ? L.cntr : xyz, L.keys6, codes, sortk::first_hist_pass(P), P, kb, L.sort.hist);
   if ((rc = launch_status()) != DMSH_OK) return rc;
   // P is even for 30- and 63-bit codes: the passes end back in `codes`.
-  rc = sortk::run_passes<KeyT>(s, n, kb, L.sort, codes, codes, L.idx, err);
+  rc = sortk::run_passes<KeyT>(s, n, kb, L.sort, codes, codes, L.idx, err, true);
   if (rc != DMSH_OK) return rc;
   rc = launch_topology<KeyT>(s, n, bvhnodes, codes, L.idx,
                             TreeletSink{kTreeletLeaves, L.treelet_count, L.treelets, n, nullptr,

@@ -226,17 +226,16 @@ int dmsh_batch_build_trimesh3(void* stream, uint32_t num_mesh, uint32_t total_tr
   launch("k_batch_centroid", bat::k_batch_centroid,
          dim3(ceil_div(n, 4u * 64u * bat::kCentroidChunks)), dim3(256), 0, s, L.bad, n, M, mesh2tri, mesh2vtx, tri2vtx, vtx2xyz, L.cntr, L.keys6, err);
   if ((rc = launch_status()) != DMSH_OK) return rc;
-  // With an even number of passes (or the top-digit passes and the finish)
-  // the sort returns to the buffer it starts from; with an odd number of
-  // plain passes it starts from the alternate key buffer.
-  uint64_t* morton_out = (P % 2 == 0 || sortk::uses_finish(P))
-                             ? L.codes
-                             : static_cast<uint64_t*>(L.sort.alt_keys);
+  // With an even number of passes the sort returns to the buffer it starts
+  // from; with an odd number it starts from the alternate key buffer.
+  uint64_t* morton_out = (P % 2 == 0) ? L.codes : static_cast<uint64_t*>(L.sort.alt_keys);
   launch("k_morton_hist", bat::k_batch_morton_hist,
          dim3(std::min<uint32_t>(sortk::num_tiles<uint64_t>(n), 2048)), dim3(256), 0, s, L.bad, n,
          M, mesh2tri, L.cntr, L.keys6, morton_out, P, kb, L.sort.hist);
   if ((rc = launch_status()) != DMSH_OK) return rc;
-  rc = sortk::run_passes<uint64_t>(s, n, kb, L.sort, morton_out, L.codes, L.idx, err);
+  // plain passes: the top digits of a composite key are mostly the mesh index,
+  // so groups of equal top bits are too long for the LDS finish
+  rc = sortk::run_passes<uint64_t>(s, n, kb, L.sort, morton_out, L.codes, L.idx, err, false);
   if (rc != DMSH_OK) return rc;
   const TreeletSink ts{kTreeletLeaves, L.treelet_count, L.treelets, n, L.root_count, L.roots};
   launch("k_topology", bat::k_batch_topology, dim3(ceil_div(n, 256)), dim3(256), 0, s, L.bad, n,

@@ -53,9 +53,9 @@ __host__ __device__ inline bool uses_finish(uint32_t num_passes) {
   return num_passes > kTopPasses + 1;
 }
 
-// The first digit whose histogram the key producers accumulate: with the
-// finish, only the top digits' (the gated full passes build the others'
-// histograms themselves, k_hist_low).
+// The first digit whose histogram the key producers of a sort that may
+// finish in LDS accumulate: with the finish, only the top digits' (the gated
+// full passes build the others' histograms themselves, k_hist_low).
 __host__ __device__ inline uint32_t first_hist_pass(uint32_t num_passes) {
   return uses_finish(num_passes) ? num_passes - kTopPasses : 0u;
 }
@@ -118,21 +118,20 @@ __global__ __launch_bounds__(256) void k_copy_keys(const KeyT* __restrict__ a, K
     b[i] = a[i];
 }
 
-// Accumulates the histograms of passes [first_hist_pass, num_passes) into the
-// LDS bins of this block.
+// Accumulates the histograms of passes [first, num_passes) into the LDS bins
+// of this block.
 template <typename KeyT>
-__device__ __forceinline__ void hist_accumulate(uint32_t (*s_h)[kRadix], KeyT k,
+__device__ __forceinline__ void hist_accumulate(uint32_t (*s_h)[kRadix], KeyT k, uint32_t first,
                                                 uint32_t num_passes, uint32_t key_bits) {
-  for (uint32_t p = first_hist_pass(num_passes); p < num_passes; ++p) {
+  for (uint32_t p = first; p < num_passes; ++p) {
     const uint32_t mask = (1u << pass_bits(key_bits, p)) - 1u;
     atomicAdd(&s_h[p][digit_of(k, p * kRadixBits, mask)], 1u);
   }
 }
 
-__device__ __forceinline__ void hist_flush(uint32_t (*s_h)[kRadix], uint32_t num_passes,
-                                           uint32_t* __restrict__ hist) {
-  for (uint32_t e = first_hist_pass(num_passes) * kRadix + threadIdx.x; e < num_passes * kRadix;
-       e += blockDim.x) {
+__device__ __forceinline__ void hist_flush(uint32_t (*s_h)[kRadix], uint32_t first,
+                                           uint32_t num_passes, uint32_t* __restrict__ hist) {
+  for (uint32_t e = first * kRadix + threadIdx.x; e < num_passes * kRadix; e += blockDim.x) {
     const uint32_t v = (&s_h[0][0])[e];
     if (v) atomicAdd(hist + e, v);
   }
@@ -140,15 +139,15 @@ __device__ __forceinline__ void hist_flush(uint32_t (*s_h)[kRadix], uint32_t num
 
 template <typename KeyT>
 __global__ __launch_bounds__(256) void k_hist_all(const KeyT* __restrict__ keys, uint32_t n,
-                                                  uint32_t num_passes, uint32_t key_bits,
-                                                  uint32_t* __restrict__ hist) {
+                                                  uint32_t first, uint32_t num_passes,
+                                                  uint32_t key_bits, uint32_t* __restrict__ hist) {
   __shared__ uint32_t s_h[kMaxPasses][kRadix];
   for (uint32_t e = threadIdx.x; e < kMaxPasses * kRadix; e += blockDim.x) (&s_h[0][0])[e] = 0;
   __syncthreads();
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
-    hist_accumulate(s_h, keys[i], num_passes, key_bits);
+    hist_accumulate(s_h, keys[i], first, num_passes, key_bits);
   __syncthreads();
-  hist_flush(s_h, num_passes, hist);
+  hist_flush(s_h, first, num_passes, hist);
 }
 
 // Histograms of digits [0, num_low) when *gate is set (the gated full passes
@@ -504,24 +503,26 @@ inline int launch_pass(hipStream_t s, uint32_t n, uint32_t key_bits, const Layou
   return launch_status();
 }
 
-// Sorts the keys in `first_src` (values: the element indices).  The histograms
-// of digits [first_hist_pass, P) must already be in L.hist (the others zero)
-// and L.tickets / L.status must be zero.  The sorted keys and permutation end in (dst_keys, dst_vals).
-//   Up to four digits: one pass per digit, alternating so that the last lands
-//   in dst; `first_src` may alias dst_keys only when the number of passes is
-//   even.
-//   Five or more (uses_finish): top passes first_src -> alt -> dst -> alt, the finish
+// Sorts the keys in `first_src` (values: the element indices).  With `finish`
+// (allowed for uses_finish(P) keys) the histograms of digits
+// [first_hist_pass(P), P) must already be in L.hist and the others zero;
+// without it, every digit's.  L.tickets / L.status must be zero.  The sorted keys and permutation end in (dst_keys, dst_vals).
+//   Plain (up to four digits, or no finish): one pass per digit, alternating
+//   so that the last lands in dst; `first_src` may alias dst_keys only when
+//   the number of passes is even.
+//   With the finish: top passes first_src -> alt -> dst -> alt, the finish
 //   alt -> dst, then the gated full passes alt -> dst -> ... (and a gated copy
 //   back to dst when their number is even).  `first_src` must not be the
 //   alternate key buffer.
 template <typename KeyT>
 inline int run_passes(hipStream_t s, uint32_t n, uint32_t key_bits, const Layout& L,
-                      const KeyT* first_src, KeyT* dst_keys, uint32_t* dst_vals, uint32_t* err) {
+                      const KeyT* first_src, KeyT* dst_keys, uint32_t* dst_vals, uint32_t* err,
+                      bool finish) {
   const uint32_t P = num_passes(key_bits);
   KeyT* alt_k = static_cast<KeyT*>(L.alt_keys);
   uint32_t* alt_v = L.alt_vals;
   int rc;
-  if (!uses_finish(P)) {
+  if (!finish || !uses_finish(P)) {
     const KeyT* src_k = first_src;
     const uint32_t* src_v = nullptr;
     for (uint32_t p = 0; p < P; ++p) {
@@ -591,8 +592,8 @@ inline int argsort(hipStream_t s, uint32_t n, KeyT* keys_inout, uint32_t* idx_ou
   if (rc != DMSH_OK) return rc;
   const uint32_t P = num_passes(key_bits);
   const uint32_t hist_blocks = std::min<uint32_t>(num_tiles<KeyT>(n), 2048);
-  launch("k_hist_all", k_hist_all<KeyT>, dim3(hist_blocks), dim3(256), 0, s, keys_inout, n, P,
-                     key_bits, L.hist);
+  launch("k_hist_all", k_hist_all<KeyT>, dim3(hist_blocks), dim3(256), 0, s, keys_inout, n,
+         first_hist_pass(P), P, key_bits, L.hist);
   if ((rc = launch_status()) != DMSH_OK) return rc;
   const KeyT* first_src = keys_inout;
   if (P % 2 == 1 && !uses_finish(P)) {
@@ -603,7 +604,7 @@ inline int argsort(hipStream_t s, uint32_t n, KeyT* keys_inout, uint32_t* idx_ou
     if ((rc = launch_status()) != DMSH_OK) return rc;
     first_src = alt_k;
   }
-  return run_passes<KeyT>(s, n, key_bits, L, first_src, keys_inout, idx_out, err);
+  return run_passes<KeyT>(s, n, key_bits, L, first_src, keys_inout, idx_out, err, true);
 }
 
 }  // namespace sortk
