@@ -124,6 +124,11 @@ int dmsh_bvhnodes_from_sorted_mortons_u64(void *stream, uint32_t n,
  * CPU counterpart del-msh-cpu/src/bvhnode2aabb3.rs:68-142 (triangles) and
  * :6-63 (points).  bvhnode2aabb [num_bvhnode][6] = (min xyz, max xyz).
  * The tree must be an LBVH layout (leaves in rows num_leaf-1 .. 2 num_leaf-2).
+ * A tree whose internal nodes also lie inside their own leaf ranges (every
+ * Morton-code LBVH) is refit subtree by subtree in LDS; any other tree in
+ * that layout by the per-leaf climb through arrival counters.  Both give
+ * the same boxes.  The workspace (about 32 bytes per leaf) holds the
+ * counters, the leaf ranges and the subtree lists.
  * ------------------------------------------------------------------- */
 size_t dmsh_refit_workspace_bytes(uint32_t num_leaf);
 int dmsh_bvhnode2aabb_trimesh3(void *stream, float *bvhnode2aabb,
