@@ -11,6 +11,8 @@ using namespace dmsh;
 
 namespace {
 inline dim3 grid_for(uint64_t n) { return dim3((uint32_t)std::max<uint64_t>(1, ceil_div(n, 256))); }
+// query sets at least this large are searched in Morton order
+constexpr uint32_t kSortedSearchMin = 1u << 16;
 }  // namespace
 
 extern "C" {
@@ -50,22 +52,50 @@ int dmsh_bvhnode2aabb_polyhedron(void* stream, uint32_t num_bvhnode, const uint3
   return launch_refit_standalone<kLeafElem>(as_stream(stream), v, 0.f, ws, ws_bytes, err);
 }
 
+size_t dmsh_polyhedron_search_workspace_bytes(uint32_t num_wtx) {
+  Arena a{nullptr, ~(size_t)0, 0};
+  a.take<uint32_t>(num_wtx ? num_wtx : 1);  // query codes
+  a.take<uint32_t>(num_wtx ? num_wtx : 1);  // processing order
+  a.take<char>(sortk::workspace_bytes<uint32_t>(num_wtx ? num_wtx : 1, 30));
+  return a.off + 256;
+}
+
 int dmsh_polyhedron_search_elem_contain_points(void* stream, uint32_t num_elem,
                                                const uint32_t* elem2idx_offset, uint32_t num_idx,
                                                const uint32_t* idx2vtx, uint32_t num_vtx,
                                                const float* vtx2xyz, const uint32_t* bvhnodes,
                                                const float* bvhnode2aabb, uint32_t num_wtx,
                                                const float* wtx2xyz, uint32_t* wtx2elem,
-                                               float* wtx2param) {
+                                               float* wtx2param, void* ws, size_t ws_bytes) {
   if (num_wtx == 0) return DMSH_OK;
   if (!wtx2xyz || !wtx2elem || !wtx2param || !elem2idx_offset ||
       (num_elem > 0 && (!bvhnodes || !bvhnode2aabb || !idx2vtx || !vtx2xyz)))
     return DMSH_ERR_INVALID;
-  if (num_elem >= (1u << 31)) return DMSH_ERR_INVALID;
+  if (num_elem >= (1u << 31) || num_wtx >= (1u << 30)) return DMSH_ERR_INVALID;
+  hipStream_t s = as_stream(stream);
+  const uint32_t* order = nullptr;
+  // Large query sets are processed in Morton order of the query points
+  // (workspace given); small ones, or no workspace, in the given order.
+  if (ws && num_elem > 0 && num_wtx >= kSortedSearchMin) {
+    if (ws_bytes < dmsh_polyhedron_search_workspace_bytes(num_wtx)) return DMSH_ERR_WORKSPACE;
+    Arena a{static_cast<char*>(ws), ws_bytes, 0};
+    uint32_t* keys = a.take<uint32_t>(num_wtx);
+    uint32_t* idx = a.take<uint32_t>(num_wtx);
+    const size_t sort_bytes = sortk::workspace_bytes<uint32_t>(num_wtx, 30);
+    char* sort_ws = a.take<char>(sort_bytes);
+    if (!a.ok()) return DMSH_ERR_WORKSPACE;
+    launch("k_query_codes", poly::k_query_codes, grid_for(num_wtx), dim3(256), 0, s, num_wtx,
+           wtx2xyz, bvhnode2aabb, keys, idx);
+    int rc = launch_status();
+    if (rc != DMSH_OK) return rc;
+    rc = sortk::argsort<uint32_t>(s, num_wtx, keys, idx, 30, sort_ws, sort_bytes);
+    if (rc != DMSH_OK) return rc;
+    order = idx;
+  }
   const poly::MeshView m{num_elem, elem2idx_offset, num_idx, idx2vtx, num_vtx, vtx2xyz};
   launch("k_search_elem_contain_points", poly::k_search_elem_contain_points, grid_for(num_wtx),
-         dim3(256), 0, as_stream(stream), m, bvhnodes, bvhnode2aabb, num_wtx, wtx2xyz, wtx2elem,
-         wtx2param, device_error_word());
+         dim3(256), 0, s, m, bvhnodes, bvhnode2aabb, num_wtx, wtx2xyz, wtx2elem, wtx2param,
+         order, device_error_word());
   return launch_status();
 }
 

@@ -36,6 +36,7 @@
 #pragma once
 
 #include "dmsh_common.hpp"
+#include "morton.hpp"
 
 namespace dmsh {
 namespace poly {
@@ -257,13 +258,39 @@ struct MeshView {
   const float* xyz;
 };
 
-// one thread per query point
+// 30-bit Morton code of each query point in the unit-cube fit of the root
+// box (aabb row 0), and the identity permutation: sorting by these codes puts
+// nearby queries in one wave, so its lanes walk the same branches and test
+// the same elements.  Only the processing order changes, not any result.
+static __global__ __launch_bounds__(256) void k_query_codes(uint32_t num_wtx,
+                                                            const float* __restrict__ wtx2xyz,
+                                                            const float* __restrict__ aabb,
+                                                            uint32_t* __restrict__ keys,
+                                                            uint32_t* __restrict__ idx) {
+  const uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= num_wtx) return;
+  float ext = fmax_sel(fmax_sel(aabb[3] - aabb[0], aabb[4] - aabb[1]), aabb[5] - aabb[2]);
+  const float s = ext > 0.f ? 1.f / ext : 1.f;
+  float T[16] = {s, 0.f, 0.f, 0.f, 0.f, s, 0.f, 0.f, 0.f, 0.f, s, 0.f,
+                 -aabb[0] * s, -aabb[1] * s, -aabb[2] * s, 1.f};
+  keys[w] = morton3_transformed<uint32_t>(T, wtx2xyz[(size_t)w * 3], wtx2xyz[(size_t)w * 3 + 1],
+                                          wtx2xyz[(size_t)w * 3 + 2]);
+  idx[w] = w;
+}
+
+// one thread per query point; with `order`, thread i takes query order[i]
 static __global__ __launch_bounds__(256) void k_search_elem_contain_points(
     MeshView m, const uint32_t* __restrict__ bvhnodes, const float* __restrict__ aabb,
     uint32_t num_wtx, const float* __restrict__ wtx2xyz, uint32_t* __restrict__ wtx2elem,
-    float* __restrict__ wtx2param, uint32_t* __restrict__ err) {
-  const uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
-  if (w >= num_wtx) return;
+    float* __restrict__ wtx2param, const uint32_t* __restrict__ order,
+    uint32_t* __restrict__ err) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= num_wtx) return;
+  const uint32_t w = order ? order[i] : i;
+  if (w >= num_wtx) {
+    raise_error(err, DMSH_DEVERR_INDEX);
+    return;
+  }
   const V3 q{wtx2xyz[(size_t)w * 3], wtx2xyz[(size_t)w * 3 + 1], wtx2xyz[(size_t)w * 3 + 2]};
   uint32_t found = DMSH_U32MAX;
   float pco[3] = {0.f, 0.f, 0.f};
@@ -295,16 +322,21 @@ static __global__ __launch_bounds__(256) void k_search_elem_contain_points(
             break;
           }
           if (n == 4 || n == 5 || n == 6 || n == 8) {
+            // all eight slots by constant index (registers, not scratch)
             V3 p[8];
             bool ok = true;
-            for (uint32_t k = 0; k < n; ++k) {
-              const uint32_t v = m.idx2vtx[i0 + k];
-              if (v >= m.num_vtx) {
-                ok = false;
-                break;
+#pragma unroll
+            for (uint32_t k = 0; k < 8; ++k) {
+              p[k] = V3{0.f, 0.f, 0.f};
+              if (k < n) {
+                const uint32_t v = m.idx2vtx[i0 + k];
+                if (v >= m.num_vtx) {
+                  ok = false;
+                } else {
+                  const float* x = m.xyz + (size_t)v * 3;
+                  p[k] = V3{x[0] - q.x, x[1] - q.y, x[2] - q.z};
+                }
               }
-              const float* x = m.xyz + (size_t)v * 3;
-              p[k] = V3{x[0] - q.x, x[1] - q.y, x[2] - q.z};
             }
             if (!ok) {
               raise_error(err, DMSH_DEVERR_INDEX);

@@ -41,11 +41,13 @@ def search_elem_contain_points(bvhnodes, bvhnode2aabb, elem2idx_offset, idx2vtx,
     bvhnodes, bvhnode2aabb, elem2idx_offset, idx2vtx, vtx2xyz, wtx2xyz, wtx2elem, wtx2param = ts
     dev = _lib.require_gpu(*ts)
     stream = stream_ptr or _lib.stream_of(dev)
-    _lib.check(_lib.lib().dmsh_polyhedron_search_elem_contain_points(
+    L = _lib.lib()
+    ws = _lib.workspace(L.dmsh_polyhedron_search_workspace_bytes(wtx2xyz.shape[0]), dev)
+    _lib.check(L.dmsh_polyhedron_search_elem_contain_points(
         stream, elem2idx_offset.shape[0] - 1, _lib.ptr(elem2idx_offset), idx2vtx.shape[0],
         _lib.ptr(idx2vtx), vtx2xyz.shape[0], _lib.ptr(vtx2xyz), _lib.ptr(bvhnodes),
         _lib.ptr(bvhnode2aabb), wtx2xyz.shape[0], _lib.ptr(wtx2xyz), _lib.ptr(wtx2elem),
-        _lib.ptr(wtx2param)), "dmsh_polyhedron_search_elem_contain_points")
+        _lib.ptr(wtx2param), _lib.ptr(ws), ws.numel()), "dmsh_polyhedron_search_elem_contain_points")
 
 
 def interpolate_values_at_points(elem2idx_offset, idx2vtx, vtx2value, wtx2elem, wtx2param,

@@ -176,9 +176,10 @@ _PROTOTYPES = {
     "dmsh_bvhnode2aabb_polyhedron": (
         c_int, [c_void_p, c_u32, c_void_p, c_u32, c_void_p, c_u32, c_void_p, c_u32, c_void_p,
                 c_void_p, c_void_p, c_size]),
+    "dmsh_polyhedron_search_workspace_bytes": (c_size, [c_u32]),
     "dmsh_polyhedron_search_elem_contain_points": (
         c_int, [c_void_p, c_u32, c_void_p, c_u32, c_void_p, c_u32, c_void_p, c_void_p, c_void_p,
-                c_u32, c_void_p, c_void_p, c_void_p]),
+                c_u32, c_void_p, c_void_p, c_void_p, c_void_p, c_size]),
     "dmsh_polyhedron_interpolate_at_points": (
         c_int, [c_void_p, c_u32, c_u32, c_void_p, c_u32, c_void_p, c_u32, c_u32, c_void_p,
                 c_void_p, c_void_p, c_void_p]),

@@ -568,13 +568,18 @@ int dmsh_bvhnode2aabb_polyhedron(void *stream, uint32_t num_bvhnode, const uint3
                                  uint32_t num_idx, const uint32_t *idx2vtx, uint32_t num_vtx,
                                  const float *vtx2xyz, float *bvhnode2aabb, void *ws,
                                  size_t ws_bytes);
+/* ws (optional, dmsh_polyhedron_search_workspace_bytes(num_wtx)): with it,
+ * sets of 65536 or more query points are searched in the Morton order of the
+ * points in the root box (only the order of work changes, not any result);
+ * ws == NULL searches in the given order. */
+size_t dmsh_polyhedron_search_workspace_bytes(uint32_t num_wtx);
 int dmsh_polyhedron_search_elem_contain_points(void *stream, uint32_t num_elem,
                                                const uint32_t *elem2idx_offset, uint32_t num_idx,
                                                const uint32_t *idx2vtx, uint32_t num_vtx,
                                                const float *vtx2xyz, const uint32_t *bvhnodes,
                                                const float *bvhnode2aabb, uint32_t num_wtx,
                                                const float *wtx2xyz, uint32_t *wtx2elem,
-                                               float *wtx2param);
+                                               float *wtx2param, void *ws, size_t ws_bytes);
 int dmsh_polyhedron_interpolate_at_points(void *stream, uint32_t num_wtx, uint32_t num_elem,
                                           const uint32_t *elem2idx_offset, uint32_t num_idx,
                                           const uint32_t *idx2vtx, uint32_t num_vtx,

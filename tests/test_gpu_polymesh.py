@@ -120,3 +120,13 @@ def test_refit_stored_tree(gpu, oracle, layout):
     PM.bvhnode2aabb_from_bvhnodes(G(off, gpu), G(iv, gpu), G(x, gpu), G(bn, gpu), aabb)
     assert bits_equal(H(aabb), oracle.polyhedron_refit(bn, off, iv, x))
     assert _lib.device_errors() == 0
+
+
+def test_search_in_morton_order(gpu, oracle):
+    """100 000 queries (>= 65536): the search runs in the Morton order of the
+    query points; every result still lands at its own query, bit-exact."""
+    off, iv, x = primitives.mixed_hex_grid(16, 12, 10, seed=7, jitter=0.2)
+    rng = np.random.default_rng(8)
+    q = rng.uniform([-0.5, -0.5, -0.5], [16.5, 12.5, 10.5], (100_000, 3)).astype(np.float32)
+    elem = _check_mesh(gpu, oracle, off, iv, x, q)
+    assert (elem != 0xFFFFFFFF).mean() > 0.7
