@@ -35,6 +35,8 @@ static __global__ __launch_bounds__(256) void k_build_init(uint32_t* __restrict_
   for (size_t i = i0; i < zero_count; i += (size_t)gridDim.x * blockDim.x) zero[i] = 0u;
 }
 
+constexpr int kCentroidItems = 4;  // elements whose loads a thread issues together
+
 template <bool TRI>
 __global__ __launch_bounds__(256) void k_centroid_aabb(uint32_t n,
                                                        const uint32_t* __restrict__ tri2vtx,
@@ -46,40 +48,73 @@ __global__ __launch_bounds__(256) void k_centroid_aabb(uint32_t n,
                                                        uint32_t* __restrict__ err) {
   __shared__ uint32_t s_red[4][6];
   uint32_t kmin[3] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu}, kmax[3] = {0u, 0u, 0u};
-  // grid-stride: a bounded number of blocks keeps the 6 global atomics cheap
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    float c[3];
-    if (TRI) {
-      const uint32_t* tv = tri2vtx + (size_t)i * 3;
-      uint32_t v[3] = {tv[0], tv[1], tv[2]};
-      if (v[0] >= num_vtx || v[1] >= num_vtx || v[2] >= num_vtx) {
-        raise_error(err, DMSH_DEVERR_INDEX);
-        v[0] = v[1] = v[2] = 0;
-      }
-      const float ratio = 1.f / 3.f;
+  // grid-stride: a bounded number of blocks keeps the 6 global atomics cheap;
+  // each round issues the loads of kCentroidItems elements before using them
+  const uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t i0 = blockIdx.x * blockDim.x + threadIdx.x; i0 < n;
+       i0 += stride * kCentroidItems) {
+    uint32_t v[kCentroidItems][3];
 #pragma unroll
-      for (int d = 0; d < 3; ++d) {
-        float cog = 0.f;
-        cog += xyz[(size_t)v[0] * 3 + d];
-        cog += xyz[(size_t)v[1] * 3 + d];
-        cog += xyz[(size_t)v[2] * 3 + d];
-        c[d] = cog * ratio;
-        cntr[(size_t)i * 3 + d] = c[d];
+    for (int u = 0; u < kCentroidItems; ++u) {
+      const uint32_t i = i0 + (uint32_t)u * stride;
+      v[u][0] = v[u][1] = v[u][2] = 0u;
+      if (TRI && i < n) {
+        const uint32_t* tv = tri2vtx + (size_t)i * 3;
+        v[u][0] = tv[0];
+        v[u][1] = tv[1];
+        v[u][2] = tv[2];
       }
-      Box b;
-      box_set(b, xyz + (size_t)v[0] * 3, 0.f);
-      box_add(b, xyz + (size_t)v[1] * 3, 0.f);
-      box_add(b, xyz + (size_t)v[2] * 3, 0.f);
-      box_store(tri_box, i, b);
-    } else {
+    }
+    float x[kCentroidItems][9];
 #pragma unroll
-      for (int d = 0; d < 3; ++d) c[d] = xyz[(size_t)i * 3 + d];
+    for (int u = 0; u < kCentroidItems; ++u) {
+      const uint32_t i = i0 + (uint32_t)u * stride;
+      if (i >= n) continue;
+      if (TRI) {
+        if (v[u][0] >= num_vtx || v[u][1] >= num_vtx || v[u][2] >= num_vtx) {
+          raise_error(err, DMSH_DEVERR_INDEX);
+          v[u][0] = v[u][1] = v[u][2] = 0;
+        }
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+#pragma unroll
+          for (int d = 0; d < 3; ++d) x[u][3 * k + d] = xyz[(size_t)v[u][k] * 3 + d];
+      } else {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) x[u][d] = xyz[(size_t)i * 3 + d];
+      }
     }
 #pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      const uint32_t key = f32_order_key(c[d]);
-      kmin[d] = key < kmin[d] ? key : kmin[d];
-      kmax[d] = key > kmax[d] ? key : kmax[d];
+    for (int u = 0; u < kCentroidItems; ++u) {
+      const uint32_t i = i0 + (uint32_t)u * stride;
+      if (i >= n) continue;
+      float c[3];
+      if (TRI) {
+        const float ratio = 1.f / 3.f;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          float cog = 0.f;
+          cog += x[u][d];
+          cog += x[u][3 + d];
+          cog += x[u][6 + d];
+          c[d] = cog * ratio;
+          cntr[(size_t)i * 3 + d] = c[d];
+        }
+        Box b;
+        box_set(b, &x[u][0], 0.f);
+        box_add(b, &x[u][3], 0.f);
+        box_add(b, &x[u][6], 0.f);
+        box_store(tri_box, i, b);
+      } else {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) c[d] = x[u][d];
+      }
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        const uint32_t key = f32_order_key(c[d]);
+        kmin[d] = key < kmin[d] ? key : kmin[d];
+        kmax[d] = key > kmax[d] ? key : kmax[d];
+      }
     }
   }
 #pragma unroll
@@ -151,11 +186,24 @@ __global__ __launch_bounds__(256) void k_morton_hist(uint32_t n, const float* __
   float T[16];
   fit_transform(keys6, T);
   __syncthreads();
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    const float* p = pts + (size_t)i * 3;
-    const KeyT code = morton3_transformed<KeyT>(T, p[0], p[1], p[2]);
-    codes[i] = code;
-    sortk::hist_accumulate(s_h, code, first, num_passes, key_bits);
+  const uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t i0 = blockIdx.x * blockDim.x + threadIdx.x; i0 < n;
+       i0 += stride * kCentroidItems) {
+    float x[kCentroidItems][3];
+#pragma unroll
+    for (int u = 0; u < kCentroidItems; ++u) {
+      const uint32_t i = i0 + (uint32_t)u * stride;
+#pragma unroll
+      for (int d = 0; d < 3; ++d) x[u][d] = i < n ? pts[(size_t)i * 3 + d] : 0.f;
+    }
+#pragma unroll
+    for (int u = 0; u < kCentroidItems; ++u) {
+      const uint32_t i = i0 + (uint32_t)u * stride;
+      if (i >= n) continue;
+      const KeyT code = morton3_transformed<KeyT>(T, x[u][0], x[u][1], x[u][2]);
+      codes[i] = code;
+      sortk::hist_accumulate(s_h, code, first, num_passes, key_bits);
+    }
   }
   __syncthreads();
   sortk::hist_flush(s_h, first, num_passes, hist);
