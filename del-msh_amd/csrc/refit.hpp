@@ -758,7 +758,7 @@ inline size_t standalone_workspace_bytes(uint32_t n) {
 // or the treelet kernels' own bounds checks.
 constexpr int kRangeWalks = 1;
 constexpr int kListThreads = 1024;
-constexpr uint32_t kListGrid = 256;
+constexpr uint32_t kListGrid = 65535;  // one round of kListThreads nodes per workgroup
 
 static __global__ __launch_bounds__(256) void k_leaf_ranges(uint32_t num_leaf,
                                                             const uint32_t* __restrict__ bvhnodes,
