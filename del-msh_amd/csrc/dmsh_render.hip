@@ -373,4 +373,18 @@ int dmsh_interpolate_staggered_grid(void* stream, uint32_t img_w, uint32_t img_h
   return launch_status();
 }
 
+int dmsh_smoothed_vertex_gradient(void* stream, uint32_t img_w, uint32_t img_h,
+                                  const float* hedge2dldr, const float* vedge2dldr,
+                                  uint32_t num_vtx, const float* vtx2xyz,
+                                  const float* transform_world2pix, float* dldw_vtx2xyz) {
+  if (num_vtx == 0) return DMSH_OK;
+  if (img_w < 2 || img_h < 2 || !hedge2dldr || !vedge2dldr || !vtx2xyz || !transform_world2pix ||
+      !dldw_vtx2xyz)
+    return DMSH_ERR_INVALID;
+  launch("k_smoothed_vertex_grad", eg::k_smoothed_vertex_grad, dim3(ceil_div(num_vtx, 256)),
+         dim3(256), 0, as_stream(stream), img_w, img_h, hedge2dldr, vedge2dldr, num_vtx, vtx2xyz,
+         transform_world2pix, dldw_vtx2xyz);
+  return launch_status();
+}
+
 }  // extern "C"

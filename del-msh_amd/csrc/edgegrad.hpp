@@ -444,12 +444,13 @@ __device__ __forceinline__ float clamp01(float x) { return x < 0.f ? 0.f : (x > 
 
 // One thread per vertex: x-velocity bilinear from the vedges (sampled at
 // (iw + 1, ih + 0.5)), y-velocity from the hedges (at (iw + 0.5, ih + 1)).
-static __global__ __launch_bounds__(256) void k_interpolate_staggered(
-    uint32_t W, uint32_t H, const float* __restrict__ hvy, const float* __restrict__ vvx,
-    uint32_t num_vtx, const float* __restrict__ vtx2xy, float* __restrict__ vtx2velo) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= num_vtx) return;
-  const float px = vtx2xy[(size_t)i * 2], py = vtx2xy[(size_t)i * 2 + 1];
+// Bilinear read of the staggered pixel-edge grids at point (px, py):
+// (x velocity from the vedges, y velocity from the hedges),
+// rasterized_edge_gradient.rs:202-248.
+__device__ __forceinline__ void interp_staggered(uint32_t W, uint32_t H,
+                                                 const float* __restrict__ hvy,
+                                                 const float* __restrict__ vvx, float px, float py,
+                                                 float& vx, float& vy) {
   {
     const float gx = px - 1.0f, gy = py - 0.5f;
     const int ix0 = clampi(f2i_sat(floorf(gx)), 0, (int)W - 2);
@@ -458,10 +459,9 @@ static __global__ __launch_bounds__(256) void k_interpolate_staggered(
     const int iy1 = iy0 + 1 < (int)H - 1 ? iy0 + 1 : (int)H - 1;
     const float tx = clamp01(gx - (float)ix0), ty = clamp01(gy - (float)iy0);
     const int w = (int)W - 1;
-    vtx2velo[(size_t)i * 2] = (((1.f - tx) * (1.f - ty) * vvx[iy0 * w + ix0] +
-                                tx * (1.f - ty) * vvx[iy0 * w + ix1]) +
-                               (1.f - tx) * ty * vvx[iy1 * w + ix0]) +
-                              tx * ty * vvx[iy1 * w + ix1];
+    vx = (((1.f - tx) * (1.f - ty) * vvx[iy0 * w + ix0] + tx * (1.f - ty) * vvx[iy0 * w + ix1]) +
+          (1.f - tx) * ty * vvx[iy1 * w + ix0]) +
+         tx * ty * vvx[iy1 * w + ix1];
   }
   {
     const float gx = px - 0.5f, gy = py - 1.0f;
@@ -470,11 +470,46 @@ static __global__ __launch_bounds__(256) void k_interpolate_staggered(
     const int ix1 = ix0 + 1 < (int)W - 1 ? ix0 + 1 : (int)W - 1;
     const int iy1 = iy0 + 1 < (int)H - 2 ? iy0 + 1 : (int)H - 2;
     const float tx = clamp01(gx - (float)ix0), ty = clamp01(gy - (float)iy0);
-    vtx2velo[(size_t)i * 2 + 1] = (((1.f - tx) * (1.f - ty) * hvy[iy0 * W + ix0] +
-                                    tx * (1.f - ty) * hvy[iy0 * W + ix1]) +
-                                   (1.f - tx) * ty * hvy[iy1 * W + ix0]) +
-                                  tx * ty * hvy[iy1 * W + ix1];
+    vy = (((1.f - tx) * (1.f - ty) * hvy[iy0 * W + ix0] + tx * (1.f - ty) * hvy[iy0 * W + ix1]) +
+          (1.f - tx) * ty * hvy[iy1 * W + ix0]) +
+         tx * ty * hvy[iy1 * W + ix1];
   }
+}
+
+static __global__ __launch_bounds__(256) void k_interpolate_staggered(
+    uint32_t W, uint32_t H, const float* __restrict__ hvy, const float* __restrict__ vvx,
+    uint32_t num_vtx, const float* __restrict__ vtx2xy, float* __restrict__ vtx2velo) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= num_vtx) return;
+  float vx, vy;
+  interp_staggered(W, H, hvy, vvx, vtx2xy[(size_t)i * 2], vtx2xy[(size_t)i * 2 + 1], vx, vy);
+  vtx2velo[(size_t)i * 2] = vx;
+  vtx2velo[(size_t)i * 2 + 1] = vy;
+}
+
+// The tail of the smoothed edge-gradient backward, per vertex (the Python
+// chain of RasterizedEdgeGradient/torch.py:237-274 in one pass): q = M [x, 1]
+// (M = world2pix, column-major T), u = (q0, q1) / q3, g = dL/du read from the
+// smoothed pixel-edge grids at u, dL/dq = (g / q3, 0, -(g . u) / q3),
+// dL/dx = dL/dq M[:, :3].
+static __global__ __launch_bounds__(256) void k_smoothed_vertex_grad(
+    uint32_t W, uint32_t H, const float* __restrict__ hvy, const float* __restrict__ vvx,
+    uint32_t num_vtx, const float* __restrict__ xyz, const float* __restrict__ T,
+    float* __restrict__ dldx) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= num_vtx) return;
+  const float x0 = xyz[(size_t)i * 3], x1 = xyz[(size_t)i * 3 + 1], x2 = xyz[(size_t)i * 3 + 2];
+  float q[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) q[k] = ((T[k] * x0 + T[4 + k] * x1) + T[8 + k] * x2) + T[12 + k];
+  const float inv_w = 1.f / q[3];
+  const float u0 = q[0] * inv_w, u1 = q[1] * inv_w;
+  float g0, g1;
+  interp_staggered(W, H, hvy, vvx, u0, u1, g0, g1);
+  const float d0 = g0 * inv_w, d1 = g1 * inv_w, d3 = -(g0 * u0 + g1 * u1) * inv_w;
+#pragma unroll
+  for (int j = 0; j < 3; ++j)
+    dldx[(size_t)i * 3 + j] = (d0 * T[j * 4] + d1 * T[j * 4 + 1]) + d3 * T[j * 4 + 3];
 }
 
 }  // namespace eg

@@ -67,3 +67,18 @@ def interpolate(hedge2vy, vedge2vx, vtx2xy, vtx2velo, stream_ptr=0):
     _lib.check(_lib.lib().dmsh_interpolate_staggered_grid(
         stream, img_w, img_h, _lib.ptr(hv), _lib.ptr(vv), xy.shape[0], _lib.ptr(xy), _lib.ptr(out)),
         "dmsh_interpolate_staggered_grid")
+
+
+def smoothed_vertex_gradient(hedge2dldr, vedge2dldr, vtx2xyz, transform_world2pix_colmajor,
+                             dldw_vtx2xyz, stream_ptr=0):
+    """dL/dx per vertex from the smoothed pixel-edge grids, through the
+    perspective projection (the tail of the smoothed backward in one launch)."""
+    args = tuple(map(as_tensor, (hedge2dldr, vedge2dldr, vtx2xyz, transform_world2pix_colmajor,
+                                 dldw_vtx2xyz)))
+    dev = _lib.require_gpu(*args)
+    hv, vv, xyz, t, out = args
+    img_h, img_w = hv.shape[0] + 1, hv.shape[1]
+    stream = stream_ptr or _lib.stream_of(dev)
+    _lib.check(_lib.lib().dmsh_smoothed_vertex_gradient(
+        stream, img_w, img_h, _lib.ptr(hv), _lib.ptr(vv), xyz.shape[0], _lib.ptr(xyz), _lib.ptr(t),
+        _lib.ptr(out)), "dmsh_smoothed_vertex_gradient")

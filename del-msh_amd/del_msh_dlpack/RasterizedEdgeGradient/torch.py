@@ -141,12 +141,12 @@ class AutogradWithSmooth(torch.autograd.Function):
             tri2vtx, vtx2xyz, transform_world2pix, pix2tri, pix2val, dldw_pix2val.contiguous())
         smooth_gradient(hedge2type, vedge2type, 100, hedge2dldr, vedge2dldr)
         # pixel position u = (q0, q1) / q3 with q = M [x, 1] (M = world2pix,
-        # possibly perspective): du/dq = [I2 / q3 | 0 | -u / q3], dq/dx = M[:, :3]
-        m = transform_world2pix
-        q = vtx2xyz.detach() @ m[:, :3].T + m[:, 3]
-        inv_w = 1.0 / q[:, 3:4]
-        u = (q[:, 0:2] * inv_w).contiguous()
-        g = interpolate(hedge2dldr, vedge2dldr, u)  # dL/du at each vertex
-        dldq = torch.cat([g * inv_w, torch.zeros_like(inv_w), -(g * u).sum(1, keepdim=True) * inv_w],
-                         dim=1)
-        return None, dldq @ m[:, :3], None, None, dldw_pix2val
+        # possibly perspective): du/dq = [I2 / q3 | 0 | -u / q3], dq/dx = M[:, :3];
+        # one kernel projects, reads dL/du from the grids and chains
+        from . import smoothed_vertex_gradient as native
+
+        xyz = vtx2xyz.detach().contiguous()
+        dldx = torch.empty_like(xyz)
+        native(hedge2dldr, vedge2dldr, xyz, transform_world2pix.T.contiguous(), dldx,
+               _lib.stream_of(xyz.device))
+        return None, dldx, None, None, dldw_pix2val

@@ -134,6 +134,8 @@ _PROTOTYPES = {
                 c_size]),
     "dmsh_interpolate_staggered_grid": (
         c_int, [c_void_p, c_u32, c_u32, c_void_p, c_void_p, c_u32, c_void_p, c_void_p]),
+    "dmsh_smoothed_vertex_gradient": (
+        c_int, [c_void_p, c_u32, c_u32, c_void_p, c_void_p, c_u32, c_void_p, c_void_p, c_void_p]),
     "dmsh_exclusive_scan_workspace_bytes": (c_size, [c_u32]),
     "dmsh_exclusive_scan_u32": (c_int, [c_void_p, c_u32, c_void_p, c_void_p, c_void_p, c_size]),
     "dmsh_vtx2elem_workspace_bytes": (c_size, [c_u32, c_u32, c_u32]),

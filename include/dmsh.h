@@ -243,6 +243,15 @@ int dmsh_smooth_gradient(void *stream, uint32_t img_w, uint32_t img_h, const uin
 int dmsh_interpolate_staggered_grid(void *stream, uint32_t img_w, uint32_t img_h,
                                     const float *hedge2vy, const float *vedge2vx,
                                     uint32_t num_vtx, const float *vtx2xy, float *vtx2velo);
+/* The tail of AutogradWithSmooth.backward (RasterizedEdgeGradient/torch.py:
+ * 237-274: project the vertices, read the smoothed grids there, chain through
+ * the projection) in one kernel: dldw_vtx2xyz [num_vtx][3] from the smoothed
+ * hedge2dldr / vedge2dldr, vtx2xyz [num_vtx][3] and the column-major
+ * transform_world2pix (4x4). */
+int dmsh_smoothed_vertex_gradient(void *stream, uint32_t img_w, uint32_t img_h,
+                                  const float *hedge2dldr, const float *vedge2dldr,
+                                  uint32_t num_vtx, const float *vtx2xyz,
+                                  const float *transform_world2pix, float *dldw_vtx2xyz);
 
 /* ---------------------------------------------------------------------
  * Attribute interpolation over pix2tri and its backward.  Replaces
