@@ -275,8 +275,12 @@ class SilhouetteBatch:
                 (int(p.mesh2pix_h[m]), int(p.mesh2pix_h[m + 1])))
 
     def hits(self, vtx2xyz: torch.Tensor) -> dict:
-        """BVHs + pix2tri (batched), occupancy, contour edges per mesh."""
-        from ..Edge2Vtx.torch import contour_for_triangle_mesh
+        """BVHs + pix2tri (batched), occupancy, contour edges per mesh.  The
+        meshes' contour extractions are enqueued back to back into slices of
+        one buffer, sharing one workspace, and their counts are read with one
+        synchronization (the same kernels and results as one call per mesh)."""
+        from .. import _lib
+        from ..Edge2Vtx import contour_for_triangle_mesh_into
 
         p = self.pipe
         p.set_vertices(vtx2xyz)
@@ -284,15 +288,31 @@ class SilhouetteBatch:
         pix2tri = p.pix2tri[: p.total_pix].clone()
         occ = (pix2tri != 0xFFFFFFFF).to(torch.float32)
         v = p.vtx2xyz
-        cedges = []
+        dev = pix2tri.device
+        sizes = [int(self.edges[m][0].shape[0]) for m in range(p.num_mesh)]
+        offs = [0]
+        for n in sizes:
+            offs.append(offs[-1] + n)
+        cbuf = torch.empty((max(offs[-1], 1), 2), dtype=torch.uint32, device=dev)
+        counts = torch.zeros((max(p.num_mesh, 1),), dtype=torch.uint32, device=dev)
+        ws = _lib.workspace(_lib.lib().dmsh_contour_workspace_bytes(max(sizes + [1])), dev)
+        live = []
         for m in range(p.num_mesh):
             (t0, t1), (v0, v1), _ = self._slices(m)
             e2v, e2t = self.edges[m]
             if t1 == t0 or e2v.shape[0] == 0:
-                cedges.append(e2v)
                 continue
-            cedges.append(contour_for_triangle_mesh(p.tri2vtx[t0:t1], v[v0:v1], self.world2ndc[m],
-                                                    e2v, e2t))
+            contour_for_triangle_mesh_into(p.tri2vtx[t0:t1], v[v0:v1],
+                                           self.world2ndc[m].T.contiguous().flatten(), e2v, e2t,
+                                           cbuf[offs[m]:offs[m + 1]], counts[m:m + 1], ws)
+            live.append(m)
+        counts_h = counts.view(torch.int32).cpu().tolist()
+        cedges = []
+        for m in range(p.num_mesh):
+            if m in live:
+                cedges.append(cbuf[offs[m]:offs[m] + counts_h[m]])
+            else:
+                cedges.append(self.edges[m][0])
         return {"pix2tri": pix2tri, "occ": occ, "cedges": cedges}
 
     @staticmethod

@@ -32,6 +32,27 @@ def contour_for_triangle_mesh(tri2vtx, vtx2xyz, transform_world2ndc, edge2vtx, e
     return cedge2vtx[:n].clone()
 
 
+def contour_for_triangle_mesh_into(tri2vtx, vtx2xyz, transform_world2ndc, edge2vtx, edge2tri,
+                                   cedge2vtx, count, ws, stream_ptr=0):
+    """As contour_for_triangle_mesh, without the read-back: the contour edges
+    go to the front of cedge2vtx (at least (num_edge, 2) uint32) and their
+    number to count (one uint32 on the device); ws is a workspace of at least
+    dmsh_contour_workspace_bytes(num_edge) bytes.  Nothing synchronizes, so a
+    caller can enqueue many meshes and read all the counts at once."""
+    args = tuple(map(as_tensor, (tri2vtx, vtx2xyz, transform_world2ndc, edge2vtx, edge2tri,
+                                 cedge2vtx, count, ws)))
+    tri2vtx, vtx2xyz, transform_world2ndc, edge2vtx, edge2tri, cedge2vtx, count, ws = args
+    dev = _lib.require_gpu(*args)
+    stream = stream_ptr or _lib.stream_of(dev)
+    _lib.check(
+        _lib.lib().dmsh_contour_for_triangle_mesh(
+            stream, tri2vtx.shape[0], _lib.ptr(tri2vtx), vtx2xyz.shape[0], _lib.ptr(vtx2xyz),
+            _lib.ptr(transform_world2ndc), edge2vtx.shape[0], _lib.ptr(edge2vtx),
+            _lib.ptr(edge2tri), _lib.ptr(cedge2vtx), _lib.ptr(count), _lib.ptr(ws), ws.numel()),
+        "dmsh_contour_for_triangle_mesh",
+    )
+
+
 def from_vtx2vtx(vtx2idx_offset, idx2vtx, edge2vtx, stream_ptr=0):
     """edge2vtx[idx] = (i, idx2vtx[idx]) for every entry idx of vertex i (in place)."""
     vtx2idx_offset, idx2vtx, edge2vtx = map(as_tensor, (vtx2idx_offset, idx2vtx, edge2vtx))
