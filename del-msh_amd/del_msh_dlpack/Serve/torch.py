@@ -221,9 +221,12 @@ class Server:
             off += px
             self.cache.unpin(r.mesh_id)
 
-    def run(self, requests, keep_results: bool = False, max_inflight: int = 2):
+    def run(self, requests, keep_results: bool = False, max_inflight: int = 3):
         """Serves `requests` (sorted by arrival, seconds from now) in real
-        time; fills r.done (same clock) and, if asked, r.result."""
+        time; fills r.done (same clock) and, if asked, r.result.  Up to
+        max_inflight steps are queued on the GPU while the host prepares the
+        next (three kept the GPU busier than two at saturation, with lower
+        p50 / p99 latency too)."""
         pending = deque(requests)
         queue: list = []
         inflight: deque = deque()
