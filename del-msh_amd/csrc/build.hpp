@@ -25,13 +25,20 @@
 namespace dmsh {
 namespace bld {
 
-// keys6[0..2] = min order keys (init 0xFFFFFFFF), keys6[3..5] = max (init 0),
-// then `zero_count` words starting at `zero` are cleared.
+// The centroid box is reduced into kBoxCopies copies of (min keys, max keys)
+// (block b takes copy b mod kBoxCopies, so the blocks' atomics spread over
+// more words); min and max over order keys do not depend on order, so the
+// copies' reduction gives the same box.
+constexpr uint32_t kBoxCopies = 8;
+
+// keys6[6c+0..2] = min order keys (init 0xFFFFFFFF), keys6[6c+3..5] = max
+// (init 0) for each copy c, then `zero_count` words starting at `zero` are
+// cleared.
 static __global__ __launch_bounds__(256) void k_build_init(uint32_t* __restrict__ keys6,
                                                     uint32_t* __restrict__ zero,
                                                     size_t zero_count) {
   const size_t i0 = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
-  if (i0 < 6) keys6[i0] = (i0 < 3) ? 0xFFFFFFFFu : 0u;
+  if (i0 < 6 * kBoxCopies) keys6[i0] = (i0 % 6 < 3) ? 0xFFFFFFFFu : 0u;
   for (size_t i = i0; i < zero_count; i += (size_t)gridDim.x * blockDim.x) zero[i] = 0u;
 }
 
@@ -143,8 +150,9 @@ __global__ __launch_bounds__(256) void k_centroid_aabb(uint32_t n,
       const uint32_t u = s_red[w][d];
       v = (d < 3) ? (u < v ? u : v) : (u > v ? u : v);
     }
-    if (d < 3) atomicMin(keys6 + d, v);
-    else atomicMax(keys6 + d, v);
+    uint32_t* box = keys6 + (blockIdx.x % kBoxCopies) * 6;
+    if (d < 3) atomicMin(box + d, v);
+    else atomicMax(box + d, v);
   }
 }
 
@@ -184,7 +192,19 @@ __global__ __launch_bounds__(256) void k_morton_hist(uint32_t n, const float* __
   for (uint32_t e = threadIdx.x; e < sortk::kMaxPasses * sortk::kRadix; e += blockDim.x)
     (&s_h[0][0])[e] = 0;
   float T[16];
-  fit_transform(keys6, T);
+  {
+    uint32_t k6[6];
+#pragma unroll
+    for (int d = 0; d < 6; ++d) k6[d] = keys6[d];
+    for (uint32_t c = 1; c < kBoxCopies; ++c) {
+#pragma unroll
+      for (int d = 0; d < 6; ++d) {
+        const uint32_t v = keys6[c * 6 + d];
+        k6[d] = d < 3 ? (v < k6[d] ? v : k6[d]) : (v > k6[d] ? v : k6[d]);
+      }
+    }
+    fit_transform(k6, T);
+  }
   __syncthreads();
   const uint32_t stride = gridDim.x * blockDim.x;
   for (uint32_t i0 = blockIdx.x * blockDim.x + threadIdx.x; i0 < n;
@@ -289,7 +309,7 @@ inline int build(hipStream_t s, uint32_t n, const uint32_t* tri2vtx, uint32_t nu
   launch("k_build_init", k_build_init, dim3(std::min<uint32_t>(ceil_div(L.zero_words, 256), 2048)),
                      dim3(256), 0, s, L.keys6, L.zero_begin, L.zero_words);
   if ((rc = launch_status()) != DMSH_OK) return rc;
-  launch("k_centroid_aabb", k_centroid_aabb<TRI>, dim3(std::min<uint32_t>(ceil_div(n, 256), 1024)),
+  launch("k_centroid_aabb", k_centroid_aabb<TRI>, dim3(std::min<uint32_t>(ceil_div(n, 256), 2048)),
                      dim3(256), 0, s, n, tri2vtx, TRI ? num_vtx : n, xyz, L.cntr, L.tri_box,
                      L.keys6, err);
   if ((rc = launch_status()) != DMSH_OK) return rc;
