@@ -30,6 +30,7 @@ namespace bld {
 // more words); min and max over order keys do not depend on order, so the
 // copies' reduction gives the same box.
 constexpr uint32_t kBoxCopies = 8;
+static_assert(6 * kBoxCopies <= 64, "the build layout keeps 64 words for the box copies");
 
 // keys6[6c+0..2] = min order keys (init 0xFFFFFFFF), keys6[6c+3..5] = max
 // (init 0) for each copy c, then `zero_count` words starting at `zero` are
