@@ -179,7 +179,6 @@ def run_batched(num_mesh: int, reps: int):
     import del_msh_dlpack.Batched.torch as Batched
 
     dev = torch.device("cuda:0")
-    L = _lib.lib()
     t0 = time.perf_counter()
     b = primitives.batch_meshes(num_mesh, seed=3)
     gen_s = time.perf_counter() - t0
@@ -196,52 +195,10 @@ def run_batched(num_mesh: int, reps: int):
         raise RuntimeError(f"device error bits 0x{errs:x} in the batched run")
 
     # plain: the single-mesh entry points, one mesh per call, same data
-    stream = torch.cuda.current_stream(dev).cuda_stream
-    m2t, m2v, m2n, m2p = b["mesh2tri"], b["mesh2vtx"], p.mesh2node_h, b["mesh2pix"]
-    ntri = np.diff(m2t.astype(np.int64))
-    nmax = int(ntri.max())
-    widths = b["mesh2width"]
-    ws_b = _lib.workspace(L.dmsh_build_workspace_bytes(nmax, 30), dev)
-    pmax = int(np.diff(m2p.astype(np.int64)).max())
-    ws_c = _lib.workspace(L.dmsh_pix2tri_workspace_bytes(nmax, int(widths.max()),
-                                                         int(widths.max())), dev)
-    del pmax
-    nodes = torch.empty_like(p.bvhnodes)
-    aabb = torch.empty_like(p.bvhnode2aabb)
-    pix = torch.empty_like(p.pix2tri)
-    pt = torch.empty_like(p.pix2t)
-    tri_ptr, xyz_ptr = p.tri2vtx.data_ptr(), p.vtx2xyz.data_ptr()
-    tr_ptr = p.transforms.data_ptr()
-
-    def plain_build():
-        for m in range(num_mesh):
-            n = int(ntri[m])
-            if n == 0:
-                continue
-            _lib.check(L.dmsh_build_trimesh3(
-                stream, n, tri_ptr + int(m2t[m]) * 12, int(m2v[m + 1] - m2v[m]),
-                xyz_ptr + int(m2v[m]) * 12, 30, nodes.data_ptr() + int(m2n[m]) * 12,
-                aabb.data_ptr() + int(m2n[m]) * 24, ws_b.data_ptr(), ws_b.numel()), "build")
-
-    def plain_cast():
-        for m in range(num_mesh):
-            w = int(widths[m])
-            if w == 0:
-                continue
-            _lib.check(L.dmsh_pix2tri(
-                stream, pix.data_ptr() + int(m2p[m]) * 4, pt.data_ptr() + int(m2p[m]) * 4,
-                int(ntri[m]), tri_ptr + int(m2t[m]) * 12, int(m2v[m + 1] - m2v[m]),
-                xyz_ptr + int(m2v[m]) * 12, w, int(m2p[m + 1] - m2p[m]) // w,
-                tr_ptr + m * 64, nodes.data_ptr() + int(m2n[m]) * 12,
-                aabb.data_ptr() + int(m2n[m]) * 24, ws_c.data_ptr(), ws_c.numel()), "cast")
-
-    plain_build_ms = Batched._time_ms(plain_build, 1)
-    plain_cast_ms = Batched._time_ms(plain_cast, 1)
-    same = (torch.equal(nodes[:p.num_rows], p.bvhnodes[:p.num_rows])
-            and torch.equal(aabb[:p.num_rows].view(torch.int32),
-                            p.bvhnode2aabb[:p.num_rows].view(torch.int32))
-            and torch.equal(pix, p.pix2tri) and torch.equal(pt.view(torch.int32),
-                                                            p.pix2t.view(torch.int32)))
+    plain = Batched.PerMeshLoop(p)
+    plain_build_ms = Batched._time_ms(plain.build, 1)
+    plain_cast_ms = Batched._time_ms(plain.cast, 1)
+    same = plain.equals_pipeline()
     errs = _lib.device_errors()
     if errs:
         raise RuntimeError(f"device error bits 0x{errs:x} in the plain loop")

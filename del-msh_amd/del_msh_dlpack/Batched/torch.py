@@ -148,6 +148,84 @@ class Pipeline:
         return pix, t
 
 
+class PerMeshLoop:
+    """The same batch through the single-mesh entry points, one mesh per
+    call (dmsh_build_trimesh3 and dmsh_pix2tri, as a caller of the reference's
+    TriMesh3.make_bvhnodes_bvhnode2aabb / Pix2Tri.by_raycasting would run it),
+    into buffers of its own laid out like the pipeline's, so the two results
+    compare byte for byte."""
+
+    def __init__(self, p: Pipeline, morton_bits: int = 30):
+        self.p = p
+        self.morton_bits = morton_bits
+        dev = p.device
+        L = _lib.lib()
+        self.ntri = np.diff(p.mesh2tri_h.astype(np.int64))
+        nmax = int(self.ntri.max()) if self.num_mesh else 0
+        self.bvhnodes = torch.empty_like(p.bvhnodes)
+        self.bvhnode2aabb = torch.empty_like(p.bvhnode2aabb)
+        self.ws_build = _lib.workspace(L.dmsh_build_workspace_bytes(max(nmax, 1), morton_bits), dev)
+        if p.has_cast:
+            wmax = int(p.mesh2width_h.max())
+            hmax = max(int(p.mesh2pix_h[m + 1] - p.mesh2pix_h[m]) // max(int(p.mesh2width_h[m]), 1)
+                       for m in range(self.num_mesh))
+            self.pix2tri = torch.empty_like(p.pix2tri)
+            self.pix2t = torch.empty_like(p.pix2t) if p.pix2t is not None else None
+            self.ws_cast = _lib.workspace(
+                L.dmsh_pix2tri_workspace_bytes(max(nmax, 1), max(wmax, 1), max(hmax, 1)), dev)
+
+    @property
+    def num_mesh(self) -> int:
+        return self.p.num_mesh
+
+    def build(self):
+        p, L = self.p, _lib.lib()
+        stream = _lib.stream_of(p.device)
+        m2t, m2v, m2n = p.mesh2tri_h, p.mesh2vtx_h, p.mesh2node_h
+        tri_ptr, xyz_ptr = p.tri2vtx.data_ptr(), p.vtx2xyz.data_ptr()
+        for m in range(self.num_mesh):
+            n = int(self.ntri[m])
+            if n == 0:
+                continue
+            _lib.check(L.dmsh_build_trimesh3(
+                stream, n, tri_ptr + int(m2t[m]) * 12, int(m2v[m + 1] - m2v[m]),
+                xyz_ptr + int(m2v[m]) * 12, self.morton_bits,
+                self.bvhnodes.data_ptr() + int(m2n[m]) * 12,
+                self.bvhnode2aabb.data_ptr() + int(m2n[m]) * 24,
+                self.ws_build.data_ptr(), self.ws_build.numel()), "dmsh_build_trimesh3")
+
+    def cast(self):
+        """Casts over this loop's own trees (build() first)."""
+        p, L = self.p, _lib.lib()
+        stream = _lib.stream_of(p.device)
+        m2t, m2v, m2n, m2p = p.mesh2tri_h, p.mesh2vtx_h, p.mesh2node_h, p.mesh2pix_h
+        tri_ptr, xyz_ptr = p.tri2vtx.data_ptr(), p.vtx2xyz.data_ptr()
+        for m in range(self.num_mesh):
+            w = int(p.mesh2width_h[m])
+            if w == 0:
+                continue
+            _lib.check(L.dmsh_pix2tri(
+                stream, self.pix2tri.data_ptr() + int(m2p[m]) * 4,
+                self.pix2t.data_ptr() + int(m2p[m]) * 4 if self.pix2t is not None else None,
+                int(self.ntri[m]), tri_ptr + int(m2t[m]) * 12, int(m2v[m + 1] - m2v[m]),
+                xyz_ptr + int(m2v[m]) * 12, w, int(m2p[m + 1] - m2p[m]) // w,
+                p.transforms.data_ptr() + m * 64, self.bvhnodes.data_ptr() + int(m2n[m]) * 12,
+                self.bvhnode2aabb.data_ptr() + int(m2n[m]) * 24, self.ws_cast.data_ptr(),
+                self.ws_cast.numel()), "dmsh_pix2tri")
+
+    def equals_pipeline(self) -> bool:
+        """Every tree row, box bit, hit index and hit coefficient bit equal."""
+        p, r = self.p, self.p.num_rows
+        same = (torch.equal(self.bvhnodes[:r], p.bvhnodes[:r])
+                and torch.equal(self.bvhnode2aabb[:r].view(torch.int32),
+                                p.bvhnode2aabb[:r].view(torch.int32)))
+        if p.has_cast:
+            same = same and torch.equal(self.pix2tri, p.pix2tri)
+            if self.pix2t is not None:
+                same = same and torch.equal(self.pix2t.view(torch.int32), p.pix2t.view(torch.int32))
+        return bool(same)
+
+
 def make_bvhnodes_bvhnode2aabb(mesh2tri, mesh2vtx, tri2vtx: torch.Tensor,
                                vtx2xyz: torch.Tensor):
     """(mesh2node [host u32, M+1], bvhnodes [R,3] u32, bvhnode2aabb [R,6] f32)."""

@@ -23,6 +23,7 @@ def bwd(tri2vtx: torch.Tensor, vtx2xyz: torch.Tensor, transform_world2pix: torch
     device = tri2vtx.device
     assert_shape_dtype_device(tri2vtx, (num_tri, 3), torch.uint32, device)
     assert_shape_dtype_device(vtx2xyz, (num_vtx, 3), torch.float32, device)
+    assert_shape_dtype_device(transform_world2pix, (4, 4), torch.float32, device)
     assert_shape_dtype_device(pix2tri, (img_h, img_w), torch.uint32, device)
     assert_shape_dtype_device(pix2val, (img_h, img_w, num_vdim), torch.float32, device)
     assert_shape_dtype_device(dldw_pix2val, (img_h, img_w, num_vdim), torch.float32, device)

@@ -161,3 +161,38 @@ def test_batch_generated_sample(gpu, oracle):
     pick = set(rng.choice(300, size=12, replace=False).tolist())
     pick |= {int(np.argmax(sizes)), int(np.argmin(sizes)), 0, 299}
     _check_against_oracle(oracle, p, b, sorted(pick))
+
+
+def test_c3_full_batch_at_size(gpu, oracle):
+    """The batched workload at its stated size: 4096 meshes (seed 3,
+    256..262144 triangles log-uniform, 64^2..512^2 images), autotuned.
+    Composite keys mesh << 30 | code take 42 bits here (six digit passes).
+    Every mesh's tree and image from the batched launches equal the ones the
+    single-mesh entry points give it, one call per mesh; the largest, the
+    smallest and 16 random meshes equal the oracle's."""
+    b = primitives.batch_meshes(4096, seed=3)
+    sizes = np.diff(b["mesh2tri"].astype(np.int64))
+    assert len(sizes) == 4096 and sizes.max() > 200_000 and sizes.min() < 400
+    assert int(b["mesh2tri"][-1]) > 1.4e8
+    assert 30 + int(np.ceil(np.log2(4096))) == 42
+    p = _pipeline(b, gpu)
+    tuned = Batched.autotune(p, reps=1)
+    print("autotuned:", {k: v for k, v in tuned.items() if k != "trials"})
+    p.run()
+    torch.cuda.synchronize()
+    no_device_errors()
+    loop = Batched.PerMeshLoop(p)
+    loop.build()
+    loop.cast()
+    torch.cuda.synchronize()
+    assert loop.equals_pipeline()
+    no_device_errors()
+    b["tris"] = {}
+    b["vtxs"] = {}
+    rng = np.random.default_rng(3)
+    pick = set(rng.choice(4096, size=16, replace=False).tolist())
+    pick |= {int(np.argmax(sizes)), int(np.argmin(sizes))}
+    for m in pick:
+        b["tris"][m] = b["tri2vtx"][b["mesh2tri"][m]:b["mesh2tri"][m + 1]]
+        b["vtxs"][m] = b["vtx2xyz"][b["mesh2vtx"][m]:b["mesh2vtx"][m + 1]]
+    _check_against_oracle(oracle, p, b, sorted(pick))

@@ -277,3 +277,70 @@ def test_batch_meshes_equal_single_mesh_path(gpu, oracle, batch64):
             want = oracle.bwd_antialias(H(c), vtx, oracle.colmajor(H(w2p)), H(occ), dl, H(pix))
             assert np.abs(g1 - want).max() <= 2e-5 * np.abs(want).max()
     no_device_errors()
+
+
+def test_c4_rasterized_edge_gradient_function_autograd(gpu, oracle, scene):
+    """RasterizedEdgeGradientFunction through torch.autograd on the C4 scene:
+    the forward is the identity on pix2vin, the vertex gradient equals the
+    oracle's edge-gradient backward to float-atomics rounding, and dL/dpix2vin
+    passes through unchanged (reference RasterizedEdgeGradient/torch.py:214-234)."""
+    s = scene
+    v = s["v"].clone().requires_grad_(True)
+    pin = s["occ"][..., None].clone().requires_grad_(True)
+    w2p = G(s["w2p"], gpu)
+    out = RasterizedEdgeGradient.RasterizedEdgeGradientFunction.apply(s["t"], v, w2p, s["pix"], pin)
+    assert torch.equal(out, pin)
+    dldw = G(s["dldw"][..., None], gpu)
+    (dldw * out).sum().backward()
+    assert torch.equal(pin.grad, dldw)
+    want = oracle.edge_gradient_bwd(s["tri"], s["vtx"], oracle.colmajor(s["w2p"]), H(s["pix"]),
+                                    H(s["occ"])[..., None], s["dldw"][..., None])
+    got = H(v.grad)
+    scale = np.abs(want).max()
+    print(f"edge-gradient scale max|dL/dx| = {scale:.4g}, max |got - want| = "
+          f"{np.abs(got - want).max():.3g}")
+    assert scale > 0
+    # float atomics add each vertex's terms in another order than the serial loop
+    assert np.abs(got - want).max() <= 1e-5 * scale + 5e-6
+    no_device_errors()
+
+
+def _reference_example1(gpu):
+    """The scene of del-msh-dlpack/tests/test_differentiable_antialias.py:29-37:
+    torus (1.3, 0.4, 64, 32) rotated about x by 1.15 and moved up 0.6,
+    world2ndc = scale(0.5), 128 x 128."""
+    import del_msh_dlpack.Vtx2Xyz.torch as Vtx2Xyz
+
+    tri2vtx, vtx2xyz = TriMesh3.torus(1.3, 0.4, 64, 32)
+    transform = Mat44.from_translation(0.0, 0.6, 0.0) @ Mat44.from_x_rotation(1.15)
+    vtx2xyz = Vtx2Xyz.transform_affine(vtx2xyz, transform)
+    world2ndc = Mat44.from_scale(0.5, 0.5, 0.5)
+    return tri2vtx, vtx2xyz, world2ndc, (128, 128)
+
+
+def test_reference_match_cpu_gpu_microedge_bwd(gpu, oracle):
+    """The reference's own CPU<->GPU check of the edge-gradient backward
+    (del-msh-dlpack/tests/test_rasterized_edge_gradient.py:58-93): the same
+    scene, dL/docc ~ U[0, 1) from torch's generator seeded 0, and its bound
+    max |gpu - cpu| < 5e-6, here against the oracle."""
+    tri2vtx, vtx2xyz, world2ndc, img_shape = _reference_example1(gpu)
+    ndc2world = world2ndc.inverse()
+    world2pix = Mat44.from_transform_ndc2pix(img_shape) @ world2ndc
+    t, v = tri2vtx.to(gpu), vtx2xyz.to(gpu)
+    bvhnodes, aabb = TriMesh3.make_bvhnodes_bvhnode2aabb(t, v)
+    pix2tri = Pix2Tri.by_raycasting(t, v, bvhnodes, aabb, ndc2world.to(gpu), img_shape)
+    pix2occ = torch.where(pix2tri == U32MAX, 0.0, 1.0).to(torch.float32).unsqueeze(-1)
+    torch.random.manual_seed(0)
+    dldw_pix2occ = torch.rand_like(pix2occ.cpu())
+    got = RasterizedEdgeGradient.bwd(t, v, world2pix.to(gpu), pix2tri, pix2occ, dldw_pix2occ.to(gpu))
+    tri, vtx = tri2vtx.numpy(), vtx2xyz.numpy()
+    bn, ab = oracle.build_trimesh3(tri, vtx)
+    wpix, _, _ = oracle.pix2tri(tri, vtx, bn, ab, oracle.colmajor(ndc2world.numpy()), 128, 128)
+    assert np.array_equal(H(pix2tri), wpix)
+    want = oracle.edge_gradient_bwd(tri, vtx, oracle.colmajor(world2pix.numpy()), wpix,
+                                    H(pix2occ), dldw_pix2occ.numpy())
+    err = np.abs(H(got) - want).max()
+    print(f"example1: max|dL/dx| = {np.abs(want).max():.4g}, max |gpu - oracle| = {err:.3g}")
+    assert np.abs(want).max() > 0
+    assert err < 5.0e-6
+    no_device_errors()

@@ -323,6 +323,35 @@ def test_pix2tri(gpu, oracle, mesh, camera):
     no_device_errors()
 
 
+def test_c1_config_at_size(gpu, oracle):
+    """The plain-path workload at its stated size: UV sphere (r = 1, 64 x 128
+    segments, 16,128 triangles), 30-bit codes, LBVH build + refit on the
+    device, orthographic ray-cast of exactly 256 x 256 rays from z = +2 over
+    xy in [-1.2, 1.2]^2 looking down -z; hit triangle, hit coefficient and
+    depth bit-exact with the oracle, and the coverage of a unit disc in a
+    2.4-wide square: pi / 2.4^2 (to the tessellation and pixel error)."""
+    tri, vtx = primitives.sphere_yup(1.0, 64, 128)
+    assert tri.shape == (16128, 3) and vtx.shape == (8066, 3)
+    t_tri, t_vtx = G(tri, gpu), G(vtx, gpu)
+    bn, aabb = TriMesh3.make_bvhnodes_bvhnode2aabb(t_tri, t_vtx)
+    wbn, waabb = oracle.build_trimesh3(tri, vtx)
+    assert np.array_equal(H(bn), wbn)
+    assert bits_equal(H(aabb), waabb)
+    n2w = _ortho_c1()
+    W = H_ = 256
+    pix, t = Pix2Tri.by_raycasting_with_coeff(t_tri, t_vtx, bn, aabb, G(n2w, gpu), (W, H_))
+    T = oracle.colmajor(n2w)
+    wpix, wt, _ = oracle.pix2tri(tri, vtx, wbn, waabb, T, W, H_, num_threads=8)
+    assert np.array_equal(H(pix), wpix)
+    assert bits_equal(H(t), wt)
+    d = torch.zeros((H_, W), dtype=torch.float32, device=gpu)
+    Pix2Depth.update(d, pix, t_tri, t_vtx, G(n2w, gpu))
+    assert bits_equal(H(d), oracle.pix2depth(wpix, tri, vtx, T, W, H_))
+    frac = np.count_nonzero(wpix != U32MAX) / (W * H_)
+    assert abs(frac - math.pi / 2.4 ** 2) < 0.01
+    no_device_errors()
+
+
 def _pixel_grid(n):
     """z = 0 grid with its vertices under the pixel centres of an n x n
     orthographic image: rays through vertices and edges tie between the
@@ -497,18 +526,16 @@ def test_full_size_build_and_cast(gpu, oracle):
     pix, t = Pix2Tri.by_raycasting_with_coeff(t_tri, t_vtx, bn, aabb, G(n2w, gpu), (W, H_))
     pix, t = H(pix), H(t)
     no_device_errors()
-    # sampled bit-exact check against the oracle's ordered search
-    rng = np.random.default_rng(5)
-    sel = rng.integers(0, W * H_, 20_000)
+    # every one of the 4,194,304 rays against the oracle's ordered recursion
     T = n2w.T.reshape(-1)
-    for ip in sel[:4000]:
-        ih, iw = divmod(int(ip), W)
-        o, d = oracle.ray_for_pixel(iw, ih, W, H_, T)
-        r = oracle.first_intersection_ray(o, d, tri, vtx, wbn, waabb)
-        if r is None:
-            assert pix[ih, iw] == U32MAX
-        else:
-            assert pix[ih, iw] == r[1] and np.float32(t[ih, iw]) == np.float32(r[0])
+    wpix, wt, _ = oracle.pix2tri(tri, vtx, wbn, waabb, T, W, H_, num_threads=16)
+    assert np.array_equal(pix, wpix)
+    assert bits_equal(t, wt)
+    # depth of every hit, bit-exact with the oracle
+    d = torch.zeros((H_, W), dtype=torch.float32, device=gpu)
+    Pix2Depth.update(d, G(pix, gpu), t_tri, t_vtx, G(n2w, gpu))
+    wd = oracle.pix2depth(wpix, tri, vtx, T, W, H_)
+    assert bits_equal(H(d), wd)
     # coverage of a sphere seen from distance 3 with fovy 45: the silhouette of
     # radius r covers (tan(asin(r/3)) / tan(22.5 deg))^2 pi / 4 of the image;
     # the radial noise (sigma 0.01) keeps the silhouette radius within 1 +- 0.05

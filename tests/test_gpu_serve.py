@@ -60,3 +60,46 @@ def test_cast_requests_rejects_bad_offsets(gpu):
                                     512, pix.data_ptr(), None, ws.data_ptr(), ws.numel()), "cast")
     assert _lib.device_errors() & 0x1
     assert np.all(H(pix) == 7)
+
+
+def test_c5_served_at_config_sizes(gpu, oracle):
+    """The served workload at its stated sizes: 100 Poisson requests (seed 7)
+    at 20,000 req/s, mesh ids Zipf(1.1) over 10,000 ids, 1k..1M triangles
+    per mesh with radial noise, perspective viewports of 256^2..1024^2, a
+    1 GiB tree cache.  Every served image (hit triangle and hit coefficient)
+    equals what the request gets alone from the single-mesh entry points, and
+    four requests (the largest and the smallest mesh, two at random) equal the
+    oracle's build and ordered ray-cast."""
+    cat = Serve.MeshCatalog(tri_range=(1000, 1_000_000), seed=7)
+    reqs = Serve.request_stream(100, rate=20000.0, num_ids=10_000, side_range=(256, 1024), seed=7)
+    sides = [r.width for r in reqs]
+    ntri = [cat.num_tri(r.mesh_id) for r in reqs]
+    print(f"sides {min(sides)}..{max(sides)}, triangles {min(ntri)}..{max(ntri)}, "
+          f"{len(set(r.mesh_id for r in reqs))} meshes")
+    assert min(sides) >= 256 and max(sides) <= 1024 and max(sides) > 700
+    assert min(ntri) >= 500 and max(ntri) > 300_000
+    alone = copy.deepcopy(reqs)
+    srv = Serve.Server(cat, budget_bytes=1 << 30, device=gpu)
+    steps = srv.run(reqs, keep_results=True)
+    torch.cuda.synchronize()
+    assert steps >= 1
+    Serve.PlainServer(cat, budget_bytes=1 << 32, device=gpu).run(alone, keep_results=True)
+    for a, b in zip(reqs, alone):
+        pa, ta = a.result
+        pb, tb = b.result
+        assert torch.equal(pa, pb), a.rid
+        assert torch.equal(ta.view(torch.int32), tb.view(torch.int32)), a.rid
+    bits = _lib.device_errors()
+    assert bits == 0, hex(bits)
+    rng = np.random.default_rng(7)
+    pick = {int(np.argmax(ntri)), int(np.argmin(ntri))} | set(rng.choice(100, 2, replace=False).tolist())
+    for k in sorted(pick):
+        r = reqs[k]
+        tri, vtx = cat.get(r.mesh_id)
+        bn, ab = oracle.build_trimesh3(tri, vtx)
+        wpix, wt, _ = oracle.pix2tri(tri, vtx, bn, ab, oracle.colmajor(r.ndc2world), r.width,
+                                     r.height, num_threads=16)
+        pix, t = r.result
+        assert np.array_equal(H(pix), wpix), r.rid
+        assert np.array_equal(H(t).view(np.uint32), wt.view(np.uint32)), r.rid
+        assert (wpix != 0xFFFFFFFF).any(), r.rid

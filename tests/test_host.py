@@ -57,3 +57,22 @@ def test_timing_registry_without_launches():
     assert _lib.timing_query("k_pix2tri") == (0, 0.0, 0.0)
     _lib.timing_enable(False)
     assert _lib.timing_query("no_such_kernel") == (0, 0.0, 0.0)
+
+
+def test_edge_gradient_bwd_checks_transform_before_any_launch():
+    """A float64 or 3x3 world-to-pixel matrix is refused on the host (it would
+    otherwise be read as 16 float32 values)."""
+    import pytest
+
+    import del_msh_dlpack.RasterizedEdgeGradient.torch as RasterizedEdgeGradient
+
+    tri = torch.zeros((1, 3), dtype=torch.uint32)
+    vtx = torch.zeros((3, 3), dtype=torch.float32)
+    pix = torch.zeros((4, 4), dtype=torch.uint32)
+    val = torch.zeros((4, 4, 1), dtype=torch.float32)
+    for bad in (torch.eye(4, dtype=torch.float64), torch.eye(3, dtype=torch.float32)):
+        with pytest.raises(AssertionError):
+            RasterizedEdgeGradient.bwd(tri, vtx, bad, pix, val, val.clone())
+    # a well-formed call on host tensors gets as far as the device check
+    with pytest.raises(NotImplementedError):
+        RasterizedEdgeGradient.bwd(tri, vtx, torch.eye(4), pix, val, val.clone())
