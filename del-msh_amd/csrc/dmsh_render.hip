@@ -6,6 +6,7 @@
 #include "edgegrad.hpp"
 #include "raycast.hpp"
 #include "sort.hpp"
+#include "topo.hpp"
 
 using namespace dmsh;
 
@@ -264,6 +265,101 @@ int dmsh_rasterized_edge_gradient_bwd(void* stream, const uint32_t* tri2vtx, uin
   return launch_status();
 }
 
+
+namespace {
+// Workspace of the ordered edge-gradient backward for P pixel pairs:
+// offsets [P + 1], scan tile sums, then per record slot (3 per pair) the
+// vertex id, the record index and the term, and the sort's workspace.
+struct EgLayout {
+  uint32_t* off;
+  uint32_t* sums;
+  uint32_t* rec_vtx;
+  uint32_t* rec_idx;
+  float* rec_term;
+  void* sort_ws;
+  size_t sort_bytes;
+};
+
+uint32_t eg_num_pairs(uint32_t img_w, uint32_t img_h) {
+  return img_w * (img_h - 1u) + (img_w - 1u) * img_h;
+}
+
+uint32_t eg_key_bits(uint32_t num_vtx) {
+  uint32_t b = 1;
+  while (b < 32 && (num_vtx - 1u) >> b) ++b;
+  return b;
+}
+
+bool eg_carve(Arena& a, uint32_t num_pair, uint32_t num_vtx, EgLayout& L) {
+  const uint32_t cap = 3u * num_pair;
+  L.off = a.take<uint32_t>((size_t)num_pair + 1);
+  L.sums = a.take<uint32_t>(topo::scan_tiles(num_pair));
+  L.rec_vtx = a.take<uint32_t>(cap);
+  L.rec_idx = a.take<uint32_t>(cap);
+  L.rec_term = a.take<float>((size_t)cap * 3);
+  L.sort_bytes = sortk::workspace_bytes<uint32_t>(cap, eg_key_bits(num_vtx));
+  L.sort_ws = a.take<char>(L.sort_bytes);
+  return a.ok();
+}
+
+bool eg_size_ok(uint32_t img_w, uint32_t img_h) {
+  const uint64_t num_pix = (uint64_t)img_w * img_h;
+  // 3 record slots per pair must stay below the sort's 2^30 elements
+  return num_pix < (1ull << 27);
+}
+}  // namespace
+
+size_t dmsh_rasterized_edge_gradient_bwd_workspace_bytes(uint32_t img_w, uint32_t img_h,
+                                                         uint32_t num_vtx) {
+  if (img_w == 0 || img_h == 0 || num_vtx == 0 || !eg_size_ok(img_w, img_h)) return 256;
+  Arena a{nullptr, 0, 0};
+  EgLayout L;
+  eg_carve(a, eg_num_pairs(img_w, img_h), num_vtx, L);
+  return a.off + 256;
+}
+
+int dmsh_rasterized_edge_gradient_bwd_ordered(
+    void* stream, const uint32_t* tri2vtx, uint32_t num_tri, const float* vtx2xyz,
+    uint32_t num_vtx, float* dldw_vtx2xyz, const float* transform_world2pix, uint32_t img_w,
+    uint32_t img_h, const uint32_t* pix2tri, uint32_t num_vdim, const float* pix2val,
+    const float* dldw_pix2val, void* ws, size_t ws_bytes) {
+  const uint64_t num_pix = (uint64_t)img_w * img_h;
+  if (num_pix == 0 || num_vtx == 0) return DMSH_OK;
+  if (!eg_size_ok(img_w, img_h) || num_vdim == 0) return DMSH_ERR_INVALID;
+  if (!tri2vtx || !vtx2xyz || !dldw_vtx2xyz || !transform_world2pix || !pix2tri || !pix2val ||
+      !dldw_pix2val)
+    return DMSH_ERR_INVALID;
+  const uint32_t num_pair = eg_num_pairs(img_w, img_h);
+  if (num_pair == 0) return DMSH_OK;
+  if (!ws || ws_bytes < dmsh_rasterized_edge_gradient_bwd_workspace_bytes(img_w, img_h, num_vtx))
+    return DMSH_ERR_WORKSPACE;
+  Arena a{static_cast<char*>(ws), ws_bytes, 0};
+  EgLayout L;
+  if (!eg_carve(a, num_pair, num_vtx, L)) return DMSH_ERR_WORKSPACE;
+  uint32_t* err = device_error_word();
+  if (!err) return DMSH_ERR_HIP;
+  hipStream_t s = as_stream(stream);
+  const uint32_t cap = 3u * num_pair;
+  const uint32_t blocks = std::min<uint32_t>(ceil_div(num_pair, 256), 16384);
+  launch("k_eg_count", eg::k_eg_count, dim3(blocks), dim3(256), 0, s, tri2vtx, num_tri, vtx2xyz,
+         num_vtx, transform_world2pix, img_w, img_h, pix2tri, num_vdim, pix2val, dldw_pix2val,
+         num_pair, L.off, err);
+  int rc = launch_status();
+  if (rc != DMSH_OK) return rc;
+  // in place: off[s] = records before pair s, off[num_pair] = all records
+  if ((rc = topo::launch_exclusive_scan(s, num_pair, L.off, L.off, L.sums)) != DMSH_OK) return rc;
+  launch("k_eg_records", eg::k_eg_records, dim3(blocks), dim3(256), 0, s, tri2vtx, num_tri,
+         vtx2xyz, num_vtx, transform_world2pix, img_w, img_h, pix2tri, num_vdim, pix2val,
+         dldw_pix2val, num_pair, L.off, L.rec_vtx, L.rec_idx, L.rec_term, err);
+  if ((rc = launch_status()) != DMSH_OK) return rc;
+  rc = sortk::sort_pairs_device_count<uint32_t>(s, cap, L.off + num_pair, L.rec_vtx, L.rec_idx,
+                                                eg_key_bits(num_vtx), L.sort_ws, L.sort_bytes);
+  if (rc != DMSH_OK) return rc;
+  launch("k_eg_sum", eg::k_eg_sum, dim3(std::min<uint32_t>(ceil_div(cap, 256), 16384)),
+         dim3(256), 0, s, (const uint32_t*)(L.off + num_pair), (const uint32_t*)L.rec_vtx,
+         (const uint32_t*)L.rec_idx, (const float*)L.rec_term, num_vtx, cap, dldw_vtx2xyz, err);
+  return launch_status();
+}
 
 int dmsh_edge_gradient_and_type(void* stream, const uint32_t* tri2vtx, uint32_t num_tri,
                                 const float* vtx2xyz, uint32_t num_vtx,

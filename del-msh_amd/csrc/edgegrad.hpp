@@ -146,6 +146,207 @@ static __global__ __launch_bounds__(256) void k_edge_grad_bwd(
 }
 
 // ---------------------------------------------------------------------------
+// Ordered variant: every vertex sums its terms in the serial loop's order, so
+// the result equals del-msh-cpu's bit for bit (rasterized_edge_gradient.rs:
+// 449-602 adds term after term into dldw_vtx2xyz; binary32 addition is not
+// associative, so only that order gives those bits).
+//   pair order s: first the vertical pairs (iw, ih0)-(iw, ih0 + 1) with iw
+//   outer and ih0 inner (pixel-y Jacobian), s = iw (H - 1) + ih0; then the
+//   horizontal pairs (iw0, ih)-(iw0 + 1, ih) with iw0 outer and ih inner
+//   (pixel-x Jacobian), s = W (H - 1) + iw0 H + ih.
+//   k_eg_count: terms per pair (0..3, one per vertex slot whose projection
+//   gradient exists), in pair order;  exclusive scan -> record offsets and
+//   the record count on the device;  k_eg_records: record r = (vertex id,
+//   r, the term's three floats), records in pair order and, inside a pair,
+//   in vertex-slot order (the loop's order);  a stable radix sort by vertex
+//   id keeps that order inside each vertex;  k_eg_sum: one thread per vertex
+//   adds its run to dldw_vtx2xyz left to right.
+// ---------------------------------------------------------------------------
+struct PairRef {
+  size_t ip0, ip1;
+  float pc0[2], pc1[2];
+  int axis;
+};
+
+__device__ __forceinline__ PairRef pair_of(uint32_t s, uint32_t img_w, uint32_t img_h) {
+  PairRef r;
+  const uint32_t num_vert = img_w * (img_h - 1u);
+  if (s < num_vert) {
+    const uint32_t iw = s / (img_h - 1u), ih0 = s - iw * (img_h - 1u);
+    r.ip0 = (size_t)ih0 * img_w + iw;
+    r.ip1 = r.ip0 + img_w;
+    r.pc0[0] = (float)iw + 0.5f;
+    r.pc0[1] = (float)ih0 + 0.5f;
+    r.pc1[0] = (float)iw + 0.5f;
+    r.pc1[1] = (float)(ih0 + 1u) + 0.5f;
+    r.axis = 1;
+  } else {
+    const uint32_t t = s - num_vert;
+    const uint32_t iw0 = t / img_h, ih = t - iw0 * img_h;
+    r.ip0 = (size_t)ih * img_w + iw0;
+    r.ip1 = r.ip0 + 1;
+    r.pc0[0] = (float)iw0 + 0.5f;
+    r.pc0[1] = (float)ih + 0.5f;
+    r.pc1[0] = (float)(iw0 + 1u) + 0.5f;
+    r.pc1[1] = (float)ih + 0.5f;
+    r.axis = 0;
+  }
+  return r;
+}
+
+// The pair's terms, as edge_pair computes them: returns the mask of vertex
+// slots of the receiving triangle `itri` that get a term; term[k] holds slot
+// k's three floats.
+__device__ __forceinline__ uint32_t pair_terms(const Ctx& c, const uint32_t* __restrict__ pix2tri,
+                                               uint32_t nv, const float* __restrict__ pix2val,
+                                               const float* __restrict__ dpix, const PairRef& r,
+                                               uint32_t& itri, float term[3][3]) {
+  const uint32_t t0 = pix2tri[r.ip0], t1 = pix2tri[r.ip1];
+  if (t0 == t1) return 0u;
+  float b[3];
+  const int h0 = bary(c, r.pc0, t1, b);
+  const bool in0 = inside(h0, b);
+  const int h1 = bary(c, r.pc1, t0, b);
+  const bool in1 = inside(h1, b);
+  if (h0 < 0 || h1 < 0) {
+    raise_error(c.err, DMSH_DEVERR_INDEX);
+    return 0u;
+  }
+  if (in0 == in1) return 0u;  // neither: shared edge; both: intersection
+  float dldpa = 0.f;
+  for (uint32_t k = 0; k < nv; ++k) {
+    const float v0 = pix2val[r.ip0 * nv + k], v1 = pix2val[r.ip1 * nv + k];
+    const float g0 = dpix[r.ip0 * nv + k], g1 = dpix[r.ip1 * nv + k];
+    dldpa += (g0 + g1) * 0.5f * (v0 - v1);
+  }
+  itri = in1 ? t1 : t0;
+  if (bary(c, in1 ? r.pc1 : r.pc0, itri, b) != 1) return 0u;
+  const uint32_t* tv = c.tri2vtx + (size_t)itri * 3;
+  uint32_t mask = 0u;
+#pragma unroll
+  for (int inode = 0; inode < 3; ++inode) {
+    float d[3];
+    if (!proj_grad(c.m, c.xyz + (size_t)tv[inode] * 3, r.axis, d)) continue;
+    term[inode][0] = b[inode] * d[0] * dldpa;
+    term[inode][1] = b[inode] * d[1] * dldpa;
+    term[inode][2] = b[inode] * d[2] * dldpa;
+    mask |= 1u << inode;
+  }
+  return mask;
+}
+
+__device__ __forceinline__ Ctx make_ctx(const uint32_t* tri2vtx, uint32_t num_tri,
+                                        const float* xyz, uint32_t num_vtx, const float* T,
+                                        uint32_t* err) {
+  Ctx c;
+  c.tri2vtx = tri2vtx;
+  c.xyz = xyz;
+  c.num_tri = num_tri;
+  c.num_vtx = num_vtx;
+  c.err = err;
+#pragma unroll
+  for (int k = 0; k < 16; ++k) c.m[k] = T[k];
+  return c;
+}
+
+static __global__ __launch_bounds__(256) void k_eg_count(
+    const uint32_t* __restrict__ tri2vtx, uint32_t num_tri, const float* __restrict__ xyz,
+    uint32_t num_vtx, const float* __restrict__ T, uint32_t img_w, uint32_t img_h,
+    const uint32_t* __restrict__ pix2tri, uint32_t nv, const float* __restrict__ pix2val,
+    const float* __restrict__ dpix, uint32_t num_pair, uint32_t* __restrict__ cnt,
+    uint32_t* err) {
+  const Ctx c = make_ctx(tri2vtx, num_tri, xyz, num_vtx, T, err);
+  for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < num_pair;
+       s += gridDim.x * blockDim.x) {
+    uint32_t itri;
+    float term[3][3];
+    const uint32_t mask = pair_terms(c, pix2tri, nv, pix2val, dpix, pair_of(s, img_w, img_h),
+                                     itri, term);
+    cnt[s] = (uint32_t)__builtin_popcount(mask);
+  }
+}
+
+static __global__ __launch_bounds__(256) void k_eg_records(
+    const uint32_t* __restrict__ tri2vtx, uint32_t num_tri, const float* __restrict__ xyz,
+    uint32_t num_vtx, const float* __restrict__ T, uint32_t img_w, uint32_t img_h,
+    const uint32_t* __restrict__ pix2tri, uint32_t nv, const float* __restrict__ pix2val,
+    const float* __restrict__ dpix, uint32_t num_pair, const uint32_t* __restrict__ off,
+    uint32_t* __restrict__ rec_vtx, uint32_t* __restrict__ rec_idx,
+    float* __restrict__ rec_term, uint32_t* err) {
+  // off [num_pair + 1]: the exclusive scan of k_eg_count's counts
+  const Ctx c = make_ctx(tri2vtx, num_tri, xyz, num_vtx, T, err);
+  for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < num_pair;
+       s += gridDim.x * blockDim.x) {
+    if (off[s + 1] == off[s]) continue;
+    uint32_t itri;
+    float term[3][3];
+    const uint32_t mask = pair_terms(c, pix2tri, nv, pix2val, dpix, pair_of(s, img_w, img_h),
+                                     itri, term);
+    uint32_t r = off[s];
+#pragma unroll
+    for (int inode = 0; inode < 3; ++inode) {
+      if (!((mask >> inode) & 1u)) continue;
+      rec_vtx[r] = tri2vtx[(size_t)itri * 3 + inode];
+      rec_idx[r] = r;
+      rec_term[(size_t)r * 3 + 0] = term[inode][0];
+      rec_term[(size_t)r * 3 + 1] = term[inode][1];
+      rec_term[(size_t)r * 3 + 2] = term[inode][2];
+      ++r;
+    }
+  }
+}
+
+// One thread per run of equal vertex ids in the sorted records; the run is in
+// pair order, so the additions are the serial loop's.
+static __global__ __launch_bounds__(256) void k_eg_sum(const uint32_t* __restrict__ num_rec,
+                                                       const uint32_t* __restrict__ vtx_sorted,
+                                                       const uint32_t* __restrict__ idx_sorted,
+                                                       const float* __restrict__ rec_term,
+                                                       uint32_t num_vtx, uint32_t cap,
+                                                       float* __restrict__ dldw, uint32_t* err) {
+  const uint32_t n = *num_rec < cap ? *num_rec : cap;
+  for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
+    const uint32_t v = vtx_sorted[j];
+    if (j > 0 && vtx_sorted[j - 1] == v) continue;
+    if (v >= num_vtx) {
+      raise_error(err, DMSH_DEVERR_INDEX);
+      continue;
+    }
+    float s0 = dldw[(size_t)v * 3 + 0], s1 = dldw[(size_t)v * 3 + 1], s2 = dldw[(size_t)v * 3 + 2];
+    uint32_t q = j;
+    // four records per round: their loads are issued together, the sums stay in order
+    while (q + 4 <= n && vtx_sorted[q + 3] == v) {
+      uint32_t id[4];
+      float t[4][3];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) id[u] = idx_sorted[q + u];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        t[u][0] = rec_term[(size_t)id[u] * 3 + 0];
+        t[u][1] = rec_term[(size_t)id[u] * 3 + 1];
+        t[u][2] = rec_term[(size_t)id[u] * 3 + 2];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        s0 += t[u][0];
+        s1 += t[u][1];
+        s2 += t[u][2];
+      }
+      q += 4;
+    }
+    for (; q < n && vtx_sorted[q] == v; ++q) {
+      const uint32_t id = idx_sorted[q];
+      s0 += rec_term[(size_t)id * 3 + 0];
+      s1 += rec_term[(size_t)id * 3 + 1];
+      s2 += rec_term[(size_t)id * 3 + 2];
+    }
+    dldw[(size_t)v * 3 + 0] = s0;
+    dldw[(size_t)v * 3 + 1] = s1;
+    dldw[(size_t)v * 3 + 2] = s2;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Smoothed variant on the staggered grid of pixel edges
 // (rasterized_edge_gradient.rs:77-432): hedge (ih, iw) lies between pixels
 // (iw, ih) and (iw, ih + 1), [H-1][W]; vedge (ih, iw) between (iw, ih) and

@@ -113,7 +113,8 @@ static __global__ __launch_bounds__(256) void k_fill_u32(uint32_t* __restrict__ 
 
 template <typename KeyT>
 __global__ __launch_bounds__(256) void k_copy_keys(const KeyT* __restrict__ a, KeyT* __restrict__ b,
-                                                   uint32_t n) {
+                                                   uint32_t n, const uint32_t* __restrict__ n_dev) {
+  if (n_dev) n = *n_dev;
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
     b[i] = a[i];
 }
@@ -139,8 +140,10 @@ __device__ __forceinline__ void hist_flush(uint32_t (*s_h)[kRadix], uint32_t fir
 
 template <typename KeyT>
 __global__ __launch_bounds__(256) void k_hist_all(const KeyT* __restrict__ keys, uint32_t n,
+                                                  const uint32_t* __restrict__ n_dev,
                                                   uint32_t first, uint32_t num_passes,
                                                   uint32_t key_bits, uint32_t* __restrict__ hist) {
+  if (n_dev) n = *n_dev;
   __shared__ uint32_t s_h[kMaxPasses][kRadix];
   for (uint32_t e = threadIdx.x; e < kMaxPasses * kRadix; e += blockDim.x) (&s_h[0][0])[e] = 0;
   __syncthreads();
@@ -174,15 +177,19 @@ __global__ __launch_bounds__(256) void k_hist_low(const uint32_t* __restrict__ g
 }
 
 // One digit pass.  IOTA: values are the element indices (first pass).  With
-// `gate`, the pass runs only if *gate is nonzero.
+// `gate`, the pass runs only if *gate is nonzero.  With `n_dev`, the element
+// count is read from the device (the grid is sized for a capacity); blocks
+// whose ticket lies past the last tile exit before publishing anything, and
+// only earlier tiles are ever looked back at, so they wait on nobody.
 template <typename KeyT, int ITEMS, bool IOTA>
 __global__ __launch_bounds__(256) void k_onesweep(
     const KeyT* __restrict__ kin, const uint32_t* __restrict__ vin, KeyT* __restrict__ kout,
-    uint32_t* __restrict__ vout, uint32_t n, uint32_t shift, uint32_t dbits,
-    const uint32_t* __restrict__ hist, uint32_t* __restrict__ status,
+    uint32_t* __restrict__ vout, uint32_t n, const uint32_t* __restrict__ n_dev, uint32_t shift,
+    uint32_t dbits, const uint32_t* __restrict__ hist, uint32_t* __restrict__ status,
     uint32_t* __restrict__ ticket, const uint32_t* __restrict__ gate,
     uint32_t* __restrict__ err) {
   if (gate && *gate == 0u) return;
+  if (n_dev) n = *n_dev;
   constexpr int kTile = kThreads * ITEMS;
   __shared__ uint32_t s_wcnt[4][kRadix];
   __shared__ uint32_t s_excl[kRadix];
@@ -199,6 +206,7 @@ __global__ __launch_bounds__(256) void k_onesweep(
   __syncthreads();
   const uint32_t tile = s_tile;
   const uint32_t tile_base = tile * (uint32_t)kTile;
+  if (tile_base >= n) return;  // past the data (a capacity-sized grid)
   const uint32_t wave_base = tile_base + wave * 64u * ITEMS;
 
   KeyT key[ITEMS];
@@ -486,7 +494,7 @@ template <typename KeyT>
 inline int launch_pass(hipStream_t s, uint32_t n, uint32_t key_bits, const Layout& L,
                        uint32_t digit, uint32_t slot, bool iota, const KeyT* sk,
                        const uint32_t* sv, KeyT* dk, uint32_t* dv, const uint32_t* gate,
-                       uint32_t* err) {
+                       uint32_t* err, const uint32_t* n_dev = nullptr) {
   constexpr int ITEMS = Tune<KeyT>::kItems;
   const uint32_t tiles = num_tiles<KeyT>(n);
   const uint32_t shift = digit * kRadixBits;
@@ -495,10 +503,10 @@ inline int launch_pass(hipStream_t s, uint32_t n, uint32_t key_bits, const Layou
   const char* label = gate ? "k_onesweep_gated" : "k_onesweep";
   if (iota) {
     launch(label, (k_onesweep<KeyT, ITEMS, true>), dim3(tiles), dim3(kThreads), 0, s, sk, sv, dk,
-           dv, n, shift, dbits, L.hist + digit * kRadix, st, L.tickets + slot, gate, err);
+           dv, n, n_dev, shift, dbits, L.hist + digit * kRadix, st, L.tickets + slot, gate, err);
   } else {
     launch(label, (k_onesweep<KeyT, ITEMS, false>), dim3(tiles), dim3(kThreads), 0, s, sk, sv, dk,
-           dv, n, shift, dbits, L.hist + digit * kRadix, st, L.tickets + slot, gate, err);
+           dv, n, n_dev, shift, dbits, L.hist + digit * kRadix, st, L.tickets + slot, gate, err);
   }
   return launch_status();
 }
@@ -514,29 +522,34 @@ inline int launch_pass(hipStream_t s, uint32_t n, uint32_t key_bits, const Layou
 //   alt -> dst, then the gated full passes alt -> dst -> ... (and a gated copy
 //   back to dst when their number is even).  `first_src` must not be the
 //   alternate key buffer.
+//   With `n_dev` (plain path only) the element count is read from the device
+//   and n is the capacity the grids are sized for; with `first_vals` the
+//   first pass carries those values instead of the element indices.
 template <typename KeyT>
 inline int run_passes(hipStream_t s, uint32_t n, uint32_t key_bits, const Layout& L,
                       const KeyT* first_src, KeyT* dst_keys, uint32_t* dst_vals, uint32_t* err,
-                      bool finish) {
+                      bool finish, const uint32_t* n_dev = nullptr,
+                      const uint32_t* first_vals = nullptr) {
   const uint32_t P = num_passes(key_bits);
   KeyT* alt_k = static_cast<KeyT*>(L.alt_keys);
   uint32_t* alt_v = L.alt_vals;
   int rc;
   if (!finish || !uses_finish(P)) {
     const KeyT* src_k = first_src;
-    const uint32_t* src_v = nullptr;
+    const uint32_t* src_v = first_vals;
     for (uint32_t p = 0; p < P; ++p) {
       const bool to_user = ((P - 1 - p) % 2) == 0;
       KeyT* dk = to_user ? dst_keys : alt_k;
       uint32_t* dv = to_user ? dst_vals : alt_v;
-      rc = launch_pass<KeyT>(s, n, key_bits, L, p, p, p == 0, src_k, src_v, dk, dv, nullptr, err);
+      rc = launch_pass<KeyT>(s, n, key_bits, L, p, p, p == 0 && first_vals == nullptr, src_k,
+                             src_v, dk, dv, nullptr, err, n_dev);
       if (rc != DMSH_OK) return rc;
       src_k = dk;
       src_v = dv;
     }
     return DMSH_OK;
   }
-  if (first_src == alt_k) return DMSH_ERR_INVALID;
+  if (first_src == alt_k || n_dev || first_vals) return DMSH_ERR_INVALID;
   const uint32_t d0 = P - kTopPasses;
   if ((rc = launch_pass<KeyT>(s, n, key_bits, L, d0, 0, true, first_src, nullptr, alt_k, alt_v,
                               nullptr, err)) != DMSH_OK)
@@ -593,18 +606,63 @@ inline int argsort(hipStream_t s, uint32_t n, KeyT* keys_inout, uint32_t* idx_ou
   const uint32_t P = num_passes(key_bits);
   const uint32_t hist_blocks = std::min<uint32_t>(num_tiles<KeyT>(n), 2048);
   launch("k_hist_all", k_hist_all<KeyT>, dim3(hist_blocks), dim3(256), 0, s, keys_inout, n,
-         first_hist_pass(P), P, key_bits, L.hist);
+         (const uint32_t*)nullptr, first_hist_pass(P), P, key_bits, L.hist);
   if ((rc = launch_status()) != DMSH_OK) return rc;
   const KeyT* first_src = keys_inout;
   if (P % 2 == 1 && !uses_finish(P)) {
     // An odd pass count would end in the scratch buffer: start from a copy.
     KeyT* alt_k = static_cast<KeyT*>(L.alt_keys);
     launch("k_copy_keys", k_copy_keys<KeyT>, dim3(std::min<uint32_t>(ceil_div(n, 256), 4096)),
-                       dim3(256), 0, s, keys_inout, alt_k, n);
+                       dim3(256), 0, s, keys_inout, alt_k, n, (const uint32_t*)nullptr);
     if ((rc = launch_status()) != DMSH_OK) return rc;
     first_src = alt_k;
   }
   return run_passes<KeyT>(s, n, key_bits, L, first_src, keys_inout, idx_out, err, true);
+}
+
+// Stable sort of (keys, values) pairs whose count lives on the device
+// (*n_dev <= n_cap): keys_inout and vals_inout hold the sorted pairs on
+// return.  Keys of at most four digits (the plain passes); the grids and the
+// workspace (workspace_bytes(n_cap, key_bits)) are sized for n_cap.
+template <typename KeyT>
+inline int sort_pairs_device_count(hipStream_t s, uint32_t n_cap, const uint32_t* n_dev,
+                                   KeyT* keys_inout, uint32_t* vals_inout, uint32_t key_bits,
+                                   void* ws, size_t ws_bytes) {
+  if (n_cap == 0) return DMSH_OK;
+  if (!keys_inout || !vals_inout || !n_dev) return DMSH_ERR_INVALID;
+  if (n_cap >= (1u << 30)) return DMSH_ERR_INVALID;
+  if (key_bits == 0 || key_bits > 8 * sizeof(KeyT)) return DMSH_ERR_INVALID;
+  const uint32_t P = num_passes(key_bits);
+  if (uses_finish(P)) return DMSH_ERR_INVALID;
+  if (!ws || ws_bytes < workspace_bytes<KeyT>(n_cap, key_bits)) return DMSH_ERR_WORKSPACE;
+  Arena a{static_cast<char*>(ws), ws_bytes, 0};
+  Layout L;
+  if (!carve<KeyT>(a, n_cap, key_bits, L)) return DMSH_ERR_WORKSPACE;
+  uint32_t* err = device_error_word();
+  if (!err) return DMSH_ERR_HIP;
+  int rc = launch_fill(s, L.hist, L.zero_words, 0u);
+  if (rc != DMSH_OK) return rc;
+  const uint32_t hist_blocks = std::min<uint32_t>(num_tiles<KeyT>(n_cap), 2048);
+  launch("k_hist_all", k_hist_all<KeyT>, dim3(hist_blocks), dim3(256), 0, s, keys_inout, n_cap,
+         n_dev, 0u, P, key_bits, L.hist);
+  if ((rc = launch_status()) != DMSH_OK) return rc;
+  const KeyT* first_src = keys_inout;
+  const uint32_t* first_vals = vals_inout;
+  if (P % 2 == 1) {
+    // an odd pass count would end in the scratch buffers: start from copies
+    KeyT* alt_k = static_cast<KeyT*>(L.alt_keys);
+    launch("k_copy_keys", k_copy_keys<KeyT>, dim3(std::min<uint32_t>(ceil_div(n_cap, 256), 4096)),
+           dim3(256), 0, s, (const KeyT*)keys_inout, alt_k, n_cap, n_dev);
+    if ((rc = launch_status()) != DMSH_OK) return rc;
+    launch("k_copy_keys", k_copy_keys<uint32_t>,
+           dim3(std::min<uint32_t>(ceil_div(n_cap, 256), 4096)), dim3(256), 0, s,
+           (const uint32_t*)vals_inout, L.alt_vals, n_cap, n_dev);
+    if ((rc = launch_status()) != DMSH_OK) return rc;
+    first_src = alt_k;
+    first_vals = L.alt_vals;
+  }
+  return run_passes<KeyT>(s, n_cap, key_bits, L, first_src, keys_inout, vals_inout, err, false,
+                          n_dev, first_vals);
 }
 
 }  // namespace sortk

@@ -6,23 +6,32 @@ from ..util_torch import as_tensor
 
 
 def bwd(tri2vtx, vtx2xyz, dldw_vtx2xyz, transform_world2pix, pix2tri, pix2val, dldw_pix2val,
-        stream_ptr=0):
-    """Accumulates into dldw_vtx2xyz; transform_world2pix is 16 floats column-major."""
+        stream_ptr=0, ordered=True):
+    """Accumulates into dldw_vtx2xyz; transform_world2pix is 16 floats column-major.
+
+    ordered=True (default): every vertex adds its terms in del-msh-cpu's loop
+    order, so the result is that loop's, bit for bit, on every run.
+    ordered=False: one launch with float atomics (faster, order not fixed)."""
     tri2vtx, vtx2xyz, dldw_vtx2xyz, transform_world2pix, pix2tri, pix2val, dldw_pix2val = map(
         as_tensor,
         (tri2vtx, vtx2xyz, dldw_vtx2xyz, transform_world2pix, pix2tri, pix2val, dldw_pix2val))
     dev = _lib.require_gpu(tri2vtx, vtx2xyz, dldw_vtx2xyz, transform_world2pix, pix2tri, pix2val,
                            dldw_pix2val)
+    _lib.check_transform(transform_world2pix, "transform_world2pix")
     img_h, img_w = pix2tri.shape
     num_vdim = pix2val.shape[2]
     stream = stream_ptr or _lib.stream_of(dev)
-    _lib.check(
-        _lib.lib().dmsh_rasterized_edge_gradient_bwd(
-            stream, _lib.ptr(tri2vtx), tri2vtx.shape[0], _lib.ptr(vtx2xyz), vtx2xyz.shape[0],
+    L = _lib.lib()
+    args = (stream, _lib.ptr(tri2vtx), tri2vtx.shape[0], _lib.ptr(vtx2xyz), vtx2xyz.shape[0],
             _lib.ptr(dldw_vtx2xyz), _lib.ptr(transform_world2pix), img_w, img_h,
-            _lib.ptr(pix2tri), num_vdim, _lib.ptr(pix2val), _lib.ptr(dldw_pix2val)),
-        "dmsh_rasterized_edge_gradient_bwd",
-    )
+            _lib.ptr(pix2tri), num_vdim, _lib.ptr(pix2val), _lib.ptr(dldw_pix2val))
+    if not ordered:
+        _lib.check(L.dmsh_rasterized_edge_gradient_bwd(*args), "dmsh_rasterized_edge_gradient_bwd")
+        return
+    ws = _lib.workspace(
+        L.dmsh_rasterized_edge_gradient_bwd_workspace_bytes(img_w, img_h, vtx2xyz.shape[0]), dev)
+    _lib.check(L.dmsh_rasterized_edge_gradient_bwd_ordered(*args, _lib.ptr(ws), ws.numel()),
+               "dmsh_rasterized_edge_gradient_bwd_ordered")
 
 
 def edge_gradient_and_type(tri2vtx, vtx2xyz, transform_world2pix, pix2tri, pix2val, dldw_pix2val,

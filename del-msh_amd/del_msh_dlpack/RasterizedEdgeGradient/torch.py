@@ -7,13 +7,17 @@ from ..util_torch import assert_shape_dtype_device
 
 
 def bwd(tri2vtx: torch.Tensor, vtx2xyz: torch.Tensor, transform_world2pix: torch.Tensor,
-        pix2tri: torch.Tensor, pix2val: torch.Tensor, dldw_pix2val: torch.Tensor) -> torch.Tensor:
+        pix2tri: torch.Tensor, pix2val: torch.Tensor, dldw_pix2val: torch.Tensor,
+        ordered: bool = True) -> torch.Tensor:
     """dL/d(vertex positions) from the silhouette edges of the rasterized image.
 
     Args:
         tri2vtx: (num_tri, 3) uint32; vtx2xyz: (num_vtx, 3) float32
         transform_world2pix: (4, 4) float32, row-major
         pix2tri: (H, W) uint32; pix2val, dldw_pix2val: (H, W, nv) float32
+        ordered: True (default) sums every vertex's terms in del-msh-cpu's loop
+            order (bit-exact with it, the same bits on every run); False adds
+            them with float atomics in one launch (faster, order not fixed)
     Returns:
         dldw_vtx2xyz: (num_vtx, 3) float32
     """
@@ -32,7 +36,7 @@ def bwd(tri2vtx: torch.Tensor, vtx2xyz: torch.Tensor, transform_world2pix: torch
     from . import bwd as native
 
     native(tri2vtx, vtx2xyz, dldw_vtx2xyz, transform_world2pix.T.contiguous().flatten(),
-           pix2tri, pix2val, dldw_pix2val, _lib.stream_of(device))
+           pix2tri, pix2val, dldw_pix2val, _lib.stream_of(device), ordered=ordered)
     return dldw_vtx2xyz
 
 

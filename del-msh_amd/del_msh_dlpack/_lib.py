@@ -70,6 +70,12 @@ _PROTOTYPES = {
         [c_void_p, c_void_p, c_u32, c_void_p, c_u32, c_void_p, c_void_p, c_u32, c_u32, c_void_p,
          c_u32, c_void_p, c_void_p],
     ),
+    "dmsh_rasterized_edge_gradient_bwd_workspace_bytes": (c_size, [c_u32, c_u32, c_u32]),
+    "dmsh_rasterized_edge_gradient_bwd_ordered": (
+        c_int,
+        [c_void_p, c_void_p, c_u32, c_void_p, c_u32, c_void_p, c_void_p, c_u32, c_u32, c_void_p,
+         c_u32, c_void_p, c_void_p, c_void_p, c_size],
+    ),
     "dmsh_pix2depth_bwd_wrt_vtx2xyz": (
         c_int,
         [c_void_p, c_void_p, c_void_p, c_u32, c_void_p, c_u32, c_void_p, c_void_p, c_u32, c_u32,
@@ -273,6 +279,13 @@ def require_gpu(*tensors: torch.Tensor) -> torch.device:
             raise ValueError(f"tensors on different devices: {dev} vs {t.device}")
     torch.cuda.set_device(dev)
     return dev
+
+
+def check_transform(t: torch.Tensor, name: str, numel: int = 16) -> None:
+    """A transform crossing the C ABI is `numel` contiguous float32 values."""
+    if t.dtype != torch.float32 or t.numel() != numel or not t.is_contiguous():
+        raise ValueError(f"{name} must be {numel} contiguous float32 values, got "
+                         f"{tuple(t.shape)} {t.dtype}")
 
 
 def stream_of(device: torch.device) -> int:

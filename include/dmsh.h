@@ -283,6 +283,14 @@ int dmsh_interpolate_bwd(void *stream, const uint32_t *pix2tri, uint32_t img_w, 
  * (del-msh-dlpack/src/rasterized_edge_gradient.rs:27-150); CPU counterpart
  * del-msh-cpu/src/rasterized_edge_gradient.rs:434-603.
  * transform_world2pix: 16 floats column-major.
+ *   dmsh_rasterized_edge_gradient_bwd: one launch, the terms added with
+ *   float atomics (as the reference GPU kernels do): equal to the CPU loop
+ *   to rounding, summation order not fixed.
+ *   dmsh_rasterized_edge_gradient_bwd_ordered: each vertex adds its terms in
+ *   the CPU loop's order (vertical pixel pairs column by column, then the
+ *   horizontal pairs), so dldw_vtx2xyz equals del-msh-cpu's bit for bit and
+ *   does not change from run to run.  Workspace: about 56 bytes per pixel
+ *   pair (2 W H pairs); img_w * img_h < 2^27.
  * ------------------------------------------------------------------- */
 int dmsh_rasterized_edge_gradient_bwd(void *stream, const uint32_t *tri2vtx,
                                       uint32_t num_tri, const float *vtx2xyz,
@@ -292,6 +300,16 @@ int dmsh_rasterized_edge_gradient_bwd(void *stream, const uint32_t *tri2vtx,
                                       const uint32_t *pix2tri,
                                       uint32_t num_vdim, const float *pix2val,
                                       const float *dldw_pix2val);
+size_t dmsh_rasterized_edge_gradient_bwd_workspace_bytes(uint32_t img_w, uint32_t img_h,
+                                                         uint32_t num_vtx);
+int dmsh_rasterized_edge_gradient_bwd_ordered(void *stream, const uint32_t *tri2vtx,
+                                              uint32_t num_tri, const float *vtx2xyz,
+                                              uint32_t num_vtx, float *dldw_vtx2xyz,
+                                              const float *transform_world2pix, uint32_t img_w,
+                                              uint32_t img_h, const uint32_t *pix2tri,
+                                              uint32_t num_vdim, const float *pix2val,
+                                              const float *dldw_pix2val, void *ws,
+                                              size_t ws_bytes);
 
 /* ---------------------------------------------------------------------
  * Contour edges: the edges whose two triangles face opposite ways along the

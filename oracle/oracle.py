@@ -306,9 +306,12 @@ def pix2depth_bwd(pix2tri_, tri2vtx, vtx2xyz, dldw_pix2depth, ndc2world_colmajor
     return out.reshape(-1, 3)
 
 
-def edge_gradient_bwd(tri2vtx, vtx2xyz, world2pix_colmajor, pix2tri_, pix2val, dldw_pix2val):
+def edge_gradient_bwd(tri2vtx, vtx2xyz, world2pix_colmajor, pix2tri_, pix2val, dldw_pix2val,
+                      dldw_init=None):
+    """dL/d(vertex) added term after term to dldw_init (zeros if None)."""
     h, w, nv = pix2val.shape
-    out = np.zeros(_f32(vtx2xyz).size, np.float32)
+    out = (np.zeros(_f32(vtx2xyz).size, np.float32) if dldw_init is None
+           else _f32(dldw_init).reshape(-1).copy())
     lib().orc_edge_gradient_bwd(_u32(tri2vtx).reshape(-1), _f32(vtx2xyz).reshape(-1), out,
                                 _f32(world2pix_colmajor), w, h, _u32(pix2tri_).reshape(-1), nv,
                                 _f32(pix2val).reshape(-1), _f32(dldw_pix2val).reshape(-1))

@@ -480,9 +480,7 @@ def test_pix2depth(gpu, oracle, camera):
 # ---------------------------------------------------------------------------
 # edge-gradient backward
 # ---------------------------------------------------------------------------
-@pytest.mark.parametrize("camera", ["ortho", "persp"])
-@pytest.mark.parametrize("nv", [1, 3])
-def test_edge_gradient_bwd(gpu, oracle, camera, nv):
+def _edge_gradient_scene(oracle, camera, W, H_, nv, seed):
     tri, vtx = primitives.sphere_yup(0.8, 48, 96)
     bn, aabb = oracle.build_trimesh3(tri, vtx)
     if camera == "ortho":
@@ -490,22 +488,62 @@ def test_edge_gradient_bwd(gpu, oracle, camera, nv):
         n2w = np.linalg.inv(w2n).astype(np.float32)
     else:
         n2w, w2n = _perspective(eye=(0.3, 0.2, 3.0))
-    W, H_ = 160, 128
     n2p = np.array([[0.5 * W, 0, 0, 0.5 * W], [0, -0.5 * H_, 0, 0.5 * H_], [0, 0, 0.5, 0.5], [0, 0, 0, 1]],
                    np.float32)
     w2p = (n2p @ w2n).astype(np.float32)
     wpix, _, _ = oracle.pix2tri(tri, vtx, bn, aabb, n2w.T.reshape(-1), W, H_, num_threads=8)
-    rng = np.random.default_rng(40 + nv)
+    rng = np.random.default_rng(seed)
     occ = (wpix != U32MAX).astype(np.float32)[..., None]
     pix2val = np.concatenate([occ, rng.random((H_, W, nv - 1), dtype=np.float32)], axis=2)
     dldw = rng.normal(size=(H_, W, nv)).astype(np.float32)
+    return tri, vtx, w2p, wpix, pix2val, dldw
+
+
+@pytest.mark.parametrize("camera", ["ortho", "persp"])
+@pytest.mark.parametrize("nv", [1, 3])
+@pytest.mark.parametrize("ordered", [True, False])
+def test_edge_gradient_bwd(gpu, oracle, camera, nv, ordered):
+    """Ordered: bit-exact with the oracle's serial loop.  Atomic: float
+    atomics reorder the per-vertex sums, so to summation rounding."""
+    W, H_ = 160, 128
+    tri, vtx, w2p, wpix, pix2val, dldw = _edge_gradient_scene(oracle, camera, W, H_, nv, 40 + nv)
     got = H(RasterizedEdgeGradient.bwd(G(tri, gpu), G(vtx, gpu), G(w2p, gpu), G(wpix, gpu),
-                                       G(pix2val, gpu), G(dldw, gpu)))
+                                       G(pix2val, gpu), G(dldw, gpu), ordered=ordered))
     want = oracle.edge_gradient_bwd(tri, vtx, w2p.T.reshape(-1), wpix, pix2val, dldw)
-    # float atomics reorder the per-vertex sums: compare to summation rounding
     scale = np.abs(want).max()
     assert scale > 0
-    assert np.abs(got - want).max() <= 1e-5 * scale + 5e-6
+    if ordered:
+        assert bits_equal(got, want)
+    else:
+        assert np.abs(got - want).max() <= 1e-5 * scale + 5e-6
+    no_device_errors()
+
+
+@pytest.mark.parametrize("shape", [(1, 90), (90, 1), (2, 2), (97, 61)])
+def test_edge_gradient_bwd_ordered_shapes_and_accumulation(gpu, oracle, shape):
+    """Ragged images (one row, one column, 2 x 2, odd sizes) and a nonzero
+    output to accumulate into: the ordered backward adds to what dldw_vtx2xyz
+    holds, term after term, as the serial loop does."""
+    from del_msh_dlpack.RasterizedEdgeGradient import bwd as native
+
+    W, H_ = shape
+    tri, vtx, w2p, wpix, pix2val, dldw = _edge_gradient_scene(oracle, "persp", W, H_, 2, 7)
+    init = np.random.default_rng(8).normal(size=vtx.shape).astype(np.float32)
+    out = G(init, gpu)
+    native(G(tri, gpu), G(vtx, gpu), out, G(oracle.colmajor(w2p), gpu), G(wpix, gpu),
+           G(pix2val, gpu), G(dldw, gpu))
+    want = oracle.edge_gradient_bwd(tri, vtx, w2p.T.reshape(-1), wpix, pix2val, dldw, dldw_init=init)
+    assert bits_equal(H(out), want)
+    no_device_errors()
+
+
+def test_edge_gradient_bwd_ordered_repeats_bit_for_bit(gpu, oracle):
+    """Three runs of the ordered backward give the same bits."""
+    W, H_ = 200, 150
+    tri, vtx, w2p, wpix, pix2val, dldw = _edge_gradient_scene(oracle, "persp", W, H_, 1, 3)
+    runs = [H(RasterizedEdgeGradient.bwd(G(tri, gpu), G(vtx, gpu), G(w2p, gpu), G(wpix, gpu),
+                                         G(pix2val, gpu), G(dldw, gpu))) for _ in range(3)]
+    assert bits_equal(runs[0], runs[1]) and bits_equal(runs[0], runs[2])
     no_device_errors()
 
 
