@@ -1,9 +1,36 @@
 // dmsh_build.hip -- C ABI: Morton codes, argsort, topology, refit, whole build.
 #include <algorithm>
+#include <mutex>
 
 #include "build.hpp"
 
 using namespace dmsh;
+
+// Scratch of the allocation-free build: one array per GPU image, held until
+// the process ends.  Host code never passes this name to a kernel (on the
+// host it is only a stand-in for the symbol); it uses the device address
+// that hipGetSymbolAddress returns for the current device.
+static __device__ __align__(256) unsigned char scratch[DMSH_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+constexpr int kScratchDevices = 64;
+void* g_scratch_ptr[kScratchDevices];
+std::mutex g_scratch_mutex;
+
+// The device address of `scratch` on the current device, looked up on the
+// first call on that device (with that device current) and kept.
+void* scratch_of_current_device() {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kScratchDevices) return nullptr;
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return nullptr;
+    g_scratch_ptr[dev] = p;
+  }
+  return g_scratch_ptr[dev];
+}
+}  // namespace
 
 extern "C" {
 
@@ -118,6 +145,23 @@ int dmsh_build_trimesh3(void* stream, uint32_t num_tri, const uint32_t* tri2vtx,
     return bld::build<uint64_t, true>(s, num_tri, tri2vtx, num_vtx, vtx2xyz, bvhnodes,
                                       bvhnode2aabb, ws, ws_bytes);
   return DMSH_ERR_INVALID;
+}
+
+int dmsh_build_trimesh3_noalloc(void* stream, uint32_t num_tri, const uint32_t* tri2vtx,
+                                uint32_t num_vtx, const float* vtx2xyz, uint32_t morton_bits,
+                                uint32_t* bvhnodes, float* bvhnode2aabb) {
+  if (morton_bits != 30 && morton_bits != 63) return DMSH_ERR_INVALID;
+  if (num_tri == 0) return DMSH_OK;
+  if (num_tri >= (1u << 30)) return DMSH_ERR_INVALID;
+  // sized before anything is looked up or enqueued
+  if (dmsh_build_workspace_bytes(num_tri, morton_bits) > (size_t)DMSH_NOALLOC_SCRATCH_BYTES)
+    return DMSH_ERR_TOO_LARGE;
+  void* ws = scratch_of_current_device();
+  if (!ws) return DMSH_ERR_HIP;
+  // the build clears what it needs zeroed (counters, histograms, tile status)
+  // with its own kernel on every call
+  return dmsh_build_trimesh3(stream, num_tri, tri2vtx, num_vtx, vtx2xyz, morton_bits, bvhnodes,
+                             bvhnode2aabb, ws, (size_t)DMSH_NOALLOC_SCRATCH_BYTES);
 }
 
 int dmsh_build_points(void* stream, uint32_t num_pnt, const float* vtx2xyz, uint32_t morton_bits,

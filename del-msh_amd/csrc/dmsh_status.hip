@@ -119,6 +119,9 @@ const char* dmsh_status_string(int status) {
     case DMSH_ERR_INVALID: return "invalid argument";
     case DMSH_ERR_HIP: return "HIP runtime error";
     case DMSH_ERR_WORKSPACE: return "workspace too small";
+    case DMSH_ERR_TOO_LARGE:
+      return "input too large for the allocation-free entry point's scratch "
+             "(DMSH_NOALLOC_SCRATCH_BYTES)";
     default: return "unknown status";
   }
 }

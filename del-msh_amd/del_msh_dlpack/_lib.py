@@ -47,6 +47,8 @@ _PROTOTYPES = {
         c_int,
         [c_void_p, c_u32, c_void_p, c_u32, c_void_p, c_u32, c_void_p, c_void_p, c_void_p, c_size],
     ),
+    "dmsh_build_trimesh3_noalloc": (
+        c_int, [c_void_p, c_u32, c_void_p, c_u32, c_void_p, c_u32, c_void_p, c_void_p]),
     "dmsh_build_points": (
         c_int,
         [c_void_p, c_u32, c_void_p, c_u32, c_void_p, c_void_p, c_void_p, c_size],

@@ -32,6 +32,8 @@ extern "C" {
 #define DMSH_ERR_INVALID 1   /* bad size / null pointer / unsupported option */
 #define DMSH_ERR_HIP 2       /* a HIP runtime call or launch failed          */
 #define DMSH_ERR_WORKSPACE 3 /* workspace smaller than *_workspace_bytes()    */
+#define DMSH_ERR_TOO_LARGE 4 /* input beyond an allocation-free entry point's
+                                fixed scratch                               */
 
 /* device error bits (dmsh_get_device_errors) */
 #define DMSH_DEVERR_INDEX 0x1u          /* index out of range in input data   */
@@ -156,6 +158,22 @@ int dmsh_build_trimesh3(void *stream, uint32_t num_tri, const uint32_t *tri2vtx,
                         uint32_t num_vtx, const float *vtx2xyz,
                         uint32_t morton_bits, uint32_t *bvhnodes,
                         float *bvhnode2aabb, void *ws, size_t ws_bytes);
+/* Allocation-free variant of dmsh_build_trimesh3.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same arguments and results as dmsh_build_trimesh3 without (ws, ws_bytes):
+ * the workspace is a fixed DMSH_NOALLOC_SCRATCH_BYTES (64 MiB) device array
+ * inside the library, one per GPU, held until the process ends.  Meshes
+ * whose dmsh_build_workspace_bytes(num_tri, morton_bits) exceeds it (above
+ * 772,846 triangles with 63-bit codes, 870,720 with 30-bit codes) are
+ * rejected with DMSH_ERR_TOO_LARGE before anything is enqueued.  It calls no
+ * hipMalloc / hipFree.  Every call on a GPU shares that GPU's scratch: calls
+ * that overlap on one GPU (on different streams, or from different host
+ * threads) race; calls in order on one stream do not. */
+#define DMSH_NOALLOC_SCRATCH_BYTES (64u << 20)
+int dmsh_build_trimesh3_noalloc(void *stream, uint32_t num_tri, const uint32_t *tri2vtx,
+                                uint32_t num_vtx, const float *vtx2xyz, uint32_t morton_bits,
+                                uint32_t *bvhnodes, float *bvhnode2aabb);
 int dmsh_build_points(void *stream, uint32_t num_pnt, const float *vtx2xyz,
                       uint32_t morton_bits, uint32_t *bvhnodes,
                       float *bvhnode2aabb, void *ws, size_t ws_bytes);
