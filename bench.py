@@ -45,11 +45,12 @@ def parse():
     ap.add_argument("--img", type=int, default=2048)
     ap.add_argument("--level", type=int, default=9)
     ap.add_argument("--pmc-workload", action="store_true",
-                    help="run only what a counter pass needs: 3 ray-casts of the bench workload "
-                         "and 3 device-to-device copies of 512 MiB (the calibration kernel)")
-    ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
-                    help="turn two rocprofv3 --pmc output folders (FETCH_SIZE, WRITE_SIZE) "
-                         "into the traffic summary text")
+                    help="run only what a counter pass needs: 3 builds, 3 ray-casts and 3 "
+                         "standalone refits of the bench workload and 3 device-to-device copies "
+                         "of 512 MiB (the calibration kernel)")
+    ap.add_argument("--summarize-counters", nargs="+", metavar="DIR",
+                    help="turn rocprofv3 --pmc output folders (one per counter pass) into the "
+                         "per-kernel counter summary text")
     ap.add_argument("--batch-meshes", type=int, default=4096,
                     help="meshes of the batched workload (0 skips it)")
     ap.add_argument("--batch-reps", type=int, default=3)
@@ -70,7 +71,7 @@ TIMED_KERNELS = ["k_pix2tri", "k_pix2tri_tier2", "k_pix2tri_tier3", "k_pack_inne
                  "k_build_init", "k_centroid_aabb", "k_morton_hist", "k_hist_all", "k_onesweep",
                  "k_sort_finish", "k_onesweep_gated", "k_copy_pairs", "k_topology",
                  "k_refit_treelet", "k_refit_top", "k_fill_u32"]
-TRAFFIC_SUMMARY = ROOT / "profiles" / "pmc_traffic_k_pix2tri_octa9_2048.txt"
+COUNTER_SUMMARY = ROOT / "profiles" / "pmc_counters_octa9_2048.txt"
 CALIB_BYTES = 512 << 20
 COPY_KERNEL = "__amd_rocclr_copyBuffer"
 
@@ -98,65 +99,156 @@ def perspective_ndc2world(eye=(0.0, 0.0, 3.0), fovy=45.0, near=1.0, far=5.0):
 # ---------------------------------------------------------------------------
 # counter passes
 # ---------------------------------------------------------------------------
-def _read_counter_csv(folder: str, counter: str):
-    """{kernel name: [value per dispatch]} from a rocprofv3 counter_collection
-    CSV (values in KiB for FETCH_SIZE / WRITE_SIZE)."""
-    files = sorted(pathlib.Path(folder).rglob("*counter_collection.csv"))
-    if not files:
-        raise SystemExit(f"no counter_collection.csv under {folder}")
-    per = {}
-    for f in files:
-        with open(f, newline="") as fh:
-            for row in csv.DictReader(fh):
-                if row.get("Counter_Name") != counter:
-                    continue
-                name = row.get("Kernel_Name", "")
-                per.setdefault(name, []).append(float(row["Counter_Value"]))
-    return per
+# Each pass is its own rocprofv3 run (the hardware collects one counter set
+# per pass); the summary joins them per kernel and per phase of the counter
+# workload (build, cast, refit: a kernel such as k_refit_treelet runs in the
+# build and in the standalone refit with different inputs).  Bytes at the
+# L2-to-fabric interface come from the size-split request counts summed over
+# every TCC instance (all channels of all XCDs): reads 32 / 64 / 128 B
+# requests, writes 64 B (WRREQ_64B) or 32 B (the rest); the bytes that go on
+# to DRAM from the 32-byte sector counts of DRAM-bound requests.  The 512 MiB
+# device copy checks both against bytes known to move.
+COUNTER_PASSES = {
+    "reads": ["TCC_EA0_RDREQ_32B_sum", "TCC_EA0_RDREQ_64B_sum", "TCC_EA0_RDREQ_128B_sum",
+              "TCC_EA0_RDREQ_DRAM_32B_sum"],
+    "writes": ["TCC_EA0_WRREQ_sum", "TCC_EA0_WRREQ_64B_sum", "TCC_EA0_WRREQ_WRITE_DRAM_32B_sum",
+               "TCC_EA0_WRREQ_ATOMIC_DRAM_32B_sum"],
+    "hits": ["TCC_HIT_sum", "TCC_MISS_sum", "TCC_EA0_RDREQ_sum", "TCC_EA0_RDREQ_DRAM_sum"],
+    "issue": ["SQ_WAVES", "SQ_INSTS_VALU", "SQ_INSTS_LDS", "SQ_INSTS_VMEM", "SQ_INSTS_SALU",
+              "SQ_ACTIVE_INST_VALU", "SQ_WAVE_CYCLES", "SQ_BUSY_CU_CYCLES", "GRBM_GUI_ACTIVE"],
+    "waits": ["SQ_WAIT_INST_LDS", "SQ_LDS_BANK_CONFLICT", "SQ_WAIT_ANY", "SQ_ACTIVE_INST_LDS",
+              "SQ_WAIT_INST_ANY", "SQ_ACTIVE_INST_ANY", "SQ_INSTS_SMEM", "SQ_BUSY_CYCLES"],
+}
+# (phase, kernel-name fragment) rows of the summary
+PROFILED_KERNELS = [
+    ("cast", "k_pix2tri<"), ("cast", "k_pack_inner"), ("cast", "k_pack_leaf"),
+    ("build", "k_centroid_aabb<true>"), ("build", "k_morton_hist"), ("build", "k_onesweep"),
+    ("build", "k_sort_finish"), ("build", "k_topology"), ("build", "k_refit_treelet<1,"),
+    ("build", "k_refit_top_lds"),
+    ("refit", "k_leaf_ranges"), ("refit", "k_list_treelets"), ("refit", "k_refit_treelet<1,"),
+    ("refit", "k_refit_top_lds"),
+]
+NUM_CU = 256
 
 
-def _match(per, short):
-    for name, vals in per.items():
-        if short in name:
-            return vals
-    return None
+def _read_counters(folders):
+    """{(phase, kernel name): {counter: [value per dispatch]}} and the
+    dispatch durations (ns), from rocprofv3 counter_collection CSV files.  The
+    counter workload runs 3 builds, 3 casts, 3 refits and 3 copies in that
+    order: per file, dispatches before the first k_pix2tri are the build's,
+    those after the last one the refit's (copies keep their own name)."""
+    per, dur = {}, {}
+    for folder in folders:
+        files = sorted(pathlib.Path(folder).rglob("*counter_collection.csv"))
+        if not files:
+            raise SystemExit(f"no counter_collection.csv under {folder}")
+        for f in files:
+            with open(f, newline="") as fh:
+                rows = list(csv.DictReader(fh))
+            cast_ids = [int(r["Dispatch_Id"]) for r in rows if "k_pix2tri<" in r["Kernel_Name"]]
+            lo, hi = (min(cast_ids), max(cast_ids)) if cast_ids else (0, 0)
+            for r in rows:
+                d = int(r["Dispatch_Id"])
+                phase = "build" if d < lo else ("refit" if d > hi else "cast")
+                name = r["Kernel_Name"]
+                if COPY_KERNEL in name:
+                    phase = f"copy{r.get('Grid_Size', '')}"
+                key = (phase, name)
+                per.setdefault(key, {}).setdefault(r["Counter_Name"], []).append(
+                    float(r["Counter_Value"]))
+                dur.setdefault(key, {})[(folder, d)] = (float(r["End_Timestamp"])
+                                                        - float(r["Start_Timestamp"]))
+    return per, {k: list(v.values()) for k, v in dur.items()}
 
 
-def summarize_pmc(fetch_dir: str, write_dir: str) -> str:
-    fetch = _read_counter_csv(fetch_dir, "FETCH_SIZE")
-    write = _read_counter_csv(write_dir, "WRITE_SIZE")
-    cf, cw = _match(fetch, COPY_KERNEL), _match(write, COPY_KERNEL)
-    kf, kw = _match(fetch, "k_pix2tri<"), _match(write, "k_pix2tri<")
-    if not (cf and cw and kf and kw):
-        raise SystemExit("counter passes lack the copy kernel or k_pix2tri")
-    calib_kib = CALIB_BYTES / 1024.0
-    # the copy reads and writes CALIB_BYTES once: the factor maps counter KiB to bytes
-    f_fac = CALIB_BYTES / (sum(cf) / len(cf) * 1024.0)
-    w_fac = CALIB_BYTES / (sum(cw) / len(cw) * 1024.0)
-    kf_avg = sum(kf) / len(kf)
-    kw_avg = sum(kw) / len(kw)
-    traffic = kf_avg * 1024.0 * f_fac + kw_avg * 1024.0 * w_fac
-    lines = [
-        "HBM traffic of k_pix2tri on the bench workload (octahedral sphere level 9, 2^21 tris,",
-        "63-bit LBVH, perspective 2048x2048), from two rocprofv3 counter passes:",
-        "  cd /tmp && export TMPDIR=/tmp",
-        "  rocprofv3 --pmc FETCH_SIZE --output-format csv -d <fetch> -- python3 bench.py --pmc-workload",
-        "  rocprofv3 --pmc WRITE_SIZE --output-format csv -d <write> -- python3 bench.py --pmc-workload",
-        "  python3 bench.py --summarize-pmc <fetch> <write>",
-        f"calibration: {COPY_KERNEL} of {CALIB_BYTES} B (read once, written once)",
-        f"  FETCH_SIZE {sum(cf) / len(cf):.1f} KiB (expected {calib_kib:.1f}) -> factor {f_fac:.4f}",
-        f"  WRITE_SIZE {sum(cw) / len(cw):.1f} KiB (expected {calib_kib:.1f}) -> factor {w_fac:.4f}",
-        f"k_pix2tri: {len(kf)} dispatches, FETCH_SIZE {kf_avg:.1f} KiB, WRITE_SIZE {kw_avg:.1f} KiB",
-        f"traffic_bytes_per_launch {traffic:.6e}",
-    ]
+def _avg(vals):
+    return sum(vals) / len(vals) if vals else float("nan")
+
+
+def summarize_counters(folders, title: str) -> str:
+    per, dur = _read_counters(folders)
+
+    def pick(phase, short):
+        for key in per:
+            if key[0] == phase and short in key[1]:
+                return key
+        return None
+
+    def val(key, c):
+        return _avg(per.get(key, {}).get(c, []))
+
+    def l2_read(key):
+        return (32 * val(key, "TCC_EA0_RDREQ_32B_sum") + 64 * val(key, "TCC_EA0_RDREQ_64B_sum")
+                + 128 * val(key, "TCC_EA0_RDREQ_128B_sum"))
+
+    def l2_write(key):
+        wr, w64 = val(key, "TCC_EA0_WRREQ_sum"), val(key, "TCC_EA0_WRREQ_64B_sum")
+        return (wr - w64) * 32 + w64 * 64
+
+    def dram_read(key):
+        return 32 * val(key, "TCC_EA0_RDREQ_DRAM_32B_sum")
+
+    def dram_write(key):
+        return 32 * (val(key, "TCC_EA0_WRREQ_WRITE_DRAM_32B_sum")
+                     + val(key, "TCC_EA0_WRREQ_ATOMIC_DRAM_32B_sum"))
+
+    copies = sorted((k for k in per if k[0].startswith("copy")),
+                    key=lambda k: -int(k[0][4:] or 0))
+    lines = [title, "Commands (MI355X, ROCm 7.2), one rocprofv3 pass per counter set:",
+             "  cd /tmp && export TMPDIR=/tmp"]
+    for key, cs in COUNTER_PASSES.items():
+        lines.append(f"  timeout -s KILL 150 rocprofv3 --pmc {' '.join(cs)} --output-format csv "
+                     f"-d <{key}> -- python3 bench.py --pmc-workload")
+    lines.append("  python3 bench.py --summarize-counters <reads> <writes> <hits> <issue> <waits>")
+    if copies:
+        c = copies[0]
+        lines.append(f"check: {COPY_KERNEL} of {CALIB_BYTES} B (read once, written once): L2<->fabric "
+                     f"read {l2_read(c) / CALIB_BYTES:.3f} x, write {l2_write(c) / CALIB_BYTES:.3f} x; "
+                     f"to / from DRAM read {dram_read(c) / CALIB_BYTES:.3f} x, write "
+                     f"{dram_write(c) / CALIB_BYTES:.3f} x")
+    lines.append("per dispatch, averaged over its dispatches in that phase: us (kernel time in the "
+                 "counter passes); MB read / written at the L2-fabric interface; MB read from / "
+                 "written to DRAM; L2 hit %; VALU busy % (SQ_ACTIVE_INST_VALU / (256 CUs x "
+                 "GRBM_GUI_ACTIVE)); VALU / LDS / VMEM instructions per wave; LDS-wait share of "
+                 "wave cycles %; LDS bank-conflict cycles per LDS instruction; waves")
+    lines.append(f"{'phase':<6}{'kernel':<24}{'us':>8}{'L2rdMB':>8}{'L2wrMB':>8}{'DRrdMB':>8}"
+                 f"{'DRwrMB':>8}{'L2hit%':>7}{'VALU%':>6}{'valu/w':>8}{'lds/w':>7}{'vmem/w':>7}"
+                 f"{'ldsW%':>6}{'bankc':>6}{'waves':>8}")
+    for phase, short in PROFILED_KERNELS:
+        key = pick(phase, short)
+        if not key:
+            continue
+        waves = val(key, "SQ_WAVES")
+        hit, miss = val(key, "TCC_HIT_sum"), val(key, "TCC_MISS_sum")
+        gui = val(key, "GRBM_GUI_ACTIVE")
+        lds = val(key, "SQ_INSTS_LDS")
+        us = _avg(dur.get(key, [])) * 1e-3
+        label = short.rstrip("<,")
+        lines.append(
+            f"{phase:<6}{label:<24}{us:>8.1f}{l2_read(key) / 1e6:>8.2f}{l2_write(key) / 1e6:>8.2f}"
+            f"{dram_read(key) / 1e6:>8.2f}{dram_write(key) / 1e6:>8.2f}"
+            f"{100 * hit / (hit + miss):>7.1f}"
+            f"{100 * val(key, 'SQ_ACTIVE_INST_VALU') / NUM_CU / gui:>6.1f}"
+            f"{val(key, 'SQ_INSTS_VALU') / waves:>8.1f}{lds / waves:>7.1f}"
+            f"{val(key, 'SQ_INSTS_VMEM') / waves:>7.1f}"
+            f"{100 * val(key, 'SQ_WAIT_INST_LDS') / val(key, 'SQ_WAVE_CYCLES'):>6.1f}"
+            f"{(val(key, 'SQ_LDS_BANK_CONFLICT') / lds) if lds else 0.0:>6.2f}{waves:>8.0f}")
+    lines.append("DRAM bytes per launch (read + write), the roofline `traffic` figures:")
+    for phase, short in PROFILED_KERNELS:
+        key = pick(phase, short)
+        if key:
+            lines.append(f"traffic_bytes_per_launch {phase}:{short.rstrip('<,')} "
+                         f"{dram_read(key) + dram_write(key):.6e}")
     return "\n".join(lines) + "\n"
 
 
-def read_traffic():
+def read_traffic(kernel: str = "cast:k_pix2tri"):
+    """DRAM-side bytes per launch of `kernel` from the committed counter summary."""
     try:
-        for line in TRAFFIC_SUMMARY.read_text().splitlines():
-            if line.startswith("traffic_bytes_per_launch"):
-                return float(line.split()[1])
+        for line in COUNTER_SUMMARY.read_text().splitlines():
+            parts = line.split()
+            if parts[:2] == ["traffic_bytes_per_launch", kernel]:
+                return float(parts[2])
     except OSError:
         return None
     return None
@@ -509,8 +601,12 @@ def run_served(num_req: int, rates, budget_bytes: int):
 # ---------------------------------------------------------------------------
 def main():
     args = parse()
-    if args.summarize_pmc:
-        sys.stdout.write(summarize_pmc(*args.summarize_pmc))
+    if args.summarize_counters:
+        sys.stdout.write(summarize_counters(
+            args.summarize_counters,
+            "Counters per kernel on the bench workload (octahedral sphere level 9 = 2^21 tris, "
+            "radial noise\nN(0,0.01^2) seed 1, 63-bit LBVH build, standalone refit of the built tree, "
+            "perspective 2048x2048 cast)."))
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
@@ -557,15 +653,24 @@ def main():
                                   ws_cast.numel()), "pix2tri")
 
     if args.pmc_workload:
-        build()
+        r_ws = _lib.workspace(L.dmsh_refit_workspace_bytes(num_tri), dev)
+        r_box = torch.empty_like(aabb)
+        for _ in range(3):
+            build()
         for _ in range(3):
             cast()
+        for _ in range(3):
+            _lib.check(L.dmsh_bvhnode2aabb_trimesh3(sp, r_box.data_ptr(), num_node,
+                                                    bvhnodes.data_ptr(), num_tri, t_tri.data_ptr(),
+                                                    num_vtx, t_vtx.data_ptr(), 0.0, r_ws.data_ptr(),
+                                                    r_ws.numel()), "refit")
         src = torch.empty(CALIB_BYTES, dtype=torch.uint8, device=dev).fill_(1)
         dst = torch.empty_like(src)
         for _ in range(3):
             dst.copy_(src)
         torch.cuda.synchronize()
-        print(json.dumps({"pmc_workload": "done", "casts": 3, "copies": 3}))
+        print(json.dumps({"pmc_workload": "done", "builds": 3, "casts": 3, "refits": 3,
+                          "copies": 3}))
         return
 
     for _ in range(args.warmup):

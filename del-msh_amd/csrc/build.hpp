@@ -322,15 +322,22 @@ inline int build(hipStream_t s, uint32_t n, const uint32_t* tri2vtx, uint32_t nu
   // P is even for 30- and 63-bit codes: the passes end back in `codes`.
   rc = sortk::run_passes<KeyT>(s, n, kb, L.sort, codes, codes, L.idx, err, true);
   if (rc != DMSH_OK) return rc;
-  rc = launch_topology<KeyT>(s, n, bvhnodes, codes, L.idx,
-                            TreeletSink{kTreeletLeaves, L.treelet_count, L.treelets, n, nullptr,
-                                        nullptr, L.top_count, L.top_list, (uint32_t)kTopSlots,
-                                        L.node2slot});
+  // the topology pass also writes every leaf's box, so the treelet pass
+  // loads boxes instead of gathering them
+  const bool leaves_in_aabb = refit_option(kRefitLeavesInAabb) == 1;
+  TreeletSink ts{kTreeletLeaves, L.treelet_count, L.treelets, n, nullptr, nullptr,
+                 L.top_count, L.top_list, (uint32_t)kTopSlots, L.node2slot};
+  if (leaves_in_aabb) {
+    ts.leaf_aabb = aabb;
+    ts.tri_box = TRI ? L.tri_box : nullptr;
+    ts.xyz = xyz;
+  }
+  rc = launch_topology<KeyT>(s, n, bvhnodes, codes, L.idx, ts);
   if (rc != DMSH_OK) return rc;
   const TreeView v{aabb, bvhnodes, n, tri2vtx, num_vtx, xyz, L.tri_box};
   return launch_refit_treelets<TRI ? kLeafTri : kLeafPoint>(
       s, v, L.flags, L.treelet_count, L.treelets, 0.f, err, L.top_count, L.top_list, L.node2slot,
-      L.treelet_next);
+      L.treelet_next, nullptr, leaves_in_aabb);
 }
 
 }  // namespace bld

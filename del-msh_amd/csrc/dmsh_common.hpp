@@ -200,6 +200,18 @@ namespace dmsh {
 // returns its device address for the current device, or nullptr on failure.
 uint32_t* device_error_word();
 
+// Launch options of the build and refit kernels (process-wide, set through
+// dmsh_set_tuning; results do not depend on them).
+enum RefitOption {
+  kRefitLeavesInAabb = 0,
+  kRefitClimb = 1,
+  kRefitQueue = 2,
+  kRefitOptionCount = 3
+};
+int refit_option(RefitOption which);
+bool set_refit_option(const char* key, int value);
+int get_refit_option(const char* key);
+
 // Opt-in kernel timing (dmsh_timing_enable): while on, a TimedScope records a
 // HIP event on the launch stream before and after the kernels it encloses,
 // filed under its label.  Off, it costs one relaxed atomic load.

@@ -102,6 +102,17 @@ int dmsh_octree_aggregate(void* stream, uint32_t num_idx, uint32_t num_vdim, con
   return rc;
 }
 
+int dmsh_permute_u32(void* stream, uint32_t n, const uint32_t* old2val, const uint32_t* new2old,
+                     uint32_t* new2val) {
+  if (n == 0) return DMSH_OK;
+  if (!old2val || !new2old || !new2val) return DMSH_ERR_INVALID;
+  uint32_t* err = device_error_word();
+  if (!err) return DMSH_ERR_HIP;
+  launch("k_permute_u32", oct::k_permute_u32, dim3(std::min<uint32_t>(ceil_div(n, 256), 8192)),
+         dim3(256), 0, as_stream(stream), n, old2val, new2old, new2val, err);
+  return launch_status();
+}
+
 int dmsh_has_duplicate_sorted_u32(void* stream, uint32_t n, const uint32_t* idx2val,
                                   uint32_t* flag) {
   if (!flag || (n > 0 && !idx2val)) return DMSH_ERR_INVALID;

@@ -84,14 +84,14 @@ int dmsh_set_tuning(const char* key, int value) {
     g_cast_waves.store(value);
     return DMSH_OK;
   }
-  return DMSH_ERR_INVALID;
+  return set_refit_option(key, value) ? DMSH_OK : DMSH_ERR_INVALID;
 }
 
 int dmsh_get_tuning(const char* key) {
   if (!key) return -1;
   if (std::strcmp(key, "pix2tri.stack") == 0) return g_cast_stack.load();
   if (std::strcmp(key, "pix2tri.waves") == 0) return g_cast_waves.load();
-  return -1;
+  return get_refit_option(key);
 }
 
 size_t dmsh_pix2tri_workspace_bytes(uint32_t num_tri, uint32_t img_w, uint32_t img_h) {

@@ -1,5 +1,6 @@
 // dmsh_status.hip -- status strings and the sticky device error word.
 #include <atomic>
+#include <cstring>
 #include <map>
 #include <mutex>
 #include <string>
@@ -15,6 +16,34 @@ namespace dmsh {
 static constexpr int kMaxDevices = 64;
 static uint32_t* g_err_ptr[kMaxDevices];
 static std::mutex g_err_mutex;
+
+// refit options: "refit.leaves_in_aabb" (0: the treelet kernel builds the
+// leaf boxes; 1: the kernel before writes them), "refit.climb" (0: each
+// level waits for its links, then its arrival counter, then the sibling's
+// box; 1: one LDS round trip per level), "refit.queue" (1: workgroups take
+// treelets from a shared counter; 0: treelet t goes to workgroup t mod grid)
+static std::atomic<int> g_refit_option[kRefitOptionCount] = {{0}, {1}, {1}};
+static const char* const kRefitOptionKey[kRefitOptionCount] = {
+    "refit.leaves_in_aabb", "refit.climb", "refit.queue"};
+
+int refit_option(RefitOption which) { return g_refit_option[which].load(); }
+
+bool set_refit_option(const char* key, int value) {
+  for (int i = 0; i < kRefitOptionCount; ++i) {
+    if (std::strcmp(key, kRefitOptionKey[i]) == 0) {
+      if (value != 0 && value != 1) return false;
+      g_refit_option[i].store(value);
+      return true;
+    }
+  }
+  return false;
+}
+
+int get_refit_option(const char* key) {
+  for (int i = 0; i < kRefitOptionCount; ++i)
+    if (std::strcmp(key, kRefitOptionKey[i]) == 0) return g_refit_option[i].load();
+  return -1;
+}
 
 uint32_t* device_error_word() {
   int dev = 0;

@@ -237,6 +237,22 @@ __global__ __launch_bounds__(256) void k_onode_sum_level(uint32_t num_onode, uin
 // ---------------------------------------------------------------------------
 // flags[k] = (val[k + 1] != val[k]), k < n - 1; the exclusive scan of them is
 // idx2jdx
+// new2val[i] = old2val[new2old[i]] (del-msh-dlpack/src/array1d.rs:17-70); an
+// index out of range raises DMSH_DEVERR_INDEX and leaves that entry as it was.
+__global__ __launch_bounds__(256) void k_permute_u32(uint32_t n, const uint32_t* __restrict__ old2val,
+                                                     const uint32_t* __restrict__ new2old,
+                                                     uint32_t* __restrict__ new2val,
+                                                     uint32_t* __restrict__ err) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const uint32_t o = new2old[i];
+    if (o >= n) {
+      raise_error(err, DMSH_DEVERR_INDEX);
+      continue;
+    }
+    new2val[i] = old2val[o];
+  }
+}
+
 __global__ __launch_bounds__(256) void k_change_flags(uint32_t n, const uint32_t* __restrict__ val,
                                                       uint32_t* __restrict__ flags,
                                                       uint32_t* __restrict__ dup) {

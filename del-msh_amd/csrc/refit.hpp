@@ -217,10 +217,58 @@ struct TreeletLds {
   uint32_t parent[K];
 };
 
-template <int LK, int K, int T>
+// A treelet's inputs when its leaf boxes are already in v.aabb (PRE): the
+// node rows, the leaf rows' parents and the leaf boxes of this thread's
+// slots.  None of these loads depends on another, so they can be issued
+// together, and a persistent workgroup issues the next treelet's while it
+// climbs the current one.
+template <int LPT>
+struct TreeletIn {
+  uint32_t row[LPT][3];
+  uint32_t lp[LPT];
+  Box lb[LPT];
+};
+
+template <int K, int T>
+__device__ __forceinline__ void treelet_prefetch(const TreeView& v, uint32_t f, uint32_t l,
+                                                 TreeletIn<K / T>& in) {
+  const uint32_t nb = v.num_leaf - 1;
+  if (f > l || l >= v.num_leaf || l - f >= (uint32_t)K) return;  // refit_treelet raises
+  const uint32_t m = l - f + 1;
+#pragma unroll
+  for (int r = 0; r < K / T; ++r) {
+    const uint32_t j = (uint32_t)r * T + threadIdx.x;
+    if (j >= m) continue;
+    const uint32_t id = f + j;
+    if (id < nb) {
+      const uint32_t* rw = v.bvhnodes + (size_t)id * 3;
+      in.row[r][0] = rw[0];
+      in.row[r][1] = rw[1];
+      in.row[r][2] = rw[2];
+    }
+    in.lp[r] = v.bvhnodes[(size_t)(nb + id) * 3];
+    in.lb[r] = box_load(v.aabb, nb + id);
+  }
+}
+
+// PRE: the leaf boxes are already in v.aabb (the kernel before wrote them);
+// phase 1 takes this thread's inputs from `pre` (treelet_prefetch).
+// CLIMB 0: a level reads the parent's links, then takes its arrival counter
+// (workgroup-scope acq_rel), then reads the sibling's box: three dependent
+// LDS round trips.  CLIMB 1: the links of the next parent are read one level
+// ahead, and the counter, the sibling's box and those links go out together,
+// one LDS round trip per level.  The box is read whether or not this is the
+// second arrival and used only if it is: LDS operations of a wave complete
+// in issue order and the LDS performs them one at a time, so the sibling's
+// box store, issued before its own counter update, has been performed when
+// that update has; the counter RMW is ordered with the wave's own LDS
+// accesses at wavefront scope (no waits), which keeps the compiler from
+// moving the box store after it or the box load before it.
+template <int LK, int K, int T, bool PRE = false, int CLIMB = 0>
 __device__ __forceinline__ void refit_treelet(const TreeView& v, uint32_t root, uint32_t f,
                                               uint32_t l, float eps, TreeletLds<K>& s,
-                                              uint32_t* __restrict__ err) {
+                                              uint32_t* __restrict__ err,
+                                              const TreeletIn<K / T>* pre = nullptr) {
   constexpr int LPT = K / T;
   const uint32_t num_leaf = v.num_leaf;
   const uint32_t nb = num_leaf - 1;
@@ -231,8 +279,25 @@ __device__ __forceinline__ void refit_treelet(const TreeView& v, uint32_t root, 
   const uint32_t m = l - f + 1;
   uint32_t lpar[LPT];
   // phase 1
+  if (PRE) {
+#pragma unroll
+    for (int r = 0; r < LPT; ++r) {
+      const uint32_t j = (uint32_t)r * T + threadIdx.x;
+      lpar[r] = DMSH_U32MAX - 1;  // no climb
+      if (j >= m) continue;
+      const uint32_t id = f + j;
+      s.flag[j] = 0u;
+      if (id < nb) {
+        s.parent[j] = pre->row[r][0];
+        s.child[j] = make_uint2(pre->row[r][1], pre->row[r][2]);
+      }
+      s.leaf[j] = pre->lb[r];
+      if (nb + id != root) lpar[r] = pre->lp[r];
+    }
+  }
 #pragma unroll
   for (int r = 0; r < LPT; ++r) {
+    if (PRE) break;
     const uint32_t j = (uint32_t)r * T + threadIdx.x;
     lpar[r] = DMSH_U32MAX - 1;  // no climb
     if (j >= m) continue;
@@ -269,7 +334,73 @@ __device__ __forceinline__ void refit_treelet(const TreeView& v, uint32_t root, 
     cnode[r] = nb + f + j;
     if (act[r]) mine[r] = s.leaf[j];
   }
-  for (uint32_t steps = 0;; ++steps) {
+  if (CLIMB == 1) {
+    uint2 ch[LPT];
+    uint32_t gp[LPT];
+#pragma unroll
+    for (int r = 0; r < LPT; ++r) {
+      ch[r] = make_uint2(DMSH_U32MAX, DMSH_U32MAX);
+      gp[r] = DMSH_U32MAX;
+      if (!act[r]) continue;
+      const uint32_t parent = cpar[r];
+      if (parent < f || parent > l || parent >= nb) continue;  // raised below
+      ch[r] = s.child[parent - f];
+      gp[r] = s.parent[parent - f];
+    }
+    for (uint32_t steps = 0;; ++steps) {
+      uint32_t old[LPT];
+      Box sb[LPT];
+      uint2 ch2[LPT];
+      uint32_t gp2[LPT];
+#pragma unroll
+      for (int r = 0; r < LPT; ++r) {
+        if (!act[r]) continue;
+        const uint32_t parent = cpar[r], node = cnode[r];
+        const uint32_t sib = (ch[r].x == node) ? ch[r].y : ch[r].x;
+        const uint32_t si = (sib >= nb) ? sib - nb - f : sib - f;
+        if (parent < f || parent > l || parent >= nb || steps >= (uint32_t)K ||
+            (ch[r].x != node && ch[r].y != node) || si >= m) {
+          raise_error(err, DMSH_DEVERR_REFIT);
+          act[r] = false;
+          continue;
+        }
+        const uint32_t pj = parent - f;
+        old[r] = __hip_atomic_fetch_add(&s.flag[pj], 1u, __ATOMIC_ACQ_REL,
+                                        __HIP_MEMORY_SCOPE_WAVEFRONT);
+        sb[r] = (sib >= nb) ? s.leaf[si] : s.inner[si];
+        const uint32_t g = gp[r];
+        ch2[r] = make_uint2(DMSH_U32MAX, DMSH_U32MAX);
+        gp2[r] = DMSH_U32MAX;
+        if (parent != root && g >= f && g <= l && g < nb) {
+          ch2[r] = s.child[g - f];
+          gp2[r] = s.parent[g - f];
+        }
+      }
+      bool more = false;
+#pragma unroll
+      for (int r = 0; r < LPT; ++r) {
+        if (!act[r]) continue;
+        if (old[r] == 0) {  // the sibling subtree finishes this node
+          act[r] = false;
+          continue;
+        }
+        const uint32_t parent = cpar[r], pj = parent - f, node = cnode[r];
+        mine[r] = (ch[r].x == node) ? box_merge(mine[r], sb[r]) : box_merge(sb[r], mine[r]);
+        s.inner[pj] = mine[r];
+        if (parent == root) {
+          act[r] = false;
+          continue;
+        }
+        cnode[r] = parent;
+        cpar[r] = gp[r];
+        ch[r] = ch2[r];
+        gp[r] = gp2[r];
+        more = true;
+      }
+      if (!more) break;
+    }
+  }
+  for (uint32_t steps = 0; CLIMB == 0; ++steps) {
     uint2 ch[LPT];
     uint32_t gp[LPT], old[LPT];
 #pragma unroll
@@ -368,7 +499,7 @@ __device__ __forceinline__ void refit_top(const TreeView& v, uint32_t* __restric
 // vary from 1 to K leaves, so a fixed deal would leave workgroups idle);
 // without it, treelet t goes to workgroup t mod grid.  A nonzero *skip (the
 // standalone refit's layout check failed) makes the kernel do nothing.
-template <int LK, int K, int T>
+template <int LK, int K, int T, bool PRE, int CLIMB>
 __global__ __launch_bounds__(T) void k_refit_treelet(TreeView v, float eps,
                                                      const uint32_t* __restrict__ treelet_count,
                                                      const uint4* __restrict__ treelets,
@@ -376,14 +507,77 @@ __global__ __launch_bounds__(T) void k_refit_treelet(TreeView v, float eps,
                                                      const uint32_t* __restrict__ skip,
                                                      uint32_t* __restrict__ err) {
   __shared__ TreeletLds<K> s;
-  __shared__ uint32_t s_t;
+  __shared__ uint32_t s_t, s_t2;
   if (skip && *skip) return;
   uint32_t count = *treelet_count;
   if (count > v.num_leaf) count = v.num_leaf;
+  if (PRE) {
+    // Two treelets in flight per workgroup: the next one's loads are issued
+    // before the current one's climb.  Treelet numbers run three deep: t is
+    // being climbed, tn's leaves are loading, t3's descriptor is loading, and
+    // thread 0 holds the ticket after t3 in a register until the climb is
+    // over, so no wait on an atomic or a descriptor sits before a climb.
+    // (Tickets come from `next`, or a fixed deal t, t + grid, ...)
+    TreeletIn<K / T> cur, nxt;
+    uint32_t t, tn, t3;
+    if (next) {
+      if (threadIdx.x == 0) {
+        const uint32_t t0 = atomicAdd(next, 3u);
+        s_t = t0;
+      }
+      __syncthreads();
+      t = s_t;
+      tn = t + 1u;
+      t3 = t + 2u;
+      __syncthreads();
+    } else {
+      t = blockIdx.x;
+      tn = t + gridDim.x;
+      t3 = tn + gridDim.x;
+    }
+    if (count == 0u) return;
+    // An opaque zero in a vector register, added to the descriptor index: a
+    // descriptor whose address the compiler can prove uniform is read with a
+    // scalar load, and every LDS wait of the climb (lgkmcnt) would wait for
+    // it too.  Descriptors of treelets past the end are never used, so their
+    // index is clamped instead of selected.
+    uint32_t vzero;
+    asm volatile("v_mov_b32 %0, 0" : "=v"(vzero));
+    const uint32_t last = count - 1u;
+    uint4 d = treelets[(t < count ? t : last) + vzero];
+    uint4 dn = treelets[(tn < count ? tn : last) + vzero];
+    if (t < count) treelet_prefetch<K, T>(v, d.y, d.z, cur);
+    // vmcnt(0) (expcnt / lgkmcnt left alone): everything issued so far is
+    // complete on entry, as it is at the end of every iteration, so the loop
+    // body waits for nothing it issues before the climb
+    __builtin_amdgcn_s_waitcnt(0x0F70);
+    while (t < count) {
+      if (tn < count) treelet_prefetch<K, T>(v, dn.y, dn.z, nxt);
+      const uint4 d3 = treelets[(t3 < count ? t3 : last) + vzero];
+      uint32_t t4 = 0u;
+      // lane 0 only; the lane id in the address keeps the compiler from
+      // turning it into a wave-wide atomic whose result is read at once,
+      // so the wait for it (and for the loads before it) comes after the
+      // climb, where the ticket is stored
+      if (next && threadIdx.x == 0) t4 = atomicAdd(next + __lane_id(), 1u);
+      refit_treelet<LK, K, T, true, CLIMB>(v, d.x, d.y, d.z, eps, s, err, &cur);
+      if (next && threadIdx.x == 0) s_t2 = t4;
+      __syncthreads();  // thread 0's ticket is visible (also after an early return)
+      t4 = next ? s_t2 : t3 + gridDim.x;
+      __syncthreads();  // read by everyone before thread 0 writes the next
+      t = tn;
+      d = dn;
+      cur = nxt;
+      tn = t3;
+      dn = d3;
+      t3 = t4;
+    }
+    return;
+  }
   if (!next) {
     for (uint32_t t = blockIdx.x; t < count; t += gridDim.x) {
       const uint4 tl = treelets[t];
-      refit_treelet<LK, K, T>(v, tl.x, tl.y, tl.z, eps, s, err);
+      refit_treelet<LK, K, T, false, CLIMB>(v, tl.x, tl.y, tl.z, eps, s, err);
     }
     return;
   }
@@ -394,7 +588,7 @@ __global__ __launch_bounds__(T) void k_refit_treelet(TreeView v, float eps,
     __syncthreads();  // every thread has read s_t before it is reused
     if (t >= count) break;
     const uint4 tl = treelets[t];
-    refit_treelet<LK, K, T>(v, tl.x, tl.y, tl.z, eps, s, err);
+    refit_treelet<LK, K, T, false, CLIMB>(v, tl.x, tl.y, tl.z, eps, s, err);
   }
 }
 
@@ -684,13 +878,26 @@ inline int launch_refit_treelets(hipStream_t s, const TreeView& v, uint32_t* fla
                                  const uint2* top_list = nullptr,
                                  const uint32_t* node2slot = nullptr,
                                  uint32_t* treelet_next = nullptr,
-                                 const uint32_t* skip = nullptr) {
+                                 const uint32_t* skip = nullptr, bool leaves_in_aabb = false) {
   const uint32_t num_leaf = v.num_leaf;
   if (num_leaf == 0) return DMSH_OK;
   const uint32_t grid = std::min<uint32_t>(kTreeletGrid, num_leaf);
-  launch("k_refit_treelet", (k_refit_treelet<LK, (int)kTreeletLeaves, kTreeletThreads>),
-         dim3(grid), dim3(kTreeletThreads), 0, s, v, eps, treelet_count, treelets, treelet_next,
-         skip, err);
+  constexpr int K = (int)kTreeletLeaves, Th = kTreeletThreads;
+  const bool climb1 = refit_option(kRefitClimb) == 1;
+  if (refit_option(kRefitQueue) == 0) treelet_next = nullptr;
+  if (leaves_in_aabb && climb1) {
+    launch("k_refit_treelet", (k_refit_treelet<LK, K, Th, true, 1>), dim3(grid), dim3(Th), 0, s,
+           v, eps, treelet_count, treelets, treelet_next, skip, err);
+  } else if (leaves_in_aabb) {
+    launch("k_refit_treelet", (k_refit_treelet<LK, K, Th, true, 0>), dim3(grid), dim3(Th), 0, s,
+           v, eps, treelet_count, treelets, treelet_next, skip, err);
+  } else if (climb1) {
+    launch("k_refit_treelet", (k_refit_treelet<LK, K, Th, false, 1>), dim3(grid), dim3(Th), 0, s,
+           v, eps, treelet_count, treelets, treelet_next, skip, err);
+  } else {
+    launch("k_refit_treelet", (k_refit_treelet<LK, K, Th, false, 0>), dim3(grid), dim3(Th), 0, s,
+           v, eps, treelet_count, treelets, treelet_next, skip, err);
+  }
   int rc = launch_status();
   if (rc != DMSH_OK || num_leaf <= kTreeletLeaves) return rc;
   if (top_count) {
@@ -760,17 +967,31 @@ constexpr int kRangeWalks = 1;
 constexpr int kListThreads = 1024;
 constexpr uint32_t kListGrid = 65535;  // one round of kListThreads nodes per workgroup
 
-static __global__ __launch_bounds__(256) void k_leaf_ranges(uint32_t num_leaf,
-                                                            const uint32_t* __restrict__ bvhnodes,
-                                                            uint32_t* __restrict__ first,
-                                                            uint32_t* __restrict__ last,
-                                                            uint32_t* __restrict__ flags,
-                                                            uint32_t* __restrict__ counters) {
+// With `boxes`, also stores each leaf's box (leaf_box, as the treelet
+// kernels would make it), so the treelet pass reads boxes instead of
+// gathering geometry.
+template <int LK>
+__global__ __launch_bounds__(256) void k_leaf_ranges(TreeView v, float eps, int boxes,
+                                                     uint32_t* __restrict__ first,
+                                                     uint32_t* __restrict__ last,
+                                                     uint32_t* __restrict__ flags,
+                                                     uint32_t* __restrict__ counters,
+                                                     uint32_t* __restrict__ err) {
+  const uint32_t num_leaf = v.num_leaf;
+  const uint32_t* __restrict__ bvhnodes = v.bvhnodes;
   const uint32_t nb = num_leaf - 1;
   const uint32_t k0 = blockIdx.x * (256u * kRangeWalks) + threadIdx.x;
   if (k0 == 0) {
 #pragma unroll
     for (int c = 0; c < 4; ++c) counters[c] = 0u;
+  }
+#pragma unroll
+  for (int r = 0; r < kRangeWalks; ++r) {
+    const uint32_t k = k0 + 256u * r;
+    if (!boxes || k >= num_leaf) continue;
+    Box b;
+    uint32_t par;
+    if (leaf_box<LK>(v, k, eps, b, par, err)) box_store(v.aabb, nb + k, b);
   }
   uint32_t node[kRangeWalks], p[kRangeWalks];
   int side[kRangeWalks];
@@ -892,8 +1113,9 @@ inline int launch_refit_standalone(hipStream_t s, const TreeView& v, float eps, 
            (const uint32_t*)nullptr, err);
     return launch_status();
   }
-  launch("k_leaf_ranges", k_leaf_ranges, dim3(ceil_div(n, 256 * kRangeWalks)), dim3(256), 0, s, n,
-         v.bvhnodes, L.first, L.last, L.flags, L.counters);
+  const bool leaves_in_aabb = refit_option(kRefitLeavesInAabb) == 1;
+  launch("k_leaf_ranges", k_leaf_ranges<LK>, dim3(ceil_div(n, 256 * kRangeWalks)), dim3(256), 0, s,
+         v, eps, leaves_in_aabb ? 1 : 0, L.first, L.last, L.flags, L.counters, err);
   int rc = launch_status();
   if (rc != DMSH_OK) return rc;
   launch("k_list_treelets", k_list_treelets<kListThreads>,
@@ -902,7 +1124,7 @@ inline int launch_refit_standalone(hipStream_t s, const TreeView& v, float eps, 
   if ((rc = launch_status()) != DMSH_OK) return rc;
   rc = launch_refit_treelets<LK>(s, v, L.flags, L.counters + 0, L.treelets, eps, err,
                                  L.counters + 1, L.top_list, L.node2slot, L.counters + 2,
-                                 L.counters + 3);
+                                 L.counters + 3, leaves_in_aabb);
   if (rc != DMSH_OK) return rc;
   launch("k_refit", k_refit<LK>, dim3(std::min<uint32_t>(ceil_div(n, 256), 1024)), dim3(256), 0,
          s, v, L.flags, eps, (const uint32_t*)(L.counters + 3), err);

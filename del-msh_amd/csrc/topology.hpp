@@ -108,7 +108,34 @@ struct TreeletSink {
   uint2* top_list = nullptr;
   uint32_t top_cap = 0;
   uint32_t* node2slot = nullptr;
+  // Optional: the leaf boxes, written into leaf_aabb (the tree's bvhnode2aabb)
+  // as the leaf rows are: from tri_box (6 floats per object) when given, else
+  // the point xyz[obj] (the box the refit's leaf_box makes for it).
+  float* leaf_aabb = nullptr;
+  const float* tri_box = nullptr;
+  const float* xyz = nullptr;
 };
+
+__device__ __forceinline__ void emit_leaf_box(const TreeletSink& ts, uint32_t leaf, uint32_t obj) {
+  const float* src;
+  float b[6];
+  if (ts.tri_box) {
+    src = ts.tri_box + (size_t)obj * 6;
+#pragma unroll
+    for (int d = 0; d < 6; ++d) b[d] = src[d];
+  } else {
+    src = ts.xyz + (size_t)obj * 3;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      b[d] = src[d] - 0.f;
+      b[d + 3] = src[d] + 0.f;
+    }
+  }
+  float2* dst = reinterpret_cast<float2*>(ts.leaf_aabb + (size_t)leaf * 6);
+  dst[0] = make_float2(b[0], b[1]);
+  dst[1] = make_float2(b[2], b[3]);
+  dst[2] = make_float2(b[4], b[5]);
+}
 
 __device__ __forceinline__ void emit_treelet(const TreeletSink& ts, uint32_t node, uint32_t f,
                                              uint32_t l, uint32_t mesh, uint32_t root_base) {
@@ -154,20 +181,24 @@ __device__ __forceinline__ void topology_node(uint32_t n, uint32_t i, uint32_t* 
   }
   if (first == split) {
     const uint32_t a = nb + split;
+    const uint32_t obj = idx2obj[split] - obj_sub;
     bvhnodes[(size_t)i * 3 + 1] = a;
     bvhnodes[(size_t)a * 3 + 0] = i;
-    bvhnodes[(size_t)a * 3 + 1] = idx2obj[split] - obj_sub;
+    bvhnodes[(size_t)a * 3 + 1] = obj;
     bvhnodes[(size_t)a * 3 + 2] = DMSH_U32MAX;
+    if (ts.leaf_aabb) emit_leaf_box(ts, a, obj);
   } else {
     bvhnodes[(size_t)i * 3 + 1] = split;
     bvhnodes[(size_t)split * 3 + 0] = i;
   }
   if (last == split + 1) {
     const uint32_t b = nb + split + 1;
+    const uint32_t obj = idx2obj[split + 1] - obj_sub;
     bvhnodes[(size_t)i * 3 + 2] = b;
     bvhnodes[(size_t)b * 3 + 0] = i;
-    bvhnodes[(size_t)b * 3 + 1] = idx2obj[split + 1] - obj_sub;
+    bvhnodes[(size_t)b * 3 + 1] = obj;
     bvhnodes[(size_t)b * 3 + 2] = DMSH_U32MAX;
+    if (ts.leaf_aabb) emit_leaf_box(ts, b, obj);
   } else {
     bvhnodes[(size_t)i * 3 + 2] = split + 1;
     bvhnodes[(size_t)(split + 1) * 3 + 0] = i;
@@ -182,6 +213,7 @@ __device__ __forceinline__ void topology_single(uint32_t* __restrict__ bvhnodes,
   bvhnodes[0] = DMSH_U32MAX;
   bvhnodes[1] = obj;
   bvhnodes[2] = DMSH_U32MAX;
+  if (ts.leaf_aabb) emit_leaf_box(ts, 0u, obj);
   if (ts.list) emit_treelet(ts, 0u, 0u, 0u, mesh, root_base);
 }
 

@@ -4,6 +4,17 @@ from .. import _lib
 from ..util_torch import as_tensor
 
 
+def permute(old2val, new2old, new2val, stream_ptr=0):
+    """new2val[i] = old2val[new2old[i]] for uint32 arrays (reference:
+    del_msh_dlpack/Array1D/__init__.py, native array1d_permute)."""
+    old2val, new2old, new2val = as_tensor(old2val), as_tensor(new2old), as_tensor(new2val)
+    dev = _lib.require_gpu(old2val, new2old, new2val)
+    stream = stream_ptr or _lib.stream_of(dev)
+    _lib.check(_lib.lib().dmsh_permute_u32(stream, old2val.shape[0], _lib.ptr(old2val),
+                                           _lib.ptr(new2old), _lib.ptr(new2val)),
+               "dmsh_permute_u32")
+
+
 def argsort(idx2val, jdx2idx, stream_ptr=0, key_bits=None):
     """Sorts idx2val in place (stable) and writes the permutation to jdx2idx."""
     idx2val, jdx2idx = as_tensor(idx2val), as_tensor(jdx2idx)

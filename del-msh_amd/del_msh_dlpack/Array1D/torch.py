@@ -1,11 +1,27 @@
-"""Array1D.argsort (reference: del_msh_dlpack/Array1D/torch.py:41-72).
+"""Array1D (reference: del_msh_dlpack/Array1D/torch.py).
 
-Stable: equal keys keep their input order (the order of del-msh-cpu's
-`sort_by_key`, del-msh-cpu/src/mortons.rs:143)."""
+argsort is stable: equal keys keep their input order (the order of
+del-msh-cpu's `sort_by_key`, del-msh-cpu/src/mortons.rs:143)."""
 
 import torch
 
 from .. import _lib
+from ..util_torch import assert_shape_dtype_device
+
+
+def permute(old2val: torch.Tensor, new2old: torch.Tensor) -> torch.Tensor:
+    """new2val[i] = old2val[new2old[i]] for (n,) uint32 arrays (reference
+    Array1D/torch.py:5-38); an index >= n raises the device index error."""
+    n = old2val.shape[0]
+    device = old2val.device
+    assert_shape_dtype_device(old2val, (n,), torch.uint32, device)
+    assert_shape_dtype_device(new2old, (n,), torch.uint32, device)
+    _lib.require_gpu(old2val, new2old)
+    new2val = torch.empty(size=(n,), device=device, dtype=torch.uint32)
+    from . import permute as native
+
+    native(old2val, new2old, new2val, _lib.stream_of(device))
+    return new2val
 
 
 def argsort(idx2val: torch.Tensor, key_bits: int | None = None):

@@ -35,6 +35,20 @@ def from_transform_ndc2pix(img_shape, device=None, dtype=torch.float32):
                  [0, 0, 0, 1]], device, dtype)
 
 
+def from_transfrom_world2unit(vtx2xyz, device):
+    """translate(0.5) @ scale(1 / longest extent) @ translate(-box centre): the
+    bounding box of the points, centred in the unit cube (reference
+    Mat44/torch.py:107-117; composed as products of 4x4 matrices there too)."""
+    xyz_min = vtx2xyz.min(dim=0).values
+    xyz_max = vtx2xyz.max(dim=0).values
+    scale = 1.0 / (xyz_max - xyz_min).max().item()
+    c = (xyz_min + xyz_max) * 0.5
+    m1 = from_translation(-c[0], -c[1], -c[2], device)
+    m2 = from_uniform_scale(scale, device)
+    m3 = from_translation(0.5, 0.5, 0.5, device)
+    return m3 @ m2 @ m1
+
+
 def from_fit_vtx2xyz_into_unit_cube(vtx2xyz: torch.Tensor) -> torch.Tensor:
     """Uniform scale s = 1 / max extent and translation -center * s + 0.5, mapping
     the bounding box of the points into [0, 1]^3 (reference Mat44/torch.py:120-140)."""

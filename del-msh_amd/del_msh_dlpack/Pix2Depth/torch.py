@@ -65,3 +65,20 @@ class AutogradFunction(torch.autograd.Function):
         bwd_wrt_vtx2xyz(dldw_vtx2xyz, pix2tri, tri2vtx, vtx2xyz.detach(),
                         dldw_pix2depth.contiguous(), transform_ndc2world)
         return dldw_vtx2xyz, None, None, None
+
+
+def pix2depth_using_interpolation(vtx2xyz, pix2tri, tri2vtx, transform_ndc2world):
+    """Depth through attribute interpolation, differentiable in vtx2xyz
+    (reference Pix2Depth/torch.py:134-146): each pixel's hit point is the
+    interpolation of the vertex positions over its triangle (the HIP
+    interpolation kernel, AutogradInterpolate), projected by world2ndc;
+    depth = (ndc z / w + 1) / 2, and 0 on the background."""
+    from ..Pix2Tri.torch import AutogradInterpolate
+
+    pix2xyz = AutogradInterpolate.apply(pix2tri, tri2vtx, vtx2xyz, vtx2xyz, transform_ndc2world)
+    transform_world2ndc = transform_ndc2world.inverse()
+    pix2xyzw = torch.cat([pix2xyz, torch.ones_like(pix2xyz[:, :, :1])], dim=-1)
+    background = pix2tri == torch.iinfo(torch.uint32).max
+    pix2ndcw = pix2xyzw @ transform_world2ndc.T
+    pix2depth = (pix2ndcw[:, :, 2] / pix2ndcw[:, :, 3] + 1.0) * 0.5
+    return torch.where(background, torch.zeros_like(pix2depth), pix2depth)

@@ -175,6 +175,7 @@ _PROTOTYPES = {
     "dmsh_octree_aggregate": (
         c_int, [c_void_p, c_u32, c_u32, c_void_p, c_void_p, c_u32, c_u32, c_void_p, c_void_p,
                 c_void_p, c_size]),
+    "dmsh_permute_u32": (c_int, [c_void_p, c_u32, c_void_p, c_void_p, c_void_p]),
     "dmsh_has_duplicate_sorted_u32": (c_int, [c_void_p, c_u32, c_void_p, c_void_p]),
     "dmsh_unique_for_sorted_workspace_bytes": (c_size, [c_u32]),
     "dmsh_unique_for_sorted_u32": (c_int, [c_void_p, c_u32, c_void_p, c_void_p, c_void_p, c_size]),

@@ -578,6 +578,11 @@ int dmsh_octree_aggregate(void *stream, uint32_t num_idx, uint32_t num_vdim, con
  * kdx2val[jdx2kdx[j]] for j in [off[idx], off[idx+1]) in j order onto
  * idx2aggval (the reference's wrapper zero-fills it first).
  * ------------------------------------------------------------------- */
+/* new2val[i] = old2val[new2old[i]].  Replaces the GPU branch of
+ * array1d_permute (del-msh-dlpack/src/array1d.rs:17-70, del_cudarc_sys
+ * array1d::permute); an index >= n raises DMSH_DEVERR_INDEX. */
+int dmsh_permute_u32(void *stream, uint32_t n, const uint32_t *old2val, const uint32_t *new2old,
+                     uint32_t *new2val);
 int dmsh_has_duplicate_sorted_u32(void *stream, uint32_t n, const uint32_t *idx2val,
                                   uint32_t *flag);
 size_t dmsh_unique_for_sorted_workspace_bytes(uint32_t n);

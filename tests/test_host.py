@@ -76,3 +76,49 @@ def test_edge_gradient_bwd_checks_transform_before_any_launch():
     # a well-formed call on host tensors gets as far as the device check
     with pytest.raises(NotImplementedError):
         RasterizedEdgeGradient.bwd(tri, vtx, torch.eye(4), pix, val, val.clone())
+
+
+# the pyo3 names the reference registers (del-msh-dlpack/src/*.rs add_functions)
+# on the hot path and the LBVH-machinery modules this build covers
+NATIVE_NAMES = [
+    "get_cuda_driver_version", "get_ptx_compiler_version",
+    "mortons_vtx2morton_from_vtx2co", "mortons_make_bvhnodes_from_sorted_mortons",
+    "array1d_permute", "array1d_argsort", "array1d_has_duplicate_sorted_array",
+    "array1d_unique_for_sorted_array", "array1d_unique_jdx2val_jdx2idx",
+    "bvhnode2aabb_update_aabb", "pix2tri_by_raycast", "pix2tri_interpolate",
+    "pix2tri_interpolate_bwd", "pix2depth_update", "pix2depth_bwd_wrt_vtx2xyz",
+    "rasterized_edge_gradient_bwd", "rasterized_edge_gradient_edge_gradient_and_type",
+    "rasterized_edge_gradient_smooth_gradient", "rasterized_edge_gradient_interpolate",
+    "differentiable_antialias_fwd", "differentiable_antialias_bwd",
+    "edge2vtx_contour_for_triangle_mesh", "edge2vtx_from_vtx2vtx",
+    "edge2elem_from_edge2vtx_of_tri2vtx_with_vtx2vtx", "vtx2elem_from_uniform_mesh",
+    "vtx2vtx_from_uniform_mesh", "vtx2vtx_laplacian_smoothing", "vtx2vtx_multiply_graph_laplacian",
+    "quad_oct_tree_bnodes_and_bnode2depth_and_bnode2onode_and_idx2bnode",
+    "quad_oct_tree_make_tree_from_binary_radix_tree", "quad_oct_tree_aggregate",
+    "offset_array_aggregate", "polyhedron_mesh_elem2center",
+    "polyhedron_mesh_bvhnode2aabb_from_bvhnodes", "polyhedron_mesh_search_elem_contain_points",
+    "polyhedron_mesh_interpolate_values_at_points",
+]
+
+
+def test_native_module_names_and_host_arrays():
+    """The native-level names exist; host arrays are refused (no CPU path in
+    this build) before anything is launched; the version queries return the
+    values of a build without CUDA."""
+    import pytest
+    from torch.utils.dlpack import to_dlpack
+
+    import del_msh_dlpack.del_msh_dlpack as N
+
+    missing = [n for n in NATIVE_NAMES if not callable(getattr(N, n, None))]
+    assert not missing, missing
+    assert issubclass(N.PanicException, BaseException)
+    assert not issubclass(N.PanicException, Exception)
+    assert N.get_cuda_driver_version() == (0xFFFFFFFF, 0xFFFFFFFF)
+    assert N.get_ptx_compiler_version() == (-1, -1, -1)
+    keys = torch.arange(8, dtype=torch.int64).to(torch.uint32)
+    idx = torch.empty(8, dtype=torch.uint32)
+    with pytest.raises(NotImplementedError):
+        N.array1d_argsort(to_dlpack(keys), to_dlpack(idx), 0)
+    with pytest.raises(NotImplementedError):
+        N.array1d_permute(keys, keys, idx, 0)
