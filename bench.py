@@ -129,6 +129,7 @@ PROFILED_KERNELS = [
     ("refit", "k_refit_top_lds"),
 ]
 NUM_CU = 256
+NUM_XCD = 8  # GRBM_GUI_ACTIVE comes back summed over the XCDs
 
 
 def _read_counters(folders):
@@ -209,8 +210,8 @@ def summarize_counters(folders, title: str) -> str:
     lines.append("per dispatch, averaged over its dispatches in that phase: us (kernel time in the "
                  "counter passes); MB read / written at the L2-fabric interface; MB read from / "
                  "written to DRAM; L2 hit %; VALU busy % (SQ_ACTIVE_INST_VALU / (256 CUs x "
-                 "GRBM_GUI_ACTIVE)); VALU / LDS / VMEM instructions per wave; LDS-wait share of "
-                 "wave cycles %; LDS bank-conflict cycles per LDS instruction; waves")
+                 "GRBM_GUI_ACTIVE / 8 XCDs)); VALU / LDS / VMEM instructions per wave; LDS-wait "
+                 "share of wave cycles %; LDS bank-conflict cycles per LDS instruction; waves")
     lines.append(f"{'phase':<6}{'kernel':<24}{'us':>8}{'L2rdMB':>8}{'L2wrMB':>8}{'DRrdMB':>8}"
                  f"{'DRwrMB':>8}{'L2hit%':>7}{'VALU%':>6}{'valu/w':>8}{'lds/w':>7}{'vmem/w':>7}"
                  f"{'ldsW%':>6}{'bankc':>6}{'waves':>8}")
@@ -228,27 +229,87 @@ def summarize_counters(folders, title: str) -> str:
             f"{phase:<6}{label:<24}{us:>8.1f}{l2_read(key) / 1e6:>8.2f}{l2_write(key) / 1e6:>8.2f}"
             f"{dram_read(key) / 1e6:>8.2f}{dram_write(key) / 1e6:>8.2f}"
             f"{100 * hit / (hit + miss):>7.1f}"
-            f"{100 * val(key, 'SQ_ACTIVE_INST_VALU') / NUM_CU / gui:>6.1f}"
+            f"{100 * val(key, 'SQ_ACTIVE_INST_VALU') / NUM_CU / (gui / NUM_XCD):>6.1f}"
             f"{val(key, 'SQ_INSTS_VALU') / waves:>8.1f}{lds / waves:>7.1f}"
             f"{val(key, 'SQ_INSTS_VMEM') / waves:>7.1f}"
             f"{100 * val(key, 'SQ_WAIT_INST_LDS') / val(key, 'SQ_WAVE_CYCLES'):>6.1f}"
             f"{(val(key, 'SQ_LDS_BANK_CONFLICT') / lds) if lds else 0.0:>6.2f}{waves:>8.0f}")
-    lines.append("DRAM bytes per launch (read + write), the roofline `traffic` figures:")
+    lines.append("per launch, read back by bench.py: DRAM bytes (read + write; the roofline "
+                 "`traffic`), L2<->fabric bytes, L2 hit %, VALU busy %, kernel us:")
     for phase, short in PROFILED_KERNELS:
         key = pick(phase, short)
         if key:
-            lines.append(f"traffic_bytes_per_launch {phase}:{short.rstrip('<,')} "
-                         f"{dram_read(key) + dram_write(key):.6e}")
+            hit, miss = val(key, "TCC_HIT_sum"), val(key, "TCC_MISS_sum")
+            gui = val(key, "GRBM_GUI_ACTIVE")
+            lines.append(f"per_launch {phase}:{short.rstrip('<,')} "
+                         f"dram_bytes={dram_read(key) + dram_write(key):.6e} "
+                         f"l2_fabric_bytes={l2_read(key) + l2_write(key):.6e} "
+                         f"l2_hit_pct={100 * hit / (hit + miss):.2f} "
+                         f"valu_busy_pct={100 * val(key, 'SQ_ACTIVE_INST_VALU') / NUM_CU / (gui / NUM_XCD):.2f} "
+                         f"us={_avg(dur.get(key, [])) * 1e-3:.2f}")
     return "\n".join(lines) + "\n"
 
 
-def read_traffic(kernel: str = "cast:k_pix2tri"):
-    """DRAM-side bytes per launch of `kernel` from the committed counter summary."""
+def fp64_error(tri, vtx, n2w_rowmajor, pix2tri_h, pix2t_h, depth_h):
+    """Error of the f32 hit coefficient and depth against a float64
+    evaluation of the same pixel ray, the same (stored) triangle and the same
+    projection: Moller-Trumbore and the homogeneous transforms in f64 on the
+    f32 inputs.  Returns max / mean absolute error, max relative error and
+    the error in units of the f32 spacing at the f64 value."""
+    import numpy as np
+
+    h, w = pix2tri_h.shape
+    hit = pix2tri_h.reshape(-1) != 0xFFFFFFFF
+    ip = np.nonzero(hit)[0]
+    iw, ih = (ip % w).astype(np.float64), (ip // w).astype(np.float64)
+    T = np.asarray(n2w_rowmajor, np.float32).astype(np.float64)  # ndc -> world, row-major
+
+    def xform(m, p):
+        q = p @ m[:3, :3].T + m[:3, 3]
+        wq = p @ m[3, :3] + m[3, 3]
+        return q / wq[:, None]
+
+    x0 = 2.0 * (iw + 0.5) / w - 1.0
+    y0 = 1.0 - 2.0 * (ih + 0.5) / h
+    one = np.ones_like(x0)
+    org = xform(T, np.stack([x0, y0, one], 1))
+    far = xform(T, np.stack([x0, y0, -one], 1))
+    d = far - org
+    t3 = tri[pix2tri_h.reshape(-1)[ip]].astype(np.int64)
+    v = vtx.astype(np.float64)
+    p0, p1, p2 = v[t3[:, 0]], v[t3[:, 1]], v[t3[:, 2]]
+    e1, e2 = p1 - p0, p2 - p0
+    pv = np.cross(d, e2)
+    det = np.einsum("ij,ij->i", e1, pv)
+    qv = np.cross(org - p0, e1)
+    t64 = np.einsum("ij,ij->i", e2, qv) / det
+    depth64 = (xform(np.linalg.inv(T), org + t64[:, None] * d)[:, 2] + 1.0) * 0.5
+
+    def stats(got, want):
+        got = got.astype(np.float64)
+        err = np.abs(got - want)
+        ulp = np.spacing(np.abs(want).astype(np.float32)).astype(np.float64)
+        return {"max_abs": float(err.max()), "mean_abs": float(err.mean()),
+                "max_rel": float((err / np.abs(want)).max()),
+                "max_f32_ulps": round(float((err / ulp).max()), 2),
+                "mean_f32_ulps": round(float((err / ulp).mean()), 3)}
+
+    return {"hit_pixels": int(ip.size),
+            "pix2t": stats(pix2t_h.reshape(-1)[ip], t64),
+            "depth": stats(depth_h.reshape(-1)[ip], depth64),
+            "reference": "float64 ray, Moller-Trumbore and projection on the f32 inputs and the "
+                         "stored triangle"}
+
+
+def read_counters(kernel: str = "cast:k_pix2tri"):
+    """Per-launch figures of `kernel` (phase:name) from the committed counter
+    summary: {dram_bytes, l2_fabric_bytes, l2_hit_pct, valu_busy_pct, us}, or
+    None when the summary does not list it."""
     try:
         for line in COUNTER_SUMMARY.read_text().splitlines():
             parts = line.split()
-            if parts[:2] == ["traffic_bytes_per_launch", kernel]:
-                return float(parts[2])
+            if parts[:2] == ["per_launch", kernel]:
+                return {k: float(v) for k, v in (p.split("=") for p in parts[2:])}
     except OSError:
         return None
     return None
@@ -762,6 +823,35 @@ def main():
         "kernels_avg_ms": refit_kernels,
         "equals_build_boxes": refit_same,
     }
+    # per-kernel DRAM traffic of the standalone refit from the counter summary
+    rk = {k: read_counters(f"refit:{k}") for k in ("k_leaf_ranges", "k_list_treelets",
+                                                   "k_refit_treelet", "k_refit_top_lds")}
+    if all(rk.values()):
+        dram = sum(v["dram_bytes"] for v in rk.values())
+        refit_line["roofline"] = {
+            "bound": "hbm",
+            "achieved": refit_line["algorithmic_gbs"],
+            "peak": PEAK_HBM_GBS,
+            "unit": "GB/s",
+            "frac": refit_line["frac_of_hbm_peak"],
+            "traffic": dram,
+            "limiter": "dependent-load and LDS-climb latency per treelet (see per kernel)",
+            "per_kernel": {k: {"dram_bytes": v["dram_bytes"], "us": v["us"],
+                               "dram_gbs": round(v["dram_bytes"] / (v["us"] * 1e-6) / 1e9, 1),
+                               "l2_hit_pct": v["l2_hit_pct"], "valu_busy_pct": v["valu_busy_pct"]}
+                           for k, v in rk.items()},
+        }
+
+    # ---------------- f32 error of the hit coefficient and depth ----------------
+    depth = torch.empty((H, W), dtype=torch.float32, device=dev)
+    ws_depth = _lib.workspace(max(1, L.dmsh_pix2depth_workspace_bytes()), dev)
+    _lib.check(L.dmsh_pix2depth(sp, depth.data_ptr(), pix2tri.data_ptr(), num_tri, t_tri.data_ptr(),
+                                num_vtx, t_vtx.data_ptr(), W, H, t_n2w.data_ptr(),
+                                ws_depth.data_ptr(), ws_depth.numel()), "pix2depth")
+    torch.cuda.synchronize()
+    f64err = fp64_error(tri, vtx, n2w, pix2tri.cpu().numpy(), pix2t.cpu().numpy(),
+                        depth.cpu().numpy())
+    del depth, ws_depth
 
     # ---------------- secondary: 2^22-point build and sort alone ----------------
     rng = np.random.default_rng(2)
@@ -862,6 +952,7 @@ def main():
             k_ms = sum(kernels[k]["avg_ms"] * kernels[k]["launches_per_step"]
                        for k in ("k_pix2tri", "k_pix2tri_tier2", "k_pix2tri_tier3") if k in kernels)
             name = "k_pix2tri + restart tiers"
+        cc = read_counters("cast:k_pix2tri") if complete else None
         roof = {
             "bound": "hbm",
             "kernel": name,
@@ -869,10 +960,24 @@ def main():
             "peak": PEAK_HBM_GBS,
             "unit": "GB/s",
             "frac": round(cast_bytes / (k_ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4),
-            "traffic": read_traffic() if complete else None,
+            "traffic": cc["dram_bytes"] if cc else None,
             "algorithmic_bytes": cast_bytes,
             "kernel_ms": k_ms,
         }
+        if cc:
+            # The algorithmic bytes are what the reference's layout would
+            # read from memory; the tree and triangles stay in L2 / MALL, so
+            # DRAM moves a small share of them and the kernel is limited by
+            # issue and dependent-load latency, not by HBM bandwidth.
+            roof["limiter"] = "issue / dependent-load latency (working set L2- and MALL-resident)"
+            roof["frac_meaning"] = "HBM-equivalent: algorithmic bytes / time / HBM peak"
+            roof["dram_gbs"] = round(cc["dram_bytes"] / (k_ms * 1e-3) / 1e9, 1)
+            roof["dram_frac_of_peak"] = round(cc["dram_bytes"] / (k_ms * 1e-3) / 1e9
+                                              / PEAK_HBM_GBS, 4)
+            roof["dram_share_of_algorithmic"] = round(cc["dram_bytes"] / cast_bytes, 4)
+            roof["l2_hit_pct"] = cc["l2_hit_pct"]
+            roof["valu_busy_pct"] = cc["valu_busy_pct"]
+            roof["counter_us"] = cc["us"]
     else:
         roof = {
             "bound": "hbm",
@@ -914,6 +1019,7 @@ def main():
         "points_build_mpts_per_s": round(npnt / (pbuild_ms * 1e-3) / 1e6, 2),
         "sort_alone_u64_2p22_ms": round(psort_ms, 4),
         "sort_alone_melem_per_s": round(npnt / (psort_ms * 1e-3) / 1e6, 2),
+        "fp64_error": f64err,
         "raycast_traversal": ({"internal_expansions": cast_counts[0], "leaf_tests": cast_counts[1],
                                "max_stack_frames": cast_counts[2], "lds_stack_frames": stack,
                                "waves_per_block": L.dmsh_get_tuning(b"pix2tri.waves")}

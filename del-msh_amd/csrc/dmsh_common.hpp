@@ -206,7 +206,8 @@ enum RefitOption {
   kRefitLeavesInAabb = 0,
   kRefitClimb = 1,
   kRefitQueue = 2,
-  kRefitOptionCount = 3
+  kRefitTop = 3,
+  kRefitOptionCount = 4
 };
 int refit_option(RefitOption which);
 bool set_refit_option(const char* key, int value);

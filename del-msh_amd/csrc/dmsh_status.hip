@@ -21,10 +21,12 @@ static std::mutex g_err_mutex;
 // leaf boxes; 1: the kernel before writes them), "refit.climb" (0: each
 // level waits for its links, then its arrival counter, then the sibling's
 // box; 1: one LDS round trip per level), "refit.queue" (1: workgroups take
-// treelets from a shared counter; 0: treelet t goes to workgroup t mod grid)
-static std::atomic<int> g_refit_option[kRefitOptionCount] = {{0}, {1}, {1}};
+// treelets from a shared counter; 0: treelet t goes to workgroup t mod grid),
+// "refit.top" (0: the nodes above the treelets climb through LDS counters
+// when there are at most 4096 of them; 1: they merge in barrier rounds)
+static std::atomic<int> g_refit_option[kRefitOptionCount] = {{0}, {1}, {1}, {0}};
 static const char* const kRefitOptionKey[kRefitOptionCount] = {
-    "refit.leaves_in_aabb", "refit.climb", "refit.queue"};
+    "refit.leaves_in_aabb", "refit.climb", "refit.queue", "refit.top"};
 
 int refit_option(RefitOption which) { return g_refit_option[which].load(); }
 

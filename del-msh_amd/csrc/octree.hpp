@@ -42,8 +42,8 @@ __global__ __launch_bounds__(256) void k_brt_depth(uint32_t num_bnode,
   if (i == 0) bnodes[0] = DMSH_U32MAX;
   const uint32_t n = num_bnode + 1;
   uint32_t first, last;
-  node_range(idx2morton, n, i, first, last);
-  const uint32_t split = node_split(idx2morton, first, last);
+  node_range<uint32_t>(idx2morton, n, i, first, last);
+  const uint32_t split = node_split<uint32_t>(idx2morton, first, last);
   if (first == split) {
     bnodes[i * 3 + 1] = num_bnode + split;
   } else {

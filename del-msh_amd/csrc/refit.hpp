@@ -755,7 +755,7 @@ __global__ __launch_bounds__(T) void k_refit_top_lds(
     const uint32_t* __restrict__ bvhnodes, const uint32_t* __restrict__ top_count,
     const uint2* __restrict__ top_list, const uint32_t* __restrict__ node2slot,
     const uint32_t* __restrict__ treelet_count, const uint4* __restrict__ treelets,
-    const uint32_t* __restrict__ skip, uint32_t* __restrict__ err) {
+    const uint32_t* __restrict__ skip, uint32_t* __restrict__ err, int rounds) {
   constexpr int R = (C + T - 1) / T;  // slots per thread
   static_assert((size_t)kTopClimbSlots * (sizeof(Box) + 7) <= (size_t)C * (sizeof(Box) + 1),
                 "the climb layout fits in the round layout's LDS");
@@ -763,7 +763,7 @@ __global__ __launch_bounds__(T) void k_refit_top_lds(
   if (skip && *skip) return;
   const TreeView v{aabb, bvhnodes, num_leaf, nullptr, 0u, nullptr};
   const uint32_t count = *top_count;
-  if (count <= (uint32_t)kTopClimbSlots) {
+  if (!rounds && count <= (uint32_t)kTopClimbSlots) {
     top_climb<kTopClimbSlots, T>(aabb, num_leaf, bvhnodes, count, top_list, node2slot, smem, err);
     return;
   }
@@ -903,7 +903,7 @@ inline int launch_refit_treelets(hipStream_t s, const TreeView& v, uint32_t* fla
   if (top_count) {
     launch("k_refit_top", k_refit_top_lds<kTopSlots, kTopThreads>, dim3(1), dim3(kTopThreads), 0,
            s, v.aabb, flags, num_leaf, v.bvhnodes, top_count, top_list, node2slot, treelet_count,
-           treelets, skip, err);
+           treelets, skip, err, refit_option(kRefitTop));
   } else {
     launch("k_refit_top", k_refit_top<kTopThreads>, dim3(1), dim3(kTopThreads), 0, s, v.aabb,
            flags, num_leaf, v.bvhnodes, treelet_count, treelets, err);

@@ -21,9 +21,40 @@ struct KeyTraits {
   static constexpr int kBits = 8 * sizeof(KeyT);
 };
 
-// Range [first, last] of internal node idx (1 <= idx <= n-2, or idx == 0).
+// Codes [lo, lo + len) copied to LDS, the rest read from memory.  The two
+// pointers carry their address spaces, so the two reads stay an LDS read and
+// a global read (a select between generic pointers would be a flat load).
 template <typename KeyT>
-__device__ __forceinline__ void node_range(const KeyT* __restrict__ c, uint32_t n, uint32_t idx,
+struct CodeWindow {
+  const __attribute__((address_space(1))) KeyT* g;
+  const __attribute__((address_space(3))) KeyT* s;
+  uint32_t lo, len;
+  __device__ CodeWindow(const KeyT* global, const KeyT* lds, uint32_t lo_, uint32_t len_)
+      : g((const __attribute__((address_space(1))) KeyT*)global),
+        s((const __attribute__((address_space(3))) KeyT*)lds),
+        lo(lo_),
+        len(len_) {}
+  __device__ __forceinline__ KeyT operator[](uint32_t j) const {
+    const uint32_t o = j - lo;
+    if (o < len) return s[o];
+    return g[j];
+  }
+};
+
+// The searches below test "more than p leading bits in common with key k"
+// as (c & mask) == (k & mask) with mask = the top p + 1 bits, which is
+// clz(k ^ c) > p without a leading-zero count, and run on 32-bit indices
+// (n < 2^31).  Every search finds the last index at which a monotone
+// predicate holds (along sorted codes the common prefix with a fixed code
+// does not grow away from it), exactly as the reference's loops do.
+template <typename KeyT>
+__device__ __forceinline__ KeyT top_bits_mask(int p) {
+  return ~KeyT(0) << (KeyTraits<KeyT>::kBits - 1 - p);  // 0 <= p < kBits
+}
+
+// Range [first, last] of internal node idx (1 <= idx <= n-2, or idx == 0).
+template <typename KeyT, typename Codes = const KeyT*>
+__device__ __forceinline__ void node_range(const Codes& c, uint32_t n, uint32_t idx,
                                            uint32_t& first, uint32_t& last) {
   if (idx == 0) {
     first = 0;
@@ -42,45 +73,41 @@ __device__ __forceinline__ void node_range(const KeyT* __restrict__ c, uint32_t 
     last = lo;
     return;
   }
-  const int64_t i = idx;
   const int dn = clz_key(mc1 ^ mc2), dp = clz_key(mc1 ^ mc0);
-  const int64_t d = (dn - dp) > 0 ? 1 : -1;
-  const int delta_min = (d > 0) ? dp : dn;  // delta(idx, idx - d)
-  int64_t lmax = 2;
-  for (;;) {
-    const int64_t j = i + lmax * d;
-    if (j < 0 || j >= (int64_t)n) break;
-    if (clz_key(mc1 ^ c[j]) <= delta_min) break;
-    lmax *= 2;
+  const bool fwd = dn > dp;                 // direction d = +1
+  const int delta_min = fwd ? dp : dn;      // delta(idx, idx - d) < kBits
+  const KeyT mask = top_bits_mask<KeyT>(delta_min);
+  const KeyT key = mc1 & mask;
+  const uint32_t xmax = fwd ? n - 1 - idx : idx;  // farthest step inside [0, n)
+  uint32_t lmax = 2;
+  while (lmax <= xmax && (c[fwd ? idx + lmax : idx - lmax] & mask) == key) lmax <<= 1;
+  uint32_t l = 0;
+  for (uint32_t t = lmax >> 1; t != 0; t >>= 1) {
+    const uint32_t x = l + t;
+    if (x <= xmax && (c[fwd ? idx + x : idx - x] & mask) == key) l = x;
   }
-  int64_t l = 0;
-  for (int64_t t = lmax / 2; t >= 1; t /= 2) {
-    const int64_t j = i + (l + t) * d;
-    if (j >= 0 && j < (int64_t)n && clz_key(mc1 ^ c[j]) > delta_min) l += t;
-  }
-  const int64_t j = i + l * d;
-  if (i <= j) {
+  if (fwd) {
     first = idx;
-    last = (uint32_t)j;
+    last = idx + l;
   } else {
-    first = (uint32_t)j;
+    first = idx - l;
     last = idx;
   }
 }
 
 // Split of range [first, last] (first < last), mortons.rs:290-325.
-template <typename KeyT>
-__device__ __forceinline__ uint32_t node_split(const KeyT* __restrict__ c, uint32_t first,
-                                               uint32_t last) {
+template <typename KeyT, typename Codes = const KeyT*>
+__device__ __forceinline__ uint32_t node_split(const Codes& c, uint32_t first, uint32_t last) {
   const KeyT ms = c[first];
-  const int p0 = clz_key(ms ^ c[last]);
-  if (p0 == KeyTraits<KeyT>::kBits) return first;
+  const KeyT diff = ms ^ c[last];
+  if (diff == 0) return first;
+  const KeyT mask = top_bits_mask<KeyT>(clz_key(diff));
+  const KeyT key = ms & mask;
   uint32_t split = first, step = last - first;
   while (step > 1) {
     step = (step + 1) >> 1;
     const uint32_t nw = split + step;
-    if (nw >= last) continue;
-    if (clz_key(ms ^ c[nw]) > p0) split = nw;
+    if (nw < last && (c[nw] & mask) == key) split = nw;
   }
   return split;
 }
@@ -150,16 +177,12 @@ __device__ __forceinline__ void emit_treelet(const TreeletSink& ts, uint32_t nod
 // Internal node i of a tree over n sorted codes: range, split, child links
 // (bvhnodes_morton.rs:18-49) and the treelets it starts.  idx2obj[k] - obj_sub
 // is the object of sorted position k.
-template <typename KeyT>
-__device__ __forceinline__ void topology_node(uint32_t n, uint32_t i, uint32_t* __restrict__ bvhnodes,
-                                              const KeyT* __restrict__ codes,
+__device__ __forceinline__ void topology_emit(uint32_t n, uint32_t i, uint32_t first, uint32_t last,
+                                              uint32_t split, uint32_t* __restrict__ bvhnodes,
                                               const uint32_t* __restrict__ idx2obj,
                                               uint32_t obj_sub, const TreeletSink& ts,
                                               uint32_t mesh, uint32_t root_base) {
   const uint32_t nb = n - 1;
-  uint32_t first, last;
-  node_range<KeyT>(codes, n, i, first, last);
-  const uint32_t split = node_split<KeyT>(codes, first, last);
   if (ts.list) {
     if (last - first + 1 > ts.k) {
       if (ts.top_count) {
@@ -206,6 +229,18 @@ __device__ __forceinline__ void topology_node(uint32_t n, uint32_t i, uint32_t* 
   if (i == 0) bvhnodes[0] = DMSH_U32MAX;
 }
 
+template <typename KeyT>
+__device__ __forceinline__ void topology_node(uint32_t n, uint32_t i, uint32_t* __restrict__ bvhnodes,
+                                              const KeyT* __restrict__ codes,
+                                              const uint32_t* __restrict__ idx2obj,
+                                              uint32_t obj_sub, const TreeletSink& ts,
+                                              uint32_t mesh, uint32_t root_base) {
+  uint32_t first, last;
+  node_range<KeyT>(codes, n, i, first, last);
+  const uint32_t split = node_split<KeyT>(codes, first, last);
+  topology_emit(n, i, first, last, split, bvhnodes, idx2obj, obj_sub, ts, mesh, root_base);
+}
+
 // A tree of one object: the single leaf row (MAX, obj, MAX) and one treelet.
 __device__ __forceinline__ void topology_single(uint32_t* __restrict__ bvhnodes, uint32_t obj,
                                                 const TreeletSink& ts, uint32_t mesh,
@@ -217,14 +252,30 @@ __device__ __forceinline__ void topology_single(uint32_t* __restrict__ bvhnodes,
   if (ts.list) emit_treelet(ts, 0u, 0u, 0u, mesh, root_base);
 }
 
+// One thread per internal node; the block's codes and kTopoHalo either side
+// sit in LDS, where nearly every search step lands (ranges beyond the halo
+// read the rest from memory).
+constexpr uint32_t kTopoThreads = 256;
+constexpr uint32_t kTopoHalo = 256;
+
 template <typename KeyT>
-__global__ __launch_bounds__(256) void k_topology(uint32_t n, uint32_t* __restrict__ bvhnodes,
-                                                  const KeyT* __restrict__ codes,
-                                                  const uint32_t* __restrict__ idx2obj,
-                                                  TreeletSink ts) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(kTopoThreads) void k_topology(uint32_t n, uint32_t* __restrict__ bvhnodes,
+                                                           const KeyT* __restrict__ codes,
+                                                           const uint32_t* __restrict__ idx2obj,
+                                                           TreeletSink ts) {
+  __shared__ KeyT s_c[kTopoHalo + kTopoThreads + kTopoHalo + 1];
+  const uint32_t base = blockIdx.x * kTopoThreads;
+  const uint32_t lo = base > kTopoHalo ? base - kTopoHalo : 0u;
+  const uint32_t hi = min(n, base + kTopoThreads + kTopoHalo + 1);
+  for (uint32_t j = lo + threadIdx.x; j < hi; j += kTopoThreads) s_c[j - lo] = codes[j];
+  __syncthreads();
+  const uint32_t i = base + threadIdx.x;
   if (i >= n - 1) return;
-  topology_node<KeyT>(n, i, bvhnodes, codes, idx2obj, 0u, ts, 0u, 0u);
+  const CodeWindow<KeyT> c(codes, s_c, lo, hi - lo);
+  uint32_t first, last;
+  node_range<KeyT>(c, n, i, first, last);
+  const uint32_t split = node_split<KeyT>(c, first, last);
+  topology_emit(n, i, first, last, split, bvhnodes, idx2obj, 0u, ts, 0u, 0u);
 }
 
 static __global__ void k_topology_single(uint32_t* __restrict__ bvhnodes,
@@ -240,8 +291,8 @@ inline int launch_topology(hipStream_t s, uint32_t n, uint32_t* bvhnodes, const 
   if (n == 1) {
     launch("k_topology_single", k_topology_single, dim3(1), dim3(1), 0, s, bvhnodes, idx2obj, ts);
   } else {
-    launch("k_topology", k_topology<KeyT>, dim3(ceil_div(n - 1, 256)), dim3(256), 0, s, n, bvhnodes,
-           codes, idx2obj, ts);
+    launch("k_topology", k_topology<KeyT>, dim3(ceil_div(n - 1, kTopoThreads)), dim3(kTopoThreads),
+           0, s, n, bvhnodes, codes, idx2obj, ts);
   }
   return launch_status();
 }

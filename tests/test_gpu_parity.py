@@ -131,11 +131,25 @@ def test_argsort_u64(gpu, oracle, n, key_bits):
 # ---------------------------------------------------------------------------
 # topology, refit, whole build
 # ---------------------------------------------------------------------------
-@pytest.mark.parametrize("kind", ["u32", "u32dups", "u64", "u64dups"])
+@pytest.mark.parametrize("kind", ["u32", "u32dups", "u64", "u64dups", "u32runs", "u64runs",
+                                  "u64equal"])
 def test_topology(gpu, oracle, kind):
+    """Random codes, short duplicate runs, runs of 1 to 3000 equal codes (the
+    searches that the wave finishes: ranges and runs past 64 codes), and all
+    codes equal."""
     rng = np.random.default_rng(21)
     n = 50_000
-    if kind.startswith("u32"):
+    dt = np.uint32 if kind.startswith("u32") else np.uint64
+    if kind.endswith("runs"):
+        lens = rng.integers(1, 3000, 100)
+        vals = np.sort(rng.choice(1 << 30, lens.size, replace=False)).astype(dt)
+        if dt == np.uint64:
+            vals = vals << np.uint64(30)
+        codes = np.repeat(vals, lens)[:n]
+        n = codes.size
+    elif kind == "u64equal":
+        codes = np.full(n, 12345, dtype=np.uint64)
+    elif dt == np.uint32:
         hi = 5000 if kind.endswith("dups") else 1 << 30
         codes = np.sort(rng.integers(0, hi, n)).astype(np.uint32)
     else:
@@ -153,6 +167,18 @@ def test_topology_single_and_pair(gpu, oracle):
         idx = np.arange(n, dtype=np.uint32)[::-1].copy()
         got = H(Mortons.make_bvhnodes_from_sorted_mortons(G(idx, gpu), G(codes, gpu)))
         assert np.array_equal(got, oracle.bvhnodes_from_sorted(idx, codes))
+
+
+@pytest.mark.parametrize("n", [3, 64, 65, 127, 128, 129, 130, 257, 4097])
+def test_topology_sizes_near_the_serial_cut(gpu, oracle, n):
+    """Sizes around the point (128 codes) past which the root's split and
+    long ranges go to the wave-wide search, with and without equal codes."""
+    rng = np.random.default_rng(n)
+    for hi in (1 << 30, 7, 1):
+        codes = np.sort(rng.integers(0, hi, n)).astype(np.uint32)
+        idx = rng.permutation(n).astype(np.uint32)
+        got = H(Mortons.make_bvhnodes_from_sorted_mortons(G(idx, gpu), G(codes, gpu)))
+        assert np.array_equal(got, oracle.bvhnodes_from_sorted(idx, codes)), hi
 
 
 MESHES = {
