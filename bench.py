@@ -56,7 +56,7 @@ def parse():
     ap.add_argument("--batch-reps", type=int, default=3)
     ap.add_argument("--serve-requests", type=int, default=2000,
                     help="requests per arrival rate of the served workload (0 skips it)")
-    ap.add_argument("--serve-rates", default="1000,4000,20000",
+    ap.add_argument("--serve-rates", default="1000,4000,10000,20000",
                     help="arrival rates, requests/s")
     ap.add_argument("--serve-budget-mib", type=int, default=4096)
     ap.add_argument("--diff-reps", type=int, default=5,
@@ -125,7 +125,8 @@ PROFILED_KERNELS = [
     ("build", "k_centroid_aabb<true>"), ("build", "k_morton_hist"), ("build", "k_onesweep"),
     ("build", "k_sort_finish"), ("build", "k_topology"), ("build", "k_refit_treelet<1,"),
     ("build", "k_refit_top_lds"),
-    ("refit", "k_leaf_ranges"), ("refit", "k_list_treelets"), ("refit", "k_refit_treelet<1,"),
+    ("refit", "k_plan_hash"), ("refit", "k_leaf_ranges"), ("refit", "k_list_treelets"),
+    ("refit", "k_refit_treelet<1,"),
     ("refit", "k_refit_top_lds"),
 ]
 NUM_CU = 256
@@ -224,7 +225,7 @@ def summarize_counters(folders, title: str) -> str:
         gui = val(key, "GRBM_GUI_ACTIVE")
         lds = val(key, "SQ_INSTS_LDS")
         us = _avg(dur.get(key, [])) * 1e-3
-        label = short.rstrip("<,")
+        label = short.split('<')[0]
         lines.append(
             f"{phase:<6}{label:<24}{us:>8.1f}{l2_read(key) / 1e6:>8.2f}{l2_write(key) / 1e6:>8.2f}"
             f"{dram_read(key) / 1e6:>8.2f}{dram_write(key) / 1e6:>8.2f}"
@@ -241,7 +242,7 @@ def summarize_counters(folders, title: str) -> str:
         if key:
             hit, miss = val(key, "TCC_HIT_sum"), val(key, "TCC_MISS_sum")
             gui = val(key, "GRBM_GUI_ACTIVE")
-            lines.append(f"per_launch {phase}:{short.rstrip('<,')} "
+            lines.append(f"per_launch {phase}:{short.split('<')[0]} "
                          f"dram_bytes={dram_read(key) + dram_write(key):.6e} "
                          f"l2_fabric_bytes={l2_read(key) + l2_write(key):.6e} "
                          f"l2_hit_pct={100 * hit / (hit + miss):.2f} "
@@ -628,6 +629,8 @@ def run_served(num_req: int, rates, budget_bytes: int):
     building trees on first sight into a cache of `budget_bytes`."""
     import copy
 
+    import numpy as np
+
     import del_msh_dlpack.Serve.torch as Serve
 
     cat = Serve.MeshCatalog(tri_range=(1000, 1_000_000), seed=7)
@@ -644,14 +647,33 @@ def run_served(num_req: int, rates, budget_bytes: int):
         srv = Serve.Server(cat, budget_bytes)
         steps = srv.run(served)
         s_stats = Serve.latency_stats(served)
+        host = np.array(srv.host_s) * 1e3
         s_stats.update({"steps": steps, "cache_hits": srv.cache.hits,
-                        "cache_misses": srv.cache.misses, "evictions": srv.cache.evicted})
+                        "cache_misses": srv.cache.misses, "evictions": srv.cache.evicted,
+                        "build": Serve.build_stats(srv.builds),
+                        "build_without_pack": Serve.build_stats(srv.builds_only),
+                        # share of the span the GPU stream was working on steps
+                        # (near 1: the GPU, not the host loop, sets the pace)
+                        "gpu_busy_share": round(Serve.gpu_busy_s(srv.step_gpu)
+                                                / s_stats["span_s"], 4),
+                        # host time to admit, build-enqueue and cast-enqueue a step
+                        "host_ms_per_step": {"mean": round(float(host.mean()), 3),
+                                             "p99": round(float(np.percentile(host, 99)), 3),
+                                             "share_of_span": round(float(host.sum() * 1e-3
+                                                                          / s_stats["span_s"]), 4)}})
         plain = copy.deepcopy(reqs)
         psrv = Serve.PlainServer(cat, budget_bytes)
         psrv.run(plain)
         p_stats = Serve.latency_stats(plain)
-        p_stats.update({"cache_hits": psrv.cache.hits, "cache_misses": psrv.cache.misses})
-        out["rates"][str(rate)] = {"served": s_stats, "plain": p_stats,
+        p_stats.update({"cache_hits": psrv.cache.hits, "cache_misses": psrv.cache.misses,
+                        "build": Serve.build_stats(psrv.builds)})
+        ratio = {"mrays_per_s": round(s_stats["mrays_per_s"] / p_stats["mrays_per_s"], 2)}
+        if s_stats["build"].get("mtri_per_s") and p_stats["build"].get("mtri_per_s"):
+            ratio["build_mtri_per_s"] = round(s_stats["build"]["mtri_per_s"]
+                                              / p_stats["build"]["mtri_per_s"], 2)
+            ratio["build_mtri_per_s_without_pack"] = round(
+                s_stats["build_without_pack"]["mtri_per_s"] / p_stats["build"]["mtri_per_s"], 2)
+        out["rates"][str(rate)] = {"served": s_stats, "plain": p_stats, "served_over_plain": ratio,
                                    "distinct_meshes": len({r.mesh_id for r in reqs}),
                                    "host_generate_s": round(gen_s, 2)}
     return out
@@ -716,6 +738,9 @@ def main():
     if args.pmc_workload:
         r_ws = _lib.workspace(L.dmsh_refit_workspace_bytes(num_tri), dev)
         r_box = torch.empty_like(aabb)
+        # every profiled refit builds its plan (the kept-plan call runs the
+        # same kernels minus the two plan kernels)
+        assert L.dmsh_set_tuning(b"refit.plan_cache", 0) == 0
         for _ in range(3):
             build()
         for _ in range(3):
@@ -788,24 +813,34 @@ def main():
                                                 t_vtx.data_ptr(), 0.0, ws_refit.data_ptr(),
                                                 ws_refit.numel()), "refit")
 
-    for _ in range(3):
-        refit()
-    r_ev = []
-    for _ in range(20):
-        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        a.record(stream)
-        refit()
-        b.record(stream)
-        r_ev.append((a, b))
-    torch.cuda.synchronize()
-    refit_ms = float(np.median([a.elapsed_time(b) for a, b in r_ev]))
+    def refit_median(reps=20):
+        for _ in range(3):
+            refit()
+        r_ev = []
+        for _ in range(reps):
+            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            a.record(stream)
+            refit()
+            b.record(stream)
+            r_ev.append((a, b))
+        torch.cuda.synchronize()
+        return float(np.median([a.elapsed_time(b) for a, b in r_ev]))
+
+    # every call builds the plan (leaf ranges, treelet and top lists) ...
+    assert L.dmsh_set_tuning(b"refit.plan_cache", 0) == 0
+    refit_plan_ms = refit_median()
+    assert L.dmsh_set_tuning(b"refit.plan_cache", 1) == 0
+    # ... or keeps it from the previous call on the same rows (hash-checked):
+    # the optimisation-loop case, where only the vertices move
+    refit_ms = refit_median()
     _lib.timing_enable(True)
     for _ in range(10):
         refit()
     torch.cuda.synchronize()
     _lib.timing_enable(False)
     refit_kernels = {}
-    for name in ("k_leaf_ranges", "k_list_treelets", "k_refit_treelet", "k_refit_top", "k_refit"):
+    for name in ("k_plan_hash", "k_plan_gate", "k_leaf_ranges", "k_list_treelets",
+                 "k_refit_treelet", "k_refit_top", "k_refit"):
         n, total, _ = _lib.timing_query(name)
         if n:
             refit_kernels[name] = round(total / n, 5)
@@ -817,6 +852,10 @@ def main():
     refit_line = {
         "entry": "dmsh_bvhnode2aabb_trimesh3 (stored tree, treelet path)",
         "ms": round(refit_ms, 4),
+        "plan": "kept in the workspace from the previous call on the same rows (hash-checked)",
+        "ms_plan_rebuilt_every_call": round(refit_plan_ms, 4),
+        "frac_of_hbm_peak_plan_rebuilt": round(refit_bytes / (refit_plan_ms * 1e-3) / 1e9
+                                               / PEAK_HBM_GBS, 4),
         "mtri_per_s": round(num_tri / (refit_ms * 1e-3) / 1e6, 1),
         "algorithmic_gbs": round(refit_bytes / (refit_ms * 1e-3) / 1e9, 1),
         "frac_of_hbm_peak": round(refit_bytes / (refit_ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4),
@@ -824,10 +863,12 @@ def main():
         "equals_build_boxes": refit_same,
     }
     # per-kernel DRAM traffic of the standalone refit from the counter summary
-    rk = {k: read_counters(f"refit:{k}") for k in ("k_leaf_ranges", "k_list_treelets",
-                                                   "k_refit_treelet", "k_refit_top_lds")}
+    rk = {k: read_counters(f"refit:{k}") for k in ("k_plan_hash", "k_leaf_ranges",
+                                                   "k_list_treelets", "k_refit_treelet",
+                                                   "k_refit_top_lds")}
     if all(rk.values()):
-        dram = sum(v["dram_bytes"] for v in rk.values())
+        kept = ("k_plan_hash", "k_refit_treelet", "k_refit_top_lds")
+        dram = sum(v["dram_bytes"] for k, v in rk.items() if k in kept)
         refit_line["roofline"] = {
             "bound": "hbm",
             "achieved": refit_line["algorithmic_gbs"],
@@ -835,6 +876,7 @@ def main():
             "unit": "GB/s",
             "frac": refit_line["frac_of_hbm_peak"],
             "traffic": dram,
+            "traffic_plan_rebuilt": sum(v["dram_bytes"] for v in rk.values()),
             "limiter": "dependent-load and LDS-climb latency per treelet (see per kernel)",
             "per_kernel": {k: {"dram_bytes": v["dram_bytes"], "us": v["us"],
                                "dram_gbs": round(v["dram_bytes"] / (v["us"] * 1e-6) / 1e9, 1),

@@ -449,11 +449,14 @@ __device__ __forceinline__ uint32_t next_tile(const BatchCast& a, uint32_t num_t
   return num_tiles;
 }
 
-// Tier 1 over all meshes: persistent single-wave workgroups.
-template <int S>
-__global__ __launch_bounds__(64) void k_batch_pix2tri(BatchCast a) {
-  __shared__ uint2 s_stack[S][64];
-  const uint32_t lane = threadIdx.x;
+// Tier 1 over all meshes: persistent workgroups of W waves, each wave taking
+// its own tiles (no barrier between them; W is the workgroup size the
+// autotuner picks).
+template <int S, int W>
+__global__ __launch_bounds__(64 * W) void k_batch_pix2tri(BatchCast a) {
+  __shared__ uint2 s_stack_all[W][S][64];
+  uint2 (*s_stack)[64] = s_stack_all[threadIdx.x >> 6];
+  const uint32_t lane = threadIdx.x & 63u;
   if (a.counts[3]) return;  // offsets rejected
   const uint32_t num_tiles = a.tile_off[a.M];
   const bool boxes_finite = a.counts[2] == 0u;

@@ -25,21 +25,24 @@
 namespace dmsh {
 namespace bld {
 
-// The centroid box is reduced into kBoxCopies copies of (min keys, max keys)
-// (block b takes copy b mod kBoxCopies, so the blocks' atomics spread over
-// more words); min and max over order keys do not depend on order, so the
-// copies' reduction gives the same box.
-constexpr uint32_t kBoxCopies = 8;
-static_assert(6 * kBoxCopies <= 64, "the build layout keeps 64 words for the box copies");
+// The centroid box is reduced into kBoxCopies copies of (min keys, max keys),
+// one per 256-byte line (block b takes copy b mod kBoxCopies): device-scope
+// atomics on one word are performed one after the other, ~14 ns apart, so
+// 2048 blocks on one set of six words would queue for tens of microseconds.
+// Min and max over order keys do not depend on order, so the copies'
+// reduction gives the same box.
+constexpr uint32_t kBoxCopies = 64;
+constexpr uint32_t kBoxStride = 64;  // words between copies
+static_assert(kBoxCopies == 64, "k_morton_hist reduces the copies with one wave");
 
-// keys6[6c+0..2] = min order keys (init 0xFFFFFFFF), keys6[6c+3..5] = max
-// (init 0) for each copy c, then `zero_count` words starting at `zero` are
-// cleared.
+// keys6[c*kBoxStride + 0..2] = min order keys (init 0xFFFFFFFF),
+// keys6[c*kBoxStride + 3..5] = max (init 0) for each copy c, then
+// `zero_count` words starting at `zero` are cleared.
 static __global__ __launch_bounds__(256) void k_build_init(uint32_t* __restrict__ keys6,
                                                     uint32_t* __restrict__ zero,
                                                     size_t zero_count) {
   const size_t i0 = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
-  if (i0 < 6 * kBoxCopies) keys6[i0] = (i0 % 6 < 3) ? 0xFFFFFFFFu : 0u;
+  if (i0 < 6 * kBoxCopies) keys6[(i0 / 6) * kBoxStride + i0 % 6] = (i0 % 6 < 3) ? 0xFFFFFFFFu : 0u;
   for (size_t i = i0; i < zero_count; i += (size_t)gridDim.x * blockDim.x) zero[i] = 0u;
 }
 
@@ -151,7 +154,7 @@ __global__ __launch_bounds__(256) void k_centroid_aabb(uint32_t n,
       const uint32_t u = s_red[w][d];
       v = (d < 3) ? (u < v ? u : v) : (u > v ? u : v);
     }
-    uint32_t* box = keys6 + (blockIdx.x % kBoxCopies) * 6;
+    uint32_t* box = keys6 + (blockIdx.x % kBoxCopies) * kBoxStride;
     if (d < 3) atomicMin(box + d, v);
     else atomicMax(box + d, v);
   }
@@ -190,23 +193,34 @@ __global__ __launch_bounds__(256) void k_morton_hist(uint32_t n, const float* __
                                                      uint32_t num_passes, uint32_t key_bits,
                                                      uint32_t* __restrict__ hist) {
   __shared__ uint32_t s_h[sortk::kMaxPasses][sortk::kRadix];
+  __shared__ uint32_t s_k6[6];
   for (uint32_t e = threadIdx.x; e < sortk::kMaxPasses * sortk::kRadix; e += blockDim.x)
     (&s_h[0][0])[e] = 0;
+  if (threadIdx.x < 64) {  // lane c reads copy c; the wave reduces them
+    uint32_t k6[6];
+#pragma unroll
+    for (int d = 0; d < 6; ++d) k6[d] = keys6[threadIdx.x * kBoxStride + d];
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+#pragma unroll
+      for (int d = 0; d < 6; ++d) {
+        const uint32_t v = __shfl_xor(k6[d], off, 64);
+        k6[d] = d < 3 ? (v < k6[d] ? v : k6[d]) : (v > k6[d] ? v : k6[d]);
+      }
+    }
+    if (threadIdx.x == 0) {
+#pragma unroll
+      for (int d = 0; d < 6; ++d) s_k6[d] = k6[d];
+    }
+  }
+  __syncthreads();
   float T[16];
   {
     uint32_t k6[6];
 #pragma unroll
-    for (int d = 0; d < 6; ++d) k6[d] = keys6[d];
-    for (uint32_t c = 1; c < kBoxCopies; ++c) {
-#pragma unroll
-      for (int d = 0; d < 6; ++d) {
-        const uint32_t v = keys6[c * 6 + d];
-        k6[d] = d < 3 ? (v < k6[d] ? v : k6[d]) : (v > k6[d] ? v : k6[d]);
-      }
-    }
+    for (int d = 0; d < 6; ++d) k6[d] = s_k6[d];
     fit_transform(k6, T);
   }
-  __syncthreads();
   const uint32_t stride = gridDim.x * blockDim.x;
   for (uint32_t i0 = blockIdx.x * blockDim.x + threadIdx.x; i0 < n;
        i0 += stride * kCentroidItems) {
@@ -256,7 +270,7 @@ struct BuildLayout {
 template <typename KeyT>
 inline bool carve(Arena& a, uint32_t n, bool tri, BuildLayout& L) {
   const uint32_t kb = key_bits_of<KeyT>();
-  L.keys6 = a.take<uint32_t>(64);
+  L.keys6 = a.take<uint32_t>(kBoxCopies * kBoxStride);
   // hist, tickets, status and the refit counters are one contiguous zero run
   L.sort.hist = a.take<uint32_t>(sortk::kMaxPasses * sortk::kRadix + sortk::kTicketWords);
   L.sort.tickets = L.sort.hist + sortk::kMaxPasses * sortk::kRadix;
@@ -307,7 +321,7 @@ inline int build(hipStream_t s, uint32_t n, const uint32_t* tri2vtx, uint32_t nu
   const uint32_t kb = key_bits_of<KeyT>();
   const uint32_t P = sortk::num_passes(kb);
   int rc;
-  launch("k_build_init", k_build_init, dim3(std::min<uint32_t>(ceil_div(L.zero_words, 256), 2048)),
+  launch("k_build_init", k_build_init, dim3(std::max<uint32_t>(2u, std::min<uint32_t>(ceil_div(L.zero_words, 256), 2048))),
                      dim3(256), 0, s, L.keys6, L.zero_begin, L.zero_words);
   if ((rc = launch_status()) != DMSH_OK) return rc;
   launch("k_centroid_aabb", k_centroid_aabb<TRI>, dim3(std::min<uint32_t>(ceil_div(n, 256), 2048)),

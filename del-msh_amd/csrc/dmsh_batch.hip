@@ -13,7 +13,9 @@ namespace {
 constexpr uint32_t kCastCounts = 256;  // see bat::BatchCast::counts
 
 std::atomic<int> g_batch_stack{20};      // LDS frames per lane of the batched tier 1
-std::atomic<int> g_batch_wgs_per_cu{16};  // persistent single-wave workgroups per CU
+std::atomic<int> g_batch_wgs_per_cu{16};  // persistent ray-cast waves per CU
+std::atomic<int> g_batch_sort_items{16};  // elements per thread of the digit passes
+std::atomic<int> g_batch_cast_waves{1};   // waves per ray-cast workgroup
 std::atomic<int> g_batch_treelet_grid{1024};
 
 // key bits of the composite (mesh << 30 | code): 30 + the bits of M - 1
@@ -47,7 +49,9 @@ inline bool carve_batch_build(Arena& a, uint32_t M, uint32_t n, BatchBuildLayout
   L.sort.hist = a.take<uint32_t>(sortk::kMaxPasses * sortk::kRadix + sortk::kTicketWords);
   L.sort.tickets = L.sort.hist + sortk::kMaxPasses * sortk::kRadix;
   L.sort.status = L.sort.tickets + sortk::kTicketWords;
-  const size_t status_words = sortk::status_words<uint64_t>(n, kb);
+  // sized for the smallest tile, so that batch.sort_items may change
+  // between the size query and the build
+  const size_t status_words = sortk::status_words_items(n, kb, sortk::kMinItems);
   a.off += sizeof(uint32_t) * status_words;
   L.flags = L.sort.status + status_words;
   a.off += sizeof(uint32_t) * nn;
@@ -96,8 +100,20 @@ inline bool carve_batch_cast(Arena& a, uint32_t M, uint32_t n, uint32_t num_pix,
 }
 
 template <int S>
-void launch_batch_tier1(hipStream_t s, uint32_t grid, const bat::BatchCast& a) {
-  launch("k_batch_pix2tri", bat::k_batch_pix2tri<S>, dim3(grid), dim3(64), 0, s, a);
+void launch_batch_tier1(hipStream_t s, uint32_t waves, const bat::BatchCast& a) {
+  switch (g_batch_cast_waves.load()) {
+    case 2:
+      launch("k_batch_pix2tri", bat::k_batch_pix2tri<S, 2>, dim3(ceil_div(waves, 2)), dim3(128), 0,
+             s, a);
+      break;
+    case 4:
+      launch("k_batch_pix2tri", bat::k_batch_pix2tri<S, 4>, dim3(ceil_div(waves, 4)), dim3(256), 0,
+             s, a);
+      break;
+    default:
+      launch("k_batch_pix2tri", bat::k_batch_pix2tri<S, 1>, dim3(waves), dim3(64), 0, s, a);
+      break;
+  }
 }
 
 // Offsets of a batch of built trees (as for the batched cast, without images).
@@ -177,6 +193,16 @@ int dmsh_batch_set_tuning(const char* key, int value) {
     g_batch_treelet_grid.store(value);
     return DMSH_OK;
   }
+  if (std::strcmp(key, "batch.sort_items") == 0) {
+    if (!sortk::valid_items(value)) return DMSH_ERR_INVALID;
+    g_batch_sort_items.store(value);
+    return DMSH_OK;
+  }
+  if (std::strcmp(key, "batch.cast_waves") == 0) {
+    if (value != 1 && value != 2 && value != 4) return DMSH_ERR_INVALID;
+    g_batch_cast_waves.store(value);
+    return DMSH_OK;
+  }
   return DMSH_ERR_INVALID;
 }
 
@@ -185,6 +211,8 @@ int dmsh_batch_get_tuning(const char* key) {
   if (std::strcmp(key, "batch.stack") == 0) return g_batch_stack.load();
   if (std::strcmp(key, "batch.wgs_per_cu") == 0) return g_batch_wgs_per_cu.load();
   if (std::strcmp(key, "batch.treelet_grid") == 0) return g_batch_treelet_grid.load();
+  if (std::strcmp(key, "batch.sort_items") == 0) return g_batch_sort_items.load();
+  if (std::strcmp(key, "batch.cast_waves") == 0) return g_batch_cast_waves.load();
   return -1;
 }
 
@@ -235,7 +263,8 @@ int dmsh_batch_build_trimesh3(void* stream, uint32_t num_mesh, uint32_t total_tr
   if ((rc = launch_status()) != DMSH_OK) return rc;
   // plain passes: the top digits of a composite key are mostly the mesh index,
   // so groups of equal top bits are too long for the LDS finish
-  rc = sortk::run_passes<uint64_t>(s, n, kb, L.sort, morton_out, L.codes, L.idx, err, false);
+  rc = sortk::run_passes<uint64_t>(s, n, kb, L.sort, morton_out, L.codes, L.idx, err, false,
+                                   nullptr, nullptr, g_batch_sort_items.load());
   if (rc != DMSH_OK) return rc;
   const TreeletSink ts{kTreeletLeaves, L.treelet_count, L.treelets, n, L.root_count, L.roots};
   launch("k_topology", bat::k_batch_topology, dim3(ceil_div(n, 256)), dim3(256), 0, s, L.bad, n,

@@ -207,7 +207,8 @@ enum RefitOption {
   kRefitClimb = 1,
   kRefitQueue = 2,
   kRefitTop = 3,
-  kRefitOptionCount = 4
+  kRefitPlanCache = 4,
+  kRefitOptionCount = 5
 };
 int refit_option(RefitOption which);
 bool set_refit_option(const char* key, int value);

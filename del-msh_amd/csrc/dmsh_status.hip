@@ -23,10 +23,12 @@ static std::mutex g_err_mutex;
 // box; 1: one LDS round trip per level), "refit.queue" (1: workgroups take
 // treelets from a shared counter; 0: treelet t goes to workgroup t mod grid),
 // "refit.top" (0: the nodes above the treelets climb through LDS counters
-// when there are at most 4096 of them; 1: they merge in barrier rounds)
-static std::atomic<int> g_refit_option[kRefitOptionCount] = {{0}, {1}, {1}, {0}};
+// when there are at most 4096 of them; 1: they merge in barrier rounds),
+// "refit.plan_cache" (1: a standalone refit whose workspace already holds
+// the plan of the same rows skips the plan kernels; 0: always rebuild it)
+static std::atomic<int> g_refit_option[kRefitOptionCount] = {{0}, {1}, {1}, {0}, {1}};
 static const char* const kRefitOptionKey[kRefitOptionCount] = {
-    "refit.leaves_in_aabb", "refit.climb", "refit.queue", "refit.top"};
+    "refit.leaves_in_aabb", "refit.climb", "refit.queue", "refit.top", "refit.plan_cache"};
 
 int refit_option(RefitOption which) { return g_refit_option[which].load(); }
 

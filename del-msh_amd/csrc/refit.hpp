@@ -832,7 +832,13 @@ __global__ __launch_bounds__(T) void k_refit_top_lds(
   for (uint32_t t = 1; __syncthreads_or(pending != 0u); ++t) {
     if (t > count || t >= kTopPending) {
       // a top tree higher than the done[] rounds can count (or a corrupt
-      // one): redo the nodes above the treelets with the global climb
+      // one): redo the nodes above the treelets with the global climb, whose
+      // arrival counters (the top nodes', all listed here) start at zero
+      for (uint32_t e = threadIdx.x; e < count; e += T) {
+        const uint32_t node = top_list[e].x;
+        if (node < nb) flags[node] = 0u;
+      }
+      __syncthreads();
       uint32_t nt = *treelet_count;
       if (nt > num_leaf) nt = num_leaf;
       refit_top<T>(v, flags, reinterpret_cast<const uint32_t*>(treelets), 4u, nt, err);
@@ -929,14 +935,17 @@ inline int launch_refit_treelets(hipStream_t s, const TreeView& v, uint32_t* fla
 //   every Karras tree) -- and lists the treelets and top nodes as the
 //   topology kernel does.  One failed check sets *bad: the treelet kernels
 //   then skip and the generic climb (gated on *bad) refits the tree instead.
+constexpr uint32_t kPlanHashBlocks = 1024;  // row-hash partials (see k_plan_hash)
+
 struct StandaloneLayout {
   uint32_t* flags;     // num_leaf - 1 arrival counters
   uint32_t* first;     // num_leaf - 1
   uint32_t* last;      // num_leaf - 1
   uint32_t* node2slot; // num_leaf - 1
-  uint32_t* counters;  // treelet count, top count, next treelet, bad
+  uint32_t* counters;  // treelet count, top count, next treelet, bad, plan needed
   uint4* treelets;     // num_leaf
   uint2* top_list;     // kTopSlots
+  unsigned long long* hash;  // -, stamp of the kept plan, tag, -, row-hash partials
 };
 
 inline bool carve_standalone(Arena& a, uint32_t n, StandaloneLayout& L) {
@@ -945,10 +954,85 @@ inline bool carve_standalone(Arena& a, uint32_t n, StandaloneLayout& L) {
   L.first = a.take<uint32_t>(nb);
   L.last = a.take<uint32_t>(nb);
   L.node2slot = a.take<uint32_t>(nb);
-  L.counters = a.take<uint32_t>(4);
+  L.counters = a.take<uint32_t>(8);
   L.treelets = a.take<uint4>(n ? n : 1);
   L.top_list = a.take<uint2>(kTopSlots);
+  L.hash = a.take<unsigned long long>(4 + kPlanHashBlocks);
   return a.ok();
+}
+
+// The plan -- leaf ranges, treelet and top lists -- depends on the rows
+// only, so a workspace that is used again for the same rows keeps it:
+// k_plan_hash sums a mixed hash of every (word index, word) of bvhnodes,
+// k_plan_gate compares the sum with the stamp kept beside the plan and sets
+// the gate of the two plan kernels when they differ (or when the workspace
+// has never held a plan, the tree failed the layout check, or its top list
+// overflowed).  A workspace fresh from an allocator holds no valid tag, so
+// its first call builds the plan.
+constexpr unsigned long long kPlanTag = 0x646d73685f706c6eull;
+
+__device__ __forceinline__ unsigned long long mix64(unsigned long long z) {
+  // splitmix64 finaliser
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+  return z ^ (z >> 31);
+}
+
+// Per word w with value x: (x * A + w * B) with a xor-shift, summed mod 2^64
+// (one 32x64 multiply-add per word; the index term keeps a permutation of
+// the rows from summing to the same value).  Four words per load where the
+// rows are 16-byte aligned.
+__device__ __forceinline__ unsigned long long word_hash(unsigned long long w, uint32_t x) {
+  unsigned long long v = (unsigned long long)x * 0x9e3779b97f4a7c15ull + w * 0xc2b2ae3d27d4eb4full;
+  return v ^ (v >> 29);
+}
+
+static __global__ __launch_bounds__(256) void k_plan_hash(const uint32_t* __restrict__ words, size_t num_words,
+                                                          unsigned long long* __restrict__ part) {
+  __shared__ unsigned long long s_part[4];
+  unsigned long long h = 0;
+  const size_t stride = (size_t)gridDim.x * 256;
+  if ((reinterpret_cast<uintptr_t>(words) & 15u) == 0) {
+    const uint4* q = reinterpret_cast<const uint4*>(words);
+    const size_t nq = num_words / 4;
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < nq; i += stride) {
+      const uint4 x = q[i];
+      const unsigned long long w = 4 * (unsigned long long)i;
+      h += word_hash(w, x.x) + word_hash(w + 1, x.y) + word_hash(w + 2, x.z) +
+           word_hash(w + 3, x.w);
+    }
+    for (size_t w = nq * 4 + (size_t)blockIdx.x * 256 + threadIdx.x; w < num_words; w += stride)
+      h += word_hash(w, words[w]);
+  } else {
+    for (size_t w = (size_t)blockIdx.x * 256 + threadIdx.x; w < num_words; w += stride)
+      h += word_hash(w, words[w]);
+  }
+  for (int o = 32; o > 0; o >>= 1) h += __shfl_xor(h, o);
+  if ((threadIdx.x & 63u) == 0) s_part[threadIdx.x >> 6] = h;
+  __syncthreads();
+  // one partial per block (the gate adds them): thousands of blocks adding
+  // into one word would queue one device-scope atomic behind the other
+  if (threadIdx.x == 0) part[blockIdx.x] = s_part[0] + s_part[1] + s_part[2] + s_part[3];
+}
+
+static __global__ __launch_bounds__(256) void k_plan_gate(unsigned long long* __restrict__ hash,
+                                                          uint32_t* __restrict__ counters,
+                                                          uint32_t num_leaf, uint32_t num_part,
+                                                          int cache) {
+  __shared__ unsigned long long s_part[4];
+  unsigned long long h = 0;
+  for (uint32_t b = threadIdx.x; b < num_part; b += 256) h += hash[4 + b];
+  for (int o = 32; o > 0; o >>= 1) h += __shfl_xor(h, o);
+  if ((threadIdx.x & 63u) == 0) s_part[threadIdx.x >> 6] = h;
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  h = (s_part[0] + s_part[1] + s_part[2] + s_part[3]) ^ mix64(num_leaf);
+  const bool keep = cache && hash[2] == kPlanTag && hash[1] == h && counters[3] == 0u &&
+                    counters[1] <= (uint32_t)kTopSlots && counters[0] <= num_leaf;
+  counters[4] = keep ? 0u : 1u;
+  counters[2] = 0u;  // the treelet kernel's ticket counter
+  hash[1] = h;
+  hash[2] = kPlanTag;
 }
 
 inline size_t standalone_workspace_bytes(uint32_t n) {
@@ -977,13 +1061,14 @@ __global__ __launch_bounds__(256) void k_leaf_ranges(TreeView v, float eps, int 
                                                      uint32_t* __restrict__ flags,
                                                      uint32_t* __restrict__ counters,
                                                      uint32_t* __restrict__ err) {
+  if (counters[4] == 0u) return;  // the kept plan is valid
   const uint32_t num_leaf = v.num_leaf;
   const uint32_t* __restrict__ bvhnodes = v.bvhnodes;
   const uint32_t nb = num_leaf - 1;
   const uint32_t k0 = blockIdx.x * (256u * kRangeWalks) + threadIdx.x;
   if (k0 == 0) {
 #pragma unroll
-    for (int c = 0; c < 4; ++c) counters[c] = 0u;
+    for (int c = 0; c < 4; ++c) counters[c] = 0u;  // not counters[4]: read by every block
   }
 #pragma unroll
   for (int r = 0; r < kRangeWalks; ++r) {
@@ -1046,6 +1131,7 @@ __global__ __launch_bounds__(T) void k_list_treelets(
     uint4* __restrict__ treelets, uint2* __restrict__ top_list, uint32_t* __restrict__ node2slot) {
   __shared__ uint32_t s_cnt[2];  // treelets, tops of this round
   __shared__ unsigned long long s_base;
+  if (counters[4] == 0u) return;  // the kept plan is valid
   const uint32_t nb = num_leaf - 1;
   const uint32_t K = kTreeletLeaves;
   for (uint32_t base = blockIdx.x * T; base < nb; base += gridDim.x * T) {
@@ -1098,8 +1184,9 @@ __global__ __launch_bounds__(T) void k_list_treelets(
   }
 }
 
-// ws: standalone_workspace_bytes(v.num_leaf).  Five launches: ranges, list,
-// treelets, top (more than kTreeletLeaves leaves), gated generic climb.
+// ws: standalone_workspace_bytes(v.num_leaf).  Launches: row hash and plan
+// gate, ranges and list (gated: idle when the workspace keeps this tree's
+// plan), treelets, top (more than kTreeletLeaves leaves), gated generic climb.
 template <int LK>
 inline int launch_refit_standalone(hipStream_t s, const TreeView& v, float eps, void* ws,
                                    size_t ws_bytes, uint32_t* err) {
@@ -1114,6 +1201,14 @@ inline int launch_refit_standalone(hipStream_t s, const TreeView& v, float eps, 
     return launch_status();
   }
   const bool leaves_in_aabb = refit_option(kRefitLeavesInAabb) == 1;
+  // leaf boxes written by k_leaf_ranges change with the geometry: no kept plan then
+  const int cache = (refit_option(kRefitPlanCache) == 1 && !leaves_in_aabb) ? 1 : 0;
+  const size_t num_words = (size_t)(2 * (size_t)n - 1) * 3;
+  const uint32_t hash_blocks = std::min<uint32_t>(ceil_div(num_words, 1024), kPlanHashBlocks);
+  launch("k_plan_hash", k_plan_hash, dim3(hash_blocks), dim3(256), 0, s, v.bvhnodes, num_words,
+         L.hash + 4);
+  launch("k_plan_gate", k_plan_gate, dim3(1), dim3(256), 0, s, L.hash, L.counters, n, hash_blocks,
+         cache);
   launch("k_leaf_ranges", k_leaf_ranges<LK>, dim3(ceil_div(n, 256 * kRangeWalks)), dim3(256), 0, s,
          v, eps, leaves_in_aabb ? 1 : 0, L.first, L.last, L.flags, L.counters, err);
   int rc = launch_status();

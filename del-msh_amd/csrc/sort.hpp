@@ -440,12 +440,24 @@ inline uint32_t num_tiles(uint32_t n) {
   return ceil_div(n, (uint64_t)kThreads * Tune<KeyT>::kItems);
 }
 
+// Elements per thread of a digit pass: the default (Tune), or 8 / 12 / 16 for
+// callers that tune it (the batched build); a workspace used with more than
+// one value is sized for the smallest, which has the most tiles.
+constexpr int kMinItems = 8;
+inline bool valid_items(int items) { return items == 8 || items == 12 || items == 16; }
+inline uint32_t num_tiles_items(uint32_t n, int items) {
+  return ceil_div(n, (uint64_t)kThreads * (uint64_t)items);
+}
+
 inline uint32_t num_passes(uint32_t key_bits) { return ceil_div(key_bits, kRadixBits); }
 
 // Status words of every pass that may run (see status_passes).
 template <typename KeyT>
 inline size_t status_words(uint32_t n, uint32_t key_bits) {
   return (size_t)status_passes(num_passes(key_bits)) * num_tiles<KeyT>(n) * kRadix;
+}
+inline size_t status_words_items(uint32_t n, uint32_t key_bits, int items) {
+  return (size_t)status_passes(num_passes(key_bits)) * num_tiles_items(n, items) * kRadix;
 }
 
 // Workspace: hist[8][256] + tickets/gate[16] + status[passes][tiles][256] +
@@ -490,13 +502,12 @@ inline int launch_fill(hipStream_t s, uint32_t* p, size_t count, uint32_t value)
 
 // One digit pass over (sk, sv) into (dk, dv); `slot` picks its status region
 // and ticket.  iota: the values are the element indices.
-template <typename KeyT>
-inline int launch_pass(hipStream_t s, uint32_t n, uint32_t key_bits, const Layout& L,
-                       uint32_t digit, uint32_t slot, bool iota, const KeyT* sk,
-                       const uint32_t* sv, KeyT* dk, uint32_t* dv, const uint32_t* gate,
-                       uint32_t* err, const uint32_t* n_dev = nullptr) {
-  constexpr int ITEMS = Tune<KeyT>::kItems;
-  const uint32_t tiles = num_tiles<KeyT>(n);
+template <typename KeyT, int ITEMS>
+inline int launch_pass_items(hipStream_t s, uint32_t n, uint32_t key_bits, const Layout& L,
+                             uint32_t digit, uint32_t slot, bool iota, const KeyT* sk,
+                             const uint32_t* sv, KeyT* dk, uint32_t* dv, const uint32_t* gate,
+                             uint32_t* err, const uint32_t* n_dev) {
+  const uint32_t tiles = num_tiles_items(n, ITEMS);
   const uint32_t shift = digit * kRadixBits;
   const uint32_t dbits = pass_bits(key_bits, digit);
   uint32_t* st = L.status + (size_t)slot * tiles * kRadix;
@@ -509,6 +520,27 @@ inline int launch_pass(hipStream_t s, uint32_t n, uint32_t key_bits, const Layou
            dv, n, n_dev, shift, dbits, L.hist + digit * kRadix, st, L.tickets + slot, gate, err);
   }
   return launch_status();
+}
+
+// items: elements per thread (0: the default of KeyT).
+template <typename KeyT>
+inline int launch_pass(hipStream_t s, uint32_t n, uint32_t key_bits, const Layout& L,
+                       uint32_t digit, uint32_t slot, bool iota, const KeyT* sk,
+                       const uint32_t* sv, KeyT* dk, uint32_t* dv, const uint32_t* gate,
+                       uint32_t* err, const uint32_t* n_dev = nullptr, int items = 0) {
+  switch (items ? items : Tune<KeyT>::kItems) {
+    case 8:
+      return launch_pass_items<KeyT, 8>(s, n, key_bits, L, digit, slot, iota, sk, sv, dk, dv,
+                                        gate, err, n_dev);
+    case 12:
+      return launch_pass_items<KeyT, 12>(s, n, key_bits, L, digit, slot, iota, sk, sv, dk, dv,
+                                         gate, err, n_dev);
+    case 16:
+      return launch_pass_items<KeyT, 16>(s, n, key_bits, L, digit, slot, iota, sk, sv, dk, dv,
+                                         gate, err, n_dev);
+    default:
+      return DMSH_ERR_INVALID;
+  }
 }
 
 // Sorts the keys in `first_src` (values: the element indices).  With `finish`
@@ -529,7 +561,7 @@ template <typename KeyT>
 inline int run_passes(hipStream_t s, uint32_t n, uint32_t key_bits, const Layout& L,
                       const KeyT* first_src, KeyT* dst_keys, uint32_t* dst_vals, uint32_t* err,
                       bool finish, const uint32_t* n_dev = nullptr,
-                      const uint32_t* first_vals = nullptr) {
+                      const uint32_t* first_vals = nullptr, int items = 0) {
   const uint32_t P = num_passes(key_bits);
   KeyT* alt_k = static_cast<KeyT*>(L.alt_keys);
   uint32_t* alt_v = L.alt_vals;
@@ -542,14 +574,14 @@ inline int run_passes(hipStream_t s, uint32_t n, uint32_t key_bits, const Layout
       KeyT* dk = to_user ? dst_keys : alt_k;
       uint32_t* dv = to_user ? dst_vals : alt_v;
       rc = launch_pass<KeyT>(s, n, key_bits, L, p, p, p == 0 && first_vals == nullptr, src_k,
-                             src_v, dk, dv, nullptr, err, n_dev);
+                             src_v, dk, dv, nullptr, err, n_dev, items);
       if (rc != DMSH_OK) return rc;
       src_k = dk;
       src_v = dv;
     }
     return DMSH_OK;
   }
-  if (first_src == alt_k || n_dev || first_vals) return DMSH_ERR_INVALID;
+  if (first_src == alt_k || n_dev || first_vals || items) return DMSH_ERR_INVALID;
   const uint32_t d0 = P - kTopPasses;
   if ((rc = launch_pass<KeyT>(s, n, key_bits, L, d0, 0, true, first_src, nullptr, alt_k, alt_v,
                               nullptr, err)) != DMSH_OK)

@@ -257,40 +257,63 @@ def _time_ms(fn, reps: int) -> float:
 
 
 def autotune(p: Pipeline, reps: int = 2, stacks=(16, 20, 24, 32), wgs_per_cu=(4, 8, 16, 32),
-             treelet_grids=(512, 1024, 2048)):
+             cast_waves=(1, 2, 4), treelet_grids=(512, 1024, 2048), sort_items=(8, 12, 16)):
     """Times the launch parameters of the batched kernels on this batch and
     keeps the fastest (results do not depend on them).  The ray-cast is tuned
-    over (LDS stack frames, persistent workgroups per CU), the build over the
-    refit grid.  Returns {"cast": {...}, "build": {...}, "trials": [...]}."""
+    over (LDS stack frames, ray-cast waves per CU), then over the workgroup
+    size (waves per workgroup) at that pair; the build over (refit grid,
+    elements per thread of the radix-sort digit passes).  Returns
+    {"cast": {...}, "build": {...}, "trials": [...]}."""
     L = _lib.lib()
     trials = []
+
+    def set_(key, value):
+        _lib.check(L.dmsh_batch_set_tuning(key.encode(), value), "tuning")
+
     best_cast = None
     if p.has_cast:
         p.build()
+        set_("batch.cast_waves", 1)
         for s in stacks:
             for w in wgs_per_cu:
-                _lib.check(L.dmsh_batch_set_tuning(b"batch.stack", s), "tuning")
-                _lib.check(L.dmsh_batch_set_tuning(b"batch.wgs_per_cu", w), "tuning")
+                set_("batch.stack", s)
+                set_("batch.wgs_per_cu", w)
                 ms = _time_ms(p.cast, reps)
                 trials.append({"stage": "cast", "batch.stack": s, "batch.wgs_per_cu": w,
-                               "ms": round(ms, 4)})
+                               "batch.cast_waves": 1, "ms": round(ms, 4)})
                 if best_cast is None or ms < best_cast[0]:
-                    best_cast = (ms, s, w)
-        _lib.check(L.dmsh_batch_set_tuning(b"batch.stack", best_cast[1]), "tuning")
-        _lib.check(L.dmsh_batch_set_tuning(b"batch.wgs_per_cu", best_cast[2]), "tuning")
+                    best_cast = (ms, s, w, 1)
+        set_("batch.stack", best_cast[1])
+        set_("batch.wgs_per_cu", best_cast[2])
+        for cw in cast_waves:
+            if cw == 1:
+                continue
+            set_("batch.cast_waves", cw)
+            ms = _time_ms(p.cast, reps)
+            trials.append({"stage": "cast", "batch.stack": best_cast[1],
+                           "batch.wgs_per_cu": best_cast[2], "batch.cast_waves": cw,
+                           "ms": round(ms, 4)})
+            if ms < best_cast[0]:
+                best_cast = (ms, best_cast[1], best_cast[2], cw)
+        set_("batch.cast_waves", best_cast[3])
     best_build = None
     for g in treelet_grids:
-        _lib.check(L.dmsh_batch_set_tuning(b"batch.treelet_grid", g), "tuning")
-        ms = _time_ms(p.build, reps)
-        trials.append({"stage": "build", "batch.treelet_grid": g, "ms": round(ms, 4)})
-        if best_build is None or ms < best_build[0]:
-            best_build = (ms, g)
-    _lib.check(L.dmsh_batch_set_tuning(b"batch.treelet_grid", best_build[1]), "tuning")
-    out = {"build": {"batch.treelet_grid": best_build[1], "ms": round(best_build[0], 4)},
+        for it in sort_items:
+            set_("batch.treelet_grid", g)
+            set_("batch.sort_items", it)
+            ms = _time_ms(p.build, reps)
+            trials.append({"stage": "build", "batch.treelet_grid": g, "batch.sort_items": it,
+                           "ms": round(ms, 4)})
+            if best_build is None or ms < best_build[0]:
+                best_build = (ms, g, it)
+    set_("batch.treelet_grid", best_build[1])
+    set_("batch.sort_items", best_build[2])
+    out = {"build": {"batch.treelet_grid": best_build[1], "batch.sort_items": best_build[2],
+                     "ms": round(best_build[0], 4)},
            "trials": trials}
     if best_cast:
         out["cast"] = {"batch.stack": best_cast[1], "batch.wgs_per_cu": best_cast[2],
-                       "ms": round(best_cast[0], 4)}
+                       "batch.cast_waves": best_cast[3], "ms": round(best_cast[0], 4)}
     return out
 
 

@@ -24,8 +24,9 @@ def bvhnode2aabb_from_bvhnodes(elem2idx_offset, idx2vtx, vtx2xyz, bvhnodes, bvhn
     dev = _lib.require_gpu(*ts)
     num_elem = elem2idx_offset.shape[0] - 1
     L = _lib.lib()
-    ws = _lib.workspace(L.dmsh_polyhedron_refit_workspace_bytes(num_elem), dev)
     stream = stream_ptr or _lib.stream_of(dev)
+    ws = _lib.kept_workspace("dmsh_bvhnode2aabb_polyhedron",
+                             L.dmsh_polyhedron_refit_workspace_bytes(num_elem), dev, stream)
     _lib.check(L.dmsh_bvhnode2aabb_polyhedron(
         stream, bvhnodes.shape[0], _lib.ptr(bvhnodes), num_elem, _lib.ptr(elem2idx_offset),
         idx2vtx.shape[0], _lib.ptr(idx2vtx), vtx2xyz.shape[0], _lib.ptr(vtx2xyz),

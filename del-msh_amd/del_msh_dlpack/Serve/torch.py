@@ -116,6 +116,37 @@ def _upload(a: np.ndarray, dev) -> torch.Tensor:
     return torch.from_numpy(np.ascontiguousarray(a)).pin_memory().to(dev, non_blocking=True)
 
 
+class PinnedRing:
+    """Page-locked staging slots for the per-step uploads (request tables,
+    mesh offsets, buffer addresses), allocated once and reused: a slot is
+    handed out again only after the copy that read it has run (an event
+    recorded behind the copy on the stream)."""
+
+    def __init__(self, slots: int = 8, nbytes: int = 1 << 16):
+        self.bufs = [torch.empty(nbytes, dtype=torch.uint8).pin_memory() for _ in range(slots)]
+        self.events = [None] * slots
+        self.k = 0
+
+    def upload(self, a: np.ndarray, dev, stream) -> torch.Tensor:
+        raw = np.ascontiguousarray(a).view(np.uint8).reshape(-1)
+        k = self.k
+        self.k = (k + 1) % len(self.bufs)
+        if self.events[k] is not None:
+            self.events[k].synchronize()
+        if self.bufs[k].numel() < raw.size:
+            self.bufs[k] = torch.empty(max(raw.size, 2 * self.bufs[k].numel()),
+                                       dtype=torch.uint8).pin_memory()
+        host = self.bufs[k][:raw.size]
+        host.numpy()[:] = raw
+        out = torch.empty(raw.size, dtype=torch.uint8, device=dev)
+        with torch.cuda.stream(stream):
+            out.copy_(host, non_blocking=True)
+            ev = torch.cuda.Event()
+            ev.record(stream)
+        self.events[k] = ev
+        return out
+
+
 class Server:
     def __init__(self, catalog: MeshCatalog, budget_bytes: int, device=None,
                  max_step_requests: int = 256, max_step_pixels: int = 1 << 27):
@@ -128,6 +159,13 @@ class Server:
         self.stream = torch.cuda.current_stream(self.device)
         # uploads of new meshes run here, beside the previous step's casts
         self.copy_stream = torch.cuda.Stream(device=self.device)
+        self.ring = PinnedRing()
+        # per step: host seconds spent preparing and enqueueing it; GPU build
+        # (+ pack) events and the triangles they built
+        self.host_s: list = []
+        self.builds: list = []       # build + pack into the traversal layout
+        self.builds_only: list = []  # the batched build alone
+        self.step_gpu: list = []     # (event before the step's work, event after its casts)
 
     def tree_bytes(self, mid: int) -> int:
         return int(self.L.dmsh_packed_bytes(self.catalog.num_tri(mid)))
@@ -155,8 +193,8 @@ class Server:
             for k, (t, v) in enumerate(meshes):
                 tri[int(m2t[k]):int(m2t[k + 1])].copy_(t, non_blocking=True)
                 vtx[int(m2v[k]):int(m2v[k + 1])].copy_(v, non_blocking=True)
-            d_m2t = _upload(m2t, dev)
-            d_m2v = _upload(m2v, dev)
+            d_m2t = self.ring.upload(m2t, dev, self.copy_stream).view(torch.uint32)
+            d_m2v = self.ring.upload(m2v, dev, self.copy_stream).view(torch.uint32)
             uploaded = torch.cuda.Event()
             uploaded.record(self.copy_stream)
         self.stream.wait_event(uploaded)
@@ -168,19 +206,28 @@ class Server:
         M, T, V = len(new_ids), int(m2t[-1]), int(m2v[-1])
         ws = _lib.workspace(L.dmsh_batch_build_workspace_bytes(M, T), dev)
         sp = self.stream.cuda_stream
+        b0 = torch.cuda.Event(enable_timing=True)
+        b0.record(self.stream)
         _lib.check(L.dmsh_batch_build_trimesh3(sp, M, T, V, _lib.ptr(d_m2t), _lib.ptr(d_m2v),
                                                _lib.ptr(tri), _lib.ptr(vtx), _lib.ptr(m2n),
                                                _lib.ptr(nodes), _lib.ptr(aabb), _lib.ptr(ws),
                                                ws.numel()), "batch build")
+        bm = torch.cuda.Event(enable_timing=True)
+        bm.record(self.stream)
         bufs = [torch.empty(int(L.dmsh_packed_bytes(int(n))), dtype=torch.uint8, device=dev)
                 for n in ntri]
-        addrs = _upload(np.array([b.data_ptr() for b in bufs], np.uint64), dev)
+        addrs = self.ring.upload(np.array([b.data_ptr() for b in bufs], np.uint64), dev,
+                                 self.stream)
         nonfinite = torch.empty(M, dtype=torch.uint32, device=dev)
         wsp = _lib.workspace(L.dmsh_batch_pack_workspace_bytes(M), dev)
         _lib.check(L.dmsh_batch_pack(sp, M, T, V, _lib.ptr(d_m2t), _lib.ptr(d_m2v), _lib.ptr(tri),
                                      _lib.ptr(vtx), _lib.ptr(m2n), _lib.ptr(nodes), _lib.ptr(aabb),
                                      _lib.ptr(addrs), _lib.ptr(nonfinite), _lib.ptr(wsp),
                                      wsp.numel()), "batch pack")
+        b1 = torch.cuda.Event(enable_timing=True)
+        b1.record(self.stream)
+        self.builds.append((b0, b1, T))
+        self.builds_only.append((b0, bm, T))
         # the tree's own non-finite flag is in its buffer; staging tensors are
         # freed here and reused only by later work on the same stream
         for k, mid in enumerate(new_ids):
@@ -193,13 +240,18 @@ class Server:
         record (outputs, event)."""
         L, dev = self.L, self.device
         recs = np.zeros(len(batch), REQ_DTYPE)
-        start = 0
-        for k, r in enumerate(batch):
-            buf, n, nf = self.cache.entries[r.mesh_id].value
-            recs[k] = (buf.data_ptr(), n, nf, r.width, r.height, start, 0,
-                       np.asarray(r.ndc2world, np.float32).T.reshape(-1))
-            start += r.width * r.height
-        d_req = _upload(recs.view(np.uint8), dev)
+        vals = [self.cache.entries[r.mesh_id].value for r in batch]
+        px = np.array([r.width * r.height for r in batch], np.int64)
+        recs["packed"] = [v[0].data_ptr() for v in vals]
+        recs["num_tri"] = [v[1] for v in vals]
+        recs["nonfinite"] = [v[2] for v in vals]
+        recs["img_w"] = [r.width for r in batch]
+        recs["img_h"] = [r.height for r in batch]
+        recs["pix_start"] = np.concatenate([[0], np.cumsum(px)[:-1]])
+        recs["ndc2world"] = np.stack([np.asarray(r.ndc2world, np.float32).T.reshape(-1)
+                                      for r in batch])
+        start = int(px.sum())
+        d_req = self.ring.upload(recs, dev, self.stream)
         pix = torch.empty(max(start, 1), dtype=torch.uint32, device=dev)
         pt = torch.empty(max(start, 1), dtype=torch.float32, device=dev)
         ws = _lib.workspace(L.dmsh_cast_requests_workspace_bytes(len(batch), start), dev)
@@ -248,11 +300,17 @@ class Server:
                 s = inflight.popleft()
                 self.retire(s, done_time(s["event"]), keep_results)
             if queue and len(inflight) < max_inflight:
+                h0 = time.perf_counter()
                 batch, new_ids, planned = admit(queue, self.cache, self.tree_bytes,
                                                 self.max_step_requests, self.max_step_pixels)
                 if batch:
+                    s0 = torch.cuda.Event(enable_timing=True)
+                    s0.record(self.stream)
                     self._build_trees(new_ids, planned)
-                    inflight.append(self.launch_step(batch))
+                    step = self.launch_step(batch)
+                    self.step_gpu.append((s0, step["event"]))
+                    inflight.append(step)
+                    self.host_s.append(time.perf_counter() - h0)
                     steps += 1
                     continue
                 if not inflight:
@@ -282,6 +340,7 @@ class PlainServer:
         self.L = _lib.lib()
         self.cache = ByteLRU(budget_bytes)
         self.stream = torch.cuda.current_stream(self.device)
+        self.builds: list = []  # (start event, end event, triangles) per build
 
     def _tree(self, mid: int):
         if mid in self.cache:
@@ -299,9 +358,14 @@ class PlainServer:
         nodes = torch.empty((2 * n - 1, 3), dtype=torch.uint32, device=dev)
         aabb = torch.empty((2 * n - 1, 6), dtype=torch.float32, device=dev)
         ws = _lib.workspace(L.dmsh_build_workspace_bytes(n, 30), dev)
+        b0 = torch.cuda.Event(enable_timing=True)
+        b0.record(self.stream)
         _lib.check(L.dmsh_build_trimesh3(self.stream.cuda_stream, n, _lib.ptr(t_tri), vtx.shape[0],
                                          _lib.ptr(t_vtx), 30, _lib.ptr(nodes), _lib.ptr(aabb),
                                          _lib.ptr(ws), ws.numel()), "build")
+        b1 = torch.cuda.Event(enable_timing=True)
+        b1.record(self.stream)
+        self.builds.append((b0, b1, n))
         val = (t_tri, t_vtx, nodes, aabb)
         self.cache.insert(mid, nbytes, val)
         return val
@@ -336,6 +400,29 @@ class PlainServer:
         torch.cuda.synchronize(self.device)
         for r, ev in zip(requests, events):
             r.done = ref.elapsed_time(ev) * 1e-3
+
+
+def build_stats(builds) -> dict:
+    """Triangles built per second of GPU build time (events around each build,
+    read after the run; the served figure includes the pack into the
+    traversal layout, the plain one does not need it)."""
+    if not builds:
+        return {"builds": 0}
+    ms = sum(a.elapsed_time(b) for a, b, _ in builds)
+    tris = sum(t for _, _, t in builds)
+    return {"builds": len(builds), "tris": int(tris), "gpu_ms": round(ms, 3),
+            "mtri_per_s": round(tris / (ms * 1e-3) / 1e6, 1)}
+
+
+def gpu_busy_s(step_gpu) -> float:
+    """Seconds the stream spent on the steps: each step from the later of its
+    start event and the previous step's end to its own end."""
+    busy, prev = 0.0, None
+    for a, b in step_gpu:
+        start = a if prev is None or prev.elapsed_time(a) > 0 else prev
+        busy += start.elapsed_time(b) * 1e-3
+        prev = b
+    return busy
 
 
 def latency_stats(requests) -> dict:

@@ -13,8 +13,9 @@ def make_bvhnode2aabb_from_bvhnodes(tri2vtx, vtx2xyz0, vtx2xyz1, bvhnodes, bvhno
     dev = _lib.require_gpu(tri2vtx, vtx2xyz0, bvhnodes, bvhnode2aabb)
     num_tri = tri2vtx.shape[0]
     num_node = bvhnodes.shape[0]
-    ws = _lib.workspace(_lib.lib().dmsh_refit_workspace_bytes(num_tri), dev)
     stream = stream_ptr or _lib.stream_of(dev)
+    ws = _lib.kept_workspace("dmsh_bvhnode2aabb_trimesh3",
+                             _lib.lib().dmsh_refit_workspace_bytes(num_tri), dev, stream)
     _lib.check(
         _lib.lib().dmsh_bvhnode2aabb_trimesh3(
             stream, _lib.ptr(bvhnode2aabb), num_node, _lib.ptr(bvhnodes), num_tri,

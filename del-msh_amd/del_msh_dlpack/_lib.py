@@ -304,3 +304,18 @@ def ptr(t: torch.Tensor | None) -> int | None:
 
 def workspace(nbytes: int, device: torch.device) -> torch.Tensor:
     return torch.empty((max(int(nbytes), 1),), dtype=torch.uint8, device=device)
+
+
+_KEPT = {}
+
+
+def kept_workspace(entry: str, nbytes: int, device: torch.device, stream: int) -> torch.Tensor:
+    """A workspace kept for `entry` on (device, stream) and handed to its later
+    calls on that stream (which run in order).  The standalone refits keep the
+    tree's plan in it, checked against a hash of the rows on every call."""
+    key = (entry, torch.device(device).index, int(stream))
+    ws = _KEPT.get(key)
+    if ws is None or ws.numel() < nbytes:
+        ws = workspace(nbytes, device)
+        _KEPT[key] = ws
+    return ws

@@ -108,12 +108,16 @@ def test_batch_results_do_not_depend_on_tuning(gpu):
     ref = (H(p.bvhnodes).copy(), H(p.bvhnode2aabb).copy(), H(p.pix2tri).copy(), H(p.pix2t).copy())
     L = _lib.lib()
     saved = {k: L.dmsh_batch_get_tuning(k) for k in (b"batch.stack", b"batch.wgs_per_cu",
-                                                      b"batch.treelet_grid")}
+                                                      b"batch.treelet_grid", b"batch.sort_items",
+                                                      b"batch.cast_waves")}
     try:
-        for stack, wgs, tg in ((16, 1, 3), (32, 64, 4096)):
+        for stack, wgs, tg, items, waves in ((16, 1, 3, 8, 2), (32, 64, 4096, 12, 4),
+                                              (24, 3, 1024, 16, 1)):
             assert L.dmsh_batch_set_tuning(b"batch.stack", stack) == 0
             assert L.dmsh_batch_set_tuning(b"batch.wgs_per_cu", wgs) == 0
             assert L.dmsh_batch_set_tuning(b"batch.treelet_grid", tg) == 0
+            assert L.dmsh_batch_set_tuning(b"batch.sort_items", items) == 0
+            assert L.dmsh_batch_set_tuning(b"batch.cast_waves", waves) == 0
             p.run()
             got = (H(p.bvhnodes), H(p.bvhnode2aabb), H(p.pix2tri), H(p.pix2t))
             assert np.array_equal(got[0], ref[0]) and bits_equal(got[1], ref[1])

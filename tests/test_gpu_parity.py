@@ -288,6 +288,59 @@ def test_refit_standalone_non_lbvh(gpu, oracle, layout, mesh):
     no_device_errors()
 
 
+def _plan_gate_word(ws, n):
+    """counters[4] of the standalone refit workspace (1: the last call built
+    the plan, 0: it kept the one already there); the layout is four arrays of
+    n - 1 words, each 256-byte aligned, then the counters."""
+    off = 0
+    for _ in range(4):
+        off = (off + 255) // 256 * 256 + 4 * max(n - 1, 1)
+    off = (off + 255) // 256 * 256
+    return int(ws[off + 16: off + 20].cpu().numpy().view(np.uint32)[0])
+
+
+def test_refit_standalone_kept_plan(gpu, oracle):
+    """One workspace through a sequence of refits: the plan is kept while the
+    rows stay the same (moved vertices only), rebuilt when they change (a
+    non-LBVH tree of the same size, another tree of the same size) and when
+    keeping is switched off; every result equals the oracle bit for bit."""
+    L = _lib.lib()
+    tri, vtx = primitives.octahedral_sphere(7, noise_sigma=0.01, seed=5)
+    n = tri.shape[0]
+    bn_a, _ = oracle.build_trimesh3(tri, vtx, morton_bits=63)
+    tri_b, vtx_b = primitives.octahedral_sphere(7, noise_sigma=0.02, seed=6)
+    bn_b, _ = oracle.build_trimesh3(tri_b, vtx_b, morton_bits=63)
+    assert not np.array_equal(bn_a, bn_b)
+    vtx_moved = (vtx * np.float32(1.01) + np.float32(0.003)).astype(np.float32)
+    ws = _lib.workspace(L.dmsh_refit_workspace_bytes(n), gpu)
+    ws.zero_()  # not a block that held an earlier test's plan
+    aabb = torch.empty((2 * n - 1, 6), dtype=torch.float32, device=gpu)
+
+    def refit(t, v, bn):
+        t_tri, t_vtx, t_bn = G(t, gpu), G(v, gpu), G(bn, gpu)
+        _lib.check(L.dmsh_bvhnode2aabb_trimesh3(
+            _lib.stream_of(gpu), _lib.ptr(aabb), bn.shape[0], _lib.ptr(t_bn), n, _lib.ptr(t_tri),
+            v.shape[0], _lib.ptr(t_vtx), 0.0, _lib.ptr(ws), ws.numel()), "refit")
+        assert bits_equal(H(aabb), oracle.refit_trimesh3(bn, t, v, 0.0))
+        return _plan_gate_word(ws, n)
+
+    assert refit(tri, vtx, bn_a) == 1          # a new workspace builds the plan
+    assert refit(tri, vtx_moved, bn_a) == 0    # same rows: kept
+    assert refit(tri, vtx, bn_a) == 0
+    swapped = swap_leaves(bn_a, 3, n - 5)
+    assert refit(tri, vtx, swapped) == 1       # rows changed (and fail the layout check)
+    assert refit(tri, vtx, swapped) == 1       # a failed check is never kept
+    assert refit(tri, vtx_moved, bn_a) == 1    # back to the first tree: rebuilt
+    assert refit(tri_b, vtx_b, bn_b) == 1      # another tree of the same size
+    assert refit(tri_b, vtx_b, bn_b) == 0
+    assert L.dmsh_set_tuning(b"refit.plan_cache", 0) == 0
+    try:
+        assert refit(tri_b, vtx_b, bn_b) == 1  # keeping switched off
+    finally:
+        assert L.dmsh_set_tuning(b"refit.plan_cache", 1) == 0
+    no_device_errors()
+
+
 @pytest.mark.parametrize("layout", ["lbvh", "relabelled"])
 def test_refit_standalone_points(gpu, oracle, layout):
     pts = np.random.default_rng(33).random((70_000, 3), dtype=np.float32)

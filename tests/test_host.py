@@ -122,3 +122,30 @@ def test_native_module_names_and_host_arrays():
         N.array1d_argsort(to_dlpack(keys), to_dlpack(idx), 0)
     with pytest.raises(NotImplementedError):
         N.array1d_permute(keys, keys, idx, 0)
+
+
+def test_tuning_keys_accept_and_reject_without_a_gpu():
+    """The launch parameters the autotuner and the benchmarks set: valid
+    values are kept, invalid ones refused with DMSH_ERR_INVALID (host only)."""
+    L = _lib.lib()
+    batch = {b"batch.sort_items": ((8, 12, 16), (0, 4, 10, 32)),
+             b"batch.cast_waves": ((1, 2, 4), (0, 3, 8)),
+             b"batch.stack": ((16, 20, 24, 32), (8, 17)),
+             b"batch.treelet_grid": ((1, 1024), (0, 70000))}
+    for key, (good, bad) in batch.items():
+        old = L.dmsh_batch_get_tuning(key)
+        try:
+            for v in good:
+                assert L.dmsh_batch_set_tuning(key, v) == 0
+                assert L.dmsh_batch_get_tuning(key) == v
+            for v in bad:
+                assert L.dmsh_batch_set_tuning(key, v) != 0
+                assert L.dmsh_batch_get_tuning(key) == good[-1]
+        finally:
+            assert L.dmsh_batch_set_tuning(key, old) == 0
+    for key in (b"refit.plan_cache", b"refit.top", b"refit.climb"):
+        old = L.dmsh_get_tuning(key)
+        assert old in (0, 1)
+        assert L.dmsh_set_tuning(key, 1 - old) == 0 and L.dmsh_get_tuning(key) == 1 - old
+        assert L.dmsh_set_tuning(key, old) == 0
+    assert L.dmsh_set_tuning(b"refit.no_such_key", 1) != 0
