@@ -680,6 +680,125 @@ def run_served(num_req: int, rates, budget_bytes: int):
 
 
 # ---------------------------------------------------------------------------
+# CPU baseline
+# ---------------------------------------------------------------------------
+def _cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _median_s(fn, warmup=3, reps=10):
+    for _ in range(warmup):
+        fn()
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        fn()
+        ts.append(time.perf_counter() - t0)
+    ts.sort()
+    return ts[len(ts) // 2] if len(ts) % 2 else 0.5 * (ts[len(ts) // 2 - 1] + ts[len(ts) // 2])
+
+
+def run_cpu_baseline(tri, vtx, n2w, W, H, nodes_h, aabb_h, pix_gpu):
+    """The oracle (a C restatement of del-msh-cpu) built for this host with
+    -O3 -march=native -ffp-contract=off and timed on the same workload: the
+    serial build (del-msh-cpu's single thread), the all-cores build
+    (oracle/dmsh_cpu_mt.c, the same tree bit for bit) and the all-cores
+    ray-cast; 3 warm-ups, then the median of 10.  Also counts the traversal
+    work of the flat closest-hit search and checks the GPU image against it."""
+    import ctypes
+    import tempfile
+
+    import numpy as np
+
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import oracle
+
+    tmp = tempfile.mkdtemp(prefix="dmsh_cpu_")
+    Lnat = oracle.load(oracle.build_native(tmp))
+    num_tri = tri.shape[0]
+    num_pix = W * H
+    nproc = os.cpu_count() or 1
+    avail = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else nproc
+    threads = max(1, min(16, avail))
+    cm = n2w.T.reshape(-1)
+    t3 = oracle._u32(tri)
+
+    def serial_build():
+        cntr = np.empty((num_tri, 3), np.float32)
+        Lnat.orc_tri2centroid(num_tri, t3.reshape(-1), oracle._f32(vtx).reshape(-1), cntr.reshape(-1))
+        T = np.empty(16, np.float32)
+        Lnat.orc_fit_into_unit_cube(num_tri, cntr.reshape(-1), T)
+        codes = np.empty(num_tri, np.uint64)
+        Lnat.orc_vtx2morton_u64(num_tri, cntr.reshape(-1), T, codes)
+        idx = np.empty(num_tri, np.uint32)
+        sk = np.empty(num_tri, np.uint64)
+        Lnat.orc_argsort_u64(num_tri, codes, idx, sk)
+        bn = np.zeros((2 * num_tri - 1) * 3, np.uint32)
+        assert Lnat.orc_bvhnodes_u64(bn, idx, sk, num_tri) == 0
+        ab = np.zeros((2 * num_tri - 1) * 6, np.float32)
+        assert Lnat.orc_bvhnode2aabb_trimesh3(ab, bn, 2 * num_tri - 1, t3.reshape(-1),
+                                              oracle._f32(vtx).reshape(-1), 0.0) == 0
+        return bn, ab
+
+    bn_s, ab_s = serial_build()
+    serial_s = _median_s(serial_build)
+    bn_m, ab_m = oracle.build_trimesh3_mt(tri, vtx, threads, L=Lnat)
+    mt_s = _median_s(lambda: oracle.build_trimesh3_mt(tri, vtx, threads, L=Lnat))
+    same_tree = bool(np.array_equal(bn_s.reshape(-1, 3), nodes_h)
+                     and np.array_equal(bn_m, nodes_h)
+                     and np.array_equal(ab_s.view(np.uint32), aabb_h.reshape(-1).view(np.uint32))
+                     and np.array_equal(ab_m.reshape(-1).view(np.uint32),
+                                        aabb_h.reshape(-1).view(np.uint32)))
+
+    def cast():
+        out = np.empty(num_pix, np.uint32)
+        st = oracle.TraceStats()
+        Lnat.orc_pix2tri(out, None, num_tri, t3.reshape(-1), oracle._f32(vtx).reshape(-1),
+                         nodes_h.reshape(-1), aabb_h.reshape(-1), W, H, cm, threads,
+                         ctypes.byref(st))
+        return out
+
+    wpix = cast()
+    cast_s = _median_s(cast)
+    mism = int(np.count_nonzero(wpix.reshape(H, W) != pix_gpu))
+    # the work of the search the kernel runs (one best hit, far children on a
+    # stack), which equals del-msh-cpu's recursion in result
+    fpix, _, (n_exp, n_leaf, fhist) = oracle.pix2tri_flat(tri, vtx, nodes_h, aabb_h, cm, W, H,
+                                                          num_threads=threads, with_t=False)
+    mism += int(np.count_nonzero(fpix != wpix.reshape(H, W)))
+    cast_counts = (n_exp, n_leaf, int(np.nonzero(fhist)[0].max()))
+    cpu = {
+        "value": round(num_tri / serial_s / 1e6, 3),
+        "unit": "Mtri/s",
+        "cores": 1,
+        "threads": 1,
+        "kind": "port",
+        "cpu_model": _cpu_model(),
+        "nproc": nproc,
+        "cores_available": avail,
+        "build_flags": "gcc -O3 -march=native -ffp-contract=off -fno-fast-math -fopenmp",
+        "timing": "3 warm-ups, median of 10",
+        "sample": (f"the full {num_tri}-triangle LBVH build with 63-bit codes (centroid, fit, "
+                   f"Morton, stable argsort, topology, refit) on one thread, as del-msh-cpu runs "
+                   f"it; trees and boxes equal the GPU build's: {same_tree}"),
+        "all_cores_build": {"value": round(num_tri / mt_s / 1e6, 3), "unit": "Mtri/s",
+                            "threads": threads, "label": "all cores (oracle/dmsh_cpu_mt.c: "
+                            "parallel stages, radix argsort, atomic-counter refit)"},
+        "all_cores_raycast": {"value": round(num_pix / cast_s / 1e6, 3), "unit": "Mrays/s",
+                              "threads": threads, "label": f"all {W}x{H} rays, del-msh-cpu's "
+                              "recursion per ray, rays spread over threads"},
+        "gpu_oracle_pix2tri_mismatches": mism,
+    }
+    return cpu, cast_counts
+
+
+# ---------------------------------------------------------------------------
 # main
 # ---------------------------------------------------------------------------
 def main():
@@ -937,37 +1056,8 @@ def main():
     cast_counts = None
     cpu = None
     if not args.no_cpu_baseline:
-        sys.path.insert(0, str(ROOT / "oracle"))
-        import oracle
-
-        nodes_h = bvhnodes.cpu().numpy()
-        aabb_h = aabb.cpu().numpy()
-        threads = min(16, os.cpu_count() or 1)
-        tc = time.perf_counter()
-        wpix, _, _ = oracle.pix2tri(tri, vtx, nodes_h, aabb_h, n2w.T.reshape(-1), W, H,
-                                    num_threads=threads, with_t=False)
-        cpu_cast_s = time.perf_counter() - tc
-        mism = int(np.count_nonzero(wpix != pix2tri.cpu().numpy()))
-        # the work of the search the kernel runs (one best hit, far children
-        # on a stack), which equals del-msh-cpu's recursion in result
-        fpix, _, (n_exp, n_leaf, fhist) = oracle.pix2tri_flat(
-            tri, vtx, nodes_h, aabb_h, n2w.T.reshape(-1), W, H, num_threads=threads, with_t=False)
-        cast_counts = (n_exp, n_leaf, int(np.nonzero(fhist)[0].max()))
-        mism += int(np.count_nonzero(fpix != wpix))
-        tb = time.perf_counter()
-        oracle.build_trimesh3(tri, vtx, morton_bits=63)
-        cpu_build_s = time.perf_counter() - tb
-        cpu = {
-            "value": round(num_tri / cpu_build_s / 1e6, 3),
-            "unit": "Mtri/s",
-            "cores": 1,
-            "kind": "port",
-            "sample": (f"full 2^21-triangle LBVH build (63-bit codes), single thread as "
-                       f"del-msh-cpu; ray-cast of all {W}x{H} rays on {threads} threads: "
-                       f"{num_pix / cpu_cast_s / 1e6:.3f} Mrays/s; GPU/oracle pix2tri mismatches: {mism}"),
-            "raycast_mrays_per_s": round(num_pix / cpu_cast_s / 1e6, 3),
-            "raycast_threads": threads,
-        }
+        cpu, cast_counts = run_cpu_baseline(tri, vtx, n2w, W, H, bvhnodes.cpu().numpy(),
+                                            aabb.cpu().numpy(), pix2tri.cpu().numpy())
 
     # ---------------- batched workload ----------------
     batched = run_batched(args.batch_meshes, args.batch_reps) if args.batch_meshes > 0 else None

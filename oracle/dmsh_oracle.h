@@ -283,6 +283,13 @@ void orc_polyhedron_interpolate(size_t num_wtx, const uint32_t *off, const uint3
                                 uint32_t num_vdim, const float *vtx2val, const uint32_t *wtx2elem,
                                 const float *wtx2param, float *wtx2val);
 
+/* ---- all-cores variant of the build (dmsh_cpu_mt.c; baseline only) ----
+ * centroid -> fit -> 63-bit Morton -> stable argsort -> topology -> refit
+ * (eps 0) on num_threads OpenMP threads; the same tree and boxes as the
+ * serial stages.  Returns 0, or a negative value on a malformed tree. */
+int orc_build_trimesh3_u64_mt(size_t num_tri, const uint32_t *tri2vtx, const float *vtx2xyz,
+                              int num_threads, uint32_t *bvhnodes, float *aabb);
+
 #ifdef __cplusplus
 }
 #endif

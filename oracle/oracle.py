@@ -32,64 +32,97 @@ def lib():
         path = _HERE / "liboracle.so"
         if not path.exists():
             raise RuntimeError(f"{path} missing: run `make -C oracle`")
-        L = ctypes.CDLL(str(path))
-        sig = {
-            "orc_expand_bits2": (ctypes.c_uint32, [ctypes.c_uint32]),
-            "orc_expand_bits3": (ctypes.c_uint32, [ctypes.c_uint32]),
-            "orc_expand_bits3_u64": (ctypes.c_uint64, [ctypes.c_uint32]),
-            "orc_morton_code2": (ctypes.c_uint32, [ctypes.c_float] * 2),
-            "orc_morton_code3": (ctypes.c_uint32, [ctypes.c_float] * 3),
-            "orc_morton_code3_u64": (ctypes.c_uint64, [ctypes.c_float] * 3),
-            "orc_vtx2morton": (ctypes.c_int, [_sz, _f32p, ctypes.c_int, _f32p, _u32p]),
-            "orc_vtx2morton_u64": (ctypes.c_int, [_sz, _f32p, _f32p, _u64p]),
-            "orc_argsort_u32": (None, [_sz, _u32p, _u32p, _u32p]),
-            "orc_argsort_u64": (None, [_sz, _u64p, _u32p, _u64p]),
-            "orc_split_u32": (_sz, [_u32p, _sz, _sz]),
-            "orc_bvhnodes_u32": (ctypes.c_int, [_u32p, _u32p, _u32p, _sz]),
-            "orc_bvhnodes_u64": (ctypes.c_int, [_u32p, _u32p, _u64p, _sz]),
-            "orc_check_morton_code_range_split_u32": (ctypes.c_int, [_u32p, _sz]),
-            "orc_check_binary_radix_tree_u32": (ctypes.c_int, [_u32p, _u32p, _sz]),
-            "orc_check_bvh_topology": (ctypes.c_int, [_u32p, _sz]),
-            "orc_bvhnode2aabb_trimesh3": (ctypes.c_int, [_f32p, _u32p, _sz, _u32p, _f32p, ctypes.c_float]),
-            "orc_bvhnode2aabb_points": (ctypes.c_int, [_f32p, _u32p, _sz, _f32p]),
-            "orc_transform_homogeneous": (ctypes.c_int, [_f32p, _f32p, _f32p]),
-            "orc_ray_for_pixel": (None, [ctypes.c_uint32] * 4 + [_f32p, _f32p, _f32p]),
-            "orc_ray_aabb": (ctypes.c_int, [_f32p, _f32p, _f32p, ctypes.POINTER(ctypes.c_float),
-                                            ctypes.POINTER(ctypes.c_float)]),
-            "orc_ray_tri": (ctypes.c_int, [_f32p, _f32p, _f32p, _f32p, _f32p,
-                                           ctypes.POINTER(ctypes.c_float), _f32p]),
-            "orc_line_tri": (ctypes.c_int, [_f32p, _f32p, _f32p, _f32p, _f32p,
-                                            ctypes.POINTER(ctypes.c_float), _f32p]),
-            "orc_mat4_inverse": (ctypes.c_int, [_f32p, _f32p]),
-            "orc_tri2centroid": (None, [_sz, _u32p, _f32p, _f32p]),
-            "orc_fit_into_unit_cube": (None, [_sz, _f32p, _f32p]),
-            "orc_first_intersection_ray": (
-                ctypes.c_int,
-                [_f32p, _f32p, _u32p, _f32p, _u32p, _sz, _f32p, ctypes.POINTER(ctypes.c_float),
-                 ctypes.POINTER(ctypes.c_uint32), _f32p, ctypes.POINTER(TraceStats)]),
-            "orc_pix2tri": (
-                ctypes.c_int,
-                [_u32p, ctypes.c_void_p, _sz, _u32p, _f32p, _u32p, _f32p, ctypes.c_uint32,
-                 ctypes.c_uint32, _f32p, ctypes.c_int, ctypes.POINTER(TraceStats)]),
-            "orc_pix2tri_flat": (
-                ctypes.c_int,
-                [_u32p, ctypes.c_void_p, _sz, _u32p, _f32p, _u32p, _f32p, ctypes.c_uint32,
-                 ctypes.c_uint32, _f32p, ctypes.c_int, ctypes.POINTER(TraceStats)]),
-            "orc_pix2depth": (ctypes.c_int, [_f32p, _u32p, _u32p, _f32p, ctypes.c_uint32,
-                                             ctypes.c_uint32, _f32p]),
-            "orc_pix2depth_bwd": (ctypes.c_int, [_f32p, _u32p, _u32p, _f32p, _f32p,
-                                                 ctypes.c_uint32, ctypes.c_uint32, _f32p]),
-            "orc_edge_gradient_bwd": (
-                None,
-                [_u32p, _f32p, _f32p, _f32p, ctypes.c_uint32, ctypes.c_uint32, _u32p,
-                 ctypes.c_uint32, _f32p, _f32p]),
-        }
-        for name, (res, args) in sig.items():
-            fn = getattr(L, name)
-            fn.restype = res
-            fn.argtypes = args
-        _L = L
+        _L = load(path)
     return _L
+
+
+def load(path):
+    """The oracle library at `path` (the default build, or a native build for
+    the CPU baseline) with its prototypes attached."""
+    L = ctypes.CDLL(str(path))
+    sig = {
+        "orc_expand_bits2": (ctypes.c_uint32, [ctypes.c_uint32]),
+        "orc_expand_bits3": (ctypes.c_uint32, [ctypes.c_uint32]),
+        "orc_expand_bits3_u64": (ctypes.c_uint64, [ctypes.c_uint32]),
+        "orc_morton_code2": (ctypes.c_uint32, [ctypes.c_float] * 2),
+        "orc_morton_code3": (ctypes.c_uint32, [ctypes.c_float] * 3),
+        "orc_morton_code3_u64": (ctypes.c_uint64, [ctypes.c_float] * 3),
+        "orc_vtx2morton": (ctypes.c_int, [_sz, _f32p, ctypes.c_int, _f32p, _u32p]),
+        "orc_vtx2morton_u64": (ctypes.c_int, [_sz, _f32p, _f32p, _u64p]),
+        "orc_argsort_u32": (None, [_sz, _u32p, _u32p, _u32p]),
+        "orc_argsort_u64": (None, [_sz, _u64p, _u32p, _u64p]),
+        "orc_split_u32": (_sz, [_u32p, _sz, _sz]),
+        "orc_bvhnodes_u32": (ctypes.c_int, [_u32p, _u32p, _u32p, _sz]),
+        "orc_bvhnodes_u64": (ctypes.c_int, [_u32p, _u32p, _u64p, _sz]),
+        "orc_check_morton_code_range_split_u32": (ctypes.c_int, [_u32p, _sz]),
+        "orc_check_binary_radix_tree_u32": (ctypes.c_int, [_u32p, _u32p, _sz]),
+        "orc_check_bvh_topology": (ctypes.c_int, [_u32p, _sz]),
+        "orc_bvhnode2aabb_trimesh3": (ctypes.c_int, [_f32p, _u32p, _sz, _u32p, _f32p, ctypes.c_float]),
+        "orc_bvhnode2aabb_points": (ctypes.c_int, [_f32p, _u32p, _sz, _f32p]),
+        "orc_transform_homogeneous": (ctypes.c_int, [_f32p, _f32p, _f32p]),
+        "orc_ray_for_pixel": (None, [ctypes.c_uint32] * 4 + [_f32p, _f32p, _f32p]),
+        "orc_ray_aabb": (ctypes.c_int, [_f32p, _f32p, _f32p, ctypes.POINTER(ctypes.c_float),
+                                        ctypes.POINTER(ctypes.c_float)]),
+        "orc_ray_tri": (ctypes.c_int, [_f32p, _f32p, _f32p, _f32p, _f32p,
+                                       ctypes.POINTER(ctypes.c_float), _f32p]),
+        "orc_line_tri": (ctypes.c_int, [_f32p, _f32p, _f32p, _f32p, _f32p,
+                                        ctypes.POINTER(ctypes.c_float), _f32p]),
+        "orc_mat4_inverse": (ctypes.c_int, [_f32p, _f32p]),
+        "orc_tri2centroid": (None, [_sz, _u32p, _f32p, _f32p]),
+        "orc_fit_into_unit_cube": (None, [_sz, _f32p, _f32p]),
+        "orc_first_intersection_ray": (
+            ctypes.c_int,
+            [_f32p, _f32p, _u32p, _f32p, _u32p, _sz, _f32p, ctypes.POINTER(ctypes.c_float),
+             ctypes.POINTER(ctypes.c_uint32), _f32p, ctypes.POINTER(TraceStats)]),
+        "orc_pix2tri": (
+            ctypes.c_int,
+            [_u32p, ctypes.c_void_p, _sz, _u32p, _f32p, _u32p, _f32p, ctypes.c_uint32,
+             ctypes.c_uint32, _f32p, ctypes.c_int, ctypes.POINTER(TraceStats)]),
+        "orc_pix2tri_flat": (
+            ctypes.c_int,
+            [_u32p, ctypes.c_void_p, _sz, _u32p, _f32p, _u32p, _f32p, ctypes.c_uint32,
+             ctypes.c_uint32, _f32p, ctypes.c_int, ctypes.POINTER(TraceStats)]),
+        "orc_pix2depth": (ctypes.c_int, [_f32p, _u32p, _u32p, _f32p, ctypes.c_uint32,
+                                         ctypes.c_uint32, _f32p]),
+        "orc_pix2depth_bwd": (ctypes.c_int, [_f32p, _u32p, _u32p, _f32p, _f32p,
+                                             ctypes.c_uint32, ctypes.c_uint32, _f32p]),
+        "orc_edge_gradient_bwd": (
+            None,
+            [_u32p, _f32p, _f32p, _f32p, ctypes.c_uint32, ctypes.c_uint32, _u32p,
+             ctypes.c_uint32, _f32p, _f32p]),
+        "orc_build_trimesh3_u64_mt": (ctypes.c_int, [_sz, _u32p, _f32p, ctypes.c_int, _u32p,
+                                                     _f32p]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    return L
+
+
+def build_native(out_dir) -> pathlib.Path:
+    """Compiles the oracle with -O3 -march=native -ffp-contract=off for the
+    host it runs on (the CPU baseline); returns the library path."""
+    import subprocess
+
+    out = pathlib.Path(out_dir) / "liboracle_native.so"
+    cmd = ["gcc", "-O3", "-march=native", "-std=c11", "-fPIC", "-ffp-contract=off",
+           "-fno-fast-math", "-fopenmp", "-shared", "-o", str(out),
+           str(_HERE / "dmsh_oracle.c"), str(_HERE / "dmsh_cpu_mt.c"), "-lm"]
+    subprocess.run(cmd, check=True, capture_output=True)
+    return out
+
+
+def build_trimesh3_mt(tri2vtx, vtx2xyz, num_threads, L=None):
+    """The all-cores variant of build_trimesh3 with 63-bit codes (eps 0)."""
+    t = _u32(tri2vtx)
+    n = t.shape[0]
+    bn = np.zeros((2 * n - 1) * 3, np.uint32)
+    aabb = np.zeros((2 * n - 1) * 6, np.float32)
+    rc = (L or lib()).orc_build_trimesh3_u64_mt(n, t.reshape(-1), _f32(vtx2xyz).reshape(-1),
+                                                 num_threads, bn, aabb)
+    assert rc == 0, rc
+    return bn.reshape(-1, 3), aabb.reshape(-1, 6)
 
 
 def _f32(a):
