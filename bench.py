@@ -208,13 +208,21 @@ def summarize_counters(folders, title: str) -> str:
                      f"read {l2_read(c) / CALIB_BYTES:.3f} x, write {l2_write(c) / CALIB_BYTES:.3f} x; "
                      f"to / from DRAM read {dram_read(c) / CALIB_BYTES:.3f} x, write "
                      f"{dram_write(c) / CALIB_BYTES:.3f} x")
+    diffs = [abs(l2_read(k) + l2_write(k) - dram_read(k) - dram_write(k))
+             / max(l2_read(k) + l2_write(k), 1.0)
+             for k in per if not k[0].startswith("copy") and per[k].get("TCC_EA0_RDREQ_32B_sum")]
+    lines.append("the DRAM-bound request counts (RDREQ_DRAM_32B, WRREQ_*_DRAM_32B) match the "
+                 f"L2-to-fabric totals for every kernel here (largest relative difference "
+                 f"{max(diffs) if diffs else float('nan'):.1e}): every L2 miss is a memory request, "
+                 "and whether the MALL (infinity cache) or HBM serves it is not visible to the L2 "
+                 "counters, so the miss bytes below bound the HBM bytes from above")
     lines.append("per dispatch, averaged over its dispatches in that phase: us (kernel time in the "
-                 "counter passes); MB read / written at the L2-fabric interface; MB read from / "
-                 "written to DRAM; L2 hit %; VALU busy % (SQ_ACTIVE_INST_VALU / (256 CUs x "
-                 "GRBM_GUI_ACTIVE / 8 XCDs)); VALU / LDS / VMEM instructions per wave; LDS-wait "
-                 "share of wave cycles %; LDS bank-conflict cycles per LDS instruction; waves")
-    lines.append(f"{'phase':<6}{'kernel':<24}{'us':>8}{'L2rdMB':>8}{'L2wrMB':>8}{'DRrdMB':>8}"
-                 f"{'DRwrMB':>8}{'L2hit%':>7}{'VALU%':>6}{'valu/w':>8}{'lds/w':>7}{'vmem/w':>7}"
+                 "counter passes); MB read / written by L2 misses (to MALL / HBM); L2 hit %; VALU "
+                 "busy % (SQ_ACTIVE_INST_VALU / (256 CUs x GRBM_GUI_ACTIVE / 8 XCDs)); VALU / LDS / "
+                 "VMEM instructions per wave; LDS-wait share of wave cycles %; LDS bank-conflict "
+                 "cycles per LDS instruction; waves")
+    lines.append(f"{'phase':<6}{'kernel':<24}{'us':>8}{'missRdMB':>9}{'missWrMB':>9}"
+                 f"{'L2hit%':>7}{'VALU%':>6}{'valu/w':>8}{'lds/w':>7}{'vmem/w':>7}"
                  f"{'ldsW%':>6}{'bankc':>6}{'waves':>8}")
     for phase, short in PROFILED_KERNELS:
         key = pick(phase, short)
@@ -227,24 +235,22 @@ def summarize_counters(folders, title: str) -> str:
         us = _avg(dur.get(key, [])) * 1e-3
         label = short.split('<')[0]
         lines.append(
-            f"{phase:<6}{label:<24}{us:>8.1f}{l2_read(key) / 1e6:>8.2f}{l2_write(key) / 1e6:>8.2f}"
-            f"{dram_read(key) / 1e6:>8.2f}{dram_write(key) / 1e6:>8.2f}"
+            f"{phase:<6}{label:<24}{us:>8.1f}{l2_read(key) / 1e6:>9.2f}{l2_write(key) / 1e6:>9.2f}"
             f"{100 * hit / (hit + miss):>7.1f}"
             f"{100 * val(key, 'SQ_ACTIVE_INST_VALU') / NUM_CU / (gui / NUM_XCD):>6.1f}"
             f"{val(key, 'SQ_INSTS_VALU') / waves:>8.1f}{lds / waves:>7.1f}"
             f"{val(key, 'SQ_INSTS_VMEM') / waves:>7.1f}"
             f"{100 * val(key, 'SQ_WAIT_INST_LDS') / val(key, 'SQ_WAVE_CYCLES'):>6.1f}"
             f"{(val(key, 'SQ_LDS_BANK_CONFLICT') / lds) if lds else 0.0:>6.2f}{waves:>8.0f}")
-    lines.append("per launch, read back by bench.py: DRAM bytes (read + write; the roofline "
-                 "`traffic`), L2<->fabric bytes, L2 hit %, VALU busy %, kernel us:")
+    lines.append("per launch, read back by bench.py: L2-miss bytes to memory (read + write; the "
+                 "roofline `traffic`), L2 hit %, VALU busy %, kernel us:")
     for phase, short in PROFILED_KERNELS:
         key = pick(phase, short)
         if key:
             hit, miss = val(key, "TCC_HIT_sum"), val(key, "TCC_MISS_sum")
             gui = val(key, "GRBM_GUI_ACTIVE")
             lines.append(f"per_launch {phase}:{short.split('<')[0]} "
-                         f"dram_bytes={dram_read(key) + dram_write(key):.6e} "
-                         f"l2_fabric_bytes={l2_read(key) + l2_write(key):.6e} "
+                         f"mem_bytes={l2_read(key) + l2_write(key):.6e} "
                          f"l2_hit_pct={100 * hit / (hit + miss):.2f} "
                          f"valu_busy_pct={100 * val(key, 'SQ_ACTIVE_INST_VALU') / NUM_CU / (gui / NUM_XCD):.2f} "
                          f"us={_avg(dur.get(key, [])) * 1e-3:.2f}")
@@ -304,7 +310,7 @@ def fp64_error(tri, vtx, n2w_rowmajor, pix2tri_h, pix2t_h, depth_h):
 
 def read_counters(kernel: str = "cast:k_pix2tri"):
     """Per-launch figures of `kernel` (phase:name) from the committed counter
-    summary: {dram_bytes, l2_fabric_bytes, l2_hit_pct, valu_busy_pct, us}, or
+    summary: {mem_bytes, l2_hit_pct, valu_busy_pct, us}, or
     None when the summary does not list it."""
     try:
         for line in COUNTER_SUMMARY.read_text().splitlines():
@@ -673,10 +679,16 @@ def run_served(num_req: int, rates, budget_bytes: int):
                                               / p_stats["build"]["mtri_per_s"], 2)
             ratio["build_mtri_per_s_without_pack"] = round(
                 s_stats["build_without_pack"]["mtri_per_s"] / p_stats["build"]["mtri_per_s"], 2)
+        progress(f"served rate {rate:g}: {ratio}")
         out["rates"][str(rate)] = {"served": s_stats, "plain": p_stats, "served_over_plain": ratio,
                                    "distinct_meshes": len({r.mesh_id for r in reqs}),
                                    "host_generate_s": round(gen_s, 2)}
     return out
+
+
+def progress(msg: str) -> None:
+    """A line on stderr as each stage ends (stdout carries only the JSON line)."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
 # ---------------------------------------------------------------------------
@@ -919,6 +931,7 @@ def main():
     build_mtri = num_tri / (build_avg * 1e-3) / 1e6
     cast_mrays = num_pix / (cast_avg * 1e-3) / 1e6
 
+    progress(f"build + cast timed: build {build_avg:.4f} ms, cast {cast_avg:.4f} ms")
     # ---------------- standalone refit of the built tree ----------------
     # dmsh_bvhnode2aabb_trimesh3 on the tree just built (the reference's
     # bvhnode2aabb call, as an optimisation loop makes it after moving the
@@ -987,7 +1000,7 @@ def main():
                                                    "k_refit_top_lds")}
     if all(rk.values()):
         kept = ("k_plan_hash", "k_refit_treelet", "k_refit_top_lds")
-        dram = sum(v["dram_bytes"] for k, v in rk.items() if k in kept)
+        dram = sum(v["mem_bytes"] for k, v in rk.items() if k in kept)
         refit_line["roofline"] = {
             "bound": "hbm",
             "achieved": refit_line["algorithmic_gbs"],
@@ -995,14 +1008,16 @@ def main():
             "unit": "GB/s",
             "frac": refit_line["frac_of_hbm_peak"],
             "traffic": dram,
-            "traffic_plan_rebuilt": sum(v["dram_bytes"] for v in rk.values()),
+            "traffic_plan_rebuilt": sum(v["mem_bytes"] for v in rk.values()),
+            "traffic_meaning": "L2-miss bytes per call (served by MALL or HBM)",
             "limiter": "dependent-load and LDS-climb latency per treelet (see per kernel)",
-            "per_kernel": {k: {"dram_bytes": v["dram_bytes"], "us": v["us"],
-                               "dram_gbs": round(v["dram_bytes"] / (v["us"] * 1e-6) / 1e9, 1),
+            "per_kernel": {k: {"miss_bytes": v["mem_bytes"], "us": v["us"],
+                               "miss_gbs": round(v["mem_bytes"] / (v["us"] * 1e-6) / 1e9, 1),
                                "l2_hit_pct": v["l2_hit_pct"], "valu_busy_pct": v["valu_busy_pct"]}
                            for k, v in rk.items()},
         }
 
+    progress("standalone refit timed")
     # ---------------- f32 error of the hit coefficient and depth ----------------
     depth = torch.empty((H, W), dtype=torch.float32, device=dev)
     ws_depth = _lib.workspace(max(1, L.dmsh_pix2depth_workspace_bytes()), dev)
@@ -1014,6 +1029,7 @@ def main():
                         depth.cpu().numpy())
     del depth, ws_depth
 
+    progress("f32 error of t and depth")
     # ---------------- secondary: 2^22-point build and sort alone ----------------
     rng = np.random.default_rng(2)
     pts = torch.from_numpy(rng.random((1 << 22, 3), dtype=np.float32)).to(dev)
@@ -1052,6 +1068,7 @@ def main():
     timed(psort, 2, lambda: keys.copy_(keys0))
     psort_ms = timed(psort, 10, lambda: keys.copy_(keys0))
 
+    progress("points build and sort alone timed")
     # ---------------- CPU baseline and traversal counts ----------------
     cast_counts = None
     cpu = None
@@ -1060,11 +1077,16 @@ def main():
                                             aabb.cpu().numpy(), pix2tri.cpu().numpy())
 
     # ---------------- batched workload ----------------
+    progress("cpu baseline done")
     batched = run_batched(args.batch_meshes, args.batch_reps) if args.batch_meshes > 0 else None
+    progress("batched done")
     served = (run_served(args.serve_requests, [float(x) for x in args.serve_rates.split(",")],
                          args.serve_budget_mib << 20) if args.serve_requests > 0 else None)
+    progress("served done")
     diff = run_diff(args.diff_reps) if args.diff_reps > 0 else None
+    progress("differentiation done")
     trees = run_trees(args.tree_reps) if args.tree_reps > 0 else None
+    progress("trees done")
 
     # ---------------- roofline of the dominant kernel ----------------
     # build: compulsory bytes per triangle with u64 codes
@@ -1092,21 +1114,22 @@ def main():
             "peak": PEAK_HBM_GBS,
             "unit": "GB/s",
             "frac": round(cast_bytes / (k_ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4),
-            "traffic": cc["dram_bytes"] if cc else None,
+            "traffic": cc["mem_bytes"] if cc else None,
             "algorithmic_bytes": cast_bytes,
             "kernel_ms": k_ms,
         }
         if cc:
             # The algorithmic bytes are what the reference's layout would
             # read from memory; the tree and triangles stay in L2 / MALL, so
-            # DRAM moves a small share of them and the kernel is limited by
+            # memory moves a small share of them and the kernel is limited by
             # issue and dependent-load latency, not by HBM bandwidth.
             roof["limiter"] = "issue / dependent-load latency (working set L2- and MALL-resident)"
             roof["frac_meaning"] = "HBM-equivalent: algorithmic bytes / time / HBM peak"
-            roof["dram_gbs"] = round(cc["dram_bytes"] / (k_ms * 1e-3) / 1e9, 1)
-            roof["dram_frac_of_peak"] = round(cc["dram_bytes"] / (k_ms * 1e-3) / 1e9
-                                              / PEAK_HBM_GBS, 4)
-            roof["dram_share_of_algorithmic"] = round(cc["dram_bytes"] / cast_bytes, 4)
+            roof["traffic_meaning"] = "L2-miss bytes per launch (served by MALL or HBM)"
+            roof["miss_gbs"] = round(cc["mem_bytes"] / (k_ms * 1e-3) / 1e9, 1)
+            roof["miss_frac_of_hbm_peak"] = round(cc["mem_bytes"] / (k_ms * 1e-3) / 1e9
+                                                  / PEAK_HBM_GBS, 4)
+            roof["miss_share_of_algorithmic"] = round(cc["mem_bytes"] / cast_bytes, 4)
             roof["l2_hit_pct"] = cc["l2_hit_pct"]
             roof["valu_busy_pct"] = cc["valu_busy_pct"]
             roof["counter_us"] = cc["us"]
