@@ -130,7 +130,11 @@ int dmsh_bvhnodes_from_sorted_mortons_u64(void *stream, uint32_t n,
  * Morton-code LBVH) is refit subtree by subtree in LDS; any other tree in
  * that layout by the per-leaf climb through arrival counters.  Both give
  * the same boxes.  The workspace (about 32 bytes per leaf) holds the
- * counters, the leaf ranges and the subtree lists.
+ * counters, the leaf ranges and the subtree lists -- the tree's plan -- with
+ * a hash of the rows they were made from: a later call with the same
+ * workspace and unchanged rows keeps the plan and skips its two kernels
+ * (tuning key "refit.plan_cache" = 0 rebuilds it every call).  Calls that
+ * share a workspace must run in order (one stream).
  * ------------------------------------------------------------------- */
 size_t dmsh_refit_workspace_bytes(uint32_t num_leaf);
 int dmsh_bvhnode2aabb_trimesh3(void *stream, float *bvhnode2aabb,
@@ -165,7 +169,7 @@ int dmsh_build_trimesh3(void *stream, uint32_t num_tri, const uint32_t *tri2vtx,
  * the workspace is a fixed DMSH_NOALLOC_SCRATCH_BYTES (64 MiB) device array
  * inside the library, one per GPU, held until the process ends.  Meshes
  * whose dmsh_build_workspace_bytes(num_tri, morton_bits) exceeds it (above
- * 772,846 triangles with 63-bit codes, 870,720 with 30-bit codes) are
+ * 772,654 triangles with 63-bit codes, 870,528 with 30-bit codes) are
  * rejected with DMSH_ERR_TOO_LARGE before anything is enqueued.  It calls no
  * hipMalloc / hipFree.  Every call on a GPU shares that GPU's scratch: calls
  * that overlap on one GPU (on different streams, or from different host
