@@ -257,7 +257,7 @@ struct BuildLayout {
   uint4* treelets;
   uint32_t* top_count;
   uint32_t* treelet_next;
-  uint2* top_list;
+  uint4* top_list;
   uint32_t* node2slot;
   float* cntr;
   float* tri_box;
@@ -287,7 +287,7 @@ inline bool carve(Arena& a, uint32_t n, bool tri, BuildLayout& L) {
   L.zero_words = (size_t)(L.treelet_next - L.sort.hist) + 1;
   L.sort.zero_words = 0;
   L.treelets = a.take<uint4>(n);
-  L.top_list = a.take<uint2>(kTopSlots);
+  L.top_list = a.take<uint4>(kTopSlots);
   L.node2slot = a.take<uint32_t>(n);
   L.cntr = tri ? a.take<float>((size_t)n * 3) : nullptr;
   L.tri_box = tri ? a.take<float>((size_t)n * 6) : nullptr;

@@ -637,7 +637,7 @@ constexpr int kTopClimbSlots = 4096;
 template <int C, int T>
 __device__ __forceinline__ void top_climb(float* __restrict__ aabb, uint32_t num_leaf,
                                           const uint32_t* __restrict__ bvhnodes, uint32_t count,
-                                          const uint2* __restrict__ top_list,
+                                          const uint4* __restrict__ top_list,
                                           const uint32_t* __restrict__ node2slot,
                                           unsigned char* __restrict__ smem,
                                           uint32_t* __restrict__ err) {
@@ -649,66 +649,56 @@ __device__ __forceinline__ void top_climb(float* __restrict__ aabb, uint32_t num
   uint32_t* s_cnt = reinterpret_cast<uint32_t*>(smem + (size_t)C * (sizeof(Box) + 6));
   const uint32_t nb = num_leaf - 1;
   for (uint32_t w = threadIdx.x; w < (count + 3) / 4; w += T) s_cnt[w] = 0u;
-  // load pass, stage by stage over this thread's slots e = r T + tid, so
-  // each stage's loads are in flight together: the records, then the node
-  // rows, then the children's / parent's slots and the boxes of children
-  // below the top
-  uint2 rec[R];
+  // load pass over this thread's slots e = r T + tid, in two stages whose
+  // loads are in flight together: the records (node, children, which
+  // children are top nodes), then the top children's slots and the boxes of
+  // the children below the top.  A slot's parent is the slot that names it
+  // as a top child.
+  uint4 rec[R];
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     const uint32_t e = (uint32_t)r * T + threadIdx.x;
-    rec[r] = e < count ? top_list[e] : make_uint2(DMSH_U32MAX, 0u);
+    rec[r] = e < count ? top_list[e] : make_uint4(DMSH_U32MAX, 0u, 0u, 0u);
   }
-  uint32_t rp[R], ra[R], rb[R];
-#pragma unroll
-  for (int r = 0; r < R; ++r) {
-    const bool ok = rec[r].x < nb;
-    const uint32_t* row = bvhnodes + (size_t)(ok ? rec[r].x : 0u) * 3;
-    rp[r] = ok ? row[0] : DMSH_U32MAX;
-    ra[r] = ok ? row[1] : 0u;
-    rb[r] = ok ? row[2] : 0u;
-  }
-  uint32_t sa[R], sb[R], sp[R];
+  for (uint32_t e = threadIdx.x; e < count; e += T) s_par[e] = (uint16_t)kNone;
+  uint32_t sa[R], sb[R];
+  Box own[R];
   bool bad = false;
   uint32_t start = 0u;
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     const uint32_t e = (uint32_t)r * T + threadIdx.x;
-    sa[r] = sb[r] = sp[r] = kNone;
+    sa[r] = sb[r] = kNone;
     if (e >= count) continue;
-    if (rec[r].x >= nb) {
-      bad = true;
-      continue;
-    }
-    const bool ta = rec[r].y & 1u, tb = rec[r].y & 2u;
-    const uint32_t a = ra[r], b = rb[r], par = rp[r];
-    if ((ta ? a >= nb : a > 2 * nb) || (tb ? b >= nb : b > 2 * nb) ||
-        (par != DMSH_U32MAX && par >= nb)) {
+    const bool ta = rec[r].w & 1u, tb = rec[r].w & 2u;
+    const uint32_t a = rec[r].y, b = rec[r].z;
+    if (rec[r].x >= nb || (ta ? a >= nb : a > 2 * nb) || (tb ? b >= nb : b > 2 * nb)) {
       bad = true;
       continue;
     }
     if (ta) sa[r] = node2slot[a];
     if (tb) sb[r] = node2slot[b];
-    if (par != DMSH_U32MAX) sp[r] = node2slot[par];
+    if (!ta && !tb) {
+      own[r] = box_merge(box_load(aabb, a), box_load(aabb, b));
+      start |= 1u << r;
+    } else if (!ta || !tb) {
+      own[r] = box_load(aabb, ta ? b : a);  // the child below the top
+    }
   }
+  __syncthreads();  // s_par cleared
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     const uint32_t e = (uint32_t)r * T + threadIdx.x;
     if (e >= count || rec[r].x >= nb) continue;
-    const bool ta = rec[r].y & 1u, tb = rec[r].y & 2u;
-    if (!ta && !tb) {
-      s_box[e] = box_merge(box_load(aabb, ra[r]), box_load(aabb, rb[r]));
-      start |= 1u << r;
-    } else if (!ta || !tb) {
-      s_box[e] = box_load(aabb, ta ? rb[r] : ra[r]);  // the child below the top
-    }
-    if ((ta && sa[r] >= count) || (tb && sb[r] >= count) ||
-        (rp[r] != DMSH_U32MAX && sp[r] >= count)) {
+    const bool ta = rec[r].w & 1u, tb = rec[r].w & 2u;
+    if ((ta && sa[r] >= count) || (tb && sb[r] >= count)) {
       bad = true;
       continue;
     }
+    if (!ta || !tb) s_box[e] = own[r];
     s_kid[e] = sa[r] | (sb[r] << 16);
-    s_par[e] = (uint16_t)sp[r];
+    if (ta) s_par[sa[r]] = (uint16_t)e;
+    if (tb) s_par[sb[r]] = (uint16_t)e;
   }
   if (__syncthreads_or(bad)) {
     if (threadIdx.x == 0) raise_error(err, DMSH_DEVERR_REFIT);
@@ -753,7 +743,7 @@ template <int C, int T>
 __global__ __launch_bounds__(T) void k_refit_top_lds(
     float* __restrict__ aabb, uint32_t* __restrict__ flags, uint32_t num_leaf,
     const uint32_t* __restrict__ bvhnodes, const uint32_t* __restrict__ top_count,
-    const uint2* __restrict__ top_list, const uint32_t* __restrict__ node2slot,
+    const uint4* __restrict__ top_list, const uint32_t* __restrict__ node2slot,
     const uint32_t* __restrict__ treelet_count, const uint4* __restrict__ treelets,
     const uint32_t* __restrict__ skip, uint32_t* __restrict__ err, int rounds) {
   constexpr int R = (C + T - 1) / T;  // slots per thread
@@ -777,20 +767,13 @@ __global__ __launch_bounds__(T) void k_refit_top_lds(
   uint8_t* s_done = smem + (size_t)C * sizeof(Box);  // round that finished the slot, or kTopPending
   const uint32_t nb = num_leaf - 1;
   // phase 1, stage by stage over this thread's slots e = r T + tid
-  uint2 rec[R];
+  uint4 rec[R];
   uint2 kid[R];
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     const uint32_t e = (uint32_t)r * T + threadIdx.x;
-    rec[r] = e < count ? top_list[e] : make_uint2(DMSH_U32MAX, 0u);
-  }
-#pragma unroll
-  for (int r = 0; r < R; ++r) {
-    kid[r] = make_uint2(0u, 0u);
-    if (rec[r].x < nb) {
-      const uint32_t* pr = bvhnodes + (size_t)rec[r].x * 3;
-      kid[r] = make_uint2(pr[1], pr[2]);
-    }
+    rec[r] = e < count ? top_list[e] : make_uint4(DMSH_U32MAX, 0u, 0u, 0u);
+    kid[r] = make_uint2(rec[r].y, rec[r].z);
   }
   bool bad = false;
   uint32_t pending = 0u;  // bit r: slot r T + tid still to merge
@@ -798,7 +781,7 @@ __global__ __launch_bounds__(T) void k_refit_top_lds(
   for (int r = 0; r < R; ++r) {
     const uint32_t e = (uint32_t)r * T + threadIdx.x;
     if (e >= count) continue;
-    const bool ta = rec[r].y & 1u, tb = rec[r].y & 2u;
+    const bool ta = rec[r].w & 1u, tb = rec[r].w & 2u;
     const uint32_t a = kid[r].x, b = kid[r].y;
     bool ok = rec[r].x < nb && (ta ? a < nb : a <= 2 * nb) && (tb ? b < nb : b <= 2 * nb);
     if (ok && ta) {
@@ -881,7 +864,7 @@ template <int LK>
 inline int launch_refit_treelets(hipStream_t s, const TreeView& v, uint32_t* flags,
                                  const uint32_t* treelet_count, const uint4* treelets, float eps,
                                  uint32_t* err, const uint32_t* top_count = nullptr,
-                                 const uint2* top_list = nullptr,
+                                 const uint4* top_list = nullptr,
                                  const uint32_t* node2slot = nullptr,
                                  uint32_t* treelet_next = nullptr,
                                  const uint32_t* skip = nullptr, bool leaves_in_aabb = false) {
@@ -944,7 +927,7 @@ struct StandaloneLayout {
   uint32_t* node2slot; // num_leaf - 1
   uint32_t* counters;  // treelet count, top count, next treelet, bad, plan needed
   uint4* treelets;     // num_leaf
-  uint2* top_list;     // kTopSlots
+  uint4* top_list;     // kTopSlots: (node, left child, right child, top-child flags)
   unsigned long long* hash;  // -, stamp of the kept plan, tag, -, row-hash partials
 };
 
@@ -956,7 +939,7 @@ inline bool carve_standalone(Arena& a, uint32_t n, StandaloneLayout& L) {
   L.node2slot = a.take<uint32_t>(nb);
   L.counters = a.take<uint32_t>(8);
   L.treelets = a.take<uint4>(n ? n : 1);
-  L.top_list = a.take<uint2>(kTopSlots);
+  L.top_list = a.take<uint4>(kTopSlots);
   L.hash = a.take<unsigned long long>(4 + kPlanHashBlocks);
   return a.ok();
 }
@@ -1128,7 +1111,7 @@ template <int T>
 __global__ __launch_bounds__(T) void k_list_treelets(
     uint32_t num_leaf, const uint32_t* __restrict__ bvhnodes, const uint32_t* __restrict__ first,
     const uint32_t* __restrict__ last, uint32_t* __restrict__ counters,
-    uint4* __restrict__ treelets, uint2* __restrict__ top_list, uint32_t* __restrict__ node2slot) {
+    uint4* __restrict__ treelets, uint4* __restrict__ top_list, uint32_t* __restrict__ node2slot) {
   __shared__ uint32_t s_cnt[2];  // treelets, tops of this round
   __shared__ unsigned long long s_base;
   if (counters[4] == 0u) return;  // the kept plan is valid
@@ -1138,7 +1121,7 @@ __global__ __launch_bounds__(T) void k_list_treelets(
     const uint32_t i = base + threadIdx.x;
     uint32_t n_tl = 0, n_top = 0;
     uint4 tl[2];
-    uint2 top = make_uint2(0u, 0u);
+    uint4 top = make_uint4(0u, 0u, 0u, 0u);
     if (i < nb) {
       const uint32_t* row = bvhnodes + (size_t)i * 3;
       const uint32_t p = row[0], a = row[1], b = row[2];
@@ -1157,7 +1140,7 @@ __global__ __launch_bounds__(T) void k_list_treelets(
         atomicOr(counters + 3, 1u);
       } else if (l - f + 1 > K) {
         n_top = 1;
-        top = make_uint2(i, (la - fa + 1 > K ? 1u : 0u) | (lb - fb + 1 > K ? 2u : 0u));
+        top = make_uint4(i, a, b, (la - fa + 1 > K ? 1u : 0u) | (lb - fb + 1 > K ? 2u : 0u));
         if (la - fa + 1 <= K) tl[n_tl++] = make_uint4(a, fa, la, 0u);
         if (lb - fb + 1 <= K) tl[n_tl++] = make_uint4(b, fb, lb, 0u);
       } else if (i == 0) {

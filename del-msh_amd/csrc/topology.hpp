@@ -121,8 +121,9 @@ __device__ __forceinline__ uint32_t node_split(const Codes& c, uint32_t first, u
 // them.  Batched builds also append each treelet root to its mesh's own list
 // (mesh_roots + root_base, counted in mesh_root_count[mesh]) for the top climb.
 // Optionally the nodes above the treelets ("top" nodes, range > k) are
-// listed too: top_list[slot] = (node, bit 0: left child is top, bit 1: right
-// child is top) and node2slot[node] = slot, for slots < top_cap; top_count
+// listed too: top_list[slot] = (node, left child, right child, bit 0: left
+// child is top, bit 1: right child is top) and node2slot[node] = slot, for
+// slots < top_cap; top_count
 // counts all of them, so a count above top_cap says the list is incomplete.
 struct TreeletSink {
   uint32_t k;
@@ -132,7 +133,7 @@ struct TreeletSink {
   uint32_t* mesh_root_count;  // batched only, zeroed by the caller
   uint32_t* mesh_roots;
   uint32_t* top_count = nullptr;  // zeroed by the caller; nullptr: no top list
-  uint2* top_list = nullptr;
+  uint4* top_list = nullptr;
   uint32_t top_cap = 0;
   uint32_t* node2slot = nullptr;
   // Optional: the leaf boxes, written into leaf_aabb (the tree's bvhnode2aabb)
@@ -188,8 +189,10 @@ __device__ __forceinline__ void topology_emit(uint32_t n, uint32_t i, uint32_t f
       if (ts.top_count) {
         const uint32_t slot = atomicAdd(ts.top_count, 1u);
         if (slot < ts.top_cap) {
-          ts.top_list[slot] = make_uint2(i, (split - first + 1 > ts.k ? 1u : 0u) |
-                                                (last - split > ts.k ? 2u : 0u));
+          ts.top_list[slot] = make_uint4(i, first == split ? nb + split : split,
+                                         last == split + 1 ? nb + split + 1 : split + 1,
+                                         (split - first + 1 > ts.k ? 1u : 0u) |
+                                             (last - split > ts.k ? 2u : 0u));
           ts.node2slot[i] = slot;
         }
       }

@@ -169,7 +169,7 @@ int dmsh_build_trimesh3(void *stream, uint32_t num_tri, const uint32_t *tri2vtx,
  * the workspace is a fixed DMSH_NOALLOC_SCRATCH_BYTES (64 MiB) device array
  * inside the library, one per GPU, held until the process ends.  Meshes
  * whose dmsh_build_workspace_bytes(num_tri, morton_bits) exceeds it (above
- * 772,654 triangles with 63-bit codes, 870,528 with 30-bit codes) are
+ * 772,032 triangles with 63-bit codes, 869,840 with 30-bit codes) are
  * rejected with DMSH_ERR_TOO_LARGE before anything is enqueued.  It calls no
  * hipMalloc / hipFree.  Every call on a GPU shares that GPU's scratch: calls
  * that overlap on one GPU (on different streams, or from different host
