@@ -259,6 +259,8 @@ struct BuildLayout {
   uint32_t* treelet_next;
   uint4* top_list;
   uint32_t* node2slot;
+  uint32_t* top_kid;
+  float* top_box;
   float* cntr;
   float* tri_box;
   void* codes;
@@ -289,6 +291,8 @@ inline bool carve(Arena& a, uint32_t n, bool tri, BuildLayout& L) {
   L.treelets = a.take<uint4>(n);
   L.top_list = a.take<uint4>(kTopSlots);
   L.node2slot = a.take<uint32_t>(n);
+  L.top_kid = a.take<uint32_t>(kTopClimbSlots);
+  L.top_box = a.take<float>((size_t)kTopClimbSlots * 12);
   L.cntr = tri ? a.take<float>((size_t)n * 3) : nullptr;
   L.tri_box = tri ? a.take<float>((size_t)n * 6) : nullptr;
   L.codes = a.take<KeyT>(n);
@@ -351,7 +355,7 @@ inline int build(hipStream_t s, uint32_t n, const uint32_t* tri2vtx, uint32_t nu
   const TreeView v{aabb, bvhnodes, n, tri2vtx, num_vtx, xyz, L.tri_box};
   return launch_refit_treelets<TRI ? kLeafTri : kLeafPoint>(
       s, v, L.flags, L.treelet_count, L.treelets, 0.f, err, L.top_count, L.top_list, L.node2slot,
-      L.treelet_next, nullptr, leaves_in_aabb);
+      L.treelet_next, nullptr, leaves_in_aabb, L.top_kid, L.top_box);
 }
 
 }  // namespace bld

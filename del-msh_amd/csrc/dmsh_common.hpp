@@ -208,7 +208,8 @@ enum RefitOption {
   kRefitQueue = 2,
   kRefitTop = 3,
   kRefitPlanCache = 4,
-  kRefitOptionCount = 5
+  kRefitCompact = 5,
+  kRefitOptionCount = 6
 };
 int refit_option(RefitOption which);
 bool set_refit_option(const char* key, int value);

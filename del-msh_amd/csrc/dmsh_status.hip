@@ -26,16 +26,18 @@ static std::mutex g_err_mutex;
 // when there are at most 4096 of them; 1: they merge in barrier rounds),
 // "refit.plan_cache" (1: a standalone refit whose workspace already holds
 // the plan of the same rows skips the plan kernels; 0: always rebuild it)
-static std::atomic<int> g_refit_option[kRefitOptionCount] = {{0}, {1}, {1}, {0}, {1}};
+static std::atomic<int> g_refit_option[kRefitOptionCount] = {{0}, {2}, {1}, {0}, {1}, {0}};
 static const char* const kRefitOptionKey[kRefitOptionCount] = {
-    "refit.leaves_in_aabb", "refit.climb", "refit.queue", "refit.top", "refit.plan_cache"};
+    "refit.leaves_in_aabb", "refit.climb", "refit.queue", "refit.top", "refit.plan_cache",
+    "refit.compact"};
+static const int kRefitOptionMax[kRefitOptionCount] = {1, 2, 1, 1, 1, 1023};
 
 int refit_option(RefitOption which) { return g_refit_option[which].load(); }
 
 bool set_refit_option(const char* key, int value) {
   for (int i = 0; i < kRefitOptionCount; ++i) {
     if (std::strcmp(key, kRefitOptionKey[i]) == 0) {
-      if (value != 0 && value != 1) return false;
+      if (value < 0 || value > kRefitOptionMax[i]) return false;
       g_refit_option[i].store(value);
       return true;
     }

@@ -56,6 +56,17 @@ __device__ __forceinline__ Box box_merge(const Box& a, const Box& b) {
   }
   return r;
 }
+// box_merge(mine, other) when mine_left, else box_merge(other, mine), with
+// the operands selected first so that both orders run the same instructions.
+__device__ __forceinline__ Box box_merge_side(const Box& mine, const Box& other, bool mine_left) {
+  Box a, b;
+#pragma unroll
+  for (int d = 0; d < 6; ++d) {
+    a.v[d] = mine_left ? mine.v[d] : other.v[d];
+    b.v[d] = mine_left ? other.v[d] : mine.v[d];
+  }
+  return box_merge(a, b);
+}
 __device__ __forceinline__ Box box_load(const float* __restrict__ aabb, uint32_t node) {
   const float2* p = reinterpret_cast<const float2*>(aabb + (size_t)node * 6);
   const float2 a = p[0], b = p[1], c = p[2];
@@ -215,6 +226,25 @@ struct TreeletLds {
   uint32_t flag[K];
   uint2 child[K];
   uint32_t parent[K];
+  uint16_t queue[K];   // CLIMB 2: the nodes whose children are both leaves
+  uint16_t queue2[K];  // CLIMB 2: the climbs still running after the first levels
+  uint32_t queued, queued2;  // zero between treelets
+};
+
+// What the treelet pass leaves for the top merge, in top-slot order, so that
+// the one workgroup of the top merge reads it with coalesced loads instead
+// of gathering it: kid[e] packs the slots of top slot e's top children (16
+// bits each; 0xFFFF for a child below the top, 0xFFFE for a slot out of
+// range) and box[2 e + side] the box of a child below the top (a treelet
+// root; side 0 is the left child).  Kept for slots below kTopClimbSlots.
+constexpr int kTopClimbSlots = 4096;
+
+struct TopStage {
+  const uint32_t* top_count = nullptr;
+  const uint4* top_list = nullptr;
+  const uint32_t* node2slot = nullptr;
+  uint32_t* kid = nullptr;
+  float* box = nullptr;
 };
 
 // A treelet's inputs when its leaf boxes are already in v.aabb (PRE): the
@@ -268,7 +298,9 @@ template <int LK, int K, int T, bool PRE = false, int CLIMB = 0>
 __device__ __forceinline__ void refit_treelet(const TreeView& v, uint32_t root, uint32_t f,
                                               uint32_t l, float eps, TreeletLds<K>& s,
                                               uint32_t* __restrict__ err,
-                                              const TreeletIn<K / T>* pre = nullptr) {
+                                              const TreeletIn<K / T>* pre = nullptr,
+                                              int compact = 4, bool staged = false,
+                                              TopStage st = TopStage{}) {
   constexpr int LPT = K / T;
   const uint32_t num_leaf = v.num_leaf;
   const uint32_t nb = num_leaf - 1;
@@ -307,6 +339,16 @@ __device__ __forceinline__ void refit_treelet(const TreeView& v, uint32_t root, 
       const uint32_t* row = v.bvhnodes + (size_t)id * 3;
       s.parent[j] = row[0];
       s.child[j] = make_uint2(row[1], row[2]);
+      if (CLIMB == 2) {
+        // a child that is a leaf of this treelet has arrived already; a node
+        // whose children are both leaves is merged first, from the queue
+        // (an id in [f, l] outside the treelet has no leaf child in it)
+        const uint32_t lo = nb + f, hi = nb + l;
+        const uint32_t nl = (row[1] >= lo && row[1] <= hi ? 1u : 0u) +
+                            (row[2] >= lo && row[2] <= hi ? 1u : 0u);
+        s.flag[j] = nl;
+        if (nl == 2u) s.queue[atomicAdd(&s.queued, 1u)] = (uint16_t)j;
+      }
     }
     Box b;
     uint32_t p;
@@ -317,6 +359,16 @@ __device__ __forceinline__ void refit_treelet(const TreeView& v, uint32_t root, 
     }
   }
   __syncthreads();
+  // the top slot of the root's parent and its left child, for the staged
+  // root box (read now, used after the climb)
+  uint32_t st_slot = DMSH_U32MAX, st_left = DMSH_U32MAX;
+  if (staged && threadIdx.x == 0) {
+    const uint32_t p = root >= nb ? v.bvhnodes[(size_t)root * 3] : s.parent[root - f];
+    if (p < nb) {
+      st_slot = st.node2slot[p];
+      st_left = v.bvhnodes[(size_t)p * 3 + 1];
+    }
+  }
   // phase 2: a thread's LPT climbs advance in lock-step, one level per round
   // (a round costs as much as the longest of them, not their sum); the
   // parent's static links are read before its arrival counter, so a level
@@ -333,6 +385,79 @@ __device__ __forceinline__ void refit_treelet(const TreeView& v, uint32_t root, 
     act[r] = cpar[r] != DMSH_U32MAX - 1;
     cnode[r] = nb + f + j;
     if (act[r]) mine[r] = s.leaf[j];
+  }
+  if constexpr (CLIMB == 2) {
+    // CLIMB 2: for trees whose layout is known good (built here, or checked
+    // by k_list_treelets).  Leaves do not climb: the nodes whose children
+    // are both leaves (queued in phase 1) are merged by the first lanes of
+    // the workgroup and climb from there, one LDS round trip per level as in
+    // CLIMB 1.  After `compact` levels the climbs still running are queued
+    // again and continued by the first lanes, so the climbs run on as few
+    // waves as they need while the other waves wait at the barrier.
+    static_assert(LPT == 1, "one leaf per thread");
+    static_assert(!PRE, "the leaf boxes come from phase 1");
+    const uint32_t starts = s.queued;
+    bool a = threadIdx.x < starts;
+    uint32_t node = DMSH_U32MAX, p = DMSH_U32MAX, gp = DMSH_U32MAX;
+    uint2 ch = make_uint2(DMSH_U32MAX, DMSH_U32MAX);
+    Box me;
+    if (a) {
+      const uint32_t j = s.queue[threadIdx.x];
+      node = f + j;
+      const uint2 c = s.child[j];
+      me = box_merge(s.leaf[c.x - nb - f], s.leaf[c.y - nb - f]);
+      s.inner[j] = me;
+      if (node == root) {
+        a = false;
+      } else {
+        p = s.parent[j];
+        ch = s.child[p - f];
+        gp = s.parent[p - f];
+      }
+    }
+    for (int stage = 0; stage < 2; ++stage) {
+      const uint32_t limit = (stage == 0 && compact > 0) ? (uint32_t)compact : (uint32_t)K;
+      for (uint32_t steps = 0; a && steps < limit; ++steps) {
+        const bool left = ch.x == node;
+        const uint32_t sib = left ? ch.y : ch.x;
+        const uint32_t pj = p - f;
+        const uint32_t old = __hip_atomic_fetch_add(&s.flag[pj], 1u, __ATOMIC_ACQ_REL,
+                                                    __HIP_MEMORY_SCOPE_WAVEFRONT);
+        const Box sb = (sib >= nb) ? s.leaf[sib - nb - f] : s.inner[sib - f];
+        uint2 ch2 = make_uint2(DMSH_U32MAX, DMSH_U32MAX);
+        uint32_t gp2 = DMSH_U32MAX;
+        if (p != root) {
+          ch2 = s.child[gp - f];
+          gp2 = s.parent[gp - f];
+        }
+        if (old == 0) {  // the sibling subtree finishes this node
+          a = false;
+          break;
+        }
+        me = box_merge_side(me, sb, left);
+        s.inner[pj] = me;
+        if (p == root) {
+          a = false;
+          break;
+        }
+        node = p;
+        p = gp;
+        ch = ch2;
+        gp = gp2;
+      }
+      if (stage == 1 || compact <= 0 || compact >= K) break;
+      if (a) s.queue2[atomicAdd(&s.queued2, 1u)] = (uint16_t)(node - f);
+      __syncthreads();
+      const uint32_t q = s.queued2;
+      a = threadIdx.x < q;
+      if (a) {
+        node = f + s.queue2[threadIdx.x];
+        me = s.inner[node - f];
+        p = s.parent[node - f];
+        ch = s.child[p - f];
+        gp = s.parent[p - f];
+      }
+    }
   }
   if (CLIMB == 1) {
     uint2 ch[LPT];
@@ -454,6 +579,14 @@ __device__ __forceinline__ void refit_treelet(const TreeView& v, uint32_t root, 
     const uint32_t j = (uint32_t)r * T + threadIdx.x;
     if (j < m && f + j < nb && s.flag[j] >= 2u) box_store(v.aabb, f + j, s.inner[j]);
   }
+  if (st_slot < (uint32_t)kTopClimbSlots) {  // thread 0
+    const Box b = root >= nb ? s.leaf[root - nb - f] : s.inner[root - f];
+    box_store(st.box, 2u * st_slot + (st_left == root ? 0u : 1u), b);
+  }
+  if (CLIMB == 2 && threadIdx.x == 0) {
+    s.queued = 0u;
+    s.queued2 = 0u;
+  }
   __syncthreads();  // LDS is reused by the next treelet
 }
 
@@ -500,18 +633,40 @@ __device__ __forceinline__ void refit_top(const TreeView& v, uint32_t* __restric
 // without it, treelet t goes to workgroup t mod grid.  A nonzero *skip (the
 // standalone refit's layout check failed) makes the kernel do nothing.
 template <int LK, int K, int T, bool PRE, int CLIMB>
-__global__ __launch_bounds__(T) void k_refit_treelet(TreeView v, float eps,
+__global__ __launch_bounds__(T) __attribute__((amdgpu_num_sgpr(72))) void k_refit_treelet(TreeView v, float eps,
                                                      const uint32_t* __restrict__ treelet_count,
                                                      const uint4* __restrict__ treelets,
                                                      uint32_t* __restrict__ next,
                                                      const uint32_t* __restrict__ skip,
-                                                     uint32_t* __restrict__ err) {
+                                                     uint32_t* __restrict__ err, int compact,
+                                                     TopStage st) {
   __shared__ TreeletLds<K> s;
   __shared__ uint32_t s_t, s_t2;
   if (skip && *skip) return;
   uint32_t count = *treelet_count;
   if (count > v.num_leaf) count = v.num_leaf;
-  if (PRE) {
+  if (CLIMB == 2) {
+    if (threadIdx.x == 0) {
+      s.queued = 0u;
+      s.queued2 = 0u;
+    }
+    __syncthreads();
+  }
+  const bool staged = st.kid != nullptr;
+  if (staged) {
+    // the top slots' top-child slots (a gather spread over the first
+    // workgroups, so that the top merge's own workgroup need not do it)
+    const uint32_t tc = *st.top_count, nb = v.num_leaf - 1;
+    const uint32_t e = blockIdx.x * T + threadIdx.x;
+    if (tc <= (uint32_t)kTopClimbSlots && e < tc) {
+      const uint4 r = st.top_list[e];
+      uint32_t sa = 0xFFFFu, sb = 0xFFFFu;
+      if (r.w & 1u) sa = r.y < nb && st.node2slot[r.y] < tc ? st.node2slot[r.y] : 0xFFFEu;
+      if (r.w & 2u) sb = r.z < nb && st.node2slot[r.z] < tc ? st.node2slot[r.z] : 0xFFFEu;
+      st.kid[e] = sa | (sb << 16);
+    }
+  }
+  if constexpr (PRE) {
     // Two treelets in flight per workgroup: the next one's loads are issued
     // before the current one's climb.  Treelet numbers run three deep: t is
     // being climbed, tn's leaves are loading, t3's descriptor is loading, and
@@ -560,7 +715,8 @@ __global__ __launch_bounds__(T) void k_refit_treelet(TreeView v, float eps,
       // so the wait for it (and for the loads before it) comes after the
       // climb, where the ticket is stored
       if (next && threadIdx.x == 0) t4 = atomicAdd(next + __lane_id(), 1u);
-      refit_treelet<LK, K, T, true, CLIMB>(v, d.x, d.y, d.z, eps, s, err, &cur);
+      refit_treelet<LK, K, T, true, CLIMB>(v, d.x, d.y, d.z, eps, s, err, &cur, compact, staged,
+                                           st);
       if (next && threadIdx.x == 0) s_t2 = t4;
       __syncthreads();  // thread 0's ticket is visible (also after an early return)
       t4 = next ? s_t2 : t3 + gridDim.x;
@@ -572,23 +728,23 @@ __global__ __launch_bounds__(T) void k_refit_treelet(TreeView v, float eps,
       dn = d3;
       t3 = t4;
     }
-    return;
-  }
-  if (!next) {
+  } else if (!next) {
     for (uint32_t t = blockIdx.x; t < count; t += gridDim.x) {
       const uint4 tl = treelets[t];
-      refit_treelet<LK, K, T, false, CLIMB>(v, tl.x, tl.y, tl.z, eps, s, err);
+      refit_treelet<LK, K, T, false, CLIMB>(v, tl.x, tl.y, tl.z, eps, s, err, nullptr, compact,
+                                            staged, st);
     }
-    return;
-  }
-  for (;;) {
-    if (threadIdx.x == 0) s_t = atomicAdd(next, 1u);
-    __syncthreads();
-    const uint32_t t = s_t;
-    __syncthreads();  // every thread has read s_t before it is reused
-    if (t >= count) break;
-    const uint4 tl = treelets[t];
-    refit_treelet<LK, K, T, false, CLIMB>(v, tl.x, tl.y, tl.z, eps, s, err);
+  } else {
+    for (;;) {
+      if (threadIdx.x == 0) s_t = atomicAdd(next, 1u);
+      __syncthreads();
+      const uint32_t t = s_t;
+      __syncthreads();  // every thread has read s_t before it is reused
+      if (t >= count) break;
+      const uint4 tl = treelets[t];
+      refit_treelet<LK, K, T, false, CLIMB>(v, tl.x, tl.y, tl.z, eps, s, err, nullptr, compact,
+                                            staged, st);
+    }
   }
 }
 
@@ -628,110 +784,126 @@ constexpr uint8_t kTopPending = 0xFFu;
 // slot e keeps its box, its parent's slot (16 bits) and its top children's
 // slots (16 bits each, 0xFFFF for a child below the top, whose box the
 // slot's own entry holds until the merge), and one byte of a packed counter
-// word.  A slot whose children are both below the top is finished in the
-// load pass and starts a climb; at each parent the arrival that completes
-// its top children merges (left, right) and climbs on, so a chain costs one
-// LDS round trip per level instead of one barrier per level.
-constexpr int kTopClimbSlots = 4096;
-
+// word.  The treelet pass staged each slot's top-child slots and the boxes
+// of its children below the top in slot order (TopStage), so the load pass
+// is three coalesced reads per slot.  A slot whose children are both below
+// the top is finished in the load pass and starts a climb; at each parent
+// the arrival that completes its top children merges (left, right) and
+// climbs on, one LDS round trip per level.  The climbs run in stages of
+// `steps` levels from a queue: the climbs still going at the end of a stage
+// are queued again, so every stage runs on as few waves as it has climbs.
 template <int C, int T>
 __device__ __forceinline__ void top_climb(float* __restrict__ aabb, uint32_t num_leaf,
-                                          const uint32_t* __restrict__ bvhnodes, uint32_t count,
-                                          const uint4* __restrict__ top_list,
-                                          const uint32_t* __restrict__ node2slot,
-                                          unsigned char* __restrict__ smem,
-                                          uint32_t* __restrict__ err) {
+                                          uint32_t count, const uint4* __restrict__ top_list,
+                                          const TopStage& st, unsigned char* __restrict__ smem,
+                                          int steps, uint32_t* __restrict__ err) {
   constexpr int R = (C + T - 1) / T;
   constexpr uint32_t kNone = 0xFFFFu;
   Box* s_box = reinterpret_cast<Box*>(smem);
   uint16_t* s_par = reinterpret_cast<uint16_t*>(smem + (size_t)C * sizeof(Box));
   uint32_t* s_kid = reinterpret_cast<uint32_t*>(smem + (size_t)C * (sizeof(Box) + 2));
   uint32_t* s_cnt = reinterpret_cast<uint32_t*>(smem + (size_t)C * (sizeof(Box) + 6));
+  uint16_t* s_q = reinterpret_cast<uint16_t*>(smem + (size_t)C * (sizeof(Box) + 7));
+  uint32_t* s_qn = reinterpret_cast<uint32_t*>(smem + (size_t)C * (sizeof(Box) + 11));
   const uint32_t nb = num_leaf - 1;
+  const uint32_t S = steps > 0 ? (uint32_t)steps : (uint32_t)C;
   for (uint32_t w = threadIdx.x; w < (count + 3) / 4; w += T) s_cnt[w] = 0u;
-  // load pass over this thread's slots e = r T + tid, in two stages whose
-  // loads are in flight together: the records (node, children, which
-  // children are top nodes), then the top children's slots and the boxes of
-  // the children below the top.  A slot's parent is the slot that names it
-  // as a top child.
-  uint4 rec[R];
-#pragma unroll
-  for (int r = 0; r < R; ++r) {
-    const uint32_t e = (uint32_t)r * T + threadIdx.x;
-    rec[r] = e < count ? top_list[e] : make_uint4(DMSH_U32MAX, 0u, 0u, 0u);
-  }
   for (uint32_t e = threadIdx.x; e < count; e += T) s_par[e] = (uint16_t)kNone;
-  uint32_t sa[R], sb[R];
+  if (threadIdx.x < 3) s_qn[threadIdx.x] = 0u;
+  uint4 rec[R];
+  uint32_t kid[R];
   Box own[R];
-  bool bad = false;
-  uint32_t start = 0u;
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     const uint32_t e = (uint32_t)r * T + threadIdx.x;
-    sa[r] = sb[r] = kNone;
+    rec[r] = make_uint4(DMSH_U32MAX, 0u, 0u, 0u);
+    kid[r] = DMSH_U32MAX;
+    if (e < count) {
+      rec[r] = top_list[e];
+      kid[r] = st.kid[e];
+      const float4* bp = reinterpret_cast<const float4*>(st.box + (size_t)e * 12);
+      const float4 x = bp[0], y = bp[1], z = bp[2];
+      const Box a{{x.x, x.y, x.z, x.w, y.x, y.y}}, b{{y.z, y.w, z.x, z.y, z.z, z.w}};
+      const bool ta = rec[r].w & 1u, tb = rec[r].w & 2u;
+      own[r] = (!ta && !tb) ? box_merge(a, b) : (ta ? b : a);
+    }
+  }
+  __syncthreads();  // s_par, s_cnt and the queue counters cleared
+  bool bad = false;
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const uint32_t e = (uint32_t)r * T + threadIdx.x;
     if (e >= count) continue;
     const bool ta = rec[r].w & 1u, tb = rec[r].w & 2u;
-    const uint32_t a = rec[r].y, b = rec[r].z;
-    if (rec[r].x >= nb || (ta ? a >= nb : a > 2 * nb) || (tb ? b >= nb : b > 2 * nb)) {
-      bad = true;
-      continue;
-    }
-    if (ta) sa[r] = node2slot[a];
-    if (tb) sb[r] = node2slot[b];
-    if (!ta && !tb) {
-      own[r] = box_merge(box_load(aabb, a), box_load(aabb, b));
-      start |= 1u << r;
-    } else if (!ta || !tb) {
-      own[r] = box_load(aabb, ta ? b : a);  // the child below the top
-    }
-  }
-  __syncthreads();  // s_par cleared
-#pragma unroll
-  for (int r = 0; r < R; ++r) {
-    const uint32_t e = (uint32_t)r * T + threadIdx.x;
-    if (e >= count || rec[r].x >= nb) continue;
-    const bool ta = rec[r].w & 1u, tb = rec[r].w & 2u;
-    if ((ta && sa[r] >= count) || (tb && sb[r] >= count)) {
+    const uint32_t sa = kid[r] & kNone, sb = kid[r] >> 16;
+    if (rec[r].x >= nb || (ta ? sa >= count : sa != kNone) || (tb ? sb >= count : sb != kNone)) {
       bad = true;
       continue;
     }
     if (!ta || !tb) s_box[e] = own[r];
-    s_kid[e] = sa[r] | (sb[r] << 16);
-    if (ta) s_par[sa[r]] = (uint16_t)e;
-    if (tb) s_par[sb[r]] = (uint16_t)e;
+    s_kid[e] = kid[r];
+    if (ta) s_par[sa] = (uint16_t)e;
+    if (tb) s_par[sb] = (uint16_t)e;
+    if (!ta && !tb) s_q[atomicAdd(s_qn, 1u)] = (uint16_t)e;
   }
   if (__syncthreads_or(bad)) {
     if (threadIdx.x == 0) raise_error(err, DMSH_DEVERR_REFIT);
     return;
   }
-  // climbs; the next parent's static slot and links are read while the
-  // current parent's counter is in flight
-#pragma unroll
-  for (int r = 0; r < R; ++r) {
-    if (!(start & (1u << r))) continue;
-    uint32_t p = s_par[(uint32_t)r * T + threadIdx.x];
-    for (uint32_t steps = 0; p != kNone; ++steps) {
-      if (steps >= count) {
-        raise_error(err, DMSH_DEVERR_REFIT);
-        break;
-      }
-      const uint32_t kid = s_kid[p];
-      const uint32_t pnext = s_par[p];
-      const uint32_t la = kid & kNone, lb = kid >> 16;
-      const uint32_t need = (la != kNone ? 1u : 0u) + (lb != kNone ? 1u : 0u);
-      const uint32_t sh = 8u * (p & 3u);
-      const uint32_t old = (__hip_atomic_fetch_add(s_cnt + (p >> 2), 1u << sh, __ATOMIC_ACQ_REL,
-                                                   __HIP_MEMORY_SCOPE_WORKGROUP) >>
-                            sh) &
-                           0xFFu;
-      if (old + 1u < need) break;  // the other top child finishes this node
-      const Box A = la != kNone ? s_box[la] : s_box[p];
-      const Box B = lb != kNone ? s_box[lb] : s_box[p];
-      s_box[p] = box_merge(A, B);
-      p = pnext;
+  // stage k reads queue k % 2 (length s_qn[k % 3]), queues the climbs still
+  // going into queue (k + 1) % 2 and clears the counter stage k + 1 appends to
+  for (uint32_t k = 0;; ++k) {
+    const uint32_t nq = s_qn[k % 3];
+    if (nq == 0u) break;
+    if (k > count) {  // more stages than slots: a parent cycle
+      if (threadIdx.x == 0) raise_error(err, DMSH_DEVERR_REFIT);
+      return;
     }
+    if (threadIdx.x == 0) s_qn[(k + 2) % 3] = 0u;
+    const uint16_t* q = s_q + (size_t)(k & 1u) * C;
+    uint16_t* qn = s_q + (size_t)((k + 1) & 1u) * C;
+    for (uint32_t i = threadIdx.x; i < nq; i += T) {
+      // a climb from slot c (finished) to its parent p; the parent's links
+      // are read one level ahead, so the counter, both children's boxes and
+      // the next links go out together: one LDS round trip per level
+      uint32_t c = q[i];
+      uint32_t p = s_par[c];
+      if (p == kNone) continue;
+      uint32_t kp = s_kid[p], pn = s_par[p];
+      bool go = true;
+      for (uint32_t j = 0; j < S; ++j) {
+        const uint32_t la = kp & kNone, lb = kp >> 16;
+        const uint32_t need = (la != kNone ? 1u : 0u) + (lb != kNone ? 1u : 0u);
+        const uint32_t sh = 8u * (p & 3u);
+        const uint32_t old = (__hip_atomic_fetch_add(s_cnt + (p >> 2), 1u << sh, __ATOMIC_ACQ_REL,
+                                                     __HIP_MEMORY_SCOPE_WORKGROUP) >>
+                              sh) &
+                             0xFFu;
+        const Box A = la != kNone ? s_box[la] : s_box[p];
+        const Box B = lb != kNone ? s_box[lb] : s_box[p];
+        uint32_t kp2 = 0u, pn2 = kNone;
+        if (pn != kNone) {
+          kp2 = s_kid[pn];
+          pn2 = s_par[pn];
+        }
+        if (old + 1u < need) {  // the other top child finishes this node
+          go = false;
+          break;
+        }
+        s_box[p] = box_merge(A, B);
+        c = p;
+        p = pn;
+        kp = kp2;
+        pn = pn2;
+        if (p == kNone) {  // the top root is done
+          go = false;
+          break;
+        }
+      }
+      if (go) qn[atomicAdd(s_qn + (k + 1) % 3, 1u)] = (uint16_t)c;
+    }
+    __syncthreads();
   }
-  __syncthreads();
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     const uint32_t e = (uint32_t)r * T + threadIdx.x;
@@ -745,16 +917,17 @@ __global__ __launch_bounds__(T) void k_refit_top_lds(
     const uint32_t* __restrict__ bvhnodes, const uint32_t* __restrict__ top_count,
     const uint4* __restrict__ top_list, const uint32_t* __restrict__ node2slot,
     const uint32_t* __restrict__ treelet_count, const uint4* __restrict__ treelets,
-    const uint32_t* __restrict__ skip, uint32_t* __restrict__ err, int rounds) {
+    const uint32_t* __restrict__ skip, uint32_t* __restrict__ err, int rounds, TopStage st,
+    int steps) {
   constexpr int R = (C + T - 1) / T;  // slots per thread
-  static_assert((size_t)kTopClimbSlots * (sizeof(Box) + 7) <= (size_t)C * (sizeof(Box) + 1),
+  static_assert((size_t)kTopClimbSlots * (sizeof(Box) + 11) + 12 <= (size_t)C * (sizeof(Box) + 1),
                 "the climb layout fits in the round layout's LDS");
   __shared__ __align__(16) unsigned char smem[(size_t)C * (sizeof(Box) + 1)];
   if (skip && *skip) return;
   const TreeView v{aabb, bvhnodes, num_leaf, nullptr, 0u, nullptr};
   const uint32_t count = *top_count;
-  if (!rounds && count <= (uint32_t)kTopClimbSlots) {
-    top_climb<kTopClimbSlots, T>(aabb, num_leaf, bvhnodes, count, top_list, node2slot, smem, err);
+  if (!rounds && st.kid && count <= (uint32_t)kTopClimbSlots) {
+    top_climb<kTopClimbSlots, T>(aabb, num_leaf, count, top_list, st, smem, steps, err);
     return;
   }
   if (count > (uint32_t)C) {
@@ -867,32 +1040,40 @@ inline int launch_refit_treelets(hipStream_t s, const TreeView& v, uint32_t* fla
                                  const uint4* top_list = nullptr,
                                  const uint32_t* node2slot = nullptr,
                                  uint32_t* treelet_next = nullptr,
-                                 const uint32_t* skip = nullptr, bool leaves_in_aabb = false) {
+                                 const uint32_t* skip = nullptr, bool leaves_in_aabb = false,
+                                 uint32_t* top_kid = nullptr, float* top_box = nullptr) {
   const uint32_t num_leaf = v.num_leaf;
   if (num_leaf == 0) return DMSH_OK;
+  TopStage st;
+  if (top_count && top_kid && top_box && num_leaf > kTreeletLeaves)
+    st = TopStage{top_count, top_list, node2slot, top_kid, top_box};
   const uint32_t grid = std::min<uint32_t>(kTreeletGrid, num_leaf);
   constexpr int K = (int)kTreeletLeaves, Th = kTreeletThreads;
-  const bool climb1 = refit_option(kRefitClimb) == 1;
+  const int climb = refit_option(kRefitClimb);
+  const int compact = refit_option(kRefitCompact);
   if (refit_option(kRefitQueue) == 0) treelet_next = nullptr;
-  if (leaves_in_aabb && climb1) {
+  if (leaves_in_aabb && climb >= 1) {
     launch("k_refit_treelet", (k_refit_treelet<LK, K, Th, true, 1>), dim3(grid), dim3(Th), 0, s,
-           v, eps, treelet_count, treelets, treelet_next, skip, err);
+           v, eps, treelet_count, treelets, treelet_next, skip, err, compact, st);
   } else if (leaves_in_aabb) {
     launch("k_refit_treelet", (k_refit_treelet<LK, K, Th, true, 0>), dim3(grid), dim3(Th), 0, s,
-           v, eps, treelet_count, treelets, treelet_next, skip, err);
-  } else if (climb1) {
+           v, eps, treelet_count, treelets, treelet_next, skip, err, compact, st);
+  } else if (climb == 2) {
+    launch("k_refit_treelet", (k_refit_treelet<LK, K, Th, false, 2>), dim3(grid), dim3(Th), 0, s,
+           v, eps, treelet_count, treelets, treelet_next, skip, err, compact, st);
+  } else if (climb == 1) {
     launch("k_refit_treelet", (k_refit_treelet<LK, K, Th, false, 1>), dim3(grid), dim3(Th), 0, s,
-           v, eps, treelet_count, treelets, treelet_next, skip, err);
+           v, eps, treelet_count, treelets, treelet_next, skip, err, compact, st);
   } else {
     launch("k_refit_treelet", (k_refit_treelet<LK, K, Th, false, 0>), dim3(grid), dim3(Th), 0, s,
-           v, eps, treelet_count, treelets, treelet_next, skip, err);
+           v, eps, treelet_count, treelets, treelet_next, skip, err, compact, st);
   }
   int rc = launch_status();
   if (rc != DMSH_OK || num_leaf <= kTreeletLeaves) return rc;
   if (top_count) {
     launch("k_refit_top", k_refit_top_lds<kTopSlots, kTopThreads>, dim3(1), dim3(kTopThreads), 0,
            s, v.aabb, flags, num_leaf, v.bvhnodes, top_count, top_list, node2slot, treelet_count,
-           treelets, skip, err, refit_option(kRefitTop));
+           treelets, skip, err, refit_option(kRefitTop), st, compact);
   } else {
     launch("k_refit_top", k_refit_top<kTopThreads>, dim3(1), dim3(kTopThreads), 0, s, v.aabb,
            flags, num_leaf, v.bvhnodes, treelet_count, treelets, err);
@@ -929,6 +1110,8 @@ struct StandaloneLayout {
   uint4* treelets;     // num_leaf
   uint4* top_list;     // kTopSlots: (node, left child, right child, top-child flags)
   unsigned long long* hash;  // -, stamp of the kept plan, tag, -, row-hash partials
+  uint32_t* top_kid;   // kTopClimbSlots (TopStage)
+  float* top_box;      // 2 kTopClimbSlots boxes (TopStage)
 };
 
 inline bool carve_standalone(Arena& a, uint32_t n, StandaloneLayout& L) {
@@ -941,6 +1124,8 @@ inline bool carve_standalone(Arena& a, uint32_t n, StandaloneLayout& L) {
   L.treelets = a.take<uint4>(n ? n : 1);
   L.top_list = a.take<uint4>(kTopSlots);
   L.hash = a.take<unsigned long long>(4 + kPlanHashBlocks);
+  L.top_kid = a.take<uint32_t>(kTopClimbSlots);
+  L.top_box = a.take<float>((size_t)kTopClimbSlots * 12);
   return a.ok();
 }
 
@@ -1202,7 +1387,7 @@ inline int launch_refit_standalone(hipStream_t s, const TreeView& v, float eps, 
   if ((rc = launch_status()) != DMSH_OK) return rc;
   rc = launch_refit_treelets<LK>(s, v, L.flags, L.counters + 0, L.treelets, eps, err,
                                  L.counters + 1, L.top_list, L.node2slot, L.counters + 2,
-                                 L.counters + 3, leaves_in_aabb);
+                                 L.counters + 3, leaves_in_aabb, L.top_kid, L.top_box);
   if (rc != DMSH_OK) return rc;
   launch("k_refit", k_refit<LK>, dim3(std::min<uint32_t>(ceil_div(n, 256), 1024)), dim3(256), 0,
          s, v, L.flags, eps, (const uint32_t*)(L.counters + 3), err);

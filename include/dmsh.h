@@ -52,8 +52,11 @@ int dmsh_get_device_errors(uint32_t *bits_out);
 
 /* Process-wide launch tuning (no reference counterpart; set by the
  * autotuner).  Keys: "pix2tri.stack" (LDS frames per lane: 12, 16, 20, 24,
- * 32; default 20), "pix2tri.waves" (waves per workgroup: 1, 2, 4).  get returns -1 for an
- * unknown key. */
+ * 32; default 20), "pix2tri.waves" (waves per workgroup: 1, 2, 4), and the
+ * refit's "refit.leaves_in_aabb" (0, 1), "refit.climb" (0, 1, 2; default 2),
+ * "refit.queue" (0, 1), "refit.top" (0, 1), "refit.plan_cache" (0, 1),
+ * "refit.compact" (levels climbed before the remaining climbs move to the
+ * first lanes, 0..1023; default 0: never).  get returns -1 for an unknown key. */
 int dmsh_set_tuning(const char *key, int value);
 int dmsh_get_tuning(const char *key);
 
@@ -169,7 +172,7 @@ int dmsh_build_trimesh3(void *stream, uint32_t num_tri, const uint32_t *tri2vtx,
  * the workspace is a fixed DMSH_NOALLOC_SCRATCH_BYTES (64 MiB) device array
  * inside the library, one per GPU, held until the process ends.  Meshes
  * whose dmsh_build_workspace_bytes(num_tri, morton_bits) exceeds it (above
- * 772,032 triangles with 63-bit codes, 869,840 with 30-bit codes) are
+ * 769,642 triangles with 63-bit codes, 867,093 with 30-bit codes) are
  * rejected with DMSH_ERR_TOO_LARGE before anything is enqueued.  It calls no
  * hipMalloc / hipFree.  Every call on a GPU shares that GPU's scratch: calls
  * that overlap on one GPU (on different streams, or from different host

@@ -143,9 +143,19 @@ def test_tuning_keys_accept_and_reject_without_a_gpu():
                 assert L.dmsh_batch_get_tuning(key) == good[-1]
         finally:
             assert L.dmsh_batch_set_tuning(key, old) == 0
-    for key in (b"refit.plan_cache", b"refit.top", b"refit.climb"):
+    for key in (b"refit.plan_cache", b"refit.top", b"refit.queue", b"refit.leaves_in_aabb"):
         old = L.dmsh_get_tuning(key)
         assert old in (0, 1)
         assert L.dmsh_set_tuning(key, 1 - old) == 0 and L.dmsh_get_tuning(key) == 1 - old
+        assert L.dmsh_set_tuning(key, 2) != 0 and L.dmsh_get_tuning(key) == 1 - old
         assert L.dmsh_set_tuning(key, old) == 0
+    for key, top in ((b"refit.climb", 2), (b"refit.compact", 1023)):
+        old = L.dmsh_get_tuning(key)
+        try:
+            for v in (0, top):
+                assert L.dmsh_set_tuning(key, v) == 0 and L.dmsh_get_tuning(key) == v
+            assert L.dmsh_set_tuning(key, top + 1) != 0 and L.dmsh_get_tuning(key) == top
+            assert L.dmsh_set_tuning(key, -1) != 0
+        finally:
+            assert L.dmsh_set_tuning(key, old) == 0
     assert L.dmsh_set_tuning(b"refit.no_such_key", 1) != 0
