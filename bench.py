@@ -69,7 +69,7 @@ def parse():
 # kernels whose launch time the bench reads back from the library's event timing
 TIMED_KERNELS = ["k_pix2tri", "k_pix2tri_tier2", "k_pix2tri_tier3", "k_pack_inner", "k_pack_leaf",
                  "k_build_init", "k_centroid_aabb", "k_morton_hist", "k_hist_all", "k_onesweep",
-                 "k_sort_finish", "k_onesweep_gated", "k_copy_pairs", "k_topology",
+                 "k_sort_finish", "k_sort_groups", "k_topology",
                  "k_refit_treelet", "k_refit_top", "k_fill_u32"]
 COUNTER_SUMMARY = ROOT / "profiles" / "pmc_counters_octa9_2048.txt"
 CALIB_BYTES = 512 << 20

@@ -299,6 +299,7 @@ inline bool carve(Arena& a, uint32_t n, bool tri, BuildLayout& L) {
   L.idx = a.take<uint32_t>(n);
   L.sort.alt_keys = a.take<KeyT>(n);
   L.sort.alt_vals = a.take<uint32_t>(n);
+  L.sort.groups = a.take<uint32_t>(sortk::group_capacity(n));
   return a.ok();
 }
 

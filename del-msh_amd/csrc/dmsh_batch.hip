@@ -72,6 +72,7 @@ inline bool carve_batch_build(Arena& a, uint32_t M, uint32_t n, BatchBuildLayout
   L.idx = a.take<uint32_t>(nn);
   L.sort.alt_keys = a.take<uint64_t>(nn);
   L.sort.alt_vals = a.take<uint32_t>(nn);
+  L.sort.groups = nullptr;  // plain passes only
   return a.ok();
 }
 

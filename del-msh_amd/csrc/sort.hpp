@@ -21,9 +21,9 @@
 // group in LDS by the whole key, ties by position (a group is a handful of
 // elements for Morton codes of spread-out geometry).  The result is the same
 // stable order as the full LSD sort.  A group too long for the finish's LDS
-// window (heavily repeated top bits) sets a gate word instead, and the full
-// set of digit passes, launched behind it and idle otherwise, sorts the
-// top-sorted sequence again from the lowest digit.
+// window (heavily repeated top bits) is listed instead, and k_sort_groups,
+// one launch that does nothing when the list is empty, sorts each listed
+// group by its low digits in one workgroup.
 #pragma once
 
 #include "dmsh_common.hpp"
@@ -42,28 +42,30 @@ constexpr uint32_t kSpinLimit = 1u << 22;
 constexpr int kLookBack = 8;
 constexpr uint32_t kTopPasses = 3;      // global passes before the finish
 constexpr uint32_t kFinishHalo = 1024;  // LDS window margin of the finish
-// One ticket per launched pass (kTopPasses + kMaxPasses at most); the last
-// word is the finish's gate.
+// One ticket per launched pass (kMaxPasses at most); the last word counts
+// the groups the finish leaves to k_sort_groups.
 constexpr uint32_t kTicketWords = 2 * kMaxPasses;
 
+// Groups longer than kFinishHalo: at most this many in n elements.
+inline uint32_t group_capacity(uint32_t n) { return n / (kFinishHalo + 1) + 1; }
+
 // Keys of five or more digits take the top passes and the finish (with four,
-// three passes, the finish and the idle gated passes cost about as much as
-// four plain passes).
+// three passes and the finish cost about as much as four plain passes).
 __host__ __device__ inline bool uses_finish(uint32_t num_passes) {
   return num_passes > kTopPasses + 1;
 }
 
 // The first digit whose histogram the key producers of a sort that may
-// finish in LDS accumulate: with the finish, only the top digits' (the gated
-// full passes build the others' histograms themselves, k_hist_low).
+// finish in LDS accumulate: with the finish, only the top digits'
+// (k_sort_groups counts the low digits of its groups itself).
 __host__ __device__ inline uint32_t first_hist_pass(uint32_t num_passes) {
   return uses_finish(num_passes) ? num_passes - kTopPasses : 0u;
 }
 
-// Passes that take a status region: every digit once, plus the top passes
-// when the finish is used.
+// Passes that take a status region: the top passes when the finish is
+// used, else every digit.
 inline uint32_t status_passes(uint32_t num_passes) {
-  return uses_finish(num_passes) ? kTopPasses + num_passes : num_passes;
+  return uses_finish(num_passes) ? kTopPasses : num_passes;
 }
 
 template <typename KeyT>
@@ -153,31 +155,8 @@ __global__ __launch_bounds__(256) void k_hist_all(const KeyT* __restrict__ keys,
   hist_flush(s_h, first, num_passes, hist);
 }
 
-// Histograms of digits [0, num_low) when *gate is set (the gated full passes
-// of the finish path; the keys may be in any order).
-template <typename KeyT>
-__global__ __launch_bounds__(256) void k_hist_low(const uint32_t* __restrict__ gate,
-                                                  const KeyT* __restrict__ keys, uint32_t n,
-                                                  uint32_t num_low, uint32_t key_bits,
-                                                  uint32_t* __restrict__ hist) {
-  if (*gate == 0u) return;
-  __shared__ uint32_t s_h[kMaxPasses][kRadix];
-  for (uint32_t e = threadIdx.x; e < kMaxPasses * kRadix; e += blockDim.x) (&s_h[0][0])[e] = 0;
-  __syncthreads();
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    const KeyT k = keys[i];
-    for (uint32_t p = 0; p < num_low; ++p)
-      atomicAdd(&s_h[p][digit_of(k, p * kRadixBits, (1u << pass_bits(key_bits, p)) - 1u)], 1u);
-  }
-  __syncthreads();
-  for (uint32_t e = threadIdx.x; e < num_low * kRadix; e += blockDim.x) {
-    const uint32_t v = (&s_h[0][0])[e];
-    if (v) atomicAdd(hist + e, v);
-  }
-}
-
 // One digit pass.  IOTA: values are the element indices (first pass).  With
-// `gate`, the pass runs only if *gate is nonzero.  With `n_dev`, the element
+// `n_dev`, the element
 // count is read from the device (the grid is sized for a capacity); blocks
 // whose ticket lies past the last tile exit before publishing anything, and
 // only earlier tiles are ever looked back at, so they wait on nobody.
@@ -186,9 +165,7 @@ __global__ __launch_bounds__(256) void k_onesweep(
     const KeyT* __restrict__ kin, const uint32_t* __restrict__ vin, KeyT* __restrict__ kout,
     uint32_t* __restrict__ vout, uint32_t n, const uint32_t* __restrict__ n_dev, uint32_t shift,
     uint32_t dbits, const uint32_t* __restrict__ hist, uint32_t* __restrict__ status,
-    uint32_t* __restrict__ ticket, const uint32_t* __restrict__ gate,
-    uint32_t* __restrict__ err) {
-  if (gate && *gate == 0u) return;
+    uint32_t* __restrict__ ticket, uint32_t* __restrict__ err) {
   if (n_dev) n = *n_dev;
   constexpr int kTile = kThreads * ITEMS;
   __shared__ uint32_t s_wcnt[4][kRadix];
@@ -323,8 +300,9 @@ __global__ __launch_bounds__(256) void k_onesweep(
 // each element finds the ends of its group from that set, at most
 // kFinishHalo - 1 slots down and kFinishHalo up, and lands at group start +
 // the number of group members whose masked key is smaller, or equal and
-// earlier.  An element whose group end lies beyond that span sets *gate (the
-// full passes then redo the sort) and writes nothing.
+// earlier.  An element whose group end lies beyond that span writes nothing;
+// the first element of such a group lists it (groups[*group_count++]) for
+// k_sort_groups.
 constexpr int kFinishThreads = 1024;
 constexpr int kFinishItems = 4;
 
@@ -334,7 +312,9 @@ __global__ __launch_bounds__(kFinishThreads) void k_sort_finish(const KeyT* __re
                                                      KeyT* __restrict__ kout,
                                                      uint32_t* __restrict__ vout, uint32_t n,
                                                      uint32_t shift, KeyT kmask,
-                                                     uint32_t* __restrict__ gate) {
+                                                     uint32_t* __restrict__ group_count,
+                                                     uint32_t* __restrict__ groups,
+                                                     uint32_t group_cap) {
   constexpr uint32_t kTW = (uint32_t)kFinishThreads * kFinishItems;
   constexpr uint32_t kG = kFinishHalo;
   constexpr uint32_t kWin = kTW + 2 * kG;
@@ -382,7 +362,6 @@ __global__ __launch_bounds__(kFinishThreads) void k_sort_finish(const KeyT* __re
     }
   }
   __syncthreads();
-  bool big = false;
 #pragma unroll
   for (int r = 0; r < kFinishItems; ++r) {
     const uint32_t li = kG + (uint32_t)r * kFinishThreads + threadIdx.x;
@@ -404,7 +383,12 @@ __global__ __launch_bounds__(kFinishThreads) void k_sort_finish(const KeyT* __re
       const bool found_b = bits != 0u;
       const uint32_t b = found_b ? w * 32u + (uint32_t)__ffs(bits) - 1u : 0u;
       if (!found_a || !found_b || a + (kG - 1) < li || b > li + kG) {
-        big = true;  // a group end lies outside the searched span
+        // a group end lies outside the searched span: the group is longer
+        // than kG, so its first element sees it too and lists it
+        if (found_a && a == li) {
+          const uint32_t slot = atomicAdd(group_count, 1u);
+          if (slot < group_cap) groups[slot] = (uint32_t)(w0 + li);
+        }
       } else {
         uint32_t rank = 0;
 #pragma unroll 4
@@ -418,20 +402,141 @@ __global__ __launch_bounds__(kFinishThreads) void k_sort_finish(const KeyT* __re
       }
     }
   }
-  if (big) atomicOr(gate, 1u);
 }
 
-// Copies (keys, values) a -> b when *gate is nonzero.
+// The groups the finish listed (more than kFinishHalo elements with equal
+// bits at and above `shift`), one workgroup per group.  The group runs from
+// its listed first element to the first later element with other top bits;
+// it is sorted by the digits below `shift`, least significant first, each
+// pass stable in element order (ranks inside a wave by ballots, then waves
+// and chunks in order), alternating between the group's range of the
+// top-sorted buffer (kin, vin) and of the output (kout, vout), with a copy
+// back to the output after an even number of passes.  Only the group's
+// ranges are touched; every other element was placed by the finish.
+constexpr int kGroupThreads = 1024;
+constexpr int kGroupItems = 4;
+constexpr uint32_t kGroupGrid = 256;
+
 template <typename KeyT>
-__global__ __launch_bounds__(256) void k_copy_pairs(const uint32_t* __restrict__ gate,
-                                                    const KeyT* __restrict__ ka,
-                                                    const uint32_t* __restrict__ va,
-                                                    KeyT* __restrict__ kb,
-                                                    uint32_t* __restrict__ vb, uint32_t n) {
-  if (*gate == 0u) return;
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    kb[i] = ka[i];
-    vb[i] = va[i];
+__global__ __launch_bounds__(kGroupThreads) void k_sort_groups(
+    KeyT* __restrict__ kin, uint32_t* __restrict__ vin, KeyT* __restrict__ kout,
+    uint32_t* __restrict__ vout, uint32_t n, uint32_t shift, KeyT kmask,
+    const uint32_t* __restrict__ group_count, const uint32_t* __restrict__ groups,
+    uint32_t group_cap) {
+  constexpr uint32_t W = kGroupThreads / 64;
+  constexpr uint32_t kChunk = (uint32_t)kGroupThreads * kGroupItems;
+  __shared__ uint32_t s_hist[kMaxPasses][kRadix];
+  __shared__ uint32_t s_wcnt[W][kRadix];
+  __shared__ uint32_t s_base[kRadix];
+  __shared__ uint32_t s_tot[kRadix];
+  __shared__ uint32_t s_wsum[4];
+  __shared__ uint32_t s_end;
+  uint32_t count = *group_count;
+  if (count > group_cap) count = group_cap;
+  const uint32_t tid = threadIdx.x, wave = tid >> 6, lane = tid & 63u;
+  const uint32_t low = shift / kRadixBits;  // digits below the group's common bits
+  for (uint32_t g = blockIdx.x; g < count; g += gridDim.x) {
+    const uint32_t a = groups[g];
+    if (a >= n) continue;  // uniform over the workgroup
+    const KeyT top = (kin[a] & kmask) >> shift;
+    if (tid == 0) s_end = n;
+    for (uint32_t e = tid; e < low * (uint32_t)kRadix; e += kGroupThreads) (&s_hist[0][0])[e] = 0u;
+    __syncthreads();
+    for (uint32_t base = a + kFinishHalo + 1; base < n; base += kGroupThreads) {
+      const uint32_t i = base + tid;
+      const bool diff = i < n && ((kin[i] & kmask) >> shift) != top;
+      if (diff) atomicMin(&s_end, i);
+      if (__syncthreads_or(diff)) break;
+    }
+    const uint32_t end = s_end;
+    for (uint32_t i = a + tid; i < end; i += kGroupThreads) {
+      const KeyT k = kin[i];
+      for (uint32_t p = 0; p < low; ++p)
+        atomicAdd(&s_hist[p][digit_of(k, p * kRadixBits, (uint32_t)kRadix - 1u)], 1u);
+    }
+    for (uint32_t p = 0; p < low; ++p) {
+      KeyT* sk = (p & 1u) ? kout : kin;
+      uint32_t* sv = (p & 1u) ? vout : vin;
+      KeyT* dk = (p & 1u) ? kin : kout;
+      uint32_t* dv = (p & 1u) ? vin : vout;
+      __syncthreads();  // histograms complete; the previous pass's stores visible
+      uint32_t v = 0, inc = 0;
+      if (tid < (uint32_t)kRadix) {
+        v = s_hist[p][tid];
+        inc = wave_inclusive_scan(v);
+        if (lane == 63u) s_wsum[wave] = inc;
+      }
+      for (uint32_t e = tid; e < W * kRadix; e += kGroupThreads) (&s_wcnt[0][0])[e] = 0u;
+      __syncthreads();
+      if (tid < (uint32_t)kRadix) {
+        uint32_t b = 0;
+        for (uint32_t w = 0; w < wave; ++w) b += s_wsum[w];
+        s_base[tid] = a + b + inc - v;
+      }
+      for (uint32_t cb = a; cb < end; cb += kChunk) {
+        KeyT key[kGroupItems];
+        uint32_t val[kGroupItems], dig[kGroupItems], pos[kGroupItems];
+#pragma unroll
+        for (int i = 0; i < kGroupItems; ++i) {
+          const uint32_t idx = cb + wave * 64u * kGroupItems + (uint32_t)i * 64u + lane;
+          const bool valid = idx < end;
+          key[i] = valid ? sk[idx] : (KeyT)0;
+          val[i] = valid ? sv[idx] : 0u;
+          dig[i] = valid ? digit_of(key[i], p * kRadixBits, (uint32_t)kRadix - 1u) : 0xFFFFFFFFu;
+        }
+        // ranks inside the wave: rounds in element order, lanes in element order
+#pragma unroll
+        for (int i = 0; i < kGroupItems; ++i) {
+          const bool valid = dig[i] != 0xFFFFFFFFu;
+          uint64_t m = __ballot(valid);
+          for (uint32_t b = 0; b < (uint32_t)kRadixBits; ++b) {
+            const bool bit = (dig[i] >> b) & 1u;
+            const uint64_t bb = __ballot(bit);
+            m &= bit ? bb : ~bb;
+          }
+          if (valid) {
+            const uint32_t rank = mask_rank(m);
+            const uint32_t old = s_wcnt[wave][dig[i]];
+            pos[i] = old + rank;
+            if (rank == 0) s_wcnt[wave][dig[i]] = old + (uint32_t)__popcll(m);
+          }
+        }
+        __syncthreads();
+        if (tid < (uint32_t)kRadix) {
+          uint32_t run = 0;
+          for (uint32_t w = 0; w < W; ++w) {
+            const uint32_t c = s_wcnt[w][tid];
+            s_wcnt[w][tid] = run;
+            run += c;
+          }
+          s_tot[tid] = run;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < kGroupItems; ++i) {
+          if (dig[i] == 0xFFFFFFFFu) continue;
+          const uint32_t out = s_base[dig[i]] + s_wcnt[wave][dig[i]] + pos[i];
+          if (out >= a && out < end) {
+            dk[out] = key[i];
+            dv[out] = val[i];
+          }
+        }
+        __syncthreads();
+        if (tid < (uint32_t)kRadix) {
+          s_base[tid] += s_tot[tid];
+          for (uint32_t w = 0; w < W; ++w) s_wcnt[w][tid] = 0u;
+        }
+        __syncthreads();
+      }
+    }
+    if (low % 2u == 0u) {  // the last pass wrote (kin, vin)
+      __syncthreads();
+      for (uint32_t i = a + tid; i < end; i += kGroupThreads) {
+        kout[i] = kin[i];
+        vout[i] = vin[i];
+      }
+    }
+    __syncthreads();  // LDS and s_end are reused by the next group
   }
 }
 
@@ -460,8 +565,8 @@ inline size_t status_words_items(uint32_t n, uint32_t key_bits, int items) {
   return (size_t)status_passes(num_passes(key_bits)) * num_tiles_items(n, items) * kRadix;
 }
 
-// Workspace: hist[8][256] + tickets/gate[16] + status[passes][tiles][256] +
-// alt keys + alt values.
+// Workspace: hist[8][256] + tickets[16] + status[passes][tiles][256] +
+// alt keys + alt values + the finish's group list.
 template <typename KeyT>
 inline size_t workspace_bytes(uint32_t n, uint32_t key_bits) {
   Arena a{nullptr, 0, 0};
@@ -469,16 +574,18 @@ inline size_t workspace_bytes(uint32_t n, uint32_t key_bits) {
   a.take<uint32_t>(status_words<KeyT>(n, key_bits));
   a.take<KeyT>(n);
   a.take<uint32_t>(n);
+  a.take<uint32_t>(group_capacity(n));
   return a.off + 256;
 }
 
 struct Layout {
   uint32_t* hist;     // [kMaxPasses][kRadix], then tickets[kTicketWords]
-  uint32_t* tickets;  // the last word is the finish's gate
+  uint32_t* tickets;  // the last word counts the groups left by the finish
   uint32_t* status;   // [status_passes][tiles][kRadix]
   size_t zero_words;  // hist + tickets + status are contiguous from `hist`
   void* alt_keys;
   uint32_t* alt_vals;
+  uint32_t* groups;   // [group_capacity(n)]: first elements of those groups
 };
 
 template <typename KeyT>
@@ -490,6 +597,7 @@ inline bool carve(Arena& a, uint32_t n, uint32_t key_bits, Layout& L) {
   L.zero_words = (size_t)(L.status - L.hist) + sw;
   L.alt_keys = a.take<KeyT>(n);
   L.alt_vals = a.take<uint32_t>(n);
+  L.groups = a.take<uint32_t>(group_capacity(n));
   return a.ok();
 }
 
@@ -505,19 +613,18 @@ inline int launch_fill(hipStream_t s, uint32_t* p, size_t count, uint32_t value)
 template <typename KeyT, int ITEMS>
 inline int launch_pass_items(hipStream_t s, uint32_t n, uint32_t key_bits, const Layout& L,
                              uint32_t digit, uint32_t slot, bool iota, const KeyT* sk,
-                             const uint32_t* sv, KeyT* dk, uint32_t* dv, const uint32_t* gate,
-                             uint32_t* err, const uint32_t* n_dev) {
+                             const uint32_t* sv, KeyT* dk, uint32_t* dv, uint32_t* err,
+                             const uint32_t* n_dev) {
   const uint32_t tiles = num_tiles_items(n, ITEMS);
   const uint32_t shift = digit * kRadixBits;
   const uint32_t dbits = pass_bits(key_bits, digit);
   uint32_t* st = L.status + (size_t)slot * tiles * kRadix;
-  const char* label = gate ? "k_onesweep_gated" : "k_onesweep";
   if (iota) {
-    launch(label, (k_onesweep<KeyT, ITEMS, true>), dim3(tiles), dim3(kThreads), 0, s, sk, sv, dk,
-           dv, n, n_dev, shift, dbits, L.hist + digit * kRadix, st, L.tickets + slot, gate, err);
+    launch("k_onesweep", (k_onesweep<KeyT, ITEMS, true>), dim3(tiles), dim3(kThreads), 0, s, sk,
+           sv, dk, dv, n, n_dev, shift, dbits, L.hist + digit * kRadix, st, L.tickets + slot, err);
   } else {
-    launch(label, (k_onesweep<KeyT, ITEMS, false>), dim3(tiles), dim3(kThreads), 0, s, sk, sv, dk,
-           dv, n, n_dev, shift, dbits, L.hist + digit * kRadix, st, L.tickets + slot, gate, err);
+    launch("k_onesweep", (k_onesweep<KeyT, ITEMS, false>), dim3(tiles), dim3(kThreads), 0, s, sk,
+           sv, dk, dv, n, n_dev, shift, dbits, L.hist + digit * kRadix, st, L.tickets + slot, err);
   }
   return launch_status();
 }
@@ -526,18 +633,18 @@ inline int launch_pass_items(hipStream_t s, uint32_t n, uint32_t key_bits, const
 template <typename KeyT>
 inline int launch_pass(hipStream_t s, uint32_t n, uint32_t key_bits, const Layout& L,
                        uint32_t digit, uint32_t slot, bool iota, const KeyT* sk,
-                       const uint32_t* sv, KeyT* dk, uint32_t* dv, const uint32_t* gate,
-                       uint32_t* err, const uint32_t* n_dev = nullptr, int items = 0) {
+                       const uint32_t* sv, KeyT* dk, uint32_t* dv, uint32_t* err,
+                       const uint32_t* n_dev = nullptr, int items = 0) {
   switch (items ? items : Tune<KeyT>::kItems) {
     case 8:
       return launch_pass_items<KeyT, 8>(s, n, key_bits, L, digit, slot, iota, sk, sv, dk, dv,
-                                        gate, err, n_dev);
+                                        err, n_dev);
     case 12:
       return launch_pass_items<KeyT, 12>(s, n, key_bits, L, digit, slot, iota, sk, sv, dk, dv,
-                                         gate, err, n_dev);
+                                         err, n_dev);
     case 16:
       return launch_pass_items<KeyT, 16>(s, n, key_bits, L, digit, slot, iota, sk, sv, dk, dv,
-                                         gate, err, n_dev);
+                                         err, n_dev);
     default:
       return DMSH_ERR_INVALID;
   }
@@ -551,9 +658,8 @@ inline int launch_pass(hipStream_t s, uint32_t n, uint32_t key_bits, const Layou
 //   so that the last lands in dst; `first_src` may alias dst_keys only when
 //   the number of passes is even.
 //   With the finish: top passes first_src -> alt -> dst -> alt, the finish
-//   alt -> dst, then the gated full passes alt -> dst -> ... (and a gated copy
-//   back to dst when their number is even).  `first_src` must not be the
-//   alternate key buffer.
+//   alt -> dst, and k_sort_groups for the groups the finish leaves (idle when
+//   there are none).  `first_src` must not be the alternate key buffer.
 //   With `n_dev` (plain path only) the element count is read from the device
 //   and n is the capacity the grids are sized for; with `first_vals` the
 //   first pass carries those values instead of the element indices.
@@ -574,7 +680,7 @@ inline int run_passes(hipStream_t s, uint32_t n, uint32_t key_bits, const Layout
       KeyT* dk = to_user ? dst_keys : alt_k;
       uint32_t* dv = to_user ? dst_vals : alt_v;
       rc = launch_pass<KeyT>(s, n, key_bits, L, p, p, p == 0 && first_vals == nullptr, src_k,
-                             src_v, dk, dv, nullptr, err, n_dev, items);
+                             src_v, dk, dv, err, n_dev, items);
       if (rc != DMSH_OK) return rc;
       src_k = dk;
       src_v = dv;
@@ -584,38 +690,25 @@ inline int run_passes(hipStream_t s, uint32_t n, uint32_t key_bits, const Layout
   if (first_src == alt_k || n_dev || first_vals || items) return DMSH_ERR_INVALID;
   const uint32_t d0 = P - kTopPasses;
   if ((rc = launch_pass<KeyT>(s, n, key_bits, L, d0, 0, true, first_src, nullptr, alt_k, alt_v,
-                              nullptr, err)) != DMSH_OK)
+                              err)) != DMSH_OK)
     return rc;
   if ((rc = launch_pass<KeyT>(s, n, key_bits, L, d0 + 1, 1, false, alt_k, alt_v, dst_keys,
-                              dst_vals, nullptr, err)) != DMSH_OK)
+                              dst_vals, err)) != DMSH_OK)
     return rc;
   if ((rc = launch_pass<KeyT>(s, n, key_bits, L, d0 + 2, 2, false, dst_keys, dst_vals, alt_k,
-                              alt_v, nullptr, err)) != DMSH_OK)
+                              alt_v, err)) != DMSH_OK)
     return rc;
-  uint32_t* gate = L.tickets + kTicketWords - 1;
+  uint32_t* group_count = L.tickets + kTicketWords - 1;
+  const uint32_t cap = group_capacity(n);
   const KeyT kmask = key_bits >= 8 * sizeof(KeyT) ? ~(KeyT)0 : (((KeyT)1 << key_bits) - 1);
   launch("k_sort_finish", k_sort_finish<KeyT>,
          dim3(ceil_div(n, (uint64_t)kFinishThreads * kFinishItems)), dim3(kFinishThreads), 0, s,
-         alt_k, alt_v,
-         dst_keys, dst_vals, n, d0 * kRadixBits, kmask, gate);
+         alt_k, alt_v, dst_keys, dst_vals, n, d0 * kRadixBits, kmask, group_count, L.groups, cap);
   if ((rc = launch_status()) != DMSH_OK) return rc;
-  launch("k_hist_low", k_hist_low<KeyT>, dim3(std::min<uint32_t>(num_tiles<KeyT>(n), 2048)),
-         dim3(256), 0, s, (const uint32_t*)gate, (const KeyT*)alt_k, n, first_hist_pass(P),
-         key_bits, L.hist);
-  if ((rc = launch_status()) != DMSH_OK) return rc;
-  for (uint32_t p = 0; p < P; ++p) {
-    const bool from_alt = p % 2 == 0;
-    rc = launch_pass<KeyT>(s, n, key_bits, L, p, kTopPasses + p, false,
-                           from_alt ? alt_k : dst_keys, from_alt ? alt_v : dst_vals,
-                           from_alt ? dst_keys : alt_k, from_alt ? dst_vals : alt_v, gate, err);
-    if (rc != DMSH_OK) return rc;
-  }
-  if (P % 2 == 0) {
-    launch("k_copy_pairs", k_copy_pairs<KeyT>, dim3(std::min<uint32_t>(ceil_div(n, 256), 2048)),
-           dim3(256), 0, s, (const uint32_t*)gate, (const KeyT*)alt_k, (const uint32_t*)alt_v,
-           dst_keys, dst_vals, n);
-    rc = launch_status();
-  }
+  launch("k_sort_groups", k_sort_groups<KeyT>, dim3(std::min<uint32_t>(cap, kGroupGrid)),
+         dim3(kGroupThreads), 0, s, alt_k, alt_v, dst_keys, dst_vals, n, d0 * kRadixBits, kmask,
+         (const uint32_t*)group_count, (const uint32_t*)L.groups, cap);
+  rc = launch_status();
   return rc;
 }
 

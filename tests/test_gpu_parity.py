@@ -92,12 +92,13 @@ def test_argsort_u32(gpu, oracle, n, spread):
 
 
 @pytest.mark.parametrize("key_bits", [40, 48, 56, 63, 64])
-@pytest.mark.parametrize("groups", [None, 4000, 300])
+@pytest.mark.parametrize("groups", [None, 4000, 300, 1])
 def test_argsort_u64_grouped(gpu, oracle, key_bits, groups):
     """Keys of five or more digits: top-digit passes, then the in-LDS finish
     per group of equal top bits.  groups=None: random keys (tiny groups);
     4000: ~250 keys per group with repeated low parts; 300: ~3300 per group,
-    beyond the finish's window, so the gated full passes run."""
+    beyond the finish's window, so k_sort_groups sorts each of them; 1: one
+    group of every key (one workgroup sorts all 10^6)."""
     n = 1_000_003
     rng = np.random.default_rng(key_bits * 7 + (groups or 1))
     mask = np.uint64((1 << key_bits) - 1) if key_bits < 64 else np.uint64(0xFFFFFFFFFFFFFFFF)
@@ -181,8 +182,21 @@ def test_topology_sizes_near_the_serial_cut(gpu, oracle, n):
         assert np.array_equal(got, oracle.bvhnodes_from_sorted(idx, codes)), hi
 
 
+def _clustered():
+    """A 32k-triangle sphere shrunk to 1e-6 beside one far triangle: every
+    clustered centroid shares its top 24 code bits, so the sort's finish
+    leaves one 32k-element group to k_sort_groups."""
+    tri, vtx = primitives.octahedral_sphere(6, noise_sigma=0.01, seed=2)
+    vtx = (vtx.astype(np.float64) * 1e-6 + 0.3).astype(np.float32)
+    far = np.array([[-1.0, -1.0, -1.0], [-1.0, -0.9, -1.0], [-0.9, -1.0, -1.0]], np.float32)
+    nv = vtx.shape[0]
+    tri = np.concatenate([tri, np.array([[nv, nv + 1, nv + 2]], tri.dtype)])
+    return tri, np.concatenate([vtx, far])
+
+
 MESHES = {
     "sphere_c1": lambda: primitives.sphere_yup(1.0, 64, 128),
+    "clustered": _clustered,
     "torus": lambda: primitives.torus_zup(1.3, 0.4, 64, 32),
     "octa6_noise": lambda: primitives.octahedral_sphere(6, noise_sigma=0.01, seed=1),
     "cylinder": lambda: primitives.cylinder_closed_end_yup(0.5, 2.0, 40, 30),
