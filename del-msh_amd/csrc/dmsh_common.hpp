@@ -209,7 +209,8 @@ enum RefitOption {
   kRefitTop = 3,
   kRefitPlanCache = 4,
   kRefitCompact = 5,
-  kRefitOptionCount = 6
+  kSortWideTiles = 6,
+  kRefitOptionCount = 7
 };
 int refit_option(RefitOption which);
 bool set_refit_option(const char* key, int value);

@@ -20,17 +20,22 @@ static std::mutex g_err_mutex;
 // refit options: "refit.leaves_in_aabb" (0: the treelet kernel builds the
 // leaf boxes; 1: the kernel before writes them), "refit.climb" (0: each
 // level waits for its links, then its arrival counter, then the sibling's
-// box; 1: one LDS round trip per level), "refit.queue" (1: workgroups take
-// treelets from a shared counter; 0: treelet t goes to workgroup t mod grid),
-// "refit.top" (0: the nodes above the treelets climb through LDS counters
-// when there are at most 4096 of them; 1: they merge in barrier rounds),
-// "refit.plan_cache" (1: a standalone refit whose workspace already holds
-// the plan of the same rows skips the plan kernels; 0: always rebuild it)
-static std::atomic<int> g_refit_option[kRefitOptionCount] = {{0}, {2}, {1}, {0}, {1}, {0}};
+// box; 1: one LDS round trip per level; 2: as 1 for trees of known-good
+// layout, climbs starting at the nodes whose children are both leaves),
+// "refit.queue" (1: workgroups take treelets from a shared counter; 0:
+// treelet t goes to workgroup t mod grid), "refit.top" (0: the nodes above
+// the treelets climb through LDS counters when there are at most 4096 of
+// them; 1: they merge in barrier rounds), "refit.plan_cache" (1: a
+// standalone refit whose workspace already holds the plan of the same rows
+// skips the plan kernels; 0: always rebuild it), "refit.compact" (levels a
+// climb runs before the climbs still going are queued again, 0..1023; 0:
+// never), and the sort's "sort.wide_tiles" (digit passes in 512-thread
+// tiles: 0 never, 1 from 2^21 elements, 2 always)
+static std::atomic<int> g_refit_option[kRefitOptionCount] = {{0}, {2}, {1}, {0}, {1}, {0}, {1}};
 static const char* const kRefitOptionKey[kRefitOptionCount] = {
     "refit.leaves_in_aabb", "refit.climb", "refit.queue", "refit.top", "refit.plan_cache",
-    "refit.compact"};
-static const int kRefitOptionMax[kRefitOptionCount] = {1, 2, 1, 1, 1, 1023};
+    "refit.compact", "sort.wide_tiles"};
+static const int kRefitOptionMax[kRefitOptionCount] = {1, 2, 1, 1, 1, 1023, 2};
 
 int refit_option(RefitOption which) { return g_refit_option[which].load(); }
 

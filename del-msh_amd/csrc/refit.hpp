@@ -865,7 +865,11 @@ __device__ __forceinline__ void top_climb(float* __restrict__ aabb, uint32_t num
     for (uint32_t i = threadIdx.x; i < nq; i += T) {
       // a climb from slot c (finished) to its parent p; the parent's links
       // are read one level ahead, so the counter, both children's boxes and
-      // the next links go out together: one LDS round trip per level
+      // the next links go out together: one LDS round trip per level.  The
+      // counter RMW is ordered at wavefront scope (no wait after it): a
+      // wave's LDS operations are performed in issue order, so the box loads
+      // issued after it see every box stored before the other child's RMW,
+      // which this one observed (as in refit_treelet's climb)
       uint32_t c = q[i];
       uint32_t p = s_par[c];
       if (p == kNone) continue;
@@ -876,7 +880,7 @@ __device__ __forceinline__ void top_climb(float* __restrict__ aabb, uint32_t num
         const uint32_t need = (la != kNone ? 1u : 0u) + (lb != kNone ? 1u : 0u);
         const uint32_t sh = 8u * (p & 3u);
         const uint32_t old = (__hip_atomic_fetch_add(s_cnt + (p >> 2), 1u << sh, __ATOMIC_ACQ_REL,
-                                                     __HIP_MEMORY_SCOPE_WORKGROUP) >>
+                                                     __HIP_MEMORY_SCOPE_WAVEFRONT) >>
                               sh) &
                              0xFFu;
         const Box A = la != kNone ? s_box[la] : s_box[p];

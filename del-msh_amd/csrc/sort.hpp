@@ -32,6 +32,8 @@ namespace dmsh {
 namespace sortk {
 
 constexpr int kThreads = 256;  // 4 waves
+constexpr int kWideThreads = 512;             // wide tiles (see k_onesweep)
+constexpr uint32_t kWideTilesFrom = 1u << 21;  // elements from which the default takes them
 constexpr int kRadixBits = 8;
 constexpr int kRadix = 1 << kRadixBits;
 constexpr int kMaxPasses = 8;
@@ -83,7 +85,8 @@ __device__ __forceinline__ uint32_t wave_inclusive_scan(uint32_t v) {
   return v;
 }
 
-// Exclusive scan of one value per thread over a 256-thread block.
+// Exclusive scan of one value per thread over the block (s_wsum: one word
+// per wave).
 __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* s_wsum) {
   const uint32_t inc = wave_inclusive_scan(v);
   const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
@@ -160,18 +163,22 @@ __global__ __launch_bounds__(256) void k_hist_all(const KeyT* __restrict__ keys,
 // count is read from the device (the grid is sized for a capacity); blocks
 // whose ticket lies past the last tile exit before publishing anything, and
 // only earlier tiles are ever looked back at, so they wait on nobody.
-template <typename KeyT, int ITEMS, bool IOTA>
-__global__ __launch_bounds__(256) void k_onesweep(
+// THREADS: 256 (4 waves) or 512 (8 waves, for large inputs: half the tiles,
+// so half the same-address tickets and a look-back chain half as long).
+template <typename KeyT, int ITEMS, bool IOTA, int THREADS = kThreads>
+__global__ __launch_bounds__(THREADS) void k_onesweep(
     const KeyT* __restrict__ kin, const uint32_t* __restrict__ vin, KeyT* __restrict__ kout,
     uint32_t* __restrict__ vout, uint32_t n, const uint32_t* __restrict__ n_dev, uint32_t shift,
     uint32_t dbits, const uint32_t* __restrict__ hist, uint32_t* __restrict__ status,
     uint32_t* __restrict__ ticket, uint32_t* __restrict__ err) {
+  static_assert(THREADS >= kRadix && THREADS % 64 == 0, "one thread per digit");
   if (n_dev) n = *n_dev;
-  constexpr int kTile = kThreads * ITEMS;
-  __shared__ uint32_t s_wcnt[4][kRadix];
+  constexpr int kTile = THREADS * ITEMS;
+  constexpr uint32_t W = THREADS / 64;
+  __shared__ uint32_t s_wcnt[W][kRadix];
   __shared__ uint32_t s_excl[kRadix];
   __shared__ uint32_t s_gbase[kRadix];
-  __shared__ uint32_t s_wsum[4];
+  __shared__ uint32_t s_wsum[W];
   __shared__ uint32_t s_tile;
   __shared__ KeyT s_key[kTile];
   __shared__ uint32_t s_val[kTile];
@@ -179,7 +186,7 @@ __global__ __launch_bounds__(256) void k_onesweep(
   const uint32_t tid = threadIdx.x, wave = tid >> 6, lane = tid & 63u;
   const uint32_t mask = (1u << dbits) - 1u;
   if (tid == 0) s_tile = atomicAdd(ticket, 1u);
-  for (uint32_t e = tid; e < 4u * kRadix; e += kThreads) (&s_wcnt[0][0])[e] = 0;
+  for (uint32_t e = tid; e < W * kRadix; e += THREADS) (&s_wcnt[0][0])[e] = 0;
   __syncthreads();
   const uint32_t tile = s_tile;
   const uint32_t tile_base = tile * (uint32_t)kTile;
@@ -214,25 +221,30 @@ __global__ __launch_bounds__(256) void k_onesweep(
     }
   }
   __syncthreads();
-  // Per digit: wave prefixes and the tile count.
+  // Per digit (thread d < kRadix): wave prefixes and the tile count.
   const uint32_t d = tid;
-  const uint32_t c0 = s_wcnt[0][d], c1 = s_wcnt[1][d], c2 = s_wcnt[2][d], c3 = s_wcnt[3][d];
-  s_wcnt[0][d] = 0;
-  s_wcnt[1][d] = c0;
-  s_wcnt[2][d] = c0 + c1;
-  s_wcnt[3][d] = c0 + c1 + c2;
-  const uint32_t cnt = c0 + c1 + c2 + c3;
+  const bool has_d = d < (uint32_t)kRadix;
+  uint32_t cnt = 0;
+  if (has_d) {
+#pragma unroll
+    for (uint32_t w = 0; w < W; ++w) {
+      const uint32_t c = s_wcnt[w][d];
+      s_wcnt[w][d] = cnt;
+      cnt += c;
+    }
+  }
   uint32_t* my_status = status + (size_t)tile * kRadix + d;
-  __hip_atomic_store(my_status, (tile == 0 ? kFlagPrefix : kFlagAgg) | cnt, __ATOMIC_RELAXED,
-                     __HIP_MEMORY_SCOPE_AGENT);
+  if (has_d)
+    __hip_atomic_store(my_status, (tile == 0 ? kFlagPrefix : kFlagAgg) | cnt, __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
   const uint32_t excl = block_exclusive_scan(cnt, s_wsum);
-  const uint32_t goff = block_exclusive_scan(hist[d], s_wsum);
+  const uint32_t goff = block_exclusive_scan(has_d ? hist[d] : 0u, s_wsum);
   // Decoupled look-back over the preceding tiles for digit d, kLookBack
   // status words per round trip: tiles that start together all publish their
   // counts at about the same time and then walk back over each other, so a
   // walk of k tiles costs k / kLookBack memory latencies instead of k.
   uint32_t prefix = 0;
-  if (tile > 0) {
+  if (has_d && tile > 0) {
     uint32_t tt = tile - 1, spins = 0;  // next tile to read
     bool done = false;
     while (!done) {
@@ -265,8 +277,10 @@ __global__ __launch_bounds__(256) void k_onesweep(
     __hip_atomic_store(my_status, kFlagPrefix | (prefix + cnt), __ATOMIC_RELAXED,
                        __HIP_MEMORY_SCOPE_AGENT);
   }
-  s_excl[d] = excl;
-  s_gbase[d] = goff + prefix - excl;  // modular; final positions are in range
+  if (has_d) {
+    s_excl[d] = excl;
+    s_gbase[d] = goff + prefix - excl;  // modular; final positions are in range
+  }
   __syncthreads();
   // Stage in digit order.
 #pragma unroll
@@ -281,7 +295,7 @@ __global__ __launch_bounds__(256) void k_onesweep(
   }
   __syncthreads();
   const uint32_t tile_n = (n - tile_base) < (uint32_t)kTile ? (n - tile_base) : (uint32_t)kTile;
-  for (uint32_t j = tid; j < tile_n; j += kThreads) {
+  for (uint32_t j = tid; j < tile_n; j += THREADS) {
     const KeyT k = s_key[j];
     const uint32_t g = s_gbase[digit_of(k, shift, mask)] + j;
     if (g < n) {
@@ -618,8 +632,22 @@ inline int launch_pass_items(hipStream_t s, uint32_t n, uint32_t key_bits, const
   const uint32_t tiles = num_tiles_items(n, ITEMS);
   const uint32_t shift = digit * kRadixBits;
   const uint32_t dbits = pass_bits(key_bits, digit);
+  // each pass's status region is sized for 256-thread tiles
   uint32_t* st = L.status + (size_t)slot * tiles * kRadix;
-  if (iota) {
+  const int wide_opt = refit_option(kSortWideTiles);
+  const bool wide = wide_opt == 2 || (wide_opt == 1 && n >= kWideTilesFrom);
+  if (wide) {
+    const uint32_t wt = ceil_div(n, (uint64_t)kWideThreads * ITEMS);
+    if (iota) {
+      launch("k_onesweep", (k_onesweep<KeyT, ITEMS, true, kWideThreads>), dim3(wt),
+             dim3(kWideThreads), 0, s, sk, sv, dk, dv, n, n_dev, shift, dbits,
+             L.hist + digit * kRadix, st, L.tickets + slot, err);
+    } else {
+      launch("k_onesweep", (k_onesweep<KeyT, ITEMS, false, kWideThreads>), dim3(wt),
+             dim3(kWideThreads), 0, s, sk, sv, dk, dv, n, n_dev, shift, dbits,
+             L.hist + digit * kRadix, st, L.tickets + slot, err);
+    }
+  } else if (iota) {
     launch("k_onesweep", (k_onesweep<KeyT, ITEMS, true>), dim3(tiles), dim3(kThreads), 0, s, sk,
            sv, dk, dv, n, n_dev, shift, dbits, L.hist + digit * kRadix, st, L.tickets + slot, err);
   } else {

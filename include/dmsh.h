@@ -56,7 +56,9 @@ int dmsh_get_device_errors(uint32_t *bits_out);
  * refit's "refit.leaves_in_aabb" (0, 1), "refit.climb" (0, 1, 2; default 2),
  * "refit.queue" (0, 1), "refit.top" (0, 1), "refit.plan_cache" (0, 1),
  * "refit.compact" (levels climbed before the remaining climbs move to the
- * first lanes, 0..1023; default 0: never).  get returns -1 for an unknown key. */
+ * first lanes, 0..1023; default 0: never), and the sort's "sort.wide_tiles"
+ * (digit passes in 512-thread tiles: 0 never, 1 from 2^21 elements (default),
+ * 2 always).  get returns -1 for an unknown key. */
 int dmsh_set_tuning(const char *key, int value);
 int dmsh_get_tuning(const char *key);
 

@@ -149,7 +149,7 @@ def test_tuning_keys_accept_and_reject_without_a_gpu():
         assert L.dmsh_set_tuning(key, 1 - old) == 0 and L.dmsh_get_tuning(key) == 1 - old
         assert L.dmsh_set_tuning(key, 2) != 0 and L.dmsh_get_tuning(key) == 1 - old
         assert L.dmsh_set_tuning(key, old) == 0
-    for key, top in ((b"refit.climb", 2), (b"refit.compact", 1023)):
+    for key, top in ((b"refit.climb", 2), (b"refit.compact", 1023), (b"sort.wide_tiles", 2)):
         old = L.dmsh_get_tuning(key)
         try:
             for v in (0, top):
