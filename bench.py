@@ -67,7 +67,7 @@ def parse():
 
 
 # kernels whose launch time the bench reads back from the library's event timing
-TIMED_KERNELS = ["k_pix2tri", "k_pix2tri_tier2", "k_pix2tri_tier3", "k_pack_inner", "k_pack_leaf",
+TIMED_KERNELS = ["k_pix2tri", "k_pix2tri_tier2", "k_pix2tri_tier3", "k_pack_tree",
                  "k_build_init", "k_centroid_aabb", "k_morton_hist", "k_hist_all", "k_onesweep",
                  "k_sort_finish", "k_sort_groups", "k_topology",
                  "k_refit_treelet", "k_refit_top", "k_fill_u32"]
@@ -121,7 +121,7 @@ COUNTER_PASSES = {
 }
 # (phase, kernel-name fragment) rows of the summary
 PROFILED_KERNELS = [
-    ("cast", "k_pix2tri<"), ("cast", "k_pack_inner"), ("cast", "k_pack_leaf"),
+    ("cast", "k_pix2tri<"), ("cast", "k_pack_tree"),
     ("build", "k_centroid_aabb<true>"), ("build", "k_morton_hist"), ("build", "k_onesweep"),
     ("build", "k_sort_finish"), ("build", "k_topology"), ("build", "k_refit_treelet<1,"),
     ("build", "k_refit_top_lds"),

@@ -120,14 +120,10 @@ int dmsh_pix2tri(void* stream, uint32_t* pix2tri, float* pix2t, uint32_t num_tri
   hipStream_t s = as_stream(stream);
   int st = sortk::launch_fill(s, L.counts, 3, 0u);
   if (st != DMSH_OK) return st;
-  if (num_tri > 1) {
-    launch("k_pack_inner", rc::k_pack_inner, dim3(ceil_div(num_tri - 1, 256)), dim3(256), 0, s,
-                       L.inner, bvhnodes, bvhnode2aabb, num_tri, L.counts + 2, err);
-    if ((st = launch_status()) != DMSH_OK) return st;
-  }
   if (num_tri > 0) {
-    launch("k_pack_leaf", rc::k_pack_leaf, dim3(ceil_div(num_tri, 256)), dim3(256), 0, s, L.leaf,
-                       bvhnodes, tri2vtx, vtx2xyz, num_tri, num_tri, num_vtx, err);
+    launch("k_pack_tree", rc::k_pack_tree, dim3(ceil_div(2 * (uint64_t)num_tri - 1, 256)),
+           dim3(256), 0, s, L.inner, L.leaf, bvhnodes, bvhnode2aabb, tri2vtx, vtx2xyz, num_tri,
+           num_vtx, L.counts + 2, err);
     if ((st = launch_status()) != DMSH_OK) return st;
   }
   rc::Packed pk{L.inner, L.leaf, bvhnode2aabb, num_tri};

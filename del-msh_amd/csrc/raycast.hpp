@@ -73,19 +73,18 @@ struct Packed {
 // ---------------------------------------------------------------------------
 // packing
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ uint32_t child_ref(const uint32_t* __restrict__ bvhnodes, uint32_t c,
-                                              uint32_t num_leaf, bool& bad) {
+// A child id in the LBVH layout: ids from num_leaf - 1 on are leaves.  The
+// child's own row is not read here: every row's shape is checked by the
+// thread that packs it (an internal row has a second child, a leaf row
+// none), which raises DMSH_DEVERR_INDEX and packs a never-hit node or
+// triangle for a row that does not fit its place in the layout.
+__device__ __forceinline__ uint32_t child_ref(uint32_t c, uint32_t num_leaf, bool& bad) {
   const uint32_t nb = num_leaf - 1, num_node = 2 * num_leaf - 1;
   if (c >= num_node) {
     bad = true;
     return 0;
   }
-  const bool leaf = bvhnodes[(size_t)c * 3 + 2] == DMSH_U32MAX;
-  if (leaf != (c >= nb)) {  // not an LBVH layout
-    bad = true;
-    return 0;
-  }
-  return leaf ? (kLeafBit | (c - nb)) : c;
+  return c >= nb ? (kLeafBit | (c - nb)) : c;
 }
 
 // Internal node i (i + 1 < num_leaf) of one tree into inner[i].
@@ -97,8 +96,8 @@ __device__ __forceinline__ void pack_inner(InnerNode* __restrict__ inner,
   const uint32_t* row = bvhnodes + (size_t)i * 3;
   const uint32_t c0 = row[1], c1 = row[2];
   bool bad = (c1 == DMSH_U32MAX);
-  const uint32_t r0 = child_ref(bvhnodes, c0, num_leaf, bad);
-  const uint32_t r1 = child_ref(bvhnodes, c1, num_leaf, bad);
+  const uint32_t r0 = child_ref(c0, num_leaf, bad);
+  const uint32_t r1 = child_ref(c1, num_leaf, bad);
   InnerNode nd;
   if (bad) {
     raise_error(err, DMSH_DEVERR_INDEX);
@@ -121,17 +120,6 @@ __device__ __forceinline__ void pack_inner(InnerNode* __restrict__ inner,
     if (!fin) atomicOr(nonfinite, 1u);
   }
   inner[i] = nd;
-}
-
-static __global__ __launch_bounds__(256) void k_pack_inner(InnerNode* __restrict__ inner,
-                                                           const uint32_t* __restrict__ bvhnodes,
-                                                           const float* __restrict__ aabb,
-                                                           uint32_t num_leaf,
-                                                           uint32_t* __restrict__ nonfinite,
-                                                           uint32_t* __restrict__ err) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i + 1 >= num_leaf) return;
-  pack_inner(inner, bvhnodes, aabb, num_leaf, i, nonfinite, err);
 }
 
 // Leaf k of one tree into leaf[k].
@@ -170,16 +158,20 @@ __device__ __forceinline__ void pack_leaf(LeafTri* __restrict__ leaf,
   leaf[k] = lt;
 }
 
-static __global__ __launch_bounds__(256) void k_pack_leaf(LeafTri* __restrict__ leaf,
-                                                          const uint32_t* __restrict__ bvhnodes,
-                                                          const uint32_t* __restrict__ tri2vtx,
-                                                          const float* __restrict__ xyz,
-                                                          uint32_t num_leaf, uint32_t num_tri,
-                                                          uint32_t num_vtx,
-                                                          uint32_t* __restrict__ err) {
-  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= num_leaf) return;
-  pack_leaf(leaf, bvhnodes, tri2vtx, xyz, num_leaf, num_tri, num_vtx, k, err);
+// The whole tree in one launch: threads [0, num_leaf - 1) pack the internal
+// nodes, the next num_leaf the leaves.
+static __global__ __launch_bounds__(256) void k_pack_tree(
+    InnerNode* __restrict__ inner, LeafTri* __restrict__ leaf,
+    const uint32_t* __restrict__ bvhnodes, const float* __restrict__ aabb,
+    const uint32_t* __restrict__ tri2vtx, const float* __restrict__ xyz, uint32_t num_leaf,
+    uint32_t num_vtx, uint32_t* __restrict__ nonfinite, uint32_t* __restrict__ err) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t nb = num_leaf - 1;
+  if (i < nb) {
+    pack_inner(inner, bvhnodes, aabb, num_leaf, i, nonfinite, err);
+  } else if (i - nb < num_leaf) {
+    pack_leaf(leaf, bvhnodes, tri2vtx, xyz, num_leaf, num_leaf, num_vtx, i - nb, err);
+  }
 }
 
 // ---------------------------------------------------------------------------

@@ -416,6 +416,28 @@ def test_pix2tri(gpu, oracle, mesh, camera):
     no_device_errors()
 
 
+def test_pix2tri_flags_rows_out_of_layout(gpu, oracle):
+    """The cast packs each row by itself: a leaf row with a second child, or
+    a child id past the last node, becomes a never-hit triangle / node and
+    raises DMSH_DEVERR_INDEX; every other ray ends as before."""
+    tri, vtx = MESHES["torus"]()
+    bn, aabb = oracle.build_trimesh3(tri, vtx)
+    n = tri.shape[0]
+    n2w = _ortho_c1()
+    W, H_ = 128, 96
+    wpix, _, _ = oracle.pix2tri(tri, vtx, bn, aabb, n2w.T.reshape(-1), W, H_, num_threads=8)
+    no_device_errors()
+    for row, col, value in ((n - 1 + 5, 2, 0), (3, 1, 2 * n + 7)):
+        bad = bn.reshape(-1, 3).copy()
+        bad[row, col] = value
+        pix = Pix2Tri.by_raycasting(G(tri, gpu), G(vtx, gpu), G(bad.reshape(bn.shape), gpu),
+                                    G(aabb, gpu), G(n2w, gpu), (W, H_))
+        assert _lib.device_errors() & 0x1
+        got = H(pix)
+        assert got.shape == wpix.shape
+        assert np.all((got == wpix) | (got == 0xFFFFFFFF) | (got < n))
+
+
 def test_c1_config_at_size(gpu, oracle):
     """The plain-path workload at its stated size: UV sphere (r = 1, 64 x 128
     segments, 16,128 triangles), 30-bit codes, LBVH build + refit on the
