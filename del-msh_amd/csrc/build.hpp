@@ -346,6 +346,7 @@ inline int build(hipStream_t s, uint32_t n, const uint32_t* tri2vtx, uint32_t nu
   const bool leaves_in_aabb = refit_option(kRefitLeavesInAabb) == 1;
   TreeletSink ts{kTreeletLeaves, L.treelet_count, L.treelets, n, nullptr, nullptr,
                  L.top_count, L.top_list, (uint32_t)kTopSlots, L.node2slot};
+  ts.stage_pos = true;
   if (leaves_in_aabb) {
     ts.leaf_aabb = aabb;
     ts.tri_box = TRI ? L.tri_box : nullptr;

@@ -237,6 +237,9 @@ struct TreeletLds {
 // bits each; 0xFFFF for a child below the top, 0xFFFE for a slot out of
 // range) and box[2 e + side] the box of a child below the top (a treelet
 // root; side 0 is the left child).  Kept for slots below kTopClimbSlots.
+// The lister of the treelets (topology kernel, k_list_treelets) knows the
+// parent's slot when it lists a treelet and puts 2 e + side in the entry's
+// last word (all ones: no parent slot).
 constexpr int kTopClimbSlots = 4096;
 
 struct TopStage {
@@ -300,7 +303,8 @@ __device__ __forceinline__ void refit_treelet(const TreeView& v, uint32_t root, 
                                               uint32_t* __restrict__ err,
                                               const TreeletIn<K / T>* pre = nullptr,
                                               int compact = 4, bool staged = false,
-                                              TopStage st = TopStage{}) {
+                                              TopStage st = TopStage{},
+                                              uint32_t stage_pos = DMSH_U32MAX) {
   constexpr int LPT = K / T;
   const uint32_t num_leaf = v.num_leaf;
   const uint32_t nb = num_leaf - 1;
@@ -359,16 +363,6 @@ __device__ __forceinline__ void refit_treelet(const TreeView& v, uint32_t root, 
     }
   }
   __syncthreads();
-  // the top slot of the root's parent and its left child, for the staged
-  // root box (read now, used after the climb)
-  uint32_t st_slot = DMSH_U32MAX, st_left = DMSH_U32MAX;
-  if (staged && threadIdx.x == 0) {
-    const uint32_t p = root >= nb ? v.bvhnodes[(size_t)root * 3] : s.parent[root - f];
-    if (p < nb) {
-      st_slot = st.node2slot[p];
-      st_left = v.bvhnodes[(size_t)p * 3 + 1];
-    }
-  }
   // phase 2: a thread's LPT climbs advance in lock-step, one level per round
   // (a round costs as much as the longest of them, not their sum); the
   // parent's static links are read before its arrival counter, so a level
@@ -579,9 +573,11 @@ __device__ __forceinline__ void refit_treelet(const TreeView& v, uint32_t root, 
     const uint32_t j = (uint32_t)r * T + threadIdx.x;
     if (j < m && f + j < nb && s.flag[j] >= 2u) box_store(v.aabb, f + j, s.inner[j]);
   }
-  if (st_slot < (uint32_t)kTopClimbSlots) {  // thread 0
+  // the root's box into its place in the top merge's staging (the
+  // treelet's entry carries 2 * parent slot + side)
+  if (staged && threadIdx.x == 0 && stage_pos < 2u * (uint32_t)kTopClimbSlots) {
     const Box b = root >= nb ? s.leaf[root - nb - f] : s.inner[root - f];
-    box_store(st.box, 2u * st_slot + (st_left == root ? 0u : 1u), b);
+    box_store(st.box, stage_pos, b);
   }
   if (CLIMB == 2 && threadIdx.x == 0) {
     s.queued = 0u;
@@ -716,7 +712,7 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_num_sgpr(72))) void k_refi
       // climb, where the ticket is stored
       if (next && threadIdx.x == 0) t4 = atomicAdd(next + __lane_id(), 1u);
       refit_treelet<LK, K, T, true, CLIMB>(v, d.x, d.y, d.z, eps, s, err, &cur, compact, staged,
-                                           st);
+                                           st, d.w);
       if (next && threadIdx.x == 0) s_t2 = t4;
       __syncthreads();  // thread 0's ticket is visible (also after an early return)
       t4 = next ? s_t2 : t3 + gridDim.x;
@@ -732,7 +728,7 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_num_sgpr(72))) void k_refi
     for (uint32_t t = blockIdx.x; t < count; t += gridDim.x) {
       const uint4 tl = treelets[t];
       refit_treelet<LK, K, T, false, CLIMB>(v, tl.x, tl.y, tl.z, eps, s, err, nullptr, compact,
-                                            staged, st);
+                                            staged, st, tl.w);
     }
   } else {
     for (;;) {
@@ -743,7 +739,7 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_num_sgpr(72))) void k_refi
       if (t >= count) break;
       const uint4 tl = treelets[t];
       refit_treelet<LK, K, T, false, CLIMB>(v, tl.x, tl.y, tl.z, eps, s, err, nullptr, compact,
-                                            staged, st);
+                                            staged, st, tl.w);
     }
   }
 }
@@ -1330,10 +1326,10 @@ __global__ __launch_bounds__(T) void k_list_treelets(
       } else if (l - f + 1 > K) {
         n_top = 1;
         top = make_uint4(i, a, b, (la - fa + 1 > K ? 1u : 0u) | (lb - fb + 1 > K ? 2u : 0u));
-        if (la - fa + 1 <= K) tl[n_tl++] = make_uint4(a, fa, la, 0u);
-        if (lb - fb + 1 <= K) tl[n_tl++] = make_uint4(b, fb, lb, 0u);
+        if (la - fa + 1 <= K) tl[n_tl++] = make_uint4(a, fa, la, 0u);  // w: side
+        if (lb - fb + 1 <= K) tl[n_tl++] = make_uint4(b, fb, lb, 1u);
       } else if (i == 0) {
-        tl[n_tl++] = make_uint4(0u, 0u, nb, 0u);
+        tl[n_tl++] = make_uint4(0u, 0u, nb, DMSH_U32MAX);
       }
     }
     if (threadIdx.x == 0) s_cnt[0] = s_cnt[1] = 0u;
@@ -1346,8 +1342,13 @@ __global__ __launch_bounds__(T) void k_list_treelets(
                          ((unsigned long long)s_cnt[1] << 32) | s_cnt[0]);
     __syncthreads();
     const uint32_t b_tl = (uint32_t)s_base, b_top = (uint32_t)(s_base >> 32);
-    for (uint32_t e = 0; e < n_tl; ++e)
-      if (b_tl + o_tl + e < num_leaf) treelets[b_tl + o_tl + e] = tl[e];
+    const uint32_t my_top = b_top + o_top;
+    for (uint32_t e = 0; e < n_tl; ++e) {
+      uint4 t = tl[e];
+      t.w = (t.w != DMSH_U32MAX && n_top && my_top < (uint32_t)kTopSlots) ? 2u * my_top + t.w
+                                                                           : DMSH_U32MAX;
+      if (b_tl + o_tl + e < num_leaf) treelets[b_tl + o_tl + e] = t;
+    }
     if (n_top && b_top + o_top < (uint32_t)kTopSlots) {
       top_list[b_top + o_top] = top;
       node2slot[i] = b_top + o_top;

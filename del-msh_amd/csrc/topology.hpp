@@ -136,6 +136,10 @@ struct TreeletSink {
   uint4* top_list = nullptr;
   uint32_t top_cap = 0;
   uint32_t* node2slot = nullptr;
+  // Single-tree builds: a treelet entry's last word is 2 * (its parent's top
+  // slot) + (1 for a right child), all ones when there is none, in place of
+  // the mesh index.
+  bool stage_pos = false;
   // Optional: the leaf boxes, written into leaf_aabb (the tree's bvhnode2aabb)
   // as the leaf rows are: from tri_box (6 floats per object) when given, else
   // the point xyz[obj] (the box the refit's leaf_box makes for it).
@@ -186,9 +190,11 @@ __device__ __forceinline__ void topology_emit(uint32_t n, uint32_t i, uint32_t f
   const uint32_t nb = n - 1;
   if (ts.list) {
     if (last - first + 1 > ts.k) {
+      uint32_t pos = DMSH_U32MAX;  // 2 * this node's top slot
       if (ts.top_count) {
         const uint32_t slot = atomicAdd(ts.top_count, 1u);
         if (slot < ts.top_cap) {
+          pos = 2u * slot;
           ts.top_list[slot] = make_uint4(i, first == split ? nb + split : split,
                                          last == split + 1 ? nb + split + 1 : split + 1,
                                          (split - first + 1 > ts.k ? 1u : 0u) |
@@ -197,12 +203,13 @@ __device__ __forceinline__ void topology_emit(uint32_t n, uint32_t i, uint32_t f
         }
       }
       if (split - first + 1 <= ts.k)
-        emit_treelet(ts, first == split ? nb + split : split, first, split, mesh, root_base);
+        emit_treelet(ts, first == split ? nb + split : split, first, split,
+                     ts.stage_pos ? pos : mesh, root_base);
       if (last - split <= ts.k)
-        emit_treelet(ts, last == split + 1 ? nb + split + 1 : split + 1, split + 1, last, mesh,
-                     root_base);
+        emit_treelet(ts, last == split + 1 ? nb + split + 1 : split + 1, split + 1, last,
+                     ts.stage_pos ? (pos == DMSH_U32MAX ? pos : pos + 1u) : mesh, root_base);
     } else if (i == 0) {
-      emit_treelet(ts, 0u, 0u, n - 1, mesh, root_base);
+      emit_treelet(ts, 0u, 0u, n - 1, ts.stage_pos ? DMSH_U32MAX : mesh, root_base);
     }
   }
   if (first == split) {
@@ -252,7 +259,7 @@ __device__ __forceinline__ void topology_single(uint32_t* __restrict__ bvhnodes,
   bvhnodes[1] = obj;
   bvhnodes[2] = DMSH_U32MAX;
   if (ts.leaf_aabb) emit_leaf_box(ts, 0u, obj);
-  if (ts.list) emit_treelet(ts, 0u, 0u, 0u, mesh, root_base);
+  if (ts.list) emit_treelet(ts, 0u, 0u, 0u, ts.stage_pos ? DMSH_U32MAX : mesh, root_base);
 }
 
 // One thread per internal node; the block's codes and kTopoHalo either side
