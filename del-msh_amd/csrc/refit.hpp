@@ -56,6 +56,16 @@ __device__ __forceinline__ Box box_merge(const Box& a, const Box& b) {
   }
   return r;
 }
+// Element i of an LDS box array, addressed with a 24-bit multiply (indices
+// here are LDS slots, far below 2^24).
+__device__ __forceinline__ const Box& lds_box(const Box* arr, uint32_t i) {
+  return *reinterpret_cast<const Box*>(reinterpret_cast<const char*>(arr) +
+                                       __umul24(i, (uint32_t)sizeof(Box)));
+}
+__device__ __forceinline__ Box& lds_box(Box* arr, uint32_t i) {
+  return *reinterpret_cast<Box*>(reinterpret_cast<char*>(arr) + __umul24(i, (uint32_t)sizeof(Box)));
+}
+
 // box_merge(mine, other) when mine_left, else box_merge(other, mine), with
 // the operands selected first so that both orders run the same instructions.
 __device__ __forceinline__ Box box_merge_side(const Box& mine, const Box& other, bool mine_left) {
@@ -399,8 +409,8 @@ __device__ __forceinline__ void refit_treelet(const TreeView& v, uint32_t root, 
       const uint32_t j = s.queue[threadIdx.x];
       node = f + j;
       const uint2 c = s.child[j];
-      me = box_merge(s.leaf[c.x - nb - f], s.leaf[c.y - nb - f]);
-      s.inner[j] = me;
+      me = box_merge(lds_box(s.leaf, c.x - nb - f), lds_box(s.leaf, c.y - nb - f));
+      lds_box(s.inner, j) = me;
       if (node == root) {
         a = false;
       } else {
@@ -417,7 +427,7 @@ __device__ __forceinline__ void refit_treelet(const TreeView& v, uint32_t root, 
         const uint32_t pj = p - f;
         const uint32_t old = __hip_atomic_fetch_add(&s.flag[pj], 1u, __ATOMIC_ACQ_REL,
                                                     __HIP_MEMORY_SCOPE_WAVEFRONT);
-        const Box sb = (sib >= nb) ? s.leaf[sib - nb - f] : s.inner[sib - f];
+        const Box sb = (sib >= nb) ? lds_box(s.leaf, sib - nb - f) : lds_box(s.inner, sib - f);
         uint2 ch2 = make_uint2(DMSH_U32MAX, DMSH_U32MAX);
         uint32_t gp2 = DMSH_U32MAX;
         if (p != root) {
@@ -429,7 +439,7 @@ __device__ __forceinline__ void refit_treelet(const TreeView& v, uint32_t root, 
           break;
         }
         me = box_merge_side(me, sb, left);
-        s.inner[pj] = me;
+        lds_box(s.inner, pj) = me;
         if (p == root) {
           a = false;
           break;
@@ -879,8 +889,8 @@ __device__ __forceinline__ void top_climb(float* __restrict__ aabb, uint32_t num
                                                      __HIP_MEMORY_SCOPE_WAVEFRONT) >>
                               sh) &
                              0xFFu;
-        const Box A = la != kNone ? s_box[la] : s_box[p];
-        const Box B = lb != kNone ? s_box[lb] : s_box[p];
+        const Box A = lds_box(s_box, la != kNone ? la : p);
+        const Box B = lds_box(s_box, lb != kNone ? lb : p);
         uint32_t kp2 = 0u, pn2 = kNone;
         if (pn != kNone) {
           kp2 = s_kid[pn];
@@ -890,7 +900,7 @@ __device__ __forceinline__ void top_climb(float* __restrict__ aabb, uint32_t num
           go = false;
           break;
         }
-        s_box[p] = box_merge(A, B);
+        lds_box(s_box, p) = box_merge(A, B);
         c = p;
         p = pn;
         kp = kp2;
