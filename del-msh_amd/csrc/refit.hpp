@@ -245,8 +245,10 @@ struct TreeletLds {
 // the one workgroup of the top merge reads it with coalesced loads instead
 // of gathering it: kid[e] packs the slots of top slot e's top children (16
 // bits each; 0xFFFF for a child below the top, 0xFFFE for a slot out of
-// range) and box[2 e + side] the box of a child below the top (a treelet
-// root; side 0 is the left child).  Kept for slots below kTopClimbSlots.
+// range) and the box of a child below the top (a treelet root; side 0 is
+// the left child) component-major, bound d of side s at
+// box[(6 s + d) kTopClimbSlots + e], so that consecutive slots' bounds are
+// consecutive words.  Kept for slots below kTopClimbSlots.
 // The lister of the treelets (topology kernel, k_list_treelets) knows the
 // parent's slot when it lists a treelet and puts 2 e + side in the entry's
 // last word (all ones: no parent slot).
@@ -587,7 +589,9 @@ __device__ __forceinline__ void refit_treelet(const TreeView& v, uint32_t root, 
   // treelet's entry carries 2 * parent slot + side)
   if (staged && threadIdx.x == 0 && stage_pos < 2u * (uint32_t)kTopClimbSlots) {
     const Box b = root >= nb ? s.leaf[root - nb - f] : s.inner[root - f];
-    box_store(st.box, stage_pos, b);
+    const uint32_t e = stage_pos >> 1, side = stage_pos & 1u;
+#pragma unroll
+    for (int d = 0; d < 6; ++d) st.box[(side * 6u + (uint32_t)d) * kTopClimbSlots + e] = b.v[d];
   }
   if (CLIMB == 2 && threadIdx.x == 0) {
     s.queued = 0u;
@@ -816,23 +820,34 @@ __device__ __forceinline__ void top_climb(float* __restrict__ aabb, uint32_t num
   for (uint32_t w = threadIdx.x; w < (count + 3) / 4; w += T) s_cnt[w] = 0u;
   for (uint32_t e = threadIdx.x; e < count; e += T) s_par[e] = (uint16_t)kNone;
   if (threadIdx.x < 3) s_qn[threadIdx.x] = 0u;
-  uint4 rec[R];
-  uint32_t kid[R];
+  // every slot's loads are issued before any of them is used (one memory
+  // latency for the pass, not one per slot of this thread)
+  // (a slot's top children are the ones kid names; its node id is needed
+  // only for the final store)
+  uint32_t node[R], kid[R];
+  Box ba[R], bb[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const uint32_t e = (uint32_t)r * T + threadIdx.x;
+    const uint32_t ec = e < count ? e : 0u;  // in bounds either way; unused past count
+    node[r] = top_list[ec].x;
+    kid[r] = st.kid[ec];
+#pragma unroll
+    for (int d = 0; d < 6; ++d) {
+      ba[r].v[d] = st.box[(uint32_t)d * C + ec];
+      bb[r].v[d] = st.box[(uint32_t)(6 + d) * C + ec];
+    }
+  }
   Box own[R];
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     const uint32_t e = (uint32_t)r * T + threadIdx.x;
-    rec[r] = make_uint4(DMSH_U32MAX, 0u, 0u, 0u);
-    kid[r] = DMSH_U32MAX;
-    if (e < count) {
-      rec[r] = top_list[e];
-      kid[r] = st.kid[e];
-      const float4* bp = reinterpret_cast<const float4*>(st.box + (size_t)e * 12);
-      const float4 x = bp[0], y = bp[1], z = bp[2];
-      const Box a{{x.x, x.y, x.z, x.w, y.x, y.y}}, b{{y.z, y.w, z.x, z.y, z.z, z.w}};
-      const bool ta = rec[r].w & 1u, tb = rec[r].w & 2u;
-      own[r] = (!ta && !tb) ? box_merge(a, b) : (ta ? b : a);
+    if (e >= count) {
+      node[r] = DMSH_U32MAX;
+      continue;
     }
+    const bool ta = (kid[r] & 0xFFFFu) != kNone, tb = (kid[r] >> 16) != kNone;
+    own[r] = (!ta && !tb) ? box_merge(ba[r], bb[r]) : (ta ? bb[r] : ba[r]);
   }
   __syncthreads();  // s_par, s_cnt and the queue counters cleared
   bool bad = false;
@@ -840,9 +855,9 @@ __device__ __forceinline__ void top_climb(float* __restrict__ aabb, uint32_t num
   for (int r = 0; r < R; ++r) {
     const uint32_t e = (uint32_t)r * T + threadIdx.x;
     if (e >= count) continue;
-    const bool ta = rec[r].w & 1u, tb = rec[r].w & 2u;
     const uint32_t sa = kid[r] & kNone, sb = kid[r] >> 16;
-    if (rec[r].x >= nb || (ta ? sa >= count : sa != kNone) || (tb ? sb >= count : sb != kNone)) {
+    const bool ta = sa != kNone, tb = sb != kNone;
+    if (node[r] >= nb || (ta && sa >= count) || (tb && sb >= count)) {
       bad = true;
       continue;
     }
@@ -917,7 +932,7 @@ __device__ __forceinline__ void top_climb(float* __restrict__ aabb, uint32_t num
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     const uint32_t e = (uint32_t)r * T + threadIdx.x;
-    if (e < count) box_store(aabb, rec[r].x, s_box[e]);
+    if (e < count) box_store(aabb, node[r], s_box[e]);
   }
 }
 
