@@ -50,8 +50,10 @@ inline bool carve_batch_build(Arena& a, uint32_t M, uint32_t n, BatchBuildLayout
   L.sort.tickets = L.sort.hist + sortk::kMaxPasses * sortk::kRadix;
   L.sort.status = L.sort.tickets + sortk::kTicketWords;
   // sized for the smallest tile, so that batch.sort_items may change
-  // between the size query and the build
-  const size_t status_words = sortk::status_words_items(n, kb, sortk::kMinItems);
+  // between the size query and the build, and for one pass per digit (the
+  // batched sort takes no LDS finish, whatever the number of digits)
+  const size_t status_words =
+      (size_t)sortk::num_passes(kb) * sortk::num_tiles_items(n, sortk::kMinItems) * sortk::kRadix;
   a.off += sizeof(uint32_t) * status_words;
   L.flags = L.sort.status + status_words;
   a.off += sizeof(uint32_t) * nn;

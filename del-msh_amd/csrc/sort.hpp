@@ -575,9 +575,6 @@ template <typename KeyT>
 inline size_t status_words(uint32_t n, uint32_t key_bits) {
   return (size_t)status_passes(num_passes(key_bits)) * num_tiles<KeyT>(n) * kRadix;
 }
-inline size_t status_words_items(uint32_t n, uint32_t key_bits, int items) {
-  return (size_t)status_passes(num_passes(key_bits)) * num_tiles_items(n, items) * kRadix;
-}
 
 // Workspace: hist[8][256] + tickets[16] + status[passes][tiles][256] +
 // alt keys + alt values + the finish's group list.

@@ -419,7 +419,9 @@ int dmsh_antialias_bwd(void *stream, uint32_t num_cedge,
  * Tuning keys (results do not depend on them): "batch.stack" (16, 20, 24,
  * 32; default 20),
  * "batch.wgs_per_cu" (persistent ray-cast workgroups per CU, 1..64),
- * "batch.treelet_grid" (refit workgroups, 1..65536).
+ * "batch.treelet_grid" (refit workgroups, 1..65536),
+ * "batch.sort_items" (elements per thread of the digit passes: 8, 12, 16),
+ * "batch.cast_waves" (waves per ray-cast workgroup: 1, 2, 4).
  * ------------------------------------------------------------------- */
 int dmsh_batch_set_tuning(const char *key, int value);
 int dmsh_batch_get_tuning(const char *key);

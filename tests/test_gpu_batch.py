@@ -111,6 +111,8 @@ def test_batch_results_do_not_depend_on_tuning(gpu):
                                                       b"batch.treelet_grid", b"batch.sort_items",
                                                       b"batch.cast_waves")}
     try:
+        # 19 meshes: 35-bit composite keys, five digit passes, each with its
+        # own status region at every tile size
         for stack, wgs, tg, items, waves in ((16, 1, 3, 8, 2), (32, 64, 4096, 12, 4),
                                               (24, 3, 1024, 16, 1)):
             assert L.dmsh_batch_set_tuning(b"batch.stack", stack) == 0
