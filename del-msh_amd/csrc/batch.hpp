@@ -251,6 +251,167 @@ static __global__ __launch_bounds__(256) void k_batch_morton_hist(
   sortk::hist_flush(s_h, 0u, num_passes, hist);
 }
 
+// The segmented sort of the composite keys: the triangles are ordered by
+// mesh, so a mesh's keys keep its range and only the 30-bit code needs
+// ordering: three passes over code bits [6, 30) scatter inside the meshes,
+// the finish orders the low 6 bits.  Up to kSegMaxMeshes meshes (the offset
+// tables take 2 x 3 KB per mesh).
+constexpr uint32_t kSegLow = 6;          // code bits the finish orders
+constexpr uint32_t kSegMaxMeshes = 8192;
+constexpr uint32_t kSegWindow = 4;       // meshes a block counts in LDS
+constexpr uint32_t kSegTables = sortk::kTopPasses * sortk::kRadix;  // words per mesh
+
+// Composite keys (as k_batch_morton_hist) and per mesh the histograms of the
+// segmented passes' digits, hseg[p][m][d] for code bits [6 + 8p, 14 + 8p).
+// Each block takes one contiguous range of triangles in rounds of 256, so it
+// meets the meshes in order: a window of kSegWindow consecutive meshes counts
+// in LDS and is flushed to hseg when a round reaches past it (then it starts
+// at the round's first mesh); a round spanning more meshes than the window
+// (meshes of fewer than 64 triangles) counts the rest with global atomics.
+static __global__ __launch_bounds__(256) void k_batch_morton_seghist(
+    const uint32_t* __restrict__ bad, uint32_t total_tri, uint32_t M,
+    const uint32_t* __restrict__ mesh2tri, const float* __restrict__ cntr,
+    const uint32_t* __restrict__ keys6, uint64_t* __restrict__ codes,
+    uint32_t* __restrict__ hseg) {
+  if (*bad) return;  // the sort then orders whatever the keys hold, in bounds
+  constexpr uint32_t kP = sortk::kTopPasses, kR = sortk::kRadix, kW = kSegWindow * kP * kR;
+  __shared__ uint32_t s_h[kW];
+  const uint32_t per = ceil_div_u32(ceil_div_u32(total_tri, gridDim.x), 256u) * 256u;
+  const uint32_t t0 = blockIdx.x * per;
+  if (t0 >= total_tri) return;  // uniform over the block
+  const uint32_t t1 = total_tri - t0 < per ? total_tri : t0 + per;
+  for (uint32_t e = threadIdx.x; e < kW; e += blockDim.x) s_h[e] = 0u;
+  uint32_t wbase = find_segment(mesh2tri, M, t0);  // the window's first mesh (uniform)
+  auto flush = [&]() {
+    for (uint32_t e = threadIdx.x; e < kW; e += blockDim.x) {
+      const uint32_t v = s_h[e];
+      const uint32_t w = e / (kP * kR), q = (e / kR) % kP, d = e % kR;
+      if (v && wbase + w < M) atomicAdd(hseg + ((size_t)q * M + wbase + w) * kR + d, v);
+      s_h[e] = 0u;
+    }
+  };
+  __syncthreads();
+  for (uint32_t r0 = t0; r0 < t1; r0 += blockDim.x) {
+    const uint32_t rl = t1 - r0 < blockDim.x ? t1 - 1 : r0 + blockDim.x - 1;
+    const uint32_t mlast = find_segment(mesh2tri, M, rl);  // uniform
+    if (mlast - wbase >= kSegWindow) {
+      __syncthreads();  // the window's counts are complete
+      flush();
+      wbase = find_segment(mesh2tri, M, r0);
+      __syncthreads();  // zeroed before the round counts
+    }
+    const uint32_t t = r0 + threadIdx.x;
+    if (t < t1) {
+      const uint32_t m = find_segment(mesh2tri, M, t);
+      float T[16];
+      bld::fit_transform(keys6 + (size_t)m * 6, T);
+      const float* p = cntr + (size_t)t * 3;
+      const uint32_t code = morton3_transformed<uint32_t>(T, p[0], p[1], p[2]);
+      codes[t] = ((uint64_t)m << kMeshKeyShift) | code;
+      const uint32_t w = m - wbase;
+#pragma unroll
+      for (uint32_t q = 0; q < kP; ++q) {
+        const uint32_t d = (code >> (kSegLow + q * sortk::kRadixBits)) & (kR - 1u);
+        if (w < kSegWindow) atomicAdd(&s_h[(w * kP + q) * kR + d], 1u);
+        else atomicAdd(hseg + ((size_t)q * M + m) * kR + d, 1u);
+      }
+    }
+  }
+  __syncthreads();
+  flush();
+}
+
+// Scatter offsets of the segmented passes (sortk::SegOff), per pass p, mesh
+// m and digit d: off[p][m][d] = mesh2tri[m] + (triangles of mesh m with a
+// smaller digit) - (triangles of meshes before m with digit d).  Meshes in
+// chunks of kSegChunk, one block per (chunk, pass), a wave per kPerWave
+// meshes with four digits per lane and every load issued first.
+// k_batch_seg_rows counts the last term only over the meshes of its own
+// chunk (the waves before from LDS, the wave's own earlier meshes from
+// registers) and stores the chunk's per-digit totals; with more than one
+// chunk, k_batch_seg_cols then subtracts the totals of the chunks before.
+// After a data error every offset is zero (the sort then stays in bounds).
+constexpr uint32_t kSegChunk = 64;
+constexpr uint32_t kSegRowThreads = 1024;
+
+static __global__ __launch_bounds__(kSegRowThreads) void k_batch_seg_rows(
+    const uint32_t* __restrict__ bad, uint32_t M, const uint32_t* __restrict__ mesh2tri,
+    const uint32_t* __restrict__ hseg, uint32_t* __restrict__ off,
+    uint32_t* __restrict__ chunk_sum) {
+  constexpr uint32_t kR = sortk::kRadix, kWaves = kSegRowThreads / 64;
+  constexpr uint32_t kPerWave = kSegChunk / kWaves;
+  __shared__ uint32_t s_cs[kWaves][kR];
+  const uint32_t c = blockIdx.x, p = blockIdx.y, wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+  const bool ok = *bad == 0u;
+  const uint32_t* H = hseg + (size_t)p * M * kR;
+  uint32_t* O = off + (size_t)p * M * kR;
+  const uint32_t mw = c * kSegChunk + wave * kPerWave;  // the wave's first mesh
+  uint32_t h[kPerWave][4];
+#pragma unroll
+  for (uint32_t i = 0; i < kPerWave; ++i) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+      h[i][j] = (ok && mw + i < M) ? H[(size_t)(mw + i) * kR + 4 * lane + j] : 0u;
+  }
+  // per mesh: mesh2tri[m] + smaller digits of the mesh - this wave's earlier
+  // meshes with the digit
+  uint32_t r[kPerWave][4], cs[4] = {0u, 0u, 0u, 0u};
+#pragma unroll
+  for (uint32_t i = 0; i < kPerWave; ++i) {
+    const uint32_t t = h[i][0] + h[i][1] + h[i][2] + h[i][3];
+    uint32_t run = sortk::wave_inclusive_scan(t) - t + ((ok && mw + i < M) ? mesh2tri[mw + i] : 0u);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      r[i][j] = run - cs[j];
+      run += h[i][j];
+      cs[j] += h[i][j];
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < 4; ++j) s_cs[wave][4 * lane + j] = cs[j];
+  __syncthreads();
+  uint32_t before[4] = {0u, 0u, 0u, 0u};  // the chunk's earlier waves
+  for (uint32_t w = 0; w < wave; ++w) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) before[j] += s_cs[w][4 * lane + j];
+  }
+#pragma unroll
+  for (uint32_t i = 0; i < kPerWave; ++i) {
+    if (mw + i < M) {
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        O[(size_t)(mw + i) * kR + 4 * lane + j] = ok ? r[i][j] - before[j] : 0u;
+    }
+  }
+  const uint32_t d = threadIdx.x;
+  if (gridDim.x > 1 && d < kR) {
+    uint32_t sum = 0;
+#pragma unroll
+    for (uint32_t w = 0; w < kWaves; ++w) sum += s_cs[w][d];
+    chunk_sum[((size_t)p * gridDim.x + c) * kR + d] = sum;
+  }
+}
+
+static __global__ __launch_bounds__(256) void k_batch_seg_cols(
+    const uint32_t* __restrict__ bad, uint32_t M, uint32_t* __restrict__ off,
+    const uint32_t* __restrict__ chunk_sum) {
+  constexpr uint32_t kR = sortk::kRadix;
+  const uint32_t c = blockIdx.x, p = blockIdx.y, d = threadIdx.x;
+  if (*bad || c == 0) return;  // the offsets stay zero / need nothing before
+  const uint32_t* S = chunk_sum + (size_t)p * gridDim.x * kR;
+  uint32_t col = 0;  // triangles with digit d in the chunks before
+  for (uint32_t k = 0; k < c; ++k) col += S[(size_t)k * kR + d];
+  uint32_t* O = off + (size_t)p * M * kR;
+  const uint32_t m0 = c * kSegChunk;
+  const uint32_t cnt = M - m0 < kSegChunk ? M - m0 : kSegChunk;
+  uint32_t o[kSegChunk];
+#pragma unroll
+  for (uint32_t i = 0; i < kSegChunk; ++i) o[i] = i < cnt ? O[(size_t)(m0 + i) * kR + d] : 0u;
+#pragma unroll
+  for (uint32_t i = 0; i < kSegChunk; ++i)
+    if (i < cnt) O[(size_t)(m0 + i) * kR + d] = o[i] - col;
+}
+
 // Internal node g of all meshes (g < inner_off[M]).
 static __global__ __launch_bounds__(256) void k_batch_topology(
     const uint32_t* __restrict__ bad, uint32_t total_tri, uint32_t M,

@@ -15,6 +15,7 @@ constexpr uint32_t kCastCounts = 256;  // see bat::BatchCast::counts
 std::atomic<int> g_batch_stack{20};      // LDS frames per lane of the batched tier 1
 std::atomic<int> g_batch_wgs_per_cu{16};  // persistent ray-cast waves per CU
 std::atomic<int> g_batch_sort_items{16};  // elements per thread of the digit passes
+std::atomic<int> g_batch_sort_seg{1};     // segmented passes + finish (up to kSegMaxMeshes)
 std::atomic<int> g_batch_cast_waves{1};   // waves per ray-cast workgroup
 std::atomic<int> g_batch_treelet_grid{1024};
 
@@ -32,6 +33,9 @@ struct BatchBuildLayout {
   uint32_t* treelet_count;  // 1
   uint32_t* root_count;     // [M]
   uint32_t* bad;            // 1: offsets rejected
+  uint32_t* hseg;     // [3][M][256] or null (more than kSegMaxMeshes meshes)
+  uint32_t* seg_off;  // [3][M][256] or null
+  uint32_t* seg_chunk;  // [3][chunks of kSegChunk meshes][256] or null
   uint32_t* zero_begin;
   size_t zero_words;
   uint32_t* inner_off;  // [M + 1]
@@ -63,8 +67,12 @@ inline bool carve_batch_build(Arena& a, uint32_t M, uint32_t n, BatchBuildLayout
   a.off += sizeof(uint32_t) * M;
   L.bad = L.root_count + M;
   a.off += sizeof(uint32_t);
+  // per-mesh digit histograms of the segmented sort (zeroed with the rest)
+  const bool seg = M <= bat::kSegMaxMeshes;
+  L.hseg = seg ? L.bad + 1 : nullptr;
+  if (seg) a.off += sizeof(uint32_t) * (size_t)M * bat::kSegTables;
   L.zero_begin = L.sort.hist;
-  L.zero_words = (size_t)(L.bad + 1 - L.sort.hist);
+  L.zero_words = (size_t)((seg ? L.hseg + (size_t)M * bat::kSegTables : L.bad + 1) - L.sort.hist);
   L.sort.zero_words = 0;
   L.inner_off = a.take<uint32_t>((size_t)M + 1);
   L.treelets = a.take<uint4>(nn);
@@ -74,7 +82,10 @@ inline bool carve_batch_build(Arena& a, uint32_t M, uint32_t n, BatchBuildLayout
   L.idx = a.take<uint32_t>(nn);
   L.sort.alt_keys = a.take<uint64_t>(nn);
   L.sort.alt_vals = a.take<uint32_t>(nn);
-  L.sort.groups = nullptr;  // plain passes only
+  L.seg_off = seg ? a.take<uint32_t>((size_t)M * bat::kSegTables) : nullptr;
+  L.seg_chunk = seg ? a.take<uint32_t>((size_t)ceil_div(M, bat::kSegChunk) * bat::kSegTables)
+                    : nullptr;
+  L.sort.groups = seg ? a.take<uint32_t>(sortk::group_capacity(nn)) : nullptr;
   return a.ok();
 }
 
@@ -206,6 +217,11 @@ int dmsh_batch_set_tuning(const char* key, int value) {
     g_batch_cast_waves.store(value);
     return DMSH_OK;
   }
+  if (std::strcmp(key, "batch.sort_seg") == 0) {
+    if (value != 0 && value != 1) return DMSH_ERR_INVALID;
+    g_batch_sort_seg.store(value);
+    return DMSH_OK;
+  }
   return DMSH_ERR_INVALID;
 }
 
@@ -216,6 +232,7 @@ int dmsh_batch_get_tuning(const char* key) {
   if (std::strcmp(key, "batch.treelet_grid") == 0) return g_batch_treelet_grid.load();
   if (std::strcmp(key, "batch.sort_items") == 0) return g_batch_sort_items.load();
   if (std::strcmp(key, "batch.cast_waves") == 0) return g_batch_cast_waves.load();
+  if (std::strcmp(key, "batch.sort_seg") == 0) return g_batch_sort_seg.load();
   return -1;
 }
 
@@ -257,17 +274,38 @@ int dmsh_batch_build_trimesh3(void* stream, uint32_t num_mesh, uint32_t total_tr
   launch("k_batch_centroid", bat::k_batch_centroid,
          dim3(ceil_div(n, 4u * 64u * bat::kCentroidChunks)), dim3(256), 0, s, L.bad, n, M, mesh2tri, mesh2vtx, tri2vtx, vtx2xyz, L.cntr, L.keys6, err);
   if ((rc = launch_status()) != DMSH_OK) return rc;
-  // With an even number of passes the sort returns to the buffer it starts
-  // from; with an odd number it starts from the alternate key buffer.
-  uint64_t* morton_out = (P % 2 == 0) ? L.codes : static_cast<uint64_t*>(L.sort.alt_keys);
-  launch("k_morton_hist", bat::k_batch_morton_hist,
-         dim3(std::min<uint32_t>(sortk::num_tiles<uint64_t>(n), 2048)), dim3(256), 0, s, L.bad, n,
-         M, mesh2tri, L.cntr, L.keys6, morton_out, P, kb, L.sort.hist);
-  if ((rc = launch_status()) != DMSH_OK) return rc;
-  // plain passes: the top digits of a composite key are mostly the mesh index,
-  // so groups of equal top bits are too long for the LDS finish
-  rc = sortk::run_passes<uint64_t>(s, n, kb, L.sort, morton_out, L.codes, L.idx, err, false,
-                                   nullptr, nullptr, g_batch_sort_items.load());
+  if (L.seg_off && g_batch_sort_seg.load() == 1) {
+    // segmented: the mesh bits never move a key, so three passes over the
+    // code's top 24 bits inside the meshes and the finish on the low 6
+    launch("k_morton_hist", bat::k_batch_morton_seghist,
+           dim3(std::min<uint32_t>(ceil_div(n, 2048), 2048)), dim3(256), 0, s, L.bad, n, M,
+           mesh2tri, L.cntr, L.keys6, L.codes, L.hseg);
+    if ((rc = launch_status()) != DMSH_OK) return rc;
+    const dim3 sgrid(ceil_div(M, bat::kSegChunk), sortk::kTopPasses);
+    launch("k_seg_offsets", bat::k_batch_seg_rows, sgrid, dim3(bat::kSegRowThreads), 0, s, L.bad,
+           M, mesh2tri, L.hseg, L.seg_off, L.seg_chunk);
+    if ((rc = launch_status()) != DMSH_OK) return rc;
+    if (sgrid.x > 1) {  // with one chunk the rows kernel finished the offsets
+      launch("k_seg_offsets", bat::k_batch_seg_cols, sgrid, dim3(256), 0, s, L.bad, M,
+             L.seg_off, (const uint32_t*)L.seg_chunk);
+      if ((rc = launch_status()) != DMSH_OK) return rc;
+    }
+    rc = sortk::run_seg_passes<uint64_t>(s, n, kb, bat::kMeshKeyShift, L.sort, L.seg_off, M,
+                                         bat::kMeshKeyShift, L.codes, L.idx, err,
+                                         g_batch_sort_items.load());
+  } else {
+    // one pass per digit of the composite key (the top digits are mostly the
+    // mesh index, so groups of equal top bits are too long for the finish);
+    // with an even number of passes the sort returns to the buffer it starts
+    // from, with an odd number it starts from the alternate key buffer
+    uint64_t* morton_out = (P % 2 == 0) ? L.codes : static_cast<uint64_t*>(L.sort.alt_keys);
+    launch("k_morton_hist", bat::k_batch_morton_hist,
+           dim3(std::min<uint32_t>(sortk::num_tiles<uint64_t>(n), 2048)), dim3(256), 0, s, L.bad,
+           n, M, mesh2tri, L.cntr, L.keys6, morton_out, P, kb, L.sort.hist);
+    if ((rc = launch_status()) != DMSH_OK) return rc;
+    rc = sortk::run_passes<uint64_t>(s, n, kb, L.sort, morton_out, L.codes, L.idx, err, false,
+                                     nullptr, nullptr, g_batch_sort_items.load());
+  }
   if (rc != DMSH_OK) return rc;
   const TreeletSink ts{kTreeletLeaves, L.treelet_count, L.treelets, n, L.root_count, L.roots};
   launch("k_topology", bat::k_batch_topology, dim3(ceil_div(n, 256)), dim3(256), 0, s, L.bad, n,

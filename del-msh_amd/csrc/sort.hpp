@@ -158,6 +158,18 @@ __global__ __launch_bounds__(256) void k_hist_all(const KeyT* __restrict__ keys,
   hist_flush(s_h, first, num_passes, hist);
 }
 
+// Scatter offsets of a segmented pass (keys = segment << shift | code, the
+// segments contiguous and in order in the input): an element lands at the
+// number of elements with its digit before it over all segments (what the
+// look-back counts) + off[segment * kRadix + digit].  Segment indices above
+// `last` (keys the producer left unwritten after a data error) read row
+// `last`; the positions stay in range because the offsets are then zero.
+struct SegOff {
+  const uint32_t* off;  // [last + 1][kRadix]
+  uint32_t shift;
+  uint32_t last;
+};
+
 // One digit pass.  IOTA: values are the element indices (first pass).  With
 // `n_dev`, the element
 // count is read from the device (the grid is sized for a capacity); blocks
@@ -165,12 +177,13 @@ __global__ __launch_bounds__(256) void k_hist_all(const KeyT* __restrict__ keys,
 // only earlier tiles are ever looked back at, so they wait on nobody.
 // THREADS: 256 (4 waves) or 512 (8 waves, for large inputs: half the tiles,
 // so half the same-address tickets and a look-back chain half as long).
-template <typename KeyT, int ITEMS, bool IOTA, int THREADS = kThreads>
+// SEG: a segmented pass (see SegOff); `hist` is not read.
+template <typename KeyT, int ITEMS, bool IOTA, int THREADS = kThreads, bool SEG = false>
 __global__ __launch_bounds__(THREADS) void k_onesweep(
     const KeyT* __restrict__ kin, const uint32_t* __restrict__ vin, KeyT* __restrict__ kout,
     uint32_t* __restrict__ vout, uint32_t n, const uint32_t* __restrict__ n_dev, uint32_t shift,
     uint32_t dbits, const uint32_t* __restrict__ hist, uint32_t* __restrict__ status,
-    uint32_t* __restrict__ ticket, uint32_t* __restrict__ err) {
+    uint32_t* __restrict__ ticket, uint32_t* __restrict__ err, SegOff seg) {
   static_assert(THREADS >= kRadix && THREADS % 64 == 0, "one thread per digit");
   if (n_dev) n = *n_dev;
   constexpr int kTile = THREADS * ITEMS;
@@ -238,7 +251,8 @@ __global__ __launch_bounds__(THREADS) void k_onesweep(
     __hip_atomic_store(my_status, (tile == 0 ? kFlagPrefix : kFlagAgg) | cnt, __ATOMIC_RELAXED,
                        __HIP_MEMORY_SCOPE_AGENT);
   const uint32_t excl = block_exclusive_scan(cnt, s_wsum);
-  const uint32_t goff = block_exclusive_scan(has_d ? hist[d] : 0u, s_wsum);
+  uint32_t goff = 0;
+  if constexpr (!SEG) goff = block_exclusive_scan(has_d ? hist[d] : 0u, s_wsum);
   // Decoupled look-back over the preceding tiles for digit d, kLookBack
   // status words per round trip: tiles that start together all publish their
   // counts at about the same time and then walk back over each other, so a
@@ -297,7 +311,13 @@ __global__ __launch_bounds__(THREADS) void k_onesweep(
   const uint32_t tile_n = (n - tile_base) < (uint32_t)kTile ? (n - tile_base) : (uint32_t)kTile;
   for (uint32_t j = tid; j < tile_n; j += THREADS) {
     const KeyT k = s_key[j];
-    const uint32_t g = s_gbase[digit_of(k, shift, mask)] + j;
+    const uint32_t dk = digit_of(k, shift, mask);
+    uint32_t g = s_gbase[dk] + j;
+    if constexpr (SEG) {
+      const KeyT sk = k >> seg.shift;
+      const uint32_t m = sk < (KeyT)seg.last ? (uint32_t)sk : seg.last;
+      g += seg.off[(size_t)m * kRadix + dk];
+    }
     if (g < n) {
       kout[g] = k;
       vout[g] = s_val[j];
@@ -448,7 +468,9 @@ __global__ __launch_bounds__(kGroupThreads) void k_sort_groups(
   uint32_t count = *group_count;
   if (count > group_cap) count = group_cap;
   const uint32_t tid = threadIdx.x, wave = tid >> 6, lane = tid & 63u;
-  const uint32_t low = shift / kRadixBits;  // digits below the group's common bits
+  // digits covering the bits below the group's common bits (a digit that
+  // straddles `shift` is constant above it inside the group)
+  const uint32_t low = (shift + kRadixBits - 1) / kRadixBits;
   for (uint32_t g = blockIdx.x; g < count; g += gridDim.x) {
     const uint32_t a = groups[g];
     if (a >= n) continue;  // uniform over the workgroup
@@ -621,14 +643,17 @@ inline int launch_fill(hipStream_t s, uint32_t* p, size_t count, uint32_t value)
 
 // One digit pass over (sk, sv) into (dk, dv); `slot` picks its status region
 // and ticket.  iota: the values are the element indices.
-template <typename KeyT, int ITEMS>
+// SEG: a segmented pass over the 8 bits at `shift` (digit then only names
+// the status region's histogram slot, which is not read).
+template <typename KeyT, int ITEMS, bool SEG = false>
 inline int launch_pass_items(hipStream_t s, uint32_t n, uint32_t key_bits, const Layout& L,
                              uint32_t digit, uint32_t slot, bool iota, const KeyT* sk,
                              const uint32_t* sv, KeyT* dk, uint32_t* dv, uint32_t* err,
-                             const uint32_t* n_dev) {
+                             const uint32_t* n_dev, SegOff seg = SegOff{nullptr, 0u, 0u},
+                             uint32_t seg_shift = 0u) {
   const uint32_t tiles = num_tiles_items(n, ITEMS);
-  const uint32_t shift = digit * kRadixBits;
-  const uint32_t dbits = pass_bits(key_bits, digit);
+  const uint32_t shift = SEG ? seg_shift : digit * kRadixBits;
+  const uint32_t dbits = SEG ? (uint32_t)kRadixBits : pass_bits(key_bits, digit);
   // each pass's status region is sized for 256-thread tiles
   uint32_t* st = L.status + (size_t)slot * tiles * kRadix;
   const int wide_opt = refit_option(kSortWideTiles);
@@ -636,20 +661,22 @@ inline int launch_pass_items(hipStream_t s, uint32_t n, uint32_t key_bits, const
   if (wide) {
     const uint32_t wt = ceil_div(n, (uint64_t)kWideThreads * ITEMS);
     if (iota) {
-      launch("k_onesweep", (k_onesweep<KeyT, ITEMS, true, kWideThreads>), dim3(wt),
+      launch("k_onesweep", (k_onesweep<KeyT, ITEMS, true, kWideThreads, SEG>), dim3(wt),
              dim3(kWideThreads), 0, s, sk, sv, dk, dv, n, n_dev, shift, dbits,
-             L.hist + digit * kRadix, st, L.tickets + slot, err);
+             L.hist + digit * kRadix, st, L.tickets + slot, err, seg);
     } else {
-      launch("k_onesweep", (k_onesweep<KeyT, ITEMS, false, kWideThreads>), dim3(wt),
+      launch("k_onesweep", (k_onesweep<KeyT, ITEMS, false, kWideThreads, SEG>), dim3(wt),
              dim3(kWideThreads), 0, s, sk, sv, dk, dv, n, n_dev, shift, dbits,
-             L.hist + digit * kRadix, st, L.tickets + slot, err);
+             L.hist + digit * kRadix, st, L.tickets + slot, err, seg);
     }
   } else if (iota) {
-    launch("k_onesweep", (k_onesweep<KeyT, ITEMS, true>), dim3(tiles), dim3(kThreads), 0, s, sk,
-           sv, dk, dv, n, n_dev, shift, dbits, L.hist + digit * kRadix, st, L.tickets + slot, err);
+    launch("k_onesweep", (k_onesweep<KeyT, ITEMS, true, kThreads, SEG>), dim3(tiles), dim3(kThreads), 0, s, sk,
+           sv, dk, dv, n, n_dev, shift, dbits, L.hist + digit * kRadix, st, L.tickets + slot, err,
+           seg);
   } else {
-    launch("k_onesweep", (k_onesweep<KeyT, ITEMS, false>), dim3(tiles), dim3(kThreads), 0, s, sk,
-           sv, dk, dv, n, n_dev, shift, dbits, L.hist + digit * kRadix, st, L.tickets + slot, err);
+    launch("k_onesweep", (k_onesweep<KeyT, ITEMS, false, kThreads, SEG>), dim3(tiles), dim3(kThreads), 0, s, sk,
+           sv, dk, dv, n, n_dev, shift, dbits, L.hist + digit * kRadix, st, L.tickets + slot, err,
+           seg);
   }
   return launch_status();
 }
@@ -735,6 +762,71 @@ inline int run_passes(hipStream_t s, uint32_t n, uint32_t key_bits, const Layout
          (const uint32_t*)group_count, (const uint32_t*)L.groups, cap);
   rc = launch_status();
   return rc;
+}
+
+// Segmented sort of composite keys (segment << seg_shift | code of
+// code_bits bits; the segments contiguous and in order, so a segment's
+// elements keep its range): three passes over the code's top 24 bits, from
+// lo = code_bits - 24 up, each scattering inside the segments through its
+// offsets (seg_off + p * seg_rows * kRadix, see SegOff), then the finish on
+// the groups of equal bits at and above lo (segment included) and
+// k_sort_groups.  The keys start in and end in `keys`, the element indices
+// end in `vals`.  L.tickets / L.status must be zero; L.hist is not read.
+template <typename KeyT, int ITEMS>
+inline int run_seg_passes_items(hipStream_t s, uint32_t n, uint32_t key_bits, uint32_t code_bits,
+                                const Layout& L, const uint32_t* seg_off, uint32_t seg_rows,
+                                uint32_t seg_shift, KeyT* keys, uint32_t* vals, uint32_t* err) {
+  if (code_bits < (uint32_t)(kTopPasses * kRadixBits) || seg_rows == 0 || !L.groups)
+    return DMSH_ERR_INVALID;
+  const uint32_t lo = code_bits - (uint32_t)(kTopPasses * kRadixBits);
+  KeyT* alt_k = static_cast<KeyT*>(L.alt_keys);
+  uint32_t* alt_v = L.alt_vals;
+  const size_t rows = (size_t)seg_rows * kRadix;
+  const SegOff s0{seg_off, seg_shift, seg_rows - 1}, s1{seg_off + rows, seg_shift, seg_rows - 1},
+      s2{seg_off + 2 * rows, seg_shift, seg_rows - 1};
+  int rc;
+  if ((rc = launch_pass_items<KeyT, ITEMS, true>(s, n, key_bits, L, 0, 0, true, keys, nullptr,
+                                                 alt_k, alt_v, err, nullptr, s0, lo)) != DMSH_OK)
+    return rc;
+  if ((rc = launch_pass_items<KeyT, ITEMS, true>(s, n, key_bits, L, 1, 1, false, alt_k, alt_v,
+                                                 keys, vals, err, nullptr, s1, lo + 8)) != DMSH_OK)
+    return rc;
+  if ((rc = launch_pass_items<KeyT, ITEMS, true>(s, n, key_bits, L, 2, 2, false, keys, vals,
+                                                 alt_k, alt_v, err, nullptr, s2, lo + 16)) !=
+      DMSH_OK)
+    return rc;
+  uint32_t* group_count = L.tickets + kTicketWords - 1;
+  const uint32_t cap = group_capacity(n);
+  const KeyT kmask = key_bits >= 8 * sizeof(KeyT) ? ~(KeyT)0 : (((KeyT)1 << key_bits) - 1);
+  launch("k_sort_finish", k_sort_finish<KeyT>,
+         dim3(ceil_div(n, (uint64_t)kFinishThreads * kFinishItems)), dim3(kFinishThreads), 0, s,
+         alt_k, alt_v, keys, vals, n, lo, kmask, group_count, L.groups, cap);
+  if ((rc = launch_status()) != DMSH_OK) return rc;
+  launch("k_sort_groups", k_sort_groups<KeyT>, dim3(std::min<uint32_t>(cap, kGroupGrid)),
+         dim3(kGroupThreads), 0, s, alt_k, alt_v, keys, vals, n, lo, kmask,
+         (const uint32_t*)group_count, (const uint32_t*)L.groups, cap);
+  return launch_status();
+}
+
+// items: elements per thread of the passes (8, 12 or 16).
+template <typename KeyT>
+inline int run_seg_passes(hipStream_t s, uint32_t n, uint32_t key_bits, uint32_t code_bits,
+                          const Layout& L, const uint32_t* seg_off, uint32_t seg_rows,
+                          uint32_t seg_shift, KeyT* keys, uint32_t* vals, uint32_t* err,
+                          int items) {
+  switch (items) {
+    case 8:
+      return run_seg_passes_items<KeyT, 8>(s, n, key_bits, code_bits, L, seg_off, seg_rows,
+                                           seg_shift, keys, vals, err);
+    case 12:
+      return run_seg_passes_items<KeyT, 12>(s, n, key_bits, code_bits, L, seg_off, seg_rows,
+                                            seg_shift, keys, vals, err);
+    case 16:
+      return run_seg_passes_items<KeyT, 16>(s, n, key_bits, code_bits, L, seg_off, seg_rows,
+                                            seg_shift, keys, vals, err);
+    default:
+      return DMSH_ERR_INVALID;
+  }
 }
 
 // Full argsort: keys_inout (sorted on return), idx_out (permutation).

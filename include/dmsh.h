@@ -421,7 +421,10 @@ int dmsh_antialias_bwd(void *stream, uint32_t num_cedge,
  * "batch.wgs_per_cu" (persistent ray-cast workgroups per CU, 1..64),
  * "batch.treelet_grid" (refit workgroups, 1..65536),
  * "batch.sort_items" (elements per thread of the digit passes: 8, 12, 16),
- * "batch.cast_waves" (waves per ray-cast workgroup: 1, 2, 4).
+ * "batch.cast_waves" (waves per ray-cast workgroup: 1, 2, 4),
+ * "batch.sort_seg" (1, the default: up to 8192 meshes, the composite keys
+ * are sorted inside their meshes by three passes over the code's top 24 bits
+ * and an LDS finish on the low 6; 0: one pass per digit of the whole key).
  * ------------------------------------------------------------------- */
 int dmsh_batch_set_tuning(const char *key, int value);
 int dmsh_batch_get_tuning(const char *key);

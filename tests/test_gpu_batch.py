@@ -101,6 +101,49 @@ def test_batch_build_and_cast_match_single_meshes(gpu, oracle):
     no_device_errors()
 
 
+def _duplicate_batch():
+    """Meshes whose codes repeat: 3000 copies of one triangle beside 600
+    spread ones (one group of equal top code bits longer than the finish's
+    1024-element margin), a mesh of copies only, and a plain torus."""
+    rng = np.random.default_rng(21)
+    base = np.array([[-.3, -.2, .1], [.4, -.1, 0], [0, .5, -.2]], np.float32)
+    spread_v = rng.uniform(-1, 1, (1800, 3)).astype(np.float32)
+    tri_a = np.concatenate([np.tile(np.array([[0, 1, 2]], np.uint32), (3000, 1)),
+                            3 + np.arange(1800, dtype=np.uint32).reshape(600, 3)])
+    tri_a = tri_a[rng.permutation(len(tri_a))]
+    vtx_a = np.concatenate([base, spread_v])
+    tri_b = np.tile(np.array([[0, 1, 2]], np.uint32), (1500, 1))
+    tri_c, vtx_c = primitives.torus_zup(0.7, 0.3, 30, 20)
+    tris, vtxs = [tri_a, tri_b, tri_c], [vtx_a, base.copy(), vtx_c]
+    csr = lambda c: np.concatenate([[0], np.cumsum(c)]).astype(np.uint32)
+    cam = np.diag([1.2, 1.2, 2.0, 1.0]).astype(np.float32)
+    return {
+        "tris": tris, "vtxs": vtxs,
+        "mesh2tri": csr([len(t) for t in tris]), "mesh2vtx": csr([len(v) for v in vtxs]),
+        "mesh2pix": csr([48 * 48] * 3),
+        "tri2vtx": np.concatenate(tris), "vtx2xyz": np.concatenate(vtxs),
+        "mesh2width": np.array([48] * 3, np.uint32), "ndc2world": np.stack([cam] * 3),
+    }
+
+
+@pytest.mark.parametrize("seg", [1, 0])
+def test_batch_duplicate_codes(gpu, oracle, seg):
+    """Long runs of equal codes: in the segmented sort they are sorted by
+    k_sort_groups on the low code bits; both sorts give the oracle's trees."""
+    L = _lib.lib()
+    old = L.dmsh_batch_get_tuning(b"batch.sort_seg")
+    try:
+        assert L.dmsh_batch_set_tuning(b"batch.sort_seg", seg) == 0
+        b = _duplicate_batch()
+        p = _pipeline(b, gpu)
+        p.run()
+        torch.cuda.synchronize()
+        _check_against_oracle(oracle, p, b, range(3))
+    finally:
+        L.dmsh_batch_set_tuning(b"batch.sort_seg", old)
+    no_device_errors()
+
+
 def test_batch_results_do_not_depend_on_tuning(gpu):
     b = _edge_batch()
     p = _pipeline(b, gpu)
@@ -109,17 +152,20 @@ def test_batch_results_do_not_depend_on_tuning(gpu):
     L = _lib.lib()
     saved = {k: L.dmsh_batch_get_tuning(k) for k in (b"batch.stack", b"batch.wgs_per_cu",
                                                       b"batch.treelet_grid", b"batch.sort_items",
-                                                      b"batch.cast_waves")}
+                                                      b"batch.cast_waves", b"batch.sort_seg")}
     try:
-        # 19 meshes: 35-bit composite keys, five digit passes, each with its
-        # own status region at every tile size
-        for stack, wgs, tg, items, waves in ((16, 1, 3, 8, 2), (32, 64, 4096, 12, 4),
-                                              (24, 3, 1024, 16, 1)):
+        # 19 meshes: 35-bit composite keys, sorted by three segmented passes
+        # and the finish, or by five digit passes, each with its own status
+        # region at every tile size
+        for stack, wgs, tg, items, waves, seg in ((16, 1, 3, 8, 2, 0), (32, 64, 4096, 12, 4, 0),
+                                                  (24, 3, 1024, 16, 1, 0), (16, 2, 7, 8, 2, 1),
+                                                  (20, 4, 512, 12, 1, 1)):
             assert L.dmsh_batch_set_tuning(b"batch.stack", stack) == 0
             assert L.dmsh_batch_set_tuning(b"batch.wgs_per_cu", wgs) == 0
             assert L.dmsh_batch_set_tuning(b"batch.treelet_grid", tg) == 0
             assert L.dmsh_batch_set_tuning(b"batch.sort_items", items) == 0
             assert L.dmsh_batch_set_tuning(b"batch.cast_waves", waves) == 0
+            assert L.dmsh_batch_set_tuning(b"batch.sort_seg", seg) == 0
             p.run()
             got = (H(p.bvhnodes), H(p.bvhnode2aabb), H(p.pix2tri), H(p.pix2t))
             assert np.array_equal(got[0], ref[0]) and bits_equal(got[1], ref[1])

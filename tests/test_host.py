@@ -130,6 +130,7 @@ def test_tuning_keys_accept_and_reject_without_a_gpu():
     L = _lib.lib()
     batch = {b"batch.sort_items": ((8, 12, 16), (0, 4, 10, 32)),
              b"batch.cast_waves": ((1, 2, 4), (0, 3, 8)),
+             b"batch.sort_seg": ((0, 1), (2, -1)),
              b"batch.stack": ((16, 20, 24, 32), (8, 17)),
              b"batch.treelet_grid": ((1, 1024), (0, 70000))}
     for key, (good, bad) in batch.items():
