@@ -457,13 +457,21 @@ __device__ __forceinline__ TreeView tree_of(const BatchTrees& b, uint32_t m) {
                   b.xyz + (size_t)vb * 3};
 }
 
-template <int K, int T>
-__global__ __launch_bounds__(T) void k_batch_refit_treelet(BatchTrees b, uint32_t total_tri,
-                                                           const uint32_t* __restrict__ count_p,
-                                                           const uint4* __restrict__ treelets,
-                                                           uint32_t* __restrict__ err) {
+// CLIMB: the treelet climb variant (refit_treelet), 2 by default as for a
+// single mesh: climbs start at the nodes whose children are both leaves.
+template <int K, int T, int CLIMB>
+__global__ __launch_bounds__(T) __attribute__((amdgpu_num_sgpr(72))) void k_batch_refit_treelet(
+    BatchTrees b, uint32_t total_tri, const uint32_t* __restrict__ count_p,
+    const uint4* __restrict__ treelets, uint32_t* __restrict__ err) {
   __shared__ TreeletLds<K> s;
   if (*b.bad) return;
+  if (CLIMB == 2) {
+    if (threadIdx.x == 0) {
+      s.queued = 0u;
+      s.queued2 = 0u;
+    }
+    __syncthreads();
+  }
   uint32_t count = *count_p;
   if (count > total_tri) count = total_tri;
   for (uint32_t t = blockIdx.x; t < count; t += gridDim.x) {
@@ -472,7 +480,8 @@ __global__ __launch_bounds__(T) void k_batch_refit_treelet(BatchTrees b, uint32_
       if (threadIdx.x == 0) raise_error(err, DMSH_DEVERR_REFIT);
       continue;
     }
-    refit_treelet<kLeafTri, K, T>(tree_of(b, tl.w), tl.x, tl.y, tl.z, 0.f, s, err);
+    refit_treelet<kLeafTri, K, T, false, CLIMB>(tree_of(b, tl.w), tl.x, tl.y, tl.z, 0.f, s, err,
+                                                nullptr, 0);
   }
 }
 

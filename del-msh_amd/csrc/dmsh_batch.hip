@@ -317,8 +317,13 @@ int dmsh_batch_build_trimesh3(void* stream, uint32_t num_mesh, uint32_t total_tr
   const bat::BatchTrees bt{L.bad,    M,        mesh2tri, mesh2vtx, mesh2node,
                            bvhnode2aabb, bvhnodes, tri2vtx, vtx2xyz};
   const uint32_t tgrid = std::min<uint32_t>((uint32_t)g_batch_treelet_grid.load(), n);
-  launch("k_refit_treelet", bat::k_batch_refit_treelet<(int)kTreeletLeaves, kTreeletThreads>,
-         dim3(tgrid), dim3(kTreeletThreads), 0, s, bt, n, L.treelet_count, L.treelets, err);
+  if (refit_option(kRefitClimb) == 2) {
+    launch("k_refit_treelet", bat::k_batch_refit_treelet<(int)kTreeletLeaves, kTreeletThreads, 2>,
+           dim3(tgrid), dim3(kTreeletThreads), 0, s, bt, n, L.treelet_count, L.treelets, err);
+  } else {
+    launch("k_refit_treelet", bat::k_batch_refit_treelet<(int)kTreeletLeaves, kTreeletThreads, 0>,
+           dim3(tgrid), dim3(kTreeletThreads), 0, s, bt, n, L.treelet_count, L.treelets, err);
+  }
   if ((rc = launch_status()) != DMSH_OK) return rc;
   launch("k_refit_top", bat::k_batch_refit_top<(int)kTreeletLeaves, 256>,
          dim3(std::min<uint32_t>(M, 1024)), dim3(256), 0, s, bt, L.inner_off, L.flags,

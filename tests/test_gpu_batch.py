@@ -150,6 +150,7 @@ def test_batch_results_do_not_depend_on_tuning(gpu):
     p.run()
     ref = (H(p.bvhnodes).copy(), H(p.bvhnode2aabb).copy(), H(p.pix2tri).copy(), H(p.pix2t).copy())
     L = _lib.lib()
+    climb = L.dmsh_get_tuning(b"refit.climb")
     saved = {k: L.dmsh_batch_get_tuning(k) for k in (b"batch.stack", b"batch.wgs_per_cu",
                                                       b"batch.treelet_grid", b"batch.sort_items",
                                                       b"batch.cast_waves", b"batch.sort_seg")}
@@ -166,6 +167,8 @@ def test_batch_results_do_not_depend_on_tuning(gpu):
             assert L.dmsh_batch_set_tuning(b"batch.sort_items", items) == 0
             assert L.dmsh_batch_set_tuning(b"batch.cast_waves", waves) == 0
             assert L.dmsh_batch_set_tuning(b"batch.sort_seg", seg) == 0
+            # the treelet climb: leaf-pair starts (2) or every leaf climbing (0)
+            assert L.dmsh_set_tuning(b"refit.climb", 0 if items == 12 else 2) == 0
             p.run()
             got = (H(p.bvhnodes), H(p.bvhnode2aabb), H(p.pix2tri), H(p.pix2t))
             assert np.array_equal(got[0], ref[0]) and bits_equal(got[1], ref[1])
@@ -173,6 +176,7 @@ def test_batch_results_do_not_depend_on_tuning(gpu):
     finally:
         for k, v in saved.items():
             L.dmsh_batch_set_tuning(k, v)
+        L.dmsh_set_tuning(b"refit.climb", climb)
     no_device_errors()
 
 
