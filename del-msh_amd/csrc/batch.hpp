@@ -412,18 +412,53 @@ static __global__ __launch_bounds__(256) void k_batch_seg_cols(
     if (i < cnt) O[(size_t)(m0 + i) * kR + d] = o[i] - col;
 }
 
-// Internal node g of all meshes (g < inner_off[M]).
-static __global__ __launch_bounds__(256) void k_batch_topology(
+// The codes of one mesh (index 0 = the mesh's first) through a window over
+// all meshes' codes.
+template <typename KeyT>
+struct MeshCodes {
+  CodeWindow<KeyT> w;
+  uint32_t tb;
+  __device__ MeshCodes(const CodeWindow<KeyT>& w_, uint32_t tb_) : w(w_), tb(tb_) {}
+  __device__ __forceinline__ KeyT operator[](uint32_t j) const { return w[tb + j]; }
+};
+
+// Internal node g of all meshes (g < inner_off[M]), a thread each.  Node g
+// of mesh m searches the codes around mesh2tri[m] + (g - inner_off[m]), so a
+// block's nodes search one nearly contiguous run of codes: that run and
+// kTopoHalo codes either side sit in LDS (kWin codes at most; any other
+// code is read from memory), as in k_topology.
+static __global__ __launch_bounds__(kTopoThreads) void k_batch_topology(
     const uint32_t* __restrict__ bad, uint32_t total_tri, uint32_t M,
     const uint32_t* __restrict__ mesh2tri, const uint32_t* __restrict__ inner_off,
     const uint32_t* __restrict__ node_off, uint32_t* __restrict__ bvhnodes,
     const uint64_t* __restrict__ codes, const uint32_t* __restrict__ idx, TreeletSink ts) {
-  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-  if (*bad || g >= total_tri || g >= inner_off[M]) return;
+  constexpr uint32_t kWin = 2 * kTopoHalo + 2 * kTopoThreads + 1;
+  __shared__ uint64_t s_c[kWin];
+  if (*bad) return;
+  const uint32_t total_inner = inner_off[M];
+  const uint32_t base = blockIdx.x * kTopoThreads;
+  if (base >= total_inner || base >= total_tri) return;  // uniform over the block
+  const uint32_t last_g = (total_inner - base < kTopoThreads ? total_inner : base + kTopoThreads) - 1u;
+  const uint32_t m_first = find_segment(inner_off, M, base);
+  const uint32_t m_last = find_segment(inner_off, M, last_g);
+  const uint32_t a_lo = base + mesh2tri[m_first] - inner_off[m_first];
+  const uint32_t a_hi = last_g + mesh2tri[m_last] - inner_off[m_last];
+  const uint32_t lo = a_lo > kTopoHalo ? a_lo - kTopoHalo : 0u;
+  uint32_t hi = total_tri - a_hi > kTopoHalo + 2u ? a_hi + kTopoHalo + 2u : total_tri;
+  if (hi - lo > kWin) hi = lo + kWin;
+  for (uint32_t j = lo + threadIdx.x; j < hi; j += kTopoThreads) s_c[j - lo] = codes[j];
+  __syncthreads();
+  const uint32_t g = base + threadIdx.x;
+  if (g > last_g) return;
   const uint32_t m = find_segment(inner_off, M, g);
   const uint32_t tb = mesh2tri[m], n = mesh2tri[m + 1] - tb;
-  topology_node<uint64_t>(n, g - inner_off[m], bvhnodes + (size_t)node_off[m] * 3, codes + tb,
-                          idx + tb, tb, ts, m, tb);
+  const MeshCodes<uint64_t> c(CodeWindow<uint64_t>(codes, s_c, lo, hi - lo), tb);
+  const uint32_t i = g - inner_off[m];
+  uint32_t first, last;
+  node_range<uint64_t>(c, n, i, first, last);
+  const uint32_t split = node_split<uint64_t>(c, first, last);
+  topology_emit(n, i, first, last, split, bvhnodes + (size_t)node_off[m] * 3, idx + tb, tb, ts, m,
+                tb);
 }
 
 // Meshes of one triangle: the single leaf row and its treelet.

@@ -308,7 +308,8 @@ int dmsh_batch_build_trimesh3(void* stream, uint32_t num_mesh, uint32_t total_tr
   }
   if (rc != DMSH_OK) return rc;
   const TreeletSink ts{kTreeletLeaves, L.treelet_count, L.treelets, n, L.root_count, L.roots};
-  launch("k_topology", bat::k_batch_topology, dim3(ceil_div(n, 256)), dim3(256), 0, s, L.bad, n,
+  launch("k_topology", bat::k_batch_topology, dim3(ceil_div(n, kTopoThreads)), dim3(kTopoThreads),
+         0, s, L.bad, n,
          M, mesh2tri, L.inner_off, mesh2node, bvhnodes, L.codes, L.idx, ts);
   if ((rc = launch_status()) != DMSH_OK) return rc;
   launch("k_batch_singles", bat::k_batch_singles, dim3(ceil_div(M, 256)), dim3(256), 0, s, L.bad,
