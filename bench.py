@@ -656,8 +656,9 @@ def run_served(num_req: int, rates, budget_bytes: int):
         host = np.array(srv.host_s) * 1e3
         s_stats.update({"steps": steps, "cache_hits": srv.cache.hits,
                         "cache_misses": srv.cache.misses, "evictions": srv.cache.evicted,
+                        # build with the pack into the traversal layout fused
+                        # into the refit (dmsh_batch_build_pack_trimesh3)
                         "build": Serve.build_stats(srv.builds),
-                        "build_without_pack": Serve.build_stats(srv.builds_only),
                         # share of the span the GPU stream was working on steps
                         # (near 1: the GPU, not the host loop, sets the pace)
                         "gpu_busy_share": round(Serve.gpu_busy_s(srv.step_gpu)
@@ -677,8 +678,6 @@ def run_served(num_req: int, rates, budget_bytes: int):
         if s_stats["build"].get("mtri_per_s") and p_stats["build"].get("mtri_per_s"):
             ratio["build_mtri_per_s"] = round(s_stats["build"]["mtri_per_s"]
                                               / p_stats["build"]["mtri_per_s"], 2)
-            ratio["build_mtri_per_s_without_pack"] = round(
-                s_stats["build_without_pack"]["mtri_per_s"] / p_stats["build"]["mtri_per_s"], 2)
         progress(f"served rate {rate:g}: {ratio}")
         out["rates"][str(rate)] = {"served": s_stats, "plain": p_stats, "served_over_plain": ratio,
                                    "distinct_meshes": len({r.mesh_id for r in reqs}),

@@ -492,12 +492,19 @@ __device__ __forceinline__ TreeView tree_of(const BatchTrees& b, uint32_t m) {
                   b.xyz + (size_t)vb * 3};
 }
 
+// Per-mesh pack sinks of the batched refit (see NoPack): a source's
+// of(m, n) is mesh m's sink; NoPackSrc packs nothing.
+struct NoPackSrc {
+  using PK = NoPack;
+  __device__ __forceinline__ NoPack of(uint32_t, uint32_t) const { return NoPack{}; }
+};
+
 // CLIMB: the treelet climb variant (refit_treelet), 2 by default as for a
 // single mesh: climbs start at the nodes whose children are both leaves.
-template <int K, int T, int CLIMB>
+template <int K, int T, int CLIMB, typename Src = NoPackSrc>
 __global__ __launch_bounds__(T) __attribute__((amdgpu_num_sgpr(72))) void k_batch_refit_treelet(
     BatchTrees b, uint32_t total_tri, const uint32_t* __restrict__ count_p,
-    const uint4* __restrict__ treelets, uint32_t* __restrict__ err) {
+    const uint4* __restrict__ treelets, uint32_t* __restrict__ err, Src src) {
   __shared__ TreeletLds<K> s;
   if (*b.bad) return;
   if (CLIMB == 2) {
@@ -515,26 +522,35 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_num_sgpr(72))) void k_batc
       if (threadIdx.x == 0) raise_error(err, DMSH_DEVERR_REFIT);
       continue;
     }
-    refit_treelet<kLeafTri, K, T, false, CLIMB>(tree_of(b, tl.w), tl.x, tl.y, tl.z, 0.f, s, err,
-                                                nullptr, 0);
+    const TreeView v = tree_of(b, tl.w);
+    refit_treelet<kLeafTri, K, T, false, CLIMB, typename Src::PK>(
+        v, tl.x, tl.y, tl.z, 0.f, s, err, nullptr, 0, false, TopStage{}, DMSH_U32MAX,
+        src.of(tl.w, v.num_leaf));
   }
 }
 
 // One workgroup per mesh with more than K triangles (grid-stride over meshes).
-template <int K, int T>
+template <int K, int T, typename Src = NoPackSrc>
 __global__ __launch_bounds__(T) void k_batch_refit_top(BatchTrees b,
                                                        const uint32_t* __restrict__ inner_off,
                                                        uint32_t* __restrict__ flags,
                                                        const uint32_t* __restrict__ root_count,
                                                        const uint32_t* __restrict__ roots,
-                                                       uint32_t* __restrict__ err) {
+                                                       uint32_t* __restrict__ err, Src src) {
+  constexpr uint32_t kList = Src::PK::kOn ? 4096u : 1u;  // merged nodes listed for the pack
+  __shared__ uint32_t s_list[kList];
+  __shared__ uint32_t s_count;
   if (*b.bad) return;
+  if (threadIdx.x == 0) s_count = 0u;
+  __syncthreads();
   for (uint32_t m = blockIdx.x; m < b.M; m += gridDim.x) {
     const uint32_t n = seg_len(b.mesh2tri, m);
     if (n <= (uint32_t)K) continue;
     uint32_t count = root_count[m];
     if (count > n) count = n;
-    refit_top<T>(tree_of(b, m), flags + inner_off[m], roots + b.mesh2tri[m], 1u, count, err);
+    refit_top<T, typename Src::PK>(tree_of(b, m), flags + inner_off[m], roots + b.mesh2tri[m], 1u,
+                                   count, err, src.of(m, n), s_list, &s_count,
+                                   Src::PK::kOn ? kList : 0u);
   }
 }
 

@@ -243,11 +243,41 @@ size_t dmsh_batch_build_workspace_bytes(uint32_t num_mesh, uint32_t total_tri) {
   return a.off + 256;
 }
 
-int dmsh_batch_build_trimesh3(void* stream, uint32_t num_mesh, uint32_t total_tri,
-                              uint32_t total_vtx, const uint32_t* mesh2tri,
-                              const uint32_t* mesh2vtx, const uint32_t* tri2vtx,
-                              const float* vtx2xyz, uint32_t* mesh2node, uint32_t* bvhnodes,
-                              float* bvhnode2aabb, void* ws, size_t ws_bytes) {
+extern "C++" {
+namespace {
+
+// The treelet and top refits of a batched build; Src: where the nodes'
+// traversal records go as they are fitted (bat::NoPackSrc: nowhere).
+template <typename Src>
+int launch_batch_refit(hipStream_t s, const bat::BatchTrees& bt, const BatchBuildLayout& L,
+                       uint32_t M, uint32_t n, uint32_t* err, Src src) {
+  const uint32_t tgrid = std::min<uint32_t>((uint32_t)g_batch_treelet_grid.load(), n);
+  if (refit_option(kRefitClimb) == 2) {
+    launch("k_refit_treelet",
+           (bat::k_batch_refit_treelet<(int)kTreeletLeaves, kTreeletThreads, 2, Src>),
+           dim3(tgrid), dim3(kTreeletThreads), 0, s, bt, n, L.treelet_count, L.treelets, err,
+           src);
+  } else {
+    launch("k_refit_treelet",
+           (bat::k_batch_refit_treelet<(int)kTreeletLeaves, kTreeletThreads, 0, Src>),
+           dim3(tgrid), dim3(kTreeletThreads), 0, s, bt, n, L.treelet_count, L.treelets, err,
+           src);
+  }
+  int rc = launch_status();
+  if (rc != DMSH_OK) return rc;
+  launch("k_refit_top", (bat::k_batch_refit_top<(int)kTreeletLeaves, 256, Src>),
+         dim3(std::min<uint32_t>(M, 1024)), dim3(256), 0, s, bt, L.inner_off, L.flags,
+         L.root_count, L.roots, err, src);
+  return launch_status();
+}
+
+// The batched build; with `packed` (and `nonfinite`) the refit also writes
+// every mesh's packed traversal buffer, as dmsh_batch_pack would.
+int batch_build(void* stream, uint32_t num_mesh, uint32_t total_tri, uint32_t total_vtx,
+                const uint32_t* mesh2tri, const uint32_t* mesh2vtx, const uint32_t* tri2vtx,
+                const float* vtx2xyz, uint32_t* mesh2node, uint32_t* bvhnodes,
+                float* bvhnode2aabb, void* ws, size_t ws_bytes, const uint64_t* packed,
+                uint32_t* nonfinite) {
   if (num_mesh == 0 || !mesh2tri || !mesh2vtx || !mesh2node) return DMSH_ERR_INVALID;
   if (total_tri >= (1u << 30)) return DMSH_ERR_INVALID;
   if (total_tri > 0 && (!tri2vtx || !vtx2xyz || !bvhnodes || !bvhnode2aabb || total_vtx == 0))
@@ -264,6 +294,7 @@ int dmsh_batch_build_trimesh3(void* stream, uint32_t num_mesh, uint32_t total_tr
   const uint32_t kb = batch_key_bits(M);
   const uint32_t P = sortk::num_passes(kb);
   int rc;
+  if (packed && (rc = sortk::launch_fill(s, nonfinite, M, 0u)) != DMSH_OK) return rc;
   launch("k_batch_init", bat::k_batch_init, dim3(std::min<uint32_t>(ceil_div(L.zero_words, 256), 2048)),
          dim3(256), 0, s, M, L.keys6, L.zero_begin, L.zero_words);
   if ((rc = launch_status()) != DMSH_OK) return rc;
@@ -317,19 +348,37 @@ int dmsh_batch_build_trimesh3(void* stream, uint32_t num_mesh, uint32_t total_tr
   if ((rc = launch_status()) != DMSH_OK) return rc;
   const bat::BatchTrees bt{L.bad,    M,        mesh2tri, mesh2vtx, mesh2node,
                            bvhnode2aabb, bvhnodes, tri2vtx, vtx2xyz};
-  const uint32_t tgrid = std::min<uint32_t>((uint32_t)g_batch_treelet_grid.load(), n);
-  if (refit_option(kRefitClimb) == 2) {
-    launch("k_refit_treelet", bat::k_batch_refit_treelet<(int)kTreeletLeaves, kTreeletThreads, 2>,
-           dim3(tgrid), dim3(kTreeletThreads), 0, s, bt, n, L.treelet_count, L.treelets, err);
-  } else {
-    launch("k_refit_treelet", bat::k_batch_refit_treelet<(int)kTreeletLeaves, kTreeletThreads, 0>,
-           dim3(tgrid), dim3(kTreeletThreads), 0, s, bt, n, L.treelet_count, L.treelets, err);
-  }
-  if ((rc = launch_status()) != DMSH_OK) return rc;
-  launch("k_refit_top", bat::k_batch_refit_top<(int)kTreeletLeaves, 256>,
-         dim3(std::min<uint32_t>(M, 1024)), dim3(256), 0, s, bt, L.inner_off, L.flags,
-         L.root_count, L.roots, err);
+  if (!packed) return launch_batch_refit(s, bt, L, M, n, err, bat::NoPackSrc{});
+  if ((rc = launch_batch_refit(s, bt, L, M, n, err, srv::PackSrc{packed, nonfinite})) != DMSH_OK)
+    return rc;
+  launch("k_pack_roots", srv::k_pack_roots, dim3(ceil_div(M, 256)), dim3(256), 0, s, M, mesh2tri,
+         (const uint32_t*)mesh2node, (const float*)bvhnode2aabb, packed,
+         (const uint32_t*)nonfinite, (const uint32_t*)L.bad);
   return launch_status();
+}
+
+}  // namespace
+}  // extern "C++"
+
+int dmsh_batch_build_trimesh3(void* stream, uint32_t num_mesh, uint32_t total_tri,
+                              uint32_t total_vtx, const uint32_t* mesh2tri,
+                              const uint32_t* mesh2vtx, const uint32_t* tri2vtx,
+                              const float* vtx2xyz, uint32_t* mesh2node, uint32_t* bvhnodes,
+                              float* bvhnode2aabb, void* ws, size_t ws_bytes) {
+  return batch_build(stream, num_mesh, total_tri, total_vtx, mesh2tri, mesh2vtx, tri2vtx, vtx2xyz,
+                     mesh2node, bvhnodes, bvhnode2aabb, ws, ws_bytes, nullptr, nullptr);
+}
+
+int dmsh_batch_build_pack_trimesh3(void* stream, uint32_t num_mesh, uint32_t total_tri,
+                                   uint32_t total_vtx, const uint32_t* mesh2tri,
+                                   const uint32_t* mesh2vtx, const uint32_t* tri2vtx,
+                                   const float* vtx2xyz, uint32_t* mesh2node, uint32_t* bvhnodes,
+                                   float* bvhnode2aabb, const uint64_t* packed,
+                                   uint32_t* nonfinite, void* ws, size_t ws_bytes) {
+  if (!packed || !nonfinite) return DMSH_ERR_INVALID;
+  if (total_tri >= rc::kMaxTri) return DMSH_ERR_INVALID;
+  return batch_build(stream, num_mesh, total_tri, total_vtx, mesh2tri, mesh2vtx, tri2vtx, vtx2xyz,
+                     mesh2node, bvhnodes, bvhnode2aabb, ws, ws_bytes, packed, nonfinite);
 }
 
 size_t dmsh_batch_pix2tri_workspace_bytes(uint32_t num_mesh, uint32_t total_tri,

@@ -87,14 +87,14 @@ __device__ __forceinline__ uint32_t child_ref(uint32_t c, uint32_t num_leaf, boo
   return c >= nb ? (kLeafBit | (c - nb)) : c;
 }
 
-// Internal node i (i + 1 < num_leaf) of one tree into inner[i].
-__device__ __forceinline__ void pack_inner(InnerNode* __restrict__ inner,
-                                           const uint32_t* __restrict__ bvhnodes,
-                                           const float* __restrict__ aabb, uint32_t num_leaf,
-                                           uint32_t i, uint32_t* __restrict__ nonfinite,
-                                           uint32_t* __restrict__ err) {
-  const uint32_t* row = bvhnodes + (size_t)i * 3;
-  const uint32_t c0 = row[1], c1 = row[2];
+// The record of an internal node with children (c0, c1) whose boxes are b0,
+// b1 (six floats each, read only when the children fit the layout).
+template <typename Box6>
+__device__ __forceinline__ void store_inner(InnerNode* __restrict__ inner, uint32_t i,
+                                            uint32_t c0, uint32_t c1, const Box6& b0,
+                                            const Box6& b1, uint32_t num_leaf,
+                                            uint32_t* __restrict__ nonfinite,
+                                            uint32_t* __restrict__ err) {
   bool bad = (c1 == DMSH_U32MAX);
   const uint32_t r0 = child_ref(c0, num_leaf, bad);
   const uint32_t r1 = child_ref(c1, num_leaf, bad);
@@ -107,8 +107,6 @@ __device__ __forceinline__ void pack_inner(InnerNode* __restrict__ inner,
     nd.c = make_float4(1.f, 0.f, 0.f, 0.f);
     nd.d = make_uint4(kLeafBit, kLeafBit, 0u, 0u);
   } else {
-    const float* b0 = aabb + (size_t)c0 * 6;
-    const float* b1 = aabb + (size_t)c1 * 6;
     nd.a = make_float4(b0[0], b0[1], b0[2], b0[3]);
     nd.b = make_float4(b0[4], b0[5], b1[0], b1[1]);
     nd.c = make_float4(b1[2], b1[3], b1[4], b1[5]);
@@ -120,6 +118,21 @@ __device__ __forceinline__ void pack_inner(InnerNode* __restrict__ inner,
     if (!fin) atomicOr(nonfinite, 1u);
   }
   inner[i] = nd;
+}
+
+// Internal node i (i + 1 < num_leaf) of one tree into inner[i].
+__device__ __forceinline__ void pack_inner(InnerNode* __restrict__ inner,
+                                           const uint32_t* __restrict__ bvhnodes,
+                                           const float* __restrict__ aabb, uint32_t num_leaf,
+                                           uint32_t i, uint32_t* __restrict__ nonfinite,
+                                           uint32_t* __restrict__ err) {
+  const uint32_t* row = bvhnodes + (size_t)i * 3;
+  const uint32_t c0 = row[1], c1 = row[2];
+  const uint32_t num_node = 2 * num_leaf - 1;
+  // the box pointers are formed only for children inside the tree
+  const float* b0 = aabb + (size_t)(c0 < num_node ? c0 : 0u) * 6;
+  const float* b1 = aabb + (size_t)(c1 < num_node ? c1 : 0u) * 6;
+  store_inner(inner, i, c0, c1, b0, b1, num_leaf, nonfinite, err);
 }
 
 // Leaf k of one tree into leaf[k].

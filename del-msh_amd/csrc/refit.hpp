@@ -144,6 +144,16 @@ struct TreeView {
   uint32_t num_idx = 0;
 };
 
+// Where a refit also writes the traversal records of the nodes it fits
+// (batched builds for the serving cache): NoPack writes none.  A sink has
+// leaf_ok(k, p0, p1, p2, tri) for triangle leaf k with its corners,
+// leaf_bad(k) for a leaf whose indices do not fit, and inner_at(i, c0, c1,
+// b0, b1, num_leaf, err) for internal node i with children c0, c1 and their
+// boxes.
+struct NoPack {
+  static constexpr bool kOn = false;
+};
+
 // Leaf box of leaf k and its parent; returns false (and raises) on a bad
 // index.  Points: the point.  Triangles: the box of the three corners grown
 // by eps (vtx2xyz.rs:39-61), or, with tri_box (the triangle boxes the
@@ -189,6 +199,43 @@ __device__ __forceinline__ bool leaf_box(const TreeView& v, uint32_t k, float ep
   } else {
     box_set(box, v.xyz + (size_t)obj * 3, 0.f);
   }
+  parent = p;
+  return true;
+}
+
+// leaf_box of a triangle leaf (no tri_box) that also hands the leaf's
+// traversal record to the pack sink from the same loads: the three corners,
+// or a degenerate record after an index that does not fit.
+template <typename PK>
+__device__ __forceinline__ bool leaf_tri_pack(const TreeView& v, uint32_t k, float eps, Box& box,
+                                              uint32_t& parent, const PK& pk, uint32_t* err) {
+  const uint32_t* row = v.bvhnodes + (size_t)(v.num_leaf - 1 + k) * 3;
+  const uint32_t p = row[0], obj = row[1], tag = row[2];
+  bool ok = tag == DMSH_U32MAX && obj < v.num_leaf;
+  uint32_t i0 = 0, i1 = 0, i2 = 0;
+  if (ok) {
+    const uint32_t* tv = v.tri2vtx + (size_t)obj * 3;
+    i0 = tv[0];
+    i1 = tv[1];
+    i2 = tv[2];
+    ok = i0 < v.num_vtx && i1 < v.num_vtx && i2 < v.num_vtx;
+  }
+  if (!ok) {
+    raise_error(err, DMSH_DEVERR_INDEX);
+    pk.leaf_bad(k);
+    return false;
+  }
+  float c[3][3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    c[0][d] = v.xyz[(size_t)i0 * 3 + d];
+    c[1][d] = v.xyz[(size_t)i1 * 3 + d];
+    c[2][d] = v.xyz[(size_t)i2 * 3 + d];
+  }
+  box_set(box, c[0], eps);
+  box_add(box, c[1], eps);
+  box_add(box, c[2], eps);
+  pk.leaf_ok(k, c[0], c[1], c[2], obj);
   parent = p;
   return true;
 }
@@ -309,14 +356,17 @@ __device__ __forceinline__ void treelet_prefetch(const TreeView& v, uint32_t f, 
 // that update has; the counter RMW is ordered with the wave's own LDS
 // accesses at wavefront scope (no waits), which keeps the compiler from
 // moving the box store after it or the box load before it.
-template <int LK, int K, int T, bool PRE = false, int CLIMB = 0>
+// PK: a pack sink (see NoPack): leaf records in phase 1, internal records
+// (children's boxes from LDS) in phase 3.
+template <int LK, int K, int T, bool PRE = false, int CLIMB = 0, typename PK = NoPack>
 __device__ __forceinline__ void refit_treelet(const TreeView& v, uint32_t root, uint32_t f,
                                               uint32_t l, float eps, TreeletLds<K>& s,
                                               uint32_t* __restrict__ err,
                                               const TreeletIn<K / T>* pre = nullptr,
                                               int compact = 4, bool staged = false,
                                               TopStage st = TopStage{},
-                                              uint32_t stage_pos = DMSH_U32MAX) {
+                                              uint32_t stage_pos = DMSH_U32MAX,
+                                              const PK& pk = PK{}) {
   constexpr int LPT = K / T;
   const uint32_t num_leaf = v.num_leaf;
   const uint32_t nb = num_leaf - 1;
@@ -368,9 +418,18 @@ __device__ __forceinline__ void refit_treelet(const TreeView& v, uint32_t root, 
     }
     Box b;
     uint32_t p;
-    if (leaf_box<LK>(v, id, eps, b, p, err)) {
+    bool fit;
+    if constexpr (PK::kOn) {
+      static_assert(LK == kLeafTri, "traversal records are made for triangle leaves");
+      fit = leaf_tri_pack(v, id, eps, b, p, pk, err);
+    } else {
+      fit = leaf_box<LK>(v, id, eps, b, p, err);
+    }
+    if (fit) {
       s.leaf[j] = b;
-      box_store(v.aabb, nb + id, b);
+      // with a pack sink only the treelet root's box is stored (for the top
+      // merge); the traversal records hold every other box
+      if (!PK::kOn || nb + id == root) box_store(v.aabb, nb + id, b);
       if (nb + id != root) lpar[r] = p;
     }
   }
@@ -583,7 +642,19 @@ __device__ __forceinline__ void refit_treelet(const TreeView& v, uint32_t root, 
 #pragma unroll
   for (int r = 0; r < LPT; ++r) {
     const uint32_t j = (uint32_t)r * T + threadIdx.x;
-    if (j < m && f + j < nb && s.flag[j] >= 2u) box_store(v.aabb, f + j, s.inner[j]);
+    if (j < m && f + j < nb && s.flag[j] >= 2u) {
+      if (!PK::kOn || f + j == root) box_store(v.aabb, f + j, s.inner[j]);
+      if constexpr (PK::kOn) {
+        // children inside the treelet (slots clamped for a broken tree,
+        // whose climb has raised already)
+        const uint2 c = s.child[j];
+        const uint32_t s0 = c.x >= nb ? c.x - nb - f : c.x - f;
+        const uint32_t s1 = c.y >= nb ? c.y - nb - f : c.y - f;
+        const Box b0 = c.x >= nb ? s.leaf[s0 < m ? s0 : 0u] : s.inner[s0 < m ? s0 : 0u];
+        const Box b1 = c.y >= nb ? s.leaf[s1 < m ? s1 : 0u] : s.inner[s1 < m ? s1 : 0u];
+        pk.inner_at(f + j, c.x, c.y, b0.v, b1.v, num_leaf, err);
+      }
+    }
   }
   // the root's box into its place in the top merge's staging (the
   // treelet's entry carries 2 * parent slot + side)
@@ -601,10 +672,17 @@ __device__ __forceinline__ void refit_treelet(const TreeView& v, uint32_t root, 
 }
 
 // Climbs from roots[e * stride], e < count, through the tree's top nodes.
-template <int T>
+// PK: a pack sink.  Its records are written after the climbs from the boxes
+// they stored (a record stored during a climb would hold up the next level's
+// counter RMW, which waits for the wave's stores): the merged nodes are
+// listed in s_list (LDS, list_cap entries, *s_count zero on entry and on
+// return); those past list_cap are written at the merge.
+template <int T, typename PK = NoPack>
 __device__ __forceinline__ void refit_top(const TreeView& v, uint32_t* __restrict__ flags,
                                           const uint32_t* __restrict__ roots, uint32_t stride,
-                                          uint32_t count, uint32_t* __restrict__ err) {
+                                          uint32_t count, uint32_t* __restrict__ err,
+                                          const PK& pk = PK{}, uint32_t* s_list = nullptr,
+                                          uint32_t* s_count = nullptr, uint32_t list_cap = 0) {
   const uint32_t nb = v.num_leaf - 1;
   for (uint32_t e = threadIdx.x; e < count; e += T) {
     uint32_t node = roots[(size_t)e * stride];
@@ -628,11 +706,37 @@ __device__ __forceinline__ void refit_top(const TreeView& v, uint32_t* __restric
         raise_error(err, DMSH_DEVERR_REFIT);
         break;
       }
-      mine = (a == node) ? box_merge(mine, box_load(v.aabb, b))
-                         : box_merge(box_load(v.aabb, a), mine);
+      const Box sib = box_load(v.aabb, a == node ? b : a);
+      if constexpr (PK::kOn) {
+        const uint32_t q = list_cap ? atomicAdd(s_count, 1u) : 0u;
+        if (q < list_cap) {
+          s_list[q] = parent;
+        } else if (a == node) {
+          pk.inner_at(parent, a, b, mine.v, sib.v, v.num_leaf, err);
+        } else {
+          pk.inner_at(parent, a, b, sib.v, mine.v, v.num_leaf, err);
+        }
+      }
+      mine = (a == node) ? box_merge(mine, sib) : box_merge(sib, mine);
       box_store(v.aabb, parent, mine);
       node = parent;
       parent = prow[0];
+    }
+  }
+  if constexpr (PK::kOn) {
+    if (list_cap) {
+      __syncthreads();  // the workgroup's boxes are stored and its list complete
+      const uint32_t listed = *s_count < list_cap ? *s_count : list_cap;
+      for (uint32_t e = threadIdx.x; e < listed; e += T) {
+        const uint32_t parent = s_list[e];
+        const uint32_t* prow = v.bvhnodes + (size_t)parent * 3;
+        const uint32_t a = prow[1], b = prow[2];  // checked at the merge
+        const Box ba = box_load(v.aabb, a), bb = box_load(v.aabb, b);
+        pk.inner_at(parent, a, b, ba.v, bb.v, v.num_leaf, err);
+      }
+      __syncthreads();  // every thread has read the count
+      if (threadIdx.x == 0) *s_count = 0u;
+      __syncthreads();
     }
   }
 }

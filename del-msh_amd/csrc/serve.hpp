@@ -42,6 +42,65 @@ __device__ __forceinline__ uint32_t* packed_flag(uint64_t addr, uint32_t n) {
   return reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(addr) + packed_flag_offset(n));
 }
 
+// The pack sink of one mesh for the batched refit: its records go straight
+// into the mesh's packed buffer, as k_pack_to_buffers writes them.
+struct TreePack {
+  static constexpr bool kOn = true;
+  rc::InnerNode* inner;
+  rc::LeafTri* leaf;
+  uint32_t* nonfinite;
+  // the records rc::pack_leaf writes, from the corners the refit loaded
+  __device__ __forceinline__ void leaf_ok(uint32_t k, const float (&p0)[3], const float (&p1)[3],
+                                          const float (&p2)[3], uint32_t tri) const {
+    rc::LeafTri lt;
+    lt.a = make_float4(p0[0], p0[1], p0[2], p1[0]);
+    lt.b = make_float4(p1[1], p1[2], p2[0], p2[1]);
+    lt.c = make_float4(p2[2], __uint_as_float(tri), 0.f, 0.f);
+    leaf[k] = lt;
+  }
+  __device__ __forceinline__ void leaf_bad(uint32_t k) const {
+    rc::LeafTri lt;
+    lt.a = make_float4(0.f, 0.f, 0.f, 0.f);
+    lt.b = make_float4(0.f, 0.f, 0.f, 0.f);
+    lt.c = make_float4(0.f, __uint_as_float(DMSH_U32MAX), 0.f, 0.f);
+    leaf[k] = lt;
+  }
+  __device__ __forceinline__ void inner_at(uint32_t i, uint32_t c0, uint32_t c1,
+                                           const float (&b0)[6], const float (&b1)[6],
+                                           uint32_t num_leaf, uint32_t* err) const {
+    rc::store_inner(inner, i, c0, c1, b0, b1, num_leaf, nonfinite, err);
+  }
+};
+
+struct PackSrc {
+  using PK = TreePack;
+  const uint64_t* dst;  // [M] packed buffer addresses
+  uint32_t* nonfinite;  // [M], zeroed before the build
+  __device__ __forceinline__ TreePack of(uint32_t m, uint32_t n) const {
+    const PackedRef p = packed_ref(dst[m], n);
+    return TreePack{p.inner, p.leaf, nonfinite + m};
+  }
+};
+
+// After a build that packed as it refit: each mesh's root box and its
+// non-finite flag into its packed buffer.
+static __global__ __launch_bounds__(256) void k_pack_roots(uint32_t M,
+                                                           const uint32_t* __restrict__ mesh2tri,
+                                                           const uint32_t* __restrict__ node_off,
+                                                           const float* __restrict__ aabb,
+                                                           const uint64_t* __restrict__ dst,
+                                                           const uint32_t* __restrict__ nonfinite,
+                                                           const uint32_t* __restrict__ bad) {
+  const uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (*bad || m >= M) return;
+  const uint32_t n = mesh2tri[m + 1] - mesh2tri[m];
+  if (n == 0) return;
+  const PackedRef p = packed_ref(dst[m], n);
+#pragma unroll
+  for (int d = 0; d < 6; ++d) p.root_box[d] = aabb[(size_t)node_off[m] * 6 + d];
+  *packed_flag(dst[m], n) = nonfinite[m];
+}
+
 // The per-mesh non-finite flags into the packed buffers (after the pack).
 static __global__ __launch_bounds__(256) void k_pack_flags(uint32_t M,
                                                            const uint32_t* __restrict__ mesh2tri,

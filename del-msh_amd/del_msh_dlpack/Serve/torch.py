@@ -149,8 +149,13 @@ class PinnedRing:
 
 class Server:
     def __init__(self, catalog: MeshCatalog, budget_bytes: int, device=None,
-                 max_step_requests: int = 256, max_step_pixels: int = 1 << 27):
+                 max_step_requests: int = 256, max_step_pixels: int = 1 << 27,
+                 fused_pack: bool = True):
         self.catalog = catalog
+        # True: one dmsh_batch_build_pack_trimesh3 per step (the refit writes
+        # the packed buffers); False: dmsh_batch_build_trimesh3, then
+        # dmsh_batch_pack (the build alone is then timed too)
+        self.fused_pack = fused_pack
         self.device = device or torch.device("cuda", torch.cuda.current_device())
         self.L = _lib.lib()
         self.cache = ByteLRU(budget_bytes)
@@ -164,7 +169,7 @@ class Server:
         # (+ pack) events and the triangles they built
         self.host_s: list = []
         self.builds: list = []       # build + pack into the traversal layout
-        self.builds_only: list = []  # the batched build alone
+        self.builds_only: list = []  # the batched build alone (unfused pack only)
         self.step_gpu: list = []     # (event before the step's work, event after its casts)
 
     def tree_bytes(self, mid: int) -> int:
@@ -206,28 +211,35 @@ class Server:
         M, T, V = len(new_ids), int(m2t[-1]), int(m2v[-1])
         ws = _lib.workspace(L.dmsh_batch_build_workspace_bytes(M, T), dev)
         sp = self.stream.cuda_stream
-        b0 = torch.cuda.Event(enable_timing=True)
-        b0.record(self.stream)
-        _lib.check(L.dmsh_batch_build_trimesh3(sp, M, T, V, _lib.ptr(d_m2t), _lib.ptr(d_m2v),
-                                               _lib.ptr(tri), _lib.ptr(vtx), _lib.ptr(m2n),
-                                               _lib.ptr(nodes), _lib.ptr(aabb), _lib.ptr(ws),
-                                               ws.numel()), "batch build")
-        bm = torch.cuda.Event(enable_timing=True)
-        bm.record(self.stream)
         bufs = [torch.empty(int(L.dmsh_packed_bytes(int(n))), dtype=torch.uint8, device=dev)
                 for n in ntri]
         addrs = self.ring.upload(np.array([b.data_ptr() for b in bufs], np.uint64), dev,
                                  self.stream)
         nonfinite = torch.empty(M, dtype=torch.uint32, device=dev)
-        wsp = _lib.workspace(L.dmsh_batch_pack_workspace_bytes(M), dev)
-        _lib.check(L.dmsh_batch_pack(sp, M, T, V, _lib.ptr(d_m2t), _lib.ptr(d_m2v), _lib.ptr(tri),
-                                     _lib.ptr(vtx), _lib.ptr(m2n), _lib.ptr(nodes), _lib.ptr(aabb),
-                                     _lib.ptr(addrs), _lib.ptr(nonfinite), _lib.ptr(wsp),
-                                     wsp.numel()), "batch pack")
+        b0 = torch.cuda.Event(enable_timing=True)
+        b0.record(self.stream)
+        if self.fused_pack:
+            _lib.check(L.dmsh_batch_build_pack_trimesh3(
+                sp, M, T, V, _lib.ptr(d_m2t), _lib.ptr(d_m2v), _lib.ptr(tri), _lib.ptr(vtx),
+                _lib.ptr(m2n), _lib.ptr(nodes), _lib.ptr(aabb), _lib.ptr(addrs),
+                _lib.ptr(nonfinite), _lib.ptr(ws), ws.numel()), "batch build + pack")
+        else:
+            _lib.check(L.dmsh_batch_build_trimesh3(sp, M, T, V, _lib.ptr(d_m2t), _lib.ptr(d_m2v),
+                                                   _lib.ptr(tri), _lib.ptr(vtx), _lib.ptr(m2n),
+                                                   _lib.ptr(nodes), _lib.ptr(aabb), _lib.ptr(ws),
+                                                   ws.numel()), "batch build")
+            bm = torch.cuda.Event(enable_timing=True)
+            bm.record(self.stream)
+            wsp = _lib.workspace(L.dmsh_batch_pack_workspace_bytes(M), dev)
+            _lib.check(L.dmsh_batch_pack(sp, M, T, V, _lib.ptr(d_m2t), _lib.ptr(d_m2v),
+                                         _lib.ptr(tri), _lib.ptr(vtx), _lib.ptr(m2n),
+                                         _lib.ptr(nodes), _lib.ptr(aabb), _lib.ptr(addrs),
+                                         _lib.ptr(nonfinite), _lib.ptr(wsp), wsp.numel()),
+                       "batch pack")
+            self.builds_only.append((b0, bm, T))
         b1 = torch.cuda.Event(enable_timing=True)
         b1.record(self.stream)
         self.builds.append((b0, b1, T))
-        self.builds_only.append((b0, bm, T))
         # the tree's own non-finite flag is in its buffer; staging tensors are
         # freed here and reused only by later work on the same stream
         for k, mid in enumerate(new_ids):

@@ -104,6 +104,11 @@ _PROTOTYPES = {
         [c_void_p, c_u32, c_u32, c_u32, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
          c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_size],
     ),
+    "dmsh_batch_build_pack_trimesh3": (
+        c_int,
+        [c_void_p, c_u32, c_u32, c_u32, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+         c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_size],
+    ),
     "dmsh_cast_requests_workspace_bytes": (c_size, [c_u32, c_u32]),
     "dmsh_cast_requests": (
         c_int, [c_void_p, c_u32, c_void_p, c_u32, c_void_p, c_void_p, c_void_p, c_size]),

@@ -484,6 +484,21 @@ int dmsh_batch_pack(void *stream, uint32_t num_mesh, uint32_t total_tri,
                     const uint32_t *bvhnodes, const float *bvhnode2aabb,
                     const uint64_t *packed, uint32_t *nonfinite, void *ws,
                     size_t ws_bytes);
+/* The batched build (as dmsh_batch_build_trimesh3, workspace
+ * dmsh_batch_build_workspace_bytes) whose refit writes every mesh's packed
+ * buffer and nonfinite flag, byte for byte what dmsh_batch_pack gives after
+ * the build; no reference counterpart (the serving cache).  mesh2node and
+ * bvhnodes are the build's; bvhnode2aabb is scratch: on return only the rows
+ * of each mesh's root, treelet roots and the nodes above them hold boxes. */
+int dmsh_batch_build_pack_trimesh3(void *stream, uint32_t num_mesh,
+                                   uint32_t total_tri, uint32_t total_vtx,
+                                   const uint32_t *mesh2tri,
+                                   const uint32_t *mesh2vtx,
+                                   const uint32_t *tri2vtx,
+                                   const float *vtx2xyz, uint32_t *mesh2node,
+                                   uint32_t *bvhnodes, float *bvhnode2aabb,
+                                   const uint64_t *packed, uint32_t *nonfinite,
+                                   void *ws, size_t ws_bytes);
 size_t dmsh_cast_requests_workspace_bytes(uint32_t num_req, uint32_t total_pix);
 int dmsh_cast_requests(void *stream, uint32_t num_req,
                        const dmsh_cast_request *requests, uint32_t total_pix,

@@ -40,6 +40,75 @@ def test_served_images_equal_each_request_alone(gpu):
     assert bits == 0, hex(bits)
 
 
+def test_fused_pack_equals_build_then_pack(gpu):
+    """dmsh_batch_build_pack_trimesh3 writes the packed buffers byte for byte
+    as dmsh_batch_build_trimesh3 followed by dmsh_batch_pack: meshes of one
+    triangle, of a single treelet and of many, an empty mesh, and runs of
+    equal codes."""
+    from del_msh_dlpack import primitives
+    L = _lib.lib()
+    rng = np.random.default_rng(4)
+    meshes = [primitives.torus_zup(0.7, 0.3, 40, 30), primitives.torus_zup(0.6, 0.2, 12, 9),
+              (np.array([[0, 1, 2]], np.uint32),
+               np.array([[0, 0, 0], [1, 0, 0], [0, 1, .5]], np.float32)),
+              (np.zeros((0, 3), np.uint32), np.zeros((0, 3), np.float32)),
+              (np.tile(np.array([[0, 1, 2]], np.uint32), (2000, 1)),
+               rng.uniform(-1, 1, (3, 3)).astype(np.float32)),
+              primitives.octahedral_sphere(6, noise_sigma=0.01, seed=3)]
+    # a mesh with an infinite vertex: its boxes are not finite
+    t_inf, v_inf = primitives.torus_zup(0.5, 0.2, 10, 8)
+    v_inf = v_inf.copy()
+    v_inf[3, 1] = np.inf
+    meshes.append((t_inf, v_inf))
+    ntri = np.array([len(t) for t, _ in meshes], np.int64)
+    nvtx = np.array([len(v) for _, v in meshes], np.int64)
+    m2t = np.concatenate([[0], np.cumsum(ntri)]).astype(np.uint32)
+    m2v = np.concatenate([[0], np.cumsum(nvtx)]).astype(np.uint32)
+    M, T, V = len(meshes), int(m2t[-1]), int(m2v[-1])
+    tri = torch.from_numpy(np.concatenate([t for t, _ in meshes])).to(gpu)
+    vtx = torch.from_numpy(np.concatenate([v for _, v in meshes])).to(gpu)
+    d_m2t, d_m2v = torch.from_numpy(m2t).to(gpu), torch.from_numpy(m2v).to(gpu)
+    rows = int((2 * ntri[ntri > 0] - 1).sum())
+    sp = torch.cuda.current_stream().cuda_stream
+    out = []
+    for fused in (False, True):
+        m2n = torch.empty(M + 1, dtype=torch.uint32, device=gpu)
+        nodes = torch.empty((rows, 3), dtype=torch.uint32, device=gpu)
+        aabb = torch.empty((rows, 6), dtype=torch.float32, device=gpu)
+        bufs = [torch.full((int(L.dmsh_packed_bytes(int(n))),), 0xA5, dtype=torch.uint8,
+                           device=gpu) for n in ntri]
+        addrs = torch.tensor([b.data_ptr() for b in bufs], dtype=torch.uint64, device=gpu)
+        nonfinite = torch.full((M,), 7, dtype=torch.uint32, device=gpu)
+        ws = _lib.workspace(L.dmsh_batch_build_workspace_bytes(M, T), gpu)
+        args = (sp, M, T, V, d_m2t.data_ptr(), d_m2v.data_ptr(), tri.data_ptr(), vtx.data_ptr(),
+                m2n.data_ptr(), nodes.data_ptr(), aabb.data_ptr())
+        if fused:
+            _lib.check(L.dmsh_batch_build_pack_trimesh3(*args, addrs.data_ptr(),
+                                                        nonfinite.data_ptr(), ws.data_ptr(),
+                                                        ws.numel()), "build + pack")
+        else:
+            _lib.check(L.dmsh_batch_build_trimesh3(*args, ws.data_ptr(), ws.numel()), "build")
+            wsp = _lib.workspace(L.dmsh_batch_pack_workspace_bytes(M), gpu)
+            _lib.check(L.dmsh_batch_pack(sp, M, T, V, d_m2t.data_ptr(), d_m2v.data_ptr(),
+                                         tri.data_ptr(), vtx.data_ptr(), m2n.data_ptr(),
+                                         nodes.data_ptr(), aabb.data_ptr(), addrs.data_ptr(),
+                                         nonfinite.data_ptr(), wsp.data_ptr(), wsp.numel()),
+                       "pack")
+        torch.cuda.synchronize()
+        out.append(([H(b).copy() for b in bufs], H(nonfinite).copy(), H(nodes).copy(),
+                    H(aabb).view(np.uint32).copy()))
+    (b_a, nf_a, n_a, a_a), (b_b, nf_b, n_b, a_b) = out
+    # the fused build keeps only the boxes its top merge needs; the roots'
+    # are among them
+    root_rows = np.concatenate([[0], np.cumsum(2 * ntri[ntri > 0] - 1)[:-1]])
+    assert np.array_equal(n_a, n_b) and np.array_equal(a_a[root_rows], a_b[root_rows])
+    assert np.array_equal(nf_a, nf_b) and nf_b[-1] != 0
+    for m in range(M):
+        assert np.array_equal(b_a[m], b_b[m]), f"mesh {m}"
+    bits = _lib.device_errors()
+    assert bits == 0, hex(bits)
+
+
 def test_cast_requests_rejects_bad_offsets(gpu):
     """A request whose pixel offset is not the running sum is rejected on the
     device: nothing is written and DMSH_DEVERR_INDEX is raised."""
