@@ -849,15 +849,24 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_num_sgpr(72))) void k_refi
                                             staged, st, tl.w);
     }
   } else {
-    for (;;) {
-      if (threadIdx.x == 0) s_t = atomicAdd(next, 1u);
-      __syncthreads();
-      const uint32_t t = s_t;
-      __syncthreads();  // every thread has read s_t before it is reused
-      if (t >= count) break;
+    // The first treelet of a workgroup is dealt (its block index), the later
+    // ones come from `next` (offset by the grid).  Thread 0 asks for the next
+    // ticket as a treelet starts and keeps it in a register until the treelet
+    // is done, so the wait in the same-address atomic's queue overlaps the
+    // treelet's loads instead of preceding them.
+    uint32_t t = blockIdx.x;
+    while (t < count) {  // uniform over the workgroup
+      uint32_t tn = 0u;
+      // the lane id in the address keeps the compiler from making it a
+      // wave-wide atomic whose result is read at once
+      if (threadIdx.x == 0) tn = gridDim.x + atomicAdd(next + __lane_id(), 1u);
       const uint4 tl = treelets[t];
       refit_treelet<LK, K, T, false, CLIMB>(v, tl.x, tl.y, tl.z, eps, s, err, nullptr, compact,
                                             staged, st, tl.w);
+      if (threadIdx.x == 0) s_t = tn;
+      __syncthreads();
+      t = s_t;
+      __syncthreads();  // every thread has read s_t before it is reused
     }
   }
 }
