@@ -199,6 +199,8 @@ namespace dmsh {
 // Per-device sticky error word (a __device__ variable in dmsh_status.hip);
 // returns its device address for the current device, or nullptr on failure.
 uint32_t* device_error_word();
+// Compute units of the current device (cached per device; 256 if unknown).
+int device_cu_count();
 
 // Launch options of the build and refit kernels (process-wide, set through
 // dmsh_set_tuning; results do not depend on them).

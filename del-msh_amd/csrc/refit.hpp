@@ -1163,7 +1163,14 @@ __global__ __launch_bounds__(T) void k_refit_top_lds(
 
 constexpr uint32_t kTreeletLeaves = 1024;  // K
 constexpr int kTreeletThreads = 1024;
-constexpr uint32_t kTreeletGrid = 1024;    // persistent workgroups
+constexpr uint32_t kTreeletGrid = 1024;    // persistent workgroups, at most
+// Two treelet workgroups fit a CU's 160 KB of LDS: the persistent grid is
+// what is resident at once (a workgroup past it would start only when a
+// resident one has run out of tickets, with its dealt first treelet last).
+constexpr uint32_t kTreeletsPerCU = 2;
+static_assert(kTreeletsPerCU * (sizeof(TreeletLds<kTreeletLeaves>) + 64) <= 160u * 1024u &&
+                  (kTreeletsPerCU + 1) * sizeof(TreeletLds<kTreeletLeaves>) > 160u * 1024u,
+              "treelet workgroups per CU");
 constexpr int kTopThreads = 1024;
 constexpr int kTopSlots = 6400;  // 25 B each: 156 KB of LDS
 
@@ -1185,7 +1192,8 @@ inline int launch_refit_treelets(hipStream_t s, const TreeView& v, uint32_t* fla
   TopStage st;
   if (top_count && top_kid && top_box && num_leaf > kTreeletLeaves)
     st = TopStage{top_count, top_list, node2slot, top_kid, top_box};
-  const uint32_t grid = std::min<uint32_t>(kTreeletGrid, num_leaf);
+  const uint32_t resident = kTreeletsPerCU * (uint32_t)device_cu_count();
+  const uint32_t grid = std::min<uint32_t>(std::min<uint32_t>(kTreeletGrid, resident), num_leaf);
   constexpr int K = (int)kTreeletLeaves, Th = kTreeletThreads;
   const int climb = refit_option(kRefitClimb);
   const int compact = refit_option(kRefitCompact);
