@@ -40,13 +40,23 @@ def test_served_images_equal_each_request_alone(gpu):
     assert bits == 0, hex(bits)
 
 
-def test_fused_pack_equals_build_then_pack(gpu):
+@pytest.mark.parametrize("climb", [2, 0])
+def test_fused_pack_equals_build_then_pack(gpu, climb):
     """dmsh_batch_build_pack_trimesh3 writes the packed buffers byte for byte
     as dmsh_batch_build_trimesh3 followed by dmsh_batch_pack: meshes of one
-    triangle, of a single treelet and of many, an empty mesh, and runs of
-    equal codes."""
+    triangle, of a single treelet and of many, an empty mesh, runs of equal
+    codes and a non-finite vertex; with either treelet climb."""
     from del_msh_dlpack import primitives
     L = _lib.lib()
+    old_climb = L.dmsh_get_tuning(b"refit.climb")
+    assert L.dmsh_set_tuning(b"refit.climb", climb) == 0
+    try:
+        _fused_pack_case(gpu, L, primitives)
+    finally:
+        L.dmsh_set_tuning(b"refit.climb", old_climb)
+
+
+def _fused_pack_case(gpu, L, primitives):
     rng = np.random.default_rng(4)
     meshes = [primitives.torus_zup(0.7, 0.3, 40, 30), primitives.torus_zup(0.6, 0.2, 12, 9),
               (np.array([[0, 1, 2]], np.uint32),
