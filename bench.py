@@ -362,7 +362,37 @@ def run_batched(num_mesh: int, reps: int):
     errs = _lib.device_errors()
     if errs:
         raise RuntimeError(f"device error bits 0x{errs:x} in the plain loop")
+    del plain
+
+    # packed: the build writes every mesh's traversal buffer as it refits and
+    # the cast reads them as one request per mesh, so no cast packs again
+    ref_pix, ref_t = p.pix2tri.clone(), p.pix2t.clone()
+    q = Batched.Pipeline(b["mesh2tri"], b["mesh2vtx"], p.tri2vtx, p.vtx2xyz, b["mesh2pix"],
+                         b["mesh2width"], torch.from_numpy(b["ndc2world"]).to(dev),
+                         with_coeff=True, packed=True)
+    q.run()
+    packed_build_ms = Batched._time_ms(q.build, reps)
+    packed_cast_ms = Batched._time_ms(q.cast, reps)
+    q.capture()
+    packed_graph_ms = Batched._time_ms(q.replay, reps)
+    torch.cuda.synchronize()
+    packed_same = bool(torch.equal(q.pix2tri, ref_pix)
+                       and torch.equal(q.pix2t.view(torch.int32), ref_t.view(torch.int32)))
+    errs = _lib.device_errors()
+    if errs:
+        raise RuntimeError(f"device error bits 0x{errs:x} in the packed pipeline")
     tri, px = p.total_tri, p.total_pix
+    packed = {
+        "build_ms": round(packed_build_ms, 3),
+        "build_mtri_per_s": round(tri / packed_build_ms / 1e3, 1),
+        "cast_ms": round(packed_cast_ms, 3),
+        "cast_mrays_per_s": round(px / packed_cast_ms / 1e3, 1),
+        "graph_build_cast_ms": round(packed_graph_ms, 3),
+        "speedup_build": round(plain_build_ms / packed_build_ms, 2),
+        "speedup_cast": round(plain_cast_ms / packed_cast_ms, 2),
+        "images_equal": packed_same,
+        "how": "dmsh_batch_build_pack_trimesh3 + dmsh_cast_requests (one request per mesh)",
+    }
     return {
         "workload": (f"{num_mesh} meshes (sphere/torus/cylinder, 256..262144 tris log-uniform, "
                      f"scale U[0.5,2], random rotation, seed 3), 30-bit codes, per-mesh "
@@ -380,6 +410,7 @@ def run_batched(num_mesh: int, reps: int):
         "speedup_cast": round(plain_cast_ms / cast_ms, 2),
         "plain_equals_batched": bool(same),
         "autotuned": {k: v for k, v in tuned.items() if k != "trials"},
+        "packed": packed,
     }
 
 

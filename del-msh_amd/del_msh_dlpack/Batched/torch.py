@@ -30,11 +30,17 @@ def node_offsets(mesh2tri) -> np.ndarray:
 class Pipeline:
     """Build and ray-cast of one batch with its workspaces allocated once, so
     that build(), cast() and run() allocate nothing; capture() records run()
-    into a HIP graph that replay() launches."""
+    into a HIP graph that replay() launches.
+
+    packed=True (with images): build() writes every mesh's packed traversal
+    buffer as it refits (dmsh_batch_build_pack_trimesh3) and cast() casts
+    each mesh's image from it as one request of dmsh_cast_requests, so no
+    cast packs the trees again; the images are the same, and bvhnode2aabb
+    is then scratch (mesh_tree is not available)."""
 
     def __init__(self, mesh2tri, mesh2vtx, tri2vtx: torch.Tensor, vtx2xyz: torch.Tensor,
                  mesh2pix=None, mesh2width=None, ndc2world: torch.Tensor | None = None,
-                 with_coeff: bool = True):
+                 with_coeff: bool = True, packed: bool = False):
         dev = tri2vtx.device
         self.device = dev
         _lib.require_gpu(tri2vtx, vtx2xyz)
@@ -80,10 +86,50 @@ class Pipeline:
             self.ws_cast = _lib.workspace(
                 L.dmsh_batch_pix2tri_workspace_bytes(self.num_mesh, self.total_tri,
                                                      self.total_pix), dev)
+        self.packed = bool(packed)
+        if self.packed:
+            if not self.has_cast:
+                raise ValueError("packed=True needs the images")
+            self._init_packed(L, dev)
         self.graph = None
+
+    def _init_packed(self, L, dev):
+        """One allocation of every mesh's packed buffer (256-byte aligned) and
+        the request table that names them, written once."""
+        from ..Serve.torch import REQ_DTYPE
+        ntri = np.diff(self.mesh2tri_h.astype(np.int64))
+        size = np.array([int(L.dmsh_packed_bytes(int(n))) for n in ntri], np.int64)
+        off = np.concatenate([[0], np.cumsum((size + 255) // 256 * 256)])
+        self.packed_buf = torch.empty(max(int(off[-1]), 1), dtype=torch.uint8, device=dev)
+        addrs = self.packed_buf.data_ptr() + off[:-1]
+        self.packed_addr = torch.from_numpy(addrs.astype(np.uint64)).to(dev)
+        self.nonfinite = torch.empty(self.num_mesh, dtype=torch.uint32, device=dev)
+        pix = np.diff(self.mesh2pix_h.astype(np.int64))
+        w = self.mesh2width_h.astype(np.int64)
+        if np.any(w == 0) or np.any(pix % np.maximum(w, 1)) or np.any(pix == 0):
+            raise ValueError("packed=True needs a non-empty image of whole rows per mesh")
+        recs = np.zeros(self.num_mesh, REQ_DTYPE)
+        recs["packed"] = addrs.astype(np.uint64)
+        recs["num_tri"] = ntri
+        recs["img_w"] = w
+        recs["img_h"] = pix // w
+        recs["pix_start"] = self.mesh2pix_h[:-1]
+        recs["ndc2world"] = self.transforms.reshape(self.num_mesh, 16).cpu().numpy()
+        self.requests = torch.from_numpy(recs.view(np.uint8)).to(dev)
+        self.ws_req = _lib.workspace(
+            L.dmsh_cast_requests_workspace_bytes(self.num_mesh, self.total_pix), dev)
 
     def build(self):
         L = _lib.lib()
+        if self.packed:
+            _lib.check(L.dmsh_batch_build_pack_trimesh3(
+                _lib.stream_of(self.device), self.num_mesh, self.total_tri, self.total_vtx,
+                _lib.ptr(self.mesh2tri), _lib.ptr(self.mesh2vtx), _lib.ptr(self.tri2vtx),
+                _lib.ptr(self.vtx2xyz), _lib.ptr(self.mesh2node), _lib.ptr(self.bvhnodes),
+                _lib.ptr(self.bvhnode2aabb), _lib.ptr(self.packed_addr), _lib.ptr(self.nonfinite),
+                _lib.ptr(self.ws_build), self.ws_build.numel()),
+                "dmsh_batch_build_pack_trimesh3")
+            return
         _lib.check(L.dmsh_batch_build_trimesh3(
             _lib.stream_of(self.device), self.num_mesh, self.total_tri, self.total_vtx,
             _lib.ptr(self.mesh2tri), _lib.ptr(self.mesh2vtx), _lib.ptr(self.tri2vtx),
@@ -95,6 +141,13 @@ class Pipeline:
         if not self.has_cast:
             raise ValueError("no images were given")
         L = _lib.lib()
+        if self.packed:
+            _lib.check(L.dmsh_cast_requests(
+                _lib.stream_of(self.device), self.num_mesh, _lib.ptr(self.requests),
+                self.total_pix, _lib.ptr(self.pix2tri),
+                _lib.ptr(self.pix2t) if self.pix2t is not None else None,
+                _lib.ptr(self.ws_req), self.ws_req.numel()), "dmsh_cast_requests")
+            return
         _lib.check(L.dmsh_batch_pix2tri(
             _lib.stream_of(self.device), self.num_mesh, self.total_tri, self.total_vtx,
             self.total_pix,
@@ -136,6 +189,8 @@ class Pipeline:
 
     # views -------------------------------------------------------------
     def mesh_tree(self, m: int):
+        if self.packed:
+            raise ValueError("a packed pipeline keeps no boxes in bvhnode2aabb")
         a, b = int(self.mesh2node_h[m]), int(self.mesh2node_h[m + 1])
         return self.bvhnodes[a:b], self.bvhnode2aabb[a:b]
 

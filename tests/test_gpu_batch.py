@@ -180,6 +180,34 @@ def test_batch_results_do_not_depend_on_tuning(gpu):
     no_device_errors()
 
 
+def test_packed_pipeline_images_equal(gpu):
+    """A packed pipeline (build into packed buffers, cast as requests) gives
+    the images of the default one, eagerly and replayed from a graph; meshes
+    of 0, 1, 2 and 1050 triangles included."""
+    b = _edge_batch()
+    p = _pipeline(b, gpu)
+    p.run()
+    torch.cuda.synchronize()
+    ref = (H(p.pix2tri).copy(), H(p.pix2t).copy())
+    q = Batched.Pipeline(b["mesh2tri"], b["mesh2vtx"], G(b["tri2vtx"], gpu),
+                         G(b["vtx2xyz"], gpu), b["mesh2pix"], b["mesh2width"],
+                         G(b["ndc2world"], gpu), True, packed=True)
+    q.run()
+    torch.cuda.synchronize()
+    assert np.array_equal(H(q.pix2tri), ref[0]) and bits_equal(H(q.pix2t), ref[1])
+    assert np.array_equal(H(q.mesh2node), H(p.mesh2node))
+    q.capture()
+    for _ in range(2):
+        q.pix2tri.fill_(0)
+        torch.cuda.synchronize()
+        q.replay()
+        torch.cuda.synchronize()
+        assert np.array_equal(H(q.pix2tri), ref[0]) and bits_equal(H(q.pix2t), ref[1])
+    with pytest.raises(ValueError):
+        q.mesh_tree(0)
+    no_device_errors()
+
+
 def test_batch_graph_replays_match_eager(gpu):
     b = _edge_batch()
     p = _pipeline(b, gpu)
