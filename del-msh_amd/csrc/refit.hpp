@@ -147,9 +147,10 @@ struct TreeView {
 // Where a refit also writes the traversal records of the nodes it fits
 // (batched builds for the serving cache): NoPack writes none.  A sink has
 // leaf_ok(k, p0, p1, p2, tri) for triangle leaf k with its corners,
-// leaf_bad(k) for a leaf whose indices do not fit, and inner_at(i, c0, c1,
-// b0, b1, num_leaf, err) for internal node i with children c0, c1 and their
-// boxes.
+// leaf_bad(k) for a leaf whose indices do not fit, inner_at(i, c0, c1, b0,
+// b1, num_leaf, err) for internal node i with children c0, c1 and their
+// boxes, and inner_part(i, part, c0, c1, x0..x3, num_leaf, err) for one
+// 16-byte part of that record.
 struct NoPack {
   static constexpr bool kOn = false;
 };
@@ -644,16 +645,31 @@ __device__ __forceinline__ void refit_treelet(const TreeView& v, uint32_t root, 
     const uint32_t j = (uint32_t)r * T + threadIdx.x;
     if (j < m && f + j < nb && s.flag[j] >= 2u) {
       if (!PK::kOn || f + j == root) box_store(v.aabb, f + j, s.inner[j]);
-      if constexpr (PK::kOn) {
-        // children inside the treelet (slots clamped for a broken tree,
-        // whose climb has raised already)
-        const uint2 c = s.child[j];
-        const uint32_t s0 = c.x >= nb ? c.x - nb - f : c.x - f;
-        const uint32_t s1 = c.y >= nb ? c.y - nb - f : c.y - f;
-        const Box b0 = c.x >= nb ? s.leaf[s0 < m ? s0 : 0u] : s.inner[s0 < m ? s0 : 0u];
-        const Box b1 = c.y >= nb ? s.leaf[s1 < m ? s1 : 0u] : s.inner[s1 < m ? s1 : 0u];
-        pk.inner_at(f + j, c.x, c.y, b0.v, b1.v, num_leaf, err);
+    }
+  }
+  if constexpr (PK::kOn) {
+    // the internal records of ids f.. in 16-byte parts, consecutive threads
+    // on consecutive parts, so that a wave's store is one contiguous
+    // kilobyte; a part is made from the two child boxes in LDS (children
+    // inside the treelet; slots clamped for a broken tree, whose climb has
+    // raised already)
+    for (uint32_t c = threadIdx.x; c < 4u * m; c += (uint32_t)T) {
+      const uint32_t j = c >> 2, part = c & 3u;
+      if (f + j >= nb || s.flag[j] < 2u) continue;
+      const uint2 ch = s.child[j];
+      const uint32_t s0 = ch.x >= nb ? ch.x - nb - f : ch.x - f;
+      const uint32_t s1 = ch.y >= nb ? ch.y - nb - f : ch.y - f;
+      const Box* b0 = ch.x >= nb ? &s.leaf[s0 < m ? s0 : 0u] : &s.inner[s0 < m ? s0 : 0u];
+      const Box* b1 = ch.y >= nb ? &s.leaf[s1 < m ? s1 : 0u] : &s.inner[s1 < m ? s1 : 0u];
+      float x0 = 0.f, x1 = 0.f, x2 = 0.f, x3 = 0.f;
+      if (part == 0u) {
+        x0 = b0->v[0]; x1 = b0->v[1]; x2 = b0->v[2]; x3 = b0->v[3];
+      } else if (part == 1u) {
+        x0 = b0->v[4]; x1 = b0->v[5]; x2 = b1->v[0]; x3 = b1->v[1];
+      } else if (part == 2u) {
+        x0 = b1->v[2]; x1 = b1->v[3]; x2 = b1->v[4]; x3 = b1->v[5];
       }
+      pk.inner_part(f + j, part, ch.x, ch.y, x0, x1, x2, x3, num_leaf, err);
     }
   }
   // the root's box into its place in the top merge's staging (the
