@@ -312,13 +312,16 @@ def _time_ms(fn, reps: int) -> float:
 
 
 def autotune(p: Pipeline, reps: int = 2, stacks=(16, 20, 24, 32), wgs_per_cu=(4, 8, 16, 32),
-             cast_waves=(1, 2, 4), treelet_grids=(512, 1024, 2048), sort_items=(8, 12, 16)):
+             cast_waves=(1, 2, 4), treelet_grids=(512, 1024, 2048), sort_items=(8, 12, 16),
+             sort_seg=(1, 0)):
     """Times the launch parameters of the batched kernels on this batch and
     keeps the fastest (results do not depend on them).  The ray-cast is tuned
     over (LDS stack frames, ray-cast waves per CU), then over the workgroup
     size (waves per workgroup) at that pair; the build over (refit grid,
-    elements per thread of the radix-sort digit passes).  Returns
-    {"cast": {...}, "build": {...}, "trials": [...]}."""
+    elements per thread of the radix-sort digit passes, the sort: three
+    segmented passes over the codes and the LDS finish, or one pass per 8-bit
+    digit of the whole composite key).  Returns {"cast": {...}, "build":
+    {...}, "trials": [...]}."""
     L = _lib.lib()
     trials = []
 
@@ -352,19 +355,22 @@ def autotune(p: Pipeline, reps: int = 2, stacks=(16, 20, 24, 32), wgs_per_cu=(4,
                 best_cast = (ms, best_cast[1], best_cast[2], cw)
         set_("batch.cast_waves", best_cast[3])
     best_build = None
-    for g in treelet_grids:
-        for it in sort_items:
-            set_("batch.treelet_grid", g)
-            set_("batch.sort_items", it)
-            ms = _time_ms(p.build, reps)
-            trials.append({"stage": "build", "batch.treelet_grid": g, "batch.sort_items": it,
-                           "ms": round(ms, 4)})
-            if best_build is None or ms < best_build[0]:
-                best_build = (ms, g, it)
+    for sg in sort_seg:
+        for g in treelet_grids:
+            for it in sort_items:
+                set_("batch.sort_seg", sg)
+                set_("batch.treelet_grid", g)
+                set_("batch.sort_items", it)
+                ms = _time_ms(p.build, reps)
+                trials.append({"stage": "build", "batch.sort_seg": sg, "batch.treelet_grid": g,
+                               "batch.sort_items": it, "ms": round(ms, 4)})
+                if best_build is None or ms < best_build[0]:
+                    best_build = (ms, g, it, sg)
     set_("batch.treelet_grid", best_build[1])
     set_("batch.sort_items", best_build[2])
+    set_("batch.sort_seg", best_build[3])
     out = {"build": {"batch.treelet_grid": best_build[1], "batch.sort_items": best_build[2],
-                     "ms": round(best_build[0], 4)},
+                     "batch.sort_seg": best_build[3], "ms": round(best_build[0], 4)},
            "trials": trials}
     if best_cast:
         out["cast"] = {"batch.stack": best_cast[1], "batch.wgs_per_cu": best_cast[2],

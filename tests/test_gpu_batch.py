@@ -250,7 +250,8 @@ def test_batch_generated_sample(gpu, oracle):
 def test_c3_full_batch_at_size(gpu, oracle):
     """The batched workload at its stated size: 4096 meshes (seed 3,
     256..262144 triangles log-uniform, 64^2..512^2 images), autotuned.
-    Composite keys mesh << 30 | code take 42 bits here (six digit passes).
+    Composite keys mesh << 30 | code take 42 bits here: six digit passes, or
+    three segmented passes over the codes and the finish, as tuned.
     Every mesh's tree and image from the batched launches equal the ones the
     single-mesh entry points give it, one call per mesh; the largest, the
     smallest and 16 random meshes equal the oracle's."""
