@@ -18,6 +18,8 @@ std::atomic<int> g_batch_sort_items{16};  // elements per thread of the digit pa
 std::atomic<int> g_batch_sort_seg{1};     // segmented passes + finish (up to kSegMaxMeshes)
 std::atomic<int> g_batch_cast_waves{1};   // waves per ray-cast workgroup
 std::atomic<int> g_batch_treelet_grid{1024};
+// a mesh's top merge: one workgroup, a thread per treelet root at a time
+constexpr int kBatchTopThreads = 1024;
 
 // key bits of the composite (mesh << 30 | code): 30 + the bits of M - 1
 inline uint32_t batch_key_bits(uint32_t M) {
@@ -265,9 +267,9 @@ int launch_batch_refit(hipStream_t s, const bat::BatchTrees& bt, const BatchBuil
   }
   int rc = launch_status();
   if (rc != DMSH_OK) return rc;
-  launch("k_refit_top", (bat::k_batch_refit_top<(int)kTreeletLeaves, 256, Src>),
-         dim3(std::min<uint32_t>(M, 1024)), dim3(256), 0, s, bt, L.inner_off, L.flags,
-         L.root_count, L.roots, err, src);
+  launch("k_refit_top", (bat::k_batch_refit_top<(int)kTreeletLeaves, kBatchTopThreads, Src>),
+         dim3(std::min<uint32_t>(M, 1024)), dim3(kBatchTopThreads), 0, s, bt, L.inner_off,
+         L.flags, L.root_count, L.roots, err, src);
   return launch_status();
 }
 
