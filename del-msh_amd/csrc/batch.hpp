@@ -105,24 +105,52 @@ static __global__ __launch_bounds__(1024) void k_batch_offsets(
   }
 }
 
-// keys6 of every mesh (min keys 0xFFFFFFFF, max keys 0), then zero words.
-static __global__ __launch_bounds__(256) void k_batch_init(uint32_t M, uint32_t* __restrict__ keys6,
+// A mesh's centroid box is accumulated in kBoxCopies copies of its six
+// order keys, each on its own 128-byte line (a wave adds into copy wave %
+// kBoxCopies), so that the waves of a large mesh do not queue their device
+// atomics on one line; k_batch_box_reduce then folds the copies into keys6.
+constexpr uint32_t kBoxCopies = 16;
+constexpr uint32_t kBoxCopyWords = 32;
+
+// The box copies of every mesh (min keys 0xFFFFFFFF, max keys 0), then zero
+// words.
+static __global__ __launch_bounds__(256) void k_batch_init(uint32_t M, uint32_t* __restrict__ copies,
                                                            uint32_t* __restrict__ zero,
                                                            size_t zero_count) {
   const size_t stride = (size_t)gridDim.x * blockDim.x;
   const size_t i0 = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
-  for (size_t i = i0; i < (size_t)M * 6; i += stride) keys6[i] = (i % 6 < 3) ? 0xFFFFFFFFu : 0u;
+  for (size_t i = i0; i < (size_t)M * kBoxCopies * 6; i += stride)
+    copies[(i / 6) * kBoxCopyWords + i % 6] = (i % 6 < 3) ? 0xFFFFFFFFu : 0u;
   for (size_t i = i0; i < zero_count; i += stride) zero[i] = 0u;
+}
+
+// keys6[m][d] = the fold of mesh m's box copies.
+static __global__ __launch_bounds__(256) void k_batch_box_reduce(
+    const uint32_t* __restrict__ bad, uint32_t M, const uint32_t* __restrict__ copies,
+    uint32_t* __restrict__ keys6) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (*bad || i >= M * 6u) return;
+  const uint32_t m = i / 6u, d = i % 6u;
+  const uint32_t* c = copies + (size_t)m * kBoxCopies * kBoxCopyWords + d;
+  uint32_t k = c[0];
+#pragma unroll
+  for (uint32_t j = 1; j < kBoxCopies; ++j) {
+    const uint32_t x = c[j * kBoxCopyWords];
+    k = d < 3 ? (x < k ? x : k) : (x > k ? x : k);
+  }
+  keys6[i] = k;
 }
 
 constexpr uint32_t kCentroidChunks = 16;  // 64-triangle chunks per wave
 
-__device__ __forceinline__ void flush_box(uint32_t* __restrict__ keys6, uint32_t m,
-                                          const uint32_t (&kmin)[3], const uint32_t (&kmax)[3]) {
+__device__ __forceinline__ void flush_box(uint32_t* __restrict__ copies, uint32_t m,
+                                          uint32_t copy, const uint32_t (&kmin)[3],
+                                          const uint32_t (&kmax)[3]) {
+  uint32_t* c = copies + ((size_t)m * kBoxCopies + copy) * kBoxCopyWords;
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
-    atomicMin(keys6 + (size_t)m * 6 + d, kmin[d]);
-    atomicMax(keys6 + (size_t)m * 6 + 3 + d, kmax[d]);
+    atomicMin(c + d, kmin[d]);
+    atomicMax(c + 3 + d, kmax[d]);
   }
 }
 
@@ -138,11 +166,12 @@ static __global__ __launch_bounds__(256) void k_batch_centroid(
     const uint32_t* __restrict__ bad, uint32_t total_tri, uint32_t M,
     const uint32_t* __restrict__ mesh2tri, const uint32_t* __restrict__ mesh2vtx,
     const uint32_t* __restrict__ tri2vtx, const float* __restrict__ xyz, float* __restrict__ cntr,
-    uint32_t* __restrict__ keys6, uint32_t* __restrict__ err) {
+    uint32_t* __restrict__ copies, uint32_t* __restrict__ err) {
   if (*bad) return;
   constexpr uint32_t kNone = 0xFFFFFFFFu;
   const uint32_t lane = threadIdx.x & 63u;
   const uint32_t wave = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const uint32_t copy = wave % kBoxCopies;
   const uint32_t w0 = wave * 64u * kCentroidChunks;
   if (w0 >= total_tri) return;
   uint32_t m = find_segment(mesh2tri, M, w0 + lane < total_tri ? w0 + lane : total_tri - 1);
@@ -211,9 +240,9 @@ static __global__ __launch_bounds__(256) void k_batch_centroid(
         }
       }
     } else if (carry != kNone && lane == 0) {
-      flush_box(keys6, carry, cmin, cmax);
+      flush_box(copies, carry, copy, cmin, cmax);
     }
-    if (head && lane != last) flush_box(keys6, mm, kmin, kmax);
+    if (head && lane != last) flush_box(copies, mm, copy, kmin, kmax);
     carry = (uint32_t)__shfl((int)mm, (int)last, 64);
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
@@ -221,7 +250,7 @@ static __global__ __launch_bounds__(256) void k_batch_centroid(
       cmax[d] = (uint32_t)__shfl((int)kmax[d], (int)last, 64);
     }
   }
-  if (carry != kNone && lane == 0) flush_box(keys6, carry, cmin, cmax);
+  if (carry != kNone && lane == 0) flush_box(copies, carry, copy, cmin, cmax);
 }
 
 // Composite key (mesh << 30 | 30-bit code under the mesh's unit-cube fit) and

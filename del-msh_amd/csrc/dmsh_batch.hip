@@ -29,7 +29,8 @@ inline uint32_t batch_key_bits(uint32_t M) {
 }
 
 struct BatchBuildLayout {
-  uint32_t* keys6;  // [M][6]
+  uint32_t* keys6;       // [M][6]
+  uint32_t* box_copies;  // [M][kBoxCopies][kBoxCopyWords]
   sortk::Layout sort;
   uint32_t* flags;          // [total_tri] top-climb counters (>= internal nodes)
   uint32_t* treelet_count;  // 1
@@ -52,6 +53,7 @@ inline bool carve_batch_build(Arena& a, uint32_t M, uint32_t n, BatchBuildLayout
   const uint32_t kb = batch_key_bits(M);
   const size_t nn = n ? n : 1;
   L.keys6 = a.take<uint32_t>((size_t)M * 6);
+  L.box_copies = a.take<uint32_t>((size_t)M * bat::kBoxCopies * bat::kBoxCopyWords);
   L.sort.hist = a.take<uint32_t>(sortk::kMaxPasses * sortk::kRadix + sortk::kTicketWords);
   L.sort.tickets = L.sort.hist + sortk::kMaxPasses * sortk::kRadix;
   L.sort.status = L.sort.tickets + sortk::kTicketWords;
@@ -298,14 +300,17 @@ int batch_build(void* stream, uint32_t num_mesh, uint32_t total_tri, uint32_t to
   int rc;
   if (packed && (rc = sortk::launch_fill(s, nonfinite, M, 0u)) != DMSH_OK) return rc;
   launch("k_batch_init", bat::k_batch_init, dim3(std::min<uint32_t>(ceil_div(L.zero_words, 256), 2048)),
-         dim3(256), 0, s, M, L.keys6, L.zero_begin, L.zero_words);
+         dim3(256), 0, s, M, L.box_copies, L.zero_begin, L.zero_words);
   if ((rc = launch_status()) != DMSH_OK) return rc;
   launch("k_batch_offsets", bat::k_batch_offsets, dim3(1), dim3(1024), 0, s, M, n, total_vtx,
          mesh2tri, mesh2vtx, L.inner_off, mesh2node, L.bad, err);
   if ((rc = launch_status()) != DMSH_OK) return rc;
   if (n == 0) return DMSH_OK;
   launch("k_batch_centroid", bat::k_batch_centroid,
-         dim3(ceil_div(n, 4u * 64u * bat::kCentroidChunks)), dim3(256), 0, s, L.bad, n, M, mesh2tri, mesh2vtx, tri2vtx, vtx2xyz, L.cntr, L.keys6, err);
+         dim3(ceil_div(n, 4u * 64u * bat::kCentroidChunks)), dim3(256), 0, s, L.bad, n, M, mesh2tri, mesh2vtx, tri2vtx, vtx2xyz, L.cntr, L.box_copies, err);
+  if ((rc = launch_status()) != DMSH_OK) return rc;
+  launch("k_batch_box_reduce", bat::k_batch_box_reduce, dim3(ceil_div((uint64_t)M * 6, 256)),
+         dim3(256), 0, s, (const uint32_t*)L.bad, M, (const uint32_t*)L.box_copies, L.keys6);
   if ((rc = launch_status()) != DMSH_OK) return rc;
   if (L.seg_off && g_batch_sort_seg.load() == 1) {
     // segmented: the mesh bits never move a key, so three passes over the
