@@ -144,6 +144,50 @@ def test_batch_duplicate_codes(gpu, oracle, seg):
     no_device_errors()
 
 
+def _tiny_batch(num_mesh: int, seed: int):
+    """Triangle soups of 0..40 triangles each: many meshes share a wave of
+    the centroid kernel, a round of the key kernel and a 64-mesh chunk of the
+    sort offsets."""
+    rng = np.random.default_rng(seed)
+    tris, vtxs = [], []
+    for _ in range(num_mesh):
+        k = int(rng.integers(0, 41))
+        nv = max(3, k)
+        vtxs.append(rng.uniform(-1, 1, (nv, 3)).astype(np.float32))
+        tris.append(rng.integers(0, nv, (k, 3)).astype(np.uint32))
+    csr = lambda c: np.concatenate([[0], np.cumsum(c)]).astype(np.uint32)
+    cam = np.diag([1.2, 1.2, 2.0, 1.0]).astype(np.float32)
+    return {
+        "tris": tris, "vtxs": vtxs,
+        "mesh2tri": csr([len(t) for t in tris]), "mesh2vtx": csr([len(v) for v in vtxs]),
+        "mesh2pix": csr([16 * 16] * num_mesh),
+        "tri2vtx": np.concatenate(tris), "vtx2xyz": np.concatenate(vtxs),
+        "mesh2width": np.array([16] * num_mesh, np.uint32),
+        "ndc2world": np.stack([cam] * num_mesh),
+    }
+
+
+@pytest.mark.parametrize("seg", [1, 0])
+def test_batch_many_tiny_meshes(gpu, seg):
+    """3000 meshes of 0..40 triangles: the batched build and cast equal the
+    single-mesh entry points mesh by mesh, with either sort."""
+    L = _lib.lib()
+    old = L.dmsh_batch_get_tuning(b"batch.sort_seg")
+    try:
+        assert L.dmsh_batch_set_tuning(b"batch.sort_seg", seg) == 0
+        b = _tiny_batch(3000, seed=8)
+        p = _pipeline(b, gpu)
+        p.run()
+        loop = Batched.PerMeshLoop(p)
+        loop.build()
+        loop.cast()
+        torch.cuda.synchronize()
+        assert loop.equals_pipeline()
+    finally:
+        L.dmsh_batch_set_tuning(b"batch.sort_seg", old)
+    no_device_errors()
+
+
 def test_batch_results_do_not_depend_on_tuning(gpu):
     b = _edge_batch()
     p = _pipeline(b, gpu)
