@@ -674,6 +674,19 @@ def run_served(num_req: int, rates, budget_bytes: int):
     out = {"workload": (f"{num_req} requests per rate, mesh id Zipf(1.1) over 10000 ids, "
                         f"1k..1M tris, perspective 256^2..1024^2, cache {budget_bytes >> 20} MiB"),
            "rates": {}}
+    # warm-up, not timed, as a server starts: the allocator's pool is
+    # reserved (the cache budget and 8 GiB for a step's buffers; otherwise
+    # the pool grows inside the first timed steps while the GPU waits on the
+    # host), and a kernel's code object loads on its first launch, so both
+    # servers first run a short stream of other mesh ids
+    import torch
+    pool = torch.empty(budget_bytes + (8 << 30), dtype=torch.uint8, device="cuda:0")
+    del pool
+    warm = Serve.request_stream(24, rate=1e5, num_ids=12, seed=11)
+    for r in warm:
+        r.mesh_id += 10_000
+    Serve.Server(cat, budget_bytes).run(copy.deepcopy(warm))
+    Serve.PlainServer(cat, budget_bytes).run(copy.deepcopy(warm))
     for rate in rates:
         reqs = Serve.request_stream(num_req, rate=rate, num_ids=10_000, seed=7)
         t0 = time.perf_counter()
