@@ -487,7 +487,7 @@ static __global__ __launch_bounds__(kTopoThreads) void k_batch_topology(
   node_range<uint64_t>(c, n, i, first, last);
   const uint32_t split = node_split<uint64_t>(c, first, last);
   topology_emit(n, i, first, last, split, bvhnodes + (size_t)node_off[m] * 3, idx + tb, tb, ts, m,
-                tb);
+                tb, inner_off[m]);
 }
 
 // Meshes of one triangle: the single leaf row and its treelet.
@@ -558,28 +558,38 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_num_sgpr(72))) void k_batc
   }
 }
 
-// One workgroup per mesh with more than K triangles (grid-stride over meshes).
+// One workgroup per mesh with more than K triangles (grid-stride over
+// meshes): the mesh's top nodes, listed by the topology kernel (top_count[m]
+// slots at top_list / node2slot + inner_off[m]), merged in LDS by rounds
+// (top_rounds) when they fit kBatchTopSlots, else climbed from the mesh's
+// treelet roots through global counters (refit_top).
+constexpr int kBatchTopSlots = 4096;
+
 template <int K, int T, typename Src = NoPackSrc>
-__global__ __launch_bounds__(T) void k_batch_refit_top(BatchTrees b,
-                                                       const uint32_t* __restrict__ inner_off,
-                                                       uint32_t* __restrict__ flags,
-                                                       const uint32_t* __restrict__ root_count,
-                                                       const uint32_t* __restrict__ roots,
-                                                       uint32_t* __restrict__ err, Src src) {
+__global__ __launch_bounds__(T) void k_batch_refit_top(
+    BatchTrees b, const uint32_t* __restrict__ inner_off, uint32_t* __restrict__ flags,
+    const uint32_t* __restrict__ root_count, const uint32_t* __restrict__ roots,
+    uint32_t* __restrict__ err, Src src, const uint32_t* __restrict__ top_count,
+    const uint4* __restrict__ top_list, const uint32_t* __restrict__ node2slot) {
   constexpr uint32_t kList = Src::PK::kOn ? 4096u : 1u;  // merged nodes listed for the pack
   __shared__ uint32_t s_list[kList];
   __shared__ uint32_t s_count;
+  __shared__ __align__(16) unsigned char s_top[(size_t)kBatchTopSlots * (sizeof(Box) + 1)];
   if (*b.bad) return;
   if (threadIdx.x == 0) s_count = 0u;
   __syncthreads();
   for (uint32_t m = blockIdx.x; m < b.M; m += gridDim.x) {
     const uint32_t n = seg_len(b.mesh2tri, m);
     if (n <= (uint32_t)K) continue;
+    const TreeView v = tree_of(b, m);
+    if (top_count && top_rounds<kBatchTopSlots, T, typename Src::PK>(
+                         v, top_count[m], top_list + inner_off[m], node2slot + inner_off[m], err,
+                         s_top, src.of(m, n)))
+      continue;
     uint32_t count = root_count[m];
     if (count > n) count = n;
-    refit_top<T, typename Src::PK>(tree_of(b, m), flags + inner_off[m], roots + b.mesh2tri[m], 1u,
-                                   count, err, src.of(m, n), s_list, &s_count,
-                                   Src::PK::kOn ? kList : 0u);
+    refit_top<T, typename Src::PK>(v, flags + inner_off[m], roots + b.mesh2tri[m], 1u, count, err,
+                                   src.of(m, n), s_list, &s_count, Src::PK::kOn ? kList : 0u);
   }
 }
 

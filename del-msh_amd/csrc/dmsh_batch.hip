@@ -36,6 +36,7 @@ struct BatchBuildLayout {
   uint32_t* treelet_count;  // 1
   uint32_t* root_count;     // [M]
   uint32_t* bad;            // 1: offsets rejected
+  uint32_t* top_count;      // [M] top nodes listed per mesh (zeroed with the rest)
   uint32_t* hseg;     // [3][M][256] or null (more than kSegMaxMeshes meshes)
   uint32_t* seg_off;  // [3][M][256] or null
   uint32_t* seg_chunk;  // [3][chunks of kSegChunk meshes][256] or null
@@ -43,6 +44,8 @@ struct BatchBuildLayout {
   size_t zero_words;
   uint32_t* inner_off;  // [M + 1]
   uint4* treelets;      // [total_tri]
+  uint4* top_list;      // [total_tri]: mesh m's top nodes from inner_off[m] on
+  uint32_t* node2slot;  // [total_tri]: mesh m's local node -> top slot, from inner_off[m]
   uint32_t* roots;      // [total_tri]
   float* cntr;          // [total_tri][3]
   uint64_t* codes;
@@ -71,15 +74,20 @@ inline bool carve_batch_build(Arena& a, uint32_t M, uint32_t n, BatchBuildLayout
   a.off += sizeof(uint32_t) * M;
   L.bad = L.root_count + M;
   a.off += sizeof(uint32_t);
+  L.top_count = L.bad + 1;
+  a.off += sizeof(uint32_t) * M;
   // per-mesh digit histograms of the segmented sort (zeroed with the rest)
   const bool seg = M <= bat::kSegMaxMeshes;
-  L.hseg = seg ? L.bad + 1 : nullptr;
+  L.hseg = seg ? L.top_count + M : nullptr;
   if (seg) a.off += sizeof(uint32_t) * (size_t)M * bat::kSegTables;
   L.zero_begin = L.sort.hist;
-  L.zero_words = (size_t)((seg ? L.hseg + (size_t)M * bat::kSegTables : L.bad + 1) - L.sort.hist);
+  L.zero_words =
+      (size_t)((seg ? L.hseg + (size_t)M * bat::kSegTables : L.top_count + M) - L.sort.hist);
   L.sort.zero_words = 0;
   L.inner_off = a.take<uint32_t>((size_t)M + 1);
   L.treelets = a.take<uint4>(nn);
+  L.top_list = a.take<uint4>(nn);
+  L.node2slot = a.take<uint32_t>(nn);
   L.roots = a.take<uint32_t>(nn);
   L.cntr = a.take<float>(nn * 3);
   L.codes = a.take<uint64_t>(nn);
@@ -271,7 +279,8 @@ int launch_batch_refit(hipStream_t s, const bat::BatchTrees& bt, const BatchBuil
   if (rc != DMSH_OK) return rc;
   launch("k_refit_top", (bat::k_batch_refit_top<(int)kTreeletLeaves, kBatchTopThreads, Src>),
          dim3(std::min<uint32_t>(M, 1024)), dim3(kBatchTopThreads), 0, s, bt, L.inner_off,
-         L.flags, L.root_count, L.roots, err, src);
+         L.flags, L.root_count, L.roots, err, src, (const uint32_t*)L.top_count,
+         (const uint4*)L.top_list, (const uint32_t*)L.node2slot);
   return launch_status();
 }
 
@@ -345,7 +354,10 @@ int batch_build(void* stream, uint32_t num_mesh, uint32_t total_tri, uint32_t to
                                      nullptr, nullptr, g_batch_sort_items.load());
   }
   if (rc != DMSH_OK) return rc;
-  const TreeletSink ts{kTreeletLeaves, L.treelet_count, L.treelets, n, L.root_count, L.roots};
+  TreeletSink ts{kTreeletLeaves, L.treelet_count, L.treelets, n, L.root_count, L.roots};
+  ts.mesh_top_count = L.top_count;
+  ts.mesh_top_list = L.top_list;
+  ts.mesh_node2slot = L.node2slot;
   launch("k_topology", bat::k_batch_topology, dim3(ceil_div(n, kTopoThreads)), dim3(kTopoThreads),
          0, s, L.bad, n,
          M, mesh2tri, L.inner_off, mesh2node, bvhnodes, L.codes, L.idx, ts);

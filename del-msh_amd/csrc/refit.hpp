@@ -919,6 +919,105 @@ __global__ __launch_bounds__(T) void k_refit_top(float* __restrict__ aabb,
 constexpr uint32_t kTopRef = 0x80000000u;
 constexpr uint8_t kTopPending = 0xFFu;
 
+// The nodes above the treelets of one tree merged in LDS by barrier rounds,
+// by the whole workgroup: `count` top slots listed as (node, left child,
+// right child, bit 0 / 1: that child is a top node) with node2slot; a child
+// below the top (a treelet root) has its box in v.aabb.  Round t merges the
+// slots whose top children finished before round t.  Then every slot's box
+// is stored and, with a pack sink, its record written (the children's boxes
+// from LDS or v.aabb).  Returns false, having changed nothing, when the list
+// does not fit C slots or the rounds a byte counts: the caller climbs then.
+template <int C, int T, typename PK = NoPack>
+__device__ __forceinline__ bool top_rounds(const TreeView& v, uint32_t count,
+                                           const uint4* __restrict__ top_list,
+                                           const uint32_t* __restrict__ node2slot,
+                                           uint32_t* __restrict__ err, unsigned char* smem,
+                                           const PK& pk = PK{}) {
+  constexpr int R = (C + T - 1) / T;  // slots per thread
+  if (count > (uint32_t)C) return false;
+  Box* s_box = reinterpret_cast<Box*>(smem);
+  uint8_t* s_done = smem + (size_t)C * sizeof(Box);  // round that finished the slot
+  const uint32_t nb = v.num_leaf - 1;
+  uint4 rec[R];
+  uint2 kid[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const uint32_t e = (uint32_t)r * T + threadIdx.x;
+    rec[r] = e < count ? top_list[e] : make_uint4(DMSH_U32MAX, 0u, 0u, 0u);
+    kid[r] = make_uint2(rec[r].y, rec[r].z);
+  }
+  bool bad = false;
+  uint32_t pending = 0u;  // bit r: slot r T + tid still to merge
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const uint32_t e = (uint32_t)r * T + threadIdx.x;
+    if (e >= count) continue;
+    const bool ta = rec[r].w & 1u, tb = rec[r].w & 2u;
+    const uint32_t a = kid[r].x, b = kid[r].y;
+    bool ok = rec[r].x < nb && (ta ? a < nb : a <= 2 * nb) && (tb ? b < nb : b <= 2 * nb);
+    if (ok && ta) {
+      kid[r].x = node2slot[a];
+      ok = kid[r].x < count;
+      kid[r].x |= kTopRef;
+    }
+    if (ok && tb) {
+      kid[r].y = node2slot[b];
+      ok = kid[r].y < count;
+      kid[r].y |= kTopRef;
+    }
+    if (!ok) {
+      bad = true;
+      continue;
+    }
+    if (!ta && !tb) {
+      s_box[e] = box_merge(box_load(v.aabb, a), box_load(v.aabb, b));
+      s_done[e] = 0u;
+    } else {
+      if (!ta || !tb) s_box[e] = box_load(v.aabb, ta ? b : a);  // the child below the top
+      s_done[e] = kTopPending;
+      pending |= 1u << r;
+    }
+  }
+  if (__syncthreads_or(bad)) {
+    if (threadIdx.x == 0) raise_error(err, DMSH_DEVERR_REFIT);
+    __syncthreads();  // LDS is reused by the caller's next tree
+    return true;
+  }
+  for (uint32_t t = 1; __syncthreads_or(pending != 0u); ++t) {
+    if (t > count || t >= kTopPending) return false;  // uniform; nothing stored yet
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      if (!(pending & (1u << r))) continue;
+      const uint32_t e = (uint32_t)r * T + threadIdx.x;
+      const uint2 k = kid[r];
+      const uint32_t ia = (k.x & kTopRef) ? k.x & ~kTopRef : e;
+      const uint32_t ib = (k.y & kTopRef) ? k.y & ~kTopRef : e;
+      const bool ra = ia == e || s_done[ia] < t;
+      const bool rb = ib == e || s_done[ib] < t;
+      if (!(ra && rb)) continue;
+      s_box[e] = box_merge(s_box[ia], s_box[ib]);
+      s_done[e] = (uint8_t)t;
+      pending &= ~(1u << r);
+    }
+  }
+  // the loop's last barrier orders these after every merge
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const uint32_t e = (uint32_t)r * T + threadIdx.x;
+    if (e >= count) continue;
+    box_store(v.aabb, rec[r].x, s_box[e]);
+    if constexpr (PK::kOn) {
+      const uint32_t a = rec[r].y, b = rec[r].z;
+      const Box ba = (kid[r].x & kTopRef) ? s_box[kid[r].x & ~kTopRef] : box_load(v.aabb, a);
+      const Box bb = (kid[r].y & kTopRef) ? s_box[kid[r].y & ~kTopRef] : box_load(v.aabb, b);
+      pk.inner_at(rec[r].x, a, b, ba.v, bb.v, v.num_leaf, err);
+    }
+  }
+  __syncthreads();  // LDS is reused by the caller's next tree
+  return true;
+}
+
+
 // Top nodes merged by climbs through LDS arrival counters (up to C slots):
 // slot e keeps its box, its parent's slot (16 bits) and its top children's
 // slots (16 bits each, 0xFFFF for a child below the top, whose box the

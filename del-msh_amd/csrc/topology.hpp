@@ -140,6 +140,14 @@ struct TreeletSink {
   // slot) + (1 for a right child), all ones when there is none, in place of
   // the mesh index.
   bool stage_pos = false;
+  // Batched builds: per-mesh top lists (used when top_count is null): mesh
+  // m's top nodes at slots mesh_top_list[top_base + slot], slot counted in
+  // mesh_top_count[m] (zeroed by the caller), mesh_node2slot[top_base +
+  // local node] = slot; top_base = the mesh's first internal node over the
+  // batch, so a mesh's slots never exceed its internal nodes.
+  uint32_t* mesh_top_count = nullptr;
+  uint4* mesh_top_list = nullptr;
+  uint32_t* mesh_node2slot = nullptr;
   // Optional: the leaf boxes, written into leaf_aabb (the tree's bvhnode2aabb)
   // as the leaf rows are: from tri_box (6 floats per object) when given, else
   // the point xyz[obj] (the box the refit's leaf_box makes for it).
@@ -186,11 +194,22 @@ __device__ __forceinline__ void topology_emit(uint32_t n, uint32_t i, uint32_t f
                                               uint32_t split, uint32_t* __restrict__ bvhnodes,
                                               const uint32_t* __restrict__ idx2obj,
                                               uint32_t obj_sub, const TreeletSink& ts,
-                                              uint32_t mesh, uint32_t root_base) {
+                                              uint32_t mesh, uint32_t root_base,
+                                              uint32_t top_base = 0u) {
   const uint32_t nb = n - 1;
   if (ts.list) {
     if (last - first + 1 > ts.k) {
       uint32_t pos = DMSH_U32MAX;  // 2 * this node's top slot
+      if (!ts.top_count && ts.mesh_top_count) {
+        const uint32_t slot = atomicAdd(ts.mesh_top_count + mesh, 1u);
+        if (slot < nb) {
+          ts.mesh_top_list[top_base + slot] =
+              make_uint4(i, first == split ? nb + split : split,
+                         last == split + 1 ? nb + split + 1 : split + 1,
+                         (split - first + 1 > ts.k ? 1u : 0u) | (last - split > ts.k ? 2u : 0u));
+          ts.mesh_node2slot[top_base + i] = slot;
+        }
+      }
       if (ts.top_count) {
         const uint32_t slot = atomicAdd(ts.top_count, 1u);
         if (slot < ts.top_cap) {
