@@ -120,6 +120,38 @@ __device__ __forceinline__ void store_inner(InnerNode* __restrict__ inner, uint3
   inner[i] = nd;
 }
 
+// 16-byte part `part` of the record store_inner writes for children (c0,
+// c1): parts 0..2 hold the children's bounds x0..x3 (box 0 bounds 0-3;
+// bounds 4, 5 of box 0 and 0, 1 of box 1; box 1 bounds 2-5), part 3 the
+// refs.  A record whose children do not fit the layout raises (part 0) and
+// is the one every ray misses; a part with a bound that is not finite sets
+// *nonfinite.  Writing the four parts from consecutive threads makes a
+// wave's store one contiguous kilobyte.
+__device__ __forceinline__ uint4 inner_part_bits(uint32_t part, uint32_t c0, uint32_t c1,
+                                                 float x0, float x1, float x2, float x3,
+                                                 uint32_t num_leaf,
+                                                 uint32_t* __restrict__ nonfinite,
+                                                 uint32_t* __restrict__ err) {
+  bool bad = (c1 == DMSH_U32MAX);
+  const uint32_t r0 = child_ref(c0, num_leaf, bad);
+  const uint32_t r1 = child_ref(c1, num_leaf, bad);
+  if (bad) {
+    if (part == 0u) raise_error(err, DMSH_DEVERR_INDEX);
+    if (part == 3u) return make_uint4(kLeafBit, kLeafBit, 0u, 0u);
+    const float4 e = part == 0u   ? make_float4(1.f, 1.f, 1.f, 0.f)
+                     : part == 1u ? make_float4(0.f, 0.f, 1.f, 1.f)
+                                  : make_float4(1.f, 0.f, 0.f, 0.f);
+    return make_uint4(__float_as_uint(e.x), __float_as_uint(e.y), __float_as_uint(e.z),
+                      __float_as_uint(e.w));
+  }
+  if (part == 3u) return make_uint4(r0, r1, 0u, 0u);
+  const bool fin = fabsf(x0) < __builtin_inff() && fabsf(x1) < __builtin_inff() &&
+                   fabsf(x2) < __builtin_inff() && fabsf(x3) < __builtin_inff();
+  if (!fin) atomicOr(nonfinite, 1u);
+  return make_uint4(__float_as_uint(x0), __float_as_uint(x1), __float_as_uint(x2),
+                    __float_as_uint(x3));
+}
+
 // Internal node i (i + 1 < num_leaf) of one tree into inner[i].
 __device__ __forceinline__ void pack_inner(InnerNode* __restrict__ inner,
                                            const uint32_t* __restrict__ bvhnodes,

@@ -65,35 +65,12 @@ struct TreePack {
     lt.c = make_float4(0.f, __uint_as_float(DMSH_U32MAX), 0.f, 0.f);
     leaf[k] = lt;
   }
-  // 16-byte part `part` of internal node i's record as rc::store_inner
-  // writes it: parts 0..2 hold the children's bounds x0..x3 (box 0 bounds
-  // 0-3; 4, 5 and box 1 bounds 0, 1; box 1 bounds 2-5), part 3 the refs
+  // 16-byte part `part` of internal node i's record (rc::inner_part_bits)
   __device__ __forceinline__ void inner_part(uint32_t i, uint32_t part, uint32_t c0, uint32_t c1,
                                              float x0, float x1, float x2, float x3,
                                              uint32_t num_leaf, uint32_t* err) const {
-    bool bad = (c1 == DMSH_U32MAX);
-    const uint32_t r0 = rc::child_ref(c0, num_leaf, bad);
-    const uint32_t r1 = rc::child_ref(c1, num_leaf, bad);
-    uint4 w;
-    if (bad) {
-      if (part == 0u) raise_error(err, DMSH_DEVERR_INDEX);
-      // the record of a node every ray misses (rc::store_inner)
-      const float4 e = part == 0u   ? make_float4(1.f, 1.f, 1.f, 0.f)
-                       : part == 1u ? make_float4(0.f, 0.f, 1.f, 1.f)
-                                    : make_float4(1.f, 0.f, 0.f, 0.f);
-      w = part == 3u ? make_uint4(rc::kLeafBit, rc::kLeafBit, 0u, 0u)
-                     : make_uint4(__float_as_uint(e.x), __float_as_uint(e.y),
-                                  __float_as_uint(e.z), __float_as_uint(e.w));
-    } else if (part == 3u) {
-      w = make_uint4(r0, r1, 0u, 0u);
-    } else {
-      const bool fin = fabsf(x0) < __builtin_inff() && fabsf(x1) < __builtin_inff() &&
-                       fabsf(x2) < __builtin_inff() && fabsf(x3) < __builtin_inff();
-      if (!fin) atomicOr(nonfinite, 1u);
-      w = make_uint4(__float_as_uint(x0), __float_as_uint(x1), __float_as_uint(x2),
-                     __float_as_uint(x3));
-    }
-    reinterpret_cast<uint4*>(inner + i)[part] = w;
+    reinterpret_cast<uint4*>(inner + i)[part] =
+        rc::inner_part_bits(part, c0, c1, x0, x1, x2, x3, num_leaf, nonfinite, err);
   }
   __device__ __forceinline__ void inner_at(uint32_t i, uint32_t c0, uint32_t c1,
                                            const float (&b0)[6], const float (&b1)[6],
